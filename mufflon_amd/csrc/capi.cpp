@@ -1,0 +1,333 @@
+// extern "C" boundary of the HIP library (include/mufflon_hip.h).
+// Every entry point converts exceptions into a status code plus a
+// thread-local message, the convention of Mufflon's core API
+// (interface.cpp:54-83, core_get_dll_error).
+#include "../../include/mufflon_hip.h"
+
+#include <cstring>
+#include <string>
+
+#include "mfh_internal.hpp"
+
+using namespace mfh;
+
+struct mfh_context {
+    Context c;
+};
+
+namespace {
+
+thread_local std::string g_lastError;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_lastError.clear();
+        return MFH_OK;
+    } catch (const Error& e) {
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_lastError = e.what();
+        return MFH_ERROR_STATE;
+    } catch (...) {
+        g_lastError = "unknown error";
+        return MFH_ERROR_STATE;
+    }
+}
+
+Context& ctx_of(mfh_context* ctx) {
+    MFH_REQUIRE(ctx != nullptr, "null context");
+    MFH_HIP(hipSetDevice(ctx->c.device));
+    return ctx->c;
+}
+
+Lod& lod_of(Context& c, int32_t id) {
+    MFH_REQUIRE(id >= 0 && id < (int32_t)c.lods.size() && c.lods[id], "unknown LoD id " + std::to_string(id));
+    return *c.lods[id];
+}
+
+Scene& scene_of(Context& c, int32_t id) {
+    MFH_REQUIRE(id >= 0 && id < (int32_t)c.scenes.size() && c.scenes[id], "unknown scene id " + std::to_string(id));
+    return *c.scenes[id];
+}
+
+void vertex_box(const float* v, int32_t n, float* box) {
+    for (int k = 0; k < 3; ++k) {
+        float lo = v[k], hi = v[k];
+        for (int32_t i = 1; i < n; ++i) {
+            const float x = v[3 * (size_t)i + k];
+            lo = (x < lo) ? x : lo;
+            hi = (hi < x) ? x : hi;
+        }
+        box[k] = lo;
+        box[3 + k] = hi;
+    }
+}
+
+void upload(Context& c, DevBuf& b, const void* src, size_t bytes) {
+    b.reserve(bytes);
+    MFH_HIP(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mfh_last_error(void) { return g_lastError.c_str(); }
+
+int mfh_create(int32_t deviceIndex, mfh_context** outCtx) {
+    return guarded([&] {
+        MFH_REQUIRE(outCtx != nullptr, "null output pointer");
+        *outCtx = nullptr;
+        int count = 0;
+        MFH_HIP(hipGetDeviceCount(&count));
+        MFH_REQUIRE(deviceIndex >= 0 && deviceIndex < count, "device index out of range");
+        auto* ctx = new mfh_context();
+        ctx->c.device = deviceIndex;
+        try {
+            MFH_HIP(hipSetDevice(deviceIndex));
+            MFH_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+            MFH_HIP(hipEventCreate(&ctx->c.evBegin));
+            MFH_HIP(hipEventCreate(&ctx->c.evEnd));
+        } catch (...) {
+            delete ctx;
+            throw;
+        }
+        *outCtx = ctx;
+    });
+}
+
+int mfh_destroy(mfh_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        Context& c = ctx->c;
+        (void)hipSetDevice(c.device);
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
+        for (Lod* l : c.lods) {
+            if (!l) continue;
+            l->verts.release(); l->tris.release(); l->nodes.release(); l->primIds.release(); l->triVerts.release();
+            delete l;
+        }
+        for (Scene* s : c.scenes) {
+            if (!s) continue;
+            s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release();
+            s->tlasNodes.release(); s->tlasPrimIds.release();
+            delete s;
+        }
+        BuildScratch& b = c.scratch;
+        for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.boxes, &b.counters,
+                          &b.flags, &b.offsets, &b.hist, &b.partials, &b.misc})
+            d->release();
+        if (c.evBegin) (void)hipEventDestroy(c.evBegin);
+        if (c.evEnd) (void)hipEventDestroy(c.evEnd);
+        if (c.stream) (void)hipStreamDestroy(c.stream);
+        delete ctx;
+    });
+}
+
+int mfh_synchronize(mfh_context* ctx) {
+    return guarded([&] { MFH_HIP(hipStreamSynchronize(ctx_of(ctx).stream)); });
+}
+
+int mfh_alloc(mfh_context* ctx, size_t bytes, void** outDevPtr) {
+    return guarded([&] {
+        ctx_of(ctx);
+        MFH_REQUIRE(outDevPtr != nullptr, "null output pointer");
+        *outDevPtr = nullptr;
+        if (hipMalloc(outDevPtr, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(kOutOfMemory, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    });
+}
+
+int mfh_free(mfh_context* ctx, void* devPtr) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        if (devPtr) MFH_HIP(hipFree(devPtr));
+    });
+}
+
+int mfh_upload(mfh_context* ctx, void* dDst, const void* src, size_t bytes) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(dDst && src, "null pointer");
+        MFH_HIP(hipMemcpyAsync(dDst, src, bytes, hipMemcpyHostToDevice, c.stream));
+        MFH_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mfh_download(mfh_context* ctx, void* dst, const void* dSrc, size_t bytes) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(dst && dSrc, "null pointer");
+        MFH_HIP(hipMemcpyAsync(dst, dSrc, bytes, hipMemcpyDeviceToHost, c.stream));
+        MFH_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mfh_lod_create(mfh_context* ctx, const float* vertices, int32_t numVertices, const uint32_t* triIndices,
+                   int32_t numTriangles, int32_t* outLodId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(vertices && triIndices && outLodId, "null pointer");
+        MFH_REQUIRE(numVertices > 0 && numTriangles > 0, "empty LoD");
+        MFH_REQUIRE((long long)numTriangles * 2 - 1 < (1ll << 31), "too many triangles");
+        for (int64_t i = 0; i < 3 * (int64_t)numTriangles; ++i)
+            MFH_REQUIRE(triIndices[i] < (uint32_t)numVertices, "triangle index out of range");
+        auto* l = new Lod();
+        try {
+            l->numVerts = numVertices;
+            l->numTris = numTriangles;
+            upload(c, l->verts, vertices, 12 * (size_t)numVertices);
+            upload(c, l->tris, triIndices, 12 * (size_t)numTriangles);
+            vertex_box(vertices, numVertices, l->box);
+            MFH_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            l->verts.release();
+            l->tris.release();
+            delete l;
+            throw;
+        }
+        c.lods.push_back(l);
+        *outLodId = (int32_t)c.lods.size() - 1;
+    });
+}
+
+int mfh_lbvh_build(mfh_context* ctx, int32_t lodId, const float* bbox) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Lod& l = lod_of(c, lodId);
+        build_lod_bvh(c, l, bbox, nullptr);
+        c.lastBuilt = lodId;
+    });
+}
+
+int mfh_lbvh_build_stages(mfh_context* ctx, int32_t lodId, const float* bbox, uint64_t* unsortedKeys,
+                          uint64_t* sortedKeys, int32_t* sortedIds, int32_t* parents, float* boxes,
+                          int32_t* offsets) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Lod& l = lod_of(c, lodId);
+        StageSink sink{unsortedKeys, sortedKeys, sortedIds, parents, boxes, offsets};
+        build_lod_bvh(c, l, bbox, &sink);
+        c.lastBuilt = lodId;
+    });
+}
+
+int mfh_lbvh_get(mfh_context* ctx, int32_t lodId, int32_t* outNumInternal, void* outNodes, int32_t* outPrimIds) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Lod& l = lod_of(c, lodId);
+        MFH_REQUIRE(l.built, "LoD BVH not built");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        if (outNumInternal) *outNumInternal = l.numInternal;
+        if (outNodes && l.numInternal > 0)
+            MFH_HIP(hipMemcpy(outNodes, l.nodes.ptr, 64 * (size_t)l.numInternal, hipMemcpyDeviceToHost));
+        if (outPrimIds) MFH_HIP(hipMemcpy(outPrimIds, l.primIds.ptr, 4 * (size_t)l.numTris, hipMemcpyDeviceToHost));
+    });
+}
+
+int mfh_scene_create(mfh_context* ctx, const int32_t* instLod, const float* worldToInstance, int32_t numInstances,
+                     int32_t* outSceneId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(instLod && worldToInstance && outSceneId, "null pointer");
+        MFH_REQUIRE(numInstances > 0, "scene without instances");
+        std::vector<DevLod> lods(c.lods.size());
+        for (size_t i = 0; i < c.lods.size(); ++i) {
+            std::memset(&lods[i], 0, sizeof(DevLod));
+            const Lod* l = c.lods[i];
+            if (!l || !l->built) continue;
+            lods[i].triVerts = l->triVerts.as<float4>();
+            lods[i].nodes = l->nodes.ptr;
+            lods[i].numInternal = l->numInternal;
+            lods[i].numPrims = l->numTris;
+            std::memcpy(lods[i].box, l->box, sizeof(l->box));
+        }
+        for (int32_t i = 0; i < numInstances; ++i) {
+            const int32_t li = instLod[i];
+            MFH_REQUIRE(li >= 0 && li < (int32_t)c.lods.size() && c.lods[li], "instance refers to an unknown LoD");
+            MFH_REQUIRE(c.lods[li]->built, "instance refers to a LoD whose BVH is not built");
+        }
+        auto* s = new Scene();
+        try {
+            s->numInstances = numInstances;
+            s->instLod.assign(instLod, instLod + numInstances);
+            s->worldToInst.assign(worldToInstance, worldToInstance + 12 * (size_t)numInstances);
+            upload(c, s->dInstLod, instLod, 4 * (size_t)numInstances);
+            upload(c, s->dWorldToInst, worldToInstance, 48 * (size_t)numInstances);
+            upload(c, s->dLods, lods.data(), sizeof(DevLod) * lods.size());
+            build_scene_bvh(c, *s, nullptr);
+            MFH_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release();
+            s->tlasNodes.release(); s->tlasPrimIds.release();
+            delete s;
+            throw;
+        }
+        c.scenes.push_back(s);
+        *outSceneId = (int32_t)c.scenes.size() - 1;
+    });
+}
+
+int mfh_scene_get(mfh_context* ctx, int32_t sceneId, float* outBox, int32_t* outNumInternal, void* outNodes,
+                  int32_t* outPrimIds) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Scene& s = scene_of(c, sceneId);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        if (outBox) std::memcpy(outBox, s.box, sizeof(s.box));
+        if (outNumInternal) *outNumInternal = s.tlasNumInternal;
+        if (outNodes && s.tlasNumInternal > 0)
+            MFH_HIP(hipMemcpy(outNodes, s.tlasNodes.ptr, 64 * (size_t)s.tlasNumInternal, hipMemcpyDeviceToHost));
+        if (outPrimIds && s.numInstances > 1)
+            MFH_HIP(hipMemcpy(outPrimIds, s.tlasPrimIds.ptr, 4 * (size_t)s.numInstances, hipMemcpyDeviceToHost));
+    });
+}
+
+int mfh_trace_closest(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, void* dHits,
+                      float* dBary) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Scene& s = scene_of(c, sceneId);
+        MFH_REQUIRE(numRays >= 0, "negative ray count");
+        if (numRays == 0) return;
+        MFH_REQUIRE(dRays && dHits, "null pointer");
+        launch_trace_closest(trace_params(s), dRays, numRays, static_cast<float4*>(dHits),
+                             reinterpret_cast<float2*>(dBary), 0, c.stream);
+    });
+}
+
+int mfh_trace_any(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, int32_t* dOccluded) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Scene& s = scene_of(c, sceneId);
+        MFH_REQUIRE(numRays >= 0, "negative ray count");
+        if (numRays == 0) return;
+        MFH_REQUIRE(dRays && dOccluded, "null pointer");
+        launch_trace_any(trace_params(s), dRays, numRays, dOccluded, 0, c.stream);
+    });
+}
+
+int mfh_timer_begin(mfh_context* ctx) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_HIP(hipEventRecord(c.evBegin, c.stream));
+    });
+}
+
+int mfh_timer_end(mfh_context* ctx, float* outMs) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(outMs != nullptr, "null output pointer");
+        MFH_HIP(hipEventRecord(c.evEnd, c.stream));
+        MFH_HIP(hipEventSynchronize(c.evEnd));
+        MFH_HIP(hipEventElapsedTime(outMs, c.evBegin, c.evEnd));
+    });
+}
+
+}  // extern "C"

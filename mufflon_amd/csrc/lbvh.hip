@@ -1,0 +1,491 @@
+// LBVH builder for gfx950: the GPU device of Mufflon's LBVHBuilder
+// (build_lbvh.cu:586-811) for triangle LoDs and for the instance level.
+//
+// Stages (one kernel each unless noted):
+//   k_morton    63-bit Morton code of each primitive centroid (build_lbvh.cu:70-125)
+//   radix sort  stable LSD sort of (code, id)           (sort_scan.hip)
+//   k_karras    Karras 2012 split hierarchy; writes parents[] in Mufflon's
+//               encoding plus explicit child lists       (build_lbvh.cu:164-245)
+//   k_refit     bottom-up AABB + SAH cost climb with one arrival counter per
+//               node; the second arriving thread merges   (build_lbvh.cu:326-411)
+//   k_mark      collapse decision per internal node      (build_lbvh.cu:417-463)
+//   scan        inclusive sum of the keep marks          (build_lbvh.cu:748-758)
+//   k_collapse  64-byte-node BVH with enlarged boxes     (build_lbvh.cu:503-566)
+//
+// Determinism. The refit merges a node in a fixed operand order whatever
+// thread arrives second: box = min/max(right, left) and
+//   cost = Ct + SA(R)/SA * c(R) + SA(L)/SA * c(L),
+// which is the order of the reference's serial CPU path (the right subtree's
+// thread always arrives second there). Whether a child is a leaf comes from
+// the explicit child list, not from the value left in the arrival counter,
+// so the reference's ambiguous encoding (thread N-1 == numInternalNodes,
+// build_lbvh.cu:371-374) cannot arise. The collapse marks use the equivalent
+// per-node form: an internal node X (not the root) is removed iff X has fewer
+// than 1023 primitives of every type and some node on the path from X up to
+// (excluding) the first node with 1023 or more, or the root, is cheaper as a
+// leaf than split. This is the set of nodes the per-leaf walks of the
+// reference mark, because the counts only grow towards the root.
+#include "mfh_device.hpp"
+#include "mfh_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <type_traits>
+
+namespace mfh {
+
+namespace {
+
+constexpr i32 kTreeHead = 0x10000000;
+constexpr int kThreads = 256;
+// Upper bound on any root path: 64 key bits + 32 index bits of a Karras tree.
+constexpr int kMaxDepth = 128;
+
+struct Box6 {
+    float v[6];
+};
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi) { return mn(mx(v, lo), hi); }
+
+__device__ __forceinline__ u64 part_by_two21(u32 x) {
+    u64 r = x;
+    r = (r | (r << 32)) & 0x001f00000000ffffull;
+    r = (r | (r << 16)) & 0x001f0000ff0000ffull;
+    r = (r | (r << 8)) & 0x100f00f00f00f00full;
+    r = (r | (r << 4)) & 0x10c30c30c30c30c3ull;
+    r = (r | (r << 2)) & 0x1249249249249249ull;
+    return r;
+}
+
+__device__ __forceinline__ u64 morton_of_point(V3 c, const Box6& bb) {
+    const V3 lo = v3(bb.v[0], bb.v[1], bb.v[2]);
+    const V3 span = v3(bb.v[3], bb.v[4], bb.v[5]) - lo;
+    const V3 num = c - lo;
+    const float nx = span.x != 0.0f ? num.x / span.x : 0.0f;
+    const float ny = span.y != 0.0f ? num.y / span.y : 0.0f;
+    const float nz = span.z != 0.0f ? num.z / span.z : 0.0f;
+    const u32 qx = (u32)clampf(nx * 2097152.0f, 0.0f, 2097151.0f);
+    const u32 qy = (u32)clampf(ny * 2097152.0f, 0.0f, 2097151.0f);
+    const u32 qz = (u32)clampf(nz * 2097152.0f, 0.0f, 2097151.0f);
+    return part_by_two21(qx) * 4 + part_by_two21(qy) * 2 + part_by_two21(qz);
+}
+
+// ------------------------------------------------------------- primitive sets
+struct TriPrims {
+    const float* verts;
+    const u32* tris;
+    __device__ __forceinline__ V3 vert(u32 i) const { return v3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]); }
+    __device__ __forceinline__ void tri(i32 p, V3& a, V3& b, V3& c) const {
+        const u32 i0 = tris[3 * p], i1 = tris[3 * p + 1], i2 = tris[3 * p + 2];
+        a = vert(i0);
+        b = vert(i1);
+        c = vert(i2);
+    }
+    __device__ __forceinline__ V3 centroid(i32 p) const {
+        V3 a, b, c;
+        tri(p, a, b, c);
+        return (a + b + c) / 3.0f;
+    }
+    __device__ __forceinline__ void box(i32 p, V3& lo, V3& hi) const {
+        V3 a, b, c;
+        tri(p, a, b, c);
+        lo = vmin(vmin(a, b), c);
+        hi = vmax(vmax(a, b), c);
+    }
+    __device__ __forceinline__ bool masked(i32) const { return false; }
+};
+
+// Inverse of a worldToInstance transform (descriptors.hpp:108-129: adjugate
+// of the 3x3 part over its determinant, translation -(inv * t)).
+__device__ __forceinline__ M34 instance_to_world(const float* w) {
+    const float r00 = w[0], r01 = w[1], r02 = w[2];
+    const float r10 = w[4], r11 = w[5], r12 = w[6];
+    const float r20 = w[8], r21 = w[9], r22 = w[10];
+    const float m00 = r11 * r22 - r21 * r12;
+    const float m01 = r21 * r02 - r01 * r22;
+    const float m02 = r01 * r12 - r11 * r02;
+    const float m10 = r20 * r12 - r10 * r22;
+    const float m11 = r00 * r22 - r20 * r02;
+    const float m12 = r10 * r02 - r00 * r12;
+    const float m20 = r10 * r21 - r20 * r11;
+    const float m21 = r20 * r01 - r00 * r21;
+    const float m22 = r00 * r11 - r10 * r01;
+    const float det = r00 * m00 + r01 * m10 + r02 * m20;
+    const float s = 1.0f / det;
+    M34 o;
+    o.m[0] = s * m00; o.m[1] = s * m01; o.m[2] = s * m02;
+    o.m[4] = s * m10; o.m[5] = s * m11; o.m[6] = s * m12;
+    o.m[8] = s * m20; o.m[9] = s * m21; o.m[10] = s * m22;
+    const float tx = w[3], ty = w[7], tz = w[11];
+    o.m[3] = -(o.m[0] * tx + o.m[1] * ty + o.m[2] * tz);
+    o.m[7] = -(o.m[4] * tx + o.m[5] * ty + o.m[6] * tz);
+    o.m[11] = -(o.m[8] * tx + o.m[9] * ty + o.m[10] * tz);
+    return o;
+}
+
+struct InstPrims {
+    const float* worldToInst;  // 12 per instance
+    const i32* instLod;
+    const DevLod* lods;
+    __device__ __forceinline__ void obox(i32 p, V3& lo, V3& hi) const {
+        const float* b = lods[instLod[p]].box;
+        lo = v3(b[0], b[1], b[2]);
+        hi = v3(b[3], b[4], b[5]);
+    }
+    __device__ __forceinline__ V3 centroid(i32 p) const {
+        V3 lo, hi;
+        obox(p, lo, hi);
+        const M34 M = instance_to_world(worldToInst + 12 * (size_t)p);
+        return xform_point(M, (lo + hi) * 0.5f);
+    }
+    // Transform of the 8 corners, folded in corner order x-fastest.
+    __device__ __forceinline__ void box(i32 p, V3& rlo, V3& rhi) const {
+        V3 lo, hi;
+        obox(p, lo, hi);
+        const M34 M = instance_to_world(worldToInst + 12 * (size_t)p);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const V3 q = xform_point(M, v3((c & 1) ? hi.x : lo.x, (c & 2) ? hi.y : lo.y, (c & 4) ? hi.z : lo.z));
+            if (c == 0) {
+                rlo = q;
+                rhi = q;
+            } else {
+                rlo = vmin(rlo, q);
+                rhi = vmax(rhi, q);
+            }
+        }
+    }
+    __device__ __forceinline__ bool masked(i32) const { return false; }
+};
+
+// ------------------------------------------------------------- cost models
+// Triangle LoD (build_lbvh.cu:34-40): Ct = 2, Cp = (1.2 tri, 4 quad, 1 sphere);
+// counts packed 3 x 10 bits, each saturating at 1023.
+struct LodCost {
+    static constexpr float Ct = 2.0f;
+    __device__ static i32 leaf() { return 1 << 20; }
+    __device__ static float cost(i32 e) {
+        const float x = (float)((e & 0x3FF00000) >> 20), y = (float)((e & 0x000FFC00) >> 10),
+                    z = (float)(e & 0x3FF);
+        return Ct + (x * 1.2f + y * 4.0f + z * 1.0f);
+    }
+    __device__ static i32 add(i32 a, i32 b) {
+        const i32 x = min(((a & 0x3FF00000) >> 20) + ((b & 0x3FF00000) >> 20), 0x3FF);
+        const i32 y = min(((a & 0x000FFC00) >> 10) + ((b & 0x000FFC00) >> 10), 0x3FF);
+        const i32 z = min((a & 0x3FF) + (b & 0x3FF), 0x3FF);
+        return (x << 20) | (y << 10) | z;
+    }
+    __device__ static bool capped(i32 e) {
+        return ((e & 0x3FF00000) >> 20) >= 1023 || ((e & 0x000FFC00) >> 10) >= 1023 || (e & 0x3FF) >= 1023;
+    }
+    __device__ static i32 sum(i32 e) { return ((e & 0x3FF00000) >> 20) + ((e & 0x000FFC00) >> 10) + (e & 0x3FF); }
+};
+// Instance level (build_lbvh.cu:41-48): Ct = 1, Cp = 8.5, plain count.
+struct InstCost {
+    static constexpr float Ct = 1.0f;
+    __device__ static i32 leaf() { return 1; }
+    __device__ static float cost(i32 e) { return Ct + (float)e * 8.5f; }
+    __device__ static i32 add(i32 a, i32 b) { return a + b; }
+    __device__ static bool capped(i32 e) { return e >= 1023; }
+    __device__ static i32 sum(i32 e) { return e; }
+};
+
+// ------------------------------------------------------------- kernels
+template <class P>
+__global__ __launch_bounds__(kThreads) void k_morton(P set, int n, Box6 bb, u64* __restrict__ keys,
+                                                    i32* __restrict__ ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = set.masked(i) ? ~0ull : morton_of_point(set.centroid(i), bb);
+    ids[i] = i;
+}
+
+__device__ __forceinline__ int lcp(const u64* keys, int n, int i1, int i2, u64 k1) {
+    if (i2 < 0 || i2 >= n) return -1;
+    const u64 k2 = keys[i2];
+    if (k1 == k2) return 64 + __clz((int)(u32)(i1 ^ i2));
+    return __clzll((long long)(k1 ^ k2));
+}
+
+__global__ __launch_bounds__(kThreads) void k_karras(const u64* __restrict__ keys, int n, i32* __restrict__ parents,
+                                                    i32* __restrict__ children) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n - 1) return;
+    const u64 k1 = keys[idx];
+    const int l1 = lcp(keys, n, idx, idx + 1, k1);
+    const int l2 = lcp(keys, n, idx, idx - 1, k1);
+    const int d = (l1 > l2) ? 1 : -1;  // never equal for distinct (key, index) pairs
+    const int minLcp = lcp(keys, n, idx, idx - d, k1);
+    int lMax = 2;
+    while (lcp(keys, n, idx, idx + lMax * d, k1) > minLcp) lMax *= 2;
+    int l = 0;
+    for (int t = lMax; t > 1;) {
+        t >>= 1;
+        if (lcp(keys, n, idx, idx + (l + t) * d, k1) > minLcp) l += t;
+    }
+    const int j = idx + l * d;
+    const int nodeLcp = lcp(keys, n, idx, j, k1);
+    int s = 0;
+    const int maxDiv = 1 << (32 - __clz(l));
+    for (int div = 2; div <= maxDiv; div <<= 1) {
+        const int t = (l + div - 1) / div;
+        if (lcp(keys, n, idx, idx + (s + t) * d, k1) > nodeLcp) s += t;
+    }
+    const int split = idx + s * d + min(d, 0);
+    const int left = (min(idx, j) == split) ? split + n - 1 : split;
+    const int right = (max(idx, j) == split + 1) ? split + n : split + 1;
+    parents[left] = ~idx;
+    parents[right] = idx;
+    children[2 * idx] = left;
+    children[2 * idx + 1] = right;
+    if (idx == 0) parents[0] = kTreeHead;
+}
+
+__device__ __forceinline__ int parent_of(const i32* parents, int node) {
+    const int p = parents[node];
+    return p < 0 ? ~p : p;
+}
+
+template <class P, class C>
+__global__ __launch_bounds__(kThreads) void k_refit(P set, int n, const i32* __restrict__ ids,
+                                                   const i32* __restrict__ parents,
+                                                   const i32* __restrict__ children, float4* boxes,
+                                                   u32* counters, uint8_t* flags, float4* triVerts) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int nInt = n - 1;
+    const int prim = ids[idx];
+    V3 lo, hi;
+    if constexpr (std::is_same<P, TriPrims>::value) {
+        // Triangle LoD: also lay the triangle out in sorted order for traversal.
+        V3 a, b, c;
+        set.tri(prim, a, b, c);
+        lo = vmin(vmin(a, b), c);
+        hi = vmax(vmax(a, b), c);
+        triVerts[3 * (size_t)idx] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
+        triVerts[3 * (size_t)idx + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+        triVerts[3 * (size_t)idx + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    } else {
+        set.box(prim, lo, hi);
+    }
+    const i32 cnt = C::leaf();
+    const int leaf = idx + nInt;
+    boxes[2 * (size_t)leaf] = make_float4(lo.x, lo.y, lo.z, __int_as_float(cnt));
+    boxes[2 * (size_t)leaf + 1] = make_float4(hi.x, hi.y, hi.z, C::cost(cnt));
+    if (n == 1) return;
+    int node = parent_of(parents, leaf);
+    for (int depth = 0; depth < kMaxDepth; ++depth) {
+        __threadfence();
+        const u32 arrived = atomicAdd(&counters[node], 1u);
+        if (arrived == 0) return;  // the sibling subtree is not finished yet
+        __threadfence();
+        const int L = children[2 * node], R = children[2 * node + 1];
+        const float4 lLo = boxes[2 * (size_t)L];
+        const float4 lHi = boxes[2 * (size_t)L + 1];
+        const float4 rLo = boxes[2 * (size_t)R];
+        const float4 rHi = boxes[2 * (size_t)R + 1];
+        const V3 Llo = v3(lLo.x, lLo.y, lLo.z), Lhi = v3(lHi.x, lHi.y, lHi.z);
+        const V3 Rlo = v3(rLo.x, rLo.y, rLo.z), Rhi = v3(rHi.x, rHi.y, rHi.z);
+        const V3 nlo = vmin(Rlo, Llo), nhi = vmax(Rhi, Lhi);
+        const float sa = surface(nlo, nhi);
+        const float saR = surface(Rlo, Rhi), saL = surface(Llo, Lhi);
+        const i32 cR = __float_as_int(rLo.w), cL = __float_as_int(lLo.w);
+        const float cost = C::Ct + saR / sa * C::cost(cR) + saL / sa * C::cost(cL);
+        const i32 c = C::add(cR, cL);
+        boxes[2 * (size_t)node] = make_float4(nlo.x, nlo.y, nlo.z, __int_as_float(c));
+        boxes[2 * (size_t)node + 1] = make_float4(nhi.x, nhi.y, nhi.z, cost);
+        flags[node] = (uint8_t)((C::capped(c) ? 1 : 0) | (C::cost(c) < cost ? 2 : 0));
+        if (node == 0) return;
+        node = parent_of(parents, node);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_mark(int nInt, const i32* __restrict__ parents,
+                                                  const uint8_t* __restrict__ flags, i32* __restrict__ marks) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nInt) return;
+    int keep = 1;
+    int y = x;
+    for (int depth = 0; y != 0 && depth < kMaxDepth; ++depth) {
+        const uint8_t f = flags[y];
+        if (f & 1) break;
+        if (f & 2) {
+            keep = 0;
+            break;
+        }
+        y = parent_of(parents, y);
+    }
+    marks[x] = keep;
+}
+
+__device__ __forceinline__ bool is_collapsed(const i32* offsets, int nInt, int node) {
+    return node > 0 && node < nInt && (offsets[node] - offsets[node - 1]) == 0;
+}
+
+template <class C>
+__global__ __launch_bounds__(kThreads) void k_collapse(int n, const i32* __restrict__ parents,
+                                                      const float4* __restrict__ boxes,
+                                                      const i32* __restrict__ offsets, BvhChild* __restrict__ out) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int nInt = n - 1;
+    if (node >= 2 * n - 1) return;
+    if (is_collapsed(offsets, nInt, node)) return;
+    const int nAfter = offsets[nInt - 1];
+    int parent = parents[node];
+    int off = parent < 0 ? 0 : 1;
+    if (parent < 0) parent = ~parent;
+    int fin = node;
+    if (fin >= nInt) {
+        for (int depth = 0; depth < kMaxDepth && is_collapsed(offsets, nInt, parent); ++depth) {
+            if (off == 1) return;  // only the left-most leaf writes the collapsed subtree
+            fin = parent;
+            parent = parents[parent];
+            off = parent < 0 ? 0 : 1;
+            if (parent < 0) parent = ~parent;
+        }
+    }
+    const float4 a = boxes[2 * (size_t)fin], b = boxes[2 * (size_t)fin + 1];
+    const i32 primCount = C::sum(__float_as_int(a.w));
+    const int outIdx = (offsets[parent] - 1) * 2;
+    const int outNode = (node >= nInt) ? node - nInt + nAfter : offsets[node] - 1;
+    const V3 lo = v3(a.x, a.y, a.z), hi = v3(b.x, b.y, b.z);
+    const V3 c = (lo + hi) * 0.5f;
+    const V3 bl = (lo - c) * 1.0001f + c;
+    const V3 bh = (hi - c) * 1.0001f + c;
+    BvhChild r;
+    r.lo[0] = bl.x; r.lo[1] = bl.y; r.lo[2] = bl.z;
+    r.hi[0] = bh.x; r.hi[1] = bh.y; r.hi[2] = bh.z;
+    r.index = outNode;
+    r.primCount = primCount;
+    out[outIdx + off] = r;
+}
+
+// World box of each instance (feeds the scene box of the instance level).
+__global__ __launch_bounds__(kThreads) void k_inst_boxes(InstPrims set, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 lo, hi;
+    set.box(i, lo, hi);
+    float* o = out + 6 * (size_t)i;
+    o[0] = lo.x; o[1] = lo.y; o[2] = lo.z;
+    o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
+}
+
+inline int blocks_for(long long n) { return (int)((n + kThreads - 1) / kThreads); }
+
+template <class T>
+void d2h(hipStream_t s, T* dst, const void* src, size_t count) {
+    if (!dst || count == 0) return;
+    MFH_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    MFH_HIP(hipStreamSynchronize(s));
+}
+
+// Shared orchestration (build_lbvh.cu:586-780). Returns the number of
+// internal nodes after the collapse; primIds and nodes receive the result.
+template <class P, class C>
+int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& nodes, DevBuf& primIds,
+                  float4* triVerts, StageSink* sink) {
+    BuildScratch& s = ctx.scratch;
+    hipStream_t st = ctx.stream;
+    s.keysA.reserve(8 * (size_t)n);
+    s.keysB.reserve(8 * (size_t)n);
+    s.idsA.reserve(4 * (size_t)n);
+    s.idsB.reserve(4 * (size_t)n);
+    s.parents.reserve(4 * (size_t)(2 * n - 1));
+    s.children.reserve(8 * (size_t)std::max(n - 1, 1));
+    s.boxes.reserve(32 * (size_t)(2 * n - 1));
+    s.counters.reserve(4 * (size_t)std::max(n - 1, 1));
+    s.flags.reserve((size_t)std::max(n - 1, 1));
+    s.offsets.reserve(4 * (size_t)std::max(n - 1, 1));
+    primIds.reserve(4 * (size_t)n);
+    Box6 bb;
+    std::memcpy(bb.v, bbox, sizeof(bb.v));
+
+    u64* keys = s.keysA.as<u64>();
+    i32* ids = s.idsA.as<i32>();
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_morton<P>), dim3(blocks_for(n)), dim3(kThreads), 0, st, set, n, bb, keys,
+                       ids);
+    MFH_HIP(hipGetLastError());
+    if (sink) d2h(st, sink->unsortedKeys, keys, n);
+    radix_sort_pairs(keys, s.keysB.as<u64>(), ids, s.idsB.as<i32>(), n, 0, 64, s, st);
+    if (sink) {
+        d2h(st, sink->sortedKeys, keys, n);
+        d2h(st, sink->sortedIds, ids, n);
+    }
+    MFH_HIP(hipMemcpyAsync(primIds.ptr, ids, 4 * (size_t)n, hipMemcpyDeviceToDevice, st));
+    i32* parents = s.parents.as<i32>();
+    i32* children = s.children.as<i32>();
+    float4* boxes = s.boxes.as<float4>();
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, keys, n, parents, children);
+        MFH_HIP(hipMemsetAsync(s.counters.ptr, 0, 4 * (size_t)(n - 1), st));
+    }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit<P, C>), dim3(blocks_for(n)), dim3(kThreads), 0, st, set, n, ids,
+                       parents, children, boxes, s.counters.as<u32>(), s.flags.as<uint8_t>(), triVerts);
+    MFH_HIP(hipGetLastError());
+    if (n == 1) {
+        MFH_HIP(hipStreamSynchronize(st));
+        return 0;
+    }
+    i32* offsets = s.offsets.as<i32>();
+    hipLaunchKernelGGL(k_mark, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, n - 1, parents,
+                       s.flags.as<uint8_t>(), offsets);
+    scan_inclusive_i32(offsets, offsets, n - 1, s, st);
+    nodes.reserve(64 * (size_t)(n - 1));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st, n,
+                       parents, boxes, offsets, nodes.as<BvhChild>());
+    MFH_HIP(hipGetLastError());
+    i32 nAfter = 0;
+    MFH_HIP(hipMemcpyAsync(&nAfter, offsets + (n - 2), 4, hipMemcpyDeviceToHost, st));
+    MFH_HIP(hipStreamSynchronize(st));
+    if (sink) {
+        d2h(st, sink->parents, parents, 2 * (size_t)n - 1);
+        d2h(st, sink->boxes, boxes, 8 * (size_t)(2 * n - 1));
+        d2h(st, sink->offsets, offsets, (size_t)n - 1);
+    }
+    return nAfter;
+}
+
+}  // namespace
+
+void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink) {
+    const int n = lod.numTris;
+    MFH_REQUIRE(n >= 1, "LoD has no triangles");
+    if (bbox) std::memcpy(lod.box, bbox, sizeof(lod.box));
+    TriPrims set{lod.verts.as<float>(), lod.tris.as<u32>()};
+    lod.triVerts.reserve(48 * (size_t)n);
+    lod.numInternal = build_generic<TriPrims, LodCost>(ctx, set, n, lod.box, lod.nodes, lod.primIds,
+                                                       lod.triVerts.as<float4>(), sink);
+    lod.built = true;
+}
+
+void build_scene_bvh(Context& ctx, Scene& sc, StageSink* sink) {
+    const int n = sc.numInstances;
+    MFH_REQUIRE(n >= 1, "scene has no instances");
+    InstPrims set{sc.dWorldToInst.as<float>(), sc.dInstLod.as<i32>(), sc.dLods.as<DevLod>()};
+    // Scene box: union of the instance world boxes (exact min/max).
+    DevBuf& tmp = ctx.scratch.misc;
+    tmp.reserve(24 * (size_t)n);
+    hipLaunchKernelGGL(k_inst_boxes, dim3(blocks_for(n)), dim3(kThreads), 0, ctx.stream, set, n, tmp.as<float>());
+    MFH_HIP(hipGetLastError());
+    std::vector<float> ib(6 * (size_t)n);
+    d2h(ctx.stream, ib.data(), tmp.ptr, ib.size());
+    for (int k = 0; k < 3; ++k) {
+        float lo = ib[k], hi = ib[3 + k];
+        for (int i = 1; i < n; ++i) {
+            const float a = ib[6 * (size_t)i + k], b = ib[6 * (size_t)i + 3 + k];
+            lo = (a < lo) ? a : lo;
+            hi = (hi < b) ? b : hi;
+        }
+        sc.box[k] = lo;
+        sc.box[3 + k] = hi;
+    }
+    if (n == 1) {
+        sc.tlasNumInternal = 0;
+        return;
+    }
+    sc.tlasNumInternal =
+        build_generic<InstPrims, InstCost>(ctx, set, n, sc.box, sc.tlasNodes, sc.tlasPrimIds, nullptr, sink);
+}
+
+}  // namespace mfh

@@ -1,0 +1,157 @@
+// Host-side internals of the HIP library behind include/mufflon_hip.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mfh {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum : int {
+    kOk = 0,
+    kInvalidArgument = 1,
+    kHipError = 2,
+    kOutOfMemory = 3,
+    kBadState = 4,
+};
+
+#define MFH_HIP(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            throw ::mfh::Error(::mfh::kHipError, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define MFH_REQUIRE(cond, msg)                                                 \
+    do {                                                                       \
+        if (!(cond)) throw ::mfh::Error(::mfh::kInvalidArgument, (msg));       \
+    } while (0)
+
+// Device buffer owned by a context; grows on demand, never shrinks.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) MFH_HIP(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+        if (hipMalloc(&ptr, n) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(kOutOfMemory, "hipMalloc of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// Scratch of one LBVH build (sizes for n primitives).
+struct BuildScratch {
+    DevBuf keysA, keysB, idsA, idsB;  // sort ping-pong
+    DevBuf parents;                   // 2n-1 (reference encoding, left child as ~parent)
+    DevBuf children;                  // 2(n-1): left, right node of each internal node
+    DevBuf boxes;                     // (2n-1) * 32 bytes: {lo, countBits}, {hi, cost}
+    DevBuf counters;                  // n-1 arrival counters
+    DevBuf flags;                     // n-1 bytes: bit0 count capped, bit1 leaf cheaper than split
+    DevBuf offsets;                   // n-1 inclusive scan of the keep marks
+    DevBuf hist;                      // radix digit counts
+    DevBuf partials;                  // scan partial sums
+    DevBuf misc;                      // small values (scene box, counts)
+};
+
+// Host copies of the intermediate stages, filled when a build is asked for them.
+struct StageSink {
+    uint64_t* unsortedKeys = nullptr;
+    uint64_t* sortedKeys = nullptr;
+    int32_t* sortedIds = nullptr;
+    int32_t* parents = nullptr;
+    float* boxes = nullptr;
+    int32_t* offsets = nullptr;
+};
+
+struct Lod {
+    int32_t numVerts = 0, numTris = 0;
+    DevBuf verts;     // float xyz per vertex
+    DevBuf tris;      // u32 x3 per triangle
+    float box[6] = {0, 0, 0, 0, 0, 0};
+    bool built = false;
+    int32_t numInternal = 0;  // internal nodes of the collapsed BVH
+    DevBuf nodes;             // numInternal * 64 bytes
+    DevBuf primIds;           // numTris, sorted order
+    DevBuf triVerts;          // numTris * 3 float4 in sorted order: {v0, id}, {v1, 0}, {v2, 0}
+};
+
+struct Scene {
+    int32_t numInstances = 0;
+    std::vector<int32_t> instLod;
+    std::vector<float> worldToInst;
+    DevBuf dInstLod, dWorldToInst, dLods;
+    float box[6] = {0, 0, 0, 0, 0, 0};
+    int32_t tlasNumInternal = 0;
+    DevBuf tlasNodes, tlasPrimIds;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t evBegin = nullptr, evEnd = nullptr;
+    std::vector<Lod*> lods;
+    std::vector<Scene*> scenes;
+    BuildScratch scratch;
+    int lastBuilt = -1;
+};
+
+// Device view of one LoD, as the traversal kernels read it.
+struct DevLod {
+    const float4* triVerts;
+    const void* nodes;
+    int32_t numInternal;
+    int32_t numPrims;
+    float box[6];
+    int32_t pad[2];
+};
+
+// LBVH over a triangle LoD; fills lod.nodes / primIds / triVerts / numInternal.
+void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink);
+// LBVH over the instances of a scene (instance-level hierarchy).
+void build_scene_bvh(Context& ctx, Scene& sc, StageSink* sink);
+
+// Stable LSD radix sort of (u64 key, i32 value) pairs; result ends in keysA/idsA.
+void radix_sort_pairs(uint64_t* keysA, uint64_t* keysB, int32_t* idsA, int32_t* idsB, int n, int beginBit,
+                      int endBit, BuildScratch& s, hipStream_t stream);
+// Inclusive prefix sum of int32 values, in place allowed.
+void scan_inclusive_i32(const int32_t* in, int32_t* out, int n, BuildScratch& s, hipStream_t stream);
+// Exclusive prefix sum of uint32 values, in place allowed.
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int n, BuildScratch& s, hipStream_t stream);
+
+struct TraceParams {
+    int32_t numInstances;
+    const float* worldToInst;
+    const int32_t* instLod;
+    const DevLod* lods;
+    const void* tlasNodes;
+    const int32_t* tlasPrimIds;
+    int32_t tlasNumInternal;
+};
+TraceParams trace_params(const Scene& sc);
+void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary,
+                          int blockSize, hipStream_t stream);
+void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, int blockSize,
+                      hipStream_t stream);
+
+}  // namespace mfh

@@ -1,0 +1,238 @@
+"""ctypes binding of the HIP library (include/mufflon_hip.h).
+
+The library is loaded from mufflon_amd/lib/libmufflon_hip.so, built in-tree
+by mufflon_amd.build. There is no fallback: if the library is missing or
+fails to load, the import of the device raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmufflon_hip.so")
+_lib = None
+
+i32 = C.c_int32
+u32p = C.POINTER(C.c_uint32)
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+u64p = C.POINTER(C.c_uint64)
+vp = C.c_void_p
+
+# name -> argtypes; every function returns int status
+_SIGS = {
+    "mfh_create": [i32, C.POINTER(vp)],
+    "mfh_destroy": [vp],
+    "mfh_synchronize": [vp],
+    "mfh_alloc": [vp, C.c_size_t, C.POINTER(vp)],
+    "mfh_free": [vp, vp],
+    "mfh_upload": [vp, vp, vp, C.c_size_t],
+    "mfh_download": [vp, vp, vp, C.c_size_t],
+    "mfh_lod_create": [vp, f32p, i32, u32p, i32, i32p],
+    "mfh_lbvh_build": [vp, i32, f32p],
+    "mfh_lbvh_get": [vp, i32, i32p, vp, i32p],
+    "mfh_lbvh_build_stages": [vp, i32, f32p, u64p, u64p, i32p, i32p, f32p, i32p],
+    "mfh_scene_create": [vp, i32p, f32p, i32, i32p],
+    "mfh_scene_get": [vp, i32, f32p, i32p, vp, i32p],
+    "mfh_trace_closest": [vp, i32, vp, i32, vp, vp],
+    "mfh_trace_any": [vp, i32, vp, i32, vp],
+    "mfh_timer_begin": [vp],
+    "mfh_timer_end": [vp, f32p],
+}
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError("HIP library not built: %s (run __graft_entry__.build())" % _LIB_PATH)
+    lib = C.CDLL(_LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.mfh_last_error.argtypes = []
+    lib.mfh_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+class HipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mfh error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class DeviceBuffer:
+    """Device allocation owned by a context (freed explicitly or with it)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = vp()
+        ctx._call("mfh_alloc", ctx.handle, self.nbytes, C.byref(p))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx._call("mfh_upload", self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes)
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._call("mfh_download", self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx._call("mfh_free", self.ctx.handle, self.ptr)
+            self.ptr = None
+
+
+class HipContext:
+    """One device context (mfh_create); the Python face of the C-ABI."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = vp()
+        self._call("mfh_create", int(device), C.byref(h))
+        self.handle = h.value
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, name)(*args)
+        if rc != 0:
+            raise HipError(rc, self.lib.mfh_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._call("mfh_destroy", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        self._call("mfh_synchronize", self.handle)
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, max(arr.nbytes, 1))
+        if arr.nbytes:
+            b.upload(arr)
+        return b
+
+    # geometry / BVH -------------------------------------------------------
+    def lod_create(self, vertices, tris):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        out = i32()
+        self._call("mfh_lod_create", self.handle, _ptr(v, f32p), len(v), _ptr(t, u32p), len(t), C.byref(out))
+        return out.value
+
+    def lbvh_build(self, lod, bbox=None):
+        b = None if bbox is None else np.ascontiguousarray(bbox, dtype=np.float32)
+        self._call("mfh_lbvh_build", self.handle, lod, _ptr(b, f32p))
+
+    def lbvh_get(self, lod, num_prims):
+        n = i32()
+        self._call("mfh_lbvh_get", self.handle, lod, C.byref(n), None, None)
+        nodes = np.zeros(max(n.value, 0) * 64, dtype=np.uint8)
+        ids = np.zeros(num_prims, dtype=np.int32)
+        self._call("mfh_lbvh_get", self.handle, lod, C.byref(n), nodes.ctypes.data if len(nodes) else None,
+                   _ptr(ids, i32p))
+        return n.value, nodes, ids
+
+    def lbvh_build_stages(self, lod, num_prims, bbox=None):
+        n = num_prims
+        b = None if bbox is None else np.ascontiguousarray(bbox, dtype=np.float32)
+        uk = np.zeros(n, np.uint64)
+        sk = np.zeros(n, np.uint64)
+        ids = np.zeros(n, np.int32)
+        par = np.zeros(max(2 * n - 1, 1), np.int32)
+        boxes = np.zeros(max(2 * n - 1, 1) * 8, np.float32)
+        offs = np.zeros(max(n - 1, 1), np.int32)
+        self._call("mfh_lbvh_build_stages", self.handle, lod, _ptr(b, f32p), _ptr(uk, u64p), _ptr(sk, u64p),
+                   _ptr(ids, i32p), _ptr(par, i32p), _ptr(boxes, f32p), _ptr(offs, i32p))
+        return dict(unsorted_keys=uk, sorted_keys=sk, sorted_ids=ids, parents=par, boxes=boxes, offsets=offs)
+
+    def scene_create(self, inst_lod, world_to_inst):
+        il = np.ascontiguousarray(inst_lod, dtype=np.int32)
+        w = np.ascontiguousarray(world_to_inst, dtype=np.float32).reshape(-1, 12)
+        assert len(il) == len(w)
+        out = i32()
+        self._call("mfh_scene_create", self.handle, _ptr(il, i32p), _ptr(w, f32p), len(il), C.byref(out))
+        return out.value
+
+    def scene_get(self, scene, num_instances):
+        box = np.zeros(6, np.float32)
+        n = i32()
+        self._call("mfh_scene_get", self.handle, scene, _ptr(box, f32p), C.byref(n), None, None)
+        nodes = np.zeros(max(n.value, 0) * 64, np.uint8)
+        ids = np.zeros(num_instances, np.int32)
+        self._call("mfh_scene_get", self.handle, scene, _ptr(box, f32p), C.byref(n),
+                   nodes.ctypes.data if len(nodes) else None, _ptr(ids, i32p))
+        return box, n.value, nodes, ids
+
+    # tracing ----------------------------------------------------------------
+    def trace_closest_dev(self, scene, d_rays, n, d_hits, d_bary=None):
+        self._call("mfh_trace_closest", self.handle, scene, d_rays.ptr, int(n), d_hits.ptr,
+                   d_bary.ptr if d_bary is not None else None)
+
+    def trace_any_dev(self, scene, d_rays, n, d_occl):
+        self._call("mfh_trace_any", self.handle, scene, d_rays.ptr, int(n), d_occl.ptr)
+
+    def trace_closest(self, scene, rays):
+        """Host convenience: rays (N,8) float32 -> t, inst, prim, bary, status."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = self.to_device(rays)
+        dh = self.alloc(16 * max(n, 1))
+        db = self.alloc(8 * max(n, 1))
+        try:
+            self.trace_closest_dev(scene, dr, n, dh, db)
+            self.synchronize()
+            h = dh.download(np.uint32, 4 * n).reshape(n, 4)
+            bary = db.download(np.float32, 2 * n).reshape(n, 2)
+        finally:
+            dr.free(); dh.free(); db.free()
+        t = h[:, 0].view(np.float32).copy()
+        return t, h[:, 1].view(np.int32).copy(), h[:, 2].view(np.int32).copy(), bary, h[:, 3].view(np.float32).copy()
+
+    def trace_any(self, scene, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = self.to_device(rays)
+        do = self.alloc(4 * max(n, 1))
+        try:
+            self.trace_any_dev(scene, dr, n, do)
+            self.synchronize()
+            o = do.download(np.int32, n)
+        finally:
+            dr.free(); do.free()
+        return o
+
+    def timer_begin(self):
+        self._call("mfh_timer_begin", self.handle)
+
+    def timer_end(self):
+        ms = C.c_float()
+        self._call("mfh_timer_end", self.handle, C.byref(ms))
+        return ms.value

@@ -1,0 +1,113 @@
+"""Procedural inputs of the benchmark workloads (synthetic, seeded).
+
+random_triangle_soup / lbvh_trace_rays: 1M random triangles with centroids
+uniform in the unit cube, edge length lognormal(mu=-5, sigma=0.5), uniformly
+random orientation (seed 7), and 2M coherent pinhole primary rays (2048x1024)
+plus 2M incoherent rays leaving random triangles in cosine-distributed
+directions about their normal.
+
+Ray records are 8 float32: origin xyz, tmax, direction xyz, 0.
+"""
+import numpy as np
+
+MAX_SCENE_SIZE = 1048576.0  # types.hpp:26, the tmax of a path segment
+SCENE_SCALE_EPS = 1e-4      # intersection.cu:16, add_epsilon offset
+
+
+def _random_rotations(rng, n):
+    # uniform random unit quaternions (Shoemake 1992)
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    q = np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                  b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)], axis=1)
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z); R[:, 0, 1] = 2 * (x * y - z * w); R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w); R[:, 1, 1] = 1 - 2 * (x * x + z * z); R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w); R[:, 2, 1] = 2 * (y * z + x * w); R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def random_triangle_soup(n, seed=7, mu=-5.0, sigma=0.5):
+    """n triangles -> (vertices (3n,3) float32, tris (n,3) uint32)."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3))
+    e = rng.lognormal(mu, sigma, n)
+    R = _random_rotations(rng, n)
+    # equilateral triangle with edge e around its centroid, in the xy plane
+    r = e / np.sqrt(3.0)
+    ang = np.array([0.0, 2 * np.pi / 3, 4 * np.pi / 3])
+    local = np.stack([np.cos(ang), np.sin(ang), np.zeros(3)], axis=1)  # (3,3)
+    pts = np.einsum("nij,kj->nki", R, local) * r[:, None, None] + c[:, None, :]
+    verts = pts.reshape(-1, 3).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    return verts, tris
+
+
+def pinhole_rays(width, height, eye=(0.5, 0.5, -1.2), target=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0),
+                 vfov_deg=50.0, tmax=MAX_SCENE_SIZE):
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    x = np.cross(np.asarray(up, np.float64), f)
+    x /= np.linalg.norm(x)
+    y = np.cross(f, x)
+    th = np.tan(np.radians(vfov_deg) * 0.5)
+    aspect = width / height
+    px = (np.arange(width) + 0.5) / width * 2 - 1
+    py = (np.arange(height) + 0.5) / height * 2 - 1
+    gx, gy = np.meshgrid(px, py)
+    d = (gx[..., None] * th * aspect * x + gy[..., None] * th * y + f).reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((len(d), 8), np.float32)
+    rays[:, 0:3] = eye
+    rays[:, 3] = tmax
+    rays[:, 4:7] = d
+    return rays
+
+
+def surface_rays(verts, tris, n, seed=7, tmax=MAX_SCENE_SIZE):
+    """Rays leaving uniformly chosen triangles at a uniform barycentric point,
+    cosine-distributed about the geometric normal, origin offset by 1e-4 along
+    the normal (the add_epsilon rule)."""
+    rng = np.random.default_rng(seed + 1)
+    t = rng.integers(0, len(tris), n)
+    v = verts.astype(np.float64)[tris[t].astype(np.int64)]  # (n,3,3)
+    u1, u2 = rng.random(n), rng.random(n)
+    su = np.sqrt(u1)
+    b0, b1 = 1 - su, u2 * su
+    p = b0[:, None] * v[:, 0] + b1[:, None] * v[:, 1] + (1 - b0 - b1)[:, None] * v[:, 2]
+    nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    flip = rng.random(n) < 0.5
+    nrm[flip] *= -1
+    # orthonormal frame (Duff et al. 2017)
+    s = np.where(nrm[:, 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (s + nrm[:, 2])
+    bb = nrm[:, 0] * nrm[:, 1] * a
+    tx = np.stack([1 + s * nrm[:, 0] ** 2 * a, s * bb, -s * nrm[:, 0]], 1)
+    ty = np.stack([bb, s + nrm[:, 1] ** 2 * a, -nrm[:, 1]], 1)
+    r1, r2 = rng.random(n), rng.random(n)
+    phi = 2 * np.pi * r1
+    ct = np.sqrt(1 - r2)
+    st = np.sqrt(r2)
+    d = (st * np.cos(phi))[:, None] * tx + (st * np.sin(phi))[:, None] * ty + ct[:, None] * nrm
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = p + SCENE_SCALE_EPS * nrm
+    rays[:, 3] = tmax
+    rays[:, 4:7] = d
+    return rays
+
+
+def lbvh_trace_workload(num_tris=1_000_000, width=2048, height=1024, num_incoherent=None, seed=7):
+    """The LBVH build + closest-hit workload: soup, coherent and incoherent rays."""
+    verts, tris = random_triangle_soup(num_tris, seed)
+    prim = pinhole_rays(width, height)
+    if num_incoherent is None:
+        num_incoherent = width * height
+    inc = surface_rays(verts, tris, num_incoherent, seed)
+    return verts, tris, np.concatenate([prim, inc], axis=0)
+
+
+IDENTITY34 = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)

@@ -1,0 +1,229 @@
+"""Oracle — test infrastructure only.
+
+ctypes access to the CPU restatement of Mufflon's CPU path
+(oracle/build/liboracle.so, built by `make -C oracle`). Only the tests, the
+smoke check and the CPU-baseline leg of the benchmark use it, as the checker
+or as the timed CPU baseline; no product code path calls it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "liboracle.so")
+_lib = None
+
+f32p = C.POINTER(C.c_float)
+i32p = C.POINTER(C.c_int32)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+
+
+class OrcScene(C.Structure):
+    _fields_ = [
+        ("numLods", C.c_int32),
+        ("lodVertOffset", i32p), ("lodTriOffset", i32p), ("lodNodeOffset", i32p), ("lodNumInternal", i32p),
+        ("verts", f32p), ("tris", u32p), ("nodes", C.c_void_p), ("primIds", i32p), ("lodBoxes", f32p),
+        ("numInstances", C.c_int32), ("worldToInst", f32p), ("instLod", i32p),
+        ("tlasNodes", C.c_void_p), ("tlasNumInternal", C.c_int32), ("tlasPrimIds", i32p),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        lib.orc_morton_point.argtypes = [f32p, f32p]
+        lib.orc_morton_point.restype = C.c_uint64
+        lib.orc_part_by_two21.argtypes = [C.c_uint32]
+        lib.orc_part_by_two21.restype = C.c_uint64
+        lib.orc_part_by_two21_batch.argtypes = [u32p, C.c_int64, u64p]
+        lib.orc_part_by_two21_batch.restype = None
+        lib.orc_morton_points.argtypes = [f32p, C.c_int64, f32p, u64p]
+        lib.orc_morton_points.restype = None
+        lib.orc_lbvh_build_tris.argtypes = [f32p, u32p, C.c_int32, f32p, u64p, u64p, i32p, i32p, f32p, i32p,
+                                            C.c_void_p, i32p]
+        lib.orc_lbvh_build_instances.argtypes = [f32p, f32p, i32p, C.c_int32, f32p, u64p, i32p, i32p, f32p, i32p,
+                                                 C.c_void_p, i32p]
+        lib.orc_instance_boxes.argtypes = [f32p, f32p, i32p, C.c_int32, f32p, f32p]
+        lib.orc_karras.argtypes = [u64p, C.c_int32, i32p]
+        lib.orc_trace_closest.argtypes = [C.POINTER(OrcScene), f32p, C.c_int32, f32p, i32p, i32p, f32p, u32p, u32p]
+        lib.orc_trace_any.argtypes = [C.POINTER(OrcScene), f32p, C.c_int32, i32p]
+        lib.orc_trace_brute.argtypes = [C.POINTER(OrcScene), f32p, C.c_int32, f32p, i32p, i32p]
+        for f in ("orc_lbvh_build_tris", "orc_lbvh_build_instances", "orc_instance_boxes", "orc_karras",
+                  "orc_trace_closest", "orc_trace_any", "orc_trace_brute"):
+            getattr(lib, f).restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def part_by_two21(x):
+    x = np.ascontiguousarray(x, np.uint32)
+    out = np.zeros(len(x), np.uint64)
+    load().orc_part_by_two21_batch(_p(x, u32p), len(x), _p(out, u64p))
+    return out
+
+
+def morton_points(points, bbox):
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    bb = np.ascontiguousarray(bbox, np.float32)
+    out = np.zeros(len(p), np.uint64)
+    load().orc_morton_points(_p(p, f32p), len(p), _p(bb, f32p), _p(out, u64p))
+    return out
+
+
+def vertex_box(verts):
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    return np.concatenate([v.min(axis=0), v.max(axis=0)]).astype(np.float32)
+
+
+def lbvh_build_tris(verts, tris, bbox=None):
+    """Serial LBVH build; returns a dict of every stage."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    n = len(t)
+    bb = vertex_box(v) if bbox is None else np.ascontiguousarray(bbox, np.float32)
+    uk = np.zeros(n, np.uint64)
+    sk = np.zeros(n, np.uint64)
+    ids = np.zeros(n, np.int32)
+    par = np.zeros(max(2 * n - 1, 1), np.int32)
+    boxes = np.zeros(max(2 * n - 1, 1) * 8, np.float32)
+    offs = np.zeros(max(n - 1, 1), np.int32)
+    nodes = np.zeros(max(n - 1, 1) * 64, np.uint8)
+    nint = C.c_int32()
+    rc = lib.orc_lbvh_build_tris(_p(v, f32p), _p(t, u32p), n, _p(bb, f32p), _p(uk, u64p), _p(sk, u64p),
+                                 _p(ids, i32p), _p(par, i32p), _p(boxes, f32p), _p(offs, i32p),
+                                 nodes.ctypes.data, C.byref(nint))
+    assert rc == 0
+    return dict(unsorted_keys=uk, sorted_keys=sk, sorted_ids=ids, parents=par, boxes=boxes, offsets=offs,
+                nodes=nodes[: nint.value * 64].copy(), num_internal=nint.value, bbox=bb)
+
+
+def instance_boxes(world_to_inst, obj_boxes, inst_lod):
+    lib = load()
+    w = np.ascontiguousarray(world_to_inst, np.float32).reshape(-1, 12)
+    ob = np.ascontiguousarray(obj_boxes, np.float32).reshape(-1, 6)
+    il = np.ascontiguousarray(inst_lod, np.int32)
+    out = np.zeros((len(il), 6), np.float32)
+    sb = np.zeros(6, np.float32)
+    lib.orc_instance_boxes(_p(w, f32p), _p(ob, f32p), _p(il, i32p), len(il), _p(out, f32p), _p(sb, f32p))
+    return out, sb
+
+
+def lbvh_build_instances(world_to_inst, obj_boxes, inst_lod, scene_box):
+    lib = load()
+    w = np.ascontiguousarray(world_to_inst, np.float32).reshape(-1, 12)
+    ob = np.ascontiguousarray(obj_boxes, np.float32).reshape(-1, 6)
+    il = np.ascontiguousarray(inst_lod, np.int32)
+    sb = np.ascontiguousarray(scene_box, np.float32)
+    n = len(il)
+    sk = np.zeros(n, np.uint64)
+    ids = np.zeros(n, np.int32)
+    par = np.zeros(max(2 * n - 1, 1), np.int32)
+    boxes = np.zeros(max(2 * n - 1, 1) * 8, np.float32)
+    offs = np.zeros(max(n - 1, 1), np.int32)
+    nodes = np.zeros(max(n - 1, 1) * 64, np.uint8)
+    nint = C.c_int32()
+    rc = lib.orc_lbvh_build_instances(_p(w, f32p), _p(ob, f32p), _p(il, i32p), n, _p(sb, f32p), _p(sk, u64p),
+                                      _p(ids, i32p), _p(par, i32p), _p(boxes, f32p), _p(offs, i32p),
+                                      nodes.ctypes.data, C.byref(nint))
+    assert rc == 0
+    return dict(sorted_keys=sk, sorted_ids=ids, parents=par, boxes=boxes, offsets=offs,
+                nodes=nodes[: nint.value * 64].copy(), num_internal=nint.value)
+
+
+def karras(sorted_keys):
+    lib = load()
+    k = np.ascontiguousarray(sorted_keys, np.uint64)
+    par = np.zeros(2 * len(k) - 1, np.int32)
+    assert lib.orc_karras(_p(k, u64p), len(k), _p(par, i32p)) == 0
+    return par
+
+
+class Scene:
+    """Flat multi-LoD, multi-instance scene for the oracle tracer."""
+
+    def __init__(self, lods, inst_lod, world_to_inst, tlas=None):
+        # lods: list of dict(verts, tris, nodes, num_internal, prim_ids, box)
+        self._keep = []
+        vo, to, no = [0], [0], [0]
+        for L in lods:
+            vo.append(vo[-1] + len(L["verts"]))
+            to.append(to[-1] + len(L["tris"]))
+            no.append(no[-1] + L["num_internal"])
+        self.verts = np.ascontiguousarray(np.concatenate([np.asarray(L["verts"], np.float32).reshape(-1, 3)
+                                                          for L in lods]))
+        self.tris = np.ascontiguousarray(np.concatenate([np.asarray(L["tris"], np.uint32).reshape(-1, 3)
+                                                         for L in lods]))
+        self.nodes = np.ascontiguousarray(np.concatenate([L["nodes"] for L in lods] + [np.zeros(64, np.uint8)]))
+        self.prim_ids = np.ascontiguousarray(np.concatenate([L["prim_ids"] for L in lods]).astype(np.int32))
+        self.boxes = np.ascontiguousarray(np.concatenate([L["box"] for L in lods]).astype(np.float32))
+        self.vo = np.array(vo, np.int32)
+        self.to = np.array(to, np.int32)
+        self.no = np.array(no, np.int32)
+        self.ni = np.array([L["num_internal"] for L in lods], np.int32)
+        self.w = np.ascontiguousarray(world_to_inst, np.float32).reshape(-1, 12)
+        self.il = np.ascontiguousarray(inst_lod, np.int32)
+        if tlas is None:
+            tlas = dict(nodes=np.zeros(64, np.uint8), num_internal=0, prim_ids=np.zeros(1, np.int32))
+        self.tn = np.ascontiguousarray(tlas["nodes"] if len(tlas["nodes"]) else np.zeros(64, np.uint8))
+        self.tp = np.ascontiguousarray(tlas["prim_ids"], np.int32)
+        s = OrcScene()
+        s.numLods = len(lods)
+        s.lodVertOffset = _p(self.vo, i32p); s.lodTriOffset = _p(self.to, i32p)
+        s.lodNodeOffset = _p(self.no, i32p); s.lodNumInternal = _p(self.ni, i32p)
+        s.verts = _p(self.verts, f32p); s.tris = _p(self.tris, u32p)
+        s.nodes = self.nodes.ctypes.data; s.primIds = _p(self.prim_ids, i32p); s.lodBoxes = _p(self.boxes, f32p)
+        s.numInstances = len(self.il); s.worldToInst = _p(self.w, f32p); s.instLod = _p(self.il, i32p)
+        s.tlasNodes = self.tn.ctypes.data; s.tlasNumInternal = int(tlas["num_internal"])
+        s.tlasPrimIds = _p(self.tp, i32p)
+        self.s = s
+
+
+def trace_closest(scene, rays, counts=False):
+    lib = load()
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(r)
+    t = np.zeros(n, np.float32)
+    inst = np.zeros(n, np.int32)
+    prim = np.zeros(n, np.int32)
+    bary = np.zeros((n, 2), np.float32)
+    nc = np.zeros(n, np.uint32) if counts else None
+    pc = np.zeros(n, np.uint32) if counts else None
+    lib.orc_trace_closest(C.byref(scene.s), _p(r, f32p), n, _p(t, f32p), _p(inst, i32p), _p(prim, i32p),
+                          _p(bary, f32p), _p(nc, u32p) if counts else None, _p(pc, u32p) if counts else None)
+    if counts:
+        return t, inst, prim, bary, nc, pc
+    return t, inst, prim, bary
+
+
+def trace_any(scene, rays):
+    lib = load()
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    o = np.zeros(len(r), np.int32)
+    lib.orc_trace_any(C.byref(scene.s), _p(r, f32p), len(r), _p(o, i32p))
+    return o
+
+
+def trace_brute(scene, rays):
+    lib = load()
+    r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(r)
+    t = np.zeros(n, np.float32)
+    inst = np.zeros(n, np.int32)
+    prim = np.zeros(n, np.int32)
+    lib.orc_trace_brute(C.byref(scene.s), _p(r, f32p), n, _p(t, f32p), _p(inst, i32p), _p(prim, i32p))
+    return t, inst, prim
