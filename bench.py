@@ -1,0 +1,177 @@
+"""Benchmark: LBVH build + closest-hit traversal on one MI355X.
+
+Workload (the configuration the ray-throughput metric is quoted on): 1,000,000
+random triangles (centroids uniform in the unit cube, edge length
+lognormal(-5, 0.5), random orientation, fp32, seed 7) and 4,194,304 rays:
+2048x1024 coherent pinhole primary rays plus 2,097,152 incoherent rays leaving
+random triangles in cosine-distributed directions. One step = LBVH build of the
+1M triangles (Morton, radix sort, Karras, refit, collapse) + closest-hit trace of
+all 4M rays, with every input resident in HBM before the timed region.
+
+Prints one JSON line. value = rays / step time (Mrays/s, build included).
+Also reported: LBVH build ms, trace-only Mrays/s, the HBM roofline of the
+trace kernel (algorithmic bytes from the per-ray node/triangle visit counts,
+over the kernel's event-timed duration), and the reference CPU path (the C++
+restatement under oracle/) timed on the host cores as a baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of one closest-hit ray: ray in, hit +
+# barycentrics out, 64 B per internal node visit, 48 B per triangle test
+RAY_IO_BYTES = 32 + 16 + 8
+NODE_BYTES = 64
+TRI_BYTES = 48
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--triangles", type=int, default=1_000_000)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
+    return ap.parse_args()
+
+
+def cpu_baseline(verts, tris, rays):
+    """The C++ restatement of the reference CPU path (oracle/) on the host
+    cores: serial LBVH build (thrust::seq order) + OpenMP closest hit over all
+    rays. Timed once after one warm-up build on a 100k-triangle sample."""
+    from oracle import oracle as O
+    cores = os.cpu_count() or 1
+    O.lbvh_build_tris(verts[: 300_000], tris[:100_000])  # warm-up
+    t0 = time.perf_counter()
+    b = O.lbvh_build_tris(verts, tris)
+    t1 = time.perf_counter()
+    lod = dict(verts=verts, tris=tris, nodes=b["nodes"], num_internal=b["num_internal"],
+               prim_ids=b["sorted_ids"], box=b["bbox"])
+    from mufflon_amd.scenes import IDENTITY34
+    scene = O.Scene([lod], np.array([0], np.int32), IDENTITY34.reshape(1, 12))
+    t2 = time.perf_counter()
+    O.trace_closest(scene, rays)
+    t3 = time.perf_counter()
+    build_s, trace_s = t1 - t0, t3 - t2
+    return dict(value=len(rays) / (build_s + trace_s) / 1e6, unit="Mrays/s", cores=cores, kind="port",
+                sample="full workload: serial LBVH build of all %d triangles (%.3f s) + closest hit of all %d "
+                       "rays on %d OpenMP threads (%.3f s, %.2f Mrays/s)" %
+                       (len(tris), build_s, len(rays), cores, trace_s, len(rays) / trace_s / 1e6),
+                lbvh_build_ms=build_s * 1e3, trace_mrays_s=len(rays) / trace_s / 1e6)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from mufflon_amd import scenes as S
+    from mufflon_amd.hip import HipContext
+
+    t_gen = time.perf_counter()
+    verts, tris, rays = S.lbvh_trace_workload(args.triangles, args.width, args.height)
+    n_rays = len(rays)
+    print("workload: %d triangles, %d rays (%.1f s to generate)" % (len(tris), n_rays, time.perf_counter() - t_gen),
+          file=sys.stderr, flush=True)
+
+    ctx = HipContext(0)
+    lod = ctx.lod_create(verts, tris)
+    bbox = np.concatenate([verts.min(axis=0), verts.max(axis=0)]).astype(np.float32)
+    d_rays = ctx.to_device(rays)
+    d_hits = ctx.alloc(16 * n_rays)
+    d_bary = ctx.alloc(8 * n_rays)
+    ctx.lbvh_build(lod, bbox)
+    scene = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+
+    build_ms, trace_ms, step_ms = [], [], []
+
+    def step(record):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.timer_begin()
+        ctx.lbvh_build(lod, bbox)
+        b = ctx.timer_end()
+        ctx.timer_begin()
+        ctx.trace_closest_dev(scene, d_rays, n_rays, d_hits, d_bary)
+        t = ctx.timer_end()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        if record:
+            build_ms.append(b)
+            trace_ms.append(t)
+            step_ms.append((t1 - t0) * 1e3)
+
+    for _ in range(args.warmup):
+        step(False)
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    ctx.synchronize()
+    total_s = time.perf_counter() - t_start
+    ms_per_step = total_s * 1e3 / args.steps
+    value = n_rays / (ms_per_step / 1e3) / 1e6
+    out = {
+        "metric": "Mrays/s",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "LBVH build + closest hit: %d random triangles (seed 7), %dx%d pinhole rays + %d "
+                        "incoherent surface rays" % (len(tris), args.width, args.height, n_rays - args.width * args.height),
+            "triangles": len(tris), "rays": n_rays,
+        },
+        "lbvh_build_ms": round(float(np.mean(build_ms)), 4),
+        "lbvh_build_ms_min": round(float(np.min(build_ms)), 4),
+        "trace_ms": round(float(np.mean(trace_ms)), 4),
+        "trace_mrays_s": round(n_rays / (np.mean(trace_ms) / 1e3) / 1e6, 3),
+    }
+    if not args.profile_only:
+        d_stats = ctx.alloc(8 * n_rays)
+        ctx.trace_closest_stats_dev(scene, d_rays, n_rays, d_hits, None, d_stats)
+        ctx.synchronize()
+        st = d_stats.download(np.uint32, 2 * n_rays).reshape(-1, 2).astype(np.int64)
+        v_node, v_prim = int(st[:, 0].sum()), int(st[:, 1].sum())
+        traffic_alg = n_rays * RAY_IO_BYTES + NODE_BYTES * v_node + TRI_BYTES * v_prim
+        achieved = traffic_alg / (np.mean(trace_ms) / 1e3) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "trace_closest_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        out["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_trace<closest>", "algorithmic_bytes": traffic_alg,
+            "node_visits_per_ray": round(v_node / n_rays, 3), "tri_tests_per_ray": round(v_prim / n_rays, 3),
+        }
+        if not args.no_cpu_baseline:
+            print("cpu baseline ...", file=sys.stderr, flush=True)
+            out["cpu_baseline"] = cpu_baseline(verts, tris, rays)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

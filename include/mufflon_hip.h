@@ -76,7 +76,8 @@ int mfh_lbvh_build_stages(mfh_context* ctx, int32_t lodId, const float* bbox, ui
 /* Scene of instances over built LoDs. worldToInstance: 12 floats per instance
  * (row-major Mat3x4). With more than one instance the instance-level LBVH is
  * built (LBVHBuilder::build(SceneDescriptor&), build_lbvh.cu:796-806).
- * The LoD BVHs must be built first. */
+ * The LoD BVHs must be built first. A LoD rebuilt later is picked up by the
+ * next trace call on the scene (its table and instance level are refreshed). */
 int mfh_scene_create(mfh_context* ctx, const int32_t* instLod, const float* worldToInstance, int32_t numInstances,
                      int32_t* outSceneId);
 /* Scene box (6 floats) and the instance-level BVH (numInternal, nodes, ids). */
@@ -92,6 +93,11 @@ int mfh_scene_get(mfh_context* ctx, int32_t sceneId, float* outBox, int32_t* out
  * barycentric weights (w0, w1) of v0 and v1. Asynchronous on the stream. */
 int mfh_trace_closest(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, void* dHits,
                       float* dBary);
+/* Closest hit as above that also writes, per ray, 2 x uint32 {internal nodes
+ * visited (an instance's object-box test counts as one), triangles tested}:
+ * the traversal work that the algorithmic byte count is made of. */
+int mfh_trace_closest_stats(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, void* dHits,
+                            float* dBary, uint32_t* dStats);
 /* Any hit (any_intersection, intersection.cu:624-753): 1 occluded, 0 free,
  * -1 stack overflow. Asynchronous on the stream. */
 int mfh_trace_any(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, int32_t* dOccluded);

@@ -73,6 +73,35 @@ void upload(Context& c, DevBuf& b, const void* src, size_t bytes) {
 
 }  // namespace
 
+namespace mfh {
+
+void refresh_scene(Context& c, Scene& s) {
+    bool stale = s.lodGeneration.size() != c.lods.size();
+    for (size_t i = 0; !stale && i < c.lods.size(); ++i)
+        stale = c.lods[i] && c.lods[i]->generation != s.lodGeneration[i];
+    if (!stale) return;
+    std::vector<DevLod> lods(c.lods.size());
+    s.lodGeneration.assign(c.lods.size(), 0);
+    for (size_t i = 0; i < c.lods.size(); ++i) {
+        std::memset(&lods[i], 0, sizeof(DevLod));
+        const Lod* l = c.lods[i];
+        if (!l || !l->built) continue;
+        s.lodGeneration[i] = l->generation;
+        lods[i].triVerts = l->triVerts.as<float4>();
+        lods[i].nodes = l->nodes.ptr;
+        lods[i].numInternal = l->numInternal;
+        lods[i].numPrims = l->numTris;
+        std::memcpy(lods[i].box, l->box, sizeof(l->box));
+    }
+    for (int32_t li : s.instLod)
+        MFH_REQUIRE(c.lods[li]->built, "instance refers to a LoD whose BVH is not built");
+    upload(c, s.dLods, lods.data(), sizeof(DevLod) * lods.size());
+    build_scene_bvh(c, s, nullptr);
+    MFH_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace mfh
+
 extern "C" {
 
 const char* mfh_last_error(void) { return g_lastError.c_str(); }
@@ -202,6 +231,7 @@ int mfh_lbvh_build(mfh_context* ctx, int32_t lodId, const float* bbox) {
         Context& c = ctx_of(ctx);
         Lod& l = lod_of(c, lodId);
         build_lod_bvh(c, l, bbox, nullptr);
+        ++l.generation;
         c.lastBuilt = lodId;
     });
 }
@@ -214,6 +244,7 @@ int mfh_lbvh_build_stages(mfh_context* ctx, int32_t lodId, const float* bbox, ui
         Lod& l = lod_of(c, lodId);
         StageSink sink{unsortedKeys, sortedKeys, sortedIds, parents, boxes, offsets};
         build_lod_bvh(c, l, bbox, &sink);
+        ++l.generation;
         c.lastBuilt = lodId;
     });
 }
@@ -237,17 +268,6 @@ int mfh_scene_create(mfh_context* ctx, const int32_t* instLod, const float* worl
         Context& c = ctx_of(ctx);
         MFH_REQUIRE(instLod && worldToInstance && outSceneId, "null pointer");
         MFH_REQUIRE(numInstances > 0, "scene without instances");
-        std::vector<DevLod> lods(c.lods.size());
-        for (size_t i = 0; i < c.lods.size(); ++i) {
-            std::memset(&lods[i], 0, sizeof(DevLod));
-            const Lod* l = c.lods[i];
-            if (!l || !l->built) continue;
-            lods[i].triVerts = l->triVerts.as<float4>();
-            lods[i].nodes = l->nodes.ptr;
-            lods[i].numInternal = l->numInternal;
-            lods[i].numPrims = l->numTris;
-            std::memcpy(lods[i].box, l->box, sizeof(l->box));
-        }
         for (int32_t i = 0; i < numInstances; ++i) {
             const int32_t li = instLod[i];
             MFH_REQUIRE(li >= 0 && li < (int32_t)c.lods.size() && c.lods[li], "instance refers to an unknown LoD");
@@ -260,8 +280,7 @@ int mfh_scene_create(mfh_context* ctx, const int32_t* instLod, const float* worl
             s->worldToInst.assign(worldToInstance, worldToInstance + 12 * (size_t)numInstances);
             upload(c, s->dInstLod, instLod, 4 * (size_t)numInstances);
             upload(c, s->dWorldToInst, worldToInstance, 48 * (size_t)numInstances);
-            upload(c, s->dLods, lods.data(), sizeof(DevLod) * lods.size());
-            build_scene_bvh(c, *s, nullptr);
+            refresh_scene(c, *s);
             MFH_HIP(hipStreamSynchronize(c.stream));
         } catch (...) {
             s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release();
@@ -297,8 +316,23 @@ int mfh_trace_closest(mfh_context* ctx, int32_t sceneId, const float* dRays, int
         MFH_REQUIRE(numRays >= 0, "negative ray count");
         if (numRays == 0) return;
         MFH_REQUIRE(dRays && dHits, "null pointer");
+        refresh_scene(c, s);
         launch_trace_closest(trace_params(s), dRays, numRays, static_cast<float4*>(dHits),
-                             reinterpret_cast<float2*>(dBary), 0, c.stream);
+                             reinterpret_cast<float2*>(dBary), nullptr, c.stream);
+    });
+}
+
+int mfh_trace_closest_stats(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, void* dHits,
+                            float* dBary, uint32_t* dStats) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Scene& s = scene_of(c, sceneId);
+        MFH_REQUIRE(numRays >= 0, "negative ray count");
+        if (numRays == 0) return;
+        MFH_REQUIRE(dRays && dHits && dStats, "null pointer");
+        refresh_scene(c, s);
+        launch_trace_closest(trace_params(s), dRays, numRays, static_cast<float4*>(dHits),
+                             reinterpret_cast<float2*>(dBary), reinterpret_cast<uint2*>(dStats), c.stream);
     });
 }
 
@@ -309,7 +343,8 @@ int mfh_trace_any(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t
         MFH_REQUIRE(numRays >= 0, "negative ray count");
         if (numRays == 0) return;
         MFH_REQUIRE(dRays && dOccluded, "null pointer");
-        launch_trace_any(trace_params(s), dRays, numRays, dOccluded, 0, c.stream);
+        refresh_scene(c, s);
+        launch_trace_any(trace_params(s), dRays, numRays, dOccluded, c.stream);
     });
 }
 

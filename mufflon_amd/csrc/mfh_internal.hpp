@@ -90,6 +90,7 @@ struct Lod {
     DevBuf tris;      // u32 x3 per triangle
     float box[6] = {0, 0, 0, 0, 0, 0};
     bool built = false;
+    uint64_t generation = 0;  // incremented by every build
     int32_t numInternal = 0;  // internal nodes of the collapsed BVH
     DevBuf nodes;             // numInternal * 64 bytes
     DevBuf primIds;           // numTris, sorted order
@@ -98,6 +99,7 @@ struct Lod {
 
 struct Scene {
     int32_t numInstances = 0;
+    std::vector<uint64_t> lodGeneration;  // generation of every LoD when the table was uploaded
     std::vector<int32_t> instLod;
     std::vector<float> worldToInst;
     DevBuf dInstLod, dWorldToInst, dLods;
@@ -149,9 +151,11 @@ struct TraceParams {
     int32_t tlasNumInternal;
 };
 TraceParams trace_params(const Scene& sc);
-void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary,
-                          int blockSize, hipStream_t stream);
-void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, int blockSize,
-                      hipStream_t stream);
+void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary, uint2* stats,
+                          hipStream_t stream);
+void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, hipStream_t stream);
+// Re-upload the LoD table of a scene (and rebuild its instance level) when a
+// LoD was rebuilt after the scene was created.
+void refresh_scene(Context& ctx, Scene& sc);
 
 }  // namespace mfh

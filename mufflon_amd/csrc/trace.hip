@@ -78,10 +78,12 @@ struct Level {
     int numInternal;
 };
 
-template <bool kAny>
+// kStats: also count per ray the internal nodes visited (object-box tests at
+// instance entry included) and the triangles tested.
+template <bool kAny, bool kStats>
 __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4* __restrict__ rays, int n,
                                                    float4* __restrict__ hits, float2* __restrict__ bary,
-                                                   int* __restrict__ occluded) {
+                                                   int* __restrict__ occluded, uint2* __restrict__ stats) {
     __shared__ int sStack[kLdsDepth * kThreads];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -106,6 +108,7 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
     const float4* objTris = nullptr;
     int nodeAddr = 0, primCount = 2, primOffset = 0;
     bool done = false;
+    unsigned nodeVisits = 0, primTests = 0;
 
     // world_to_object_space: transform the ray, test the object box.
     auto to_object = [&](int inst) -> bool {
@@ -121,6 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
         const float objTMax = hitT * sc;
         const DevLod& L = p.lods[p.instLod[inst]];
         float ot;
+        if (kStats) ++nodeVisits;
         if (!slab(L.box, L.box + 3, fr, objTMax, ot)) return false;
         cur = fr;
         scale = sc;
@@ -143,6 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
         const float4 a = tris[3 * (size_t)pos], b = tris[3 * (size_t)pos + 1], c = tris[3 * (size_t)pos + 2];
         float t, w0, w1;
         const int prim = __float_as_int(a.w);
+        if (kStats) ++primTests;
         if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
             better(t, inst, prim, hitT, hInst, hPrim)) {
             hitT = t;
@@ -181,6 +186,7 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
         }
         while (nodeAddr < lv.numInternal && primCount > 0) {
             const float4* nd = reinterpret_cast<const float4*>(lv.bvh + 2 * (size_t)nodeAddr);
+            if (kStats) ++nodeVisits;
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
             const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
             const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
@@ -246,6 +252,7 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
         hits[i] = make_float4(hitT, __int_as_float(hInst), __int_as_float(hPrim), st.overflow ? 1.0f : 0.0f);
         if (bary) bary[i] = make_float2(hw0, hw1);
     }
+    if (kStats) stats[i] = make_uint2(nodeVisits, primTests);
 }
 
 }  // namespace
@@ -262,21 +269,23 @@ TraceParams trace_params(const Scene& sc) {
     return p;
 }
 
-void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary, int blockSize,
+void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary, uint2* stats,
                           hipStream_t stream) {
-    (void)blockSize;
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_trace<false>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, p,
-                       reinterpret_cast<const float4*>(rays), n, hits, bary, nullptr);
+    const dim3 grid((n + kThreads - 1) / kThreads);
+    if (stats)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<false, true>), grid, dim3(kThreads), 0, stream, p,
+                           reinterpret_cast<const float4*>(rays), n, hits, bary, nullptr, stats);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<false, false>), grid, dim3(kThreads), 0, stream, p,
+                           reinterpret_cast<const float4*>(rays), n, hits, bary, nullptr, nullptr);
     MFH_HIP(hipGetLastError());
 }
 
-void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, int blockSize,
-                      hipStream_t stream) {
-    (void)blockSize;
+void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, hipStream_t stream) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_trace<true>, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, stream, p,
-                       reinterpret_cast<const float4*>(rays), n, nullptr, nullptr, occluded);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<true, false>), dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                       stream, p, reinterpret_cast<const float4*>(rays), n, nullptr, nullptr, occluded, nullptr);
     MFH_HIP(hipGetLastError());
 }
 

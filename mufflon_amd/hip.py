@@ -35,6 +35,7 @@ _SIGS = {
     "mfh_scene_create": [vp, i32p, f32p, i32, i32p],
     "mfh_scene_get": [vp, i32, f32p, i32p, vp, i32p],
     "mfh_trace_closest": [vp, i32, vp, i32, vp, vp],
+    "mfh_trace_closest_stats": [vp, i32, vp, i32, vp, vp, vp],
     "mfh_trace_any": [vp, i32, vp, i32, vp],
     "mfh_timer_begin": [vp],
     "mfh_timer_end": [vp, f32p],
@@ -195,6 +196,25 @@ class HipContext:
     def trace_closest_dev(self, scene, d_rays, n, d_hits, d_bary=None):
         self._call("mfh_trace_closest", self.handle, scene, d_rays.ptr, int(n), d_hits.ptr,
                    d_bary.ptr if d_bary is not None else None)
+
+    def trace_closest_stats_dev(self, scene, d_rays, n, d_hits, d_bary, d_stats):
+        self._call("mfh_trace_closest_stats", self.handle, scene, d_rays.ptr, int(n), d_hits.ptr,
+                   d_bary.ptr if d_bary is not None else None, d_stats.ptr)
+
+    def trace_stats(self, scene, rays):
+        """Per-ray (internal nodes visited, triangles tested) of closest hit."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = self.to_device(rays)
+        dh = self.alloc(16 * max(n, 1))
+        ds = self.alloc(8 * max(n, 1))
+        try:
+            self.trace_closest_stats_dev(scene, dr, n, dh, None, ds)
+            self.synchronize()
+            st = ds.download(np.uint32, 2 * n).reshape(n, 2)
+        finally:
+            dr.free(); dh.free(); ds.free()
+        return st[:, 0].copy(), st[:, 1].copy()
 
     def trace_any_dev(self, scene, d_rays, n, d_occl):
         self._call("mfh_trace_any", self.handle, scene, d_rays.ptr, int(n), d_occl.ptr)

@@ -38,7 +38,7 @@ def random_triangle_soup(n, seed=7, mu=-5.0, sigma=0.5):
     r = e / np.sqrt(3.0)
     ang = np.array([0.0, 2 * np.pi / 3, 4 * np.pi / 3])
     local = np.stack([np.cos(ang), np.sin(ang), np.zeros(3)], axis=1)  # (3,3)
-    pts = np.einsum("nij,kj->nki", R, local) * r[:, None, None] + c[:, None, :]
+    pts = np.matmul(R, local.T).transpose(0, 2, 1) * r[:, None, None] + c[:, None, :]
     verts = pts.reshape(-1, 3).astype(np.float32)
     tris = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
     return verts, tris
@@ -72,13 +72,18 @@ def surface_rays(verts, tris, n, seed=7, tmax=MAX_SCENE_SIZE):
     the normal (the add_epsilon rule)."""
     rng = np.random.default_rng(seed + 1)
     t = rng.integers(0, len(tris), n)
-    v = verts.astype(np.float64)[tris[t].astype(np.int64)]  # (n,3,3)
+    idx = tris[t].astype(np.int64)
+    v0, v1, v2 = (verts[idx[:, k]].astype(np.float64) for k in range(3))
     u1, u2 = rng.random(n), rng.random(n)
     su = np.sqrt(u1)
     b0, b1 = 1 - su, u2 * su
-    p = b0[:, None] * v[:, 0] + b1[:, None] * v[:, 1] + (1 - b0 - b1)[:, None] * v[:, 2]
-    nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
-    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    p = b0[:, None] * v0 + b1[:, None] * v1 + (1 - b0 - b1)[:, None] * v2
+    e1, e2 = v1 - v0, v2 - v0
+    nrm = np.empty_like(e1)
+    nrm[:, 0] = e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1]
+    nrm[:, 1] = e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2]
+    nrm[:, 2] = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+    nrm /= np.sqrt((nrm * nrm).sum(axis=1))[:, None]
     flip = rng.random(n) < 0.5
     nrm[flip] *= -1
     # orthonormal frame (Duff et al. 2017)
@@ -92,7 +97,7 @@ def surface_rays(verts, tris, n, seed=7, tmax=MAX_SCENE_SIZE):
     ct = np.sqrt(1 - r2)
     st = np.sqrt(r2)
     d = (st * np.cos(phi))[:, None] * tx + (st * np.sin(phi))[:, None] * ty + ct[:, None] * nrm
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d /= np.sqrt((d * d).sum(axis=1))[:, None]
     rays = np.zeros((n, 8), np.float32)
     rays[:, 0:3] = p + SCENE_SCALE_EPS * nrm
     rays[:, 3] = tmax

@@ -79,3 +79,15 @@ def test_lbvh_trace_workload_full(hip_ctx, oracle):
     oscene, _ = U.oracle_scene(oracle, [(v, t)], np.array([0], np.int32), w)
     hit = _compare(hip_ctx, oracle, sc, oscene, rays)
     assert hit.sum() > 100_000
+
+
+def test_visit_counts_match_oracle(hip_ctx, oracle):
+    """The GPU traversal visits the same nodes and triangles as the oracle."""
+    v, t = S.random_triangle_soup(50_000, seed=31, mu=-4.5)
+    w = S.IDENTITY34.reshape(1, 12)
+    sc = _gpu_scene(hip_ctx, [(v, t)], np.array([0], np.int32), w)
+    oscene, _ = U.oracle_scene(oracle, [(v, t)], np.array([0], np.int32), w)
+    rays = np.concatenate([S.pinhole_rays(128, 64), S.surface_rays(v, t, 8000, seed=31)])
+    gn, gp = hip_ctx.trace_stats(sc, rays)
+    _, _, _, _, on, op = oracle.trace_closest(oscene, rays, counts=True)
+    assert np.array_equal(gn, on) and np.array_equal(gp, op)
