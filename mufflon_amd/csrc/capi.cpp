@@ -146,7 +146,7 @@ int mfh_destroy(mfh_context* ctx) {
             delete s;
         }
         BuildScratch& b = c.scratch;
-        for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.boxes, &b.counters,
+        for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.ranges, &b.pending, &b.boxes, &b.counters,
                           &b.flags, &b.offsets, &b.hist, &b.partials, &b.misc})
             d->release();
         if (c.evBegin) (void)hipEventDestroy(c.evBegin);

@@ -208,7 +208,7 @@ __device__ __forceinline__ int lcp(const u64* keys, int n, int i1, int i2, u64 k
 }
 
 __global__ __launch_bounds__(kThreads) void k_karras(const u64* __restrict__ keys, int n, i32* __restrict__ parents,
-                                                    i32* __restrict__ children) {
+                                                    i32* __restrict__ children, int2* __restrict__ ranges) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n - 1) return;
     const u64 k1 = keys[idx];
@@ -238,6 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_karras(const u64* __restrict__ key
     parents[right] = idx;
     children[2 * idx] = left;
     children[2 * idx + 1] = right;
+    ranges[idx] = make_int2(min(idx, j), max(idx, j));
     if (idx == 0) parents[0] = kTreeHead;
 }
 
@@ -246,57 +247,110 @@ __device__ __forceinline__ int parent_of(const i32* parents, int node) {
     return p < 0 ? ~p : p;
 }
 
+// Merge of internal node `node` from its two finished children, in the fixed
+// operand order (right, left); writes the node's box, count, cost and flags.
+template <class C>
+__device__ __forceinline__ void merge_node(int node, const i32* __restrict__ children, float4* boxes,
+                                           uint8_t* flags) {
+    const int L = children[2 * node], R = children[2 * node + 1];
+    const float4 lLo = boxes[2 * (size_t)L];
+    const float4 lHi = boxes[2 * (size_t)L + 1];
+    const float4 rLo = boxes[2 * (size_t)R];
+    const float4 rHi = boxes[2 * (size_t)R + 1];
+    const V3 Llo = v3(lLo.x, lLo.y, lLo.z), Lhi = v3(lHi.x, lHi.y, lHi.z);
+    const V3 Rlo = v3(rLo.x, rLo.y, rLo.z), Rhi = v3(rHi.x, rHi.y, rHi.z);
+    const V3 nlo = vmin(Rlo, Llo), nhi = vmax(Rhi, Lhi);
+    const float sa = surface(nlo, nhi);
+    const float saR = surface(Rlo, Rhi), saL = surface(Llo, Lhi);
+    const i32 cR = __float_as_int(rLo.w), cL = __float_as_int(lLo.w);
+    const float cost = C::Ct + saR / sa * C::cost(cR) + saL / sa * C::cost(cL);
+    const i32 c = C::add(cR, cL);
+    boxes[2 * (size_t)node] = make_float4(nlo.x, nlo.y, nlo.z, __int_as_float(c));
+    boxes[2 * (size_t)node + 1] = make_float4(nhi.x, nhi.y, nhi.z, cost);
+    flags[node] = (uint8_t)((C::capped(c) ? 1 : 0) | (C::cost(c) < cost ? 2 : 0));
+}
+
+// Refit, phase 1. Block b owns the leaves [b*kRefitTile, (b+1)*kRefitTile).
+// Every internal node whose leaf range lies inside that span has both
+// subtrees finished by threads of the same workgroup, so the arrival
+// protocol needs only workgroup-scope ordering (no L2 write-back or
+// invalidate across XCDs). A finished subtree whose parent straddles the
+// span is appended to `pending` for phase 2.
+constexpr int kRefitThreads = 256;
+constexpr int kRefitIpt = 8;
+constexpr int kRefitTile = kRefitThreads * kRefitIpt;
+
 template <class P, class C>
-__global__ __launch_bounds__(kThreads) void k_refit(P set, int n, const i32* __restrict__ ids,
-                                                   const i32* __restrict__ parents,
-                                                   const i32* __restrict__ children, float4* boxes,
-                                                   u32* counters, uint8_t* flags, float4* triVerts) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n) return;
+__global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, const i32* __restrict__ ids,
+                                                              const i32* __restrict__ parents,
+                                                              const i32* __restrict__ children,
+                                                              const int2* __restrict__ ranges, float4* boxes,
+                                                              u32* counters, uint8_t* flags, float4* triVerts,
+                                                              i32* pending, u32* pendingCount) {
     const int nInt = n - 1;
-    const int prim = ids[idx];
-    V3 lo, hi;
-    if constexpr (std::is_same<P, TriPrims>::value) {
-        // Triangle LoD: also lay the triangle out in sorted order for traversal.
-        V3 a, b, c;
-        set.tri(prim, a, b, c);
-        lo = vmin(vmin(a, b), c);
-        hi = vmax(vmax(a, b), c);
-        triVerts[3 * (size_t)idx] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
-        triVerts[3 * (size_t)idx + 1] = make_float4(b.x, b.y, b.z, 0.0f);
-        triVerts[3 * (size_t)idx + 2] = make_float4(c.x, c.y, c.z, 0.0f);
-    } else {
-        set.box(prim, lo, hi);
+    const int first = blockIdx.x * kRefitTile;
+    const int last = min(first + kRefitTile, n) - 1;
+    for (int k = 0; k < kRefitIpt; ++k) {
+        const int idx = first + k * kRefitThreads + (int)threadIdx.x;
+        if (idx > last) break;
+        const int prim = ids[idx];
+        V3 lo, hi;
+        if constexpr (std::is_same<P, TriPrims>::value) {
+            // Triangle LoD: also lay the triangle out in BVH order for traversal.
+            V3 a, b, c;
+            set.tri(prim, a, b, c);
+            lo = vmin(vmin(a, b), c);
+            hi = vmax(vmax(a, b), c);
+            triVerts[3 * (size_t)idx] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
+            triVerts[3 * (size_t)idx + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+            triVerts[3 * (size_t)idx + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+        } else {
+            set.box(prim, lo, hi);
+        }
+        const i32 cnt = C::leaf();
+        const int leaf = idx + nInt;
+        boxes[2 * (size_t)leaf] = make_float4(lo.x, lo.y, lo.z, __int_as_float(cnt));
+        boxes[2 * (size_t)leaf + 1] = make_float4(hi.x, hi.y, hi.z, C::cost(cnt));
+        if (n == 1) continue;
+        int child = leaf;
+        int node = parent_of(parents, leaf);
+        for (int depth = 0; depth < kMaxDepth; ++depth) {
+            const int2 r = ranges[node];
+            if (r.x < first || r.y > last) {
+                pending[atomicAdd(pendingCount, 1u)] = child;
+                break;
+            }
+            __threadfence_block();
+            if (atomicAdd(&counters[node], 1u) == 0) break;  // sibling subtree not finished yet
+            __threadfence_block();
+            merge_node<C>(node, children, boxes, flags);
+            if (node == 0) break;
+            child = node;
+            node = parent_of(parents, node);
+        }
     }
-    const i32 cnt = C::leaf();
-    const int leaf = idx + nInt;
-    boxes[2 * (size_t)leaf] = make_float4(lo.x, lo.y, lo.z, __int_as_float(cnt));
-    boxes[2 * (size_t)leaf + 1] = make_float4(hi.x, hi.y, hi.z, C::cost(cnt));
-    if (n == 1) return;
-    int node = parent_of(parents, leaf);
-    for (int depth = 0; depth < kMaxDepth; ++depth) {
-        __threadfence();
-        const u32 arrived = atomicAdd(&counters[node], 1u);
-        if (arrived == 0) return;  // the sibling subtree is not finished yet
-        __threadfence();
-        const int L = children[2 * node], R = children[2 * node + 1];
-        const float4 lLo = boxes[2 * (size_t)L];
-        const float4 lHi = boxes[2 * (size_t)L + 1];
-        const float4 rLo = boxes[2 * (size_t)R];
-        const float4 rHi = boxes[2 * (size_t)R + 1];
-        const V3 Llo = v3(lLo.x, lLo.y, lLo.z), Lhi = v3(lHi.x, lHi.y, lHi.z);
-        const V3 Rlo = v3(rLo.x, rLo.y, rLo.z), Rhi = v3(rHi.x, rHi.y, rHi.z);
-        const V3 nlo = vmin(Rlo, Llo), nhi = vmax(Rhi, Lhi);
-        const float sa = surface(nlo, nhi);
-        const float saR = surface(Rlo, Rhi), saL = surface(Llo, Lhi);
-        const i32 cR = __float_as_int(rLo.w), cL = __float_as_int(lLo.w);
-        const float cost = C::Ct + saR / sa * C::cost(cR) + saL / sa * C::cost(cL);
-        const i32 c = C::add(cR, cL);
-        boxes[2 * (size_t)node] = make_float4(nlo.x, nlo.y, nlo.z, __int_as_float(c));
-        boxes[2 * (size_t)node + 1] = make_float4(nhi.x, nhi.y, nhi.z, cost);
-        flags[node] = (uint8_t)((C::capped(c) ? 1 : 0) | (C::cost(c) < cost ? 2 : 0));
-        if (node == 0) return;
-        node = parent_of(parents, node);
+}
+
+// Refit, phase 2: one workgroup climbs from every pending subtree root
+// through the nodes that straddle tile borders (their number is about the
+// tile count times the tree depth), with workgroup-scope ordering again.
+constexpr int kTopThreads = 1024;
+template <class C>
+__global__ __launch_bounds__(kTopThreads) void k_refit_top(const i32* __restrict__ parents,
+                                                          const i32* __restrict__ children, float4* boxes,
+                                                          u32* counters, uint8_t* flags, const i32* pending,
+                                                          const u32* pendingCount) {
+    const int count = (int)*pendingCount;
+    for (int k = threadIdx.x; k < count; k += kTopThreads) {
+        int node = parent_of(parents, pending[k]);
+        for (int depth = 0; depth < kMaxDepth; ++depth) {
+            __threadfence_block();
+            if (atomicAdd(&counters[node], 1u) == 0) break;
+            __threadfence_block();
+            merge_node<C>(node, children, boxes, flags);
+            if (node == 0) break;
+            node = parent_of(parents, node);
+        }
     }
 }
 
@@ -393,6 +447,8 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
     s.idsB.reserve(4 * (size_t)n);
     s.parents.reserve(4 * (size_t)(2 * n - 1));
     s.children.reserve(8 * (size_t)std::max(n - 1, 1));
+    s.ranges.reserve(8 * (size_t)std::max(n - 1, 1));
+    s.pending.reserve(4 * (size_t)n + 4);
     s.boxes.reserve(32 * (size_t)(2 * n - 1));
     s.counters.reserve(4 * (size_t)std::max(n - 1, 1));
     s.flags.reserve((size_t)std::max(n - 1, 1));
@@ -417,12 +473,22 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
     i32* children = s.children.as<i32>();
     float4* boxes = s.boxes.as<float4>();
     if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, keys, n, parents, children);
+        hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, keys, n, parents, children,
+                           s.ranges.as<int2>());
         MFH_HIP(hipMemsetAsync(s.counters.ptr, 0, 4 * (size_t)(n - 1), st));
     }
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit<P, C>), dim3(blocks_for(n)), dim3(kThreads), 0, st, set, n, ids,
-                       parents, children, boxes, s.counters.as<u32>(), s.flags.as<uint8_t>(), triVerts);
+    i32* pending = s.pending.as<i32>();
+    u32* pendingCount = reinterpret_cast<u32*>(pending + n);
+    MFH_HIP(hipMemsetAsync(pendingCount, 0, 4, st));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_tiles<P, C>), dim3((n + kRefitTile - 1) / kRefitTile),
+                       dim3(kRefitThreads), 0, st, set, n, ids, parents, children, s.ranges.as<int2>(), boxes,
+                       s.counters.as<u32>(), s.flags.as<uint8_t>(), triVerts, pending, pendingCount);
     MFH_HIP(hipGetLastError());
+    if (n > 1) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_top<C>), dim3(1), dim3(kTopThreads), 0, st, parents, children,
+                           boxes, s.counters.as<u32>(), s.flags.as<uint8_t>(), pending, pendingCount);
+        MFH_HIP(hipGetLastError());
+    }
     if (n == 1) {
         MFH_HIP(hipStreamSynchronize(st));
         return 0;
