@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (torch), whose FETCH_SIZE / "
+                         "WRITE_SIZE counters scale the traversal kernel's to HBM bytes")
     return ap.parse_args()
 
 
@@ -86,6 +89,16 @@ def main():
     n_rays = len(rays)
     print("workload: %d triangles, %d rays (%.1f s to generate)" % (len(tris), n_rays, time.perf_counter() - t_gen),
           file=sys.stderr, flush=True)
+
+    if args.copy_calibration_mib:
+        import torch
+        a = torch.empty(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
+        a.fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     ctx = HipContext(0)
     lod = ctx.lod_create(verts, tris)

@@ -49,145 +49,97 @@ __device__ __forceinline__ bool better(float t, int inst, int prim, float hitT, 
     return prim < hPrim;
 }
 
-struct Stack {
-    int* lds;  // column of this lane: lds[k * kThreads]
-    int spill[kStackCap - kLdsDepth];
-    int sp;
-    bool overflow;
-    __device__ __forceinline__ void push(int v) {
-        if (sp < kLdsDepth)
-            lds[sp * kThreads] = v;
-        else if (sp < kStackCap)
-            spill[sp - kLdsDepth] = v;
-        else {
-            overflow = true;
-            return;
-        }
-        ++sp;
-    }
-    __device__ __forceinline__ int pop() {
-        --sp;
-        return (sp < kLdsDepth) ? lds[sp * kThreads] : spill[sp - kLdsDepth];
-    }
+// The instance transition of world_to_object_space (intersection.cu:321-349):
+// transform the world ray into the instance, scale tmax, test the object box.
+struct ObjectEntry {
+    FastRay ray;
+    float scale, tmax;
+    bool hit;
 };
+__device__ __forceinline__ ObjectEntry enter_object(const TraceParams& p, int inst, V3 wo, V3 wd, float hitT,
+                                                    DevLod& lod) {
+    ObjectEntry e;
+    const float4* w = reinterpret_cast<const float4*>(p.worldToInst + 12 * (size_t)inst);
+    const float4 a = w[0], b = w[1], c = w[2];
+    const M34 M{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w}};
+    const V3 d = xform_dir(M, wd);
+    e.scale = len(d);
+    e.ray = make_fast_ray(xform_point(M, wo), d / e.scale);
+    e.tmax = hitT * e.scale;
+    lod = p.lods[p.instLod[inst]];
+    float ot;
+    e.hit = slab(lod.box, lod.box + 3, e.ray, e.tmax, ot);
+    return e;
+}
 
-struct Level {
-    const BvhChild* bvh;
-    const float4* tris;
-    const int* primIds;
-    int numInternal;
-};
-
-// kStats: also count per ray the internal nodes visited (object-box tests at
-// instance entry included) and the triangles tested.
-template <bool kAny, bool kStats>
+// kAny: stop at the first accepted hit. kStats: count per ray the internal
+// nodes visited (an object-box test at instance entry counts as one) and the
+// triangles tested. kSingle: the scene has one instance (no instance level).
+template <bool kAny, bool kStats, bool kSingle>
 __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4* __restrict__ rays, int n,
                                                    float4* __restrict__ hits, float2* __restrict__ bary,
                                                    int* __restrict__ occluded, uint2* __restrict__ stats) {
     __shared__ int sStack[kLdsDepth * kThreads];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    int* const lstack = sStack + threadIdx.x;
+    int spill[kStackCap - kLdsDepth];
+    int sp = 0;
+    bool overflow = false;
+#define MFH_PUSH(v)                                   \
+    do {                                              \
+        if (sp < kLdsDepth)                           \
+            lstack[sp * kThreads] = (v);              \
+        else if (sp < kStackCap)                      \
+            spill[sp - kLdsDepth] = (v);              \
+        if (sp < kStackCap) ++sp; else overflow = true; \
+    } while (0)
+#define MFH_POP() (--sp, (sp < kLdsDepth) ? lstack[sp * kThreads] : spill[sp - kLdsDepth])
+
     const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
-    const FastRay world = make_fast_ray(v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z));
+    const V3 wo = v3(r0.x, r0.y, r0.z), wd = v3(r1.x, r1.y, r1.z);
+    FastRay cur = make_fast_ray(wo, wd);
     float hitT = r0.w;
     int hInst = -1, hPrim = -1;
     float hw0 = 0.0f, hw1 = 0.0f;
-
-    Stack st;
-    st.lds = sStack + threadIdx.x;
-    st.sp = 0;
-    st.overflow = false;
-
-    const Level top{(const BvhChild*)p.tlasNodes, nullptr, p.tlasPrimIds, p.tlasNumInternal};
-    Level lv = top;
-    FastRay cur = world;
     float scale = 1.0f;
-    bool inObj = false;
-    int curInst = -1;
-    int objPrims = 0;
-    const float4* objTris = nullptr;
-    int nodeAddr = 0, primCount = 2, primOffset = 0;
-    bool done = false;
     unsigned nodeVisits = 0, primTests = 0;
 
-    // world_to_object_space: transform the ray, test the object box.
-    auto to_object = [&](int inst) -> bool {
-        M34 M;
-        const float4* w = reinterpret_cast<const float4*>(p.worldToInst + 12 * (size_t)inst);
-        const float4 a = w[0], b = w[1], c = w[2];
-        M.m[0] = a.x; M.m[1] = a.y; M.m[2] = a.z; M.m[3] = a.w;
-        M.m[4] = b.x; M.m[5] = b.y; M.m[6] = b.z; M.m[7] = b.w;
-        M.m[8] = c.x; M.m[9] = c.y; M.m[10] = c.z; M.m[11] = c.w;
-        const V3 d = xform_dir(M, world.d);
-        const float sc = len(d);
-        const FastRay fr = make_fast_ray(xform_point(M, world.o), d / sc);
-        const float objTMax = hitT * sc;
-        const DevLod& L = p.lods[p.instLod[inst]];
-        float ot;
-        if (kStats) ++nodeVisits;
-        if (!slab(L.box, L.box + 3, fr, objTMax, ot)) return false;
-        cur = fr;
-        scale = sc;
-        hitT = objTMax;
-        objPrims = L.numPrims;
-        objTris = L.triVerts;
-        lv.bvh = (const BvhChild*)L.nodes;
-        lv.tris = L.triVerts;
-        lv.numInternal = L.numInternal;
-        return true;
-    };
-    auto to_world = [&]() {
-        cur = world;
-        hitT /= scale;
-        scale = 1.0f;
-        lv = top;
-        inObj = false;
-    };
-    auto test_tri = [&](const float4* tris, int pos, int inst) -> bool {
-        const float4 a = tris[3 * (size_t)pos], b = tris[3 * (size_t)pos + 1], c = tris[3 * (size_t)pos + 2];
-        float t, w0, w1;
-        const int prim = __float_as_int(a.w);
-        if (kStats) ++primTests;
-        if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
-            better(t, inst, prim, hitT, hInst, hPrim)) {
-            hitT = t;
-            hInst = inst;
-            hPrim = prim;
-            hw0 = w0;
-            hw1 = w1;
-            return true;
-        }
-        return false;
-    };
+    // current level: the instance BVH or the BVH of the entered object
+    const BvhChild* bvh = (const BvhChild*)p.tlasNodes;
+    const float4* tris = nullptr;
+    int numInternal = p.tlasNumInternal;
+    int objPrims = 0;
+    bool inObj = false;
+    int curInst = -1;
+    int nodeAddr = 0, primCount = 2, primOffset = 0;
+    bool done = false;
 
-    if (p.numInstances == 1) {
-        if (!to_object(0))
+    if (kSingle) {
+        DevLod lod;
+        const ObjectEntry e = enter_object(p, 0, wo, wd, hitT, lod);
+        if (kStats) ++nodeVisits;
+        if (!e.hit) {
             primCount = 0;
-        else {
+        } else {
+            cur = e.ray;
+            scale = e.scale;
+            hitT = e.tmax;
+            bvh = (const BvhChild*)lod.nodes;
+            tris = lod.triVerts;
+            numInternal = lod.numInternal;
             inObj = true;
             curInst = 0;
-            if (objPrims == 1) primCount = 1;
+            if (lod.numPrims == 1) primCount = 1;
         }
     }
 
-    while (!done && (st.sp > 0 || primCount > 0)) {
-        if (primCount == 0) {
-            nodeAddr = st.pop();
-            if (nodeAddr == kSentinel) {
-                if (st.sp == 0) break;
-                to_world();
-                nodeAddr = st.pop();
-            }
-            if (nodeAddr >= lv.numInternal)
-                primCount = st.pop();
-            else
-                primCount = 2;
-            primOffset = 0;
-        }
-        while (nodeAddr < lv.numInternal && primCount > 0) {
-            const float4* nd = reinterpret_cast<const float4*>(lv.bvh + 2 * (size_t)nodeAddr);
-            if (kStats) ++nodeVisits;
+    while (true) {
+        // descend through internal nodes, nearer child first, farther pushed
+        while (nodeAddr < numInternal && primCount > 0) {
+            const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
             const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+            if (kStats) ++nodeVisits;
             const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
             const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
             const int lIdx = __float_as_int(q1.z), lCnt = __float_as_int(q1.w);
@@ -200,59 +152,128 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
             if (t0 && t1) {
                 const bool farLeft = c1 < c0;
                 const int push = farLeft ? lIdx : rIdx;
-                if (push >= lv.numInternal) st.push(farLeft ? lCnt : rCnt);
-                st.push(push);
+                if (push >= numInternal) MFH_PUSH(farLeft ? lCnt : rCnt);
+                MFH_PUSH(push);
             }
             const bool nearLeft = c0 <= c1;
             nodeAddr = nearLeft ? lIdx : rIdx;
-            if (nodeAddr >= lv.numInternal) {
-                primCount = nearLeft ? lCnt : rCnt;
-                primOffset = 0;
-            } else
-                primCount = 2;
+            primCount = (nodeAddr >= numInternal) ? (nearLeft ? lCnt : rCnt) : 2;
+            primOffset = 0;
             if (!t0 && !t1) primCount = 0;
         }
-        if (st.overflow) break;
-        if (nodeAddr >= lv.numInternal && primCount > 0) {
-            const int leafId = nodeAddr - lv.numInternal + primOffset;
+        if (overflow) break;
+        // the primitives of a leaf, one after the other
+        bool descend = false;
+        while (nodeAddr >= numInternal && primCount > 0) {
+            const int leafId = nodeAddr - numInternal + primOffset;
             ++primOffset;
             --primCount;
-            if (!inObj) {
+            if (!kSingle && !inObj) {
                 const int inst = p.tlasPrimIds[leafId];
-                if (to_object(inst)) {
-                    if (objPrims == 1) {
-                        if (test_tri(objTris, 0, inst) && kAny) done = true;
-                        // back to the instance level at once
-                        cur = world;
-                        hitT /= scale;
-                        scale = 1.0f;
-                        lv = top;
-                    } else {
-                        if (primCount > 0) {
-                            st.push(primCount);
-                            st.push(nodeAddr + primOffset);
-                        }
-                        st.push(kSentinel);
-                        inObj = true;
-                        nodeAddr = 0;
-                        primCount = 2;
-                        primOffset = 0;
-                        curInst = inst;
+                DevLod lod;
+                const ObjectEntry e = enter_object(p, inst, wo, wd, hitT, lod);
+                if (kStats) ++nodeVisits;
+                if (!e.hit) continue;
+                if (lod.numPrims == 1) {
+                    // single-primitive object: test it and return to the instance level
+                    const float4 a = lod.triVerts[0], b = lod.triVerts[1], c = lod.triVerts[2];
+                    float t, w0, w1;
+                    if (kStats) ++primTests;
+                    const float objT = e.tmax;
+                    float ht = objT;
+                    if (ray_triangle(e.ray, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
+                        t > 0.0f && better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
+                        ht = t;
+                        hInst = inst;
+                        hPrim = __float_as_int(a.w);
+                        hw0 = w0;
+                        hw1 = w1;
+                        if (kAny) done = true;
                     }
+                    hitT = ht / e.scale;
+                    if (done) break;
+                    continue;
                 }
-            } else if (test_tri(lv.tris, leafId, curInst) && kAny) {
-                done = true;
+                if (primCount > 0) {
+                    MFH_PUSH(primCount);
+                    MFH_PUSH(nodeAddr + primOffset);
+                }
+                MFH_PUSH(kSentinel);
+                cur = e.ray;
+                scale = e.scale;
+                hitT = e.tmax;
+                bvh = (const BvhChild*)lod.nodes;
+                tris = lod.triVerts;
+                numInternal = lod.numInternal;
+                objPrims = lod.numPrims;
+                inObj = true;
+                curInst = inst;
+                nodeAddr = 0;
+                primCount = objPrims == 1 ? 1 : 2;
+                primOffset = 0;
+                descend = true;
+                break;
+            }
+            const float4 a = tris[3 * (size_t)leafId], b = tris[3 * (size_t)leafId + 1],
+                         c = tris[3 * (size_t)leafId + 2];
+            float t, w0, w1;
+            const int prim = __float_as_int(a.w);
+            if (kStats) ++primTests;
+            if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
+                better(t, curInst, prim, hitT, hInst, hPrim)) {
+                hitT = t;
+                hInst = curInst;
+                hPrim = prim;
+                hw0 = w0;
+                hw1 = w1;
+                if (kAny) {
+                    done = true;
+                    break;
+                }
             }
         }
+        if (done || overflow) break;
+        if (descend) continue;
+        // pop the next subtree (or return from an object to the instance level)
+        if (sp == 0) break;
+        nodeAddr = MFH_POP();
+        if (!kSingle && nodeAddr == kSentinel) {
+            if (sp == 0) break;
+            cur = make_fast_ray(wo, wd);
+            hitT /= scale;
+            scale = 1.0f;
+            bvh = (const BvhChild*)p.tlasNodes;
+            numInternal = p.tlasNumInternal;
+            inObj = false;
+            nodeAddr = MFH_POP();
+        }
+        primCount = (nodeAddr >= numInternal) ? MFH_POP() : 2;
+        primOffset = 0;
     }
+#undef MFH_PUSH
+#undef MFH_POP
     if (kAny) {
-        occluded[i] = st.overflow ? -1 : (hInst != -1 ? 1 : 0);
+        occluded[i] = overflow ? -1 : (hInst != -1 ? 1 : 0);
     } else {
-        if (inObj) to_world();
-        hits[i] = make_float4(hitT, __int_as_float(hInst), __int_as_float(hPrim), st.overflow ? 1.0f : 0.0f);
+        if (inObj) hitT /= scale;  // object_to_world_space
+        hits[i] = make_float4(hitT, __int_as_float(hInst), __int_as_float(hPrim), overflow ? 1.0f : 0.0f);
         if (bary) bary[i] = make_float2(hw0, hw1);
     }
     if (kStats) stats[i] = make_uint2(nodeVisits, primTests);
+}
+
+template <bool kAny, bool kStats>
+void launch_trace(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary, int* occ, uint2* stats,
+                  hipStream_t stream) {
+    const dim3 grid((n + kThreads - 1) / kThreads), block(kThreads);
+    const float4* r = reinterpret_cast<const float4*>(rays);
+    if (p.numInstances == 1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, true>), grid, block, 0, stream, p, r, n, hits, bary,
+                           occ, stats);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, false>), grid, block, 0, stream, p, r, n, hits,
+                           bary, occ, stats);
+    MFH_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -272,21 +293,15 @@ TraceParams trace_params(const Scene& sc) {
 void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary, uint2* stats,
                           hipStream_t stream) {
     if (n <= 0) return;
-    const dim3 grid((n + kThreads - 1) / kThreads);
     if (stats)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<false, true>), grid, dim3(kThreads), 0, stream, p,
-                           reinterpret_cast<const float4*>(rays), n, hits, bary, nullptr, stats);
+        launch_trace<false, true>(p, rays, n, hits, bary, nullptr, stats, stream);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<false, false>), grid, dim3(kThreads), 0, stream, p,
-                           reinterpret_cast<const float4*>(rays), n, hits, bary, nullptr, nullptr);
-    MFH_HIP(hipGetLastError());
+        launch_trace<false, false>(p, rays, n, hits, bary, nullptr, nullptr, stream);
 }
 
 void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, hipStream_t stream) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<true, false>), dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0,
-                       stream, p, reinterpret_cast<const float4*>(rays), n, nullptr, nullptr, occluded, nullptr);
-    MFH_HIP(hipGetLastError());
+    launch_trace<true, false>(p, rays, n, nullptr, nullptr, occluded, nullptr, stream);
 }
 
 }  // namespace mfh
