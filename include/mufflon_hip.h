@@ -102,6 +102,112 @@ int mfh_trace_closest_stats(mfh_context* ctx, int32_t sceneId, const float* dRay
  * -1 stack overflow. Asynchronous on the stream. */
 int mfh_trace_any(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t numRays, int32_t* dOccluded);
 
+/* ------------------------------------------------------------------------
+ * Path tracer (the GPU device of Mufflon's "Pathtracer"/"PT" renderer,
+ * gpu_pt.cpp / gpu_pt.cu / pt_common.hpp).
+ * ------------------------------------------------------------------------ */
+
+/* Material record (64 bytes). type follows Mufflon's Materials enum
+ * (material_definitions.hpp:25-46); textures are constant (1x1) values. */
+#define MFH_MATERIAL_EMISSIVE 0
+#define MFH_MATERIAL_LAMBERT 1
+typedef struct mfh_material {
+    int32_t type;
+    float albedo[3];   /* LAMBERT: albedo */
+    float radiance[3]; /* EMISSIVE: radiance */
+    float scale;       /* EMISSIVE: radiance scale (MatEmissive::NonTexParams) */
+    int32_t reserved[8];
+} mfh_material;
+
+/* Point light (lights.hpp:46-50): position and intensity (W/sr). */
+typedef struct mfh_point_light {
+    float position[3];
+    float intensity[3];
+} mfh_point_light;
+
+/* Pinhole camera (PinholeParams, pinhole.hpp:43-51); tanVFov = tan(vFov / 2). */
+typedef struct mfh_pinhole {
+    float position[3];
+    float tanVFov;
+    float viewDir[3];
+    float up[3];
+    int32_t width;
+    int32_t height;
+} mfh_pinhole;
+
+/* PtParameters (pt_params.hpp:8-14, parameter.hpp:327-366). */
+typedef struct mfh_pt_params {
+    int32_t seed;          /* "Seed", default 0 */
+    int32_t minPathLength; /* "Min. path length", default 0 */
+    int32_t maxPathLength; /* "Max. path length", default 16 */
+    int32_t neeCount;      /* "NEE count", default 1 */
+} mfh_pt_params;
+
+#define MFH_TARGET_RADIANCE 0 /* 3 floats per pixel, row-major */
+#define MFH_TARGET_HIT_ID 1   /* 2 floats per pixel: instance, primitive */
+
+/* Light tree as built on the host (light_tree.cpp:340-447), Mufflon layout. */
+typedef struct mfh_light_tree_info {
+    float posRootFlux;
+    uint32_t posRootType;
+    uint32_t posInternalCount;
+    uint32_t posLightCount;
+    uint32_t memBytes;
+    int32_t mapCount;
+    float dirRootFlux;
+    float backgroundFlux;
+} mfh_light_tree_info;
+
+/* Emissive triangle in world space for the light tree (AreaLightTriangleDesc,
+ * lights.hpp:77-81, plus the primitive it was made from). */
+typedef struct mfh_area_light {
+    float points[9];
+    float uv[6];
+    int32_t material;
+    int32_t instance;
+    int32_t primitive;
+} mfh_area_light;
+
+/* Host-only light-tree build (no device needed): area lights and point
+ * lights in the order Mufflon gathers them, the scene box (for the Hilbert
+ * ordering) and the materials (for the emitted flux). Capacities: outMem at
+ * least 64 * (numAreaLights + numPointLights) * 2 bytes, outMap* at least
+ * numAreaLights entries. */
+int mfh_light_tree_build(const mfh_area_light* areaLights, int32_t numAreaLights, const mfh_point_light* pointLights,
+                         int32_t numPointLights, const float* sceneBox, const mfh_material* materials,
+                         int32_t numMaterials, mfh_light_tree_info* outInfo, void* outMem, size_t memCapacity,
+                         uint64_t* outMapKeys, uint32_t* outMapCodes, int32_t mapCapacity);
+
+/* Shading attributes of a LoD: per-vertex normals (xyz) and uvs, and a
+ * material index per triangle (PolygonsDescriptor normals/uvs/matIndices). */
+int mfh_lod_set_shading(mfh_context* ctx, int32_t lodId, const float* normals, const float* uvs,
+                        const uint16_t* materialIndices);
+
+/* Path tracer over a scene. Emissive triangles of every instance become area
+ * lights (world_container.cpp:869-990); point lights are added; the light tree
+ * is built on the host and uploaded. Starts reset. Replaces
+ * render_enable_renderer + the scene-descriptor upload of
+ * RendererBase::pre_iteration (renderer_base.hpp:41-61). */
+int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materials, int32_t numMaterials,
+                  const mfh_point_light* pointLights, int32_t numPointLights, const mfh_pinhole* camera,
+                  const mfh_pt_params* params, int32_t* outPtId);
+/* Re-seed the per-pixel RNGs and clear the targets (render_reset,
+ * GpuPathTracer::post_reset, gpu_pt.cpp:33-38). */
+int mfh_pt_reset(mfh_context* ctx, int32_t ptId);
+/* Run iterations (render_iterate, interface.cpp:3611-3684); asynchronous. */
+int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations);
+/* Running mean of a target over the completed iterations (mufflon_get_target_image). */
+int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out);
+/* Index of the last completed iteration, -1 after a reset (render_get_current_iteration). */
+int mfh_pt_get_iteration(mfh_context* ctx, int32_t ptId, int32_t* outIteration);
+/* Number of NaN contributions dropped since the last reset. */
+int mfh_pt_get_nan_count(mfh_context* ctx, int32_t ptId, uint32_t* outCount);
+/* The host-built light tree: info, then (when non-NULL) the memory blob
+ * (info.memBytes), and the primitive -> tree-path map (info.mapCount entries,
+ * keys = instance << 32 | primitive, sorted). */
+int mfh_pt_get_light_tree(mfh_context* ctx, int32_t ptId, mfh_light_tree_info* outInfo, void* outMem,
+                          uint64_t* outMapKeys, uint32_t* outMapCodes);
+
 /* Event timing on the context's stream: begin records an event, end records
  * a second one, waits for it and returns the elapsed milliseconds. */
 int mfh_timer_begin(mfh_context* ctx);

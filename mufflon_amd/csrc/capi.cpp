@@ -2,56 +2,19 @@
 // Every entry point converts exceptions into a status code plus a
 // thread-local message, the convention of Mufflon's core API
 // (interface.cpp:54-83, core_get_dll_error).
-#include "../../include/mufflon_hip.h"
-
 #include <cstring>
 #include <string>
 
-#include "mfh_internal.hpp"
+#include "capi_util.hpp"
 
 using namespace mfh;
 
-struct mfh_context {
-    Context c;
-};
+std::string& mfh::last_error() {
+    thread_local std::string msg;
+    return msg;
+}
 
 namespace {
-
-thread_local std::string g_lastError;
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        g_lastError.clear();
-        return MFH_OK;
-    } catch (const Error& e) {
-        g_lastError = e.what();
-        return e.code;
-    } catch (const std::exception& e) {
-        g_lastError = e.what();
-        return MFH_ERROR_STATE;
-    } catch (...) {
-        g_lastError = "unknown error";
-        return MFH_ERROR_STATE;
-    }
-}
-
-Context& ctx_of(mfh_context* ctx) {
-    MFH_REQUIRE(ctx != nullptr, "null context");
-    MFH_HIP(hipSetDevice(ctx->c.device));
-    return ctx->c;
-}
-
-Lod& lod_of(Context& c, int32_t id) {
-    MFH_REQUIRE(id >= 0 && id < (int32_t)c.lods.size() && c.lods[id], "unknown LoD id " + std::to_string(id));
-    return *c.lods[id];
-}
-
-Scene& scene_of(Context& c, int32_t id) {
-    MFH_REQUIRE(id >= 0 && id < (int32_t)c.scenes.size() && c.scenes[id], "unknown scene id " + std::to_string(id));
-    return *c.scenes[id];
-}
 
 void vertex_box(const float* v, int32_t n, float* box) {
     for (int k = 0; k < 3; ++k) {
@@ -66,12 +29,12 @@ void vertex_box(const float* v, int32_t n, float* box) {
     }
 }
 
-void upload(Context& c, DevBuf& b, const void* src, size_t bytes) {
+}  // namespace
+
+void mfh::upload(Context& c, DevBuf& b, const void* src, size_t bytes) {
     b.reserve(bytes);
     MFH_HIP(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, c.stream));
 }
-
-}  // namespace
 
 namespace mfh {
 
@@ -104,7 +67,7 @@ void refresh_scene(Context& c, Scene& s) {
 
 extern "C" {
 
-const char* mfh_last_error(void) { return g_lastError.c_str(); }
+const char* mfh_last_error(void) { return last_error().c_str(); }
 
 int mfh_create(int32_t deviceIndex, mfh_context** outCtx) {
     return guarded([&] {
@@ -137,6 +100,7 @@ int mfh_destroy(mfh_context* ctx) {
         for (Lod* l : c.lods) {
             if (!l) continue;
             l->verts.release(); l->tris.release(); l->nodes.release(); l->primIds.release(); l->triVerts.release();
+            l->normals.release(); l->uvs.release(); l->matIdx.release();
             delete l;
         }
         for (Scene* s : c.scenes) {
@@ -145,6 +109,7 @@ int mfh_destroy(mfh_context* ctx) {
             s->tlasNodes.release(); s->tlasPrimIds.release();
             delete s;
         }
+        destroy_tracers(c);
         BuildScratch& b = c.scratch;
         for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.ranges, &b.pending, &b.boxes, &b.counters,
                           &b.flags, &b.offsets, &b.hist, &b.partials, &b.misc})
@@ -211,6 +176,8 @@ int mfh_lod_create(mfh_context* ctx, const float* vertices, int32_t numVertices,
         try {
             l->numVerts = numVertices;
             l->numTris = numTriangles;
+            l->hostVerts.assign(vertices, vertices + 3 * (size_t)numVertices);
+            l->hostTris.assign(triIndices, triIndices + 3 * (size_t)numTriangles);
             upload(c, l->verts, vertices, 12 * (size_t)numVertices);
             upload(c, l->tris, triIndices, 12 * (size_t)numTriangles);
             vertex_box(vertices, numVertices, l->box);

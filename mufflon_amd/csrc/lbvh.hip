@@ -57,13 +57,16 @@ __device__ __forceinline__ u64 part_by_two21(u32 x) {
     return r;
 }
 
+// Mufflon's sdiv (util/types.hpp): x / (d + 1e-20 * sgn(d)), sgn(0) = 1.
+__device__ __forceinline__ float sdiv(float x, float d) { return x / (d + 1e-20f * (d < 0.0f ? -1.0f : 1.0f)); }
+
 __device__ __forceinline__ u64 morton_of_point(V3 c, const Box6& bb) {
     const V3 lo = v3(bb.v[0], bb.v[1], bb.v[2]);
     const V3 span = v3(bb.v[3], bb.v[4], bb.v[5]) - lo;
     const V3 num = c - lo;
-    const float nx = span.x != 0.0f ? num.x / span.x : 0.0f;
-    const float ny = span.y != 0.0f ? num.y / span.y : 0.0f;
-    const float nz = span.z != 0.0f ? num.z / span.z : 0.0f;
+    const float nx = sdiv(num.x, span.x);
+    const float ny = sdiv(num.y, span.y);
+    const float nz = sdiv(num.z, span.z);
     const u32 qx = (u32)clampf(nx * 2097152.0f, 0.0f, 2097151.0f);
     const u32 qy = (u32)clampf(ny * 2097152.0f, 0.0f, 2097151.0f);
     const u32 qz = (u32)clampf(nz * 2097152.0f, 0.0f, 2097151.0f);

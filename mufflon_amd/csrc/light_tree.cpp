@@ -1,0 +1,357 @@
+// Host light-tree builder: LightTreeBuilder::build (light_tree.cpp:340-447),
+// create_light_tree (:119-266), fill_map (:268-302), get_flux (lights.cpp:11-90)
+// and the Hilbert ordering (sfcurves.hpp), producing Mufflon's byte layout.
+#include "light_tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace mfh {
+
+namespace {
+
+using u16 = uint16_t;
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr size_t kNodeBytes = 32;
+constexpr size_t kTriBytes = 64;
+constexpr size_t kPointBytes = 32;
+
+struct V3 {
+    float x, y, z;
+};
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 mul(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 divs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float len(V3 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// sfcurves.hpp:16-24, 40-71
+u64 part_by_two16(u16 x) {
+    u64 r = x;
+    r = (r | (r << 16)) & 0x0000ff0000ffull;
+    r = (r | (r << 8)) & 0x00f00f00f00full;
+    r = (r | (r << 4)) & 0x0c30c30c30c3ull;
+    r = (r | (r << 2)) & 0x249249249249ull;
+    return r;
+}
+u64 gray_to_binary(u64 n) {
+    n ^= n >> 32;
+    n ^= n >> 16;
+    n ^= n >> 8;
+    n ^= n >> 4;
+    n ^= n >> 2;
+    n ^= n >> 1;
+    return n;
+}
+u64 binary_to_gray(u64 n) { return n ^ (n >> 1); }
+u64 hilbert_code(u16 a, u16 b, u16 c) {
+    return gray_to_binary(part_by_two16(a) | (part_by_two16(b) << 1u) | (part_by_two16(c) << 2u));
+}
+
+// rng.hpp:12-17 and the GoldenRatio2D sampler (rng.hpp:114-140)
+u64 scramble_seed(u32 x) {
+    u64 z = (u64)x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30u)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27u)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31u);
+}
+struct GoldenRatio2D {
+    u64 state;
+    explicit GoldenRatio2D(u32 seed) : state(scramble_seed(seed)) {}
+    u64 next() {
+        u64 s = state;
+        state += 11400714819323198485ull;
+        s = binary_to_gray(s);
+        s = ((s >> 1) & 0x2222222222222222ull) | ((s << 1) & 0x4444444444444444ull) | (s & 0x9999999999999999ull);
+        s = ((s >> 2) & 0x0c0c0c0c0c0c0c0cull) | ((s << 2) & 0x3030303030303030ull) | (s & 0xc3c3c3c3c3c3c3c3ull);
+        s = ((s >> 4) & 0x00f000f000f000f0ull) | ((s << 4) & 0x0f000f000f000f00ull) | (s & 0xf00ff00ff00ff00full);
+        s = ((s >> 8) & 0x0000ff000000ff00ull) | ((s << 8) & 0x00ff000000ff0000ull) | (s & 0xff0000ffff0000ffull);
+        s = ((s >> 16) & 0x00000000ffff0000ull) | ((s << 16) & 0x0000ffff00000000ull) | (s & 0xffff00000000ffffull);
+        return s;
+    }
+};
+inline float sample_uniform(u32 u) { return (float)u / 4294967810.0f; }
+
+// float -> u32 as the reference's x86-64 build converts it (through a signed
+// 64-bit integer, then truncation).
+inline u32 f2u(float v) { return (u32)(int64_t)v; }
+
+struct PosLight {
+    bool isTri;
+    HostAreaTriangle tri;
+    HostPointLight point;
+};
+
+// The sort key reads the light description as the stored record
+// (light_tree.cpp:366-379 passes the variant's storage to get_center): for a
+// triangle it is points[0] + (points[1] + points[2]) / 3.
+V3 sort_center(const PosLight& l) {
+    if (!l.isTri) return {l.point.position[0], l.point.position[1], l.point.position[2]};
+    const V3 p0{l.tri.points[0][0], l.tri.points[0][1], l.tri.points[0][2]};
+    const V3 p1{l.tri.points[1][0], l.tri.points[1][1], l.tri.points[1][2]};
+    const V3 p2{l.tri.points[2][0], l.tri.points[2][1], l.tri.points[2][2]};
+    return add(p0, divs(add(p1, p2), 3.0f));
+}
+
+struct TriRecord {  // AreaLightTriangle (lights.hpp:91-107), 64 bytes
+    float posV[3][3];
+    u16 material;
+    u16 pad0;
+    float uvV[3][2];
+};
+static_assert(sizeof(TriRecord) == 64, "AreaLightTriangle record");
+struct PointRecord {  // PointLight (lights.hpp:46-50), 32 bytes
+    float position[3];
+    u16 mediumIndex;
+    u16 pad0;
+    float intensity[3];
+    float pad1;
+};
+static_assert(sizeof(PointRecord) == 32, "PointLight record");
+struct NodeRecord {  // LightSubTree::Node (light_tree.hpp:57-90), 32 bytes packed
+    float leftFlux;
+    u32 leftOffset;
+    u16 leftType;
+    u16 rightType;
+    u32 rightOffset;
+    float rightFlux;
+    float center[3];
+};
+static_assert(sizeof(NodeRecord) == 32, "LightSubTree::Node record");
+
+struct Builder {
+    std::vector<uint8_t>& mem;
+    const EmissionLookup& emission;
+
+    float flux_of(u32 offset, u16 type) const {
+        const uint8_t* p = mem.data() + offset;
+        if (type == kInternalNode) {
+            NodeRecord n;
+            std::memcpy(&n, p, sizeof(n));
+            return n.leftFlux + n.rightFlux;
+        }
+        if (type == kPointLight) {
+            PointRecord r;
+            std::memcpy(&r, p, sizeof(r));
+            const float f0 = r.intensity[0] * 4.f * kPi, f1 = r.intensity[1] * 4.f * kPi,
+                        f2 = r.intensity[2] * 4.f * kPi;
+            return f0 + f1 + f2;
+        }
+        // area triangle: 128 golden-ratio samples of the emission (lights.cpp:24-41)
+        TriRecord r;
+        std::memcpy(&r, p, sizeof(r));
+        const V3 e1{r.posV[1][0], r.posV[1][1], r.posV[1][2]}, e2{r.posV[2][0], r.posV[2][1], r.posV[2][2]};
+        const float area = len(cross(e1, e2)) / 2.0f;
+        u32 seed;
+        std::memcpy(&seed, &area, 4);
+        GoldenRatio2D gen(seed);
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 128; ++i) {
+            const u64 x = gen.next();
+            const float ux = sample_uniform((u32)(x & 0xffffffffull)), uy = sample_uniform((u32)(x >> 32));
+            const float s = 1.f - std::sqrt(1.f - ux);
+            const float t = (1.f - s) * uy;
+            const float u = r.uvV[0][0] + r.uvV[1][0] * s + r.uvV[2][0] * t;
+            const float v = r.uvV[0][1] + r.uvV[1][1] * s + r.uvV[2][1] * t;
+            float rad[3];
+            emission.radiance(r.material, u, v, rad);
+            for (int k = 0; k < 3; ++k) sum[k] += rad[k];
+        }
+        float f[3];
+        for (int k = 0; k < 3; ++k) f[k] = sum[k] / 128.0f * area * 2 * kPi;
+        return f[0] + f[1] + f[2];
+    }
+    V3 center_of(u32 offset, u16 type) const {
+        const uint8_t* p = mem.data() + offset;
+        if (type == kInternalNode) {
+            NodeRecord n;
+            std::memcpy(&n, p, sizeof(n));
+            return {n.center[0], n.center[1], n.center[2]};
+        }
+        if (type == kPointLight) {
+            PointRecord r;
+            std::memcpy(&r, p, sizeof(r));
+            return {r.position[0], r.position[1], r.position[2]};
+        }
+        TriRecord r;
+        std::memcpy(&r, p, sizeof(r));
+        const V3 v0{r.posV[0][0], r.posV[0][1], r.posV[0][2]}, v1{r.posV[1][0], r.posV[1][1], r.posV[1][2]},
+            v2{r.posV[2][0], r.posV[2][1], r.posV[2][2]};
+        return add(v0, divs(add(v1, v2), 3.0f));
+    }
+    void make_node(size_t index, u32 lOff, u16 lType, u32 rOff, u16 rType) {
+        NodeRecord n;
+        n.leftFlux = flux_of(lOff, lType);
+        n.leftOffset = lOff;
+        n.leftType = lType;
+        n.rightType = rType;
+        n.rightOffset = rOff;
+        n.rightFlux = flux_of(rOff, rType);
+        const V3 c = divs(add(center_of(lOff, lType), center_of(rOff, rType)), 2.0f);
+        n.center[0] = c.x;
+        n.center[1] = c.y;
+        n.center[2] = c.z;
+        std::memcpy(mem.data() + index * kNodeBytes, &n, sizeof(n));
+    }
+};
+
+size_t num_internal_nodes(size_t n) {
+    if (n <= 1) return 0;
+    const size_t height = (size_t)std::log2((double)n);
+    return ((size_t(1) << height) - 1) + (n - (size_t(1) << height));
+}
+
+}  // namespace
+
+void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vector<HostPointLight>& points,
+                      const float sceneBox[6], const EmissionLookup& emission, LightTreeHost& out) {
+    out = LightTreeHost();
+    std::vector<PosLight> lights;
+    for (const auto& t : tris) lights.push_back(PosLight{true, t, {}});
+    for (const auto& p : points) lights.push_back(PosLight{false, {}, p});
+    const size_t n = lights.size();
+    out.posLightCount = (u32)n;
+    if (n == 0) return;
+
+    const V3 scale{sceneBox[3] - sceneBox[0], sceneBox[4] - sceneBox[1], sceneBox[5] - sceneBox[2]};
+    std::sort(lights.begin(), lights.end(), [&scale](const PosLight& a, const PosLight& b) {
+        const V3 ca = sort_center(a), cb = sort_center(b);
+        const u32 x[3] = {f2u(ca.x * scale.x), f2u(ca.y * scale.y), f2u(ca.z * scale.z)};
+        const u32 y[3] = {f2u(cb.x * scale.x), f2u(cb.y * scale.y), f2u(cb.z * scale.z)};
+        u64 codeA = hilbert_code((u16)(x[0] >> 16u), (u16)(x[1] >> 16u), (u16)(x[2] >> 16u));
+        u64 codeB = hilbert_code((u16)(y[0] >> 16u), (u16)(y[1] >> 16u), (u16)(y[2] >> 16u));
+        if (codeA == codeB) {
+            codeA = hilbert_code((u16)(x[0] & 0xffff), (u16)(x[1] & 0xffff), (u16)(x[2] & 0xffff));
+            codeB = hilbert_code((u16)(y[0] & 0xffff), (u16)(y[1] & 0xffff), (u16)(y[2] & 0xffff));
+        }
+        return codeA < codeB;
+    });
+
+    const size_t nInt = num_internal_nodes(n);
+    out.posInternalCount = (u32)nInt;
+    std::vector<u32> offsets(n);
+    std::vector<u16> types(n);
+    size_t memSize = nInt * kNodeBytes;
+    for (size_t i = 0; i < n; ++i) {
+        offsets[i] = (u32)memSize;
+        types[i] = lights[i].isTri ? kAreaLightTriangle : kPointLight;
+        memSize += lights[i].isTri ? kTriBytes : kPointBytes;
+    }
+    out.mem.assign(memSize, 0);
+    for (size_t i = 0; i < n; ++i) {
+        uint8_t* dst = out.mem.data() + offsets[i];
+        if (lights[i].isTri) {
+            const HostAreaTriangle& d = lights[i].tri;
+            TriRecord r;
+            std::memset(&r, 0, sizeof(r));
+            for (int k = 0; k < 3; ++k) {
+                r.posV[0][k] = d.points[0][k];
+                r.posV[1][k] = d.points[1][k] - d.points[0][k];
+                r.posV[2][k] = d.points[2][k] - d.points[0][k];
+            }
+            for (int k = 0; k < 2; ++k) {
+                r.uvV[0][k] = d.uv[0][k];
+                r.uvV[1][k] = d.uv[1][k] - d.uv[0][k];
+                r.uvV[2][k] = d.uv[2][k] - d.uv[0][k];
+            }
+            r.material = d.material;
+            std::memcpy(dst, &r, sizeof(r));
+        } else {
+            const HostPointLight& d = lights[i].point;
+            PointRecord r;
+            std::memset(&r, 0, sizeof(r));
+            std::memcpy(r.position, d.position, 12);
+            std::memcpy(r.intensity, d.intensity, 12);
+            std::memcpy(dst, &r, sizeof(r));
+        }
+    }
+
+    Builder b{out.mem, emission};
+    if (n == 1) {
+        out.posRootType = types[0];
+        out.posRootFlux = b.flux_of(offsets[0], types[0]);
+    } else {
+        const size_t height = (size_t)std::log2((double)n);
+        const size_t extraNodes = n - (size_t(1) << height);
+        if (extraNodes > 0) {
+            const size_t startNode = (size_t(1) << height) - 1;
+            const size_t startLight = n - 2 * extraNodes;
+            for (size_t i = 0; i < extraNodes; ++i) {
+                const size_t l = startLight + 2 * i, r = l + 1;
+                b.make_node(startNode + i, offsets[l], types[l], offsets[r], types[r]);
+            }
+            const size_t startInnerNode = (size_t(1) << (height - 1)) - 1;
+            for (size_t i = 0; i < extraNodes / 2; ++i) {
+                const size_t l = startNode + 2 * i, r = l + 1;
+                b.make_node(startInnerNode + i, (u32)(l * kNodeBytes), kInternalNode, (u32)(r * kNodeBytes),
+                            kInternalNode);
+            }
+            if (extraNodes % 2 != 0) {
+                const size_t l = startNode + extraNodes - 1;
+                b.make_node(startInnerNode + extraNodes / 2, (u32)(l * kNodeBytes), kInternalNode, offsets[0],
+                            types[0]);
+            }
+        }
+        const size_t startLight = (extraNodes % 2 == 0) ? 0 : 1;
+        const size_t levelNodeCount = n - 2 * extraNodes - startLight;
+        const size_t nodeCount = levelNodeCount / 2;
+        const size_t startNodeIndex = (size_t(1) << height) - nodeCount - 1;
+        for (size_t i = 0; i < nodeCount; ++i) {
+            const size_t l = startLight + 2 * i, r = l + 1;
+            b.make_node(startNodeIndex + i, offsets[l], types[l], offsets[r], types[r]);
+        }
+        if (height > 1) {
+            for (size_t level = height - 1; level >= 1; --level) {
+                const size_t nodes = size_t(1) << level;
+                const size_t innerNode = nodes / 2 - 1;
+                for (size_t i = 0; i < nodes / 2; ++i) {
+                    const size_t l = nodes - 1 + 2 * i, r = l + 1;
+                    b.make_node(innerNode + i, (u32)(l * kNodeBytes), kInternalNode, (u32)(r * kNodeBytes),
+                                kInternalNode);
+                }
+            }
+        }
+        out.posRootType = kInternalNode;
+        NodeRecord root;
+        std::memcpy(&root, out.mem.data(), sizeof(root));
+        out.posRootFlux = root.leftFlux + root.rightFlux;
+    }
+
+    // Primitive -> tree path (fill_map, light_tree.cpp:268-302); u32 wrap-around as in the reference.
+    std::vector<std::pair<u64, u32>> map;
+    {
+        int height = 0;
+        while ((size_t(1) << (height + 1)) <= n) ++height;  // ilog2
+        const u32 extraNodes = (u32)n - (1u << height);
+        const u32 lowerLightCount = (1u << height) - extraNodes;
+        u32 i = 0;
+        auto key = [](const HostAreaTriangle& t) { return ((u64)(u32)t.instance << 32) | (u64)(u32)t.primitive; };
+        if (extraNodes > 0) {
+            for (; i < lowerLightCount; ++i) {
+                if (!lights[i].isTri) continue;
+                u32 code = ((1u << height) - 1) - (lowerLightCount - i - 1);
+                code <<= (32 - height) & 31;
+                map.push_back({key(lights[i].tri), code});
+            }
+            height += 1;
+        }
+        for (; i < (u32)n; ++i) {
+            if (!lights[i].isTri) continue;
+            const u32 code = (u32)(i - lowerLightCount) << ((32 - height) & 31);
+            map.push_back({key(lights[i].tri), code});
+        }
+    }
+    std::sort(map.begin(), map.end());
+    for (const auto& kv : map) {
+        out.mapKeys.push_back(kv.first);
+        out.mapCodes.push_back(kv.second);
+    }
+}
+
+}  // namespace mfh

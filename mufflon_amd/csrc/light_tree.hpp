@@ -1,0 +1,59 @@
+// Host-built light tree of the path tracer, in the byte layout of Mufflon's
+// LightSubTree (light_tree.hpp:56-110): internal 32-byte nodes first, then the
+// light records (AreaLightTriangle 64 B, PointLight 32 B, lights.hpp:46-124).
+// As in Mufflon, the tree is built once on the host per scene and copied to
+// the device verbatim; every device path reads the same bytes.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace mfh {
+
+enum LightType : uint16_t {
+    kPointLight = 0,
+    kSpotLight = 1,
+    kAreaLightTriangle = 2,
+    kAreaLightQuad = 3,
+    kAreaLightSphere = 4,
+    kDirectionalLight = 5,
+    kEnvmapLight = 6,
+    kInternalNode = 0xFFFF,
+};
+
+struct HostAreaTriangle {
+    float points[3][3];  // world space
+    float uv[3][2];
+    uint16_t material;
+    int32_t instance, primitive;
+};
+struct HostPointLight {
+    float position[3];
+    float intensity[3];
+};
+
+// Emission of a material at any uv (the benchmark materials carry constant
+// emission: a 1x1 texture times a scale).
+struct EmissionLookup {
+    virtual ~EmissionLookup() = default;
+    virtual void radiance(uint16_t material, float u, float v, float out[3]) const = 0;
+};
+
+struct LightTreeHost {
+    // positional subtree
+    std::vector<uint8_t> mem;
+    float posRootFlux = 0.0f;
+    uint16_t posRootType = kInternalNode;
+    uint32_t posInternalCount = 0;
+    uint32_t posLightCount = 0;
+    // directional subtree and background (absent in the supported scenes)
+    float dirRootFlux = 0.0f;
+    float backgroundFlux = 0.0f;  // sum of the background flux channels
+    // primitive (instance << 32 | primitive) -> left-aligned tree path, sorted by key
+    std::vector<uint64_t> mapKeys;
+    std::vector<uint32_t> mapCodes;
+};
+
+void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vector<HostPointLight>& points,
+                      const float sceneBox[6], const EmissionLookup& emission, LightTreeHost& out);
+
+}  // namespace mfh

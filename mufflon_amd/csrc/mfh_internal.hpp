@@ -90,6 +90,13 @@ struct Lod {
     int32_t numVerts = 0, numTris = 0;
     DevBuf verts;     // float xyz per vertex
     DevBuf tris;      // u32 x3 per triangle
+    // host copies and shading attributes (normals, uvs, material per triangle)
+    std::vector<float> hostVerts;
+    std::vector<uint32_t> hostTris;
+    std::vector<float> hostUvs;
+    std::vector<uint16_t> hostMatIdx;
+    bool hasShading = false;
+    DevBuf normals, uvs, matIdx;
     float box[6] = {0, 0, 0, 0, 0, 0};
     bool built = false;
     uint64_t generation = 0;  // incremented by every build
@@ -110,12 +117,15 @@ struct Scene {
     DevBuf tlasNodes, tlasPrimIds;
 };
 
+struct PathTracer;
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t evBegin = nullptr, evEnd = nullptr;
     std::vector<Lod*> lods;
     std::vector<Scene*> scenes;
+    std::vector<PathTracer*> tracers;
     BuildScratch scratch;
     int lastBuilt = -1;
 };
@@ -129,6 +139,9 @@ struct DevLod {
     float box[6];
     int32_t pad[2];
 };
+
+void destroy_tracers(Context& ctx);
+void upload(Context& c, DevBuf& b, const void* src, size_t bytes);
 
 // LBVH over a triangle LoD; fills lod.nodes / primIds / triVerts / numInternal.
 void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink);

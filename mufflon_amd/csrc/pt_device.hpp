@@ -1,0 +1,73 @@
+// Device-side description of a path-tracer scene (the role of Mufflon's
+// SceneDescriptor + PtParameters + render buffers for the pt renderer).
+#pragma once
+#include "mfh_device.hpp"
+#include "mfh_internal.hpp"
+
+namespace mfh {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kMaxSceneSize = 1048576.0f;  // types.hpp:26
+constexpr float kSceneEps = 1e-4f;          // intersection.cu:16
+constexpr float kInfPdf = 4294967296.f;      // AreaPdf::infinite() (util/types.hpp:52-54)
+constexpr float kDistSqEps = 1e-10f;        // light_sampling.hpp:78
+
+constexpr int kMatEmissive = 0;  // Materials enum (material_definitions.hpp:25-27)
+constexpr int kMatLambert = 1;
+
+constexpr uint16_t kLightPoint = 0;
+constexpr uint16_t kLightAreaTriangle = 2;
+constexpr uint16_t kLightInternalNode = 0xFFFF;
+
+// mfh_material of include/mufflon_hip.h (64 bytes)
+struct PtMaterialRecord {
+    int32_t type;
+    float albedo[3];
+    float radiance[3];
+    float scale;
+    int32_t reserved[8];
+};
+static_assert(sizeof(PtMaterialRecord) == 64, "material record");
+
+// Object-space shading attributes of one LoD.
+struct DevPtLod {
+    const float* verts;     // xyz per vertex
+    const uint32_t* tris;   // 3 per triangle
+    const float* normals;   // xyz per vertex
+    const float* uvs;       // uv per vertex
+    const uint16_t* matIdx; // per triangle
+};
+
+struct PtScene {
+    TraceParams trace;
+    const DevPtLod* ptLods;
+    const PtMaterialRecord* materials;
+    // light tree (Mufflon layout, positional subtree)
+    const uint8_t* lightMem;
+    float posRootFlux;
+    uint32_t posRootType;
+    float dirRootFlux;
+    float bgFlux;
+    const uint64_t* mapKeys;
+    const uint32_t* mapCodes;
+    int32_t mapCount;
+    // pinhole camera (PinholeParams, pinhole.hpp:43-51)
+    float camPos[3];
+    float tanVFov;
+    float camDir[3];
+    float camUp[3];
+    int32_t width, height;
+    // PtParameters (pt_params.hpp:8-14)
+    int32_t seed, minPathLength, maxPathLength, neeCount;
+    // per-pixel state and targets
+    uint64_t* rngs;
+    float* cumRadiance;  // 3 per pixel, running mean
+    float* cumHitId;     // 2 per pixel, running mean
+    uint32_t* nanCounter;
+    int32_t iteration;   // index of the iteration being run (0 after a reset)
+};
+
+void launch_pt_init_rng(uint64_t* rngs, int num, int seed, hipStream_t stream);
+void launch_pt_iterate(const PtScene& S, hipStream_t stream);
+
+}  // namespace mfh

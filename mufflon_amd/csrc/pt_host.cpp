@@ -1,0 +1,382 @@
+// Host side of the path tracer: scene-descriptor assembly (area lights from
+// emissive triangles, light tree, materials, camera), per-pixel state, and
+// the extern "C" entry points of include/mufflon_hip.h for the pt renderer.
+#include <algorithm>
+#include <cstring>
+
+#include "capi_util.hpp"
+#include "light_tree.hpp"
+#include "pt_device.hpp"
+
+namespace mfh {
+
+struct PathTracer {
+    int32_t scene = -1;
+    std::vector<PtMaterialRecord> materials;
+    mfh_pinhole camera{};
+    mfh_pt_params params{};
+    LightTreeHost tree;
+    DevBuf dMaterials, dPtLods, dLightMem, dMapKeys, dMapCodes;
+    DevBuf dRngs, dRadiance, dHitId, dNan;
+    int32_t iteration = -1;
+    uint64_t sceneGeneration = ~0ull;
+};
+
+void destroy_tracers(Context& c) {
+    for (PathTracer* t : c.tracers) {
+        if (!t) continue;
+        for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dRngs,
+                          &t->dRadiance, &t->dHitId, &t->dNan})
+            b->release();
+        delete t;
+    }
+    c.tracers.clear();
+}
+
+namespace {
+
+// descriptors.hpp:108-129 (host copy of the device inversion)
+void instance_to_world(const float* w, float* o) {
+    const float r00 = w[0], r01 = w[1], r02 = w[2];
+    const float r10 = w[4], r11 = w[5], r12 = w[6];
+    const float r20 = w[8], r21 = w[9], r22 = w[10];
+    const float m00 = r11 * r22 - r21 * r12;
+    const float m01 = r21 * r02 - r01 * r22;
+    const float m02 = r01 * r12 - r11 * r02;
+    const float m10 = r20 * r12 - r10 * r22;
+    const float m11 = r00 * r22 - r20 * r02;
+    const float m12 = r10 * r02 - r00 * r12;
+    const float m20 = r10 * r21 - r20 * r11;
+    const float m21 = r20 * r01 - r00 * r21;
+    const float m22 = r00 * r11 - r10 * r01;
+    const float det = r00 * m00 + r01 * m10 + r02 * m20;
+    const float s = 1.0f / det;
+    const float a[9] = {m00, m01, m02, m10, m11, m12, m20, m21, m22};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) o[4 * r + c] = s * a[3 * r + c];
+    const float t[3] = {w[3], w[7], w[11]};
+    for (int r = 0; r < 3; ++r) o[4 * r + 3] = -(o[4 * r] * t[0] + o[4 * r + 1] * t[1] + o[4 * r + 2] * t[2]);
+}
+
+struct MaterialEmission : EmissionLookup {
+    const std::vector<PtMaterialRecord>& mats;
+    explicit MaterialEmission(const std::vector<PtMaterialRecord>& m) : mats(m) {}
+    void radiance(uint16_t material, float, float, float out[3]) const override {
+        const PtMaterialRecord& m = mats[material];
+        for (int k = 0; k < 3; ++k) out[k] = m.type == kMatEmissive ? m.radiance[k] * m.scale : 0.0f;
+    }
+};
+
+PathTracer& tracer_of(Context& c, int32_t id) {
+    MFH_REQUIRE(id >= 0 && id < (int32_t)c.tracers.size() && c.tracers[id], "unknown path tracer id");
+    return *c.tracers[id];
+}
+
+// Area lights of every emissive triangle, per LoD and instance, in world space
+// (world_container.cpp:895-937).
+std::vector<HostAreaTriangle> gather_area_lights(const Context& c, const Scene& sc,
+                                                 const std::vector<PtMaterialRecord>& mats) {
+    std::vector<HostAreaTriangle> out;
+    for (size_t lodId = 0; lodId < c.lods.size(); ++lodId) {
+        const Lod* l = c.lods[lodId];
+        if (!l || !l->hasShading) continue;
+        bool emissive = false;
+        for (uint16_t m : l->hostMatIdx) emissive |= mats[m].type == kMatEmissive;
+        if (!emissive) continue;
+        for (int32_t inst = 0; inst < sc.numInstances; ++inst) {
+            if (sc.instLod[inst] != (int32_t)lodId) continue;
+            float M[12];
+            instance_to_world(sc.worldToInst.data() + 12 * (size_t)inst, M);
+            const float det = M[0] * (M[5] * M[10] - M[9] * M[6]) + M[1] * (M[8] * M[6] - M[4] * M[10]) +
+                              M[2] * (M[4] * M[9] - M[8] * M[5]);
+            const bool mirroring = det < 0.0f;
+            for (int32_t p = 0; p < l->numTris; ++p) {
+                const uint16_t mi = l->hostMatIdx[p];
+                if (mats[mi].type != kMatEmissive) continue;
+                HostAreaTriangle t;
+                t.material = mi;
+                t.instance = inst;
+                t.primitive = p;
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t vi = l->hostTris[3 * (size_t)p + k];
+                    const float* v = l->hostVerts.data() + 3 * (size_t)vi;
+                    for (int r = 0; r < 3; ++r)
+                        t.points[k][r] = M[4 * r] * v[0] + M[4 * r + 1] * v[1] + M[4 * r + 2] * v[2] + M[4 * r + 3];
+                    t.uv[k][0] = l->hostUvs[2 * (size_t)vi];
+                    t.uv[k][1] = l->hostUvs[2 * (size_t)vi + 1];
+                }
+                if (mirroring) {
+                    std::swap(t.points[1], t.points[2]);
+                    std::swap(t.uv[1], t.uv[2]);
+                }
+                out.push_back(t);
+            }
+        }
+    }
+    return out;
+}
+
+PtScene device_scene(Context& c, PathTracer& t) {
+    Scene& sc = scene_of(c, t.scene);
+    refresh_scene(c, sc);
+    // per-LoD shading table (pointers may move when a LoD is rebuilt)
+    std::vector<DevPtLod> lods(c.lods.size());
+    for (size_t i = 0; i < c.lods.size(); ++i) {
+        std::memset(&lods[i], 0, sizeof(DevPtLod));
+        const Lod* l = c.lods[i];
+        if (!l) continue;
+        lods[i].verts = l->verts.as<float>();
+        lods[i].tris = l->tris.as<uint32_t>();
+        lods[i].normals = l->normals.as<float>();
+        lods[i].uvs = l->uvs.as<float>();
+        lods[i].matIdx = l->matIdx.as<uint16_t>();
+    }
+    upload(c, t.dPtLods, lods.data(), sizeof(DevPtLod) * lods.size());
+    PtScene S;
+    std::memset(&S, 0, sizeof(S));
+    S.trace = trace_params(sc);
+    S.ptLods = t.dPtLods.as<DevPtLod>();
+    S.materials = t.dMaterials.as<PtMaterialRecord>();
+    S.lightMem = t.dLightMem.as<uint8_t>();
+    S.posRootFlux = t.tree.posRootFlux;
+    S.posRootType = t.tree.posRootType;
+    S.dirRootFlux = t.tree.dirRootFlux;
+    S.bgFlux = t.tree.backgroundFlux;
+    S.mapKeys = t.dMapKeys.as<uint64_t>();
+    S.mapCodes = t.dMapCodes.as<uint32_t>();
+    S.mapCount = (int32_t)t.tree.mapKeys.size();
+    std::memcpy(S.camPos, t.camera.position, 12);
+    S.tanVFov = t.camera.tanVFov;
+    std::memcpy(S.camDir, t.camera.viewDir, 12);
+    std::memcpy(S.camUp, t.camera.up, 12);
+    S.width = t.camera.width;
+    S.height = t.camera.height;
+    S.seed = t.params.seed;
+    S.minPathLength = t.params.minPathLength;
+    S.maxPathLength = t.params.maxPathLength;
+    S.neeCount = t.params.neeCount;
+    S.rngs = t.dRngs.as<uint64_t>();
+    S.cumRadiance = t.dRadiance.as<float>();
+    S.cumHitId = t.dHitId.as<float>();
+    S.nanCounter = t.dNan.as<uint32_t>();
+    return S;
+}
+
+void reset(Context& c, PathTracer& t) {
+    const int num = t.camera.width * t.camera.height;
+    launch_pt_init_rng(t.dRngs.as<uint64_t>(), num, t.params.seed * (num + 1), c.stream);
+    MFH_HIP(hipMemsetAsync(t.dRadiance.ptr, 0, 12 * (size_t)num, c.stream));
+    MFH_HIP(hipMemsetAsync(t.dHitId.ptr, 0, 8 * (size_t)num, c.stream));
+    MFH_HIP(hipMemsetAsync(t.dNan.ptr, 0, 4, c.stream));
+    t.iteration = -1;
+}
+
+}  // namespace
+}  // namespace mfh
+
+using namespace mfh;
+
+extern "C" {
+
+int mfh_lod_set_shading(mfh_context* ctx, int32_t lodId, const float* normals, const float* uvs,
+                        const uint16_t* materialIndices) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Lod& l = lod_of(c, lodId);
+        MFH_REQUIRE(normals && uvs && materialIndices, "null pointer");
+        l.hostUvs.assign(uvs, uvs + 2 * (size_t)l.numVerts);
+        l.hostMatIdx.assign(materialIndices, materialIndices + (size_t)l.numTris);
+        upload(c, l.normals, normals, 12 * (size_t)l.numVerts);
+        upload(c, l.uvs, uvs, 8 * (size_t)l.numVerts);
+        upload(c, l.matIdx, materialIndices, 2 * (size_t)l.numTris);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        l.hasShading = true;
+    });
+}
+
+int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materials, int32_t numMaterials,
+                  const mfh_point_light* pointLights, int32_t numPointLights, const mfh_pinhole* camera,
+                  const mfh_pt_params* params, int32_t* outPtId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        Scene& sc = scene_of(c, sceneId);
+        MFH_REQUIRE(materials && numMaterials > 0 && camera && params && outPtId, "null pointer");
+        MFH_REQUIRE(numPointLights == 0 || pointLights, "null point lights");
+        MFH_REQUIRE(camera->width > 0 && camera->height > 0 && camera->width < 65536 && camera->height < 65536,
+                    "camera resolution out of range");
+        MFH_REQUIRE(params->maxPathLength >= 0 && params->neeCount >= 0, "invalid path tracer parameters");
+        for (int32_t i = 0; i < numMaterials; ++i)
+            MFH_REQUIRE(materials[i].type == MFH_MATERIAL_EMISSIVE || materials[i].type == MFH_MATERIAL_LAMBERT,
+                        "unsupported material type " + std::to_string(materials[i].type));
+        for (int32_t li : sc.instLod) {
+            const Lod* l = c.lods[li];
+            MFH_REQUIRE(l->hasShading, "a LoD of the scene has no shading attributes (mfh_lod_set_shading)");
+            for (uint16_t m : l->hostMatIdx) MFH_REQUIRE(m < numMaterials, "material index out of range");
+        }
+        refresh_scene(c, sc);
+        auto* t = new PathTracer();
+        try {
+            t->scene = sceneId;
+            t->materials.resize(numMaterials);
+            std::memcpy(t->materials.data(), materials, sizeof(mfh_material) * (size_t)numMaterials);
+            t->camera = *camera;
+            t->params = *params;
+            std::vector<HostPointLight> pl(numPointLights);
+            for (int32_t i = 0; i < numPointLights; ++i) {
+                std::memcpy(pl[i].position, pointLights[i].position, 12);
+                std::memcpy(pl[i].intensity, pointLights[i].intensity, 12);
+            }
+            const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t->materials);
+            MFH_REQUIRE(!area.empty() || !pl.empty(), "scene without lights");
+            build_light_tree(area, pl, sc.box, MaterialEmission(t->materials), t->tree);
+            upload(c, t->dMaterials, t->materials.data(), sizeof(PtMaterialRecord) * t->materials.size());
+            upload(c, t->dLightMem, t->tree.mem.data(), std::max<size_t>(t->tree.mem.size(), 16));
+            const size_t nm = t->tree.mapKeys.size();
+            t->dMapKeys.reserve(8 * std::max<size_t>(nm, 1));
+            t->dMapCodes.reserve(4 * std::max<size_t>(nm, 1));
+            if (nm) {
+                upload(c, t->dMapKeys, t->tree.mapKeys.data(), 8 * nm);
+                upload(c, t->dMapCodes, t->tree.mapCodes.data(), 4 * nm);
+            }
+            const size_t num = (size_t)camera->width * camera->height;
+            t->dRngs.reserve(8 * num);
+            t->dRadiance.reserve(12 * num);
+            t->dHitId.reserve(8 * num);
+            t->dNan.reserve(4);
+            reset(c, *t);
+            MFH_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dRngs,
+                              &t->dRadiance, &t->dHitId, &t->dNan})
+                b->release();
+            delete t;
+            throw;
+        }
+        c.tracers.push_back(t);
+        *outPtId = (int32_t)c.tracers.size() - 1;
+    });
+}
+
+int mfh_pt_reset(mfh_context* ctx, int32_t ptId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        reset(c, tracer_of(c, ptId));
+    });
+}
+
+int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(iterations >= 0, "negative iteration count");
+        PtScene S = device_scene(c, t);
+        for (int32_t k = 0; k < iterations; ++k) {
+            S.iteration = ++t.iteration;
+            launch_pt_iterate(S, c.stream);
+        }
+    });
+}
+
+int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output pointer");
+        const size_t num = (size_t)t.camera.width * t.camera.height;
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        if (target == MFH_TARGET_RADIANCE)
+            MFH_HIP(hipMemcpy(out, t.dRadiance.ptr, 12 * num, hipMemcpyDeviceToHost));
+        else if (target == MFH_TARGET_HIT_ID)
+            MFH_HIP(hipMemcpy(out, t.dHitId.ptr, 8 * num, hipMemcpyDeviceToHost));
+        else
+            throw Error(kInvalidArgument, "unknown render target " + std::to_string(target));
+    });
+}
+
+int mfh_pt_get_iteration(mfh_context* ctx, int32_t ptId, int32_t* outIteration) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(outIteration != nullptr, "null output pointer");
+        *outIteration = tracer_of(c, ptId).iteration;
+    });
+}
+
+int mfh_pt_get_nan_count(mfh_context* ctx, int32_t ptId, uint32_t* outCount) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(outCount != nullptr, "null output pointer");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipMemcpy(outCount, t.dNan.ptr, 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int mfh_pt_get_light_tree(mfh_context* ctx, int32_t ptId, mfh_light_tree_info* outInfo, void* outMem,
+                          uint64_t* outMapKeys, uint32_t* outMapCodes) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(outInfo != nullptr, "null output pointer");
+        outInfo->posRootFlux = t.tree.posRootFlux;
+        outInfo->posRootType = t.tree.posRootType;
+        outInfo->posInternalCount = t.tree.posInternalCount;
+        outInfo->posLightCount = t.tree.posLightCount;
+        outInfo->memBytes = (uint32_t)t.tree.mem.size();
+        outInfo->mapCount = (int32_t)t.tree.mapKeys.size();
+        outInfo->dirRootFlux = t.tree.dirRootFlux;
+        outInfo->backgroundFlux = t.tree.backgroundFlux;
+        if (outMem && !t.tree.mem.empty()) std::memcpy(outMem, t.tree.mem.data(), t.tree.mem.size());
+        if (outMapKeys) std::memcpy(outMapKeys, t.tree.mapKeys.data(), 8 * t.tree.mapKeys.size());
+        if (outMapCodes) std::memcpy(outMapCodes, t.tree.mapCodes.data(), 4 * t.tree.mapCodes.size());
+    });
+}
+
+int mfh_light_tree_build(const mfh_area_light* areaLights, int32_t numAreaLights, const mfh_point_light* pointLights,
+                         int32_t numPointLights, const float* sceneBox, const mfh_material* materials,
+                         int32_t numMaterials, mfh_light_tree_info* outInfo, void* outMem, size_t memCapacity,
+                         uint64_t* outMapKeys, uint32_t* outMapCodes, int32_t mapCapacity) {
+    return guarded([&] {
+        MFH_REQUIRE(sceneBox && materials && outInfo, "null pointer");
+        MFH_REQUIRE(numAreaLights >= 0 && numPointLights >= 0, "negative light count");
+        MFH_REQUIRE(numAreaLights == 0 || areaLights, "null area lights");
+        MFH_REQUIRE(numPointLights == 0 || pointLights, "null point lights");
+        std::vector<PtMaterialRecord> mats(numMaterials);
+        std::memcpy(mats.data(), materials, sizeof(mfh_material) * (size_t)numMaterials);
+        std::vector<HostAreaTriangle> area(numAreaLights);
+        for (int32_t i = 0; i < numAreaLights; ++i) {
+            const mfh_area_light& a = areaLights[i];
+            MFH_REQUIRE(a.material >= 0 && a.material < numMaterials, "material index out of range");
+            for (int k = 0; k < 3; ++k) {
+                for (int r = 0; r < 3; ++r) area[i].points[k][r] = a.points[3 * k + r];
+                area[i].uv[k][0] = a.uv[2 * k];
+                area[i].uv[k][1] = a.uv[2 * k + 1];
+            }
+            area[i].material = (uint16_t)a.material;
+            area[i].instance = a.instance;
+            area[i].primitive = a.primitive;
+        }
+        std::vector<HostPointLight> pl(numPointLights);
+        for (int32_t i = 0; i < numPointLights; ++i) {
+            std::memcpy(pl[i].position, pointLights[i].position, 12);
+            std::memcpy(pl[i].intensity, pointLights[i].intensity, 12);
+        }
+        LightTreeHost tree;
+        build_light_tree(area, pl, sceneBox, MaterialEmission(mats), tree);
+        MFH_REQUIRE(tree.mem.size() <= memCapacity, "light tree memory capacity too small");
+        MFH_REQUIRE((int32_t)tree.mapKeys.size() <= mapCapacity, "light tree map capacity too small");
+        outInfo->posRootFlux = tree.posRootFlux;
+        outInfo->posRootType = tree.posRootType;
+        outInfo->posInternalCount = tree.posInternalCount;
+        outInfo->posLightCount = tree.posLightCount;
+        outInfo->memBytes = (uint32_t)tree.mem.size();
+        outInfo->mapCount = (int32_t)tree.mapKeys.size();
+        outInfo->dirRootFlux = tree.dirRootFlux;
+        outInfo->backgroundFlux = tree.backgroundFlux;
+        if (outMem && !tree.mem.empty()) std::memcpy(outMem, tree.mem.data(), tree.mem.size());
+        if (outMapKeys && !tree.mapKeys.empty()) std::memcpy(outMapKeys, tree.mapKeys.data(), 8 * tree.mapKeys.size());
+        if (outMapCodes && !tree.mapCodes.empty())
+            std::memcpy(outMapCodes, tree.mapCodes.data(), 4 * tree.mapCodes.size());
+    });
+}
+
+}  // extern "C"

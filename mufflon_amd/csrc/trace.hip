@@ -1,76 +1,12 @@
-// Closest-hit and any-hit BVH traversal for gfx950: the GPU device of
-// Mufflon's first_intersection / any_intersection (intersection.cu:366-753).
-//
-// One ray per lane. The traversal stack keeps its lowest kLdsDepth entries in
-// LDS (one column per lane, so the 64 lanes of a wave touch 64 consecutive
-// dwords) and spills deeper entries to a per-lane private array. Nodes are
-// Mufflon's 64-byte collapsed LBVH nodes, read as four 16-byte loads; the
-// triangles of a leaf are contiguous 48-byte records in BVH order
-// ({v0, primId}, {v1}, {v2}), so a leaf test needs no index indirection.
-//
-// Visiting order and box test are those of the reference CPU path: the slab
-// test of intersection.cu:57-75 (computed without its early outs, which give
-// the same accept/reject and entry distance), nearer child first with ties
-// to the left, the farther child pushed, leaves pushed as (count, index), and
-// the two-level instance transition of intersection.cu:321-361. Closest hit is
-// the lexicographic minimum of (t, instance, primitive).
-#include "mfh_device.hpp"
-#include "mfh_internal.hpp"
+// Closest-hit and any-hit trace kernels over ray buffers (one ray per lane),
+// built on the traversal of traverse.hpp.
+#include "traverse.hpp"
 
 namespace mfh {
 
 namespace {
 
-constexpr int kSentinel = (int)0xFFFFFFFF;
-constexpr int kStackCap = 128;
-constexpr int kThreads = 256;
-constexpr int kLdsDepth = 16;
-
-// Slab test, reference CPU semantics (intersection.cu:57-75).
-__device__ __forceinline__ bool slab(const float* lo, const float* hi, const FastRay& r, float tmax, float& cmin) {
-    const float x0 = lo[0] * r.inv.x - r.odiv.x, x1 = hi[0] * r.inv.x - r.odiv.x;
-    const float y0 = lo[1] * r.inv.y - r.odiv.y, y1 = hi[1] * r.inv.y - r.odiv.y;
-    const float z0 = lo[2] * r.inv.z - r.odiv.z, z1 = hi[2] * r.inv.z - r.odiv.z;
-    const float mnx = mn(x0, x1), mxx = mx(x0, x1);
-    const float mny = mn(y0, y1), mxy = mx(y0, y1);
-    const float mnz = mn(z0, z1), mxz = mx(z0, z1);
-    const float cmx = mn(mn(mxx, mxy), mxz);
-    cmin = mx(mx(mnx, mny), mnz);
-    return (cmx >= 0.0f) && (cmin <= tmax) && (cmin <= cmx);
-}
-
-// Accept a hit closer than the current one; at equal distance the lower
-// (instance, primitive) pair wins. A hit exactly at the initial tmax (no hit
-// yet) is rejected, as the reference's strict t < tmax test does.
-__device__ __forceinline__ bool better(float t, int inst, int prim, float hitT, int hInst, int hPrim) {
-    if (t < hitT) return true;
-    if (t > hitT || hInst == -1) return false;
-    if (inst != hInst) return inst < hInst;
-    return prim < hPrim;
-}
-
-// The instance transition of world_to_object_space (intersection.cu:321-349):
-// transform the world ray into the instance, scale tmax, test the object box.
-struct ObjectEntry {
-    FastRay ray;
-    float scale, tmax;
-    bool hit;
-};
-__device__ __forceinline__ ObjectEntry enter_object(const TraceParams& p, int inst, V3 wo, V3 wd, float hitT,
-                                                    DevLod& lod) {
-    ObjectEntry e;
-    const float4* w = reinterpret_cast<const float4*>(p.worldToInst + 12 * (size_t)inst);
-    const float4 a = w[0], b = w[1], c = w[2];
-    const M34 M{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w}};
-    const V3 d = xform_dir(M, wd);
-    e.scale = len(d);
-    e.ray = make_fast_ray(xform_point(M, wo), d / e.scale);
-    e.tmax = hitT * e.scale;
-    lod = p.lods[p.instLod[inst]];
-    float ot;
-    e.hit = slab(lod.box, lod.box + 3, e.ray, e.tmax, ot);
-    return e;
-}
+constexpr int kThreads = kTraceThreads;
 
 // kAny: stop at the first accepted hit. kStats: count per ray the internal
 // nodes visited (an object-box test at instance entry counts as one) and the
@@ -82,184 +18,16 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
     __shared__ int sStack[kLdsDepth * kThreads];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int* const lstack = sStack + threadIdx.x;
-    int spill[kStackCap - kLdsDepth];
-    int sp = 0;
-    bool overflow = false;
-#define MFH_PUSH(v)                                   \
-    do {                                              \
-        if (sp < kLdsDepth)                           \
-            lstack[sp * kThreads] = (v);              \
-        else if (sp < kStackCap)                      \
-            spill[sp - kLdsDepth] = (v);              \
-        if (sp < kStackCap) ++sp; else overflow = true; \
-    } while (0)
-#define MFH_POP() (--sp, (sp < kLdsDepth) ? lstack[sp * kThreads] : spill[sp - kLdsDepth])
-
     const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
-    const V3 wo = v3(r0.x, r0.y, r0.z), wd = v3(r1.x, r1.y, r1.z);
-    FastRay cur = make_fast_ray(wo, wd);
-    float hitT = r0.w;
-    int hInst = -1, hPrim = -1;
-    float hw0 = 0.0f, hw1 = 0.0f;
-    float scale = 1.0f;
-    unsigned nodeVisits = 0, primTests = 0;
-
-    // current level: the instance BVH or the BVH of the entered object
-    const BvhChild* bvh = (const BvhChild*)p.tlasNodes;
-    const float4* tris = nullptr;
-    int numInternal = p.tlasNumInternal;
-    int objPrims = 0;
-    bool inObj = false;
-    int curInst = -1;
-    int nodeAddr = 0, primCount = 2, primOffset = 0;
-    bool done = false;
-
-    if (kSingle) {
-        DevLod lod;
-        const ObjectEntry e = enter_object(p, 0, wo, wd, hitT, lod);
-        if (kStats) ++nodeVisits;
-        if (!e.hit) {
-            primCount = 0;
-        } else {
-            cur = e.ray;
-            scale = e.scale;
-            hitT = e.tmax;
-            bvh = (const BvhChild*)lod.nodes;
-            tris = lod.triVerts;
-            numInternal = lod.numInternal;
-            inObj = true;
-            curInst = 0;
-            if (lod.numPrims == 1) primCount = 1;
-        }
-    }
-
-    while (true) {
-        // descend through internal nodes, nearer child first, farther pushed
-        while (nodeAddr < numInternal && primCount > 0) {
-            const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
-            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
-            if (kStats) ++nodeVisits;
-            const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
-            const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
-            const int lIdx = __float_as_int(q1.z), lCnt = __float_as_int(q1.w);
-            const int rIdx = __float_as_int(q3.z), rCnt = __float_as_int(q3.w);
-            float c0, c1;
-            const bool t0 = slab(lLo, lHi, cur, hitT, c0);
-            const bool t1 = slab(rLo, rHi, cur, hitT, c1);
-            if (!t0) c0 = 1e38f;
-            if (!t1) c1 = 1e38f;
-            if (t0 && t1) {
-                const bool farLeft = c1 < c0;
-                const int push = farLeft ? lIdx : rIdx;
-                if (push >= numInternal) MFH_PUSH(farLeft ? lCnt : rCnt);
-                MFH_PUSH(push);
-            }
-            const bool nearLeft = c0 <= c1;
-            nodeAddr = nearLeft ? lIdx : rIdx;
-            primCount = (nodeAddr >= numInternal) ? (nearLeft ? lCnt : rCnt) : 2;
-            primOffset = 0;
-            if (!t0 && !t1) primCount = 0;
-        }
-        if (overflow) break;
-        // the primitives of a leaf, one after the other
-        bool descend = false;
-        while (nodeAddr >= numInternal && primCount > 0) {
-            const int leafId = nodeAddr - numInternal + primOffset;
-            ++primOffset;
-            --primCount;
-            if (!kSingle && !inObj) {
-                const int inst = p.tlasPrimIds[leafId];
-                DevLod lod;
-                const ObjectEntry e = enter_object(p, inst, wo, wd, hitT, lod);
-                if (kStats) ++nodeVisits;
-                if (!e.hit) continue;
-                if (lod.numPrims == 1) {
-                    // single-primitive object: test it and return to the instance level
-                    const float4 a = lod.triVerts[0], b = lod.triVerts[1], c = lod.triVerts[2];
-                    float t, w0, w1;
-                    if (kStats) ++primTests;
-                    const float objT = e.tmax;
-                    float ht = objT;
-                    if (ray_triangle(e.ray, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
-                        t > 0.0f && better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
-                        ht = t;
-                        hInst = inst;
-                        hPrim = __float_as_int(a.w);
-                        hw0 = w0;
-                        hw1 = w1;
-                        if (kAny) done = true;
-                    }
-                    hitT = ht / e.scale;
-                    if (done) break;
-                    continue;
-                }
-                if (primCount > 0) {
-                    MFH_PUSH(primCount);
-                    MFH_PUSH(nodeAddr + primOffset);
-                }
-                MFH_PUSH(kSentinel);
-                cur = e.ray;
-                scale = e.scale;
-                hitT = e.tmax;
-                bvh = (const BvhChild*)lod.nodes;
-                tris = lod.triVerts;
-                numInternal = lod.numInternal;
-                objPrims = lod.numPrims;
-                inObj = true;
-                curInst = inst;
-                nodeAddr = 0;
-                primCount = objPrims == 1 ? 1 : 2;
-                primOffset = 0;
-                descend = true;
-                break;
-            }
-            const float4 a = tris[3 * (size_t)leafId], b = tris[3 * (size_t)leafId + 1],
-                         c = tris[3 * (size_t)leafId + 2];
-            float t, w0, w1;
-            const int prim = __float_as_int(a.w);
-            if (kStats) ++primTests;
-            if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
-                better(t, curInst, prim, hitT, hInst, hPrim)) {
-                hitT = t;
-                hInst = curInst;
-                hPrim = prim;
-                hw0 = w0;
-                hw1 = w1;
-                if (kAny) {
-                    done = true;
-                    break;
-                }
-            }
-        }
-        if (done || overflow) break;
-        if (descend) continue;
-        // pop the next subtree (or return from an object to the instance level)
-        if (sp == 0) break;
-        nodeAddr = MFH_POP();
-        if (!kSingle && nodeAddr == kSentinel) {
-            if (sp == 0) break;
-            cur = make_fast_ray(wo, wd);
-            hitT /= scale;
-            scale = 1.0f;
-            bvh = (const BvhChild*)p.tlasNodes;
-            numInternal = p.tlasNumInternal;
-            inObj = false;
-            nodeAddr = MFH_POP();
-        }
-        primCount = (nodeAddr >= numInternal) ? MFH_POP() : 2;
-        primOffset = 0;
-    }
-#undef MFH_PUSH
-#undef MFH_POP
+    const TraceHit h = trace_ray<kAny, kStats, kSingle, kThreads>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z),
+                                                                   r0.w, sStack + threadIdx.x);
     if (kAny) {
-        occluded[i] = overflow ? -1 : (hInst != -1 ? 1 : 0);
+        occluded[i] = h.overflow ? -1 : (h.inst != -1 ? 1 : 0);
     } else {
-        if (inObj) hitT /= scale;  // object_to_world_space
-        hits[i] = make_float4(hitT, __int_as_float(hInst), __int_as_float(hPrim), overflow ? 1.0f : 0.0f);
-        if (bary) bary[i] = make_float2(hw0, hw1);
+        hits[i] = make_float4(h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.overflow ? 1.0f : 0.0f);
+        if (bary) bary[i] = make_float2(h.w0, h.w1);
     }
-    if (kStats) stats[i] = make_uint2(nodeVisits, primTests);
+    if (kStats) stats[i] = make_uint2(h.nodeVisits, h.primTests);
 }
 
 template <bool kAny, bool kStats>

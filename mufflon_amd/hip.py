@@ -19,6 +19,22 @@ i32p = C.POINTER(C.c_int32)
 u64p = C.POINTER(C.c_uint64)
 vp = C.c_void_p
 
+# C-ABI record layouts (include/mufflon_hip.h)
+MATERIAL_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("radiance", "<f4", 3), ("scale", "<f4"),
+                           ("reserved", "<i4", 8)])
+PINHOLE_DTYPE = np.dtype([("position", "<f4", 3), ("tanVFov", "<f4"), ("viewDir", "<f4", 3), ("up", "<f4", 3),
+                          ("width", "<i4"), ("height", "<i4")])
+PT_PARAMS_DTYPE = np.dtype([("seed", "<i4"), ("minPathLength", "<i4"), ("maxPathLength", "<i4"),
+                            ("neeCount", "<i4")])
+POINT_LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("intensity", "<f4", 3)])
+AREA_LIGHT_DTYPE = np.dtype([("points", "<f4", 9), ("uv", "<f4", 6), ("material", "<i4"), ("instance", "<i4"),
+                             ("primitive", "<i4")])
+LIGHT_TREE_INFO_DTYPE = np.dtype([("posRootFlux", "<f4"), ("posRootType", "<u4"), ("posInternalCount", "<u4"),
+                                  ("posLightCount", "<u4"), ("memBytes", "<u4"), ("mapCount", "<i4"),
+                                  ("dirRootFlux", "<f4"), ("backgroundFlux", "<f4")])
+TARGET_RADIANCE = 0
+TARGET_HIT_ID = 1
+
 # name -> argtypes; every function returns int status
 _SIGS = {
     "mfh_create": [i32, C.POINTER(vp)],
@@ -37,6 +53,15 @@ _SIGS = {
     "mfh_trace_closest": [vp, i32, vp, i32, vp, vp],
     "mfh_trace_closest_stats": [vp, i32, vp, i32, vp, vp, vp],
     "mfh_trace_any": [vp, i32, vp, i32, vp],
+    "mfh_lod_set_shading": [vp, i32, f32p, f32p, vp],
+    "mfh_pt_create": [vp, i32, vp, i32, vp, i32, vp, vp, i32p],
+    "mfh_pt_reset": [vp, i32],
+    "mfh_pt_iterate": [vp, i32, i32],
+    "mfh_pt_get_target": [vp, i32, i32, f32p],
+    "mfh_pt_get_iteration": [vp, i32, i32p],
+    "mfh_pt_get_nan_count": [vp, i32, u32p],
+    "mfh_pt_get_light_tree": [vp, i32, vp, vp, u64p, u32p],
+    "mfh_light_tree_build": [vp, i32, vp, i32, f32p, vp, i32, vp, vp, C.c_size_t, u64p, u32p, i32],
     "mfh_timer_begin": [vp],
     "mfh_timer_end": [vp, f32p],
 }
@@ -62,6 +87,48 @@ def load():
     lib.mfh_last_error.restype = C.c_char_p
     _lib = lib
     return lib
+
+
+def camera_record(cam):
+    r = np.zeros(1, PINHOLE_DTYPE)
+    r["position"] = cam["position"]
+    r["tanVFov"] = cam["tan_vfov"]
+    r["viewDir"] = cam["view_dir"]
+    r["up"] = cam["up"]
+    r["width"] = cam["width"]
+    r["height"] = cam["height"]
+    return r
+
+
+def params_record(seed=0, min_path_length=0, max_path_length=16, nee_count=1):
+    r = np.zeros(1, PT_PARAMS_DTYPE)
+    r["seed"], r["minPathLength"], r["maxPathLength"], r["neeCount"] = seed, min_path_length, max_path_length, \
+        nee_count
+    return r
+
+
+def light_tree_build(area_lights, point_lights, scene_box, materials):
+    """Host-only light-tree build (no device): returns (info, mem, keys, codes)."""
+    lib = load()
+    area = np.ascontiguousarray(area_lights, AREA_LIGHT_DTYPE) if area_lights is not None else \
+        np.zeros(0, AREA_LIGHT_DTYPE)
+    pts = np.ascontiguousarray(point_lights, POINT_LIGHT_DTYPE) if point_lights is not None else \
+        np.zeros(0, POINT_LIGHT_DTYPE)
+    mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+    box = np.ascontiguousarray(scene_box, np.float32)
+    n = len(area) + len(pts)
+    info = np.zeros(1, LIGHT_TREE_INFO_DTYPE)
+    mem = np.zeros(max(128 * n, 16), np.uint8)
+    keys = np.zeros(max(len(area), 1), np.uint64)
+    codes = np.zeros(max(len(area), 1), np.uint32)
+    rc = lib.mfh_light_tree_build(area.ctypes.data if len(area) else None, len(area),
+                                  pts.ctypes.data if len(pts) else None, len(pts), _ptr(box, f32p), mats.ctypes.data,
+                                  len(mats), info.ctypes.data, mem.ctypes.data, mem.nbytes, _ptr(keys, u64p),
+                                  _ptr(codes, u32p), len(keys))
+    if rc != 0:
+        raise HipError(rc, lib.mfh_last_error().decode())
+    i = info[0]
+    return i, mem[: i["memBytes"]].copy(), keys[: i["mapCount"]].copy(), codes[: i["mapCount"]].copy()
 
 
 class HipError(RuntimeError):
@@ -248,6 +315,60 @@ class HipContext:
         finally:
             dr.free(); do.free()
         return o
+
+    # path tracer --------------------------------------------------------------
+    def lod_set_shading(self, lod, normals, uvs, mat_idx):
+        n = np.ascontiguousarray(normals, np.float32)
+        uv = np.ascontiguousarray(uvs, np.float32)
+        m = np.ascontiguousarray(mat_idx, np.uint16)
+        self._call("mfh_lod_set_shading", self.handle, lod, _ptr(n, f32p), _ptr(uv, f32p), m.ctypes.data)
+
+    def pt_create(self, scene, materials, camera, params, point_lights=None):
+        mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        cam = camera if isinstance(camera, np.ndarray) else camera_record(camera)
+        par = params if isinstance(params, np.ndarray) else params_record(**params)
+        pts = np.ascontiguousarray(point_lights, POINT_LIGHT_DTYPE) if point_lights is not None else None
+        out = i32()
+        self._call("mfh_pt_create", self.handle, scene, mats.ctypes.data, len(mats),
+                   pts.ctypes.data if pts is not None and len(pts) else None, 0 if pts is None else len(pts),
+                   cam.ctypes.data, par.ctypes.data, C.byref(out))
+        self._pt_size = getattr(self, "_pt_size", {})
+        self._pt_size[out.value] = (int(cam["width"][0]), int(cam["height"][0]))
+        return out.value
+
+    def pt_reset(self, pt):
+        self._call("mfh_pt_reset", self.handle, pt)
+
+    def pt_iterate(self, pt, iterations=1):
+        self._call("mfh_pt_iterate", self.handle, pt, int(iterations))
+
+    def pt_target(self, pt, target=TARGET_RADIANCE):
+        w, h = self._pt_size[pt]
+        ch = 3 if target == TARGET_RADIANCE else 2
+        out = np.zeros(w * h * ch, np.float32)
+        self._call("mfh_pt_get_target", self.handle, pt, target, _ptr(out, f32p))
+        return out.reshape(h, w, ch)
+
+    def pt_iteration(self, pt):
+        out = i32()
+        self._call("mfh_pt_get_iteration", self.handle, pt, C.byref(out))
+        return out.value
+
+    def pt_nan_count(self, pt):
+        out = C.c_uint32()
+        self._call("mfh_pt_get_nan_count", self.handle, pt, C.byref(out))
+        return out.value
+
+    def pt_light_tree(self, pt):
+        info = np.zeros(1, LIGHT_TREE_INFO_DTYPE)
+        self._call("mfh_pt_get_light_tree", self.handle, pt, info.ctypes.data, None, None, None)
+        i = info[0]
+        mem = np.zeros(max(int(i["memBytes"]), 1), np.uint8)
+        keys = np.zeros(max(int(i["mapCount"]), 1), np.uint64)
+        codes = np.zeros(max(int(i["mapCount"]), 1), np.uint32)
+        self._call("mfh_pt_get_light_tree", self.handle, pt, info.ctypes.data, mem.ctypes.data, _ptr(keys, u64p),
+                   _ptr(codes, u32p))
+        return i, mem[: i["memBytes"]], keys[: i["mapCount"]], codes[: i["mapCount"]]
 
     def timer_begin(self):
         self._call("mfh_timer_begin", self.handle)

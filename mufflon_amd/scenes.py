@@ -116,3 +116,137 @@ def lbvh_trace_workload(num_tris=1_000_000, width=2048, height=1024, num_incoher
 
 
 IDENTITY34 = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
+
+
+# ---------------------------------------------------------------- Cornell box
+MAT_EMISSIVE = 0  # Mufflon's Materials enum (material_definitions.hpp:25-27)
+MAT_LAMBERT = 1
+
+from mufflon_amd.hip import MATERIAL_DTYPE, AREA_LIGHT_DTYPE  # noqa: E402
+
+
+def material(kind, albedo=(0, 0, 0), radiance=(0, 0, 0), scale=1.0):
+    m = np.zeros(1, MATERIAL_DTYPE)
+    m["type"] = kind
+    m["albedo"] = albedo
+    m["radiance"] = radiance
+    m["scale"] = scale
+    return m
+
+
+class MeshBuilder:
+    """Triangle mesh with per-vertex normals / uvs and per-triangle material."""
+
+    def __init__(self):
+        self.v, self.n, self.uv, self.t, self.m = [], [], [], [], []
+
+    def _vbase(self):
+        return sum(len(a) for a in self.v)
+
+    def add_quad(self, p0, p1, p2, p3, normal, mat):
+        """Quad p0 p1 p2 p3 (counter-clockwise seen from the normal side) as 2 triangles."""
+        b = self._vbase()
+        P = np.array([p0, p1, p2, p3], np.float64)
+        self.v.append(P)
+        self.n.append(np.tile(np.asarray(normal, np.float64), (4, 1)))
+        self.uv.append(np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float64))
+        self.t.append(np.array([[b, b + 1, b + 2], [b, b + 2, b + 3]]))
+        self.m += [mat, mat]
+
+    def add_box(self, center, size, angle_deg, mat):
+        cx, y0, cz = center
+        sx, sy, sz = size
+        a = np.radians(angle_deg)
+        R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+
+        def P(x, y, z):
+            return R @ np.array([x * sx / 2, y * sy, z * sz / 2]) + np.array([cx, y0, cz])
+
+        faces = [  # (corners ccw from outside, outward normal)
+            ([(-1, 1, -1), (-1, 1, 1), (1, 1, 1), (1, 1, -1)], (0, 1, 0)),
+            ([(-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)], (0, -1, 0)),
+            ([(-1, 0, 1), (1, 0, 1), (1, 1, 1), (-1, 1, 1)], (0, 0, 1)),
+            ([(1, 0, -1), (-1, 0, -1), (-1, 1, -1), (1, 1, -1)], (0, 0, -1)),
+            ([(1, 0, 1), (1, 0, -1), (1, 1, -1), (1, 1, 1)], (1, 0, 0)),
+            ([(-1, 0, -1), (-1, 0, 1), (-1, 1, 1), (-1, 1, -1)], (-1, 0, 0)),
+        ]
+        for corners, nrm in faces:
+            self.add_quad(*[P(*c) for c in corners], normal=R @ np.array(nrm, np.float64), mat=mat)
+
+    def add_uv_sphere(self, center, radius, rings, segments, mat):
+        b = self._vbase()
+        c = np.asarray(center, np.float64)
+        th = np.linspace(0, np.pi, rings + 1)  # polar angle
+        ph = np.linspace(0, 2 * np.pi, segments + 1)
+        T, Pp = np.meshgrid(th, ph, indexing="ij")
+        n = np.stack([np.sin(T) * np.cos(Pp), np.cos(T), np.sin(T) * np.sin(Pp)], -1).reshape(-1, 3)
+        self.v.append(c + radius * n)
+        self.n.append(n)
+        self.uv.append(np.stack([(Pp / (2 * np.pi)).reshape(-1), (T / np.pi).reshape(-1)], -1))
+        W = segments + 1
+        tris = []
+        for r in range(rings):
+            i0 = b + r * W + np.arange(segments)
+            i1 = i0 + 1
+            j0 = i0 + W
+            j1 = j0 + 1
+            if r > 0:  # skip the degenerate triangles at the north pole
+                tris.append(np.stack([i0, i1, j1], 1))
+            if r < rings - 1:
+                tris.append(np.stack([i0, j1, j0], 1))
+        self.t.append(np.concatenate(tris))
+        self.m += [mat] * sum(len(x) for x in tris)
+
+    def arrays(self):
+        v = np.concatenate(self.v).astype(np.float32)
+        n = np.concatenate(self.n)
+        n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+        uv = np.concatenate(self.uv).astype(np.float32)
+        t = np.concatenate(self.t).astype(np.uint32)
+        m = np.array(self.m, np.uint16)
+        return v, n, uv, t, m
+
+
+def cornell_box(sphere_rings=128, sphere_segments=256, width=256, height=256):
+    """Procedural Cornell box: 5 wall quads, 2 boxes, a UV sphere and a
+    0.25-scale ceiling quad light (radiance 17/12/4). Box [-1,1]x[0,2]x[-1,1]."""
+    WHITE, RED, GREEN, LIGHT = 1, 2, 3, 0
+    mats = np.concatenate([
+        material(MAT_EMISSIVE, radiance=(17.0, 12.0, 4.0), scale=1.0),
+        material(MAT_LAMBERT, albedo=(0.73, 0.73, 0.73)),
+        material(MAT_LAMBERT, albedo=(0.65, 0.05, 0.05)),
+        material(MAT_LAMBERT, albedo=(0.12, 0.45, 0.15)),
+    ])
+    mb = MeshBuilder()
+    mb.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0), WHITE)      # floor
+    mb.add_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0), WHITE)     # ceiling
+    mb.add_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1), WHITE)    # back
+    mb.add_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0), RED)      # left
+    mb.add_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0), GREEN)       # right
+    mb.add_box((-0.40, 0.0, -0.35), (0.55, 1.2, 0.55), 18.0, WHITE)
+    mb.add_box((0.45, 0.0, -0.40), (0.50, 0.5, 0.50), -17.0, WHITE)
+    mb.add_uv_sphere((0.05, 0.32, 0.45), 0.32, sphere_rings, sphere_segments, WHITE)
+    L = 0.25  # quarter of the ceiling edge
+    mb.add_quad((-L, 1.99, -L), (L, 1.99, -L), (L, 1.99, L), (-L, 1.99, L), (0, -1, 0), LIGHT)
+    v, n, uv, t, m = mb.arrays()
+    vfov = np.radians(40.0)
+    cam = dict(position=np.array([0.0, 1.0, 3.8], np.float32), view_dir=np.array([0, 0, -1], np.float32),
+               up=np.array([0, 1, 0], np.float32), tan_vfov=np.float32(np.tan(vfov / 2)), width=width,
+               height=height)
+    return dict(verts=v, normals=n, uvs=uv, tris=t, mat_idx=m, materials=mats, camera=cam)
+
+
+def emissive_area_lights(verts, uvs, tris, mat_idx, materials, instance=0):
+    """Area lights of the emissive triangles of one identity-transformed
+    instance, in Mufflon's gathering order (world_container.cpp:895-937)."""
+    emissive = materials["type"][mat_idx] == MAT_EMISSIVE
+    idx = np.nonzero(emissive)[0]
+    out = np.zeros(len(idx), AREA_LIGHT_DTYPE)
+    for j, p in enumerate(idx):
+        t = tris[p]
+        out[j]["points"] = verts[t].reshape(-1)
+        out[j]["uv"] = uvs[t].reshape(-1)
+        out[j]["material"] = mat_idx[p]
+        out[j]["instance"] = instance
+        out[j]["primitive"] = p
+    return out

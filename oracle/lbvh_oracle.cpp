@@ -44,12 +44,16 @@ inline u64 part_by_two21(u32 x) {
 
 inline float clampf(float v, float lo, float hi) { return mn(mx(v, lo), hi); }
 
+// sdiv of util/types.hpp:124-128: x / (d + 1e-20 * sgn(d)), with sgn(0) = 1
+// (Epsilon's sgn is unavailable; a zero sign would make the "safe" division
+// divide by zero).
+inline float sdiv(float x, float d) { return x / (d + 1e-20f * (d < 0.0f ? -1.0f : 1.0f)); }
+
 // build_lbvh.cu:70-78 (u64 variant) and :81-85 (normalize_position with sdiv).
 inline u64 morton_of_point(V3 c, const Box& bb) {
     const V3 span = bb.hi - bb.lo;
     const V3 num = c - bb.lo;
-    const V3 n{span.x != 0.0f ? num.x / span.x : 0.0f, span.y != 0.0f ? num.y / span.y : 0.0f,
-               span.z != 0.0f ? num.z / span.z : 0.0f};
+    const V3 n{sdiv(num.x, span.x), sdiv(num.y, span.y), sdiv(num.z, span.z)};
     const u32 qx = (u32)clampf(n.x * 2097152.0f, 0.0f, 2097151.0f);
     const u32 qy = (u32)clampf(n.y * 2097152.0f, 0.0f, 2097151.0f);
     const u32 qz = (u32)clampf(n.z * 2097152.0f, 0.0f, 2097151.0f);

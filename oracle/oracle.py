@@ -227,3 +227,77 @@ def trace_brute(scene, rays):
     prim = np.zeros(n, np.int32)
     lib.orc_trace_brute(C.byref(scene.s), _p(r, f32p), n, _p(t, f32p), _p(inst, i32p), _p(prim, i32p))
     return t, inst, prim
+
+
+class OrcPtScene(C.Structure):
+    _fields_ = [
+        ("geo", OrcScene),
+        ("normals", f32p), ("uvs", f32p), ("matIdx", C.c_void_p), ("materials", C.c_void_p),
+        ("numMaterials", C.c_int32),
+        ("lightMem", C.c_void_p), ("posRootFlux", C.c_float), ("posRootType", C.c_uint32),
+        ("posInternalCount", C.c_uint32), ("posLightCount", C.c_uint32), ("dirRootFlux", C.c_float),
+        ("bgFlux", C.c_float), ("mapKeys", u64p), ("mapCodes", u32p), ("mapCount", C.c_int32),
+        ("sceneBox", C.c_float * 6), ("camPos", C.c_float * 3), ("tanVFov", C.c_float),
+        ("camDir", C.c_float * 3), ("camUp", C.c_float * 3), ("width", C.c_int32), ("height", C.c_int32),
+    ]
+
+
+class PtScene:
+    """Oracle path-tracer scene over an oracle traversal Scene (single LoD list
+    order), with shading arrays concatenated like the geometry, the host-built
+    light tree (info, mem, keys, codes) and a pinhole camera dict."""
+
+    def __init__(self, scene, normals, uvs, mat_idx, materials, light_tree, scene_box, camera):
+        self.scene = scene
+        self.normals = np.ascontiguousarray(normals, np.float32)
+        self.uvs = np.ascontiguousarray(uvs, np.float32)
+        self.mat_idx = np.ascontiguousarray(mat_idx, np.uint16)
+        self.materials = np.ascontiguousarray(materials)
+        info, mem, keys, codes = light_tree
+        self.mem = np.ascontiguousarray(mem if len(mem) else np.zeros(16, np.uint8))
+        self.keys = np.ascontiguousarray(keys if len(keys) else np.zeros(1, np.uint64), np.uint64)
+        self.codes = np.ascontiguousarray(codes if len(codes) else np.zeros(1, np.uint32), np.uint32)
+        s = OrcPtScene()
+        s.geo = scene.s
+        s.normals = _p(self.normals, f32p)
+        s.uvs = _p(self.uvs, f32p)
+        s.matIdx = self.mat_idx.ctypes.data
+        s.materials = self.materials.ctypes.data
+        s.numMaterials = len(self.materials)
+        s.lightMem = self.mem.ctypes.data
+        s.posRootFlux = float(info["posRootFlux"])
+        s.posRootType = int(info["posRootType"])
+        s.posInternalCount = int(info["posInternalCount"])
+        s.posLightCount = int(info["posLightCount"])
+        s.dirRootFlux = float(info["dirRootFlux"])
+        s.bgFlux = float(info["backgroundFlux"])
+        s.mapKeys = _p(self.keys, u64p)
+        s.mapCodes = _p(self.codes, u32p)
+        s.mapCount = int(info["mapCount"])
+        for k in range(6):
+            s.sceneBox[k] = float(scene_box[k])
+        for k in range(3):
+            s.camPos[k] = float(camera["position"][k])
+            s.camDir[k] = float(camera["view_dir"][k])
+            s.camUp[k] = float(camera["up"][k])
+        s.tanVFov = float(camera["tan_vfov"])
+        s.width = int(camera["width"])
+        s.height = int(camera["height"])
+        self.s = s
+
+
+def pt_render(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations):
+    """Oracle path tracer: running means of Radiance (h,w,3), Hit ID (h,w,2), NaN count."""
+    lib = load()
+    if not hasattr(lib, "_pt_ready"):
+        lib.orc_pt_render.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      f32p, f32p, u32p]
+        lib.orc_pt_render.restype = C.c_int
+        lib._pt_ready = True
+    w, h = pt_scene.s.width, pt_scene.s.height
+    rad = np.zeros(w * h * 3, np.float32)
+    hid = np.zeros(w * h * 2, np.float32)
+    nans = C.c_uint32()
+    lib.orc_pt_render(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
+                      _p(rad, f32p), _p(hid, f32p), C.byref(nans))
+    return rad.reshape(h, w, 3), hid.reshape(h, w, 2), nans.value

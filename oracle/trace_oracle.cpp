@@ -32,29 +32,9 @@ struct Node {  // lbvh.hpp:13-18, 32 bytes packed
 
 }  // namespace
 
-extern "C" {
-// Flat scene layout shared with the tests (all arrays host memory).
-struct OrcScene {
-    int32_t numLods;
-    const int32_t* lodVertOffset;   // numLods+1 (in vertices)
-    const int32_t* lodTriOffset;    // numLods+1 (in triangles); primIds share it
-    const int32_t* lodNodeOffset;   // numLods+1 (in 64-byte internal nodes)
-    const int32_t* lodNumInternal;  // numLods
-    const float* verts;             // xyz per vertex
-    const uint32_t* tris;           // 3 local vertex indices per triangle
-    const void* nodes;              // concatenated BVHs
-    const int32_t* primIds;         // concatenated, local primitive ids
-    const float* lodBoxes;          // 6 per LoD (object AABB)
-    int32_t numInstances;
-    const float* worldToInst;       // 12 per instance
-    const int32_t* instLod;         // LoD of each instance
-    const void* tlasNodes;
-    int32_t tlasNumInternal;
-    const int32_t* tlasPrimIds;
-};
-}
+#include "trace_core.hpp"
 
-namespace {
+namespace orc {
 
 struct Lod {
     const Node* bvh;
@@ -110,12 +90,6 @@ bool slab(const Box& b, const FastRay& r, float tmax, float& cmin) {
     const float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
     return slab(lo, hi, r, tmax, cmin);
 }
-
-struct Hit {
-    float t;
-    i32 inst, prim;
-    float w0, w1;
-};
 
 // Strictly closer wins; at equal t the lower (instance, primitive) pair wins.
 // Without a previous hit, t must be strictly below tmax (intersection.cu:205).
@@ -297,7 +271,7 @@ bool traverse(const OrcScene& s, const FastRay& fray, float& hitT, Hit& h, bool 
     return h.inst != kIgnore;
 }
 
-}  // namespace
+}  // namespace orc
 
 extern "C" {
 

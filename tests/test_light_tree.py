@@ -1,0 +1,117 @@
+"""Host light-tree builder (Mufflon's LightTreeBuilder layout) and the oracle
+path tracer's closed-form cases. No device needed."""
+import numpy as np
+import pytest
+
+import pt_util as P
+from mufflon_amd import hip
+from mufflon_amd import scenes as S
+
+NODE = np.dtype([("lflux", "<f4"), ("loff", "<u4"), ("ltype", "<u2"), ("rtype", "<u2"), ("roff", "<u4"),
+                 ("rflux", "<f4"), ("center", "<f4", 3)])
+TRI = np.dtype([("posV", "<f4", 9), ("material", "<u2"), ("pad", "<u2"), ("uvV", "<f4", 6)])
+POINT = np.dtype([("position", "<f4", 3), ("medium", "<u2"), ("pad", "<u2"), ("intensity", "<f4", 3),
+                  ("pad1", "<f4")])
+INTERNAL = 0xFFFF
+
+
+def _random_lights(n_area, n_point, seed):
+    rng = np.random.default_rng(seed)
+    mats = np.concatenate([S.material(S.MAT_EMISSIVE, radiance=(2.0, 3.0, 4.0), scale=1.5),
+                           S.material(S.MAT_EMISSIVE, radiance=(1.0, 0.5, 0.25)),
+                           S.material(S.MAT_LAMBERT, albedo=(0.5, 0.5, 0.5))])
+    al = np.zeros(n_area, hip.AREA_LIGHT_DTYPE)
+    for i in range(n_area):
+        c = rng.uniform(0.1, 0.9, 3)
+        al[i]["points"] = (c + rng.normal(scale=0.05, size=(3, 3))).reshape(-1)
+        al[i]["uv"] = rng.random(6)
+        al[i]["material"] = i % 2
+        al[i]["instance"] = i % 3
+        al[i]["primitive"] = 1000 + i
+    pl = np.zeros(n_point, hip.POINT_LIGHT_DTYPE)
+    pl["position"] = rng.uniform(0.1, 0.9, (n_point, 3))
+    pl["intensity"] = rng.uniform(0.5, 3.0, (n_point, 3))
+    return al, pl, mats
+
+
+def _walk(mem, info):
+    """Yield (offset, type) of every leaf and check internal flux sums."""
+    leaves = []
+    stack = [(0, int(info["posRootType"]))]
+    while stack:
+        off, typ = stack.pop()
+        if typ == INTERNAL:
+            nd = np.frombuffer(mem[off:off + 32].tobytes(), NODE)[0]
+            for o, t, f in ((nd["loff"], nd["ltype"], nd["lflux"]), (nd["roff"], nd["rtype"], nd["rflux"])):
+                stack.append((int(o), int(t)))
+                assert f > 0
+        else:
+            leaves.append((off, typ))
+    return leaves
+
+
+@pytest.mark.parametrize("n_area,n_point", [(1, 0), (2, 0), (3, 0), (5, 2), (13, 0), (0, 9), (32, 32)])
+def test_tree_structure(n_area, n_point):
+    al, pl, mats = _random_lights(n_area, n_point, n_area * 7 + n_point)
+    box = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    info, mem, keys, codes = hip.light_tree_build(al, pl, box, mats)
+    n = n_area + n_point
+    assert info["posLightCount"] == n and info["mapCount"] == n_area
+    h = int(np.floor(np.log2(n)))
+    assert info["posInternalCount"] == (0 if n == 1 else (2 ** h - 1) + (n - 2 ** h))
+    leaves = _walk(mem, info)
+    assert len(leaves) == n
+    assert sorted(t for _, t in leaves) == sorted([2] * n_area + [0] * n_point)
+    # every primitive's code leads to its own triangle record
+    for key, code in zip(keys, codes):
+        off, typ = 0, int(info["posRootType"])
+        while typ == INTERNAL:
+            nd = np.frombuffer(mem[off:off + 32].tobytes(), NODE)[0]
+            if code & 0x80000000:
+                off, typ = int(nd["roff"]), int(nd["rtype"])
+            else:
+                off, typ = int(nd["loff"]), int(nd["ltype"])
+            code = (int(code) << 1) & 0xFFFFFFFF
+        assert typ == 2
+        rec = np.frombuffer(mem[off:off + 64].tobytes(), TRI)[0]
+        src = al[al["primitive"] == (int(key) & 0xFFFFFFFF)][0]
+        assert np.array_equal(rec["posV"][:3], src["points"][:3])
+        assert np.array_equal(rec["posV"][3:6], (src["points"][3:6] - src["points"][:3]).astype(np.float32))
+    # root flux: float sum of the two root children (each a float sum below)
+    if n > 1:
+        nd = np.frombuffer(mem[:32].tobytes(), NODE)[0]
+        assert info["posRootFlux"] == np.float32(nd["lflux"] + nd["rflux"])
+        total = 0.0
+        for off, typ in leaves:
+            if typ == 0:
+                r = np.frombuffer(mem[off:off + 32].tobytes(), POINT)[0]
+                total += float(np.sum(r["intensity"].astype(np.float64) * 4 * np.pi))
+            else:
+                r = np.frombuffer(mem[off:off + 64].tobytes(), TRI)[0]
+                e1, e2 = r["posV"][3:6].astype(np.float64), r["posV"][6:9].astype(np.float64)
+                area = np.linalg.norm(np.cross(e1, e2)) / 2
+                m = mats[r["material"]]
+                total += float(np.sum(m["radiance"].astype(np.float64) * m["scale"]) * area * 2 * np.pi)
+        assert abs(float(info["posRootFlux"]) - total) <= 1e-5 * total
+
+
+def test_oracle_pt_sees_the_light_exactly(oracle):
+    """Pixels whose camera ray hits a light triangle in every iteration carry
+    exactly the light's radiance (no MIS on the first segment, no further
+    bounce from an emitter)."""
+    c = P.cornell(8, 16, 96)
+    ps = P.oracle_pt_scene(oracle, c)
+    rad, hit, nans = oracle.pt_render(ps, 1, 0, 8, 1, 4)
+    light_prims = np.nonzero(c["mat_idx"] == 0)[0]
+    on_light = np.isin(hit[..., 1], light_prims) & (hit[..., 0] == 0)
+    assert on_light.sum() > 10 and nans == 0
+    assert np.array_equal(rad[on_light], np.tile(np.float32([17.0, 12.0, 4.0]), (int(on_light.sum()), 1)))
+
+
+def test_oracle_pt_is_deterministic_and_seeded(oracle):
+    c = P.cornell(8, 16, 32)
+    ps = P.oracle_pt_scene(oracle, c)
+    a, _, _ = oracle.pt_render(ps, 1, 0, 8, 1, 2)
+    b, _, _ = oracle.pt_render(ps, 1, 0, 8, 1, 2)
+    d, _, _ = oracle.pt_render(ps, 2, 0, 8, 1, 2)
+    assert np.array_equal(a, b) and not np.array_equal(a, d)

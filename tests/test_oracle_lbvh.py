@@ -39,10 +39,13 @@ def test_morton_clamps_and_corners(oracle):
     half = 1 << 20
     assert codes[2] == LC.interleave21(([0], [half], [top]))[0]
     assert codes[3] == LC.interleave21(([half], [half], [half]))[0]
-    # zero span on an axis maps that axis to 0 (sdiv)
+    # zero span on an axis: sdiv divides by 1e-20, so the point on the plane
+    # maps to 0 and a point one ulp above it clamps to the top
     flat = np.array([0, 0, 1, 1, 1, 1], np.float32)
-    c = oracle.morton_points(np.array([[0.5, 0.5, 1.0]], np.float32), flat)
+    pts2 = np.array([[0.5, 0.5, 1.0], [0.5, 0.5, np.nextafter(np.float32(1), np.float32(2))]], np.float32)
+    c = oracle.morton_points(pts2, flat)
     assert c[0] == LC.interleave21(([half], [half], [0]))[0]
+    assert c[1] == LC.interleave21(([half], [half], [top]))[0]
 
 
 def _soup_with_duplicates(n, seed):
