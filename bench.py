@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
+    ap.add_argument("--pt-iterations", type=int, default=64,
+                    help="Cornell-box path-tracer iterations measured next to the main workload (0: skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (torch), whose FETCH_SIZE / "
                          "WRITE_SIZE counters scale the traversal kernel's to HBM bytes")
@@ -73,6 +75,51 @@ def cpu_baseline(verts, tris, rays):
                        "rays on %d OpenMP threads (%.3f s, %.2f Mrays/s)" %
                        (len(tris), build_s, len(rays), cores, trace_s, len(rays) / trace_s / 1e6),
                 lbvh_build_ms=build_s * 1e3, trace_mrays_s=len(rays) / trace_s / 1e6)
+
+
+def pt_cornell(args, cpu_baseline_too):
+    """Megakernel path tracer on the procedural Cornell box: 256x256, 128x256
+    sphere, depth 8, NEE 1, seed 1; Mpaths/s = pixels * iterations / time."""
+    from mufflon_amd import scenes as S
+    from mufflon_amd.hip import HipContext
+    c = S.cornell_box(128, 256, 256, 256)
+    v = c["verts"]
+    box = np.concatenate([v.min(0), v.max(0)]).astype(np.float32)
+    ctx = HipContext(0)
+    lod = ctx.lod_create(v, c["tris"])
+    ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
+    ctx.lbvh_build(lod, box)
+    sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+    pt = ctx.pt_create(sc, c["materials"], c["camera"], dict(seed=1, max_path_length=8, nee_count=1))
+    ctx.pt_iterate(pt, 4)
+    ctx.pt_reset(pt)
+    ctx.synchronize()
+    n = args.pt_iterations
+    ctx.timer_begin()
+    ctx.pt_iterate(pt, n)
+    ms = ctx.timer_end()
+    pixels = 256 * 256
+    out = {"workload": "Cornell box 256x256, 65,060 triangles, depth 8, NEE 1, seed 1, megakernel",
+           "iterations": n, "ms_per_iteration": round(ms / n, 4),
+           "mpaths_s": round(pixels * n / (ms / 1e3) / 1e6, 3)}
+    if cpu_baseline_too:
+        from oracle import oracle as O
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        lt = ctx.pt_light_tree(pt)
+        import scene_util as U
+        scene, _ = U.oracle_scene(O, [(v, c["tris"])], np.array([0], np.int32), S.IDENTITY34.reshape(1, 12))
+        ps = O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], lt, box, c["camera"])
+        O.pt_render(ps, 1, 0, 8, 1, 1)
+        k = 8
+        t0 = time.perf_counter()
+        O.pt_render(ps, 1, 0, 8, 1, k)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(pixels * k / dt / 1e6, 4), "unit": "Mpaths/s",
+                               "cores": os.cpu_count() or 1, "kind": "port",
+                               "sample": "%d iterations of the same Cornell workload on %d OpenMP threads (%.2f s)"
+                                         % (k, os.cpu_count() or 1, dt)}
+    ctx.close()
+    return out
 
 
 def main():
@@ -183,6 +230,9 @@ def main():
             print("cpu baseline ...", file=sys.stderr, flush=True)
             out["cpu_baseline"] = cpu_baseline(verts, tris, rays)
     ctx.close()
+    if args.pt_iterations > 0 and not args.profile_only:
+        print("path tracer ...", file=sys.stderr, flush=True)
+        out["pt_cornell"] = pt_cornell(args, not args.no_cpu_baseline)
     print(json.dumps(out))
 
 
