@@ -34,3 +34,78 @@ def gpu_pt(ctx, c, seed=1, max_path_length=8, nee_count=1, min_path_length=0):
                        dict(seed=seed, min_path_length=min_path_length, max_path_length=max_path_length,
                             nee_count=nee_count))
     return pt
+
+
+def _m34(M):
+    return np.asarray(M, np.float64)[:3, :4].reshape(-1).astype(np.float32)
+
+
+def _affine(R=np.eye(3), s=(1, 1, 1), t=(0, 0, 0)):
+    M = np.eye(4)
+    M[:3, :3] = np.asarray(R) @ np.diag(s)
+    M[:3, 3] = t
+    return M
+
+
+def _rot_y(deg):
+    a = np.radians(deg)
+    return np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+
+
+def cornell_instanced(size=48):
+    """The Cornell box as separate LoDs placed by instances: walls (identity),
+    a unit sphere instanced three times (scaled / moved), a unit cube twice
+    (rotated, anisotropic scale) and the light quad (rotated half-turn, scaled)."""
+    base = S.cornell_box(8, 16, size, size)
+    mats = base["materials"]
+    lods = []
+
+    def mesh(fn):
+        mb = S.MeshBuilder()
+        fn(mb)
+        v, n, uv, t, m = mb.arrays()
+        return dict(verts=v, normals=n, uvs=uv, tris=t, mat_idx=m)
+
+    lods.append(mesh(lambda mb: (
+        mb.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0), 1),
+        mb.add_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0), 1),
+        mb.add_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1), 1),
+        mb.add_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0), 2),
+        mb.add_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0), 3))))
+    lods.append(mesh(lambda mb: mb.add_uv_sphere((0, 0, 0), 1.0, 8, 16, 1)))
+    lods.append(mesh(lambda mb: mb.add_box((0, 0, 0), (1, 1, 1), 0.0, 1)))
+    lods.append(mesh(lambda mb: mb.add_quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1), (0, -1, 0), 0)))
+    inst_to_world = [
+        (0, _affine()),
+        (1, _affine(s=(0.3, 0.3, 0.3), t=(0.1, 0.3, 0.45))),
+        (1, _affine(s=(0.2, 0.25, 0.2), t=(-0.6, 0.25, 0.5))),
+        (1, _affine(s=(0.15, 0.15, 0.15), t=(0.6, 1.3, -0.2))),
+        (2, _affine(_rot_y(20), s=(0.5, 1.1, 0.5), t=(-0.4, 0.0, -0.35))),
+        (2, _affine(_rot_y(-15), s=(0.45, 0.45, 0.5), t=(0.45, 0.0, -0.4))),
+        (3, _affine(_rot_y(180), s=(0.3, 1.0, 0.2), t=(0.0, 1.99, 0.0))),
+    ]
+    inst_lod = np.array([i for i, _ in inst_to_world], np.int32)
+    w2i = np.stack([_m34(np.linalg.inv(M)) for _, M in inst_to_world])
+    return dict(lods=lods, inst_lod=inst_lod, world_to_inst=w2i, materials=mats, camera=base["camera"])
+
+
+def gpu_pt_instanced(ctx, c, seed=1, max_path_length=8, nee_count=1):
+    ids = []
+    for L in c["lods"]:
+        lod = ctx.lod_create(L["verts"], L["tris"])
+        ctx.lod_set_shading(lod, L["normals"], L["uvs"], L["mat_idx"])
+        ctx.lbvh_build(lod)
+        ids.append(lod)
+    sc = ctx.scene_create(np.array(ids, np.int32)[c["inst_lod"]], c["world_to_inst"])
+    pt = ctx.pt_create(sc, c["materials"], c["camera"],
+                       dict(seed=seed, max_path_length=max_path_length, nee_count=nee_count))
+    return pt, sc
+
+
+def oracle_pt_scene_instanced(O, c, light_tree, scene_box):
+    geoms = [(L["verts"], L["tris"]) for L in c["lods"]]
+    scene, _ = U.oracle_scene(O, geoms, c["inst_lod"], c["world_to_inst"])
+    normals = np.concatenate([L["normals"] for L in c["lods"]])
+    uvs = np.concatenate([L["uvs"] for L in c["lods"]])
+    mat_idx = np.concatenate([L["mat_idx"] for L in c["lods"]])
+    return O.PtScene(scene, normals, uvs, mat_idx, c["materials"], light_tree, scene_box, c["camera"])

@@ -115,3 +115,27 @@ def test_oracle_pt_is_deterministic_and_seeded(oracle):
     b, _, _ = oracle.pt_render(ps, 1, 0, 8, 1, 2)
     d, _, _ = oracle.pt_render(ps, 2, 0, 8, 1, 2)
     assert np.array_equal(a, b) and not np.array_equal(a, d)
+
+
+def test_oracle_pt_instanced_renders(oracle):
+    """The instanced Cornell variant renders light through transformed lights."""
+    c = P.cornell_instanced(32)
+    # light gathered in world space as the device path does: instance 6 holds the light quad
+    L = c["lods"][3]
+    Minv = np.eye(4)
+    Minv[:3, :] = c["world_to_inst"][6].reshape(3, 4)
+    M = np.linalg.inv(Minv)
+    al = np.zeros(2, hip.AREA_LIGHT_DTYPE)
+    for j in range(2):
+        t = L["tris"][j]
+        p = (M[:3, :3] @ L["verts"][t].T.astype(np.float64)).T + M[:3, 3]
+        al[j]["points"] = p.astype(np.float32).reshape(-1)
+        al[j]["uv"] = L["uvs"][t].reshape(-1)
+        al[j]["material"] = 0
+        al[j]["instance"] = 6
+        al[j]["primitive"] = j
+    box = np.array([-1, 0, -1, 1, 2, 1], np.float32)
+    lt = hip.light_tree_build(al, None, box, c["materials"])
+    ps = P.oracle_pt_scene_instanced(oracle, c, lt, box)
+    rad, hit, nans = oracle.pt_render(ps, 1, 0, 6, 1, 2)
+    assert nans == 0 and rad.mean() > 0.01 and (hit[..., 0] == 6).any()
