@@ -107,17 +107,61 @@ int mfh_trace_any(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t
  * gpu_pt.cpp / gpu_pt.cu / pt_common.hpp).
  * ------------------------------------------------------------------------ */
 
-/* Material record (64 bytes). type follows Mufflon's Materials enum
- * (material_definitions.hpp:25-46); textures are constant (1x1) values. */
+/* Material record (128 bytes). type follows Mufflon's Materials enum
+ * (material_definitions.hpp:25-46); the world_add_material descriptor it
+ * replaces is core_interface.h's MaterialParams (inner/outer medium, layer
+ * parameters). Texture slots hold a texture id of mfh_texture_create or -1
+ * for the constant in the record (Mufflon's 1x1 value textures,
+ * world_add_texture_value). The inner medium of MICROFACET is
+ * {refractionIndex, absorption}; every other type has inner = outer medium
+ * (Material::compute_medium, material.cpp:54-60). */
 #define MFH_MATERIAL_EMISSIVE 0
 #define MFH_MATERIAL_LAMBERT 1
+#define MFH_MATERIAL_TORRANCE 3
+#define MFH_MATERIAL_MICROFACET 10
+#define MFH_NDF_GGX 0
+#define MFH_NDF_BECKMANN 1
+#define MFH_SHADOWING_VCAVITY 0
+#define MFH_SHADOWING_SMITH 1
 typedef struct mfh_material {
     int32_t type;
-    float albedo[3];   /* LAMBERT: albedo */
-    float radiance[3]; /* EMISSIVE: radiance */
-    float scale;       /* EMISSIVE: radiance scale (MatEmissive::NonTexParams) */
-    int32_t reserved[8];
+    float albedo[3];             /* LAMBERT, TORRANCE: albedo (ALBEDO texture) */
+    float radiance[3];           /* EMISSIVE: radiance (EMISSION texture) */
+    float scale[3];              /* EMISSIVE: radiance scale (MatEmissive::NonTexParams) */
+    float roughness[2];          /* TORRANCE, MICROFACET: alpha (ROUGHNESS texture) */
+    int32_t ndf;                 /* TORRANCE, MICROFACET: MFH_NDF_* */
+    int32_t shadowing;           /* TORRANCE, MICROFACET: MFH_SHADOWING_* */
+    float refractionIndex;       /* MICROFACET: inner medium */
+    float absorption[3];         /* MICROFACET: inner medium absorption per unit length */
+    float outerRefractionIndex;  /* medium on the normal side, 1 = vacuum */
+    float outerAbsorption[3];
+    int32_t albedoTexture;       /* -1 or a texture id */
+    int32_t roughnessTexture;    /* -1 or a texture id (1 channel: isotropic, 2+: x/y) */
+    int32_t emissionTexture;     /* must be -1 (emission textures are not supported) */
+    int32_t reserved[7];
 } mfh_material;
+
+/* Textures (world_add_texture / world_add_texture_value, interface.cpp:1938-2100)
+ * with the CPU texture's sampling (cputexture.cpp:372-411): NEAREST reads
+ * texel floor(uv * size); LINEAR blends 4 texels with weights frac(uv * size)
+ * (no half-texel offset); both wrap x and y by a non-negative modulo. */
+#define MFH_SAMPLING_NEAREST 0
+#define MFH_SAMPLING_LINEAR 1
+/* texels: width * height * channels floats, row-major, 1 <= channels <= 4. */
+int mfh_texture_create(mfh_context* ctx, int32_t width, int32_t height, int32_t channels, int32_t samplingMode,
+                       const float* texels, int32_t* outTextureId);
+
+/* Background (BackgroundDesc, background.hpp; world_set_env_light...):
+ * COLORED = constant radiance color * scale; ENVMAP = polar (equirectangular)
+ * map texture * scale, importance sampled through a summed-area table. */
+#define MFH_BACKGROUND_COLORED 0
+#define MFH_BACKGROUND_ENVMAP 1
+typedef struct mfh_background {
+    int32_t type;
+    float color[3];
+    float scale;
+    int32_t envmapTexture; /* ENVMAP: texture id (RGB, polar) */
+} mfh_background;
 
 /* Point light (lights.hpp:46-50): position and intensity (W/sr). */
 typedef struct mfh_point_light {
@@ -207,6 +251,20 @@ int mfh_pt_get_nan_count(mfh_context* ctx, int32_t ptId, uint32_t* outCount);
  * keys = instance << 32 | primitive, sorted). */
 int mfh_pt_get_light_tree(mfh_context* ctx, int32_t ptId, mfh_light_tree_info* outInfo, void* outMem,
                           uint64_t* outMapKeys, uint32_t* outMapCodes);
+
+/* Host-only background preparation (no device needed), shared by the device
+ * upload and any CPU renderer: the flux the light tree weighs the background
+ * with (Background::acquire_const, background.cpp:30-66, and its
+ * compute_envmap_flux / compute_constant_flux, :78-135) into outFlux[3], and
+ * for ENVMAP the summed-area table of create_summed_area_table
+ * (texture_sampling.hpp:49-110) into outSat (width * height floats).
+ * envTexels as in mfh_texture_create (NULL for COLORED). */
+int mfh_background_build(const mfh_background* bg, const float* envTexels, int32_t width, int32_t height,
+                         int32_t channels, const float* sceneBox, float* outFlux, float* outSat);
+/* Sets the background of a path tracer (world_set_env_light +
+ * the light-tree background of LightTreeBuilder, light_tree.cpp:592-597) and
+ * resets it. The default background is COLORED black. */
+int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* bg);
 
 /* Event timing on the context's stream: begin records an event, end records
  * a second one, waits for it and returns the elapsed milliseconds. */

@@ -110,6 +110,11 @@ int mfh_destroy(mfh_context* ctx) {
             delete s;
         }
         destroy_tracers(c);
+        for (Texture* t : c.textures) {
+            if (!t) continue;
+            t->dev.release();
+            delete t;
+        }
         BuildScratch& b = c.scratch;
         for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.ranges, &b.pending, &b.boxes, &b.counters,
                           &b.flags, &b.offsets, &b.hist, &b.partials, &b.misc})

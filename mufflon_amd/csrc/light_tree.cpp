@@ -354,4 +354,60 @@ void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vect
     }
 }
 
+void build_background(int32_t type, const float color[3], float scale, const float* envRgba, int32_t width,
+                      int32_t height, const float sceneBox[6], BackgroundHost& out) {
+    out = BackgroundHost{};
+    out.type = type;
+    for (int k = 0; k < 3; ++k) out.color[k] = color[k];
+    out.scale = scale;
+    const V3 diag{sceneBox[3] - sceneBox[0], sceneBox[4] - sceneBox[1], sceneBox[5] - sceneBox[2]};
+    if (type == 0) {
+        // compute_constant_flux (background.cpp:125-135); the COLORED descriptor
+        // multiplies by the color only (:36-40)
+        const float solidAngle = 4 * kPi;
+        const float area = 2 * kPi * (diag.x * diag.y + diag.x * diag.z + diag.y * diag.z);
+        const float f = solidAngle * area;
+        for (int k = 0; k < 3; ++k) out.flux[k] = f * color[k];
+    } else {
+        out.width = width;
+        out.height = height;
+        out.sat.assign((size_t)width * height, 0.0f);
+        auto read = [&](int x, int y) { return envRgba + 4 * ((size_t)y * width + x); };
+        // create_summed_area_table, polar map (texture_sampling.hpp:84-101)
+        const float lum[3] = {0.212671f, 0.715160f, 0.072169f};
+        float accumLumY = 0.f;
+        for (int y = 0; y < height; ++y) {
+            const float sinTheta = std::sin(kPi * static_cast<float>(y + 0.5f) / static_cast<float>(height));
+            float accumLumX = 0.f;
+            for (int x = 0; x < width; ++x) {
+                const float* t = read(x, y);
+                const float luminance = lum[0] * t[0] + lum[1] * t[1] + lum[2] * t[2] + 0.0f * t[3];
+                accumLumX += luminance * sinTheta;
+                out.sat[(size_t)y * width + x] = accumLumX;
+            }
+            accumLumY += accumLumX;
+            out.sat[(size_t)y * width + width - 1] = accumLumY;
+        }
+        // compute_envmap_flux, polar map (background.cpp:78-122); the ENVMAP
+        // descriptor scales the flux once more (:46-54)
+        const float avgPixelSolidAngle = 4 * kPi / (float)(width * height);
+        float flux[3] = {0.0f, 0.0f, 0.0f};
+        for (int y = 0; y < height; ++y) {
+            const float sinTheta = std::sin(kPi * (y + 0.5f) / float(height));
+            for (int x = 0; x < width; ++x) {
+                const float* t = read(x, y);
+                const float phi = 2 * kPi * (x + 0.5f) / float(width);
+                const V3 dir{sinTheta * (float)std::cos((double)phi), std::sqrt(1.0f - sinTheta * sinTheta),
+                             sinTheta * (float)std::sin((double)phi)};
+                const float surface = diag.y * diag.z * std::fabs(dir.x) + diag.x * diag.z * std::fabs(dir.y) +
+                                      diag.x * diag.y * std::fabs(dir.z);
+                const float w = avgPixelSolidAngle / (2.0f * kPi * kPi * sinTheta);
+                for (int k = 0; k < 3; ++k) flux[k] += (t[k] * scale) * w * surface;
+            }
+        }
+        for (int k = 0; k < 3; ++k) out.flux[k] = flux[k] * scale;
+    }
+    out.fluxSum = out.flux[0] + out.flux[1] + out.flux[2];
+}
+
 }  // namespace mfh

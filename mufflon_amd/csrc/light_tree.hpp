@@ -53,6 +53,25 @@ struct LightTreeHost {
     std::vector<uint32_t> mapCodes;
 };
 
+// Background of the light tree (BackgroundDesc, background.hpp): COLORED
+// (type 0) or a polar ENVMAP (type 1) with its summed-area table.
+struct BackgroundHost {
+    int32_t type = 0;
+    float color[3] = {0.0f, 0.0f, 0.0f};
+    float scale = 1.0f;
+    int32_t envmapTexture = -1;
+    int32_t width = 0, height = 0;
+    std::vector<float> sat;  // width * height, R32F
+    float flux[3] = {0.0f, 0.0f, 0.0f};
+    float fluxSum = 0.0f;    // ei::sum(flux), the background's weight in the tree
+};
+
+// Background::acquire_const (background.cpp:30-66): flux for the scene box and,
+// for ENVMAP, create_summed_area_table (texture_sampling.hpp:49-110).
+// envRgba: 4 floats per texel (NULL for COLORED).
+void build_background(int32_t type, const float color[3], float scale, const float* envRgba, int32_t width,
+                      int32_t height, const float sceneBox[6], BackgroundHost& out);
+
 void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vector<HostPointLight>& points,
                       const float sceneBox[6], const EmissionLookup& emission, LightTreeHost& out);
 

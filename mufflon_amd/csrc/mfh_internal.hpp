@@ -119,6 +119,14 @@ struct Scene {
 
 struct PathTracer;
 
+// A texture of the context: RGBA32F texels (missing channels are 0), row-major,
+// kept on the host too (background flux and summed-area table are host-built).
+struct Texture {
+    int32_t width = 0, height = 0, channels = 0, sampling = 0;
+    std::vector<float> host;  // 4 * width * height
+    DevBuf dev;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -126,6 +134,7 @@ struct Context {
     std::vector<Lod*> lods;
     std::vector<Scene*> scenes;
     std::vector<PathTracer*> tracers;
+    std::vector<Texture*> textures;
     BuildScratch scratch;
     int lastBuilt = -1;
 };

@@ -12,22 +12,54 @@ constexpr float kSceneEps = 1e-4f;          // intersection.cu:16
 constexpr float kInfPdf = 4294967296.f;      // AreaPdf::infinite() (util/types.hpp:52-54)
 constexpr float kDistSqEps = 1e-10f;        // light_sampling.hpp:78
 
-constexpr int kMatEmissive = 0;  // Materials enum (material_definitions.hpp:25-27)
+constexpr int kMatEmissive = 0;  // Materials enum (material_definitions.hpp:25-39)
 constexpr int kMatLambert = 1;
+constexpr int kMatTorrance = 3;
+constexpr int kMatMicrofacet = 10;
+constexpr int kNdfGgx = 0;
+constexpr int kNdfBeckmann = 1;
+constexpr int kShadowingVCavity = 0;
+constexpr int kShadowingSmith = 1;
 
 constexpr uint16_t kLightPoint = 0;
 constexpr uint16_t kLightAreaTriangle = 2;
 constexpr uint16_t kLightInternalNode = 0xFFFF;
 
-// mfh_material of include/mufflon_hip.h (64 bytes)
+// mfh_material of include/mufflon_hip.h (128 bytes)
 struct PtMaterialRecord {
     int32_t type;
     float albedo[3];
     float radiance[3];
-    float scale;
-    int32_t reserved[8];
+    float scale[3];
+    float roughness[2];
+    int32_t ndf;
+    int32_t shadowing;
+    float ior;
+    float absorption[3];
+    float outerIor;
+    float outerAbsorption[3];
+    int32_t albedoTexture;
+    int32_t roughnessTexture;
+    int32_t emissionTexture;
+    int32_t reserved[7];
 };
-static_assert(sizeof(PtMaterialRecord) == 64, "material record");
+static_assert(sizeof(PtMaterialRecord) == 128, "material record");
+
+// A texture as the kernels read it: RGBA32F texels, row-major.
+struct DevTexture {
+    const float4* texels;
+    int32_t width, height, channels, sampling;
+    int32_t pad[2];
+};
+
+// Background (BackgroundDesc): COLORED (0) or polar ENVMAP (1).
+struct DevBackground {
+    int32_t type;
+    float color[3];
+    float scale;
+    int32_t envmapTexture;
+    const float* sat;  // width * height summed-area table (ENVMAP)
+};
 
 // Object-space shading attributes of one LoD.
 struct DevPtLod {
@@ -42,6 +74,9 @@ struct PtScene {
     TraceParams trace;
     const DevPtLod* ptLods;
     const PtMaterialRecord* materials;
+    const DevTexture* textures;
+    DevBackground background;
+    float sceneBox[6];
     // light tree (Mufflon layout, positional subtree)
     const uint8_t* lightMem;
     float posRootFlux;

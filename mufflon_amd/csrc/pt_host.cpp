@@ -16,7 +16,8 @@ struct PathTracer {
     mfh_pinhole camera{};
     mfh_pt_params params{};
     LightTreeHost tree;
-    DevBuf dMaterials, dPtLods, dLightMem, dMapKeys, dMapCodes;
+    BackgroundHost background;
+    DevBuf dMaterials, dPtLods, dLightMem, dMapKeys, dMapCodes, dTextures, dSat;
     DevBuf dRngs, dRadiance, dHitId, dNan;
     int32_t iteration = -1;
     uint64_t sceneGeneration = ~0ull;
@@ -25,8 +26,8 @@ struct PathTracer {
 void destroy_tracers(Context& c) {
     for (PathTracer* t : c.tracers) {
         if (!t) continue;
-        for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dRngs,
-                          &t->dRadiance, &t->dHitId, &t->dNan})
+        for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dTextures,
+                          &t->dSat, &t->dRngs, &t->dRadiance, &t->dHitId, &t->dNan})
             b->release();
         delete t;
     }
@@ -63,9 +64,30 @@ struct MaterialEmission : EmissionLookup {
     explicit MaterialEmission(const std::vector<PtMaterialRecord>& m) : mats(m) {}
     void radiance(uint16_t material, float, float, float out[3]) const override {
         const PtMaterialRecord& m = mats[material];
-        for (int k = 0; k < 3; ++k) out[k] = m.type == kMatEmissive ? m.radiance[k] * m.scale : 0.0f;
+        for (int k = 0; k < 3; ++k) out[k] = m.type == kMatEmissive ? m.radiance[k] * m.scale[k] : 0.0f;
     }
 };
+
+void validate_material(const Context& c, const mfh_material& m) {
+    const int32_t t = m.type;
+    MFH_REQUIRE(t == MFH_MATERIAL_EMISSIVE || t == MFH_MATERIAL_LAMBERT || t == MFH_MATERIAL_TORRANCE ||
+                    t == MFH_MATERIAL_MICROFACET,
+                "unsupported material type " + std::to_string(t));
+    auto tex_ok = [&](int32_t id) {
+        return id == -1 || (id >= 0 && id < (int32_t)c.textures.size() && c.textures[id] != nullptr);
+    };
+    MFH_REQUIRE(tex_ok(m.albedoTexture) && tex_ok(m.roughnessTexture), "unknown texture id in a material");
+    MFH_REQUIRE(m.emissionTexture == -1, "emission textures are not supported");
+    if (t == MFH_MATERIAL_TORRANCE || t == MFH_MATERIAL_MICROFACET) {
+        MFH_REQUIRE(m.ndf == MFH_NDF_GGX || m.ndf == MFH_NDF_BECKMANN, "unknown NDF");
+        MFH_REQUIRE(m.shadowing == MFH_SHADOWING_VCAVITY || m.shadowing == MFH_SHADOWING_SMITH,
+                    "unknown shadowing model");
+        MFH_REQUIRE(!(m.ndf == MFH_NDF_BECKMANN && m.shadowing == MFH_SHADOWING_SMITH),
+                    "Smith shadowing with the Beckmann NDF is not supported");
+    }
+    if (t == MFH_MATERIAL_MICROFACET) MFH_REQUIRE(m.refractionIndex > 0.0f, "refraction index must be positive");
+    MFH_REQUIRE(m.outerRefractionIndex > 0.0f, "outer refraction index must be positive");
+}
 
 PathTracer& tracer_of(Context& c, int32_t id) {
     MFH_REQUIRE(id >= 0 && id < (int32_t)c.tracers.size() && c.tracers[id], "unknown path tracer id");
@@ -132,16 +154,34 @@ PtScene device_scene(Context& c, PathTracer& t) {
         lods[i].matIdx = l->matIdx.as<uint16_t>();
     }
     upload(c, t.dPtLods, lods.data(), sizeof(DevPtLod) * lods.size());
+    std::vector<DevTexture> texs(std::max<size_t>(c.textures.size(), 1));
+    std::memset(texs.data(), 0, sizeof(DevTexture) * texs.size());
+    for (size_t i = 0; i < c.textures.size(); ++i) {
+        const Texture* x = c.textures[i];
+        texs[i].texels = x->dev.as<float4>();
+        texs[i].width = x->width;
+        texs[i].height = x->height;
+        texs[i].channels = x->channels;
+        texs[i].sampling = x->sampling;
+    }
+    upload(c, t.dTextures, texs.data(), sizeof(DevTexture) * texs.size());
     PtScene S;
     std::memset(&S, 0, sizeof(S));
     S.trace = trace_params(sc);
     S.ptLods = t.dPtLods.as<DevPtLod>();
     S.materials = t.dMaterials.as<PtMaterialRecord>();
+    S.textures = t.dTextures.as<DevTexture>();
+    S.background.type = t.background.type;
+    for (int k = 0; k < 3; ++k) S.background.color[k] = t.background.color[k];
+    S.background.scale = t.background.scale;
+    S.background.envmapTexture = t.background.envmapTexture;
+    S.background.sat = t.dSat.as<float>();
+    std::memcpy(S.sceneBox, sc.box, sizeof(S.sceneBox));
     S.lightMem = t.dLightMem.as<uint8_t>();
     S.posRootFlux = t.tree.posRootFlux;
     S.posRootType = t.tree.posRootType;
     S.dirRootFlux = t.tree.dirRootFlux;
-    S.bgFlux = t.tree.backgroundFlux;
+    S.bgFlux = t.background.fluxSum;
     S.mapKeys = t.dMapKeys.as<uint64_t>();
     S.mapCodes = t.dMapCodes.as<uint32_t>();
     S.mapCount = (int32_t)t.tree.mapKeys.size();
@@ -205,9 +245,7 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
         MFH_REQUIRE(camera->width > 0 && camera->height > 0 && camera->width < 65536 && camera->height < 65536,
                     "camera resolution out of range");
         MFH_REQUIRE(params->maxPathLength >= 0 && params->neeCount >= 0, "invalid path tracer parameters");
-        for (int32_t i = 0; i < numMaterials; ++i)
-            MFH_REQUIRE(materials[i].type == MFH_MATERIAL_EMISSIVE || materials[i].type == MFH_MATERIAL_LAMBERT,
-                        "unsupported material type " + std::to_string(materials[i].type));
+        for (int32_t i = 0; i < numMaterials; ++i) validate_material(c, materials[i]);
         for (int32_t li : sc.instLod) {
             const Lod* l = c.lods[li];
             MFH_REQUIRE(l->hasShading, "a LoD of the scene has no shading attributes (mfh_lod_set_shading)");
@@ -227,8 +265,11 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
                 std::memcpy(pl[i].intensity, pointLights[i].intensity, 12);
             }
             const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t->materials);
-            MFH_REQUIRE(!area.empty() || !pl.empty(), "scene without lights");
             build_light_tree(area, pl, sc.box, MaterialEmission(t->materials), t->tree);
+            const float black[3] = {0.0f, 0.0f, 0.0f};
+            build_background(MFH_BACKGROUND_COLORED, black, 1.0f, nullptr, 0, 0, sc.box, t->background);
+            t->tree.backgroundFlux = t->background.fluxSum;
+            t->dSat.reserve(16);
             upload(c, t->dMaterials, t->materials.data(), sizeof(PtMaterialRecord) * t->materials.size());
             upload(c, t->dLightMem, t->tree.mem.data(), std::max<size_t>(t->tree.mem.size(), 16));
             const size_t nm = t->tree.mapKeys.size();
@@ -246,8 +287,8 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
             reset(c, *t);
             MFH_HIP(hipStreamSynchronize(c.stream));
         } catch (...) {
-            for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dRngs,
-                              &t->dRadiance, &t->dHitId, &t->dNan})
+            for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dTextures,
+                              &t->dSat, &t->dRngs, &t->dRadiance, &t->dHitId, &t->dNan})
                 b->release();
             delete t;
             throw;
@@ -269,6 +310,7 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations) {
         Context& c = ctx_of(ctx);
         PathTracer& t = tracer_of(c, ptId);
         MFH_REQUIRE(iterations >= 0, "negative iteration count");
+        MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "scene without lights");
         PtScene S = device_scene(c, t);
         for (int32_t k = 0; k < iterations; ++k) {
             S.iteration = ++t.iteration;
@@ -328,6 +370,82 @@ int mfh_pt_get_light_tree(mfh_context* ctx, int32_t ptId, mfh_light_tree_info* o
         if (outMem && !t.tree.mem.empty()) std::memcpy(outMem, t.tree.mem.data(), t.tree.mem.size());
         if (outMapKeys) std::memcpy(outMapKeys, t.tree.mapKeys.data(), 8 * t.tree.mapKeys.size());
         if (outMapCodes) std::memcpy(outMapCodes, t.tree.mapCodes.data(), 4 * t.tree.mapCodes.size());
+    });
+}
+
+int mfh_texture_create(mfh_context* ctx, int32_t width, int32_t height, int32_t channels, int32_t samplingMode,
+                       const float* texels, int32_t* outTextureId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(texels && outTextureId, "null pointer");
+        MFH_REQUIRE(width > 0 && height > 0 && (int64_t)width * height <= (int64_t)1 << 28, "texture size out of range");
+        MFH_REQUIRE(channels >= 1 && channels <= 4, "texture channels must be 1..4");
+        MFH_REQUIRE(samplingMode == MFH_SAMPLING_NEAREST || samplingMode == MFH_SAMPLING_LINEAR, "unknown sampling mode");
+        auto* t = new Texture();
+        try {
+            t->width = width;
+            t->height = height;
+            t->channels = channels;
+            t->sampling = samplingMode;
+            const size_t n = (size_t)width * height;
+            t->host.assign(4 * n, 0.0f);
+            for (size_t i = 0; i < n; ++i)
+                for (int k = 0; k < channels; ++k) t->host[4 * i + k] = texels[(size_t)channels * i + k];
+            upload(c, t->dev, t->host.data(), 16 * n);
+            MFH_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            t->dev.release();
+            delete t;
+            throw;
+        }
+        c.textures.push_back(t);
+        *outTextureId = (int32_t)c.textures.size() - 1;
+    });
+}
+
+int mfh_background_build(const mfh_background* bg, const float* envTexels, int32_t width, int32_t height,
+                         int32_t channels, const float* sceneBox, float* outFlux, float* outSat) {
+    return guarded([&] {
+        MFH_REQUIRE(bg && sceneBox && outFlux, "null pointer");
+        MFH_REQUIRE(bg->type == MFH_BACKGROUND_COLORED || bg->type == MFH_BACKGROUND_ENVMAP, "unknown background type");
+        BackgroundHost out;
+        if (bg->type == MFH_BACKGROUND_ENVMAP) {
+            MFH_REQUIRE(envTexels && outSat, "an environment map needs texels and a SAT output");
+            MFH_REQUIRE(width > 0 && height > 0 && channels >= 1 && channels <= 4, "invalid environment map");
+            std::vector<float> rgba(4 * (size_t)width * height, 0.0f);
+            for (size_t i = 0; i < (size_t)width * height; ++i)
+                for (int k = 0; k < channels; ++k) rgba[4 * i + k] = envTexels[(size_t)channels * i + k];
+            build_background(bg->type, bg->color, bg->scale, rgba.data(), width, height, sceneBox, out);
+            std::memcpy(outSat, out.sat.data(), 4 * out.sat.size());
+        } else {
+            build_background(bg->type, bg->color, bg->scale, nullptr, 0, 0, sceneBox, out);
+        }
+        for (int k = 0; k < 3; ++k) outFlux[k] = out.flux[k];
+    });
+}
+
+int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* bg) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(bg != nullptr, "null background");
+        MFH_REQUIRE(bg->type == MFH_BACKGROUND_COLORED || bg->type == MFH_BACKGROUND_ENVMAP, "unknown background type");
+        Scene& sc = scene_of(c, t.scene);
+        refresh_scene(c, sc);
+        BackgroundHost out;
+        if (bg->type == MFH_BACKGROUND_ENVMAP) {
+            const int32_t id = bg->envmapTexture;
+            MFH_REQUIRE(id >= 0 && id < (int32_t)c.textures.size() && c.textures[id], "unknown environment map texture");
+            const Texture& x = *c.textures[id];
+            build_background(bg->type, bg->color, bg->scale, x.host.data(), x.width, x.height, sc.box, out);
+            out.envmapTexture = id;
+            upload(c, t.dSat, out.sat.data(), 4 * out.sat.size());
+        } else {
+            build_background(bg->type, bg->color, bg->scale, nullptr, 0, 0, sc.box, out);
+        }
+        t.background = out;
+        t.tree.backgroundFlux = out.fluxSum;
+        reset(c, t);
     });
 }
 

@@ -20,8 +20,17 @@ u64p = C.POINTER(C.c_uint64)
 vp = C.c_void_p
 
 # C-ABI record layouts (include/mufflon_hip.h)
-MATERIAL_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("radiance", "<f4", 3), ("scale", "<f4"),
-                           ("reserved", "<i4", 8)])
+MATERIAL_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("radiance", "<f4", 3), ("scale", "<f4", 3),
+                           ("roughness", "<f4", 2), ("ndf", "<i4"), ("shadowing", "<i4"),
+                           ("refractionIndex", "<f4"), ("absorption", "<f4", 3), ("outerRefractionIndex", "<f4"),
+                           ("outerAbsorption", "<f4", 3), ("albedoTexture", "<i4"), ("roughnessTexture", "<i4"),
+                           ("emissionTexture", "<i4"), ("reserved", "<i4", 7)])
+assert MATERIAL_DTYPE.itemsize == 128
+BACKGROUND_DTYPE = np.dtype([("type", "<i4"), ("color", "<f4", 3), ("scale", "<f4"), ("envmapTexture", "<i4")])
+BACKGROUND_COLORED = 0
+BACKGROUND_ENVMAP = 1
+SAMPLING_NEAREST = 0
+SAMPLING_LINEAR = 1
 PINHOLE_DTYPE = np.dtype([("position", "<f4", 3), ("tanVFov", "<f4"), ("viewDir", "<f4", 3), ("up", "<f4", 3),
                           ("width", "<i4"), ("height", "<i4")])
 PT_PARAMS_DTYPE = np.dtype([("seed", "<i4"), ("minPathLength", "<i4"), ("maxPathLength", "<i4"),
@@ -62,6 +71,9 @@ _SIGS = {
     "mfh_pt_get_nan_count": [vp, i32, u32p],
     "mfh_pt_get_light_tree": [vp, i32, vp, vp, u64p, u32p],
     "mfh_light_tree_build": [vp, i32, vp, i32, f32p, vp, i32, vp, vp, C.c_size_t, u64p, u32p, i32],
+    "mfh_texture_create": [vp, i32, i32, i32, i32, f32p, i32p],
+    "mfh_background_build": [vp, f32p, i32, i32, i32, f32p, f32p, f32p],
+    "mfh_pt_set_background": [vp, i32, vp],
     "mfh_timer_begin": [vp],
     "mfh_timer_end": [vp, f32p],
 }
@@ -105,6 +117,38 @@ def params_record(seed=0, min_path_length=0, max_path_length=16, nee_count=1):
     r["seed"], r["minPathLength"], r["maxPathLength"], r["neeCount"] = seed, min_path_length, max_path_length, \
         nee_count
     return r
+
+
+def background_record(kind=BACKGROUND_COLORED, color=(0.0, 0.0, 0.0), scale=1.0, envmap_texture=-1):
+    r = np.zeros(1, BACKGROUND_DTYPE)
+    r["type"] = kind
+    r["color"] = color
+    r["scale"] = scale
+    r["envmapTexture"] = envmap_texture
+    return r
+
+
+def background_build(background, env_texels, scene_box):
+    """Host-only background preparation: (flux[3], sat (h, w) or None).
+    env_texels: (h, w, c) float32 for an ENVMAP background, else None."""
+    lib = load()
+    bg = background if isinstance(background, np.ndarray) else background_record(**background)
+    box = np.ascontiguousarray(scene_box, np.float32)
+    flux = np.zeros(3, np.float32)
+    if int(bg["type"][0]) == BACKGROUND_ENVMAP:
+        t = np.ascontiguousarray(env_texels, np.float32)
+        if t.ndim == 2:
+            t = t[:, :, None]
+        h, w, c = t.shape
+        sat = np.zeros((h, w), np.float32)
+        rc = lib.mfh_background_build(bg.ctypes.data, _ptr(t, f32p), w, h, c, _ptr(box, f32p), _ptr(flux, f32p),
+                                      _ptr(sat, f32p))
+    else:
+        sat = None
+        rc = lib.mfh_background_build(bg.ctypes.data, None, 0, 0, 0, _ptr(box, f32p), _ptr(flux, f32p), None)
+    if rc != 0:
+        raise HipError(rc, lib.mfh_last_error().decode())
+    return flux, sat
 
 
 def light_tree_build(area_lights, point_lights, scene_box, materials):
@@ -322,6 +366,20 @@ class HipContext:
         uv = np.ascontiguousarray(uvs, np.float32)
         m = np.ascontiguousarray(mat_idx, np.uint16)
         self._call("mfh_lod_set_shading", self.handle, lod, _ptr(n, f32p), _ptr(uv, f32p), m.ctypes.data)
+
+    def texture_create(self, texels, sampling=SAMPLING_LINEAR):
+        """Texture from (h, w) or (h, w, c) float32 texels; returns its id."""
+        t = np.ascontiguousarray(texels, np.float32)
+        if t.ndim == 2:
+            t = t[:, :, None]
+        h, w, c = t.shape
+        out = i32()
+        self._call("mfh_texture_create", self.handle, w, h, c, sampling, _ptr(t, f32p), C.byref(out))
+        return out.value
+
+    def pt_set_background(self, pt, background):
+        bg = background if isinstance(background, np.ndarray) else background_record(**background)
+        self._call("mfh_pt_set_background", self.handle, pt, bg.ctypes.data)
 
     def pt_create(self, scene, materials, camera, params, point_lights=None):
         mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)

@@ -119,18 +119,38 @@ IDENTITY34 = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
 
 
 # ---------------------------------------------------------------- Cornell box
-MAT_EMISSIVE = 0  # Mufflon's Materials enum (material_definitions.hpp:25-27)
+MAT_EMISSIVE = 0  # Mufflon's Materials enum (material_definitions.hpp:25-39)
 MAT_LAMBERT = 1
+MAT_TORRANCE = 3
+MAT_MICROFACET = 10
+NDF_GGX = 0
+NDF_BECKMANN = 1
+SHADOWING_VCAVITY = 0
+SHADOWING_SMITH = 1
 
-from mufflon_amd.hip import MATERIAL_DTYPE, AREA_LIGHT_DTYPE  # noqa: E402
+from mufflon_amd.hip import MATERIAL_DTYPE, AREA_LIGHT_DTYPE, POINT_LIGHT_DTYPE  # noqa: E402
+from mufflon_amd.hip import SAMPLING_LINEAR, SAMPLING_NEAREST  # noqa: E402
 
 
-def material(kind, albedo=(0, 0, 0), radiance=(0, 0, 0), scale=1.0):
+def material(kind, albedo=(0, 0, 0), radiance=(0, 0, 0), scale=1.0, roughness=(0.0, 0.0), ndf=NDF_GGX,
+             shadowing=SHADOWING_VCAVITY, ior=1.0, absorption=(0, 0, 0), outer_ior=1.0, outer_absorption=(0, 0, 0),
+             albedo_texture=-1, roughness_texture=-1):
+    """One mfh_material record; roughness may be a scalar (isotropic)."""
     m = np.zeros(1, MATERIAL_DTYPE)
     m["type"] = kind
     m["albedo"] = albedo
     m["radiance"] = radiance
-    m["scale"] = scale
+    m["scale"] = np.broadcast_to(np.asarray(scale, np.float32), (3,))
+    m["roughness"] = np.broadcast_to(np.asarray(roughness, np.float32), (2,))
+    m["ndf"] = ndf
+    m["shadowing"] = shadowing
+    m["refractionIndex"] = ior
+    m["absorption"] = absorption
+    m["outerRefractionIndex"] = outer_ior
+    m["outerAbsorption"] = outer_absorption
+    m["albedoTexture"] = albedo_texture
+    m["roughnessTexture"] = roughness_texture
+    m["emissionTexture"] = -1
     return m
 
 
@@ -234,6 +254,73 @@ def cornell_box(sphere_rings=128, sphere_segments=256, width=256, height=256):
                up=np.array([0, 1, 0], np.float32), tan_vfov=np.float32(np.tan(vfov / 2)), width=width,
                height=height)
     return dict(verts=v, normals=n, uvs=uv, tris=t, mat_idx=m, materials=mats, camera=cam)
+
+
+def procedural_texture(h, w, channels, lo, hi, seed, checker=8):
+    """Checker pattern modulated by noise, values in [lo, hi], (h, w, c) float32."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    chk = (((x * checker) // w + (y * checker) // h) % 2).astype(np.float64)
+    t = 0.6 * chk[:, :, None] + 0.4 * rng.random((h, w, channels))
+    return (lo + (hi - lo) * t).astype(np.float32)
+
+
+def procedural_envmap(h=32, w=64, seed=5, sun=(0.35, 0.30), sun_radiance=60.0):
+    """Polar environment map: sky gradient plus a bright sun disc at (u, v)."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    V, U = np.meshgrid(v, u, indexing="ij")
+    sky = np.stack([0.2 + 0.5 * V, 0.3 + 0.5 * V, 0.6 + 0.4 * V], -1) * (0.9 + 0.2 * rng.random((h, w, 1)))
+    d2 = ((U - sun[0]) * 2) ** 2 + (V - sun[1]) ** 2
+    sky[d2 < 0.004] = sun_radiance
+    return sky.astype(np.float32)
+
+
+def cornell_materials(sphere_rings=32, sphere_segments=64, width=64, height=64):
+    """Cornell box with every supported material kind: textured Lambert back
+    wall (texture 0), a GGX V-cavity Torrance sphere with a roughness texture
+    (texture 1), an anisotropic GGX Smith Torrance box, an absorbing
+    dielectric MICROFACET box (ior 1.5), a Beckmann Torrance sphere, the
+    ceiling light, and two point lights. Textures are returned as
+    (texels, sampling) in id order; the environment map is texture 2."""
+    WHITE, RED, GREEN, LIGHT, BACK, ROUGH_SPHERE, ANISO, GLASS, BECK = 1, 2, 3, 0, 4, 5, 6, 7, 8
+    mats = np.concatenate([
+        material(MAT_EMISSIVE, radiance=(17.0, 12.0, 4.0), scale=(1.0, 1.0, 1.0)),
+        material(MAT_LAMBERT, albedo=(0.73, 0.73, 0.73)),
+        material(MAT_LAMBERT, albedo=(0.65, 0.05, 0.05)),
+        material(MAT_LAMBERT, albedo=(0.12, 0.45, 0.15)),
+        material(MAT_LAMBERT, albedo_texture=0),
+        material(MAT_TORRANCE, albedo=(0.9, 0.8, 0.6), roughness_texture=1),
+        material(MAT_TORRANCE, albedo=(0.7, 0.75, 0.8), roughness=(0.08, 0.4), shadowing=SHADOWING_SMITH),
+        material(MAT_MICROFACET, roughness=0.15, ior=1.5, absorption=(0.4, 0.1, 0.05)),
+        material(MAT_TORRANCE, albedo=(0.95, 0.6, 0.3), roughness=0.3, ndf=NDF_BECKMANN),
+    ])
+    mb = MeshBuilder()
+    mb.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0), WHITE)
+    mb.add_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0), WHITE)
+    mb.add_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1), BACK)
+    mb.add_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0), RED)
+    mb.add_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0), GREEN)
+    mb.add_box((-0.40, 0.0, -0.35), (0.55, 1.2, 0.55), 18.0, ANISO)
+    mb.add_box((0.45, 0.0, -0.40), (0.50, 0.5, 0.50), -17.0, GLASS)
+    mb.add_uv_sphere((0.05, 0.32, 0.45), 0.32, sphere_rings, sphere_segments, ROUGH_SPHERE)
+    mb.add_uv_sphere((-0.6, 1.45, 0.2), 0.18, sphere_rings // 2, sphere_segments // 2, BECK)
+    L = 0.25
+    mb.add_quad((-L, 1.99, -L), (L, 1.99, -L), (L, 1.99, L), (-L, 1.99, L), (0, -1, 0), LIGHT)
+    v, n, uv, t, m = mb.arrays()
+    vfov = np.radians(40.0)
+    cam = dict(position=np.array([0.0, 1.0, 3.8], np.float32), view_dir=np.array([0, 0, -1], np.float32),
+               up=np.array([0, 1, 0], np.float32), tan_vfov=np.float32(np.tan(vfov / 2)), width=width,
+               height=height)
+    textures = [(procedural_texture(64, 64, 3, 0.1, 0.9, seed=21), SAMPLING_LINEAR),
+                (procedural_texture(32, 32, 1, 0.05, 0.8, seed=22), SAMPLING_NEAREST),
+                (procedural_envmap(), SAMPLING_LINEAR)]
+    points = np.zeros(2, POINT_LIGHT_DTYPE)
+    points["position"] = [(0.6, 1.7, 0.6), (-0.7, 0.4, 0.8)]
+    points["intensity"] = [(0.8, 0.6, 0.3), (0.2, 0.3, 0.6)]
+    return dict(verts=v, normals=n, uvs=uv, tris=t, mat_idx=m, materials=mats, camera=cam, textures=textures,
+                point_lights=points, envmap=2)
 
 
 def emissive_area_lights(verts, uvs, tris, mat_idx, materials, instance=0):

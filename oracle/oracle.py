@@ -66,6 +66,8 @@ def load():
 
 
 def _p(a, t):
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("oracle arrays must be C-contiguous")
     return a.ctypes.data_as(t)
 
 
@@ -229,6 +231,16 @@ def trace_brute(scene, rays):
     return t, inst, prim
 
 
+class OrcTexture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("channels", C.c_int32), ("sampling", C.c_int32),
+                ("texels", f32p)]
+
+
+class OrcBackground(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("scale", C.c_float), ("envmapTexture", C.c_int32),
+                ("sat", f32p), ("flux", C.c_float * 3)]
+
+
 class OrcPtScene(C.Structure):
     _fields_ = [
         ("geo", OrcScene),
@@ -239,7 +251,123 @@ class OrcPtScene(C.Structure):
         ("bgFlux", C.c_float), ("mapKeys", u64p), ("mapCodes", u32p), ("mapCount", C.c_int32),
         ("sceneBox", C.c_float * 6), ("camPos", C.c_float * 3), ("tanVFov", C.c_float),
         ("camDir", C.c_float * 3), ("camUp", C.c_float * 3), ("width", C.c_int32), ("height", C.c_int32),
+        ("textures", C.POINTER(OrcTexture)), ("numTextures", C.c_int32), ("background", OrcBackground),
     ]
+
+
+class Textures:
+    """Texture table in the oracle layout: each entry (texels (h, w, c) float32,
+    sampling 0 nearest / 1 linear) becomes RGBA float texels."""
+
+    def __init__(self, textures=()):
+        self.data = []
+        self.arr = (OrcTexture * max(len(textures), 1))()
+        for i, (texels, sampling) in enumerate(textures):
+            t = np.asarray(texels, np.float32)
+            if t.ndim == 2:
+                t = t[:, :, None]
+            h, w, c = t.shape
+            rgba = np.zeros((h, w, 4), np.float32)
+            rgba[:, :, :c] = t
+            self.data.append(rgba)
+            self.arr[i].width, self.arr[i].height, self.arr[i].channels = w, h, c
+            self.arr[i].sampling = int(sampling)
+            self.arr[i].texels = _p(rgba, f32p)
+        self.count = len(textures)
+
+
+def texture_sample(textures, index, u, v):
+    lib = load()
+    out = np.zeros(4, np.float32)
+    lib.orc_texture_sample.argtypes = [C.POINTER(OrcTexture), C.c_float, C.c_float, f32p]
+    lib.orc_texture_sample(C.byref(textures.arr[index]), u, v, _p(out, f32p))
+    return out
+
+
+def _bsdf_lib():
+    lib = load()
+    if not hasattr(lib, "_bsdf_ready"):
+        lib.orc_bsdf_sample.argtypes = [C.c_void_p, C.POINTER(OrcTexture), C.c_float, C.c_float, f32p, C.c_float,
+                                        C.c_float, C.c_uint64, f32p]
+        lib.orc_bsdf_evaluate.argtypes = [C.c_void_p, C.POINTER(OrcTexture), C.c_float, C.c_float, f32p, f32p, f32p]
+        lib.orc_eval_ndf.argtypes = [C.c_int32, C.c_float, C.c_float, f32p]
+        lib.orc_eval_ndf.restype = C.c_float
+        lib.orc_fresnel_dielectric.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
+        lib.orc_fresnel_dielectric.restype = C.c_float
+        lib.orc_background_sample.argtypes = [C.POINTER(OrcPtScene), C.c_float, C.c_float, f32p]
+        lib.orc_background_evaluate.argtypes = [C.POINTER(OrcPtScene), f32p, f32p]
+        lib._bsdf_ready = True
+    return lib
+
+
+def bsdf_sample(material, in_ts, u0, u1, i0, textures=None, uv=(0.0, 0.0)):
+    """Tangent-space BSDF sample: (valid, throughput[3], excident[3], pdf_forw, pdf_back)."""
+    lib = _bsdf_lib()
+    m = np.ascontiguousarray(material)
+    tx = textures or Textures()
+    out = np.zeros(9, np.float32)
+    lib.orc_bsdf_sample(m.ctypes.data, tx.arr, uv[0], uv[1], _p(np.ascontiguousarray(in_ts, np.float32), f32p), u0, u1,
+                        int(i0), _p(out, f32p))
+    return bool(out[0]), out[1:4].copy(), out[4:7].copy(), float(out[7]), float(out[8])
+
+
+def bsdf_evaluate(material, in_ts, ex_ts, textures=None, uv=(0.0, 0.0)):
+    """Tangent-space BSDF value[3], pdf_forw, pdf_back."""
+    lib = _bsdf_lib()
+    m = np.ascontiguousarray(material)
+    tx = textures or Textures()
+    out = np.zeros(5, np.float32)
+    lib.orc_bsdf_evaluate(m.ctypes.data, tx.arr, uv[0], uv[1], _p(np.ascontiguousarray(in_ts, np.float32), f32p),
+                          _p(np.ascontiguousarray(ex_ts, np.float32), f32p), _p(out, f32p))
+    return out[:3].copy(), float(out[3]), float(out[4])
+
+
+def eval_ndf(ndf, rx, ry, h):
+    return _bsdf_lib().orc_eval_ndf(ndf, rx, ry, _p(np.ascontiguousarray(h, np.float32), f32p))
+
+
+def fresnel_dielectric(n_i, n_t, cos_i):
+    ct = np.zeros(1, np.float32)
+    f = _bsdf_lib().orc_fresnel_dielectric(n_i, n_t, cos_i, _p(ct, f32p))
+    return f, float(ct[0])
+
+
+def background_scene(background, textures, scene_box, sat=None, flux=(0.0, 0.0, 0.0)):
+    """A bare OrcPtScene holding only a background (for the sampler checks)."""
+    s = OrcPtScene()
+    _fill_background(s, background, textures, sat, flux)
+    for k in range(6):
+        s.sceneBox[k] = float(scene_box[k])
+    return s
+
+
+def _fill_background(s, background, textures, sat, flux):
+    s.textures = textures.arr
+    s.numTextures = textures.count
+    if background is not None:
+        s.background.type = int(background["type"])
+        for k in range(3):
+            s.background.color[k] = float(background["color"][k])
+            s.background.flux[k] = float(flux[k])
+        s.background.scale = float(background["scale"])
+        s.background.envmapTexture = int(background["envmapTexture"])
+        if sat is not None:
+            s.background.sat = _p(sat, f32p)
+    else:
+        s.background.type = 0
+        s.background.scale = 1.0
+
+
+def background_sample(bscene, u0, u1):
+    out = np.zeros(7, np.float32)
+    _bsdf_lib().orc_background_sample(C.byref(bscene), u0, u1, _p(out, f32p))
+    return out[:3].copy(), float(out[3]), out[4:7].copy()
+
+
+def background_evaluate(bscene, direction):
+    out = np.zeros(4, np.float32)
+    _bsdf_lib().orc_background_evaluate(C.byref(bscene), _p(np.ascontiguousarray(direction, np.float32), f32p), _p(out, f32p))
+    return out[:3].copy(), float(out[3])
 
 
 class PtScene:
@@ -247,8 +375,11 @@ class PtScene:
     order), with shading arrays concatenated like the geometry, the host-built
     light tree (info, mem, keys, codes) and a pinhole camera dict."""
 
-    def __init__(self, scene, normals, uvs, mat_idx, materials, light_tree, scene_box, camera):
+    def __init__(self, scene, normals, uvs, mat_idx, materials, light_tree, scene_box, camera, textures=None,
+                 background=None, background_sat=None, background_flux=(0.0, 0.0, 0.0)):
         self.scene = scene
+        self.textures = textures or Textures()
+        self.sat = None if background_sat is None else np.ascontiguousarray(background_sat, np.float32)
         self.normals = np.ascontiguousarray(normals, np.float32)
         self.uvs = np.ascontiguousarray(uvs, np.float32)
         self.mat_idx = np.ascontiguousarray(mat_idx, np.uint16)
@@ -283,6 +414,7 @@ class PtScene:
         s.tanVFov = float(camera["tan_vfov"])
         s.width = int(camera["width"])
         s.height = int(camera["height"])
+        _fill_background(s, background, self.textures, self.sat, background_flux)
         self.s = s
 
 

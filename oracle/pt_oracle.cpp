@@ -1,25 +1,42 @@
 // Oracle — test infrastructure only.
 //
 // CPU restatement of Mufflon's pure path tracer iteration for the supported
-// scene content (triangles; LAMBERT and EMISSIVE materials with constant
-// textures; area-triangle and point lights through the light tree; pinhole
-// camera; vacuum), following
+// scene content (triangles; EMISSIVE, LAMBERT, TORRANCE and MICROFACET
+// materials with constant values or textures; homogeneous media; area-triangle
+// and point lights through the light tree; COLORED or polar ENVMAP
+// background; pinhole camera), following
 //   pt_common.hpp:75-156       pt_sample (RNG consumption order, NEE, MIS)
-//   random_walk.hpp:38-112     walk (sample, Russian roulette, first hit)
+//   random_walk.hpp:38-112     walk (sample, Russian roulette, first hit,
+//                              medium transmission)
 //   path_util.hpp:117-748      PathVertex (incident encoding, evaluate, sample,
 //                              get_emission, create_surface / create_void)
 //   pt_common.hpp:27-68        PtVertexExt (incidentPdf for MIS)
 //   pinhole.hpp:43-84          pinholecam_sample_ray
-//   material_sampling.hpp:68-237, lambert.hpp, emissive.hpp
-//   light_tree_sampling.hpp:195-329 (connect), :340-412 (light_pdf)
-//   light_sampling.hpp:118-140,308-375 (area triangle, point connect_light)
+//   material_sampling.hpp:33-237 fetch / sample / evaluate wrappers
+//   lambert.hpp, emissive.hpp, microfacet_base.hpp, microfacet_specular.hpp
+//   (TORRANCE), microfacet_full.hpp (MICROFACET), blend_fresnel.hpp:16-23
+//   cputexture.cpp:372-411     CPU texture sampling (nearest / linear)
+//   light_tree_sampling.hpp:195-329 (connect), :340-425 (light_pdf,
+//                              background_pdf)
+//   light_sampling.hpp:118-140,225-303,308-375,447-520 (area triangle, point,
+//                              background connect_light, evaluate_background)
+//   texture_sampling.hpp:113-155 (importance_sample_texture)
+//   textures/interface.hpp:180-200 (direction_to_uv, sample_polar_map)
 //   intersection.cu:366-621 (hit attributes), tangent_space.hpp:10-60
-//   rng.hpp:12-93, sampling.hpp:23-46,175-200 (PCG64, samplers, percentage_of)
+//   rng.hpp:12-93, sampling.hpp:23-127,175-200 (PCG64, samplers,
+//                              percentage_of, rescale_sample)
 //   render_target.hpp:62-76,189-195 (contribute, NaN check)
 //   output_handler_variance.hpp:7-31 (running mean), cpu_pt.cpp:18-45 (seeds)
 // Every float expression keeps the reference's operand order; the build uses
-// -ffp-contract=off. Epsilon functions (normalize, sgn, clamp) are this
-// project's definitions: normalize(v) = v / len(v), sgn(x) = x < 0 ? -1 : 1.
+// -ffp-contract=off. Functions of the absent Epsilon library and libm calls
+// have this project's definitions, shared with the device code:
+//   normalize(v) = v / len(v); sgn(x) = x < 0 ? -1 : 1; min/max are the
+//   compare-selects mn / mx; sq(x) = x * x; sdiv(x, d) = x / (d + 1e-20 sgn d);
+//   floor(float) -> int; mod(a, m) is the non-negative remainder;
+//   sin, cos, acos, atan2, exp, log and pow of a float are evaluated in double
+//   and rounded once; float -> u64 conversion maps NaN and negatives to 0.
+// Media are looked up by value: each material carries its outer medium and
+// (MICROFACET) its inner medium, the camera and point lights sit in vacuum.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -37,10 +54,36 @@ constexpr float kSceneEps = 1e-4f;          // intersection.cu:16
 constexpr float kInfPdf = 4294967296.f;      // AreaPdf/AngularPdf::infinite()
 constexpr float kDistSqEps = 1e-10f;        // light_sampling.hpp:78
 constexpr uint16_t kInternalNode = 0xFFFF;
+constexpr u64 kU64Max = ~0ull;
 
 inline float sgnf(float x) { return x < 0.0f ? -1.0f : 1.0f; }
 inline float clampf(float v, float lo, float hi) { return mn(mx(v, lo), hi); }
+inline float sq(float x) { return x * x; }
 inline V3 normalize(V3 v) { return v / len(v); }
+inline float sdiv(float x, float d) { return x / (d + 1e-20f * sgnf(d)); }
+inline bool nonzero(V3 v) { return v.x != 0.0f || v.y != 0.0f || v.z != 0.0f; }
+inline float fsin(float x) { return (float)std::sin((double)x); }
+inline float fcos(float x) { return (float)std::cos((double)x); }
+inline float facos(float x) { return (float)std::acos((double)x); }
+inline float fatan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline float fexp(float x) { return (float)std::exp((double)x); }
+inline float flog(float x) { return (float)std::log((double)x); }
+inline float fpow(float x, float y) { return (float)std::pow((double)x, (double)y); }
+inline int ifloor(float x) { return (int)std::floor(x); }
+inline int imod(int a, int m) {
+    const int r = a % m;
+    return r < 0 ? r + m : r;
+}
+inline u64 f2u64(float v) {
+    if (!(v > 0.0f)) return 0;  // NaN, negative, zero
+    if (v >= 18446744073709551616.0f) return kU64Max;
+    return (u64)v;
+}
+inline u64 d2u64(double v) {
+    if (!(v > 0.0)) return 0;
+    if (v >= 18446744073709551616.0) return kU64Max;
+    return (u64)v;
+}
 
 struct Pcg64 {  // rng.hpp:70-93
     u64 s;
@@ -67,20 +110,50 @@ struct Pcg64 {  // rng.hpp:70-93
 };
 
 inline float sample_uniform(u32 u) { return (float)u / 4294967810.0f; }
+// sampling.hpp:39-46
 inline u64 percentage_of(u64 num, float p) {
-    const u64 pfix = (u64)(p * 2147483648.0f);
+    const u64 pfix = f2u64(p * 2147483648.0f);
     return ((pfix * (num & 0x7fffffff)) >> 31) + (pfix * (num >> 31));
+}
+// sampling.hpp:72-85 (an empty interval maps to its left end)
+inline u64 rescale_sample(u64 x, u64 pLeft, u64 pRight) {
+    const u64 interval = pRight - pLeft;
+    const u64 xrel = x - pLeft;
+    if (interval == 0) return 0;
+    return xrel * (kU64Max / interval) + d2u64((double)xrel * (double)(kU64Max % interval) / (double)interval);
 }
 
 }  // namespace
 
 extern "C" {
-struct OrcMaterial {  // mfh_material, 64 bytes
+struct OrcMaterial {  // mfh_material, 128 bytes
     int32_t type;
     float albedo[3];
     float radiance[3];
+    float scale[3];
+    float roughness[2];
+    int32_t ndf;
+    int32_t shadowing;
+    float ior;
+    float absorption[3];
+    float outerIor;
+    float outerAbsorption[3];
+    int32_t albedoTexture;
+    int32_t roughnessTexture;
+    int32_t emissionTexture;
+    int32_t reserved[7];
+};
+struct OrcTexture {
+    int32_t width, height, channels, sampling;  // sampling: 0 nearest, 1 linear
+    const float* texels;                        // 4 floats per texel (RGBA), row-major
+};
+struct OrcBackground {
+    int32_t type;  // 0 colored, 1 envmap
+    float color[3];
     float scale;
-    int32_t reserved[8];
+    int32_t envmapTexture;
+    const float* sat;  // width * height (envmap)
+    float flux[3];
 };
 struct OrcPtScene {
     OrcScene geo;
@@ -105,39 +178,544 @@ struct OrcPtScene {
     float camDir[3];
     float camUp[3];
     int32_t width, height;
+    const OrcTexture* textures;
+    int32_t numTextures;
+    OrcBackground background;
 };
 }
 
 namespace {
 
-enum Mat : int32_t { kEmissive = 0, kLambert = 1 };
+enum Mat : int32_t { kEmissive = 0, kLambert = 1, kTorrance = 3, kMicrofacet = 10 };
+enum Ndf : int32_t { kGgx = 0, kBeckmann = 1 };
+enum Shadowing : int32_t { kVCavity = 0, kSmith = 1 };
 
 struct TS {  // TangentSpace
     V3 shadingN{0, 0, 0}, geoN{0, 0, 0}, shadingTX{0, 0, 0}, shadingTY{0, 0, 0};
 };
 
+struct Medium {
+    float ior;
+    V3 absorption;
+};
+const Medium kVacuum{1.0f, {0.0f, 0.0f, 0.0f}};
+// Medium::get_transmission (medium.hpp:40-42)
+inline V3 transmission(const Medium& m, float distance) {
+    const V3 e = m.absorption * -distance;
+    return {fexp(e.x), fexp(e.y), fexp(e.z)};
+}
+
+// ParameterPack of a fetched material (material_sampling.hpp:33-53)
+struct MatParams {
+    int32_t type = -1;
+    V3 albedo{0, 0, 0};
+    V3 radiance{0, 0, 0};
+    float rx = 0, ry = 0;
+    int32_t ndf = kGgx, shadowing = kVCavity;
+    Medium inner = kVacuum, outer = kVacuum;
+    // MaterialDescriptorBase::get_medium (material_definitions.hpp:53-56)
+    const Medium& medium(float xDotN) const { return xDotN < 0.0f ? inner : outer; }
+    bool reflective() const { return type == kLambert || type == kTorrance || type == kMicrofacet; }
+    bool refractive() const { return type == kMicrofacet; }
+    bool halfvector() const { return type == kTorrance || type == kMicrofacet; }
+};
+
+// ---------------------------------------------------------------- textures
+struct F4 {
+    float x, y, z, w;
+};
+inline F4 texel(const OrcTexture& t, int x, int y) {
+    const int ix = imod(x, t.width), iy = imod(y, t.height);
+    const float* p = t.texels + 4 * ((size_t)iy * t.width + ix);
+    return {p[0], p[1], p[2], p[3]};
+}
+// CpuTexture::sample_nearest / sample_linear / sample111_* (cputexture.cpp:372-411)
+F4 tex_sample(const OrcTexture& t, float u, float v) {
+    if (t.width == 1 && t.height == 1) return texel(t, 0, 0);
+    if (t.sampling == 0) return texel(t, ifloor(u * (float)t.width), ifloor(v * (float)t.height));
+    float fx = u * (float)t.width, fy = v * (float)t.height;
+    const int ix = ifloor(fx), iy = ifloor(fy);
+    fx -= (float)ix;
+    fy -= (float)iy;
+    const F4 a = texel(t, ix, iy), b = texel(t, ix + 1, iy), c = texel(t, ix, iy + 1), d = texel(t, ix + 1, iy + 1);
+    F4 s;
+    s.x = a.x * (1.0f - fx) * (1.0f - fy);
+    s.y = a.y * (1.0f - fx) * (1.0f - fy);
+    s.z = a.z * (1.0f - fx) * (1.0f - fy);
+    s.w = a.w * (1.0f - fx) * (1.0f - fy);
+    s.x += b.x * fx * (1.0f - fy);
+    s.y += b.y * fx * (1.0f - fy);
+    s.z += b.z * fx * (1.0f - fy);
+    s.w += b.w * fx * (1.0f - fy);
+    s.x += c.x * (1.0f - fx) * fy;
+    s.y += c.y * (1.0f - fx) * fy;
+    s.z += c.z * (1.0f - fx) * fy;
+    s.w += c.w * (1.0f - fx) * fy;
+    s.x += d.x * fx * fy;
+    s.y += d.y * fx * fy;
+    s.z += d.z * fx * fy;
+    s.w += d.w * fx * fy;
+    return s;
+}
+
+// fetch (material_sampling.hpp:33-53) with the per-type fetch of lambert.hpp:10-15,
+// emissive.hpp:9-14, microfacet_specular.hpp:11-22, microfacet_full.hpp:12-25
+MatParams fetch(const OrcPtScene& S, const OrcMaterial& m, float u, float v) {
+    MatParams p;
+    p.type = m.type;
+    p.outer = {m.outerIor, {m.outerAbsorption[0], m.outerAbsorption[1], m.outerAbsorption[2]}};
+    p.inner = m.type == kMicrofacet ? Medium{m.ior, {m.absorption[0], m.absorption[1], m.absorption[2]}} : p.outer;
+    if (m.type == kLambert || m.type == kTorrance) {
+        if (m.albedoTexture >= 0) {
+            const F4 t = tex_sample(S.textures[m.albedoTexture], u, v);
+            p.albedo = {t.x, t.y, t.z};
+        } else {
+            p.albedo = {m.albedo[0], m.albedo[1], m.albedo[2]};
+        }
+    }
+    if (m.type == kTorrance || m.type == kMicrofacet) {
+        if (m.roughnessTexture >= 0) {
+            const OrcTexture& tx = S.textures[m.roughnessTexture];
+            const F4 t = tex_sample(tx, u, v);
+            p.rx = t.x;
+            p.ry = tx.channels > 1 ? t.y : t.x;
+        } else {
+            p.rx = m.roughness[0];
+            p.ry = m.roughness[1];
+        }
+        p.ndf = m.ndf;
+        p.shadowing = m.shadowing;
+    }
+    if (m.type == kEmissive)
+        p.radiance = {m.radiance[0] * m.scale[0], m.radiance[1] * m.scale[1], m.radiance[2] * m.scale[2]};
+    return p;
+}
+
+// ---------------------------------------------------------------- microfacets
+// compute_half_vector (microfacet_base.hpp:15-27)
+V3 half_vector(float nIn, float nEx, V3 in, V3 ex) {
+    const V3 h = nIn * in + nEx * ex;
+    const float l = len(h) * sgnf(h.z);
+    return {sdiv(h.x, l), sdiv(h.y, l), sdiv(h.z, l)};
+}
+// Boundary::get_halfTS (microfacet_base.hpp:45-52)
+V3 boundary_half(const Medium& incident, const Medium& other, V3 in, V3 ex) {
+    const Medium& exM = in.z * ex.z > 0.0f ? incident : other;
+    return half_vector(incident.ior, exM.ior, in, ex);
+}
+
+// microfacet_base.hpp:73-80
+float g_vcavity(float wDotH, float wDotN, float hDotN) {
+    if (wDotH * wDotN <= 0.0f) return 0.0f;
+    return mn(1.0f, 2.0f * std::fabs(sdiv(hDotN * wDotN, wDotH)));
+}
+// microfacet_base.hpp:83-102 (GGX)
+float g_smith(float wDotH, V3 w, float rx, float ry) {
+    if (wDotH * w.z <= 0.0f) return 0.0f;
+    const float oneOverASquared = sdiv(sq(w.x * rx) + sq(w.y * ry), w.z * w.z);
+    return 2.0f / (1.0f + std::sqrt(1.f + oneOverASquared));
+}
+// microfacet_base.hpp:113-120
+float modify_g(float g) {
+    return g < 0.11f ? fpow(g + 0.74299714456847f, 8.0f) - 0.092874643071059f
+                     : ((1.139400367544197f * g - 3.317658394008854f) * g + 3.317115685385115f) * g -
+                           0.13885765892046f;
+}
+float g_reflection(int shadowing, float gi, float ge) {
+    return shadowing == kSmith ? gi * ge : modify_g(mn(gi, ge));
+}
+float g_transmission(int shadowing, float gi, float ge) {
+    return shadowing == kSmith ? gi * ge : mx(0.0f, modify_g(gi) + modify_g(ge) - 1.0f);
+}
+float g_term(const MatParams& p, float wDotH, V3 w, float hDotN) {
+    return p.shadowing == kSmith ? g_smith(wDotH, w, p.rx, p.ry) : g_vcavity(wDotH, w.z, hDotN);
+}
+
+// eval_ndf (microfacet_base.hpp:281-310)
+float eval_ndf(const MatParams& p, V3 h) {
+    if (p.ndf == kGgx) {
+        const float norm = mx(kPi * p.rx * p.ry, 1e-30f);
+        const float cosThetaSq = h.z * h.z;
+        const float rx = sdiv(h.x, p.rx);
+        const float ry = sdiv(h.y, p.ry);
+        const float tmp = rx * rx + ry * ry + cosThetaSq;
+        return sdiv(1.0f, norm * tmp * tmp);
+    }
+    const float norm = kPi * p.rx * p.ry;
+    const float rx = sdiv(h.x, p.rx);
+    const float ry = sdiv(h.y, p.ry);
+    const float nDotHSq = h.z * h.z;
+    const float e = fexp(-(rx * rx + ry * ry) / nDotHSq);
+    return sdiv(e, norm * nDotHSq * nDotHSq);
+}
+
+struct DirSample {
+    V3 dir;
+    float pdf;
+};
+// sample_ndf (microfacet_base.hpp:312-366)
+DirSample sample_ndf(const MatParams& p, float u0, float u1) {
+    if (p.ndf == kGgx) {
+        const float phi = 2.0f * kPi * u0;
+        const float e = std::sqrt(u1 / (1.0f - u1));
+        float slopeX = e * fcos(phi);
+        float slopeY = e * fsin(phi);
+        const float norm = kPi * p.rx * p.ry;
+        const float tmp = 1.0f + slopeX * slopeX + slopeY * slopeY;
+        slopeX *= p.rx;
+        slopeY *= p.ry;
+        const V3 dir = normalize({-slopeX, -slopeY, 1.0f});
+        return {dir, 1.0f / mx(norm * tmp * tmp * dir.z * dir.z * dir.z, 1e-20f)};
+    }
+    const float phi = 2 * kPi * u0;
+    const float xi = u1 + 1e-20f;
+    const float e = std::sqrt(-flog(xi));
+    const float slopeX = e * fcos(phi);
+    const float slopeY = e * fsin(phi);
+    const V3 dir = normalize({-p.rx * slopeX, -p.ry * slopeY, 1.0f});
+    return {dir, xi / mx(kPi * p.rx * p.ry * dir.z * dir.z * dir.z, 1e-20f)};
+}
+
+// sample_visible_normal_vcavity (microfacet_base.hpp:137-156)
+V3 visible_normal_vcavity(V3 in, V3 cav, u64& rnd, float& cosI) {
+    const V3 adj{-cav.x, -cav.y, cav.z};
+    const float iDotHOut = dot(in, cav);
+    const float iDotHadj = dot(in, adj);
+    const float t = in.z > 0.0f ? mx(0.0f, iDotHadj) / (mx(0.0f, iDotHOut) + mx(0.0f, iDotHadj))
+                                : mx(0.0f, -iDotHadj) / (mx(0.0f, -iDotHOut) + mx(0.0f, -iDotHadj));
+    const u64 probAdj = percentage_of(kU64Max, t);
+    if (rnd < probAdj) {
+        rnd = rescale_sample(rnd, 0, probAdj - 1);
+        cosI = iDotHadj;
+        return adj;
+    }
+    rnd = rescale_sample(rnd, probAdj, kU64Max);
+    cosI = iDotHOut;
+    return cav;
+}
+
+// sample_visible_slopes_ggx (microfacet_base.hpp:213-244)
+void visible_slopes_ggx(float tanTheta, float u0, float u1, u64& rnd2, float& sx, float& sy) {
+    if (tanTheta < 0.0001f) {
+        const float r = std::sqrt(sdiv(u0, (1.f - u0)));
+        const float phi = 6.28318530718f * u1;
+        sx = r * fcos(phi);
+        sy = r * fsin(phi);
+        return;
+    }
+    const float a1 = 1.0f / tanTheta;
+    const float g1 = 2.f / (1.f + std::sqrt(1.0f + 1.0f / (a1 * a1)));
+    const float a = 2.0f * u0 / g1 - 1.0f;
+    const float tmp = sdiv(1.0f, a * a - 1.0f);
+    const float d = tmp * std::sqrt(tanTheta * tanTheta - (a * a - tanTheta * tanTheta) / tmp);
+    const float slopeX1 = tanTheta * tmp - d;
+    const float slopeX2 = tanTheta * tmp + d;
+    sx = (a < 0.f || slopeX2 > 1.0f / tanTheta) ? slopeX1 : slopeX2;
+    float s;
+    if (rnd2 & (1ull << 63)) {
+        rnd2 *= 2;
+        s = 1.0f;
+    } else {
+        rnd2 *= 2;
+        s = -1.0f;
+    }
+    const float z = sdiv((u1 * (u1 * (u1 * 0.27385f - 0.73369f) + 0.46341f)),
+                         (u1 * (u1 * (u1 * 0.093073f + 0.309420f) - 1.000000f) + 0.597999f));
+    sy = s * z * std::sqrt(1.0f + sx * sx);
+}
+
+// sample_visible_normal_smith (microfacet_base.hpp:247-278), GGX
+V3 visible_normal_smith(const MatParams& p, V3 in, float u0, float u1, u64& rnd2) {
+    V3 st = in * V3{p.rx, p.ry, 1.0f};
+    st = normalize(st);
+    if (in.z < 0.0f) st = -st;
+    const float sinTheta = std::sqrt((1.0f - st.z) * (1.0f + st.z));
+    const float tanTheta = sinTheta / st.z;
+    const float cosPhi = sinTheta > 1e-5f ? st.x / sinTheta : 0.0f;
+    const float sinPhi = sinTheta > 1e-5f ? st.y / sinTheta : 0.0f;
+    float sx, sy;
+    visible_slopes_ggx(tanTheta, u0, u1, rnd2, sx, sy);
+    const float tmp = cosPhi * sx - sinPhi * sy;
+    sy = sinPhi * sx + cosPhi * sy;
+    sx = tmp;
+    sx *= p.rx;
+    sy *= p.ry;
+    return normalize({-sx, -sy, 1.0f});
+}
+
+struct Refraction {
+    float f, cosTAbs;
+};
+// fresnel_dielectric (blend_fresnel.hpp:16-23)
+Refraction fresnel_dielectric(float n_i, float n_t, float cosIAbs) {
+    const float eta = sq(n_i / n_t);
+    const float cosTAbs = std::sqrt(mx(0.0f, 1.0f - eta * (1.0f - cosIAbs * cosIAbs)));
+    const float rParl = sdiv(n_t * cosIAbs - n_i * cosTAbs, n_t * cosIAbs + n_i * cosTAbs);
+    const float rPerp = sdiv(n_t * cosTAbs - n_i * cosIAbs, n_t * cosTAbs + n_i * cosIAbs);
+    return {0.5f * (rParl * rParl + rPerp * rPerp), cosTAbs};
+}
+
+// ---------------------------------------------------------------- BSDF in tangent space
+struct TsSample {
+    bool valid = false;
+    V3 throughput{0, 0, 0};
+    V3 excident{0, 0, 0};
+    float pdfF = 0, pdfB = 0;
+};
+
+// Shared half-vector sampling of TORRANCE and MICROFACET.
+V3 sample_half(const MatParams& p, V3 in, float u0, float u1, u64& rnd, float& cavityPdf, float& iDotH) {
+    if (p.shadowing == kSmith) {
+        const V3 h = visible_normal_smith(p, in, u0, u1, rnd);
+        cavityPdf = eval_ndf(p, h);
+        cavityPdf *= h.z;
+        iDotH = dot(in, h);
+        return h;
+    }
+    const DirSample cav = sample_ndf(p, u0, u1);
+    float cosI;
+    const V3 h = visible_normal_vcavity(in, cav.dir, rnd, cosI);
+    iDotH = cosI;
+    cavityPdf = cav.pdf;
+    return h;
+}
+
+// TORRANCE sample (microfacet_specular.hpp:26-83)
+TsSample torrance_sample(const MatParams& p, V3 in, float u0, float u1, u64 i0) {
+    TsSample r;
+    u64 rnd = i0;
+    float cavityPdf, iDotH;
+    const V3 h = sample_half(p, in, u0, u1, rnd, cavityPdf, iDotH);
+    const V3 ex = (2.0f * iDotH) * h - in;
+    float ge, gi, g;
+    ge = g_term(p, iDotH, ex, h.z);
+    gi = g_term(p, iDotH, in, h.z);
+    g = g_reflection(p.shadowing, gi, ge);
+    if (ge == 0.0f || gi == 0.0f) return r;
+    r.throughput = p.albedo * sdiv(g, gi);
+    r.excident = ex;
+    r.pdfF = cavityPdf * sdiv(gi, std::fabs(4.0f * in.z * h.z));
+    r.pdfB = cavityPdf * sdiv(ge, std::fabs(4.0f * ex.z * h.z));
+    r.valid = true;
+    return r;
+}
+
+// MICROFACET sample (microfacet_full.hpp:35-126)
+TsSample microfacet_sample(const MatParams& p, V3 in, const Medium& incident, const Medium& other, float u0,
+                           float u1, u64 i0) {
+    TsSample r;
+    u64 rnd = i0;
+    float cavityPdf, iDotH;
+    const V3 h = sample_half(p, in, u0, u1, rnd, cavityPdf, iDotH);
+    const float n_i = incident.ior;
+    const float n_t = other.ior;
+    const float eta = n_i / n_t;
+    const float iDotHabs = std::fabs(iDotH);
+    const Refraction f = fresnel_dielectric(n_i, n_t, iDotHabs);
+    const bool reflect = ((float)rnd / (float)(kU64Max - 1)) < f.f;
+    float eDotH, eDotHabs;
+    V3 ex;
+    if (reflect) {
+        ex = (2.0f * iDotH) * h - in;
+        eDotH = iDotH;
+        eDotHabs = iDotHabs;
+    } else {
+        eDotHabs = f.cosTAbs;
+        eDotH = eDotHabs * -sgnf(iDotH);
+        ex = (sgnf(iDotH) * (eta * iDotHabs - eDotHabs)) * h - eta * in;
+    }
+    const float ge = g_term(p, eDotH, ex, h.z);
+    const float gi = g_term(p, iDotH, in, h.z);
+    if (ge == 0.0f || gi == 0.0f) return r;
+    float throughput, pdfF, pdfB;
+    if (reflect) {
+        const float g = g_reflection(p.shadowing, gi, ge);
+        throughput = sdiv(g, gi);
+        const float common = cavityPdf * sdiv(f.f, 4.0f * h.z);
+        pdfF = common * sdiv(gi, std::fabs(in.z));
+        pdfB = common * sdiv(ge, std::fabs(ex.z));
+    } else {
+        const float g = g_transmission(p.shadowing, gi, ge);
+        throughput = sdiv(g, gi);
+        const float common =
+            cavityPdf * (1.0f - f.f) * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_t * eDotH) * h.z);
+        pdfF = common * sdiv(gi * n_t * n_t, std::fabs(in.z));
+        pdfB = common * sdiv(ge * n_i * n_i, std::fabs(ex.z));
+    }
+    r.throughput = {throughput, throughput, throughput};
+    r.excident = ex;
+    r.pdfF = pdfF;
+    r.pdfB = pdfB;
+    r.valid = true;
+    return r;
+}
+
+// Lambert sample (lambert.hpp:18-35), sample_dir_cosine (sampling.hpp:95-101)
+TsSample lambert_sample(const MatParams& p, V3 in, float u0, float u1) {
+    TsSample r;
+    const float cosT = std::sqrt(u0);
+    const float sinT = std::sqrt(1.0f - u0);
+    const float phi = u1 * 2 * kPi;
+    const V3 ex0{sinT * fsin(phi), sinT * fcos(phi), cosT};
+    r.excident = ex0 * sgnf(in.z);
+    r.throughput = p.albedo;
+    r.pdfF = ex0.z / kPi;
+    r.pdfB = std::fabs(in.z) / kPi;
+    r.valid = true;
+    return r;
+}
+
+TsSample sample_ts(const MatParams& p, V3 inTS, float u0, float u1, u64 i0) {
+    const Medium& incident = p.medium(inTS.z);
+    const Medium& other = p.medium(-inTS.z);
+    switch (p.type) {
+        case kLambert: return lambert_sample(p, inTS, u0, u1);
+        case kTorrance: return torrance_sample(p, inTS, u0, u1, i0);
+        case kMicrofacet: return microfacet_sample(p, inTS, incident, other, u0, u1, i0);
+        default: return TsSample{};  // EMISSIVE: not sampleable
+    }
+}
+
+struct TsEval {
+    V3 value{0, 0, 0};
+    float pdfF = 0, pdfB = 0;
+};
+
+TsEval evaluate_ts(const MatParams& p, V3 in, V3 ex) {
+    TsEval r;
+    const Medium& incident = p.medium(in.z);
+    const Medium& other = p.medium(-in.z);
+    if (p.type == kLambert) {  // lambert.hpp:37-49
+        if (in.z * ex.z < 0.0f) return r;
+        r.value = p.albedo / kPi;
+        r.pdfF = std::fabs(ex.z) / kPi;
+        r.pdfB = std::fabs(in.z) / kPi;
+        return r;
+    }
+    if (p.type == kTorrance) {  // microfacet_specular.hpp:86-122
+        if (in.z * ex.z < 0.0f) return r;
+        const V3 h = boundary_half(incident, other, in, ex);
+        const float iDotH = dot(in, h);
+        const float ge = g_term(p, iDotH, ex, h.z);
+        const float gi = g_term(p, iDotH, in, h.z);
+        const float g = g_reflection(p.shadowing, gi, ge);
+        if (ge == 0.0f || gi == 0.0f) return r;
+        const float d = eval_ndf(p, h);
+        r.value = p.albedo * sdiv(g * d, std::fabs(4.0f * in.z * ex.z));
+        r.pdfF = sdiv(gi * d, std::fabs(4.0f * in.z));
+        r.pdfB = sdiv(ge * d, std::fabs(4.0f * ex.z));
+        return r;
+    }
+    if (p.type == kMicrofacet) {  // microfacet_full.hpp:129-187
+        const bool isReflection = in.z * ex.z > 0.0f;
+        const V3 h = boundary_half(incident, other, in, ex);
+        const float iDotH = dot(in, h);
+        const float eDotH = dot(ex, h);
+        const float n_i = incident.ior;
+        const float n_t = other.ior;
+        const float ge = g_term(p, eDotH, ex, h.z);
+        const float gi = g_term(p, iDotH, in, h.z);
+        if (ge == 0.0f || gi == 0.0f) return r;
+        const float d = eval_ndf(p, h);
+        const float f = fresnel_dielectric(n_i, n_t, std::fabs(iDotH)).f;
+        if (isReflection) {
+            const float g = g_reflection(p.shadowing, gi, ge);
+            const float common = d * f / 4.0f;
+            const float v = sdiv(g * common, in.z * ex.z);
+            r.value = {v, v, v};
+            r.pdfF = sdiv(gi * common, std::fabs(in.z));
+            r.pdfB = sdiv(ge * common, std::fabs(ex.z));
+            return r;
+        }
+        const float common = sdiv((1 - f) * std::fabs(d * iDotH * eDotH), sq(n_i * iDotH + n_t * eDotH));
+        const float g = g_transmission(p.shadowing, gi, ge);
+        const float bsdf = g * common * sdiv(n_t * n_t, std::fabs(in.z * ex.z));
+        r.value = {bsdf, bsdf, bsdf};
+        r.pdfF = gi * common * sdiv(n_t * n_t, std::fabs(in.z));
+        r.pdfB = ge * common * sdiv(n_i * n_i, std::fabs(ex.z));
+        return r;
+    }
+    return r;  // EMISSIVE
+}
+
+// ---------------------------------------------------------------- world-space wrappers
+struct PathSample {
+    bool valid;
+    V3 throughput, excident;
+    float pdfF, pdfB;
+};
+
+// materials::sample (material_sampling.hpp:68-146), view paths (adjoint = false)
+PathSample mat_sample(const TS& ts, const MatParams& m, V3 incident, float u0, float u1, u64 i0) {
+    PathSample r{false, {0, 0, 0}, {0, 0, 0}, 0, 0};
+    const float iDotG = -dot(incident, ts.geoN);
+    const float iDotN = -dot(incident, ts.shadingN);
+    if (iDotG * iDotN <= 0.0f) return r;
+    const V3 inTS{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN};
+    const TsSample s = sample_ts(m, inTS, u0, u1, i0);
+    if (!s.valid || !nonzero(s.throughput)) return r;
+    V3 g = s.excident.x * ts.shadingTX + s.excident.y * ts.shadingTY + s.excident.z * ts.shadingN;
+    g = normalize(g);
+    const float eDotG = dot(g, ts.geoN);
+    const float eDotN = s.excident.z;
+    if (eDotG * eDotN <= 0.0f) return r;
+    r.throughput = s.throughput;
+    r.excident = g;
+    r.pdfF = mn(s.pdfF, 1e9f);
+    r.pdfB = mn(s.pdfB, 1e9f);
+    r.valid = true;
+    return r;
+}
+
+struct EvalValue {
+    V3 value;
+    float cosOut, pdfF, pdfB;
+};
+
+// materials::evaluate (material_sampling.hpp:160-237), connections (no lightNormal)
+EvalValue mat_evaluate(const TS& ts, const MatParams& m, V3 incident, V3 excident) {
+    EvalValue z{{0, 0, 0}, 0, 0, 0};
+    const float iDotN = -dot(incident, ts.shadingN);
+    const float eDotN = dot(excident, ts.shadingN);
+    if (!m.reflective() && iDotN * eDotN > 0.0f) return z;
+    if (!m.refractive() && iDotN * eDotN < 0.0f) return z;
+    const float iDotG = -dot(incident, ts.geoN);
+    const float eDotG = dot(excident, ts.geoN);
+    if (eDotG * eDotN <= 0.0f || iDotG * iDotN <= 0.0f) return z;
+    const V3 inTS{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN};
+    const V3 exTS{dot(excident, ts.shadingTX), dot(excident, ts.shadingTY), eDotN};
+    const TsEval e = evaluate_ts(m, inTS, exTS);
+    if (!nonzero(e.value)) return z;
+    EvalValue r;
+    r.value = e.value;
+    r.cosOut = std::fabs(eDotN);
+    r.pdfF = mn(e.pdfF, 1e9f);
+    r.pdfB = mn(e.pdfB, 1e9f);
+    return r;
+}
+
+// emission (emissive.hpp:39-44); (0,0,0) for other types
+V3 mat_emission(const MatParams& m, V3 geoN, V3 excident) {
+    if (m.type != kEmissive) return {0, 0, 0};
+    const float cosOut = dot(geoN, excident);
+    if (cosOut <= 0.0f) return {0, 0, 0};
+    return m.radiance;
+}
+
+// ---------------------------------------------------------------- scene access
 struct Vertex {
     bool isCamera = true, isVoid = false;
     V3 position{0, 0, 0};
     V3 incEnc{0, 0, 0};  // (x, y, dist * sgn(z)) for surfaces, raw dir for void
     TS ts;
     i32 inst = -1, prim = -1;
-    float sp0 = 0, sp1 = 0;  // surface parameters (barycentrics)
-    const OrcMaterial* mat = nullptr;
+    MatParams mat;
     float incidentPdf = 1.0f;  // PtVertexExt
     V3 incident_dir() const {
         const float z = sgnf(incEnc.z) * std::sqrt(clampf(1.0f - incEnc.x * incEnc.x - incEnc.y * incEnc.y, 0.0f, 1.0f));
         return {incEnc.x, incEnc.y, z};
     }
     V3 geo_normal() const { return (isCamera || isVoid) ? V3{0, 0, 0} : ts.geoN; }
-};
-
-struct Ctx {
-    const OrcPtScene& S;
-    V3 vert(i32 lodBase, u32 i) const {
-        const float* v = S.geo.verts + 3 * ((size_t)lodBase + i);
-        return {v[0], v[1], v[2]};
-    }
 };
 
 M34 mat_of(const OrcScene& g, i32 inst) {
@@ -241,89 +819,126 @@ TS tangent_space(const OrcPtScene& S, const HitInfo& hit) {
     return {sn, hit.geoN, tx, ty};
 }
 
-const OrcMaterial* material_of(const OrcPtScene& S, i32 inst, i32 prim) {
+const OrcMaterial& material_of(const OrcPtScene& S, i32 inst, i32 prim) {
     const i32 lod = S.geo.instLod[inst];
     const i32 tb = S.geo.lodTriOffset[lod];
-    return S.materials + S.matIdx[(size_t)tb + prim];
+    return S.materials[S.matIdx[(size_t)tb + prim]];
 }
 
-// ---------------------------------------------------------------- materials
-struct PathSample {
-    bool valid;
-    V3 throughput, excident;
-    float pdfF, pdfB;
-};
+// ---------------------------------------------------------------- background
+inline float projected_area(V3 dir, const float box[6]) {  // sampling.hpp:160-167
+    const V3 sides{box[3] - box[0], box[4] - box[1], box[5] - box[2]};
+    const float projAx = sides.y * sides.z * std::fabs(dir.x);
+    const float projAy = sides.x * sides.z * std::fabs(dir.y);
+    const float projAz = sides.x * sides.y * std::fabs(dir.z);
+    return projAx + projAy + projAz;
+}
+inline float box_diag_len(const float box[6]) {
+    return len(V3{box[3] - box[0], box[4] - box[1], box[5] - box[2]});
+}
 
-// materials::sample (material_sampling.hpp:77-146) with Lambert (lambert.hpp:18-35)
-PathSample mat_sample(const TS& ts, const OrcMaterial& m, V3 incident, float u0, float u1) {
-    PathSample r{false, {0, 0, 0}, {0, 0, 0}, 0, 0};
-    const float iDotG = -dot(incident, ts.geoN);
-    const float iDotN = -dot(incident, ts.shadingN);
-    if (iDotG * iDotN <= 0.0f) return r;
-    const V3 inTS{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN};
-    if (m.type != kLambert) return r;  // emissive: invalid sample
-    // sample_dir_cosine (sampling.hpp:95-101)
-    const float cosT = std::sqrt(u0);
-    const float sinT = std::sqrt(1.0f - u0);
-    const float phi = u1 * 2 * kPi;
-    const float sphi = (float)std::sin((double)phi), cphi = (float)std::cos((double)phi);
-    const V3 exTS0{sinT * sphi, sinT * cphi, cosT};
-    const V3 exTS = exTS0 * sgnf(inTS.z);
-    r.throughput = {m.albedo[0], m.albedo[1], m.albedo[2]};
-    r.pdfF = exTS0.z / kPi;
-    r.pdfB = std::fabs(inTS.z) / kPi;
-    if (r.throughput.x == 0.0f && r.throughput.y == 0.0f && r.throughput.z == 0.0f) return r;
-    V3 g = exTS.x * ts.shadingTX + exTS.y * ts.shadingTY + exTS.z * ts.shadingN;
-    g = normalize(g);
-    const float eDotG = dot(g, ts.geoN);
-    const float eDotN = exTS.z;
-    if (eDotG * eDotN <= 0.0f) return r;
-    r.excident = g;
-    r.pdfF = mn(r.pdfF, 1e9f);
-    r.pdfB = mn(r.pdfB, 1e9f);
-    r.valid = true;
+const V3 kLumWeight{0.212671f, 0.715160f, 0.072169f};
+
+// summed_details::lower_bound (texture_sampling.hpp:21-37)
+int sat_lower_bound(const float* sat, int width, float val, int max, int constCoord, bool row) {
+    int left = 0;
+    int count = max + 1;
+    while (count > 0) {
+        const int step = count / 2;
+        const int curr = left + step;
+        const float x = row ? sat[(size_t)curr * width + constCoord] : sat[(size_t)constCoord * width + curr];
+        if (x < val) {
+            left = curr + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    return left;
+}
+
+struct EnvSample {
+    int tx, ty;
+    float u, v, pdf;
+};
+// importance_sample_texture (texture_sampling.hpp:119-155)
+EnvSample importance_sample_sat(const float* sat, int W, int H, float u0, float u1) {
+    const int bx = W - 1, by = H - 1;
+    const float highestRowwise = sat[(size_t)by * W + bx];
+    const float x = highestRowwise * u0;
+    const int row = sat_lower_bound(sat, W, x, by, bx, true);
+    const float vr0 = row == 0 ? 0.0f : sat[(size_t)(row - 1) * W + bx];
+    const float vr1 = sat[(size_t)row * W + bx];
+    const float rowVal = ((float)row + (x - vr0) / (vr1 - vr0)) / (float)H;
+    const float highestColumnwise = vr1 - vr0;
+    const float y = highestColumnwise * u1;
+    const int column = sat_lower_bound(sat, W, y, bx, row, false);
+    const float vc0 = column == 0 ? 0.f : sat[(size_t)row * W + column - 1];
+    const float vc1 = column == bx ? highestRowwise : sat[(size_t)row * W + column];
+    const float columnVal = ((float)column + (y - vc0) / (vc1 - vc0)) / (float)W;
+    const float pdf = (vc1 - vc0) * (float)W * (float)H / highestRowwise;
+    return {column, row, columnVal, rowVal, pdf};
+}
+
+struct EnvDir {
+    V3 dir;
+    float pdf;
+    V3 radiance;
+};
+// sample_light_dir (light_sampling.hpp:225-290): COLORED and polar ENVMAP
+EnvDir background_sample_dir(const OrcPtScene& S, float u0, float u1) {
+    const OrcBackground& bg = S.background;
+    EnvDir r;
+    if (bg.type == 0) {  // sample_dir_sphere_uniform (sampling.hpp:101-109)
+        const float cosTheta = u0 * 2.0f - 1.0f;
+        const float sinTheta = std::sqrt((1.0f - cosTheta) * (1.0f + cosTheta));
+        const float phi = u1 * 2.0f * kPi;
+        r.dir = {sinTheta * fsin(phi), sinTheta * fcos(phi), cosTheta};
+        r.pdf = 1.f / (4.f * kPi);
+        r.radiance = V3{bg.color[0], bg.color[1], bg.color[2]} * bg.scale;
+        return r;
+    }
+    const OrcTexture& env = S.textures[bg.envmapTexture];
+    const EnvSample s = importance_sample_sat(bg.sat, env.width, env.height, u0, u1);
+    const float phi = s.u * 2.f * kPi;
+    const float theta = (1.f - s.v) * kPi;
+    const float sinTheta = fsin(theta);
+    const float cosTheta = fcos(theta);
+    const float sinPhi = fsin(phi);
+    const float cosPhi = fcos(phi);
+    r.dir = {sinTheta * cosPhi, cosTheta, sinTheta * sinPhi};
+    r.pdf = s.pdf / (2.f * kPi * kPi * sinTheta);
+    const F4 t = texel(env, s.tx, s.ty);
+    r.radiance = V3{t.x, t.y, t.z} * bg.scale;
     return r;
 }
 
-struct EvalValue {
+struct BgEval {
     V3 value;
-    float cosOut, pdfF, pdfB;
+    float pdfBack;
 };
-
-// materials::evaluate (material_sampling.hpp:160-237) with Lambert (lambert.hpp:37-49)
-EvalValue mat_evaluate(const TS& ts, const OrcMaterial& m, V3 incident, V3 excident) {
-    EvalValue z{{0, 0, 0}, 0, 0, 0};
-    const float iDotN = -dot(incident, ts.shadingN);
-    const float eDotN = dot(excident, ts.shadingN);
-    // flags: LAMBERT is REFLECTIVE only, EMISSIVE neither
-    const bool reflective = m.type == kLambert;
-    if (!reflective && iDotN * eDotN > 0.0f) return z;
-    if (iDotN * eDotN < 0.0f) return z;  // not refractive
-    const float iDotG = -dot(incident, ts.geoN);
-    const float eDotG = dot(excident, ts.geoN);
-    if (eDotG * eDotN <= 0.0f || iDotG * iDotN <= 0.0f) return z;
-    const V3 inTS{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN};
-    const V3 exTS{dot(excident, ts.shadingTX), dot(excident, ts.shadingTY), eDotN};
-    if (m.type != kLambert) return z;
-    if (inTS.z * exTS.z < 0.0f) return z;
-    EvalValue r;
-    r.value = {m.albedo[0] / kPi, m.albedo[1] / kPi, m.albedo[2] / kPi};
-    r.pdfF = std::fabs(exTS.z) / kPi;
-    r.pdfB = std::fabs(inTS.z) / kPi;
-    if (r.value.x == 0.0f && r.value.y == 0.0f && r.value.z == 0.0f) return z;
-    r.pdfF = mn(r.pdfF, 1e9f);
-    r.pdfB = mn(r.pdfB, 1e9f);
-    r.cosOut = std::fabs(eDotN);
-    return r;
-}
-
-// emission of a material (emissive.hpp:39-44); (0,0,0) for others
-V3 mat_emission(const OrcMaterial& m, V3 geoN, V3 excident, float* pdf) {
-    if (m.type != kEmissive) return {0, 0, 0};
-    const float cosOut = dot(geoN, excident);
-    if (cosOut <= 0.0f) return {0, 0, 0};
-    if (pdf) *pdf = cosOut / kPi;
-    return {m.radiance[0] * m.scale, m.radiance[1] * m.scale, m.radiance[2] * m.scale};
+// evaluate_background (light_sampling.hpp:467-515): COLORED and polar ENVMAP
+BgEval evaluate_background(const OrcPtScene& S, V3 direction) {
+    const OrcBackground& bg = S.background;
+    if (bg.type == 0) return {V3{bg.color[0], bg.color[1], bg.color[2]} * bg.scale, 1.0f / (4.0f * kPi)};
+    const OrcTexture& env = S.textures[bg.envmapTexture];
+    // sample_polar_map / direction_to_uv (textures/interface.hpp:180-200)
+    const float v0 = facos(direction.y) / kPi;
+    const float u = fatan2(direction.z, direction.x) / (kPi * 2.0f);
+    float v = 1.0f - v0;
+    const float halfTexel = 0.5f / (float)env.height;
+    v = clampf(v, halfTexel, 1.0f - halfTexel);
+    const F4 t = tex_sample(env, u, v);
+    V3 radiance{t.x, t.y, t.z};
+    const int W = env.width, H = env.height;
+    const float cdf = bg.sat[(size_t)(H - 1) * W + (W - 1)] / (float)(H * W);
+    float pdfScale = dot(kLumWeight, radiance);
+    const int pixelY = ifloor(v * (float)H);
+    const float sinPixel = fsin(kPi * (float)((float)pixelY + 0.5f) / (float)H);
+    const float sinJac = std::sqrt(1.0f - direction.y * direction.y);
+    pdfScale *= sinPixel / (2.0f * kPi * kPi * sinJac);
+    radiance = radiance * bg.scale;
+    return {radiance, pdfScale / cdf};
 }
 
 // ---------------------------------------------------------------- lights
@@ -331,7 +946,6 @@ struct Nee {
     V3 position;
     float creationPdf;
     V3 dir;
-    float dirPdf;
     float cosOut;
     V3 diffIrradiance;
     float dist;
@@ -362,7 +976,7 @@ struct NodeRec {
 
 Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 pos, float u0, float u1) {
     Nee n;
-    if (type == 0) {  // point (light_sampling.hpp:308-327)
+    if (type == 0) {  // point (light_sampling.hpp:308-327); point lights sit in vacuum
         PointLightRec L;
         std::memcpy(&L, light, sizeof(L));
         const V3 lp{L.position[0], L.position[1], L.position[2]};
@@ -373,7 +987,6 @@ Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 p
         n.position = lp;
         n.creationPdf = kInfPdf;
         n.dir = d;
-        n.dirPdf = 1.0f / (4 * kPi);
         n.cosOut = 0.0f;
         n.diffIrradiance = {L.intensity[0] / distSq, L.intensity[1] / distSq, L.intensity[2] / distSq};
         n.dist = dist;
@@ -394,8 +1007,10 @@ Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 p
     const float s = 1.f - std::sqrt(1.f - u0);
     const float t = (1.f - s) * u1;
     const V3 position = p0 + e1 * s + e2 * t;
-    const OrcMaterial& m = S.materials[L.material];
-    const V3 rad = mat_emission(m, {0, 0, 1}, {0, 0, 1}, nullptr);
+    const float lu = L.uvV[0][0] + L.uvV[1][0] * s + L.uvV[2][0] * t;
+    const float lv = L.uvV[0][1] + L.uvV[1][1] * s + L.uvV[2][1] * t;
+    const MatParams m = fetch(S, S.materials[L.material], lu, lv);
+    const V3 rad = mat_emission(m, {0, 0, 1}, {0, 0, 1});
     const float pdf = 1.0f / area;
     const V3 intensity = rad * area;
     V3 d = position - pos;
@@ -404,25 +1019,46 @@ Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 p
     d = d / dist;
     const float cosOut = dot(normal, -d);
     V3 value{0, 0, 0};
-    float vc = 0.0f, vpdf = 0.0f;
+    float vc = 0.0f;
     if (cosOut > 0.0f) {
         value = intensity;
         vc = cosOut;
-        vpdf = cosOut / kPi;
     }
+    V3 diffIrradiance{value.x / distSq, value.y / distSq, value.z / distSq};
+    diffIrradiance = diffIrradiance * transmission(m.outer, dist);
     n.position = position;
     n.creationPdf = pdf;
     n.dir = d;
-    n.dirPdf = vpdf;
     n.cosOut = vc;
-    n.diffIrradiance = {value.x / distSq, value.y / distSq, value.z / distSq};
+    n.diffIrradiance = diffIrradiance;
     n.dist = dist;
     n.geoNormal = normal;
     n.distSq = distSq;
     return n;
 }
 
-// lights::connect (light_tree_sampling.hpp:195-329), positional lights only
+// connect_light(BackgroundDesc) (light_sampling.hpp:447-462)
+Nee connect_background(const OrcPtScene& S, V3 pos, float u0, float u1) {
+    const EnvDir s = background_sample_dir(S, u0, u1);
+    Nee n;
+    n.position = pos + s.dir * box_diag_len(S.sceneBox);
+    n.creationPdf = s.pdf * std::fabs(1.0f) / (kMaxSceneSize * kMaxSceneSize);
+    n.dir = s.dir;
+    n.cosOut = 1.0f;
+    n.diffIrradiance = s.radiance / s.pdf;
+    n.dist = kMaxSceneSize;
+    n.geoNormal = {0, 0, 0};
+    n.distSq = kMaxSceneSize * kMaxSceneSize;
+    return n;
+}
+
+inline Nee adjust_pdf(Nee n, float p) {  // light_tree_sampling.hpp:16-19
+    n.creationPdf *= p;
+    n.diffIrradiance = n.diffIrradiance / p;
+    return n;
+}
+
+// lights::connect (light_tree_sampling.hpp:195-329); the directional subtree is empty
 Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, float u0, float u1) {
     u64 rndChoice = numIndices > 0 ? seed + index * (~0ull / numIndices) : seed;
     if (rndChoice == ~0ull) --rndChoice;
@@ -431,10 +1067,10 @@ Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, fl
     const float dirProb = S.dirRootFlux / fluxSum;
     const float posProb = S.posRootFlux / fluxSum;
     const u64 rightEnv = percentage_of(~0ull, envProb);
+    if (rndChoice < rightEnv) return adjust_pdf(connect_background(S, pos, u0, u1), envProb);
     u64 right = percentage_of(~0ull, envProb + dirProb);
     u64 left = rightEnv;
-    (void)rightEnv;
-    // (env and directional subtrees are empty in the supported scenes)
+    (void)left;
     left = right;
     right = ~0ull;
     float lightProb = posProb;
@@ -457,13 +1093,10 @@ Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, fl
             lightProb *= (1.0f - probLeft);
         }
     }
-    Nee n = connect_light(S, type, S.lightMem + offset, pos, u0, u1);
-    n.creationPdf *= lightProb;  // adjustPdf (light_tree_sampling.hpp:16-19)
-    n.diffIrradiance = n.diffIrradiance / lightProb;
-    return n;
+    return adjust_pdf(connect_light(S, type, S.lightMem + offset, pos, u0, u1), lightProb);
 }
 
-// light_pdf (light_tree_sampling.hpp:340-390) -> connect pdf of a hit emitter
+// light_pdf (light_tree_sampling.hpp:340-413) -> connect pdf of a hit emitter
 float light_pdf(const OrcPtScene& S, i32 inst, i32 prim) {
     const float treeFlux = S.dirRootFlux + S.posRootFlux + S.bgFlux;
     float p = S.posRootFlux / treeFlux;
@@ -507,6 +1140,12 @@ float light_pdf(const OrcPtScene& S, i32 inst, i32 prim) {
     return p / area;
 }
 
+// background_pdf (light_tree_sampling.hpp:419-423)
+float background_pdf(const OrcPtScene& S, float pdfBack) {
+    const float p = S.bgFlux / (S.dirRootFlux + S.posRootFlux + S.bgFlux);
+    return pdfBack * p;
+}
+
 // ---------------------------------------------------------------- accumulation
 struct PixelAcc {
     float rad[3] = {0, 0, 0};
@@ -542,7 +1181,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                 const u64 x = rng.next();
                 const float nu0 = sample_uniform((u32)x), nu1 = sample_uniform((u32)(x >> 32));
                 const Nee nee = connect(S, (u64)i, (u64)neeCount, neeSeed, vtx.position, nu0, nu1);
-                EvalValue value = mat_evaluate(vtx.ts, *vtx.mat, vtx.incident_dir(), nee.dir);
+                EvalValue value = mat_evaluate(vtx.ts, vtx.mat, vtx.incident_dir(), nee.dir);
                 if (nee.cosOut != 0) value.cosOut *= nee.cosOut;
                 const V3 radiance = value.value * nee.diffIrradiance;
                 if ((radiance.x > 0.0f || radiance.y > 0.0f || radiance.z > 0.0f) && value.cosOut > 0.0f) {
@@ -561,12 +1200,12 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         // walk (random_walk.hpp:38-112)
         const u64 x0 = rng.next();
         const u64 x1 = rng.next();
-        (void)x1;
         const float u0 = sample_uniform((u32)x0), u1 = sample_uniform((u32)(x0 >> 32));
         const float rndRoulette = sample_uniform((u32)rng.next());
         V3 excident, origin = vtx.position;
         float pdfF;
         V3 sThroughput;
+        Medium medium = kVacuum;
         if (vtx.isCamera) {  // pinholecam_sample_ray (pinhole.hpp:56-84)
             const float w = (float)S.width, h = (float)S.height;
             float cx = ((float)px + u0) / w * 2.0f - 1.0f;
@@ -581,11 +1220,12 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             pdfF = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
             sThroughput = {1.0f, 1.0f, 1.0f};
         } else {
-            const PathSample ps = mat_sample(vtx.ts, *vtx.mat, vtx.incident_dir(), u0, u1);
+            const PathSample ps = mat_sample(vtx.ts, vtx.mat, vtx.incident_dir(), u0, u1, x1);
             if (!ps.valid) break;  // CANCEL
             excident = ps.excident;
             pdfF = ps.pdfF;
             sThroughput = ps.throughput;
+            medium = vtx.mat.medium(dot(excident, vtx.ts.geoN));  // path_util.hpp:346-351
         }
         throughput = throughput * sThroughput;
         const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
@@ -593,6 +1233,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         throughput = throughput / cont;
         V3 rayOrigin = origin;
         const HitInfo hit = first_intersection(S, rayOrigin, excident, vtx.geo_normal(), kMaxSceneSize);
+        throughput = throughput * transmission(medium, hit.hit ? hit.t : kMaxSceneSize);
         Vertex next;
         next.isCamera = false;
         if (!hit.hit) {
@@ -605,9 +1246,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             next.incEnc = {excident.x, excident.y, hit.t * sgnf(excident.z)};
             next.inst = hit.inst;
             next.prim = hit.prim;
-            next.sp0 = hit.b0;
-            next.sp1 = hit.b1;
-            next.mat = material_of(S, hit.inst, hit.prim);
+            next.mat = fetch(S, material_of(S, hit.inst, hit.prim), hit.u, hit.v);
             const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
             next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
         }
@@ -619,16 +1258,18 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         ++pathLen;
         if (pathLen >= minPL) {
             V3 emission{0, 0, 0};
-            if (!vtx.isVoid) {
-                const V3 inc = vtx.incident_dir();
-                emission = mat_emission(*vtx.mat, vtx.ts.geoN, -inc, nullptr);
-                if (emission.x != 0.0f || emission.y != 0.0f || emission.z != 0.0f) {
-                    const float connectPdf = light_pdf(S, vtx.inst, vtx.prim);
-                    if (pathLen > 1) {
-                        const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
-                        emission = emission * mis;
-                    }
-                }
+            float connectPdf = 0.0f;
+            if (vtx.isVoid) {  // get_emission, VOID (path_util.hpp:415-424)
+                const BgEval b = evaluate_background(S, vtx.incEnc);
+                emission = b.value;
+                if (nonzero(emission)) connectPdf = background_pdf(S, b.pdfBack);
+            } else {
+                emission = mat_emission(vtx.mat, vtx.ts.geoN, -vtx.incident_dir());
+                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim);
+            }
+            if (nonzero(emission) && pathLen > 1) {
+                const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
+                emission = emission * mis;
             }
             const V3 c = throughput * emission;
             const float cv[3] = {c.x, c.y, c.z};
@@ -636,6 +1277,13 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         }
         if (vtx.isVoid) break;
     } while (pathLen < maxPL);
+}
+
+MatParams params_of(const OrcMaterial& m, const OrcTexture* textures, float u, float v) {
+    OrcPtScene S;
+    std::memset(&S, 0, sizeof(S));
+    S.textures = textures;
+    return fetch(S, m, u, v);
 }
 
 }  // namespace
@@ -676,6 +1324,79 @@ int orc_pt_render(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int3
         }
     }
     if (outNans) *outNans = nans;
+    return 0;
+}
+
+// BSDF of one material in tangent space (shading frame = identity), for the
+// closed-form checks: sample -> (valid, throughput[3], excident[3], pdfF, pdfB)
+// and evaluate -> (value[3], pdfF, pdfB). uv selects texels.
+int orc_bsdf_sample(const OrcMaterial* m, const OrcTexture* textures, float u, float v, const float* inTS, float u0,
+                    float u1, uint64_t i0, float* out9) {
+    const MatParams p = params_of(*m, textures, u, v);
+    const TsSample s = sample_ts(p, {inTS[0], inTS[1], inTS[2]}, u0, u1, i0);
+    out9[0] = s.valid ? 1.0f : 0.0f;
+    out9[1] = s.throughput.x;
+    out9[2] = s.throughput.y;
+    out9[3] = s.throughput.z;
+    out9[4] = s.excident.x;
+    out9[5] = s.excident.y;
+    out9[6] = s.excident.z;
+    out9[7] = s.pdfF;
+    out9[8] = s.pdfB;
+    return 0;
+}
+int orc_bsdf_evaluate(const OrcMaterial* m, const OrcTexture* textures, float u, float v, const float* inTS,
+                      const float* exTS, float* out5) {
+    const MatParams p = params_of(*m, textures, u, v);
+    const TsEval e = evaluate_ts(p, {inTS[0], inTS[1], inTS[2]}, {exTS[0], exTS[1], exTS[2]});
+    out5[0] = e.value.x;
+    out5[1] = e.value.y;
+    out5[2] = e.value.z;
+    out5[3] = e.pdfF;
+    out5[4] = e.pdfB;
+    return 0;
+}
+// eval_ndf and fresnel_dielectric for the closed-form checks.
+float orc_eval_ndf(int32_t ndf, float rx, float ry, const float* h) {
+    MatParams p;
+    p.ndf = ndf;
+    p.rx = rx;
+    p.ry = ry;
+    return eval_ndf(p, {h[0], h[1], h[2]});
+}
+float orc_fresnel_dielectric(float n_i, float n_t, float cosIAbs, float* cosTAbs) {
+    const Refraction r = fresnel_dielectric(n_i, n_t, cosIAbs);
+    if (cosTAbs) *cosTAbs = r.cosTAbs;
+    return r.f;
+}
+// Texture sampling (CPU semantics).
+int orc_texture_sample(const OrcTexture* t, float u, float v, float* out4) {
+    const F4 s = tex_sample(*t, u, v);
+    out4[0] = s.x;
+    out4[1] = s.y;
+    out4[2] = s.z;
+    out4[3] = s.w;
+    return 0;
+}
+// Background: sample a direction (dir[3], pdf, radiance[3]) and evaluate one
+// (radiance[3], pdf.back). Needs only S->background, S->textures, S->sceneBox.
+int orc_background_sample(const OrcPtScene* S, float u0, float u1, float* out7) {
+    const EnvDir d = background_sample_dir(*S, u0, u1);
+    out7[0] = d.dir.x;
+    out7[1] = d.dir.y;
+    out7[2] = d.dir.z;
+    out7[3] = d.pdf;
+    out7[4] = d.radiance.x;
+    out7[5] = d.radiance.y;
+    out7[6] = d.radiance.z;
+    return 0;
+}
+int orc_background_evaluate(const OrcPtScene* S, const float* dir, float* out4) {
+    const BgEval b = evaluate_background(*S, {dir[0], dir[1], dir[2]});
+    out4[0] = b.value.x;
+    out4[1] = b.value.y;
+    out4[2] = b.value.z;
+    out4[3] = b.pdfBack;
     return 0;
 }
 

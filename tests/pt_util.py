@@ -109,3 +109,52 @@ def oracle_pt_scene_instanced(O, c, light_tree, scene_box):
     uvs = np.concatenate([L["uvs"] for L in c["lods"]])
     mat_idx = np.concatenate([L["mat_idx"] for L in c["lods"]])
     return O.PtScene(scene, normals, uvs, mat_idx, c["materials"], light_tree, scene_box, c["camera"])
+
+
+def background_of(c, kind):
+    """'none' (black), 'colored' or 'envmap' background record for scene c."""
+    if kind == "envmap":
+        return hip.background_record(hip.BACKGROUND_ENVMAP, scale=0.5, envmap_texture=c["envmap"])
+    if kind == "colored":
+        return hip.background_record(hip.BACKGROUND_COLORED, color=(0.3, 0.4, 0.6), scale=0.8)
+    return hip.background_record(hip.BACKGROUND_COLORED)
+
+
+def gpu_pt_materials(ctx, c, background="envmap", seed=1, max_path_length=8, nee_count=1):
+    ids = np.array([ctx.texture_create(t, s) for t, s in c["textures"]], np.int32)
+    mats = c["materials"].copy()
+    for f in ("albedoTexture", "roughnessTexture"):
+        used = mats[f] >= 0
+        mats[f][used] = ids[mats[f][used]]
+    lod = ctx.lod_create(c["verts"], c["tris"])
+    ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
+    ctx.lbvh_build(lod, scene_box(c))
+    sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+    pt = ctx.pt_create(sc, mats, c["camera"], dict(seed=seed, max_path_length=max_path_length, nee_count=nee_count),
+                       point_lights=c["point_lights"])
+    bg = background_of(c, background)
+    dev_bg = bg.copy()
+    if int(bg["type"][0]) == hip.BACKGROUND_ENVMAP:
+        dev_bg["envmapTexture"] = ids[c["envmap"]]
+    ctx.pt_set_background(pt, dev_bg)
+    return pt, sc, bg
+
+
+def oracle_pt_scene_materials(O, c, background, light_tree=None, box=None):
+    """Oracle scene of cornell_materials; without light_tree the host builders
+    make the light tree (area + point lights) and the background."""
+    v, t = c["verts"], c["tris"]
+    scene, _ = U.oracle_scene(O, [(v, t)], np.array([0], np.int32), S.IDENTITY34.reshape(1, 12))
+    box = scene_box(c) if box is None else box
+    if light_tree is None:
+        al = S.emissive_area_lights(v, c["uvs"], t, c["mat_idx"], c["materials"])
+        light_tree = hip.light_tree_build(al, c["point_lights"], box, c["materials"])
+    bg = background if isinstance(background, np.ndarray) else background_of(c, background)
+    env = c["textures"][c["envmap"]][0] if int(bg["type"][0]) == hip.BACKGROUND_ENVMAP else None
+    flux, sat = hip.background_build(bg, env, box)
+    info = light_tree[0].copy()
+    info["backgroundFlux"] = np.float32(np.float32(flux[0] + flux[1]) + flux[2])
+    lt = (info,) + tuple(light_tree[1:])
+    tx = O.Textures(c["textures"])
+    return O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], lt, box, c["camera"],
+                     textures=tx, background=bg[0], background_sat=sat, background_flux=flux)

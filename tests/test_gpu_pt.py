@@ -8,6 +8,7 @@ import pytest
 
 import pt_util as P
 from mufflon_amd import hip
+from mufflon_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
 
@@ -87,3 +88,28 @@ def test_pt_instanced_scene(hip_ctx, oracle):
           (np.array_equal(g_rad.view(np.uint32), o_rad.view(np.uint32)), float(diff.max())), flush=True)
     assert not (diff > RTOL * np.abs(o_rad) + ATOL).any()
     assert g_rad.mean() > 0.01
+
+
+@pytest.mark.parametrize("background", ["none", "colored", "envmap"])
+def test_pt_material_zoo(hip_ctx, oracle, background):
+    """TORRANCE (GGX V-cavity with a roughness texture, anisotropic GGX Smith,
+    Beckmann), absorbing MICROFACET glass, a linear albedo texture, point
+    lights, the ceiling light and each background, against the oracle."""
+    c = S.cornell_materials(16, 32, 48, 48)
+    print("pt parity material zoo, background %s" % background, flush=True)
+    pt, sc, bg = P.gpu_pt_materials(hip_ctx, c, background)
+    hip_ctx.pt_iterate(pt, 6)
+    g_rad = hip_ctx.pt_target(pt, hip.TARGET_RADIANCE)
+    g_hit = hip_ctx.pt_target(pt, hip.TARGET_HIT_ID)
+    box, _, _, _ = hip_ctx.scene_get(sc, 1)
+    lt = hip_ctx.pt_light_tree(pt)
+    ps = P.oracle_pt_scene_materials(oracle, c, bg, light_tree=lt, box=box)
+    o_rad, o_hit, o_nans = oracle.pt_render(ps, 1, 0, 8, 1, 6)
+    assert np.array_equal(g_hit, o_hit)
+    diff = np.abs(g_rad - o_rad)
+    bad = diff > RTOL * np.abs(o_rad) + ATOL
+    print("material zoo %s: bit-exact=%s, pixels out of tolerance=%d, max abs diff=%.3g" %
+          (background, np.array_equal(g_rad.view(np.uint32), o_rad.view(np.uint32)), int(bad.any(axis=2).sum()),
+           float(diff.max())), flush=True)
+    assert not bad.any()
+    assert hip_ctx.pt_nan_count(pt) == o_nans

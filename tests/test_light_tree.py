@@ -139,3 +139,16 @@ def test_oracle_pt_instanced_renders(oracle):
     ps = P.oracle_pt_scene_instanced(oracle, c, lt, box)
     rad, hit, nans = oracle.pt_render(ps, 1, 0, 6, 1, 2)
     assert nans == 0 and rad.mean() > 0.01 and (hit[..., 0] == 6).any()
+
+
+@pytest.mark.parametrize("background", ["none", "colored", "envmap"])
+def test_oracle_pt_material_zoo(oracle, background):
+    """Every material kind, textures, point lights and each background render
+    without NaNs, and a lit background adds energy."""
+    c = S.cornell_materials(8, 16, 24, 24)
+    ps = P.oracle_pt_scene_materials(oracle, c, background)
+    rad, hit, nans = oracle.pt_render(ps, 1, 0, 8, 1, 2)
+    assert nans == 0 and np.isfinite(rad).all() and rad.mean() > 0.01
+    if background != "none":
+        base, _, _ = oracle.pt_render(P.oracle_pt_scene_materials(oracle, c, "none"), 1, 0, 8, 1, 2)
+        assert rad.sum() > base.sum()
