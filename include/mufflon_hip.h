@@ -187,8 +187,18 @@ typedef struct mfh_pt_params {
     int32_t neeCount;      /* "NEE count", default 1 */
 } mfh_pt_params;
 
-#define MFH_TARGET_RADIANCE 0 /* 3 floats per pixel, row-major */
-#define MFH_TARGET_HIT_ID 1   /* 2 floats per pixel: instance, primitive */
+/* Render targets of the pt renderer (PtTargets, pt_params.hpp:16-26;
+ * render_targets.hpp:13-74), running means per pixel, row-major. Radiance
+ * and Hit ID are enabled when a path tracer is created. */
+#define MFH_TARGET_RADIANCE 0  /* 3 floats per pixel */
+#define MFH_TARGET_HIT_ID 1    /* 2 floats per pixel: instance, primitive */
+#define MFH_TARGET_POSITION 2  /* 3 floats, guide-weighted */
+#define MFH_TARGET_DEPTH 3     /* 1 float */
+#define MFH_TARGET_NORMAL 4    /* 3 floats (shading normal) */
+#define MFH_TARGET_UV 5        /* 2 floats */
+#define MFH_TARGET_ALBEDO 6    /* 3 floats */
+#define MFH_TARGET_LIGHTNESS 7 /* 1 float */
+#define MFH_TARGET_COUNT 8
 
 /* Light tree as built on the host (light_tree.cpp:340-447), Mufflon layout. */
 typedef struct mfh_light_tree_info {
@@ -240,7 +250,10 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
 int mfh_pt_reset(mfh_context* ctx, int32_t ptId);
 /* Run iterations (render_iterate, interface.cpp:3611-3684); asynchronous. */
 int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations);
-/* Running mean of a target over the completed iterations (mufflon_get_target_image). */
+/* Enable or disable a render target (render_enable_render_target /
+ * render_disable_render_target, interface.cpp:3876-3940); resets the path tracer. */
+int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable);
+/* Running mean of an enabled target over the completed iterations (mufflon_get_target_image). */
 int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out);
 /* Index of the last completed iteration, -1 after a reset (render_get_current_iteration). */
 int mfh_pt_get_iteration(mfh_context* ctx, int32_t ptId, int32_t* outIteration);

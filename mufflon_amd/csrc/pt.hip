@@ -98,15 +98,24 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
     int* const lstack = sStack + threadIdx.x;
     u64 rng = S.rngs[pixel];
 
-    float accR = 0.0f, accG = 0.0f, accB = 0.0f, accH0 = 0.0f, accH1 = 0.0f;
+    // per-iteration target values: Radiance 0-2, Hit ID 3-4, Position 5-7,
+    // Depth 8, Normal 9-11, UV 12-13, Albedo 14-16, Lightness 17
+    float acc[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) acc[k] = 0.0f;
+    const u32 mask = S.targetMask;
     u32 nans = 0;
-    auto add = [&](float& dst, float v) {
+    // RenderBuffer::contribute + check_nan (render_target.hpp:62-76,189-195)
+    auto add = [&](int target, int slot, float v) {
+        if (!((mask >> target) & 1u)) return;
         if (v != v) {
             ++nans;
             v = 0.0f;
         }
-        dst += v;
+        acc[slot] += v;
     };
+    float guideWeight = 1.0f;
+    float footA = 0.0f;  // FootprintV0::m_a (footprint.hpp:137-170)
 
     const V3 viewDir = v3(S.camDir[0], S.camDir[1], S.camDir[2]);
     const V3 up = v3(S.camUp[0], S.camUp[1], S.camUp[2]);
@@ -145,9 +154,10 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
                         const float hitPdf = vPdf * fabsf(nee.cosOut) / nee.distSq;
                         const float mis = 1.0f / ((float)S.neeCount + hitPdf / nee.creationPdf);
                         const V3 cc = throughput * vCos * radiance * mis;
-                        add(accR, cc.x);
-                        add(accG, cc.y);
-                        add(accB, cc.z);
+                        add(0, 0, cc.x);
+                        add(0, 1, cc.y);
+                        add(0, 2, cc.z);
+                        add(7, 17, guideWeight * vCos);
                     }
                 }
             }
@@ -188,6 +198,10 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
             trace_ray<false, false, kSingle, kPtThreads>(S.trace, rayOrigin, excident, kMaxSceneSize, lstack);
         const bool isVoid = hit.inst == -1;
         throughput = throughput * transmission(medium, isVoid ? kMaxSceneSize : hit.t);
+        // PtVertexExt::update (pt_common.hpp:45-61): footprint and guide weight
+        footA += 1.0f / fsqrt(pdfF);
+        guideWeight = 1.f / (1.f + footA * footA / 1.0f);
+        float hu = 0.0f, hv = 0.0f;
         hInst = hit.inst;
         hPrim = hit.prim;
         float incidentPdf;
@@ -196,7 +210,6 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
             incidentPdf = pdfF * 1.0f;
         } else {
             position = rayOrigin + excident * hit.t;
-            float hu, hv;
             ts = hit_frame(S, hit, hu, hv);
             incEnc = v3(excident.x, excident.y, hit.t * sgnf(excident.z));
             mat = fetch(S, material_index(S, hit.inst, hit.prim), hu, hv);
@@ -205,8 +218,8 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
         }
         isCamera = false;
         if (pathLen == 0) {
-            add(accH0, (float)hInst);
-            add(accH1, (float)hPrim);
+            add(1, 3, (float)hInst);
+            add(1, 4, (float)hPrim);
         }
         ++pathLen;
         if (pathLen >= S.minPathLength) {
@@ -226,24 +239,53 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
                 const float mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / incidentPdf));
                 em = em * mis;
             }
-            const V3 cc = throughput * em;  // contribute (render_target.hpp:62-76) even when zero
-            add(accR, cc.x);
-            add(accG, cc.y);
-            add(accB, cc.z);
+            const V3 cc = throughput * em;  // contributed even when zero
+            add(0, 0, cc.x);
+            add(0, 1, cc.y);
+            add(0, 2, cc.z);
+            if (mask & 0xFCu) {  // Position .. Lightness
+                const V3 pos = isVoid ? v3(0, 0, 0) : position;
+                add(2, 5, guideWeight * pos.x);
+                add(2, 6, guideWeight * pos.y);
+                add(2, 7, guideWeight * pos.z);
+                add(3, 8, guideWeight * (isVoid ? 0.0f : fabsf(incEnc.z)));
+                const V3 nrm = isVoid ? v3(0, 0, 0) : ts.shadingN;
+                add(4, 9, guideWeight * nrm.x);
+                add(4, 10, guideWeight * nrm.y);
+                add(4, 11, guideWeight * nrm.z);
+                add(5, 12, guideWeight * hu);
+                add(5, 13, guideWeight * hv);
+                V3 alb = v3(0, 0, 0);
+                if (!isVoid) {
+                    if (mat.type == kMatLambert || mat.type == kMatTorrance)
+                        alb = mat.albedo;
+                    else if (mat.type == kMatMicrofacet)
+                        alb = v3(1.0f / (1.0f + mat.inner.absorption.x), 1.0f / (1.0f + mat.inner.absorption.y),
+                                 1.0f / (1.0f + mat.inner.absorption.z));
+                }
+                add(6, 14, guideWeight * alb.x);
+                add(6, 15, guideWeight * alb.y);
+                add(6, 16, guideWeight * alb.z);
+                add(7, 17, guideWeight * ((em.x + em.y + em.z) / 3.0f));
+            }
         }
         if (isVoid) break;
     } while (pathLen < S.maxPathLength);
 
     S.rngs[pixel] = rng;
-    // running mean (output_handler_variance.hpp:7-31)
+    // running means (output_handler_variance.hpp:7-31)
     const float denom = mx(1.0f, (float)S.iteration + 1);
-    float* cr = S.cumRadiance + 3 * pixel;
-    cr[0] += (accR - cr[0]) / denom;
-    cr[1] += (accG - cr[1]) / denom;
-    cr[2] += (accB - cr[2]) / denom;
-    float* ch = S.cumHitId + 2 * pixel;
-    ch[0] += (accH0 - ch[0]) / denom;
-    ch[1] += (accH1 - ch[1]) / denom;
+    int slot = 0;
+#pragma unroll
+    for (int t = 0; t < kNumTargets; ++t) {
+        const int nc = kTargetChannels[t];
+        if ((mask >> t) & 1u) {
+            float* c = S.cum[t] + (size_t)nc * pixel;
+#pragma unroll
+            for (int k = 0; k < nc; ++k) c[k] += (acc[slot + k] - c[k]) / denom;
+        }
+        slot += nc;
+    }
     if (nans) atomicAdd(S.nanCounter, nans);
 }
 

@@ -21,6 +21,10 @@ constexpr int kNdfBeckmann = 1;
 constexpr int kShadowingVCavity = 0;
 constexpr int kShadowingSmith = 1;
 
+// Render targets: Radiance, Hit ID, Position, Depth, Normal, UV, Albedo, Lightness
+constexpr int kNumTargets = 8;
+constexpr int kTargetChannels[kNumTargets] = {3, 2, 3, 1, 3, 2, 3, 1};
+
 constexpr uint16_t kLightPoint = 0;
 constexpr uint16_t kLightAreaTriangle = 2;
 constexpr uint16_t kLightInternalNode = 0xFFFF;
@@ -96,8 +100,8 @@ struct PtScene {
     int32_t seed, minPathLength, maxPathLength, neeCount;
     // per-pixel state and targets
     uint64_t* rngs;
-    float* cumRadiance;  // 3 per pixel, running mean
-    float* cumHitId;     // 2 per pixel, running mean
+    float* cum[kNumTargets];  // running means of the enabled targets (null when disabled)
+    uint32_t targetMask;
     uint32_t* nanCounter;
     int32_t iteration;   // index of the iteration being run (0 after a reset)
 };

@@ -18,7 +18,9 @@ struct PathTracer {
     LightTreeHost tree;
     BackgroundHost background;
     DevBuf dMaterials, dPtLods, dLightMem, dMapKeys, dMapCodes, dTextures, dSat;
-    DevBuf dRngs, dRadiance, dHitId, dNan;
+    DevBuf dRngs, dNan;
+    DevBuf dTargets[kNumTargets];
+    uint32_t targetMask = (1u << MFH_TARGET_RADIANCE) | (1u << MFH_TARGET_HIT_ID);
     int32_t iteration = -1;
     uint64_t sceneGeneration = ~0ull;
 };
@@ -27,8 +29,9 @@ void destroy_tracers(Context& c) {
     for (PathTracer* t : c.tracers) {
         if (!t) continue;
         for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dTextures,
-                          &t->dSat, &t->dRngs, &t->dRadiance, &t->dHitId, &t->dNan})
+                          &t->dSat, &t->dRngs, &t->dNan})
             b->release();
+        for (DevBuf& b : t->dTargets) b.release();
         delete t;
     }
     c.tracers.clear();
@@ -196,8 +199,8 @@ PtScene device_scene(Context& c, PathTracer& t) {
     S.maxPathLength = t.params.maxPathLength;
     S.neeCount = t.params.neeCount;
     S.rngs = t.dRngs.as<uint64_t>();
-    S.cumRadiance = t.dRadiance.as<float>();
-    S.cumHitId = t.dHitId.as<float>();
+    for (int k = 0; k < kNumTargets; ++k) S.cum[k] = (t.targetMask >> k) & 1u ? t.dTargets[k].as<float>() : nullptr;
+    S.targetMask = t.targetMask;
     S.nanCounter = t.dNan.as<uint32_t>();
     return S;
 }
@@ -205,8 +208,11 @@ PtScene device_scene(Context& c, PathTracer& t) {
 void reset(Context& c, PathTracer& t) {
     const int num = t.camera.width * t.camera.height;
     launch_pt_init_rng(t.dRngs.as<uint64_t>(), num, t.params.seed * (num + 1), c.stream);
-    MFH_HIP(hipMemsetAsync(t.dRadiance.ptr, 0, 12 * (size_t)num, c.stream));
-    MFH_HIP(hipMemsetAsync(t.dHitId.ptr, 0, 8 * (size_t)num, c.stream));
+    for (int k = 0; k < kNumTargets; ++k) {
+        if (!((t.targetMask >> k) & 1u)) continue;
+        t.dTargets[k].reserve(4 * (size_t)kTargetChannels[k] * num);
+        MFH_HIP(hipMemsetAsync(t.dTargets[k].ptr, 0, 4 * (size_t)kTargetChannels[k] * num, c.stream));
+    }
     MFH_HIP(hipMemsetAsync(t.dNan.ptr, 0, 4, c.stream));
     t.iteration = -1;
 }
@@ -281,15 +287,14 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
             }
             const size_t num = (size_t)camera->width * camera->height;
             t->dRngs.reserve(8 * num);
-            t->dRadiance.reserve(12 * num);
-            t->dHitId.reserve(8 * num);
             t->dNan.reserve(4);
             reset(c, *t);
             MFH_HIP(hipStreamSynchronize(c.stream));
         } catch (...) {
             for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dTextures,
-                              &t->dSat, &t->dRngs, &t->dRadiance, &t->dHitId, &t->dNan})
+                              &t->dSat, &t->dRngs, &t->dNan})
                 b->release();
+            for (DevBuf& b : t->dTargets) b.release();
             delete t;
             throw;
         }
@@ -326,12 +331,25 @@ int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out
         MFH_REQUIRE(out != nullptr, "null output pointer");
         const size_t num = (size_t)t.camera.width * t.camera.height;
         MFH_HIP(hipStreamSynchronize(c.stream));
-        if (target == MFH_TARGET_RADIANCE)
-            MFH_HIP(hipMemcpy(out, t.dRadiance.ptr, 12 * num, hipMemcpyDeviceToHost));
-        else if (target == MFH_TARGET_HIT_ID)
-            MFH_HIP(hipMemcpy(out, t.dHitId.ptr, 8 * num, hipMemcpyDeviceToHost));
-        else
-            throw Error(kInvalidArgument, "unknown render target " + std::to_string(target));
+        MFH_REQUIRE(target >= 0 && target < kNumTargets, "unknown render target " + std::to_string(target));
+        MFH_REQUIRE((t.targetMask >> target) & 1u, "render target " + std::to_string(target) + " is not enabled");
+        MFH_HIP(hipMemcpy(out, t.dTargets[target].ptr, 4 * (size_t)kTargetChannels[target] * num,
+                          hipMemcpyDeviceToHost));
+    });
+}
+
+int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(target >= 0 && target < kNumTargets, "unknown render target " + std::to_string(target));
+        const uint32_t bit = 1u << target;
+        const uint32_t mask = enable ? (t.targetMask | bit) : (t.targetMask & ~bit);
+        if (mask == t.targetMask) return;
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        t.targetMask = mask;
+        if (!enable) t.dTargets[target].release();
+        reset(c, t);
     });
 }
 

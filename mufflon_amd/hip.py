@@ -43,6 +43,13 @@ LIGHT_TREE_INFO_DTYPE = np.dtype([("posRootFlux", "<f4"), ("posRootType", "<u4")
                                   ("dirRootFlux", "<f4"), ("backgroundFlux", "<f4")])
 TARGET_RADIANCE = 0
 TARGET_HIT_ID = 1
+TARGET_POSITION = 2
+TARGET_DEPTH = 3
+TARGET_NORMAL = 4
+TARGET_UV = 5
+TARGET_ALBEDO = 6
+TARGET_LIGHTNESS = 7
+TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)
 
 # name -> argtypes; every function returns int status
 _SIGS = {
@@ -67,6 +74,7 @@ _SIGS = {
     "mfh_pt_reset": [vp, i32],
     "mfh_pt_iterate": [vp, i32, i32],
     "mfh_pt_get_target": [vp, i32, i32, f32p],
+    "mfh_pt_enable_target": [vp, i32, i32, i32],
     "mfh_pt_get_iteration": [vp, i32, i32p],
     "mfh_pt_get_nan_count": [vp, i32, u32p],
     "mfh_pt_get_light_tree": [vp, i32, vp, vp, u64p, u32p],
@@ -400,9 +408,12 @@ class HipContext:
     def pt_iterate(self, pt, iterations=1):
         self._call("mfh_pt_iterate", self.handle, pt, int(iterations))
 
+    def pt_enable_target(self, pt, target, enable=True):
+        self._call("mfh_pt_enable_target", self.handle, pt, target, 1 if enable else 0)
+
     def pt_target(self, pt, target=TARGET_RADIANCE):
         w, h = self._pt_size[pt]
-        ch = 3 if target == TARGET_RADIANCE else 2
+        ch = TARGET_CHANNELS[target]
         out = np.zeros(w * h * ch, np.float32)
         self._call("mfh_pt_get_target", self.handle, pt, target, _ptr(out, f32p))
         return out.reshape(h, w, ch)

@@ -433,3 +433,29 @@ def pt_render(pt_scene, seed, min_path_length, max_path_length, nee_count, itera
     lib.orc_pt_render(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
                       _p(rad, f32p), _p(hid, f32p), C.byref(nans))
     return rad.reshape(h, w, 3), hid.reshape(h, w, 2), nans.value
+
+
+TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)  # Radiance, Hit ID, Position, Depth, Normal, UV, Albedo, Lightness
+
+
+def pt_render_targets(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, targets):
+    """Oracle path tracer: {target: running mean (h, w, channels)} for the
+    requested target ids, and the NaN count."""
+    lib = load()
+    if not hasattr(lib, "_ptt_ready"):
+        lib.orc_pt_render_targets.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_int32, C.c_uint32, C.POINTER(f32p), u32p]
+        lib.orc_pt_render_targets.restype = C.c_int
+        lib._ptt_ready = True
+    w, h = pt_scene.s.width, pt_scene.s.height
+    mask = 0
+    bufs = {}
+    ptrs = (f32p * 8)()
+    for t in targets:
+        mask |= 1 << t
+        bufs[t] = np.zeros(w * h * TARGET_CHANNELS[t], np.float32)
+        ptrs[t] = _p(bufs[t], f32p)
+    nans = C.c_uint32()
+    lib.orc_pt_render_targets(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
+                              mask, ptrs, C.byref(nans))
+    return {t: b.reshape(h, w, TARGET_CHANNELS[t]) for t, b in bufs.items()}, nans.value

@@ -1147,26 +1147,50 @@ float background_pdf(const OrcPtScene& S, float pdfBack) {
 }
 
 // ---------------------------------------------------------------- accumulation
+// Targets of PtTargets (pt_params.hpp:16-26) in this library's numbering:
+// Radiance 0 (3), Hit ID 1 (2), Position 2 (3), Depth 3 (1), Normal 4 (3),
+// UV 5 (2), Albedo 6 (3), Lightness 7 (1).
+constexpr int kNumTargets = 8;
+constexpr int kTargetChannels[kNumTargets] = {3, 2, 3, 1, 3, 2, 3, 1};
+constexpr int kTargetOffset[kNumTargets] = {0, 3, 5, 8, 9, 12, 14, 17};
 struct PixelAcc {
-    float rad[3] = {0, 0, 0};
-    float hit[2] = {0, 0};
+    u32 mask = 1u;
+    float v[18] = {};
     u32 nans = 0;
-    void add(float* dst, const float* v, int c) {
-        for (int k = 0; k < c; ++k) {
-            float x = v[k];
-            if (std::isnan(x)) {
+    // RenderBuffer::contribute with check_nan (render_target.hpp:62-76,189-195)
+    void add(int target, const float* x) {
+        if (!(mask & (1u << target))) return;
+        for (int k = 0; k < kTargetChannels[target]; ++k) {
+            float y = x[k];
+            if (std::isnan(y)) {
                 ++nans;
-                x = 0.0f;
+                y = 0.0f;
             }
-            dst[k] += x;
+            v[kTargetOffset[target] + k] += y;
         }
     }
+    void add3(int target, V3 a) {
+        const float x[3] = {a.x, a.y, a.z};
+        add(target, x);
+    }
+    void add1(int target, float a) { add(target, &a); }
 };
+
+// materials::albedo (lambert, microfacet_specular, microfacet_full, emissive)
+V3 mat_albedo(const MatParams& m) {
+    if (m.type == kLambert || m.type == kTorrance) return m.albedo;
+    if (m.type == kMicrofacet)
+        return {1.0f / (1.0f + m.inner.absorption.x), 1.0f / (1.0f + m.inner.absorption.y),
+                1.0f / (1.0f + m.inner.absorption.z)};
+    return {0, 0, 0};
+}
 
 // pt_sample (pt_common.hpp:75-156)
 void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& rng, PixelAcc& acc) {
     const int minPL = params[1], maxPL = params[2], neeCount = params[3];
     V3 throughput{1.0f, 1.0f, 1.0f};
+    float guideWeight = 1.0f;
+    float footA = 0.0f;  // FootprintV0::m_a (footprint.hpp:137-170); m_P stays 1
     Vertex vtx;
     vtx.position = {S.camPos[0], S.camPos[1], S.camPos[2]};
     vtx.incidentPdf = 1.0f;  // mis_start_pdf(inf, inf, 1)
@@ -1190,9 +1214,8 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                     if (!occluded) {
                         const float hitPdf = value.pdfF * std::fabs(nee.cosOut) / nee.distSq;
                         const float mis = 1.0f / ((float)neeCount + hitPdf / nee.creationPdf);
-                        const V3 c = throughput * value.cosOut * radiance * mis;
-                        const float cv[3] = {c.x, c.y, c.z};
-                        acc.add(acc.rad, cv, 3);
+                        acc.add3(0, throughput * value.cosOut * radiance * mis);
+                        acc.add1(7, guideWeight * value.cosOut);
                     }
                 }
             }
@@ -1233,7 +1256,13 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         throughput = throughput / cont;
         V3 rayOrigin = origin;
         const HitInfo hit = first_intersection(S, rayOrigin, excident, vtx.geo_normal(), kMaxSceneSize);
-        throughput = throughput * transmission(medium, hit.hit ? hit.t : kMaxSceneSize);
+        const float hitDist = hit.hit ? hit.t : kMaxSceneSize;
+        throughput = throughput * transmission(medium, hitDist);
+        // PtVertexExt::update (pt_common.hpp:45-61): FootprintV0::add_segment of a
+        // perspective segment, guide weight 1 / (1 + solid angle)
+        footA += 1.0f / std::sqrt(pdfF);
+        guideWeight = 1.f / (1.f + footA * footA / 1.0f);
+        float hu = 0.0f, hv = 0.0f;
         Vertex next;
         next.isCamera = false;
         if (!hit.hit) {
@@ -1247,13 +1276,15 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             next.inst = hit.inst;
             next.prim = hit.prim;
             next.mat = fetch(S, material_of(S, hit.inst, hit.prim), hit.u, hit.v);
+            hu = hit.u;
+            hv = hit.v;
             const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
             next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
         }
         vtx = next;
         if (pathLen == 0) {
-            const float hv[2] = {(float)vtx.inst, (float)vtx.prim};
-            acc.add(acc.hit, hv, 2);
+            const float hid[2] = {(float)vtx.inst, (float)vtx.prim};
+            acc.add(1, hid);
         }
         ++pathLen;
         if (pathLen >= minPL) {
@@ -1271,9 +1302,15 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                 const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
                 emission = emission * mis;
             }
-            const V3 c = throughput * emission;
-            const float cv[3] = {c.x, c.y, c.z};
-            acc.add(acc.rad, cv, 3);
+            acc.add3(0, throughput * emission);
+            const V3 pos = vtx.isVoid ? V3{0, 0, 0} : vtx.position;
+            acc.add3(2, guideWeight * pos);
+            acc.add1(3, guideWeight * (vtx.isVoid ? 0.0f : std::fabs(vtx.incEnc.z)));
+            acc.add3(4, guideWeight * (vtx.isVoid ? V3{0, 0, 0} : vtx.ts.shadingN));
+            const float uvc[2] = {guideWeight * hu, guideWeight * hv};
+            acc.add(5, uvc);
+            acc.add3(6, guideWeight * (vtx.isVoid ? V3{0, 0, 0} : mat_albedo(vtx.mat)));
+            acc.add1(7, guideWeight * ((emission.x + emission.y + emission.z) / 3.0f));
         }
         if (vtx.isVoid) break;
     } while (pathLen < maxPL);
@@ -1291,40 +1328,49 @@ MatParams params_of(const OrcMaterial& m, const OrcTexture* textures, float u, f
 extern "C" {
 
 // Runs `iterations` iterations from a reset (fresh per-pixel RNGs) and returns
-// the running means of the Radiance (3 floats / pixel) and Hit ID (2 floats /
-// pixel) targets, row-major, plus the number of NaN contributions dropped.
-int orc_pt_render(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength, int32_t neeCount,
-                  int32_t iterations, float* outRadiance, float* outHitId, uint32_t* outNans) {
+// the running means of the enabled targets (targetMask bit t -> outTargets[t],
+// kTargetChannels[t] floats per pixel, row-major) plus the number of NaN
+// contributions dropped.
+int orc_pt_render_targets(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
+                          int32_t neeCount, int32_t iterations, uint32_t targetMask, float** outTargets,
+                          uint32_t* outNans) {
     const int W = S->width, H = S->height;
     const int num = W * H;
     std::vector<Pcg64> rngs;
     rngs.reserve(num);
     const int s = seed * (num + 1);  // cpu_pt.cpp:40-45
     for (int i = 0; i < num; ++i) rngs.emplace_back((u32)(i + s));
-    std::memset(outRadiance, 0, sizeof(float) * 3 * (size_t)num);
-    if (outHitId) std::memset(outHitId, 0, sizeof(float) * 2 * (size_t)num);
+    for (int t = 0; t < kNumTargets; ++t)
+        if (targetMask & (1u << t)) std::memset(outTargets[t], 0, sizeof(float) * kTargetChannels[t] * (size_t)num);
     const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
     u32 nans = 0;
     for (int it = 0; it < iterations; ++it) {
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : nans)
         for (int pixel = 0; pixel < num; ++pixel) {
             PixelAcc acc;
+            acc.mask = targetMask;
             pt_sample(*S, params, pixel % W, pixel / W, rngs[pixel], acc);
             nans += acc.nans;
             const float denom = mx(1.0f, (float)it + 1);
-            for (int c = 0; c < 3; ++c) {
-                float& cum = outRadiance[3 * (size_t)pixel + c];
-                cum += (acc.rad[c] - cum) / denom;
-            }
-            if (outHitId)
-                for (int c = 0; c < 2; ++c) {
-                    float& cum = outHitId[2 * (size_t)pixel + c];
-                    cum += (acc.hit[c] - cum) / denom;
+            for (int t = 0; t < kNumTargets; ++t) {
+                if (!(targetMask & (1u << t))) continue;
+                for (int c = 0; c < kTargetChannels[t]; ++c) {
+                    float& cum = outTargets[t][(size_t)kTargetChannels[t] * pixel + c];
+                    cum += (acc.v[kTargetOffset[t] + c] - cum) / denom;
                 }
+            }
         }
     }
     if (outNans) *outNans = nans;
     return 0;
+}
+
+// Radiance and Hit ID only.
+int orc_pt_render(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength, int32_t neeCount,
+                  int32_t iterations, float* outRadiance, float* outHitId, uint32_t* outNans) {
+    float* t[kNumTargets] = {outRadiance, outHitId, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return orc_pt_render_targets(S, seed, minPathLength, maxPathLength, neeCount, iterations, outHitId ? 3u : 1u, t,
+                                 outNans);
 }
 
 // BSDF of one material in tangent space (shading frame = identity), for the

@@ -113,3 +113,28 @@ def test_pt_material_zoo(hip_ctx, oracle, background):
            float(diff.max())), flush=True)
     assert not bad.any()
     assert hip_ctx.pt_nan_count(pt) == o_nans
+
+
+def test_pt_all_targets(hip_ctx, oracle):
+    """Every PtTargets target (Radiance, Hit ID, Position, Depth, Normal, UV,
+    Albedo, Lightness) against the oracle, with textures and an env map."""
+    c = S.cornell_materials(12, 24, 40, 40)
+    print("pt parity all targets", flush=True)
+    pt, sc, bg = P.gpu_pt_materials(hip_ctx, c, "envmap")
+    for t in range(8):
+        hip_ctx.pt_enable_target(pt, t, True)
+    hip_ctx.pt_iterate(pt, 4)
+    box, _, _, _ = hip_ctx.scene_get(sc, 1)
+    ps = P.oracle_pt_scene_materials(oracle, c, bg, light_tree=hip_ctx.pt_light_tree(pt), box=box)
+    ref, nans = oracle.pt_render_targets(ps, 1, 0, 8, 1, 4, list(range(8)))
+    for t in range(8):
+        g = hip_ctx.pt_target(pt, t)
+        o = ref[t]
+        bad = np.abs(g - o) > RTOL * np.abs(o) + ATOL
+        print("target %d: bit-exact=%s, out of tolerance=%d" %
+              (t, np.array_equal(g.view(np.uint32), o.view(np.uint32)), int(bad.sum())), flush=True)
+        assert not bad.any(), t
+    assert hip_ctx.pt_nan_count(pt) == nans
+    hip_ctx.pt_enable_target(pt, hip.TARGET_DEPTH, False)
+    with pytest.raises(hip.HipError):
+        hip_ctx.pt_target(pt, hip.TARGET_DEPTH)

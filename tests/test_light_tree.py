@@ -152,3 +152,32 @@ def test_oracle_pt_material_zoo(oracle, background):
     if background != "none":
         base, _, _ = oracle.pt_render(P.oracle_pt_scene_materials(oracle, c, "none"), 1, 0, 8, 1, 2)
         assert rad.sum() > base.sum()
+
+
+def test_oracle_targets_closed_forms(oracle):
+    """Single-bounce Lambert scene: the guide-weighted first-hit targets follow
+    from the geometry (depth of a pixel seeing the back wall straight on, unit
+    normals, albedo of the hit material) and the Hit ID target equals the
+    Radiance run's Hit ID."""
+    c = P.cornell(8, 16, 16)
+    ps = P.oracle_pt_scene(oracle, c)
+    ref, nans = oracle.pt_render_targets(ps, 1, 0, 1, 1, 1, list(range(8)))
+    rad, hit, _ = oracle.pt_render(ps, 1, 0, 1, 1, 1)
+    assert nans == 0 and np.array_equal(ref[0], rad) and np.array_equal(ref[1], hit)
+    # max path length 1: one segment, guide weight g = 1 / (1 + 1 / pdf_cam)
+    n = ref[4]
+    g = np.linalg.norm(n, axis=2)
+    hitmask = hit[..., 0] >= 0
+    depth = ref[3][..., 0]
+    pos = ref[2]
+    # position / depth / normal carry the same weight g: position = g * (eye + depth/g * dir)
+    eye = np.array(c["camera"]["position"], np.float64)
+    g = np.where(hitmask, g, 1.0)
+    dist = np.linalg.norm(pos / g[..., None] - eye, axis=2)
+    np.testing.assert_allclose(dist[hitmask], depth[hitmask] / g[hitmask], rtol=1e-4)
+    alb = ref[6] / g[..., None]
+    mats = c["materials"]
+    prim = hit[..., 1].astype(int)
+    expect = np.where(mats["type"][c["mat_idx"][prim]][..., None] == S.MAT_LAMBERT,
+                      mats["albedo"][c["mat_idx"][prim]], 0.0)
+    np.testing.assert_allclose(alb[hitmask], expect[hitmask], rtol=1e-5, atol=1e-6)
