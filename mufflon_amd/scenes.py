@@ -337,3 +337,167 @@ def emissive_area_lights(verts, uvs, tris, mat_idx, materials, instance=0):
         out[j]["instance"] = instance
         out[j]["primitive"] = p
     return out
+
+
+# ---------------------------------------------------------------- terrain scene
+def _value_noise(n, freq, rng):
+    """Bilinear (smoothstep) value noise of lattice frequency freq on an n x n grid."""
+    lat = rng.random((freq + 1, freq + 1)).astype(np.float32)
+    t = np.linspace(0.0, freq, n, dtype=np.float64)
+    i = np.minimum(t.astype(np.int64), freq - 1)
+    f = (t - i).astype(np.float32)
+    f = f * f * (3 - 2 * f)
+    a = lat[i][:, i]
+    b = lat[i][:, i + 1]
+    c = lat[i + 1][:, i]
+    d = lat[i + 1][:, i + 1]
+    fx = f[None, :]
+    fy = f[:, None]
+    return (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy
+
+
+def fbm_heightfield(n, octaves=8, seed=3, base_freq=4, gain=0.5):
+    rng = np.random.default_rng(seed)
+    h = np.zeros((n, n), np.float32)
+    amp = 1.0
+    for o in range(octaves):
+        h += amp * _value_noise(n, base_freq << o, rng)
+        amp *= gain
+    return h
+
+
+def material_palette(count=64, seed=3):
+    """60% Lambert (albedo U[0.1, 0.9]), 30% GGX Torrance (roughness
+    U[0.05, 0.8]), 10% dielectric MICROFACET (ior 1.5)."""
+    rng = np.random.default_rng(seed)
+    n_l = int(round(0.6 * count))
+    n_t = int(round(0.3 * count))
+    mats = []
+    for _ in range(n_l):
+        mats.append(material(MAT_LAMBERT, albedo=rng.uniform(0.1, 0.9, 3)))
+    for _ in range(n_t):
+        mats.append(material(MAT_TORRANCE, albedo=rng.uniform(0.4, 0.95, 3), roughness=rng.uniform(0.05, 0.8)))
+    for _ in range(count - n_l - n_t):
+        mats.append(material(MAT_MICROFACET, roughness=rng.uniform(0.05, 0.3), ior=1.5))
+    return np.concatenate(mats)
+
+
+def grid_mesh(height, cell, amp):
+    """Heightfield mesh of an n x n height grid: vertices, normals, uvs, triangles."""
+    n = height.shape[0]
+    ys = height * amp
+    gx, gz = np.meshgrid(np.arange(n, dtype=np.float32) * cell, np.arange(n, dtype=np.float32) * cell)
+    verts = np.stack([gx, ys, gz], -1).reshape(-1, 3).astype(np.float32)
+    dx = np.gradient(ys, cell, axis=1)
+    dz = np.gradient(ys, cell, axis=0)
+    nrm = np.stack([-dx, np.ones_like(dx), -dz], -1)
+    nrm /= np.linalg.norm(nrm, axis=2, keepdims=True)
+    uvs = np.stack(np.meshgrid(np.linspace(0, 1, n, dtype=np.float32), np.linspace(0, 1, n, dtype=np.float32)),
+                   -1).reshape(-1, 2)
+    i = np.arange(n - 1, dtype=np.uint32)
+    I, J = np.meshgrid(i, i)  # I: column, J: row
+    v00 = (J * n + I).reshape(-1)
+    v10 = v00 + 1
+    v01 = v00 + n
+    v11 = v01 + 1
+    # counter-clockwise seen from +y
+    tris = np.empty((2 * len(v00), 3), np.uint32)
+    tris[0::2] = np.stack([v00, v01, v10], -1)
+    tris[1::2] = np.stack([v10, v01, v11], -1)
+    return verts, nrm.reshape(-1, 3).astype(np.float32), uvs.astype(np.float32), tris
+
+
+def procedural_rock(rings=26, segments=40, seed=3):
+    """UV sphere of 2 * segments * (rings - 1) triangles (2000 for 26 x 40),
+    radius perturbed by smooth noise."""
+    rng = np.random.default_rng(seed)
+    mb = MeshBuilder()
+    mb.add_uv_sphere((0, 0, 0), 1.0, rings, segments, 0)
+    v, n, uv, t, m = mb.arrays()
+    dirs = v / np.maximum(np.linalg.norm(v, axis=1, keepdims=True), 1e-9)
+    r = np.ones(len(v), np.float64)
+    for k in range(4):
+        a = rng.normal(size=3)
+        r += 0.25 / (k + 1) * np.sin((dirs @ a) * (2 + 2 * k) + rng.uniform(0, 6.3))
+    v = (dirs * r[:, None] * np.array([1.0, 0.7, 1.0])).astype(np.float32)
+    return v, n, uv, t
+
+
+def terrain_scene(grid=2236, rocks=4096, seed=3, width=1920, height=1080, num_point_lights=1024, env_size=(1024, 2048),
+                  palette=64):
+    """Procedural heightfield terrain (grid x grid vertices, 2 (grid-1)^2
+    triangles, fBm 8 octaves), instances of one procedural 2000-triangle rock,
+    a 60/30/10 Lambert / GGX / dielectric palette, lognormal point lights and
+    a procedural polar environment map. Returns LoDs (terrain first), instance
+    transforms (world-to-instance 3x4), materials, point lights, textures and
+    the camera."""
+    rng = np.random.default_rng(seed)
+    cell = 1.0
+    amp = 180.0
+    hf = fbm_heightfield(grid, 8, seed)
+    tv, tn, tuv, tt = grid_mesh(hf, cell, amp)
+    mats = material_palette(palette, seed)
+    kinds = mats["type"]
+    weights = np.where(kinds == MAT_LAMBERT, 0.6 / np.sum(kinds == MAT_LAMBERT),
+                       np.where(kinds == MAT_TORRANCE, 0.3 / np.sum(kinds == MAT_TORRANCE),
+                                0.1 / np.sum(kinds == MAT_MICROFACET)))
+    tile = 16
+    ntile = (grid - 1 + tile - 1) // tile
+    tile_mat = rng.choice(palette, size=(ntile, ntile), p=weights).astype(np.uint16)
+    cells = np.arange(grid - 1)
+    CI, CJ = np.meshgrid(cells // tile, cells // tile)  # column tile, row tile
+    tm = np.repeat(tile_mat[CJ, CI].reshape(-1), 2)
+    rv, rn, ruv, rt = procedural_rock(seed=seed)
+    rm = rng.choice(palette, size=len(rt) // 50 + 1, p=weights).astype(np.uint16)
+    rock_mat = np.repeat(rm, 50)[:len(rt)]
+    lods = [dict(verts=tv, normals=tn, uvs=tuv, tris=tt, mat_idx=tm),
+            dict(verts=rv, normals=rn, uvs=ruv, tris=rt, mat_idx=rock_mat)]
+    extent = (grid - 1) * cell
+    # instances: terrain (identity) + rocks on the surface
+    w2i = [IDENTITY34.copy()]
+    inst_lod = [0]
+    px = rng.uniform(0.02, 0.98, rocks) * extent
+    pz = rng.uniform(0.02, 0.98, rocks) * extent
+    gi = np.clip((px / cell).astype(int), 0, grid - 1)
+    gj = np.clip((pz / cell).astype(int), 0, grid - 1)
+    py = hf[gj, gi] * amp
+    for k in range(rocks):
+        s = rng.uniform(1.0, 6.0)
+        a = rng.uniform(0, 2 * np.pi)
+        tilt = rng.normal(scale=0.2, size=2)
+        R = _rot(a, tilt)
+        M = np.eye(4)
+        M[:3, :3] = R * s
+        M[:3, 3] = (px[k], py[k] - 0.3 * s, pz[k])
+        w2i.append(np.linalg.inv(M)[:3, :4].reshape(-1).astype(np.float32))
+        inst_lod.append(1)
+    pl = np.zeros(num_point_lights, POINT_LIGHT_DTYPE)
+    lx = rng.uniform(0, extent, num_point_lights)
+    lz = rng.uniform(0, extent, num_point_lights)
+    ly = hf[np.clip((lz / cell).astype(int), 0, grid - 1), np.clip((lx / cell).astype(int), 0, grid - 1)] * amp
+    pl["position"] = np.stack([lx, ly + rng.uniform(5, 60, num_point_lights), lz], -1)
+    power = rng.lognormal(mean=6.0, sigma=1.0, size=num_point_lights)
+    tint = rng.uniform(0.6, 1.0, (num_point_lights, 3))
+    pl["intensity"] = power[:, None] * tint / (4 * np.pi)
+    env = procedural_envmap(env_size[0], env_size[1], seed=seed, sun=(0.3, 0.72), sun_radiance=200.0)
+    eye = np.array([0.05 * extent, float(hf.max()) * amp + 40.0, 0.05 * extent], np.float32)
+    target = np.array([0.5 * extent, amp * 0.5, 0.5 * extent], np.float32)
+    view = target - eye
+    view /= np.linalg.norm(view)
+    right = np.cross(view, [0, 1, 0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, view).astype(np.float32)
+    cam = dict(position=eye, view_dir=view.astype(np.float32), up=up, tan_vfov=np.float32(np.tan(np.radians(25.0))),
+               width=width, height=height)
+    return dict(lods=lods, inst_lod=np.array(inst_lod, np.int32), world_to_inst=np.stack(w2i).astype(np.float32),
+                materials=mats, point_lights=pl, textures=[(env, SAMPLING_LINEAR)], envmap=0, camera=cam)
+
+
+def _rot(a, tilt):
+    ca, sa = np.cos(a), np.sin(a)
+    Ry = np.array([[ca, 0, sa], [0, 1, 0], [-sa, 0, ca]])
+    cx, sx = np.cos(tilt[0]), np.sin(tilt[0])
+    cz, sz = np.cos(tilt[1]), np.sin(tilt[1])
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Rx @ Ry

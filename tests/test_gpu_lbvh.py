@@ -104,3 +104,11 @@ def test_instance_level_bvh(hip_ctx, oracle):
     assert nint == ob["num_internal"]
     assert np.array_equal(ids, ob["sorted_ids"])
     assert np.array_equal(nodes, ob["nodes"])
+
+
+@pytest.mark.slow
+def test_lbvh_ten_million_terrain_triangles(hip_ctx, oracle):
+    """The 2236 x 2236 fBm heightfield (9,990,450 triangles), every stage bit-exact."""
+    hf = S.fbm_heightfield(2236, 8, 3)
+    v, _, _, t = S.grid_mesh(hf, 1.0, 180.0)
+    _compare_build(hip_ctx, oracle, v, t)
