@@ -253,6 +253,31 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations);
 /* Enable or disable a render target (render_enable_render_target /
  * render_disable_render_target, interface.cpp:3876-3940); resets the path tracer. */
 int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable);
+/* Execution form of an iteration. MEGAKERNEL: one thread runs a pixel's
+ * whole path (GpuPathTracer's sample_pt kernel, gpu_pt.cu:54-138).
+ * WAVEFRONT: persistent kernels drain ray queues stage by stage (shade,
+ * shadow rays, extension rays) with wave-level compaction; with useGraph
+ * each iteration is replayed from a captured HIP graph. Both forms produce
+ * identical results. No counterpart in the reference (its GPU device is the
+ * megakernel only). */
+#define MFH_PT_MEGAKERNEL 0
+#define MFH_PT_WAVEFRONT 1
+int mfh_pt_set_mode(mfh_context* ctx, int32_t ptId, int32_t mode, int32_t useGraph);
+/* Launch shape of the wavefront trace stages: block size x LDS stack depth
+ * (64x8, 64x16, 128x8, 128x16, 256x8, 256x16, 256x24) and the queue entries
+ * a wave grabs at once (multiple of 64). */
+typedef struct mfh_wavefront_tuning {
+    int32_t traceBlock;
+    int32_t stackDepth;
+    int32_t queueChunk;
+} mfh_wavefront_tuning;
+int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning* tuning);
+int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* outTuning);
+/* Times `iterations` wavefront iterations for every tuning candidate (block x
+ * stack depth x chunk 64/256/1024), keeps the fastest and resets the path
+ * tracer. */
+int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wavefront_tuning* outBest,
+                    float* outMsPerIteration);
 /* Running mean of an enabled target over the completed iterations (mufflon_get_target_image). */
 int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out);
 /* Index of the last completed iteration, -1 after a reset (render_get_current_iteration). */

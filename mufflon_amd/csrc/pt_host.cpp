@@ -7,6 +7,7 @@
 #include "capi_util.hpp"
 #include "light_tree.hpp"
 #include "pt_device.hpp"
+#include "pt_wavefront.hpp"
 
 namespace mfh {
 
@@ -23,7 +24,33 @@ struct PathTracer {
     uint32_t targetMask = (1u << MFH_TARGET_RADIANCE) | (1u << MFH_TARGET_HIT_ID);
     int32_t iteration = -1;
     uint64_t sceneGeneration = ~0ull;
+    // wavefront form
+    int32_t mode = MFH_PT_MEGAKERNEL;
+    bool useGraph = false;
+    WfTuning tune;
+    DevBuf wfCounters, wfTp, wfAcc, wfRay[2][4], wfSh[3], wfShPath;
+    int wfPaths = 0, wfMaxLen = -1, wfNee = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<uint8_t> graphKey;  // bytes of (S, W, tune) the graph was captured with
 };
+
+void drop_graph(PathTracer& t) {
+    if (t.graphExec) (void)hipGraphExecDestroy(t.graphExec);
+    if (t.graph) (void)hipGraphDestroy(t.graph);
+    t.graphExec = nullptr;
+    t.graph = nullptr;
+    t.graphKey.clear();
+}
+
+void release_wavefront(PathTracer& t) {
+    drop_graph(t);
+    for (DevBuf* b : {&t.wfCounters, &t.wfTp, &t.wfAcc, &t.wfSh[0], &t.wfSh[1], &t.wfSh[2], &t.wfShPath}) b->release();
+    for (auto& r : t.wfRay)
+        for (DevBuf& b : r) b.release();
+    t.wfPaths = 0;
+    t.wfMaxLen = t.wfNee = -1;
+}
 
 void destroy_tracers(Context& c) {
     for (PathTracer* t : c.tracers) {
@@ -32,6 +59,7 @@ void destroy_tracers(Context& c) {
                           &t->dSat, &t->dRngs, &t->dNan})
             b->release();
         for (DevBuf& b : t->dTargets) b.release();
+        release_wavefront(*t);
         delete t;
     }
     c.tracers.clear();
@@ -217,6 +245,116 @@ void reset(Context& c, PathTracer& t) {
     t.iteration = -1;
 }
 
+int num_cus(const Context& c) {
+    int n = 0;
+    MFH_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, c.device));
+    return std::max(n, 1);
+}
+
+// Buffers of the wavefront form for the current resolution, path length and
+// NEE count; the counter block holds queue lengths, shadow counts, work
+// counters and the device iteration counter.
+WfState wavefront_state(Context& c, PathTracer& t) {
+    const int N = t.camera.width * t.camera.height;
+    const int L = std::max(t.params.maxPathLength, 1);
+    const int nee = std::max(t.params.neeCount, 1);
+    MFH_REQUIRE(t.params.neeCount <= 32, "the wavefront form supports at most 32 NEE samples per vertex");
+    const int numCounters = (L + 1) + L + 3 * L;
+    if (N != t.wfPaths || L != t.wfMaxLen || nee != t.wfNee) {
+        drop_graph(t);
+        t.wfCounters.reserve(4 * (size_t)(numCounters + 1));
+        t.wfTp.reserve(16 * (size_t)N);
+        t.wfAcc.reserve(4 * 18 * (size_t)N);
+        for (auto& r : t.wfRay)
+            for (DevBuf& b : r) b.reserve(16 * (size_t)N);
+        for (DevBuf& b : t.wfSh) b.reserve(16 * (size_t)N * nee);
+        t.wfShPath.reserve(8 * (size_t)N);
+        t.wfPaths = N;
+        t.wfMaxLen = L;
+        t.wfNee = nee;
+    }
+    WfState W;
+    std::memset(&W, 0, sizeof(W));
+    W.numPaths = N;
+    uint32_t slots = 0;
+    int slot = 0;
+    for (int k = 0; k < kNumTargets; ++k) {
+        if ((t.targetMask >> k) & 1u)
+            for (int j = 0; j < kTargetChannels[k]; ++j) slots |= 1u << (slot + j);
+        slot += kTargetChannels[k];
+    }
+    W.slotMask = slots;
+    W.chunk = t.tune.chunk;
+    W.numCounters = numCounters;
+    W.qBase = 0;
+    W.shBase = L + 1;
+    W.workBase = 2 * L + 1;
+    W.counters = t.wfCounters.as<int>();
+    W.iteration = t.wfCounters.as<int>() + numCounters;
+    W.tp = t.wfTp.as<float4>();
+    W.acc = t.wfAcc.as<float>();
+    for (int b = 0; b < 2; ++b) {
+        W.rayO[b] = t.wfRay[b][0].as<float4>();
+        W.rayD[b] = t.wfRay[b][1].as<float4>();
+        W.rayM[b] = t.wfRay[b][2].as<float4>();
+        W.hit[b] = t.wfRay[b][3].as<float4>();
+    }
+    W.shO = t.wfSh[0].as<float4>();
+    W.shD = t.wfSh[1].as<float4>();
+    W.shC = t.wfSh[2].as<float4>();
+    W.shPath = t.wfShPath.as<int2>();
+    if (t.tune.traceGrid <= 0 || t.tune.shadeGrid <= 0) {
+        const int cus = num_cus(c);
+        t.tune.traceGrid = 0;
+        t.tune.shadeGrid = wf_shade_grid(cus);
+    }
+    return W;
+}
+
+// Runs `iterations` wavefront iterations, replaying a captured graph when enabled.
+void run_wavefront(Context& c, PathTracer& t, PtScene& S, int32_t iterations) {
+    WfState W = wavefront_state(c, t);
+    if (t.tune.traceGrid <= 0) t.tune.traceGrid = wf_grid(S, t.tune, num_cus(c));
+    // the device iteration counter continues from the host's
+    const int32_t it = t.iteration;
+    MFH_HIP(hipMemcpyAsync(W.iteration, &it, 4, hipMemcpyHostToDevice, c.stream));
+    if (!t.useGraph) {
+        for (int32_t k = 0; k < iterations; ++k) {
+            S.iteration = ++t.iteration;
+            launch_pt_wavefront(S, W, t.tune, c.stream);
+        }
+        MFH_HIP(hipStreamSynchronize(c.stream));  // the host copy of the iteration counter is read above
+        return;
+    }
+    PtScene Sk = S;
+    Sk.iteration = 0;  // the graph reads the device counter
+    std::vector<uint8_t> key(sizeof(Sk) + sizeof(W) + sizeof(WfTuning));
+    std::memcpy(key.data(), &Sk, sizeof(Sk));
+    std::memcpy(key.data() + sizeof(Sk), &W, sizeof(W));
+    std::memcpy(key.data() + sizeof(Sk) + sizeof(W), &t.tune, sizeof(WfTuning));
+    if (!t.graphExec || key != t.graphKey) {
+        drop_graph(t);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            launch_pt_wavefront(Sk, W, t.tune, c.stream);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        MFH_HIP(hipStreamEndCapture(c.stream, &t.graph));
+        MFH_HIP(hipGraphInstantiate(&t.graphExec, t.graph, nullptr, nullptr, 0));
+        t.graphKey = key;
+    }
+    for (int32_t k = 0; k < iterations; ++k) {
+        ++t.iteration;
+        MFH_HIP(hipGraphLaunch(t.graphExec, c.stream));
+    }
+    MFH_HIP(hipStreamSynchronize(c.stream));
+}
+
 }  // namespace
 }  // namespace mfh
 
@@ -317,10 +455,102 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations) {
         MFH_REQUIRE(iterations >= 0, "negative iteration count");
         MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "scene without lights");
         PtScene S = device_scene(c, t);
+        if (t.mode == MFH_PT_WAVEFRONT) {
+            run_wavefront(c, t, S, iterations);
+            return;
+        }
         for (int32_t k = 0; k < iterations; ++k) {
             S.iteration = ++t.iteration;
             launch_pt_iterate(S, c.stream);
         }
+    });
+}
+
+int mfh_pt_set_mode(mfh_context* ctx, int32_t ptId, int32_t mode, int32_t useGraph) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(mode == MFH_PT_MEGAKERNEL || mode == MFH_PT_WAVEFRONT, "unknown path tracer mode");
+        MFH_REQUIRE(!(mode == MFH_PT_WAVEFRONT && t.params.neeCount > 32),
+                    "the wavefront form supports at most 32 NEE samples per vertex");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        t.mode = mode;
+        t.useGraph = useGraph != 0;
+        if (mode != MFH_PT_WAVEFRONT) release_wavefront(t);
+    });
+}
+
+int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning* tuning) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(tuning != nullptr, "null tuning");
+        int v = -1;
+        for (int k = 0; k < kWfNumTraceVariants; ++k)
+            if (kWfTraceShapes[k].block == tuning->traceBlock && kWfTraceShapes[k].lds == tuning->stackDepth) v = k;
+        MFH_REQUIRE(v >= 0, "unsupported trace block size / LDS stack depth combination");
+        MFH_REQUIRE(tuning->queueChunk >= 64 && tuning->queueChunk % 64 == 0 && tuning->queueChunk <= 65536,
+                    "queue chunk must be a multiple of 64 in [64, 65536]");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        t.tune.traceVariant = v;
+        t.tune.chunk = tuning->queueChunk;
+        t.tune.traceGrid = 0;
+        drop_graph(t);
+    });
+}
+
+int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output");
+        out->traceBlock = kWfTraceShapes[t.tune.traceVariant].block;
+        out->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
+        out->queueChunk = t.tune.chunk;
+    });
+}
+
+int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wavefront_tuning* outBest,
+                    float* outMsPerIteration) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(iterations > 0, "autotune needs at least one iteration per candidate");
+        MFH_REQUIRE(t.mode == MFH_PT_WAVEFRONT, "autotune applies to the wavefront form");
+        MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "scene without lights");
+        PtScene S = device_scene(c, t);
+        const int chunks[3] = {64, 256, 1024};
+        float best = 3.4e38f;
+        WfTuning bestTune = t.tune;
+        for (int v = 0; v < kWfNumTraceVariants; ++v) {
+            for (int ch : chunks) {
+                t.tune.traceVariant = v;
+                t.tune.chunk = ch;
+                t.tune.traceGrid = 0;
+                drop_graph(t);
+                run_wavefront(c, t, S, 1);  // warm-up (and graph capture)
+                MFH_HIP(hipEventRecord(c.evBegin, c.stream));
+                run_wavefront(c, t, S, iterations);
+                MFH_HIP(hipEventRecord(c.evEnd, c.stream));
+                MFH_HIP(hipEventSynchronize(c.evEnd));
+                float ms = 0.0f;
+                MFH_HIP(hipEventElapsedTime(&ms, c.evBegin, c.evEnd));
+                ms /= (float)iterations;
+                if (ms < best) {
+                    best = ms;
+                    bestTune = t.tune;
+                }
+            }
+        }
+        t.tune = bestTune;
+        drop_graph(t);
+        reset(c, t);
+        if (outBest) {
+            outBest->traceBlock = kWfTraceShapes[t.tune.traceVariant].block;
+            outBest->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
+            outBest->queueChunk = t.tune.chunk;
+        }
+        if (outMsPerIteration) *outMsPerIteration = best;
     });
 }
 
@@ -349,6 +579,7 @@ int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t
         MFH_HIP(hipStreamSynchronize(c.stream));
         t.targetMask = mask;
         if (!enable) t.dTargets[target].release();
+        drop_graph(t);
         reset(c, t);
     });
 }
@@ -463,6 +694,7 @@ int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* 
         }
         t.background = out;
         t.tree.backgroundFlux = out.fluxSum;
+        drop_graph(t);
         reset(c, t);
     });
 }

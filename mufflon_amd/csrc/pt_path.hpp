@@ -1,0 +1,287 @@
+// Path-level building blocks of the path tracer shared by the megakernel and
+// the wavefront stages: the surface / void vertex made from a hit, the
+// per-iteration target accumulator, emission and target contributions, next-
+// event candidates and the BSDF walk. Both drivers call exactly these
+// functions in the order of pt_sample (pt_common.hpp:75-156), so they produce
+// the same bits.
+#pragma once
+#include "pt_shading.hpp"
+#include "traverse.hpp"
+
+namespace mfh {
+namespace path {
+
+using namespace shade;
+
+__device__ __forceinline__ M34 load_m34(const float* w) {
+    const float4* q = reinterpret_cast<const float4*>(w);
+    const float4 a = q[0], b = q[1], c = q[2];
+    return M34{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w}};
+}
+
+// intersection.cu:16-24
+__device__ __forceinline__ V3 add_epsilon(V3 o, V3 dir, V3 geoN) {
+    const V3 off = geoN * kSceneEps;
+    return dot(geoN, dir) >= 0.0f ? o + off : o - off;
+}
+
+// Hit attributes of first_intersection (intersection.cu:524-621): shading
+// frame of tangent_space_geom_to_shader (tangent_space.hpp:10-60) and the
+// interpolated uv.
+__device__ __forceinline__ TS hit_frame(const PtScene& S, int inst, int prim, float w0, float w1, float& hu,
+                                        float& hv) {
+    const int lod = S.trace.instLod[inst];
+    const DevPtLod L = S.ptLods[lod];
+    const u32 i0 = L.tris[3 * (size_t)prim], i1 = L.tris[3 * (size_t)prim + 1], i2 = L.tris[3 * (size_t)prim + 2];
+    const V3 v0 = v3(L.verts[3 * (size_t)i0], L.verts[3 * (size_t)i0 + 1], L.verts[3 * (size_t)i0 + 2]);
+    const V3 v1 = v3(L.verts[3 * (size_t)i1], L.verts[3 * (size_t)i1 + 1], L.verts[3 * (size_t)i1 + 2]);
+    const V3 v2 = v3(L.verts[3 * (size_t)i2], L.verts[3 * (size_t)i2 + 1], L.verts[3 * (size_t)i2 + 2]);
+    const float* uv = L.uvs;
+    const float u0x = uv[2 * (size_t)i0], u0y = uv[2 * (size_t)i0 + 1];
+    const float u1x = uv[2 * (size_t)i1], u1y = uv[2 * (size_t)i1 + 1];
+    const float u2x = uv[2 * (size_t)i2], u2y = uv[2 * (size_t)i2 + 1];
+    const V3 dx0 = v1 - v0;
+    const V3 dx1 = v2 - v0;
+    const float du0x = u1x - u0x, du0y = u1y - u0y;
+    const float du1x = u2x - u0x, du1y = u2y - u0y;
+    const float det = du0x * du1y - du0y * du1x;
+    V3 tangentX;
+    if (det >= 1e-5f || det <= -1e5f)
+        tangentX = (dx0 * du1y - dx1 * du0y) / det;
+    else
+        tangentX = dx0;
+    const V3 geoN0 = cross(dx0, dx1);
+    const float w2 = 1.f - w0 - w1;
+    hu = u0x * w0 + u1x * w1 + u2x * w2;
+    hv = u0y * w0 + u1y * w1 + u2y * w2;
+    const M34 M = load_m34(S.trace.worldToInst + 12 * (size_t)inst);
+    TS ts;
+    ts.geoN = normalize(xform_normal(M, geoN0));
+    tangentX = normalize(xform_normal(M, tangentX));
+    const float* nr = L.normals;
+    const V3 n0 = v3(nr[3 * (size_t)i0], nr[3 * (size_t)i0 + 1], nr[3 * (size_t)i0 + 2]);
+    const V3 n1 = v3(nr[3 * (size_t)i1], nr[3 * (size_t)i1 + 1], nr[3 * (size_t)i1 + 2]);
+    const V3 n2 = v3(nr[3 * (size_t)i2], nr[3 * (size_t)i2 + 1], nr[3 * (size_t)i2 + 2]);
+    V3 sn = smul(w0, n0) + smul(w1, n1) + smul(1.f - (w0 + w1), n2);
+    sn = normalize(xform_normal(M, sn));
+    ts.shadingN = sn;
+    ts.shadingTX = normalize(tangentX - sn * dot(tangentX, sn));
+    ts.shadingTY = cross(sn, ts.shadingTX);
+    return ts;
+}
+
+__device__ __forceinline__ int material_index(const PtScene& S, int inst, int prim) {
+    const int lod = S.trace.instLod[inst];
+    return (int)S.ptLods[lod].matIdx[prim];
+}
+
+// Per-iteration target values of one path: Radiance 0-2, Hit ID 3-4,
+// Position 5-7, Depth 8, Normal 9-11, UV 12-13, Albedo 14-16, Lightness 17.
+// contribute + check_nan of RenderBuffer (render_target.hpp:62-76,189-195).
+struct Acc {
+    float v[18];
+    u32 mask;
+    u32 nans;
+    __device__ __forceinline__ void init(u32 m) {
+#pragma unroll
+        for (int k = 0; k < 18; ++k) v[k] = 0.0f;
+        mask = m;
+        nans = 0;
+    }
+    __device__ __forceinline__ void add(int target, int slot, float x) {
+        if (!((mask >> target) & 1u)) return;
+        if (x != x) {
+            ++nans;
+            x = 0.0f;
+        }
+        v[slot] += x;
+    }
+};
+
+// The vertex a walk ends in: a surface hit or the void.
+struct Vertex {
+    bool isVoid;
+    V3 position;
+    V3 incEnc;  // (x, y, dist * sgn(z)) for surfaces, the raw direction for the void
+    TS ts;
+    int inst, prim;
+    MatParams mat;
+    float incidentPdf;
+    float hu, hv;
+    __device__ __forceinline__ V3 incident() const {
+        const float iz = sgnf(incEnc.z) * fsqrt(clampf(1.0f - incEnc.x * incEnc.x - incEnc.y * incEnc.y, 0.0f, 1.0f));
+        return v3(incEnc.x, incEnc.y, iz);
+    }
+};
+
+// create_surface / create_void + PtVertexExt::update (path_util.hpp:629-665,
+// pt_common.hpp:45-61) for the segment rayOrigin + t * dir with forward pdf
+// pdfF; inst == -1 is a miss.
+__device__ __forceinline__ void make_vertex(const PtScene& S, V3 rayOrigin, V3 dir, float pdfF, float t, int inst,
+                                            int prim, float w0, float w1, Vertex& vx) {
+    vx.isVoid = inst == -1;
+    vx.inst = inst;
+    vx.prim = prim;
+    vx.hu = 0.0f;
+    vx.hv = 0.0f;
+    if (vx.isVoid) {
+        vx.incEnc = dir;
+        vx.incidentPdf = pdfF * 1.0f;
+        vx.mat.type = -1;
+        return;
+    }
+    vx.position = rayOrigin + dir * t;
+    vx.ts = hit_frame(S, inst, prim, w0, w1, vx.hu, vx.hv);
+    vx.incEnc = v3(dir.x, dir.y, t * sgnf(dir.z));
+    vx.mat = fetch(S, material_index(S, inst, prim), vx.hu, vx.hv);
+    const float inCosAbs = fabsf(dot(dir, vx.ts.shadingN));
+    vx.incidentPdf = pdfF * fabsf(inCosAbs) / (t * t);
+}
+
+// Contributions of a new vertex at path length pathLen (pt_common.hpp:128-151):
+// Hit ID on the first segment, then emission with MIS and the guide-weighted
+// auxiliary targets.
+__device__ __forceinline__ void vertex_targets(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
+                                               float guideWeight, Acc& acc) {
+    if (pathLen == 1) {
+        acc.add(1, 3, (float)vx.inst);
+        acc.add(1, 4, (float)vx.prim);
+    }
+    if (pathLen < S.minPathLength) return;
+    V3 em;
+    float connectPdf = 0.0f;
+    if (vx.isVoid) {  // get_emission, VOID (path_util.hpp:415-424)
+        float pdfBack;
+        em = evaluate_background(S, vx.incEnc, pdfBack);
+        if (nonzero(em)) connectPdf = background_pdf(S, pdfBack);
+    } else {
+        em = emission(vx.mat, vx.ts.geoN, neg(vx.incident()));
+        if (nonzero(em)) connectPdf = light_pdf(S, vx.inst, vx.prim);
+    }
+    if (nonzero(em) && pathLen > 1) {
+        const float mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / vx.incidentPdf));
+        em = em * mis;
+    }
+    const V3 cc = throughput * em;
+    acc.add(0, 0, cc.x);
+    acc.add(0, 1, cc.y);
+    acc.add(0, 2, cc.z);
+    if (acc.mask & 0xFCu) {
+        const V3 pos = vx.isVoid ? v3(0, 0, 0) : vx.position;
+        acc.add(2, 5, guideWeight * pos.x);
+        acc.add(2, 6, guideWeight * pos.y);
+        acc.add(2, 7, guideWeight * pos.z);
+        acc.add(3, 8, guideWeight * (vx.isVoid ? 0.0f : fabsf(vx.incEnc.z)));
+        const V3 nrm = vx.isVoid ? v3(0, 0, 0) : vx.ts.shadingN;
+        acc.add(4, 9, guideWeight * nrm.x);
+        acc.add(4, 10, guideWeight * nrm.y);
+        acc.add(4, 11, guideWeight * nrm.z);
+        acc.add(5, 12, guideWeight * vx.hu);
+        acc.add(5, 13, guideWeight * vx.hv);
+        V3 alb = v3(0, 0, 0);  // materials::albedo
+        if (!vx.isVoid) {
+            if (vx.mat.type == kMatLambert || vx.mat.type == kMatTorrance)
+                alb = vx.mat.albedo;
+            else if (vx.mat.type == kMatMicrofacet)
+                alb = v3(1.0f / (1.0f + vx.mat.inner.absorption.x), 1.0f / (1.0f + vx.mat.inner.absorption.y),
+                         1.0f / (1.0f + vx.mat.inner.absorption.z));
+        }
+        acc.add(6, 14, guideWeight * alb.x);
+        acc.add(6, 15, guideWeight * alb.y);
+        acc.add(6, 16, guideWeight * alb.z);
+        acc.add(7, 17, guideWeight * ((em.x + em.y + em.z) / 3.0f));
+    }
+}
+
+// One next-event estimate of pt_common.hpp:95-124 up to the visibility test:
+// on success the shadow segment (origin a, unit direction, length) and the
+// Radiance / Lightness values to add when it is unoccluded.
+struct NeeCandidate {
+    V3 a, dir;
+    float tmax;
+    V3 radiance;
+    float lightness;
+};
+__device__ __forceinline__ bool nee_candidate(const PtScene& S, const Vertex& vx, V3 throughput, float guideWeight,
+                                              int i, u64 neeSeed, u64 x, NeeCandidate& c) {
+    const Nee nee = connect(S, (u64)i, (u64)S.neeCount, neeSeed, vx.position, sample_uniform((u32)x),
+                            sample_uniform((u32)(x >> 32)));
+    float vCos, vPdf;
+    const V3 value = mat_evaluate(vx.ts, vx.mat, vx.incident(), nee.dir, vCos, vPdf);
+    if (nee.cosOut != 0) vCos *= nee.cosOut;
+    const V3 radiance = value * nee.diffIrradiance;
+    if (!((radiance.x > 0.0f || radiance.y > 0.0f || radiance.z > 0.0f) && vCos > 0.0f)) return false;
+    // any_intersection (intersection.cu:624-637)
+    const V3 a = add_epsilon(vx.position, nee.dir, vx.ts.geoN);
+    const V3 b = add_epsilon(nee.position, neg(nee.dir), nee.geoNormal);
+    const V3 d = b - a;
+    c.a = a;
+    c.tmax = len(d);
+    c.dir = d / c.tmax;
+    const float hitPdf = vPdf * fabsf(nee.cosOut) / nee.distSq;
+    const float mis = 1.0f / ((float)S.neeCount + hitPdf / nee.creationPdf);
+    c.radiance = throughput * vCos * radiance * mis;
+    c.lightness = guideWeight * vCos;
+    return true;
+}
+
+__device__ __forceinline__ void add_nee(const NeeCandidate& c, Acc& acc) {
+    acc.add(0, 0, c.radiance.x);
+    acc.add(0, 1, c.radiance.y);
+    acc.add(0, 2, c.radiance.z);
+    acc.add(7, 17, c.lightness);
+}
+
+// walk (random_walk.hpp:38-112) from a surface vertex up to the next ray:
+// BSDF sample, throughput, Russian roulette. Returns false when the path ends.
+__device__ __forceinline__ bool walk_surface(const Vertex& vx, u64 x0, u64 x1, float rndRoulette, V3& throughput,
+                                             V3& rayOrigin, V3& excident, float& pdfF, V3& absorption) {
+    const float u0 = sample_uniform((u32)x0), u1 = sample_uniform((u32)(x0 >> 32));
+    V3 sThroughput;
+    if (!mat_sample(vx.ts, vx.mat, vx.incident(), u0, u1, x1, sThroughput, excident, pdfF)) return false;
+    absorption = vx.mat.medium(dot(excident, vx.ts.geoN)).absorption;  // path_util.hpp:346-351
+    throughput = throughput * sThroughput;
+    const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
+    if (rndRoulette >= cont) return false;
+    throughput = throughput / cont;
+    rayOrigin = add_epsilon(vx.position, excident, vx.ts.geoN);
+    return true;
+}
+
+// The camera's walk (pinholecam_sample_ray, pinhole.hpp:56-84): the first
+// ray of pixel (px, py); Russian roulette always continues at throughput 1.
+__device__ __forceinline__ void camera_ray(const PtScene& S, int px, int py, u64 x0, V3& dir, float& pdfF) {
+    const float u0 = sample_uniform((u32)x0), u1 = sample_uniform((u32)(x0 >> 32));
+    const V3 viewDir = v3(S.camDir[0], S.camDir[1], S.camDir[2]);
+    const V3 up = v3(S.camUp[0], S.camUp[1], S.camUp[2]);
+    const float w = (float)S.width, h = (float)S.height;
+    float cx = ((float)px + u0) / w * 2.0f - 1.0f;
+    float cy = ((float)py + u1) / h * 2.0f - 1.0f;
+    cx *= S.tanVFov;
+    cy *= S.tanVFov;
+    const float aspect = w / h;
+    const V3 xAxis = cross(viewDir, up);
+    dir = normalize(xAxis * (cx * aspect) + up * cy + viewDir * 1.0f);
+    const float pixelArea = (2 * S.tanVFov) * (2 * S.tanVFov) * aspect;
+    const float cosOut = dot(viewDir, dir);
+    pdfF = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
+}
+
+// Running means of the enabled targets (output_handler_variance.hpp:7-31).
+__device__ __forceinline__ void accumulate(const PtScene& S, size_t pixel, int iteration, const float* v, u32 mask) {
+    const float denom = mx(1.0f, (float)iteration + 1);
+    int slot = 0;
+#pragma unroll
+    for (int t = 0; t < kNumTargets; ++t) {
+        const int nc = kTargetChannels[t];
+        if ((mask >> t) & 1u) {
+            float* c = S.cum[t] + (size_t)nc * pixel;
+#pragma unroll
+            for (int k = 0; k < nc; ++k) c[k] += (v[slot + k] - c[k]) / denom;
+        }
+        slot += nc;
+    }
+}
+
+}  // namespace path
+}  // namespace mfh

@@ -1,0 +1,361 @@
+// Wavefront form of the path tracer for gfx950: one iteration of pt_sample
+// (pt_common.hpp:75-156) split into stages over ray queues.
+//
+//   begin     zero the queue / work counters, advance the iteration counter
+//   primary   camera ray + closest hit for every pixel (queue 0 = all paths)
+//   per bounce k:
+//     shade   hit -> vertex, Hit ID / emission / auxiliary targets, next-event
+//             candidates into the shadow queue, BSDF walk into queue k+1
+//     shadow  any-hit of the candidates; unoccluded ones are added
+//     extend  closest hit of queue k+1
+//   finish    running means of the enabled targets
+//
+// The trace, shade and shadow kernels are persistent: a grid sized to the
+// chip's occupancy drains its queue in chunks grabbed by one atomic per wave,
+// so the launch shape does not depend on the queue length (which lives in
+// device memory) and a whole iteration can be captured in a HIP graph.
+// Queue appends are compacted per wave64 with ballot + mbcnt and one atomic.
+// Every stage calls the same path:: functions as the megakernel, and each
+// path's contributions reach its accumulator in the megakernel's order
+// (emission of vertex k, then its next-event estimates, then vertex k+1), so
+// both drivers produce the same bits.
+#include "pt_path.hpp"
+#include "pt_wavefront.hpp"
+
+namespace mfh {
+
+namespace {
+
+using namespace shade;
+using namespace path;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Wave-wide append of the lanes with `take` set to a queue with length at
+// *count: returns this lane's slot (valid where take is set).
+__device__ __forceinline__ int wave_append(int* count, bool take) {
+    const unsigned long long m = __ballot(take);
+    if (m == 0ull) return -1;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(count, (int)__popcll(m));
+    base = __shfl(base, leader);
+    const unsigned lo = (unsigned)m, hi = (unsigned)(m >> 32);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    return base + rank;
+}
+
+// Grab the next chunk of a queue for this wave; every lane gets the same base.
+__device__ __forceinline__ int wave_grab(int* work, int chunk) {
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(work, chunk);
+    return __shfl(base, 0);
+}
+
+// Per-path accumulator in HBM: slot-major (slot * N + path).
+__device__ __forceinline__ void load_acc(const WfState& W, u32 mask, int p, Acc& acc) {
+    acc.init(mask);
+    int slot = 0;
+#pragma unroll
+    for (int t = 0; t < kNumTargets; ++t) {
+        const int nc = kTargetChannels[t];
+        if ((mask >> t) & 1u) {
+#pragma unroll
+            for (int k = 0; k < nc; ++k) acc.v[slot + k] = W.acc[(size_t)(slot + k) * W.numPaths + p];
+        }
+        slot += nc;
+    }
+}
+__device__ __forceinline__ void store_acc(const WfState& W, u32 mask, int p, const Acc& acc) {
+    int slot = 0;
+#pragma unroll
+    for (int t = 0; t < kNumTargets; ++t) {
+        const int nc = kTargetChannels[t];
+        if ((mask >> t) & 1u) {
+#pragma unroll
+            for (int k = 0; k < nc; ++k) W.acc[(size_t)(slot + k) * W.numPaths + p] = acc.v[slot + k];
+        }
+        slot += nc;
+    }
+}
+
+__global__ void k_wf_begin(WfState W) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < W.numCounters) W.counters[i] = 0;
+    if (i == 0) {
+        W.counters[W.qBase] = W.numPaths;  // queue 0 holds every path
+        ++*W.iteration;
+    }
+}
+
+// Camera rays and their closest hits, one thread per pixel in 16x16 tiles.
+template <bool kSingle>
+__global__ __launch_bounds__(256) void k_wf_primary(PtScene S, WfState W) {
+    __shared__ int sStack[kLdsDepth * 256];
+    const int px = blockIdx.x * 16 + (threadIdx.x % 16);
+    const int py = blockIdx.y * 16 + (threadIdx.x / 16);
+    if (px >= S.width || py >= S.height) return;
+    const int p = px + py * S.width;
+    u64 rng = S.rngs[p];
+    pcg_next(rng);  // create_camera's RndSet2
+    const u64 x0 = pcg_next(rng);
+    pcg_next(rng);  // the i0 word of the camera's RndSet2_1
+    pcg_next(rng);  // roulette (always continues at throughput 1)
+    V3 dir;
+    float pdfF;
+    camera_ray(S, px, py, x0, dir, pdfF);
+    const V3 o = v3(S.camPos[0], S.camPos[1], S.camPos[2]);
+    const TraceHit h = trace_ray<false, false, kSingle, 256>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
+    S.rngs[p] = rng;
+    W.rayO[0][p] = float4{o.x, o.y, o.z, pdfF};
+    W.rayD[0][p] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
+    W.rayM[0][p] = float4{0.0f, 0.0f, 0.0f, h.w1};
+    W.hit[0][p] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
+    W.tp[p] = float4{1.0f, 1.0f, 1.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 18; ++s)
+        if ((W.slotMask >> s) & 1u) W.acc[(size_t)s * W.numPaths + p] = 0.0f;
+}
+
+// Closest hit of queue `buf` (length at counters[count]).
+template <bool kSingle, int kBlock, int kLds>
+__global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int buf, int count, int work) {
+    __shared__ int sStack[kLds * kBlock];
+    const int n = W.counters[count];
+    int* const lstack = sStack + threadIdx.x;
+    for (;;) {
+        const int base = wave_grab(W.counters + work, W.chunk);
+        if (base >= n) break;
+        const int end = min(base + W.chunk, n);
+        for (int q = base + lane_id(); q < end; q += 64) {
+            const float4 o = W.rayO[buf][q];
+            const float4 d = W.rayD[buf][q];
+            const TraceHit h = trace_ray<false, false, kSingle, kBlock, kLds>(S.trace, v3(o.x, o.y, o.z),
+                                                                               v3(d.x, d.y, d.z), kMaxSceneSize, lstack);
+            W.hit[buf][q] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
+            W.rayM[buf][q].w = h.w1;
+        }
+    }
+}
+
+// Any hit of the shadow candidates of one bounce; visible ones are added to
+// their path's accumulator in candidate order.
+template <bool kSingle, int kBlock, int kLds>
+__global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S, WfState W, int count, int work) {
+    __shared__ int sStack[kLds * kBlock];
+    const int n = W.counters[count];
+    int* const lstack = sStack + threadIdx.x;
+    u32 nans = 0;
+    for (;;) {
+        const int base = wave_grab(W.counters + work, W.chunk);
+        if (base >= n) break;
+        const int end = min(base + W.chunk, n);
+        for (int s = base + lane_id(); s < end; s += 64) {
+            const int2 pm = W.shPath[s];
+            const int p = pm.x;
+            const u32 cand = (u32)pm.y;
+            Acc acc;
+            bool loaded = false;
+            for (int i = 0; i < S.neeCount; ++i) {
+                if (!((cand >> i) & 1u)) continue;
+                const size_t e = (size_t)s * S.neeCount + i;
+                const float4 a = W.shO[e];
+                const float4 d = W.shD[e];
+                const TraceHit h = trace_ray<true, false, kSingle, kBlock, kLds>(S.trace, v3(a.x, a.y, a.z),
+                                                                                  v3(d.x, d.y, d.z), a.w, lstack);
+                if (h.inst == -1) {
+                    if (!loaded) {
+                        load_acc(W, S.targetMask, p, acc);
+                        loaded = true;
+                    }
+                    const float4 c = W.shC[e];
+                    NeeCandidate nc;
+                    nc.radiance = v3(c.x, c.y, c.z);
+                    nc.lightness = c.w;
+                    add_nee(nc, acc);
+                }
+            }
+            if (loaded) {
+                store_acc(W, S.targetMask, p, acc);
+                nans += acc.nans;
+            }
+        }
+    }
+    if (nans) atomicAdd(S.nanCounter, nans);
+}
+
+// Shade the hits of queue `buf` at path length k + 1.
+__global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, int buf, int count, int nextCount,
+                                                  int shCount, int work) {
+    const int n = W.counters[count];
+    const int pathLen = k + 1;
+    const int nbuf = buf ^ 1;
+    u32 nans = 0;
+    for (;;) {
+        const int base = wave_grab(W.counters + work, W.chunk);
+        if (base >= n) break;
+        const int end = base + W.chunk;
+        for (int q0 = base; q0 < end; q0 += 64) {
+            const int q = q0 + lane_id();
+            const bool active = q < n;
+            bool doNee = false, cont = false;
+            int p = 0;
+            u64 rng = 0;
+            V3 throughput = v3(0, 0, 0), rayOrigin = v3(0, 0, 0), dir = v3(0, 0, 0), absorption = v3(0, 0, 0);
+            float footA = 0.0f, guideWeight = 0.0f, pdfF = 0.0f;
+            Vertex vx;
+            vx.isVoid = true;
+            Acc acc;
+            if (active) {
+                const float4 o = W.rayO[buf][q];
+                const float4 d = W.rayD[buf][q];
+                const float4 m = W.rayM[buf][q];
+                const float4 h = W.hit[buf][q];
+                p = __float_as_int(d.w);
+                rng = S.rngs[p];
+                const float4 t4 = W.tp[p];
+                throughput = v3(t4.x, t4.y, t4.z);
+                footA = t4.w;
+                const int inst = __float_as_int(h.y);
+                throughput = throughput * transmission(Medium{1.0f, v3(m.x, m.y, m.z)}, inst == -1 ? kMaxSceneSize : h.x);
+                footA += 1.0f / fsqrt(o.w);
+                guideWeight = 1.f / (1.f + footA * footA / 1.0f);
+                make_vertex(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, h.x, inst, __float_as_int(h.z), h.w, m.w, vx);
+                load_acc(W, S.targetMask, p, acc);
+                vertex_targets(S, vx, pathLen, throughput, guideWeight, acc);
+                const bool more = !vx.isVoid && pathLen < S.maxPathLength;
+                doNee = more && pathLen + 1 >= S.minPathLength && pathLen + 1 <= S.maxPathLength;
+                cont = more;
+            }
+            // next-event candidates: one shadow slot per path that estimates
+            const int sh = wave_append(W.counters + shCount, doNee);
+            if (doNee) {
+                const u64 neeSeed = pcg_next(rng);
+                u32 cand = 0;
+                for (int i = 0; i < S.neeCount; ++i) {
+                    const u64 x = pcg_next(rng);
+                    NeeCandidate c;
+                    if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
+                        const size_t e = (size_t)sh * S.neeCount + i;
+                        W.shO[e] = float4{c.a.x, c.a.y, c.a.z, c.tmax};
+                        W.shD[e] = float4{c.dir.x, c.dir.y, c.dir.z, 0.0f};
+                        W.shC[e] = float4{c.radiance.x, c.radiance.y, c.radiance.z, c.lightness};
+                        cand |= 1u << i;
+                    }
+                }
+                W.shPath[sh] = int2{p, (int)cand};
+            }
+            if (cont) {
+                const u64 x0 = pcg_next(rng);
+                const u64 x1 = pcg_next(rng);
+                const float rndRoulette = sample_uniform((u32)pcg_next(rng));
+                cont = walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, dir, pdfF, absorption);
+            }
+            const int nq = wave_append(W.counters + nextCount, cont);
+            if (cont) {
+                W.rayO[nbuf][nq] = float4{rayOrigin.x, rayOrigin.y, rayOrigin.z, pdfF};
+                W.rayD[nbuf][nq] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
+                W.rayM[nbuf][nq] = float4{absorption.x, absorption.y, absorption.z, 0.0f};
+                W.tp[p] = float4{throughput.x, throughput.y, throughput.z, footA};
+            }
+            if (active) {
+                S.rngs[p] = rng;
+                store_acc(W, S.targetMask, p, acc);
+                nans += acc.nans;
+            }
+        }
+    }
+    if (nans) atomicAdd(S.nanCounter, nans);
+}
+
+__global__ void k_wf_finish(PtScene S, WfState W) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W.numPaths) return;
+    float v[18];
+#pragma unroll
+    for (int s = 0; s < 18; ++s) v[s] = ((W.slotMask >> s) & 1u) ? W.acc[(size_t)s * W.numPaths + p] : 0.0f;
+    accumulate(S, (size_t)p, *W.iteration, v, S.targetMask);
+}
+
+template <bool kSingle, int kBlock, int kLds>
+struct TraceKernels {
+    static void extend(const PtScene& S, const WfState& W, int grid, int buf, int count, int work, hipStream_t st) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kSingle, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
+                           W, buf, count, work);
+    }
+    static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, hipStream_t st) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kSingle, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
+                           W, count, work);
+    }
+    static int occupancy() {
+        int a = 0, b = 0;
+        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kSingle, kBlock, kLds>, kBlock, 0));
+        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kSingle, kBlock, kLds>, kBlock, 0));
+        return std::max(1, std::min(a, b));
+    }
+};
+
+using ExtendFn = void (*)(const PtScene&, const WfState&, int, int, int, int, hipStream_t);
+using ShadowFn = void (*)(const PtScene&, const WfState&, int, int, int, hipStream_t);
+using OccFn = int (*)();
+
+struct TraceVariant {
+    ExtendFn extend;
+    ShadowFn shadow;
+    OccFn occupancy;
+};
+
+template <bool kSingle, int kBlock, int kLds>
+constexpr TraceVariant variant() {
+    return {&TraceKernels<kSingle, kBlock, kLds>::extend, &TraceKernels<kSingle, kBlock, kLds>::shadow,
+            &TraceKernels<kSingle, kBlock, kLds>::occupancy};
+}
+
+// Tuning space of the trace stages: block size x LDS stack depth.
+const TraceVariant kVariants[2][kWfNumTraceVariants] = {
+    {variant<false, 64, 8>(), variant<false, 64, 16>(), variant<false, 128, 8>(), variant<false, 128, 16>(),
+     variant<false, 256, 8>(), variant<false, 256, 16>(), variant<false, 256, 24>()},
+    {variant<true, 64, 8>(), variant<true, 64, 16>(), variant<true, 128, 8>(), variant<true, 128, 16>(),
+     variant<true, 256, 8>(), variant<true, 256, 16>(), variant<true, 256, 24>()},
+};
+
+}  // namespace
+
+const WfTraceShape kWfTraceShapes[kWfNumTraceVariants] = {{64, 8}, {64, 16}, {128, 8}, {128, 16},
+                                                           {256, 8}, {256, 16}, {256, 24}};
+
+int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs) {
+    const TraceVariant& v = kVariants[S.trace.numInstances == 1 ? 1 : 0][tune.traceVariant];
+    return v.occupancy() * numCUs;
+}
+
+int wf_shade_grid(int numCUs) {
+    int b = 0;
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shade, 256, 0));
+    return std::max(1, b) * numCUs;
+}
+
+void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, hipStream_t stream) {
+    const bool single = S.trace.numInstances == 1;
+    const TraceVariant& v = kVariants[single ? 1 : 0][tune.traceVariant];
+    hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
+    const dim3 pgrid((S.width + 15) / 16, (S.height + 15) / 16);
+    if (single)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<true>), pgrid, dim3(256), 0, stream, S, W);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<false>), pgrid, dim3(256), 0, stream, S, W);
+    // pt_sample's loop is a do-while: the first hit is shaded even at maximum path length 0
+    const int L = S.maxPathLength > 1 ? S.maxPathLength : 1;
+    int work = W.workBase;
+    for (int k = 0; k < L; ++k) {
+        const int buf = k & 1;
+        hipLaunchKernelGGL(k_wf_shade, dim3(tune.shadeGrid), dim3(256), 0, stream, S, W, k, buf, W.qBase + k,
+                           W.qBase + k + 1, W.shBase + k, work++);
+        v.shadow(S, W, tune.traceGrid, W.shBase + k, work++, stream);
+        if (k + 1 < L) v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, stream);
+    }
+    hipLaunchKernelGGL(k_wf_finish, dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S, W);
+    MFH_HIP(hipGetLastError());
+}
+
+}  // namespace mfh

@@ -1,0 +1,50 @@
+// Device state of the wavefront path tracer: per-path state, double-buffered
+// ray / hit queues, the shadow queue and the counters the persistent kernels
+// drain, all in HBM.
+#pragma once
+#include "pt_device.hpp"
+
+namespace mfh {
+
+struct WfState {
+    int numPaths;
+    uint32_t slotMask;  // accumulator slots of the enabled targets
+    int chunk;          // queue entries grabbed per wave (multiple of 64)
+    int numCounters;
+    int qBase;          // counters[qBase + k]: length of queue k (k = 0 .. maxPathLength)
+    int shBase;         // counters[shBase + k]: shadow slots of bounce k
+    int workBase;       // counters[workBase + j]: work counter of launch j
+    int* counters;
+    int* iteration;     // device iteration counter (index of the iteration being run)
+    float4* tp;         // per path: throughput xyz, footprint
+    float* acc;         // per path: 18 accumulator slots, slot-major
+    float4* rayO[2];    // per queue slot: origin xyz, forward pdf
+    float4* rayD[2];    // direction xyz, path index (int bits)
+    float4* rayM[2];    // medium absorption xyz, hit barycentric w1
+    float4* hit[2];     // t, instance, primitive (int bits), w0
+    float4* shO;        // per shadow entry (slot * neeCount + i): origin xyz, tmax
+    float4* shD;        // direction xyz
+    float4* shC;        // radiance xyz, lightness
+    int2* shPath;       // per shadow slot: path index, candidate bits
+};
+
+struct WfTraceShape {
+    int block, lds;
+};
+constexpr int kWfNumTraceVariants = 7;
+extern const WfTraceShape kWfTraceShapes[kWfNumTraceVariants];
+
+struct WfTuning {
+    int traceVariant = 5;  // 256 threads, 16 LDS stack entries
+    int chunk = 256;
+    int traceGrid = 0;
+    int shadeGrid = 0;
+};
+
+// Grid of the persistent trace kernels for a variant (occupancy x CUs) and of the shade kernel.
+int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs);
+int wf_shade_grid(int numCUs);
+// One iteration: begin, primary, maxPathLength x (shade, shadow, extend), finish.
+void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, hipStream_t stream);
+
+}  // namespace mfh

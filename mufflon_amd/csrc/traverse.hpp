@@ -83,20 +83,20 @@ struct TraceHit {
 // Traverse from (wo, wd) up to tmax. lstack: this lane's LDS stack column
 // (stride kStride ints). On return t is the world-space distance (or the
 // round-tripped tmax on a miss) and inst/prim are -1 on a miss.
-template <bool kAny, bool kStats, bool kSingle, int kStride>
+template <bool kAny, bool kStats, bool kSingle, int kStride, int kLds = kLdsDepth>
 __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd, float tmax, int* const lstack) {
-    int spill[kStackCap - kLdsDepth];
+    int spill[kStackCap - kLds];
     int sp = 0;
     bool overflow = false;
 #define MFH_PUSH(v)                                     \
     do {                                                \
-        if (sp < kLdsDepth)                             \
+        if (sp < kLds)                                  \
             lstack[sp * kStride] = (v);                 \
         else if (sp < kStackCap)                        \
-            spill[sp - kLdsDepth] = (v);                \
+            spill[sp - kLds] = (v);                     \
         if (sp < kStackCap) ++sp; else overflow = true; \
     } while (0)
-#define MFH_POP() (--sp, (sp < kLdsDepth) ? lstack[sp * kStride] : spill[sp - kLdsDepth])
+#define MFH_POP() (--sp, (sp < kLds) ? lstack[sp * kStride] : spill[sp - kLds])
 
     FastRay cur = make_fast_ray(wo, wd);
     float hitT = tmax;

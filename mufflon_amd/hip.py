@@ -50,6 +50,9 @@ TARGET_UV = 5
 TARGET_ALBEDO = 6
 TARGET_LIGHTNESS = 7
 TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)
+PT_MEGAKERNEL = 0
+PT_WAVEFRONT = 1
+TUNING_DTYPE = np.dtype([("traceBlock", "<i4"), ("stackDepth", "<i4"), ("queueChunk", "<i4")])
 
 # name -> argtypes; every function returns int status
 _SIGS = {
@@ -75,6 +78,10 @@ _SIGS = {
     "mfh_pt_iterate": [vp, i32, i32],
     "mfh_pt_get_target": [vp, i32, i32, f32p],
     "mfh_pt_enable_target": [vp, i32, i32, i32],
+    "mfh_pt_set_mode": [vp, i32, i32, i32],
+    "mfh_pt_set_tuning": [vp, i32, vp],
+    "mfh_pt_get_tuning": [vp, i32, vp],
+    "mfh_pt_autotune": [vp, i32, i32, vp, f32p],
     "mfh_pt_get_iteration": [vp, i32, i32p],
     "mfh_pt_get_nan_count": [vp, i32, u32p],
     "mfh_pt_get_light_tree": [vp, i32, vp, vp, u64p, u32p],
@@ -407,6 +414,25 @@ class HipContext:
 
     def pt_iterate(self, pt, iterations=1):
         self._call("mfh_pt_iterate", self.handle, pt, int(iterations))
+
+    def pt_set_mode(self, pt, mode, use_graph=False):
+        self._call("mfh_pt_set_mode", self.handle, pt, mode, 1 if use_graph else 0)
+
+    def pt_set_tuning(self, pt, trace_block=256, stack_depth=16, queue_chunk=256):
+        r = np.zeros(1, TUNING_DTYPE)
+        r["traceBlock"], r["stackDepth"], r["queueChunk"] = trace_block, stack_depth, queue_chunk
+        self._call("mfh_pt_set_tuning", self.handle, pt, r.ctypes.data)
+
+    def pt_get_tuning(self, pt):
+        r = np.zeros(1, TUNING_DTYPE)
+        self._call("mfh_pt_get_tuning", self.handle, pt, r.ctypes.data)
+        return {k: int(r[k][0]) for k in TUNING_DTYPE.names}
+
+    def pt_autotune(self, pt, iterations=2):
+        r = np.zeros(1, TUNING_DTYPE)
+        ms = np.zeros(1, np.float32)
+        self._call("mfh_pt_autotune", self.handle, pt, iterations, r.ctypes.data, _ptr(ms, f32p))
+        return {k: int(r[k][0]) for k in TUNING_DTYPE.names}, float(ms[0])
 
     def pt_enable_target(self, pt, target, enable=True):
         self._call("mfh_pt_enable_target", self.handle, pt, target, 1 if enable else 0)

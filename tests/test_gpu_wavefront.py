@@ -1,0 +1,87 @@
+"""Wavefront form of the path tracer (persistent queue kernels, optional HIP
+graph replay) against the megakernel and the oracle: identical bits for every
+target, scene kind and launch shape."""
+import numpy as np
+import pytest
+
+import pt_util as P
+from mufflon_amd import hip
+from mufflon_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _targets(ctx, pt, targets):
+    return {t: ctx.pt_target(pt, t) for t in targets}
+
+
+def _same(a, b):
+    return all(np.array_equal(a[t].view(np.uint32), b[t].view(np.uint32)) for t in a)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_wavefront_equals_megakernel_material_zoo(hip_ctx, use_graph):
+    c = S.cornell_materials(12, 24, 40, 40)
+    print("wavefront vs megakernel, material zoo, graph=%s" % use_graph, flush=True)
+    targets = list(range(8))
+    out = []
+    for mode in (hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT):
+        pt, sc, bg = P.gpu_pt_materials(hip_ctx, c, "envmap")
+        for t in targets:
+            hip_ctx.pt_enable_target(pt, t, True)
+        hip_ctx.pt_set_mode(pt, mode, use_graph)
+        hip_ctx.pt_iterate(pt, 3)
+        assert hip_ctx.pt_iteration(pt) == 2
+        out.append((_targets(hip_ctx, pt, targets), hip_ctx.pt_nan_count(pt)))
+    for t in targets:
+        print("target %d identical: %s" % (t, np.array_equal(out[0][0][t].view(np.uint32),
+                                                              out[1][0][t].view(np.uint32))), flush=True)
+    assert _same(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+
+
+def test_wavefront_matches_oracle_cornell(hip_ctx, oracle):
+    c = P.cornell(16, 32, 48)
+    print("wavefront vs oracle, cornell 48x48", flush=True)
+    pt = P.gpu_pt(hip_ctx, c, nee_count=2, max_path_length=6)
+    hip_ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
+    hip_ctx.pt_iterate(pt, 5)
+    ps = P.oracle_pt_scene(oracle, c, hip_ctx.pt_light_tree(pt))
+    o_rad, o_hit, _ = oracle.pt_render(ps, 1, 0, 6, 2, 5)
+    assert np.array_equal(hip_ctx.pt_target(pt, hip.TARGET_HIT_ID), o_hit)
+    assert np.array_equal(hip_ctx.pt_target(pt).view(np.uint32), o_rad.view(np.uint32))
+
+
+def test_wavefront_instanced_and_every_tuning(hip_ctx):
+    """Instance level (TLAS traversal in the queue kernels) under every
+    trace block / stack depth and two chunk sizes, plus reset and continue."""
+    c = P.cornell_instanced(32)
+    print("wavefront instanced, tuning sweep", flush=True)
+    pt, sc = P.gpu_pt_instanced(hip_ctx, c)
+    hip_ctx.pt_iterate(pt, 2)
+    ref = hip_ctx.pt_target(pt)
+    hip_ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
+    for block, depth in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8), (256, 16), (256, 24)]:
+        for chunk in (64, 1024):
+            hip_ctx.pt_set_tuning(pt, block, depth, chunk)
+            hip_ctx.pt_reset(pt)
+            hip_ctx.pt_iterate(pt, 1)
+            hip_ctx.pt_iterate(pt, 1)
+            got = hip_ctx.pt_target(pt)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, chunk)
+    assert hip_ctx.pt_get_tuning(pt) == dict(traceBlock=256, stackDepth=24, queueChunk=1024)
+
+
+def test_autotune_picks_a_shape_and_resets(hip_ctx):
+    c = P.cornell(8, 16, 32)
+    pt = P.gpu_pt(hip_ctx, c)
+    hip_ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
+    best, ms = hip_ctx.pt_autotune(pt, 1)
+    print("autotune best %s %.3f ms" % (best, ms), flush=True)
+    assert ms > 0 and best["queueChunk"] in (64, 256, 1024)
+    assert hip_ctx.pt_iteration(pt) == -1
+    hip_ctx.pt_iterate(pt, 2)
+    a = hip_ctx.pt_target(pt)
+    hip_ctx.pt_set_mode(pt, hip.PT_MEGAKERNEL)
+    hip_ctx.pt_reset(pt)
+    hip_ctx.pt_iterate(pt, 2)
+    assert np.array_equal(a.view(np.uint32), hip_ctx.pt_target(pt).view(np.uint32))
