@@ -264,18 +264,21 @@ int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t
 #define MFH_PT_WAVEFRONT 1
 int mfh_pt_set_mode(mfh_context* ctx, int32_t ptId, int32_t mode, int32_t useGraph);
 /* Launch shape of the wavefront trace stages: block size x LDS stack depth
- * (64x8, 64x16, 128x8, 128x16, 256x8, 256x16, 256x24) and the queue entries
- * a wave grabs at once (multiple of 64). */
+ * (64x8, 64x16, 128x8, 128x16, 256x8, 256x16, 256x24), the largest queue
+ * chunk a wave grabs at once (multiple of 64; smaller for short queues), and
+ * the number of leading bounces whose extension and shadow rays are traced
+ * in sorted order (origin cell, direction octant). */
 typedef struct mfh_wavefront_tuning {
     int32_t traceBlock;
     int32_t stackDepth;
     int32_t queueChunk;
+    int32_t sortBounces;
 } mfh_wavefront_tuning;
 int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning* tuning);
 int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* outTuning);
 /* Times `iterations` wavefront iterations for every tuning candidate (block x
- * stack depth x chunk 64/256/1024), keeps the fastest and resets the path
- * tracer. */
+ * stack depth x chunk 256/1024 x sorted bounces 0/1/2), keeps the fastest and
+ * resets the path tracer. */
 int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wavefront_tuning* outBest,
                     float* outMsPerIteration);
 /* Running mean of an enabled target over the completed iterations (mufflon_get_target_image). */

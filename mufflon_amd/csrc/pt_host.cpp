@@ -28,7 +28,7 @@ struct PathTracer {
     int32_t mode = MFH_PT_MEGAKERNEL;
     bool useGraph = false;
     WfTuning tune;
-    DevBuf wfCounters, wfTp, wfAcc, wfRay[2][4], wfSh[3], wfShPath;
+    DevBuf wfCounters, wfTp, wfAcc, wfRay[2][4], wfSh[3], wfShPath, wfSortKeys[2], wfSortIds[2];
     int wfPaths = 0, wfMaxLen = -1, wfNee = -1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -45,7 +45,9 @@ void drop_graph(PathTracer& t) {
 
 void release_wavefront(PathTracer& t) {
     drop_graph(t);
-    for (DevBuf* b : {&t.wfCounters, &t.wfTp, &t.wfAcc, &t.wfSh[0], &t.wfSh[1], &t.wfSh[2], &t.wfShPath}) b->release();
+    for (DevBuf* b : {&t.wfCounters, &t.wfTp, &t.wfAcc, &t.wfSh[0], &t.wfSh[1], &t.wfSh[2], &t.wfShPath,
+                      &t.wfSortKeys[0], &t.wfSortKeys[1], &t.wfSortIds[0], &t.wfSortIds[1]})
+        b->release();
     for (auto& r : t.wfRay)
         for (DevBuf& b : r) b.release();
     t.wfPaths = 0;
@@ -269,6 +271,10 @@ WfState wavefront_state(Context& c, PathTracer& t) {
             for (DevBuf& b : r) b.reserve(16 * (size_t)N);
         for (DevBuf& b : t.wfSh) b.reserve(16 * (size_t)N * nee);
         t.wfShPath.reserve(8 * (size_t)N);
+        for (int b = 0; b < 2; ++b) {
+            t.wfSortKeys[b].reserve(8 * (size_t)N);
+            t.wfSortIds[b].reserve(4 * (size_t)N);
+        }
         t.wfPaths = N;
         t.wfMaxLen = L;
         t.wfNee = nee;
@@ -303,6 +309,14 @@ WfState wavefront_state(Context& c, PathTracer& t) {
     W.shD = t.wfSh[1].as<float4>();
     W.shC = t.wfSh[2].as<float4>();
     W.shPath = t.wfShPath.as<int2>();
+    for (int b = 0; b < 2; ++b) {
+        W.sortKeys[b] = t.wfSortKeys[b].as<uint64_t>();
+        W.sortIds[b] = t.wfSortIds[b].as<int32_t>();
+    }
+    // radix-sort scratch of the context sized up front (no allocation inside a captured graph)
+    const size_t tiles = ((size_t)N + 4095) / 4096;
+    c.scratch.hist.reserve(4 * 256 * tiles);
+    c.scratch.partials.reserve(4 * (256 * tiles / 2048 + 64));
     if (t.tune.traceGrid <= 0 || t.tune.shadeGrid <= 0) {
         const int cus = num_cus(c);
         t.tune.traceGrid = 0;
@@ -321,7 +335,7 @@ void run_wavefront(Context& c, PathTracer& t, PtScene& S, int32_t iterations) {
     if (!t.useGraph) {
         for (int32_t k = 0; k < iterations; ++k) {
             S.iteration = ++t.iteration;
-            launch_pt_wavefront(S, W, t.tune, c.stream);
+            launch_pt_wavefront(S, W, t.tune, c.scratch, c.stream);
         }
         MFH_HIP(hipStreamSynchronize(c.stream));  // the host copy of the iteration counter is read above
         return;
@@ -337,7 +351,7 @@ void run_wavefront(Context& c, PathTracer& t, PtScene& S, int32_t iterations) {
         MFH_HIP(hipStreamSynchronize(c.stream));
         MFH_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         try {
-            launch_pt_wavefront(Sk, W, t.tune, c.stream);
+            launch_pt_wavefront(Sk, W, t.tune, c.scratch, c.stream);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(c.stream, &g);
@@ -492,8 +506,10 @@ int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning
         MFH_REQUIRE(tuning->queueChunk >= 64 && tuning->queueChunk % 64 == 0 && tuning->queueChunk <= 65536,
                     "queue chunk must be a multiple of 64 in [64, 65536]");
         MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_REQUIRE(tuning->sortBounces >= 0 && tuning->sortBounces <= 64, "sortBounces out of range");
         t.tune.traceVariant = v;
         t.tune.chunk = tuning->queueChunk;
+        t.tune.sortBounces = tuning->sortBounces;
         t.tune.traceGrid = 0;
         drop_graph(t);
     });
@@ -507,6 +523,7 @@ int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* out)
         out->traceBlock = kWfTraceShapes[t.tune.traceVariant].block;
         out->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
         out->queueChunk = t.tune.chunk;
+        out->sortBounces = t.tune.sortBounces;
     });
 }
 
@@ -519,13 +536,16 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
         MFH_REQUIRE(t.mode == MFH_PT_WAVEFRONT, "autotune applies to the wavefront form");
         MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "scene without lights");
         PtScene S = device_scene(c, t);
-        const int chunks[3] = {64, 256, 1024};
+        const int chunks[2] = {256, 1024};
+        const int sorts[3] = {0, 1, 2};
         float best = 3.4e38f;
         WfTuning bestTune = t.tune;
         for (int v = 0; v < kWfNumTraceVariants; ++v) {
-            for (int ch : chunks) {
+            for (int cs = 0; cs < 6; ++cs) {
+                const int ch = chunks[cs % 2];
                 t.tune.traceVariant = v;
                 t.tune.chunk = ch;
+                t.tune.sortBounces = sorts[cs / 2];
                 t.tune.traceGrid = 0;
                 drop_graph(t);
                 run_wavefront(c, t, S, 1);  // warm-up (and graph capture)
@@ -549,6 +569,7 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
             outBest->traceBlock = kWfTraceShapes[t.tune.traceVariant].block;
             outBest->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
             outBest->queueChunk = t.tune.chunk;
+            outBest->sortBounces = t.tune.sortBounces;
         }
         if (outMsPerIteration) *outMsPerIteration = best;
     });

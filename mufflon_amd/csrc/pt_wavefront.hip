@@ -52,6 +52,32 @@ __device__ __forceinline__ int wave_grab(int* work, int chunk) {
     return __shfl(base, 0);
 }
 
+// Queue walk of a persistent grid: a wave's first chunk is its wave index
+// (no atomic, so waves beyond a short queue exit at once); later chunks come
+// from the work counter, offset past the statically assigned ones.
+struct QueueWalk {
+    int chunk, waves, base;
+    __device__ __forceinline__ QueueWalk(int n, int maxChunk);
+    __device__ __forceinline__ void next(int* work) { base = waves * chunk + wave_grab(work, chunk); }
+};
+
+// Chunk for a queue of n entries: about kChunksPerWave grabs per wave of the
+// grid (a multiple of 64, at least 64, at most the tuned maximum), so short
+// queues still spread over every wave and long ones need few atomics.
+constexpr int kChunksPerWave = 4;
+__device__ __forceinline__ int queue_chunk(int n, int maxChunk) {
+    const int waves = (int)(gridDim.x * blockDim.x) / 64;
+    int c = (n + waves * kChunksPerWave - 1) / (waves * kChunksPerWave);
+    c = (c + 63) & ~63;
+    return c < 64 ? 64 : (c > maxChunk ? maxChunk : c);
+}
+
+__device__ __forceinline__ QueueWalk::QueueWalk(int n, int maxChunk) {
+    chunk = queue_chunk(n, maxChunk);
+    waves = (int)(gridDim.x * blockDim.x) / 64;
+    base = (int)((blockIdx.x * blockDim.x + threadIdx.x) / 64) * chunk;
+}
+
 // Per-path accumulator in HBM: slot-major (slot * N + path).
 __device__ __forceinline__ void load_acc(const WfState& W, u32 mask, int p, Acc& acc) {
     acc.init(mask);
@@ -119,15 +145,15 @@ __global__ __launch_bounds__(256) void k_wf_primary(PtScene S, WfState W) {
 
 // Closest hit of queue `buf` (length at counters[count]).
 template <bool kSingle, int kBlock, int kLds>
-__global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int buf, int count, int work) {
+__global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int buf, int count, int work,
+                                                      const int* perm) {
     __shared__ int sStack[kLds * kBlock];
     const int n = W.counters[count];
     int* const lstack = sStack + threadIdx.x;
-    for (;;) {
-        const int base = wave_grab(W.counters + work, W.chunk);
-        if (base >= n) break;
-        const int end = min(base + W.chunk, n);
-        for (int q = base + lane_id(); q < end; q += 64) {
+    for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
+        const int end = min(wk.base + wk.chunk, n);
+        for (int qs = wk.base + lane_id(); qs < end; qs += 64) {
+            const int q = perm ? perm[qs] : qs;
             const float4 o = W.rayO[buf][q];
             const float4 d = W.rayD[buf][q];
             const TraceHit h = trace_ray<false, false, kSingle, kBlock, kLds>(S.trace, v3(o.x, o.y, o.z),
@@ -141,21 +167,18 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int 
 // Any hit of the shadow candidates of one bounce; visible ones are added to
 // their path's accumulator in candidate order.
 template <bool kSingle, int kBlock, int kLds>
-__global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S, WfState W, int count, int work) {
+__global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S, WfState W, int count, int work, const int* perm) {
     __shared__ int sStack[kLds * kBlock];
     const int n = W.counters[count];
     int* const lstack = sStack + threadIdx.x;
     u32 nans = 0;
-    for (;;) {
-        const int base = wave_grab(W.counters + work, W.chunk);
-        if (base >= n) break;
-        const int end = min(base + W.chunk, n);
-        for (int s = base + lane_id(); s < end; s += 64) {
+    for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
+        const int end = min(wk.base + wk.chunk, n);
+        for (int ss = wk.base + lane_id(); ss < end; ss += 64) {
+            const int s = perm ? perm[ss] : ss;
             const int2 pm = W.shPath[s];
-            const int p = pm.x;
             const u32 cand = (u32)pm.y;
-            Acc acc;
-            bool loaded = false;
+            u32 visible = 0;
             for (int i = 0; i < S.neeCount; ++i) {
                 if (!((cand >> i) & 1u)) continue;
                 const size_t e = (size_t)s * S.neeCount + i;
@@ -163,20 +186,20 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S, WfState W, int 
                 const float4 d = W.shD[e];
                 const TraceHit h = trace_ray<true, false, kSingle, kBlock, kLds>(S.trace, v3(a.x, a.y, a.z),
                                                                                   v3(d.x, d.y, d.z), a.w, lstack);
-                if (h.inst == -1) {
-                    if (!loaded) {
-                        load_acc(W, S.targetMask, p, acc);
-                        loaded = true;
-                    }
-                    const float4 c = W.shC[e];
+                if (h.inst == -1) visible |= 1u << i;
+            }
+            if (visible) {  // add the unoccluded estimates in candidate order
+                Acc acc;
+                load_acc(W, S.targetMask, pm.x, acc);
+                for (int i = 0; i < S.neeCount; ++i) {
+                    if (!((visible >> i) & 1u)) continue;
+                    const float4 c = W.shC[(size_t)s * S.neeCount + i];
                     NeeCandidate nc;
                     nc.radiance = v3(c.x, c.y, c.z);
                     nc.lightness = c.w;
                     add_nee(nc, acc);
                 }
-            }
-            if (loaded) {
-                store_acc(W, S.targetMask, p, acc);
+                store_acc(W, S.targetMask, pm.x, acc);
                 nans += acc.nans;
             }
         }
@@ -191,11 +214,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, i
     const int pathLen = k + 1;
     const int nbuf = buf ^ 1;
     u32 nans = 0;
-    for (;;) {
-        const int base = wave_grab(W.counters + work, W.chunk);
-        if (base >= n) break;
-        const int end = base + W.chunk;
-        for (int q0 = base; q0 < end; q0 += 64) {
+    for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
+        const int end = wk.base + wk.chunk;
+        for (int q0 = wk.base; q0 < end; q0 += 64) {
             const int q = q0 + lane_id();
             const bool active = q < n;
             bool doNee = false, cont = false;
@@ -268,6 +289,57 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, i
     if (nans) atomicAdd(S.nanCounter, nans);
 }
 
+// 9-bit-per-axis Morton code of the origin in the scene box, then the
+// direction octant: rays from nearby origins heading the same way become
+// neighbours. Entries past the queue length sort last.
+__device__ __forceinline__ u32 spread9(u32 x) {
+    x &= 0x1FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+__device__ __forceinline__ u64 ray_key(const PtScene& S, float4 o, float4 d) {
+    u32 q[3];
+    const float* b = S.sceneBox;
+    const float oc[3] = {o.x, o.y, o.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float span = b[3 + a] - b[a];
+        const float f = span > 0.0f ? (oc[a] - b[a]) / span : 0.0f;
+        const float c = f * 512.0f;
+        q[a] = c <= 0.0f ? 0u : (c >= 511.0f ? 511u : (u32)c);  // NaN -> 0
+    }
+    const u32 m = spread9(q[0]) | (spread9(q[1]) << 1) | (spread9(q[2]) << 2);
+    const u32 oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return (u64)((m << 3) | oct);
+}
+// shadow = false: extension queue `buf`; true: the shadow slots (first candidate's segment)
+__global__ void k_wf_sort_keys(PtScene S, WfState W, int buf, int count, bool shadow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W.numPaths) return;
+    const int n = W.counters[count];
+    // every entry of the queue sorts before every entry past it, so the first n
+    // sorted ids are a permutation of [0, n)
+    u64 key = 0xFFFFFFFFull;
+    if (i < n) {
+        key = 0xFFFFFFFEull;
+        if (!shadow) {
+            key = ray_key(S, W.rayO[buf][i], W.rayD[buf][i]);
+        } else {
+            const u32 cand = (u32)W.shPath[i].y;
+            if (cand) {
+                const int first = __ffs((int)cand) - 1;
+                const size_t e = (size_t)i * S.neeCount + first;
+                key = ray_key(S, W.shO[e], W.shD[e]);
+            }
+        }
+    }
+    W.sortKeys[0][i] = key;
+    W.sortIds[0][i] = i;
+}
+
 __global__ void k_wf_finish(PtScene S, WfState W) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= W.numPaths) return;
@@ -279,13 +351,15 @@ __global__ void k_wf_finish(PtScene S, WfState W) {
 
 template <bool kSingle, int kBlock, int kLds>
 struct TraceKernels {
-    static void extend(const PtScene& S, const WfState& W, int grid, int buf, int count, int work, hipStream_t st) {
+    static void extend(const PtScene& S, const WfState& W, int grid, int buf, int count, int work, const int* perm,
+                       hipStream_t st) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kSingle, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
-                           W, buf, count, work);
+                           W, buf, count, work, perm);
     }
-    static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, hipStream_t st) {
+    static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, const int* perm,
+                       hipStream_t st) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kSingle, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
-                           W, count, work);
+                           W, count, work, perm);
     }
     static int occupancy() {
         int a = 0, b = 0;
@@ -295,8 +369,8 @@ struct TraceKernels {
     }
 };
 
-using ExtendFn = void (*)(const PtScene&, const WfState&, int, int, int, int, hipStream_t);
-using ShadowFn = void (*)(const PtScene&, const WfState&, int, int, int, hipStream_t);
+using ExtendFn = void (*)(const PtScene&, const WfState&, int, int, int, int, const int*, hipStream_t);
+using ShadowFn = void (*)(const PtScene&, const WfState&, int, int, int, const int*, hipStream_t);
 using OccFn = int (*)();
 
 struct TraceVariant {
@@ -335,7 +409,16 @@ int wf_shade_grid(int numCUs) {
     return std::max(1, b) * numCUs;
 }
 
-void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, hipStream_t stream) {
+// Sorted order of a queue: keys, then a 4-pass LSD radix sort over every path
+// slot (the queue length stays on the device); the permutation ends in sortIds[0].
+void sort_queue(const PtScene& S, const WfState& W, int buf, int count, bool shadow, BuildScratch& scratch,
+                hipStream_t stream) {
+    hipLaunchKernelGGL(k_wf_sort_keys, dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S, W, buf, count, shadow);
+    radix_sort_pairs(W.sortKeys[0], W.sortKeys[1], W.sortIds[0], W.sortIds[1], W.numPaths, 0, 32, scratch, stream);
+}
+
+void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, BuildScratch& scratch,
+                         hipStream_t stream) {
     const bool single = S.trace.numInstances == 1;
     const TraceVariant& v = kVariants[single ? 1 : 0][tune.traceVariant];
     hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
@@ -351,8 +434,13 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
         const int buf = k & 1;
         hipLaunchKernelGGL(k_wf_shade, dim3(tune.shadeGrid), dim3(256), 0, stream, S, W, k, buf, W.qBase + k,
                            W.qBase + k + 1, W.shBase + k, work++);
-        v.shadow(S, W, tune.traceGrid, W.shBase + k, work++, stream);
-        if (k + 1 < L) v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, stream);
+        const bool sortNow = k < tune.sortBounces;
+        if (sortNow) sort_queue(S, W, 0, W.shBase + k, true, scratch, stream);
+        v.shadow(S, W, tune.traceGrid, W.shBase + k, work++, sortNow ? W.sortIds[0] : nullptr, stream);
+        if (k + 1 < L) {
+            if (sortNow) sort_queue(S, W, buf ^ 1, W.qBase + k + 1, false, scratch, stream);
+            v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, sortNow ? W.sortIds[0] : nullptr, stream);
+        }
     }
     hipLaunchKernelGGL(k_wf_finish, dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S, W);
     MFH_HIP(hipGetLastError());

@@ -26,6 +26,9 @@ struct WfState {
     float4* shD;        // direction xyz
     float4* shC;        // radiance xyz, lightness
     int2* shPath;       // per shadow slot: path index, candidate bits
+    // ray sorting: keys / ids of the radix sort (numPaths each), result in sortIds[0]
+    uint64_t* sortKeys[2];
+    int32_t* sortIds[2];
 };
 
 struct WfTraceShape {
@@ -37,6 +40,7 @@ extern const WfTraceShape kWfTraceShapes[kWfNumTraceVariants];
 struct WfTuning {
     int traceVariant = 5;  // 256 threads, 16 LDS stack entries
     int chunk = 256;
+    int sortBounces = 0;   // extension / shadow queues of the first bounces traced in sorted order
     int traceGrid = 0;
     int shadeGrid = 0;
 };
@@ -45,6 +49,7 @@ struct WfTuning {
 int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs);
 int wf_shade_grid(int numCUs);
 // One iteration: begin, primary, maxPathLength x (shade, shadow, extend), finish.
-void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, hipStream_t stream);
+void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, BuildScratch& scratch,
+                         hipStream_t stream);
 
 }  // namespace mfh
