@@ -151,6 +151,10 @@ typedef struct mfh_material {
 int mfh_texture_create(mfh_context* ctx, int32_t width, int32_t height, int32_t channels, int32_t samplingMode,
                        const float* texels, int32_t* outTextureId);
 
+/* Replace the texels of a texture (same size and channels). Path tracers
+ * read the new values from their next iteration; reset them to restart. */
+int mfh_texture_update(mfh_context* ctx, int32_t textureId, const float* texels);
+
 /* Background (BackgroundDesc, background.hpp; world_set_env_light...):
  * COLORED = constant radiance color * scale; ENVMAP = polar (equirectangular)
  * map texture * scale, importance sampled through a summed-area table. */
@@ -306,6 +310,24 @@ int mfh_background_build(const mfh_background* bg, const float* envTexels, int32
  * the light-tree background of LightTreeBuilder, light_tree.cpp:592-597) and
  * resets it. The default background is COLORED black. */
 int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* bg);
+
+/* Adjoint (path replay) of the Radiance target. No counterpart in the
+ * reference (it has no differentiable renderer). For the image of
+ * `iterations` iterations from a reset, and radianceAdjoint = dLoss/dRadiance
+ * (host, 3 floats per pixel, row-major), replays the same paths and
+ * accumulates dLoss/d(texel) of every albedo texture (LAMBERT, TORRANCE) and
+ * dLoss/d(emission scale) of every material: the derivative of the
+ * estimator with its random numbers fixed (Russian roulette included;
+ * light-selection probabilities held fixed, exact for one emissive material).
+ * Maximum path length 32. Asynchronous; read with the getters below. */
+int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const float* radianceAdjoint);
+/* Gradient of an albedo texture: width * height * channels floats. */
+int mfh_pt_get_texture_gradient(mfh_context* ctx, int32_t ptId, int32_t textureId, float* out);
+/* Gradient with respect to a material's emission scale: 3 floats. */
+int mfh_pt_get_emission_gradient(mfh_context* ctx, int32_t ptId, int32_t materialIndex, float* out);
+/* Replace one material of a path tracer (world_get_material + material
+ * edits); the light tree is rebuilt and the path tracer reset. */
+int mfh_pt_set_material(mfh_context* ctx, int32_t ptId, int32_t materialIndex, const mfh_material* material);
 
 /* Event timing on the context's stream: begin records an event, end records
  * a second one, waits for it and returns the elapsed milliseconds. */

@@ -18,8 +18,8 @@ LIBNAME = os.path.join(LIB, "libmufflon_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "capi.cpp", "light_tree.cpp",
-           "pt_host.cpp"]
+SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "pt_adjoint.hip", "capi.cpp",
+           "light_tree.cpp", "pt_host.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

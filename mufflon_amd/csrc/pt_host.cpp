@@ -7,6 +7,7 @@
 #include "capi_util.hpp"
 #include "light_tree.hpp"
 #include "pt_device.hpp"
+#include "pt_adjoint.hpp"
 #include "pt_wavefront.hpp"
 
 namespace mfh {
@@ -18,6 +19,7 @@ struct PathTracer {
     mfh_pt_params params{};
     LightTreeHost tree;
     BackgroundHost background;
+    std::vector<HostPointLight> pointLights;
     DevBuf dMaterials, dPtLods, dLightMem, dMapKeys, dMapCodes, dTextures, dSat;
     DevBuf dRngs, dNan;
     DevBuf dTargets[kNumTargets];
@@ -33,6 +35,10 @@ struct PathTracer {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     std::vector<uint8_t> graphKey;  // bytes of (S, W, tune) the graph was captured with
+    // adjoint
+    DevBuf adjRngs, adjDelta, adjEmGrad, adjTexGradPtrs;
+    std::vector<DevBuf> adjTexGrad;  // per texture id
+    bool adjValid = false;
 };
 
 void drop_graph(PathTracer& t) {
@@ -62,6 +68,8 @@ void destroy_tracers(Context& c) {
             b->release();
         for (DevBuf& b : t->dTargets) b.release();
         release_wavefront(*t);
+        for (DevBuf* b : {&t->adjRngs, &t->adjDelta, &t->adjEmGrad, &t->adjTexGradPtrs}) b->release();
+        for (DevBuf& b : t->adjTexGrad) b.release();
         delete t;
     }
     c.tracers.clear();
@@ -423,6 +431,7 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
                 std::memcpy(pl[i].intensity, pointLights[i].intensity, 12);
             }
             const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t->materials);
+            t->pointLights = pl;
             build_light_tree(area, pl, sc.box, MaterialEmission(t->materials), t->tree);
             const float black[3] = {0.0f, 0.0f, 0.0f};
             build_background(MFH_BACKGROUND_COLORED, black, 1.0f, nullptr, 0, 0, sc.box, t->background);
@@ -477,6 +486,104 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations) {
             S.iteration = ++t.iteration;
             launch_pt_iterate(S, c.stream);
         }
+    });
+}
+
+int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const float* radianceAdjoint) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(radianceAdjoint != nullptr, "null adjoint image");
+        MFH_REQUIRE(iterations > 0, "the adjoint needs at least one iteration");
+        MFH_REQUIRE(t.params.maxPathLength <= kAdjMaxVertices,
+                    "the adjoint replays paths of at most " + std::to_string(kAdjMaxVertices) + " vertices");
+        MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "scene without lights");
+        PtScene S = device_scene(c, t);
+        const int num = t.camera.width * t.camera.height;
+        t.adjRngs.reserve(8 * (size_t)num);
+        upload(c, t.adjDelta, radianceAdjoint, 12 * (size_t)num);
+        const size_t nm = t.materials.size();
+        t.adjEmGrad.reserve(12 * nm);
+        MFH_HIP(hipMemsetAsync(t.adjEmGrad.ptr, 0, 12 * nm, c.stream));
+        // gradient buffers for every texture used as an albedo
+        t.adjTexGrad.resize(c.textures.size());
+        std::vector<float*> ptrs(std::max<size_t>(c.textures.size(), 1), nullptr);
+        for (const PtMaterialRecord& m : t.materials) {
+            if ((m.type == kMatLambert || m.type == kMatTorrance) && m.albedoTexture >= 0) {
+                const Texture& x = *c.textures[m.albedoTexture];
+                DevBuf& g = t.adjTexGrad[m.albedoTexture];
+                g.reserve(16 * (size_t)x.width * x.height);
+                ptrs[m.albedoTexture] = g.as<float>();
+            }
+        }
+        for (size_t k = 0; k < c.textures.size(); ++k)
+            if (ptrs[k]) MFH_HIP(hipMemsetAsync(ptrs[k], 0, 16 * (size_t)c.textures[k]->width * c.textures[k]->height,
+                                                c.stream));
+        upload(c, t.adjTexGradPtrs, ptrs.data(), sizeof(float*) * ptrs.size());
+        AdjointState A;
+        A.delta = t.adjDelta.as<float>();
+        A.iterations = iterations;
+        A.rngs = t.adjRngs.as<uint64_t>();
+        A.texGrad = t.adjTexGradPtrs.as<float* const>();
+        A.emissionGrad = t.adjEmGrad.as<float>();
+        launch_pt_adjoint(S, A, t.params.seed * (num + 1), c.stream);
+        t.adjValid = true;
+    });
+}
+
+int mfh_pt_get_texture_gradient(mfh_context* ctx, int32_t ptId, int32_t textureId, float* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output");
+        MFH_REQUIRE(t.adjValid, "no adjoint has been run");
+        MFH_REQUIRE(textureId >= 0 && textureId < (int32_t)t.adjTexGrad.size() && t.adjTexGrad[textureId].ptr,
+                    "the texture has no gradient (not an albedo texture of this path tracer)");
+        const Texture& x = *c.textures[textureId];
+        const size_t n = (size_t)x.width * x.height;
+        std::vector<float> rgba(4 * n);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipMemcpy(rgba.data(), t.adjTexGrad[textureId].ptr, 16 * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < x.channels; ++k) out[(size_t)x.channels * i + k] = rgba[4 * i + k];
+    });
+}
+
+int mfh_pt_get_emission_gradient(mfh_context* ctx, int32_t ptId, int32_t materialIndex, float* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output");
+        MFH_REQUIRE(t.adjValid, "no adjoint has been run");
+        MFH_REQUIRE(materialIndex >= 0 && materialIndex < (int32_t)t.materials.size(), "material index out of range");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipMemcpy(out, t.adjEmGrad.as<float>() + 3 * (size_t)materialIndex, 12, hipMemcpyDeviceToHost));
+    });
+}
+
+int mfh_pt_set_material(mfh_context* ctx, int32_t ptId, int32_t materialIndex, const mfh_material* material) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(material != nullptr, "null material");
+        MFH_REQUIRE(materialIndex >= 0 && materialIndex < (int32_t)t.materials.size(), "material index out of range");
+        validate_material(c, *material);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        std::memcpy(&t.materials[materialIndex], material, sizeof(mfh_material));
+        Scene& sc = scene_of(c, t.scene);
+        std::vector<HostPointLight> pl = t.pointLights;
+        const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t.materials);
+        build_light_tree(area, pl, sc.box, MaterialEmission(t.materials), t.tree);
+        t.tree.backgroundFlux = t.background.fluxSum;
+        upload(c, t.dMaterials, t.materials.data(), sizeof(PtMaterialRecord) * t.materials.size());
+        upload(c, t.dLightMem, t.tree.mem.data(), std::max<size_t>(t.tree.mem.size(), 16));
+        const size_t nm = t.tree.mapKeys.size();
+        if (nm) {
+            upload(c, t.dMapKeys, t.tree.mapKeys.data(), 8 * nm);
+            upload(c, t.dMapCodes, t.tree.mapCodes.data(), 4 * nm);
+        }
+        drop_graph(t);
+        reset(c, t);
     });
 }
 
@@ -670,6 +777,21 @@ int mfh_texture_create(mfh_context* ctx, int32_t width, int32_t height, int32_t 
         }
         c.textures.push_back(t);
         *outTextureId = (int32_t)c.textures.size() - 1;
+    });
+}
+
+int mfh_texture_update(mfh_context* ctx, int32_t textureId, const float* texels) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(texels != nullptr, "null texels");
+        MFH_REQUIRE(textureId >= 0 && textureId < (int32_t)c.textures.size() && c.textures[textureId],
+                    "unknown texture id");
+        Texture& t = *c.textures[textureId];
+        const size_t n = (size_t)t.width * t.height;
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < t.channels; ++k) t.host[4 * i + k] = texels[(size_t)t.channels * i + k];
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipMemcpy(t.dev.ptr, t.host.data(), 16 * n, hipMemcpyHostToDevice));
     });
 }
 

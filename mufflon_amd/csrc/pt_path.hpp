@@ -138,17 +138,11 @@ __device__ __forceinline__ void make_vertex(const PtScene& S, V3 rayOrigin, V3 d
     vx.incidentPdf = pdfF * fabsf(inCosAbs) / (t * t);
 }
 
-// Contributions of a new vertex at path length pathLen (pt_common.hpp:128-151):
-// Hit ID on the first segment, then emission with MIS and the guide-weighted
-// auxiliary targets.
-__device__ __forceinline__ void vertex_targets(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
-                                               float guideWeight, Acc& acc) {
-    if (pathLen == 1) {
-        acc.add(1, 3, (float)vx.inst);
-        acc.add(1, 4, (float)vx.prim);
-    }
-    if (pathLen < S.minPathLength) return;
-    V3 em;
+// Radiance a new vertex contributes (pt_common.hpp:134-143): emission (or the
+// background) with the MIS weight of hitting the emitter, times the
+// throughput; em receives the weighted emission.
+__device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
+                                                    V3& em) {
     float connectPdf = 0.0f;
     if (vx.isVoid) {  // get_emission, VOID (path_util.hpp:415-424)
         float pdfBack;
@@ -162,7 +156,21 @@ __device__ __forceinline__ void vertex_targets(const PtScene& S, const Vertex& v
         const float mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / vx.incidentPdf));
         em = em * mis;
     }
-    const V3 cc = throughput * em;
+    return throughput * em;
+}
+
+// Contributions of a new vertex at path length pathLen (pt_common.hpp:128-151):
+// Hit ID on the first segment, then emission with MIS and the guide-weighted
+// auxiliary targets.
+__device__ __forceinline__ void vertex_targets(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
+                                               float guideWeight, Acc& acc) {
+    if (pathLen == 1) {
+        acc.add(1, 3, (float)vx.inst);
+        acc.add(1, 4, (float)vx.prim);
+    }
+    if (pathLen < S.minPathLength) return;
+    V3 em;
+    const V3 cc = emission_contribution(S, vx, pathLen, throughput, em);
     acc.add(0, 0, cc.x);
     acc.add(0, 1, cc.y);
     acc.add(0, 2, cc.z);
@@ -201,6 +209,7 @@ struct NeeCandidate {
     float tmax;
     V3 radiance;
     float lightness;
+    int lightMaterial;
 };
 __device__ __forceinline__ bool nee_candidate(const PtScene& S, const Vertex& vx, V3 throughput, float guideWeight,
                                               int i, u64 neeSeed, u64 x, NeeCandidate& c) {
@@ -222,6 +231,7 @@ __device__ __forceinline__ bool nee_candidate(const PtScene& S, const Vertex& vx
     const float mis = 1.0f / ((float)S.neeCount + hitPdf / nee.creationPdf);
     c.radiance = throughput * vCos * radiance * mis;
     c.lightness = guideWeight * vCos;
+    c.lightMaterial = nee.lightMaterial;
     return true;
 }
 
@@ -234,18 +244,29 @@ __device__ __forceinline__ void add_nee(const NeeCandidate& c, Acc& acc) {
 
 // walk (random_walk.hpp:38-112) from a surface vertex up to the next ray:
 // BSDF sample, throughput, Russian roulette. Returns false when the path ends.
+// The walk also reports the throughput before the roulette (tPrime) and the
+// continuation probability, which the adjoint replay differentiates.
 __device__ __forceinline__ bool walk_surface(const Vertex& vx, u64 x0, u64 x1, float rndRoulette, V3& throughput,
-                                             V3& rayOrigin, V3& excident, float& pdfF, V3& absorption) {
+                                             V3& rayOrigin, V3& excident, float& pdfF, V3& absorption, V3& tPrime,
+                                             float& cont) {
     const float u0 = sample_uniform((u32)x0), u1 = sample_uniform((u32)(x0 >> 32));
     V3 sThroughput;
+    cont = 1.0f;
     if (!mat_sample(vx.ts, vx.mat, vx.incident(), u0, u1, x1, sThroughput, excident, pdfF)) return false;
     absorption = vx.mat.medium(dot(excident, vx.ts.geoN)).absorption;  // path_util.hpp:346-351
     throughput = throughput * sThroughput;
-    const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
+    tPrime = throughput;
+    cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
     if (rndRoulette >= cont) return false;
     throughput = throughput / cont;
     rayOrigin = add_epsilon(vx.position, excident, vx.ts.geoN);
     return true;
+}
+__device__ __forceinline__ bool walk_surface(const Vertex& vx, u64 x0, u64 x1, float rndRoulette, V3& throughput,
+                                             V3& rayOrigin, V3& excident, float& pdfF, V3& absorption) {
+    V3 tPrime;
+    float cont;
+    return walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, excident, pdfF, absorption, tPrime, cont);
 }
 
 // The camera's walk (pinholecam_sample_ray, pinhole.hpp:56-84): the first

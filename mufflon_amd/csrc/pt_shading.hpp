@@ -576,6 +576,7 @@ struct Nee {
     V3 diffIrradiance;
     V3 geoNormal;
     float distSq;
+    int lightMaterial;  // material of an area light, -1 for point lights and the background
 };
 
 __device__ __forceinline__ float4 ld4(const uint8_t* p) { return *reinterpret_cast<const float4*>(p); }
@@ -599,6 +600,7 @@ __device__ Nee connect_light(const PtScene& S, uint16_t type, u32 offset, V3 pos
         n.diffIrradiance = v3(b.x / distSq, b.y / distSq, b.z / distSq);
         n.geoNormal = v3(0, 0, 0);
         n.distSq = distSq;
+        n.lightMaterial = -1;
         return n;
     }
     const float4 q0 = ld4(light), q1 = ld4(light + 16), q2 = ld4(light + 32), q3 = ld4(light + 48);
@@ -636,6 +638,7 @@ __device__ Nee connect_light(const PtScene& S, uint16_t type, u32 offset, V3 pos
     n.diffIrradiance = v3(value.x / distSq, value.y / distSq, value.z / distSq) * transmission(m.outer, dist);
     n.geoNormal = normal;
     n.distSq = distSq;
+    n.lightMaterial = matIdx;
     return n;
 }
 
@@ -662,6 +665,7 @@ __device__ Nee connect(const PtScene& S, u64 index, u64 numIndices, u64 seed, V3
         n.diffIrradiance = radiance / pdf;
         n.geoNormal = v3(0, 0, 0);
         n.distSq = kMaxSceneSize * kMaxSceneSize;
+        n.lightMaterial = -1;
         lightProb = envProb;
     } else {
         u64 left = percentage_of(~0ull, envProb + dirProb);

@@ -79,6 +79,11 @@ _SIGS = {
     "mfh_pt_get_target": [vp, i32, i32, f32p],
     "mfh_pt_enable_target": [vp, i32, i32, i32],
     "mfh_pt_set_mode": [vp, i32, i32, i32],
+    "mfh_pt_adjoint": [vp, i32, i32, f32p],
+    "mfh_pt_get_texture_gradient": [vp, i32, i32, f32p],
+    "mfh_pt_get_emission_gradient": [vp, i32, i32, f32p],
+    "mfh_pt_set_material": [vp, i32, i32, vp],
+    "mfh_texture_update": [vp, i32, f32p],
     "mfh_pt_set_tuning": [vp, i32, vp],
     "mfh_pt_get_tuning": [vp, i32, vp],
     "mfh_pt_autotune": [vp, i32, i32, vp, f32p],
@@ -414,6 +419,28 @@ class HipContext:
 
     def pt_iterate(self, pt, iterations=1):
         self._call("mfh_pt_iterate", self.handle, pt, int(iterations))
+
+    def texture_update(self, tex, texels):
+        t = np.ascontiguousarray(texels, np.float32)
+        self._call("mfh_texture_update", self.handle, tex, _ptr(t, f32p))
+
+    def pt_set_material(self, pt, index, material):
+        m = np.ascontiguousarray(material, MATERIAL_DTYPE).reshape(1)
+        self._call("mfh_pt_set_material", self.handle, pt, index, m.ctypes.data)
+
+    def pt_adjoint(self, pt, iterations, radiance_adjoint):
+        d = np.ascontiguousarray(radiance_adjoint, np.float32)
+        self._call("mfh_pt_adjoint", self.handle, pt, iterations, _ptr(d, f32p))
+
+    def pt_texture_gradient(self, pt, tex, shape):
+        out = np.zeros(int(np.prod(shape)), np.float32)
+        self._call("mfh_pt_get_texture_gradient", self.handle, pt, tex, _ptr(out, f32p))
+        return out.reshape(shape)
+
+    def pt_emission_gradient(self, pt, material_index):
+        out = np.zeros(3, np.float32)
+        self._call("mfh_pt_get_emission_gradient", self.handle, pt, material_index, _ptr(out, f32p))
+        return out
 
     def pt_set_mode(self, pt, mode, use_graph=False):
         self._call("mfh_pt_set_mode", self.handle, pt, mode, 1 if use_graph else 0)

@@ -501,3 +501,27 @@ def _rot(a, tilt):
     Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
     Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
     return Rz @ Rx @ Ry
+
+
+def cornell_textured(sphere_rings=128, sphere_segments=256, size=256, wall_res=512, sphere_res=256, seed=3,
+                     wall_sampling=None, sphere_sampling=None):
+    """The Cornell box with a procedural RGB albedo texture on the back wall
+    (texture 0, LAMBERT) and on the sphere (texture 1, a GGX TORRANCE sphere
+    of roughness 0.3): the parameters of the adjoint checks."""
+    wall_sampling = SAMPLING_LINEAR if wall_sampling is None else wall_sampling
+    sphere_sampling = SAMPLING_LINEAR if sphere_sampling is None else sphere_sampling
+    c = cornell_box(sphere_rings, sphere_segments, size, size)
+    mats = np.concatenate([c["materials"], material(MAT_LAMBERT, albedo_texture=0),
+                           material(MAT_TORRANCE, albedo_texture=1, roughness=0.3)])
+    BACK, SPHERE = 4, 5
+    mi = c["mat_idx"].copy()
+    # back wall: the third quad (triangles 4, 5); sphere: the triangles after the two boxes
+    mi[4:6] = BACK
+    mi[10 + 24:10 + 24 + 2 * sphere_segments * (sphere_rings - 1)] = SPHERE
+    c["mat_idx"] = mi
+    c["materials"] = mats
+    c["textures"] = [(procedural_texture(wall_res, wall_res, 3, 0.15, 0.85, seed=seed), wall_sampling),
+                     (procedural_texture(sphere_res, sphere_res, 3, 0.3, 0.9, seed=seed + 1), sphere_sampling)]
+    c["envmap"] = None
+    c["point_lights"] = None
+    return c

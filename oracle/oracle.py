@@ -459,3 +459,29 @@ def pt_render_targets(pt_scene, seed, min_path_length, max_path_length, nee_coun
     lib.orc_pt_render_targets(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
                               mask, ptrs, C.byref(nans))
     return {t: b.reshape(h, w, TARGET_CHANNELS[t]) for t, b in bufs.items()}, nans.value
+
+
+def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, delta, tex_shapes):
+    """FP64 adjoint of the oracle: delta (h, w, 3) = dLoss/dRadiance of the
+    image of `iterations` iterations from a reset. tex_shapes: {texture id:
+    (h, w)} of the albedo textures to differentiate. Returns ({id: (h, w, 4)
+    float64}, emission gradient (materials, 3) float64)."""
+    lib = load()
+    if not hasattr(lib, "_adj_ready"):
+        dp = C.POINTER(C.c_double)
+        lib.orc_pt_adjoint.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       f32p, C.POINTER(dp), dp]
+        lib.orc_pt_adjoint.restype = C.c_int
+        lib._adj_ready = True
+    dp = C.POINTER(C.c_double)
+    n_tex = max(pt_scene.textures.count, 1)
+    grads = {}
+    ptrs = (dp * n_tex)()
+    for t, (h, w) in tex_shapes.items():
+        grads[t] = np.zeros((h, w, 4), np.float64)
+        ptrs[t] = grads[t].ctypes.data_as(dp)
+    em = np.zeros((len(pt_scene.materials), 3), np.float64)
+    d = np.ascontiguousarray(delta, np.float32)
+    lib.orc_pt_adjoint(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
+                       _p(d, f32p), ptrs, em.ctypes.data_as(dp))
+    return grads, em

@@ -951,6 +951,7 @@ struct Nee {
     float dist;
     V3 geoNormal;
     float distSq;
+    int lightMaterial = -1;  // material of an area light (for the emission gradient)
 };
 
 struct TriLight {  // AreaLightTriangle record
@@ -1034,6 +1035,7 @@ Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 p
     n.dist = dist;
     n.geoNormal = normal;
     n.distSq = distSq;
+    n.lightMaterial = L.material;
     return n;
 }
 
@@ -1323,9 +1325,242 @@ MatParams params_of(const OrcMaterial& m, const OrcTexture* textures, float u, f
     return fetch(S, m, u, v);
 }
 
+// ---------------------------------------------------------------- adjoint
+// Path replay of pt_sample recording what the derivative needs; the float
+// path is the primal's, the gradient arithmetic is in double.
+struct AdjRec {
+    int tex = -1;
+    float hu = 0, hv = 0;
+    V3 rho{0, 0, 0};
+    double neeW[3] = {0, 0, 0};
+    double emW[3] = {0, 0, 0};
+    double G[3] = {0, 0, 0};
+    float cont = 1.0f;
+    float tp = 0.0f;
+    int astar = 0;
+};
+
+struct AdjOut {
+    double* const* texGrad;  // per texture id: 4 doubles per texel or null
+    double* emGrad;          // 3 per material
+};
+
+void texel_weights(const OrcTexture& t, float u, float v, int* idx, double* w, int& n) {
+    if (t.width == 1 && t.height == 1) {
+        idx[0] = 0;
+        w[0] = 1.0;
+        n = 1;
+        return;
+    }
+    if (t.sampling == 0) {
+        idx[0] = imod(ifloor(v * (float)t.height), t.height) * t.width + imod(ifloor(u * (float)t.width), t.width);
+        w[0] = 1.0;
+        n = 1;
+        return;
+    }
+    float fx = u * (float)t.width, fy = v * (float)t.height;
+    const int ix = ifloor(fx), iy = ifloor(fy);
+    fx -= (float)ix;
+    fy -= (float)iy;
+    const int x0 = imod(ix, t.width), x1 = imod(ix + 1, t.width), y0 = imod(iy, t.height), y1 = imod(iy + 1, t.height);
+    idx[0] = y0 * t.width + x0;
+    idx[1] = y0 * t.width + x1;
+    idx[2] = y1 * t.width + x0;
+    idx[3] = y1 * t.width + x1;
+    w[0] = (1.0 - fx) * (1.0 - fy);
+    w[1] = fx * (1.0 - fy);
+    w[2] = (1.0 - fx) * fy;
+    w[3] = (double)fx * fy;
+    n = 4;
+}
+
+void add_em_grad(const OrcPtScene& S, const AdjOut& out, int mat, const double* w) {
+    const OrcMaterial& m = S.materials[mat];
+    for (int c = 0; c < 3; ++c)
+        if (m.scale[c] != 0.0f) out.emGrad[3 * mat + c] += w[c] / m.scale[c];
+}
+
+void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& rng, const double delta[3],
+                     const AdjOut& out) {
+    const int minPL = params[1], maxPL = params[2], neeCount = params[3];
+    std::vector<AdjRec> rec(maxPL + 2);
+    V3 throughput{1.0f, 1.0f, 1.0f};
+    float footA = 0.0f;  // footprint (feeds only the auxiliary targets)
+    Vertex vtx;
+    vtx.position = {S.camPos[0], S.camPos[1], S.camPos[2]};
+    rng.next();
+    const V3 viewDir{S.camDir[0], S.camDir[1], S.camDir[2]};
+    const V3 up{S.camUp[0], S.camUp[1], S.camUp[2]};
+    int pathLen = 0;
+    do {
+        if (pathLen > 0 && pathLen + 1 >= minPL && pathLen + 1 <= maxPL) {
+            const u64 neeSeed = rng.next();
+            for (int i = 0; i < neeCount; ++i) {
+                const u64 x = rng.next();
+                const float nu0 = sample_uniform((u32)x), nu1 = sample_uniform((u32)(x >> 32));
+                const Nee nee = connect(S, (u64)i, (u64)neeCount, neeSeed, vtx.position, nu0, nu1);
+                EvalValue value = mat_evaluate(vtx.ts, vtx.mat, vtx.incident_dir(), nee.dir);
+                if (nee.cosOut != 0) value.cosOut *= nee.cosOut;
+                const V3 radiance = value.value * nee.diffIrradiance;
+                if ((radiance.x > 0.0f || radiance.y > 0.0f || radiance.z > 0.0f) && value.cosOut > 0.0f) {
+                    if (!any_intersection(S, vtx.position, nee.position, vtx.ts.geoN, nee.geoNormal, nee.dir)) {
+                        const float hitPdf = value.pdfF * std::fabs(nee.cosOut) / nee.distSq;
+                        const float mis = 1.0f / ((float)neeCount + hitPdf / nee.creationPdf);
+                        const V3 c = throughput * value.cosOut * radiance * mis;
+                        const double wgt[3] = {delta[0] * c.x, delta[1] * c.y, delta[2] * c.z};
+                        for (int k = 0; k < 3; ++k) rec[pathLen].neeW[k] += wgt[k];
+                        if (nee.lightMaterial >= 0) add_em_grad(S, out, nee.lightMaterial, wgt);
+                    }
+                }
+            }
+        }
+        const u64 x0 = rng.next();
+        const u64 x1 = rng.next();
+        const float u0 = sample_uniform((u32)x0), u1 = sample_uniform((u32)(x0 >> 32));
+        const float rndRoulette = sample_uniform((u32)rng.next());
+        V3 excident, origin = vtx.position;
+        float pdfF;
+        V3 sThroughput;
+        Medium medium = kVacuum;
+        if (vtx.isCamera) {
+            const float w = (float)S.width, h = (float)S.height;
+            float cx = ((float)px + u0) / w * 2.0f - 1.0f;
+            float cy = ((float)py + u1) / h * 2.0f - 1.0f;
+            cx *= S.tanVFov;
+            cy *= S.tanVFov;
+            const float aspect = w / h;
+            const V3 xAxis = cross(viewDir, up);
+            excident = normalize(xAxis * (cx * aspect) + up * cy + viewDir * 1.0f);
+            const float pixelArea = (2 * S.tanVFov) * (2 * S.tanVFov) * aspect;
+            const float cosOut = dot(viewDir, excident);
+            pdfF = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
+            sThroughput = {1.0f, 1.0f, 1.0f};
+        } else {
+            const PathSample ps = mat_sample(vtx.ts, vtx.mat, vtx.incident_dir(), u0, u1, x1);
+            if (!ps.valid) break;
+            excident = ps.excident;
+            pdfF = ps.pdfF;
+            sThroughput = ps.throughput;
+            medium = vtx.mat.medium(dot(excident, vtx.ts.geoN));
+        }
+        throughput = throughput * sThroughput;
+        const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
+        if (pathLen > 0) {
+            AdjRec& r = rec[pathLen];
+            r.cont = cont;
+            const V3 t = throughput;
+            r.astar = (t.x < t.y) ? ((t.y < t.z) ? 2 : 1) : ((t.x < t.z) ? 2 : 0);
+            r.tp = r.astar == 0 ? t.x : (r.astar == 1 ? t.y : t.z);
+        }
+        if (rndRoulette >= cont) break;
+        throughput = throughput / cont;
+        V3 rayOrigin = origin;
+        const HitInfo hit = first_intersection(S, rayOrigin, excident, vtx.geo_normal(), kMaxSceneSize);
+        const float hitDist = hit.hit ? hit.t : kMaxSceneSize;
+        throughput = throughput * transmission(medium, hitDist);
+        footA += 1.0f / std::sqrt(pdfF);
+        Vertex next;
+        next.isCamera = false;
+        int matIdx = -1;
+        if (!hit.hit) {
+            next.isVoid = true;
+            next.incEnc = excident;
+            next.incidentPdf = pdfF * 1.0f;
+        } else {
+            next.position = rayOrigin + excident * hit.t;
+            next.ts = tangent_space(S, hit);
+            next.incEnc = {excident.x, excident.y, hit.t * sgnf(excident.z)};
+            next.inst = hit.inst;
+            next.prim = hit.prim;
+            const i32 lod = S.geo.instLod[hit.inst];
+            matIdx = S.matIdx[(size_t)S.geo.lodTriOffset[lod] + hit.prim];
+            next.mat = fetch(S, S.materials[matIdx], hit.u, hit.v);
+            const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
+            next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
+        }
+        vtx = next;
+        ++pathLen;
+        AdjRec& r = rec[pathLen];
+        if (!vtx.isVoid) {
+            r.hu = hit.u;
+            r.hv = hit.v;
+            r.rho = vtx.mat.albedo;
+            if (vtx.mat.type == kLambert || vtx.mat.type == kTorrance) r.tex = S.materials[matIdx].albedoTexture;
+        }
+        if (pathLen >= minPL) {
+            V3 emission{0, 0, 0};
+            float connectPdf = 0.0f;
+            if (vtx.isVoid) {
+                const BgEval b = evaluate_background(S, vtx.incEnc);
+                emission = b.value;
+                if (nonzero(emission)) connectPdf = background_pdf(S, b.pdfBack);
+            } else {
+                emission = mat_emission(vtx.mat, vtx.ts.geoN, -vtx.incident_dir());
+                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim);
+            }
+            if (nonzero(emission) && pathLen > 1) {
+                const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
+                emission = emission * mis;
+            }
+            const V3 c = throughput * emission;
+            r.emW[0] = delta[0] * c.x;
+            r.emW[1] = delta[1] * c.y;
+            r.emW[2] = delta[2] * c.z;
+            if (!vtx.isVoid && vtx.mat.type == kEmissive) add_em_grad(S, out, matIdx, r.emW);
+        }
+        if (vtx.isVoid) break;
+    } while (pathLen < maxPL);
+    const int last = pathLen;
+    for (int v = last - 1; v >= 0; --v)
+        for (int k = 0; k < 3; ++k) rec[v].G[k] = rec[v + 1].G[k] + rec[v + 1].emW[k] + rec[v + 1].neeW[k];
+    for (int j = 1; j <= last; ++j) {
+        const AdjRec& r = rec[j];
+        if (r.tex < 0 || !out.texGrad[r.tex]) continue;
+        int idx[4];
+        double w[4];
+        int nt = 0;
+        texel_weights(S.textures[r.tex], r.hu, r.hv, idx, w, nt);
+        const double rho[3] = {r.rho.x, r.rho.y, r.rho.z};
+        for (int c = 0; c < 3; ++c) {
+            if (rho[c] == 0.0) continue;
+            double g = (r.G[c] + r.neeW[c]) / rho[c];
+            double sumD = 0.0;
+            for (int m = j; m < last; ++m) {
+                if (!(rec[m].cont < 1.0f)) continue;
+                const double D = (double)rec[m].tp / rec[m].cont * ((rec[m].astar == c ? 1.0 / rho[c] : 0.0) - sumD);
+                g -= D * (rec[m].G[0] + rec[m].G[1] + rec[m].G[2]);
+                sumD += D;
+            }
+            for (int q = 0; q < nt; ++q) out.texGrad[r.tex][4 * (size_t)idx[q] + c] += w[q] * g;
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// Adjoint of the Radiance image of `iterations` iterations from a reset:
+// delta = dLoss/dRadiance (3 floats per pixel). texGrad[t] (4 doubles per
+// texel, or null) and emGrad (3 per material) receive the gradients; the sum
+// runs serially in pixel order.
+int orc_pt_adjoint(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength, int32_t neeCount,
+                   int32_t iterations, const float* delta, double* const* texGrad, double* emGrad) {
+    const int W = S->width, H = S->height;
+    const int num = W * H;
+    const int s = seed * (num + 1);
+    const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
+    const AdjOut out{texGrad, emGrad};
+    for (int pixel = 0; pixel < num; ++pixel) {
+        const double d[3] = {(double)(delta[3 * (size_t)pixel] * (1.0f / (float)iterations)),
+                             (double)(delta[3 * (size_t)pixel + 1] * (1.0f / (float)iterations)),
+                             (double)(delta[3 * (size_t)pixel + 2] * (1.0f / (float)iterations))};
+        if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+        Pcg64 rng((u32)(pixel + s));
+        for (int it = 0; it < iterations; ++it) pt_adjoint_path(*S, params, pixel % W, pixel / W, rng, d, out);
+    }
+    return 0;
+}
 
 // Runs `iterations` iterations from a reset (fresh per-pixel RNGs) and returns
 // the running means of the enabled targets (targetMask bit t -> outTargets[t],

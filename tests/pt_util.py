@@ -158,3 +158,31 @@ def oracle_pt_scene_materials(O, c, background, light_tree=None, box=None):
     tx = O.Textures(c["textures"])
     return O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], lt, box, c["camera"],
                      textures=tx, background=bg[0], background_sat=sat, background_flux=flux)
+
+
+def gpu_pt_textured(ctx, c, seed=1, max_path_length=8, nee_count=1):
+    """Path tracer of a scene with albedo textures (no background, no point
+    lights); returns (pt, scene id, device texture ids)."""
+    ids = np.array([ctx.texture_create(t, s) for t, s in c["textures"]], np.int32)
+    mats = c["materials"].copy()
+    for f in ("albedoTexture", "roughnessTexture"):
+        used = mats[f] >= 0
+        mats[f][used] = ids[mats[f][used]]
+    lod = ctx.lod_create(c["verts"], c["tris"])
+    ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
+    ctx.lbvh_build(lod, scene_box(c))
+    sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+    pt = ctx.pt_create(sc, mats, c["camera"], dict(seed=seed, max_path_length=max_path_length, nee_count=nee_count))
+    return pt, sc, ids
+
+
+def oracle_pt_scene_textured(O, c, textures=None, light_tree=None):
+    v, t = c["verts"], c["tris"]
+    scene, _ = U.oracle_scene(O, [(v, t)], np.array([0], np.int32), S.IDENTITY34.reshape(1, 12))
+    box = scene_box(c)
+    if light_tree is None:
+        al = S.emissive_area_lights(v, c["uvs"], t, c["mat_idx"], c["materials"])
+        light_tree = hip.light_tree_build(al, None, box, c["materials"])
+    tx = O.Textures(textures if textures is not None else c["textures"])
+    return O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], light_tree, box, c["camera"],
+                     textures=tx)

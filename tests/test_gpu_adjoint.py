@@ -1,0 +1,116 @@
+"""Adjoint (path replay) on the GPU: against the oracle's FP64 adjoint on the
+same paths, against central finite differences of GPU renders with
+identical seeds (the configuration of the differentiable benchmark: the
+Cornell box with a 512x512 albedo texture on the back wall and a 256x256 one
+on the sphere, 256x256 pixels, 4 paths per pixel), and through the
+torch.autograd binding."""
+import numpy as np
+import pytest
+
+import pt_util as P
+from mufflon_amd import hip
+from mufflon_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(hip_ctx, c, depth):
+    pt, sc, ids = P.gpu_pt_textured(hip_ctx, c, max_path_length=depth)
+    return pt, ids
+
+
+def test_adjoint_matches_oracle_fp64(hip_ctx, oracle):
+    c = S.cornell_textured(12, 24, 40, 16, 16)
+    print("adjoint vs oracle fp64, 40x40", flush=True)
+    depth, its = 8, 3
+    pt, ids = _setup(hip_ctx, c, depth)
+    hip_ctx.pt_iterate(pt, its)
+    img = hip_ctx.pt_target(pt)
+    rng = np.random.default_rng(5)
+    delta = rng.normal(size=img.shape).astype(np.float32)
+    hip_ctx.pt_adjoint(pt, its, delta)
+    g_gpu = [hip_ctx.pt_texture_gradient(pt, int(i), (16, 16, 3)) for i in ids]
+    e_gpu = hip_ctx.pt_emission_gradient(pt, 0)
+    ps = P.oracle_pt_scene_textured(oracle, c, light_tree=hip_ctx.pt_light_tree(pt))
+    o_img, _, _ = oracle.pt_render(ps, 1, 0, depth, 1, its)
+    assert np.array_equal(img.view(np.uint32), o_img.view(np.uint32))
+    g_orc, e_orc = oracle.pt_adjoint(ps, 1, 0, depth, 1, its, delta, {0: (16, 16), 1: (16, 16)})
+    for t in (0, 1):
+        ref = g_orc[t][..., :3]
+        scale = np.abs(ref).max()
+        err = np.abs(g_gpu[t] - ref)
+        print("texture %d: max |grad| %.4g, max abs err %.3g" % (t, scale, float(err.max())), flush=True)
+        assert (err <= 1e-3 * np.abs(ref) + 1e-5 * scale).all()
+    np.testing.assert_allclose(e_gpu, e_orc[0], rtol=1e-3)
+
+
+def test_adjoint_finite_differences_config3(hip_ctx):
+    """256x256, 4 spp, L2 loss to a target rendered from a texture of seed 11;
+    64 random texels with a non-negligible gradient."""
+    c = S.cornell_textured(128, 256, 256, 512, 256)
+    print("adjoint FD check, 256x256, 4 spp", flush=True)
+    its, depth = 4, 8
+    pt, ids = _setup(hip_ctx, c, depth)
+    textures = [t.copy() for t, _ in c["textures"]]
+    target_tex = [S.procedural_texture(512, 512, 3, 0.15, 0.85, seed=11),
+                  S.procedural_texture(256, 256, 3, 0.3, 0.9, seed=12)]
+
+    def render(texs):
+        for i, t in zip(ids, texs):
+            hip_ctx.texture_update(int(i), t)
+        hip_ctx.pt_reset(pt)
+        hip_ctx.pt_iterate(pt, its)
+        return hip_ctx.pt_target(pt).astype(np.float64)
+
+    target = render(target_tex)
+    img = render(textures)
+    delta = (2.0 * (img - target)).astype(np.float32)
+    hip_ctx.timer_begin()
+    hip_ctx.pt_adjoint(pt, its, delta)
+    ms = hip_ctx.timer_end()
+    print("adjoint %d spp: %.2f ms" % (its, ms), flush=True)
+    grads = [hip_ctx.pt_texture_gradient(pt, int(i), t.shape) for i, t in zip(ids, textures)]
+    rng = np.random.default_rng(7)
+    checked = agreed = flips = 0
+    rels = []
+    for t in (0, 1):
+        g = grads[t]
+        cand = np.argwhere(np.abs(g) > 1e-2 * np.abs(g).max())
+        for k in rng.choice(len(cand), size=min(32, len(cand)), replace=False):
+            y, x, ch = cand[k]
+            fds = []
+            for eps in (1e-3, 5e-4):
+                tp = [a.copy() for a in textures]
+                tm = [a.copy() for a in textures]
+                tp[t][y, x, ch] += eps
+                tm[t][y, x, ch] -= eps
+                fds.append((np.sum((render(tp) - target) ** 2) - np.sum((render(tm) - target) ** 2)) / (2 * eps))
+            if abs(fds[0] - fds[1]) > 2e-3 * abs(fds[0]):
+                flips += 1  # a roulette / light choice flipped inside the step
+                continue
+            checked += 1
+            rel = abs(fds[0] - g[y, x, ch]) / abs(fds[0])
+            rels.append(rel)
+            agreed += rel <= 1e-3
+    print("FD check: %d texels compared, %d within 1e-3 relative, %d skipped (discontinuity), median rel %.2e" %
+          (checked, agreed, flips, float(np.median(rels))), flush=True)
+    assert checked >= 48 and agreed >= 0.9 * checked
+
+
+def test_torch_autograd_function(hip_ctx):
+    torch = pytest.importorskip("torch")
+    from mufflon_amd import torch_pt
+    c = S.cornell_textured(8, 16, 32, 16, 16)
+    pt, ids = _setup(hip_ctx, c, 6)
+    tracer = torch_pt.Tracer(hip_ctx, pt, [int(i) for i in ids], [t.shape for t, _ in c["textures"]], 32, 32)
+    texs = [torch.tensor(t, requires_grad=True) for t, _ in c["textures"]]
+    target = torch.zeros(32, 32, 3)
+    img = torch_pt.render(tracer, texs, 2)
+    loss = ((img - target) ** 2).sum()
+    loss.backward()
+    # the same through the C-ABI directly (float atomics: equal up to summation order)
+    hip_ctx.pt_adjoint(pt, 2, (2.0 * (img.detach().numpy() - 0.0)).astype(np.float32))
+    for i, t in zip(ids, texs):
+        direct = hip_ctx.pt_texture_gradient(pt, int(i), tuple(t.shape))
+        np.testing.assert_allclose(t.grad.numpy(), direct, rtol=1e-4, atol=1e-6 * np.abs(direct).max())
+    assert float(texs[0].grad.abs().sum()) > 0
