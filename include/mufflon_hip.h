@@ -285,6 +285,17 @@ int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* outT
  * resets the path tracer. */
 int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wavefront_tuning* outBest,
                     float* outMsPerIteration);
+/* Batches: several path tracers (scenes of any size, resolution, path
+ * length) iterated together by the wavefront stages, one launch per stage
+ * over the paths of all scenes (optionally replayed from a HIP graph). The
+ * batch resets its path tracers on creation and on mfh_batch_reset; their
+ * targets are read with mfh_pt_get_target. No counterpart in the reference. */
+int mfh_batch_create(mfh_context* ctx, const int32_t* ptIds, int32_t count, int32_t useGraph, int32_t* outBatchId);
+int mfh_batch_reset(mfh_context* ctx, int32_t batchId);
+int mfh_batch_iterate(mfh_context* ctx, int32_t batchId, int32_t iterations);
+int mfh_batch_set_tuning(mfh_context* ctx, int32_t batchId, const mfh_wavefront_tuning* tuning);
+int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mfh_wavefront_tuning* outBest,
+                       float* outMsPerIteration);
 /* Running mean of an enabled target over the completed iterations (mufflon_get_target_image). */
 int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out);
 /* Index of the last completed iteration, -1 after a reset (render_get_current_iteration). */

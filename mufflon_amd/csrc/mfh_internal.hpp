@@ -118,6 +118,7 @@ struct Scene {
 };
 
 struct PathTracer;
+struct PtBatch;
 
 // A texture of the context: RGBA32F texels (missing channels are 0), row-major,
 // kept on the host too (background flux and summed-area table are host-built).
@@ -134,6 +135,7 @@ struct Context {
     std::vector<Lod*> lods;
     std::vector<Scene*> scenes;
     std::vector<PathTracer*> tracers;
+    std::vector<PtBatch*> batches;
     std::vector<Texture*> textures;
     BuildScratch scratch;
     int lastBuilt = -1;

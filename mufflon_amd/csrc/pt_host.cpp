@@ -60,7 +60,43 @@ void release_wavefront(PathTracer& t) {
     t.wfMaxLen = t.wfNee = -1;
 }
 
+// A batch of path tracers iterated together by the wavefront stages.
+struct PtBatch {
+    std::vector<int32_t> tracers;
+    std::vector<int32_t> offsets;  // first global path of every tracer, then the total
+    bool useGraph = false;
+    WfTuning tune;
+    int32_t iteration = -1;
+    DevBuf dScenes, dOffsets;
+    DevBuf counters, tp, acc, ray[2][4], sh[3], shPath, sortKeys[2], sortIds[2];
+    int paths = 0, maxLen = 0, nee = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<uint8_t> graphKey;
+};
+
+void drop_batch_graph(PtBatch& b) {
+    if (b.graphExec) (void)hipGraphExecDestroy(b.graphExec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.graphExec = nullptr;
+    b.graph = nullptr;
+    b.graphKey.clear();
+}
+
+void destroy_batch(PtBatch* b) {
+    drop_batch_graph(*b);
+    for (DevBuf* d : {&b->dScenes, &b->dOffsets, &b->counters, &b->tp, &b->acc, &b->sh[0], &b->sh[1], &b->sh[2],
+                      &b->shPath, &b->sortKeys[0], &b->sortKeys[1], &b->sortIds[0], &b->sortIds[1]})
+        d->release();
+    for (auto& r : b->ray)
+        for (DevBuf& d : r) d.release();
+    delete b;
+}
+
 void destroy_tracers(Context& c) {
+    for (PtBatch* b : c.batches)
+        if (b) destroy_batch(b);
+    c.batches.clear();
     for (PathTracer* t : c.tracers) {
         if (!t) continue;
         for (DevBuf* b : {&t->dMaterials, &t->dPtLods, &t->dLightMem, &t->dMapKeys, &t->dMapCodes, &t->dTextures,
@@ -317,6 +353,10 @@ WfState wavefront_state(Context& c, PathTracer& t) {
     W.shD = t.wfSh[1].as<float4>();
     W.shC = t.wfSh[2].as<float4>();
     W.shPath = t.wfShPath.as<int2>();
+    W.neeStride = nee;
+    W.scenes = nullptr;
+    W.sceneOffset = nullptr;
+    W.numScenes = 1;
     for (int b = 0; b < 2; ++b) {
         W.sortKeys[b] = t.wfSortKeys[b].as<uint64_t>();
         W.sortIds[b] = t.wfSortIds[b].as<int32_t>();
@@ -584,6 +624,230 @@ int mfh_pt_set_material(mfh_context* ctx, int32_t ptId, int32_t materialIndex, c
         }
         drop_graph(t);
         reset(c, t);
+    });
+}
+
+namespace mfh {
+namespace {
+
+PtBatch& batch_of(Context& c, int32_t id) {
+    MFH_REQUIRE(id >= 0 && id < (int32_t)c.batches.size() && c.batches[id], "unknown batch id");
+    return *c.batches[id];
+}
+
+// Device state of a batch: per-scene descriptors, path offsets, and the
+// wavefront buffers over the paths of all scenes.
+WfState batch_state(Context& c, PtBatch& b, std::vector<PtScene>& scenes) {
+    scenes.clear();
+    int32_t maxLen = 1, nee = 1;
+    uint32_t slotMask = 0;
+    for (int32_t id : b.tracers) {
+        PathTracer& t = tracer_of(c, id);
+        MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "a scene of the batch has no lights");
+        scenes.push_back(device_scene(c, t));
+        maxLen = std::max(maxLen, t.params.maxPathLength);
+        nee = std::max(nee, t.params.neeCount);
+        int slot = 0;
+        for (int k = 0; k < kNumTargets; ++k) {
+            if ((t.targetMask >> k) & 1u)
+                for (int j = 0; j < kTargetChannels[k]; ++j) slotMask |= 1u << (slot + j);
+            slot += kTargetChannels[k];
+        }
+    }
+    upload(c, b.dScenes, scenes.data(), sizeof(PtScene) * scenes.size());
+    const int N = b.offsets.back();
+    const int numCounters = (maxLen + 1) + maxLen + 3 * maxLen;
+    if (N != b.paths || maxLen != b.maxLen || nee != b.nee) {
+        drop_batch_graph(b);
+        b.counters.reserve(4 * (size_t)(numCounters + 1));
+        b.tp.reserve(16 * (size_t)N);
+        b.acc.reserve(4 * 18 * (size_t)N);
+        for (auto& r : b.ray)
+            for (DevBuf& d : r) d.reserve(16 * (size_t)N);
+        for (DevBuf& d : b.sh) d.reserve(16 * (size_t)N * nee);
+        b.shPath.reserve(8 * (size_t)N);
+        b.paths = N;
+        b.maxLen = maxLen;
+        b.nee = nee;
+    }
+    WfState W;
+    std::memset(&W, 0, sizeof(W));
+    W.numPaths = N;
+    W.slotMask = slotMask;
+    W.chunk = b.tune.chunk;
+    W.numCounters = numCounters;
+    W.qBase = 0;
+    W.shBase = maxLen + 1;
+    W.workBase = 2 * maxLen + 1;
+    W.counters = b.counters.as<int>();
+    W.iteration = b.counters.as<int>() + numCounters;
+    W.tp = b.tp.as<float4>();
+    W.acc = b.acc.as<float>();
+    for (int k = 0; k < 2; ++k) {
+        W.rayO[k] = b.ray[k][0].as<float4>();
+        W.rayD[k] = b.ray[k][1].as<float4>();
+        W.rayM[k] = b.ray[k][2].as<float4>();
+        W.hit[k] = b.ray[k][3].as<float4>();
+    }
+    W.shO = b.sh[0].as<float4>();
+    W.shD = b.sh[1].as<float4>();
+    W.shC = b.sh[2].as<float4>();
+    W.shPath = b.shPath.as<int2>();
+    W.neeStride = nee;
+    W.scenes = b.dScenes.as<PtScene>();
+    W.sceneOffset = b.dOffsets.as<int>();
+    W.numScenes = (int)b.tracers.size();
+    if (b.tune.traceGrid <= 0) b.tune.traceGrid = wf_grid_batch(b.tune, num_cus(c));
+    if (b.tune.shadeGrid <= 0) b.tune.shadeGrid = wf_shade_grid(num_cus(c));
+    return W;
+}
+
+void run_batch(Context& c, PtBatch& b, int32_t iterations) {
+    std::vector<PtScene> scenes;
+    WfState W = batch_state(c, b, scenes);
+    const int32_t it = b.iteration;
+    MFH_HIP(hipMemcpyAsync(W.iteration, &it, 4, hipMemcpyHostToDevice, c.stream));
+    if (!b.useGraph) {
+        for (int32_t k = 0; k < iterations; ++k) launch_pt_wavefront_batch(W, b.tune, b.maxLen, c.stream);
+    } else {
+        std::vector<uint8_t> key(sizeof(W) + sizeof(WfTuning) + sizeof(PtScene) * scenes.size());
+        std::memcpy(key.data(), &W, sizeof(W));
+        std::memcpy(key.data() + sizeof(W), &b.tune, sizeof(WfTuning));
+        std::memcpy(key.data() + sizeof(W) + sizeof(WfTuning), scenes.data(), sizeof(PtScene) * scenes.size());
+        if (!b.graphExec || key != b.graphKey) {
+            drop_batch_graph(b);
+            MFH_HIP(hipStreamSynchronize(c.stream));
+            MFH_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                launch_pt_wavefront_batch(W, b.tune, b.maxLen, c.stream);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            MFH_HIP(hipStreamEndCapture(c.stream, &b.graph));
+            MFH_HIP(hipGraphInstantiate(&b.graphExec, b.graph, nullptr, nullptr, 0));
+            b.graphKey = key;
+        }
+        for (int32_t k = 0; k < iterations; ++k) MFH_HIP(hipGraphLaunch(b.graphExec, c.stream));
+    }
+    b.iteration += iterations;
+    for (int32_t id : b.tracers) tracer_of(c, id).iteration = b.iteration;
+    MFH_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+}  // namespace mfh
+
+int mfh_batch_create(mfh_context* ctx, const int32_t* ptIds, int32_t count, int32_t useGraph, int32_t* outBatchId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(ptIds && outBatchId && count > 0, "a batch needs at least one path tracer");
+        auto* b = new PtBatch();
+        try {
+            int64_t total = 0;
+            for (int32_t i = 0; i < count; ++i) {
+                PathTracer& t = tracer_of(c, ptIds[i]);
+                MFH_REQUIRE(t.params.neeCount <= 32, "the wavefront form supports at most 32 NEE samples per vertex");
+                b->tracers.push_back(ptIds[i]);
+                b->offsets.push_back((int32_t)total);
+                total += (int64_t)t.camera.width * t.camera.height;
+            }
+            MFH_REQUIRE(total < (int64_t)1 << 30, "too many paths in one batch");
+            b->offsets.push_back((int32_t)total);
+            b->useGraph = useGraph != 0;
+            upload(c, b->dOffsets, b->offsets.data(), 4 * b->offsets.size());
+            for (int32_t id : b->tracers) reset(c, tracer_of(c, id));
+            MFH_HIP(hipStreamSynchronize(c.stream));
+        } catch (...) {
+            destroy_batch(b);
+            throw;
+        }
+        c.batches.push_back(b);
+        *outBatchId = (int32_t)c.batches.size() - 1;
+    });
+}
+
+int mfh_batch_reset(mfh_context* ctx, int32_t batchId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PtBatch& b = batch_of(c, batchId);
+        for (int32_t id : b.tracers) reset(c, tracer_of(c, id));
+        b.iteration = -1;
+    });
+}
+
+int mfh_batch_iterate(mfh_context* ctx, int32_t batchId, int32_t iterations) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PtBatch& b = batch_of(c, batchId);
+        MFH_REQUIRE(iterations >= 0, "negative iteration count");
+        for (int32_t id : b.tracers)
+            MFH_REQUIRE(tracer_of(c, id).iteration == b.iteration,
+                        "a path tracer of the batch was iterated or reset on its own; reset the batch");
+        run_batch(c, b, iterations);
+    });
+}
+
+int mfh_batch_set_tuning(mfh_context* ctx, int32_t batchId, const mfh_wavefront_tuning* tuning) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PtBatch& b = batch_of(c, batchId);
+        MFH_REQUIRE(tuning != nullptr, "null tuning");
+        int v = -1;
+        for (int k = 0; k < kWfNumTraceVariants; ++k)
+            if (kWfTraceShapes[k].block == tuning->traceBlock && kWfTraceShapes[k].lds == tuning->stackDepth) v = k;
+        MFH_REQUIRE(v >= 0, "unsupported trace block size / LDS stack depth combination");
+        MFH_REQUIRE(tuning->queueChunk >= 64 && tuning->queueChunk % 64 == 0 && tuning->queueChunk <= 65536,
+                    "queue chunk must be a multiple of 64 in [64, 65536]");
+        MFH_REQUIRE(tuning->sortBounces == 0, "batches do not sort rays");
+        b.tune.traceVariant = v;
+        b.tune.chunk = tuning->queueChunk;
+        b.tune.traceGrid = 0;
+        drop_batch_graph(b);
+    });
+}
+
+int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mfh_wavefront_tuning* outBest,
+                       float* outMsPerIteration) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PtBatch& b = batch_of(c, batchId);
+        MFH_REQUIRE(iterations > 0, "autotune needs at least one iteration per candidate");
+        float best = 3.4e38f;
+        WfTuning bestTune = b.tune;
+        for (int v = 0; v < kWfNumTraceVariants; ++v) {
+            for (int ch : {256, 1024}) {
+                b.tune.traceVariant = v;
+                b.tune.chunk = ch;
+                b.tune.traceGrid = 0;
+                drop_batch_graph(b);
+                run_batch(c, b, 1);
+                MFH_HIP(hipEventRecord(c.evBegin, c.stream));
+                run_batch(c, b, iterations);
+                MFH_HIP(hipEventRecord(c.evEnd, c.stream));
+                MFH_HIP(hipEventSynchronize(c.evEnd));
+                float ms = 0.0f;
+                MFH_HIP(hipEventElapsedTime(&ms, c.evBegin, c.evEnd));
+                ms /= (float)iterations;
+                if (ms < best) {
+                    best = ms;
+                    bestTune = b.tune;
+                }
+            }
+        }
+        b.tune = bestTune;
+        drop_batch_graph(b);
+        for (int32_t id : b.tracers) reset(c, tracer_of(c, id));
+        b.iteration = -1;
+        if (outBest) {
+            outBest->traceBlock = kWfTraceShapes[b.tune.traceVariant].block;
+            outBest->stackDepth = kWfTraceShapes[b.tune.traceVariant].lds;
+            outBest->queueChunk = b.tune.chunk;
+            outBest->sortBounces = 0;
+        }
+        if (outMsPerIteration) *outMsPerIteration = best;
     });
 }
 

@@ -19,6 +19,9 @@
 // path's contributions reach its accumulator in the megakernel's order
 // (emission of vertex k, then its next-event estimates, then vertex k+1), so
 // both drivers produce the same bits.
+#include <array>
+#include <cstring>
+
 #include "pt_path.hpp"
 #include "pt_wavefront.hpp"
 
@@ -105,6 +108,44 @@ __device__ __forceinline__ void store_acc(const WfState& W, u32 mask, int p, con
     }
 }
 
+// Scene of a path: the kernel argument for one scene, or the batch entry
+// whose path range holds p (binary search over the scene offsets).
+__device__ __forceinline__ int scene_index(const WfState& W, int p, int& local) {
+    int lo = 0, hi = W.numScenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (W.sceneOffset[mid] <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    local = p - W.sceneOffset[lo];
+    return lo;
+}
+template <bool kBatch>
+__device__ __forceinline__ const PtScene& scene_ref(const PtScene& S, const WfState& W, int s) {
+    if constexpr (kBatch)
+        return W.scenes[s];
+    else
+        return S;
+}
+template <bool kBatch>
+__device__ __forceinline__ int locate(const WfState& W, int p, int& local) {
+    if constexpr (kBatch) return scene_index(W, p, local);
+    local = p;
+    return 0;
+}
+
+// Traversal whose instance-level specialization is fixed (kSingle) or chosen
+// per lane from the ray's scene (batches mix single- and multi-instance scenes).
+template <int kMode, bool kAny, int kStride, int kLds>
+__device__ __forceinline__ TraceHit trace_mode(const TraceParams& tp, V3 o, V3 d, float tmax, int* lstack) {
+    if constexpr (kMode == 1) return trace_ray<kAny, false, true, kStride, kLds>(tp, o, d, tmax, lstack);
+    if constexpr (kMode == 0) return trace_ray<kAny, false, false, kStride, kLds>(tp, o, d, tmax, lstack);
+    if (tp.numInstances == 1) return trace_ray<kAny, false, true, kStride, kLds>(tp, o, d, tmax, lstack);
+    return trace_ray<kAny, false, false, kStride, kLds>(tp, o, d, tmax, lstack);
+}
+
 __global__ void k_wf_begin(WfState W) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < W.numCounters) W.counters[i] = 0;
@@ -114,15 +155,28 @@ __global__ void k_wf_begin(WfState W) {
     }
 }
 
-// Camera rays and their closest hits, one thread per pixel in 16x16 tiles.
-template <bool kSingle>
-__global__ __launch_bounds__(256) void k_wf_primary(PtScene S, WfState W) {
+// Camera rays and their closest hits, one thread per pixel: 16x16 tiles for
+// one scene, one path per thread in path order for a batch (kMode 2).
+template <int kMode>
+__global__ __launch_bounds__(256) void k_wf_primary(PtScene S0, WfState W) {
     __shared__ int sStack[kLdsDepth * 256];
-    const int px = blockIdx.x * 16 + (threadIdx.x % 16);
-    const int py = blockIdx.y * 16 + (threadIdx.x / 16);
-    if (px >= S.width || py >= S.height) return;
-    const int p = px + py * S.width;
-    u64 rng = S.rngs[p];
+    constexpr bool kBatch = kMode == 2;
+    int px, py, p, local, s = 0;
+    if constexpr (kBatch) {
+        p = blockIdx.x * 256 + threadIdx.x;
+        if (p >= W.numPaths) return;
+        s = scene_index(W, p, local);
+        const int w = W.scenes[s].width;
+        px = local % w;
+        py = local / w;
+    } else {
+        px = blockIdx.x * 16 + (threadIdx.x % 16);
+        py = blockIdx.y * 16 + (threadIdx.x / 16);
+        if (px >= S0.width || py >= S0.height) return;
+        p = local = px + py * S0.width;
+    }
+    const PtScene& S = scene_ref<kBatch>(S0, W, s);
+    u64 rng = S.rngs[local];
     pcg_next(rng);  // create_camera's RndSet2
     const u64 x0 = pcg_next(rng);
     pcg_next(rng);  // the i0 word of the camera's RndSet2_1
@@ -131,23 +185,24 @@ __global__ __launch_bounds__(256) void k_wf_primary(PtScene S, WfState W) {
     float pdfF;
     camera_ray(S, px, py, x0, dir, pdfF);
     const V3 o = v3(S.camPos[0], S.camPos[1], S.camPos[2]);
-    const TraceHit h = trace_ray<false, false, kSingle, 256>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
-    S.rngs[p] = rng;
+    const TraceHit h = trace_mode<kMode, false, 256, kLdsDepth>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
+    S.rngs[local] = rng;
     W.rayO[0][p] = float4{o.x, o.y, o.z, pdfF};
     W.rayD[0][p] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
     W.rayM[0][p] = float4{0.0f, 0.0f, 0.0f, h.w1};
     W.hit[0][p] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
     W.tp[p] = float4{1.0f, 1.0f, 1.0f, 0.0f};
 #pragma unroll
-    for (int s = 0; s < 18; ++s)
-        if ((W.slotMask >> s) & 1u) W.acc[(size_t)s * W.numPaths + p] = 0.0f;
+    for (int k = 0; k < 18; ++k)
+        if ((W.slotMask >> k) & 1u) W.acc[(size_t)k * W.numPaths + p] = 0.0f;
 }
 
 // Closest hit of queue `buf` (length at counters[count]).
-template <bool kSingle, int kBlock, int kLds>
-__global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int buf, int count, int work,
+template <int kMode, int kBlock, int kLds>
+__global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S0, WfState W, int buf, int count, int work,
                                                       const int* perm) {
     __shared__ int sStack[kLds * kBlock];
+    constexpr bool kBatch = kMode == 2;
     const int n = W.counters[count];
     int* const lstack = sStack + threadIdx.x;
     for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
@@ -156,8 +211,11 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int 
             const int q = perm ? perm[qs] : qs;
             const float4 o = W.rayO[buf][q];
             const float4 d = W.rayD[buf][q];
-            const TraceHit h = trace_ray<false, false, kSingle, kBlock, kLds>(S.trace, v3(o.x, o.y, o.z),
-                                                                               v3(d.x, d.y, d.z), kMaxSceneSize, lstack);
+            int local;
+            const int s = locate<kBatch>(W, __float_as_int(d.w), local);
+            const PtScene& S = scene_ref<kBatch>(S0, W, s);
+            const TraceHit h = trace_mode<kMode, false, kBlock, kLds>(S.trace, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
+                                                                      kMaxSceneSize, lstack);
             W.hit[buf][q] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
             W.rayM[buf][q].w = h.w1;
         }
@@ -166,26 +224,29 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S, WfState W, int 
 
 // Any hit of the shadow candidates of one bounce; visible ones are added to
 // their path's accumulator in candidate order.
-template <bool kSingle, int kBlock, int kLds>
-__global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S, WfState W, int count, int work, const int* perm) {
+template <int kMode, int kBlock, int kLds>
+__global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S0, WfState W, int count, int work, const int* perm) {
     __shared__ int sStack[kLds * kBlock];
+    constexpr bool kBatch = kMode == 2;
     const int n = W.counters[count];
     int* const lstack = sStack + threadIdx.x;
-    u32 nans = 0;
     for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
         const int end = min(wk.base + wk.chunk, n);
         for (int ss = wk.base + lane_id(); ss < end; ss += 64) {
-            const int s = perm ? perm[ss] : ss;
-            const int2 pm = W.shPath[s];
+            const int sl = perm ? perm[ss] : ss;
+            const int2 pm = W.shPath[sl];
             const u32 cand = (u32)pm.y;
+            int local;
+            const int si = locate<kBatch>(W, pm.x, local);
+            const PtScene& S = scene_ref<kBatch>(S0, W, si);
             u32 visible = 0;
             for (int i = 0; i < S.neeCount; ++i) {
                 if (!((cand >> i) & 1u)) continue;
-                const size_t e = (size_t)s * S.neeCount + i;
+                const size_t e = (size_t)sl * W.neeStride + i;
                 const float4 a = W.shO[e];
                 const float4 d = W.shD[e];
-                const TraceHit h = trace_ray<true, false, kSingle, kBlock, kLds>(S.trace, v3(a.x, a.y, a.z),
-                                                                                  v3(d.x, d.y, d.z), a.w, lstack);
+                const TraceHit h = trace_mode<kMode, true, kBlock, kLds>(S.trace, v3(a.x, a.y, a.z), v3(d.x, d.y, d.z),
+                                                                         a.w, lstack);
                 if (h.inst == -1) visible |= 1u << i;
             }
             if (visible) {  // add the unoccluded estimates in candidate order
@@ -193,34 +254,33 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S, WfState W, int 
                 load_acc(W, S.targetMask, pm.x, acc);
                 for (int i = 0; i < S.neeCount; ++i) {
                     if (!((visible >> i) & 1u)) continue;
-                    const float4 c = W.shC[(size_t)s * S.neeCount + i];
+                    const float4 c = W.shC[(size_t)sl * W.neeStride + i];
                     NeeCandidate nc;
                     nc.radiance = v3(c.x, c.y, c.z);
                     nc.lightness = c.w;
                     add_nee(nc, acc);
                 }
                 store_acc(W, S.targetMask, pm.x, acc);
-                nans += acc.nans;
+                if (acc.nans) atomicAdd(S.nanCounter, acc.nans);
             }
         }
     }
-    if (nans) atomicAdd(S.nanCounter, nans);
 }
 
 // Shade the hits of queue `buf` at path length k + 1.
-__global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, int buf, int count, int nextCount,
+template <bool kBatch>
+__global__ __launch_bounds__(256) void k_wf_shade(PtScene S0, WfState W, int k, int buf, int count, int nextCount,
                                                   int shCount, int work) {
     const int n = W.counters[count];
     const int pathLen = k + 1;
     const int nbuf = buf ^ 1;
-    u32 nans = 0;
     for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
         const int end = wk.base + wk.chunk;
         for (int q0 = wk.base; q0 < end; q0 += 64) {
             const int q = q0 + lane_id();
             const bool active = q < n;
             bool doNee = false, cont = false;
-            int p = 0;
+            int p = 0, local = 0, si = 0;
             u64 rng = 0;
             V3 throughput = v3(0, 0, 0), rayOrigin = v3(0, 0, 0), dir = v3(0, 0, 0), absorption = v3(0, 0, 0);
             float footA = 0.0f, guideWeight = 0.0f, pdfF = 0.0f;
@@ -233,7 +293,9 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, i
                 const float4 m = W.rayM[buf][q];
                 const float4 h = W.hit[buf][q];
                 p = __float_as_int(d.w);
-                rng = S.rngs[p];
+                si = locate<kBatch>(W, p, local);
+                const PtScene& S = scene_ref<kBatch>(S0, W, si);
+                rng = S.rngs[local];
                 const float4 t4 = W.tp[p];
                 throughput = v3(t4.x, t4.y, t4.z);
                 footA = t4.w;
@@ -251,13 +313,14 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, i
             // next-event candidates: one shadow slot per path that estimates
             const int sh = wave_append(W.counters + shCount, doNee);
             if (doNee) {
+                const PtScene& S = scene_ref<kBatch>(S0, W, si);
                 const u64 neeSeed = pcg_next(rng);
                 u32 cand = 0;
                 for (int i = 0; i < S.neeCount; ++i) {
                     const u64 x = pcg_next(rng);
                     NeeCandidate c;
                     if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
-                        const size_t e = (size_t)sh * S.neeCount + i;
+                        const size_t e = (size_t)sh * W.neeStride + i;
                         W.shO[e] = float4{c.a.x, c.a.y, c.a.z, c.tmax};
                         W.shD[e] = float4{c.dir.x, c.dir.y, c.dir.z, 0.0f};
                         W.shC[e] = float4{c.radiance.x, c.radiance.y, c.radiance.z, c.lightness};
@@ -280,13 +343,13 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S, WfState W, int k, i
                 W.tp[p] = float4{throughput.x, throughput.y, throughput.z, footA};
             }
             if (active) {
-                S.rngs[p] = rng;
+                const PtScene& S = scene_ref<kBatch>(S0, W, si);
+                S.rngs[local] = rng;
                 store_acc(W, S.targetMask, p, acc);
-                nans += acc.nans;
+                if (acc.nans) atomicAdd(S.nanCounter, acc.nans);
             }
         }
     }
-    if (nans) atomicAdd(S.nanCounter, nans);
 }
 
 // 9-bit-per-axis Morton code of the origin in the scene box, then the
@@ -331,7 +394,7 @@ __global__ void k_wf_sort_keys(PtScene S, WfState W, int buf, int count, bool sh
             const u32 cand = (u32)W.shPath[i].y;
             if (cand) {
                 const int first = __ffs((int)cand) - 1;
-                const size_t e = (size_t)i * S.neeCount + first;
+                const size_t e = (size_t)i * W.neeStride + first;
                 key = ray_key(S, W.shO[e], W.shD[e]);
             }
         }
@@ -340,31 +403,35 @@ __global__ void k_wf_sort_keys(PtScene S, WfState W, int buf, int count, bool sh
     W.sortIds[0][i] = i;
 }
 
-__global__ void k_wf_finish(PtScene S, WfState W) {
+template <bool kBatch>
+__global__ void k_wf_finish(PtScene S0, WfState W) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= W.numPaths) return;
+    int local;
+    const int s = locate<kBatch>(W, p, local);
+    const PtScene& S = scene_ref<kBatch>(S0, W, s);
     float v[18];
 #pragma unroll
-    for (int s = 0; s < 18; ++s) v[s] = ((W.slotMask >> s) & 1u) ? W.acc[(size_t)s * W.numPaths + p] : 0.0f;
-    accumulate(S, (size_t)p, *W.iteration, v, S.targetMask);
+    for (int k = 0; k < 18; ++k) v[k] = ((W.slotMask >> k) & 1u) ? W.acc[(size_t)k * W.numPaths + p] : 0.0f;
+    accumulate(S, (size_t)local, *W.iteration, v, S.targetMask);
 }
 
-template <bool kSingle, int kBlock, int kLds>
+template <int kMode, int kBlock, int kLds>
 struct TraceKernels {
     static void extend(const PtScene& S, const WfState& W, int grid, int buf, int count, int work, const int* perm,
                        hipStream_t st) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kSingle, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
                            W, buf, count, work, perm);
     }
     static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, const int* perm,
                        hipStream_t st) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kSingle, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
                            W, count, work, perm);
     }
     static int occupancy() {
         int a = 0, b = 0;
-        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kSingle, kBlock, kLds>, kBlock, 0));
-        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kSingle, kBlock, kLds>, kBlock, 0));
+        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kMode, kBlock, kLds>, kBlock, 0));
+        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kMode, kBlock, kLds>, kBlock, 0));
         return std::max(1, std::min(a, b));
     }
 };
@@ -379,19 +446,20 @@ struct TraceVariant {
     OccFn occupancy;
 };
 
-template <bool kSingle, int kBlock, int kLds>
+template <int kMode, int kBlock, int kLds>
 constexpr TraceVariant variant() {
-    return {&TraceKernels<kSingle, kBlock, kLds>::extend, &TraceKernels<kSingle, kBlock, kLds>::shadow,
-            &TraceKernels<kSingle, kBlock, kLds>::occupancy};
+    return {&TraceKernels<kMode, kBlock, kLds>::extend, &TraceKernels<kMode, kBlock, kLds>::shadow,
+            &TraceKernels<kMode, kBlock, kLds>::occupancy};
 }
 
-// Tuning space of the trace stages: block size x LDS stack depth.
-const TraceVariant kVariants[2][kWfNumTraceVariants] = {
-    {variant<false, 64, 8>(), variant<false, 64, 16>(), variant<false, 128, 8>(), variant<false, 128, 16>(),
-     variant<false, 256, 8>(), variant<false, 256, 16>(), variant<false, 256, 24>()},
-    {variant<true, 64, 8>(), variant<true, 64, 16>(), variant<true, 128, 8>(), variant<true, 128, 16>(),
-     variant<true, 256, 8>(), variant<true, 256, 16>(), variant<true, 256, 24>()},
-};
+// Tuning space of the trace stages: block size x LDS stack depth, for
+// multi-instance scenes (0), single-instance scenes (1) and batches (2).
+template <int kMode>
+constexpr std::array<TraceVariant, kWfNumTraceVariants> variants() {
+    return {variant<kMode, 64, 8>(), variant<kMode, 64, 16>(), variant<kMode, 128, 8>(), variant<kMode, 128, 16>(),
+            variant<kMode, 256, 8>(), variant<kMode, 256, 16>(), variant<kMode, 256, 24>()};
+}
+const std::array<TraceVariant, kWfNumTraceVariants> kVariants[3] = {variants<0>(), variants<1>(), variants<2>()};
 
 }  // namespace
 
@@ -403,10 +471,13 @@ int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs) {
     return v.occupancy() * numCUs;
 }
 
+int wf_grid_batch(const WfTuning& tune, int numCUs) { return kVariants[2][tune.traceVariant].occupancy() * numCUs; }
+
 int wf_shade_grid(int numCUs) {
-    int b = 0;
-    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shade, 256, 0));
-    return std::max(1, b) * numCUs;
+    int a = 0, b = 0;
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_shade<false>, 256, 0));
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shade<true>, 256, 0));
+    return std::max(1, std::min(a, b)) * numCUs;
 }
 
 // Sorted order of a queue: keys, then a 4-pass LSD radix sort over every path
@@ -424,16 +495,16 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
     hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
     const dim3 pgrid((S.width + 15) / 16, (S.height + 15) / 16);
     if (single)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<true>), pgrid, dim3(256), 0, stream, S, W);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<1>), pgrid, dim3(256), 0, stream, S, W);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<false>), pgrid, dim3(256), 0, stream, S, W);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<0>), pgrid, dim3(256), 0, stream, S, W);
     // pt_sample's loop is a do-while: the first hit is shaded even at maximum path length 0
     const int L = S.maxPathLength > 1 ? S.maxPathLength : 1;
     int work = W.workBase;
     for (int k = 0; k < L; ++k) {
         const int buf = k & 1;
-        hipLaunchKernelGGL(k_wf_shade, dim3(tune.shadeGrid), dim3(256), 0, stream, S, W, k, buf, W.qBase + k,
-                           W.qBase + k + 1, W.shBase + k, work++);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shade<false>), dim3(tune.shadeGrid), dim3(256), 0, stream, S, W, k,
+                           buf, W.qBase + k, W.qBase + k + 1, W.shBase + k, work++);
         const bool sortNow = k < tune.sortBounces;
         if (sortNow) sort_queue(S, W, 0, W.shBase + k, true, scratch, stream);
         v.shadow(S, W, tune.traceGrid, W.shBase + k, work++, sortNow ? W.sortIds[0] : nullptr, stream);
@@ -442,7 +513,29 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
             v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, sortNow ? W.sortIds[0] : nullptr, stream);
         }
     }
-    hipLaunchKernelGGL(k_wf_finish, dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S, W);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_finish<false>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S,
+                       W);
+    MFH_HIP(hipGetLastError());
+}
+
+void launch_pt_wavefront_batch(const WfState& W, const WfTuning& tune, int maxLen, hipStream_t stream) {
+    PtScene none;
+    std::memset(&none, 0, sizeof(none));
+    const TraceVariant& v = kVariants[2][tune.traceVariant];
+    hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<2>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, none,
+                       W);
+    const int L = maxLen > 1 ? maxLen : 1;
+    int work = W.workBase;
+    for (int k = 0; k < L; ++k) {
+        const int buf = k & 1;
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shade<true>), dim3(tune.shadeGrid), dim3(256), 0, stream, none, W, k,
+                           buf, W.qBase + k, W.qBase + k + 1, W.shBase + k, work++);
+        v.shadow(none, W, tune.traceGrid, W.shBase + k, work++, nullptr, stream);
+        if (k + 1 < L) v.extend(none, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, nullptr, stream);
+    }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_finish<true>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, none,
+                       W);
     MFH_HIP(hipGetLastError());
 }
 

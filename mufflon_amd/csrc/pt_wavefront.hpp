@@ -29,6 +29,11 @@ struct WfState {
     // ray sorting: keys / ids of the radix sort (numPaths each), result in sortIds[0]
     uint64_t* sortKeys[2];
     int32_t* sortIds[2];
+    int neeStride;      // shadow entries per shadow slot (the largest NEE count)
+    // batches of scenes: path p belongs to scene s with sceneOffset[s] <= p < sceneOffset[s + 1]
+    const PtScene* scenes;
+    const int* sceneOffset;
+    int numScenes;
 };
 
 struct WfTraceShape {
@@ -51,5 +56,10 @@ int wf_shade_grid(int numCUs);
 // One iteration: begin, primary, maxPathLength x (shade, shadow, extend), finish.
 void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, BuildScratch& scratch,
                          hipStream_t stream);
+// One iteration of a batch of scenes (W.scenes, W.sceneOffset): every stage is
+// a single launch over the paths of all scenes; maxLen = the longest maximum
+// path length of the batch.
+void launch_pt_wavefront_batch(const WfState& W, const WfTuning& tune, int maxLen, hipStream_t stream);
+int wf_grid_batch(const WfTuning& tune, int numCUs);
 
 }  // namespace mfh

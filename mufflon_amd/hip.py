@@ -80,6 +80,11 @@ _SIGS = {
     "mfh_pt_enable_target": [vp, i32, i32, i32],
     "mfh_pt_set_mode": [vp, i32, i32, i32],
     "mfh_pt_adjoint": [vp, i32, i32, f32p],
+    "mfh_batch_create": [vp, i32p, i32, i32, i32p],
+    "mfh_batch_reset": [vp, i32],
+    "mfh_batch_iterate": [vp, i32, i32],
+    "mfh_batch_set_tuning": [vp, i32, vp],
+    "mfh_batch_autotune": [vp, i32, i32, vp, f32p],
     "mfh_pt_get_texture_gradient": [vp, i32, i32, f32p],
     "mfh_pt_get_emission_gradient": [vp, i32, i32, f32p],
     "mfh_pt_set_material": [vp, i32, i32, vp],
@@ -419,6 +424,29 @@ class HipContext:
 
     def pt_iterate(self, pt, iterations=1):
         self._call("mfh_pt_iterate", self.handle, pt, int(iterations))
+
+    def batch_create(self, pts, use_graph=False):
+        ids = np.ascontiguousarray(pts, np.int32)
+        out = i32()
+        self._call("mfh_batch_create", self.handle, _ptr(ids, i32p), len(ids), 1 if use_graph else 0, C.byref(out))
+        return out.value
+
+    def batch_reset(self, batch):
+        self._call("mfh_batch_reset", self.handle, batch)
+
+    def batch_iterate(self, batch, iterations=1):
+        self._call("mfh_batch_iterate", self.handle, batch, iterations)
+
+    def batch_set_tuning(self, batch, trace_block=256, stack_depth=16, queue_chunk=256):
+        r = np.zeros(1, TUNING_DTYPE)
+        r["traceBlock"], r["stackDepth"], r["queueChunk"], r["sortBounces"] = trace_block, stack_depth, queue_chunk, 0
+        self._call("mfh_batch_set_tuning", self.handle, batch, r.ctypes.data)
+
+    def batch_autotune(self, batch, iterations=1):
+        r = np.zeros(1, TUNING_DTYPE)
+        ms = np.zeros(1, np.float32)
+        self._call("mfh_batch_autotune", self.handle, batch, iterations, r.ctypes.data, _ptr(ms, f32p))
+        return {k: int(r[k][0]) for k in TUNING_DTYPE.names}, float(ms[0])
 
     def texture_update(self, tex, texels):
         t = np.ascontiguousarray(texels, np.float32)
