@@ -525,3 +525,62 @@ def cornell_textured(sphere_rings=128, sphere_segments=256, size=256, wall_res=5
     c["envmap"] = None
     c["point_lights"] = None
     return c
+
+
+def batch_scene(index, seed0=100, palette=64, point_lights=32, env_size=(64, 128)):
+    """One scene of the uneven batch: seed seed0 + index; triangle count
+    log-uniform in [10k, 2M] (a random triangle soup or a heightfield, by
+    seed parity), resolution log-uniform in [128^2, 1024^2], maximum path
+    length uniform in {4..16}, the 60/30/10 material palette, point lights and
+    a small procedural environment map."""
+    seed = seed0 + index
+    rng = np.random.default_rng(seed)
+    tris = int(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))
+    side = int(np.exp(rng.uniform(np.log(128), np.log(1024))))
+    depth = int(rng.integers(4, 17))
+    mats = material_palette(palette, seed)
+    kinds = mats["type"]
+    weights = np.where(kinds == MAT_LAMBERT, 0.6 / np.sum(kinds == MAT_LAMBERT),
+                       np.where(kinds == MAT_TORRANCE, 0.3 / np.sum(kinds == MAT_TORRANCE),
+                                0.1 / np.sum(kinds == MAT_MICROFACET)))
+    if seed % 2 == 0:
+        v, t = random_triangle_soup(tris, seed=seed)
+        n_tri = len(t)
+        # per-vertex normals of the soup: the face normal
+        fn = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+        fn /= np.maximum(np.linalg.norm(fn, axis=1, keepdims=True), 1e-20)
+        normals = np.repeat(fn, 3, axis=0).astype(np.float32) if len(v) == 3 * n_tri else \
+            np.tile(np.float32([0, 1, 0]), (len(v), 1))
+        uvs = np.zeros((len(v), 2), np.float32)
+        eye, target = np.float32([0.5, 0.5, -1.2]), np.float32([0.5, 0.5, 0.5])
+        extent = 1.0
+    else:
+        g = max(int(np.sqrt(tris / 2)) + 1, 8)
+        hf = fbm_heightfield(g, 6, seed)
+        v, normals, uvs, t = grid_mesh(hf, 1.0, 0.25 * g)
+        n_tri = len(t)
+        extent = float(g - 1)
+        eye = np.float32([0.05 * extent, float(hf.max()) * 0.25 * g + 0.2 * extent, 0.05 * extent])
+        target = np.float32([0.5 * extent, 0.1 * g, 0.5 * extent])
+    mat_idx = rng.choice(palette, size=n_tri, p=weights).astype(np.uint16)
+    view = target - eye
+    view /= np.linalg.norm(view)
+    right = np.cross(view, [0, 1, 0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, view).astype(np.float32)
+    cam = dict(position=eye.astype(np.float32), view_dir=view.astype(np.float32), up=up,
+               tan_vfov=np.float32(np.tan(np.radians(25.0))), width=side, height=side)
+    lo, hi = v.min(0), v.max(0)
+    pl = np.zeros(point_lights, POINT_LIGHT_DTYPE)
+    pl["position"] = lo + rng.random((point_lights, 3)) * (hi - lo) + np.float32([0, 0.3, 0]) * (hi - lo)
+    pl["intensity"] = rng.lognormal(mean=0.0, sigma=1.0, size=(point_lights, 1)) * extent * extent * \
+        rng.uniform(0.6, 1.0, (point_lights, 3)) / (4 * np.pi)
+    env = procedural_envmap(env_size[0], env_size[1], seed=seed, sun=(0.3, 0.72), sun_radiance=50.0)
+    # material 0 (a LAMBERT of the palette) reads its albedo from texture 1: the differentiable parameter
+    mats["albedoTexture"][0] = 1
+    albedo = procedural_texture(64, 64, 3, 0.1, 0.9, seed=seed)
+    return dict(lods=[dict(verts=v.astype(np.float32), normals=normals.astype(np.float32), uvs=uvs, tris=t,
+                           mat_idx=mat_idx)],
+                inst_lod=np.array([0], np.int32), world_to_inst=IDENTITY34.reshape(1, 12).copy(), materials=mats,
+                point_lights=pl, textures=[(env, SAMPLING_LINEAR), (albedo, SAMPLING_LINEAR)], envmap=0, camera=cam,
+                max_path_length=depth)

@@ -35,3 +35,19 @@ def path_tracer(ctx, scene, seed=1, max_path_length=16, nee_count=1, background_
         ctx.pt_set_background(pt, hip.background_record(hip.BACKGROUND_ENVMAP, scale=background_scale,
                                                         envmap_texture=int(tex[scene["envmap"]])))
     return pt, sc, tex
+
+
+def uneven_batch(ctx, count=48, seed0=100, use_graph=True, scenes=None):
+    """The uneven batch: `count` procedural scenes (scenes.batch_scene) as path
+    tracers and one batch over them. Returns (batch id, path tracer ids,
+    texture ids per scene, scene dicts)."""
+    from mufflon_amd import scenes as S
+    if scenes is None:
+        scenes = [S.batch_scene(i, seed0) for i in range(count)]
+    pts, texs = [], []
+    for sc in scenes:
+        pt, _, tex = path_tracer(ctx, sc, max_path_length=sc["max_path_length"])
+        pts.append(pt)
+        texs.append(tex)
+    batch = ctx.batch_create(pts, use_graph)
+    return batch, pts, texs, scenes
