@@ -257,6 +257,17 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations);
 /* Enable or disable a render target (render_enable_render_target /
  * render_disable_render_target, interface.cpp:3876-3940); resets the path tracer. */
 int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable);
+/* Record the running variance of a target (render_enable_render_target with
+ * variance = true, OutputHandler::enable_render_target, output_handler.hpp:391-405;
+ * enabling also enables the target). Disabling drops only the variance
+ * (disable_render_target with variance = true, :407-430). Resets the path
+ * tracer. The variance follows UpdateIter (output_handler_variance.hpp:7-31):
+ * diff = x - mean; mean += diff / max(1, i + 1);
+ * var += (diff * (x - mean) - var) / max(1, i) for iteration index i. */
+int mfh_pt_enable_variance(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable);
+/* Copy a target's running variance (same layout as mfh_pt_get_target;
+ * mufflon_get_target_image with variance = true, interface.cpp:241-262). */
+int mfh_pt_get_variance(mfh_context* ctx, int32_t ptId, int32_t target, float* out);
 /* Execution form of an iteration. MEGAKERNEL: one thread runs a pixel's
  * whole path (GpuPathTracer's sample_pt kernel, gpu_pt.cu:54-138).
  * WAVEFRONT: persistent kernels drain ray queues stage by stage (shade,

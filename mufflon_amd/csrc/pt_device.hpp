@@ -101,7 +101,9 @@ struct PtScene {
     // per-pixel state and targets
     uint64_t* rngs;
     float* cum[kNumTargets];  // running means of the enabled targets (null when disabled)
+    float* var[kNumTargets];  // running variances of the targets in varianceMask (null otherwise)
     uint32_t targetMask;
+    uint32_t varianceMask;    // subset of targetMask
     uint32_t* nanCounter;
     int32_t iteration;   // index of the iteration being run (0 after a reset)
 };

@@ -23,7 +23,9 @@ struct PathTracer {
     DevBuf dMaterials, dPtLods, dLightMem, dMapKeys, dMapCodes, dTextures, dSat;
     DevBuf dRngs, dNan;
     DevBuf dTargets[kNumTargets];
+    DevBuf dVariance[kNumTargets];
     uint32_t targetMask = (1u << MFH_TARGET_RADIANCE) | (1u << MFH_TARGET_HIT_ID);
+    uint32_t varianceMask = 0;  // subset of targetMask
     int32_t iteration = -1;
     uint64_t sceneGeneration = ~0ull;
     // wavefront form
@@ -103,6 +105,7 @@ void destroy_tracers(Context& c) {
                           &t->dSat, &t->dRngs, &t->dNan})
             b->release();
         for (DevBuf& b : t->dTargets) b.release();
+        for (DevBuf& b : t->dVariance) b.release();
         release_wavefront(*t);
         for (DevBuf* b : {&t->adjRngs, &t->adjDelta, &t->adjEmGrad, &t->adjTexGradPtrs}) b->release();
         for (DevBuf& b : t->adjTexGrad) b.release();
@@ -273,8 +276,12 @@ PtScene device_scene(Context& c, PathTracer& t) {
     S.maxPathLength = t.params.maxPathLength;
     S.neeCount = t.params.neeCount;
     S.rngs = t.dRngs.as<uint64_t>();
-    for (int k = 0; k < kNumTargets; ++k) S.cum[k] = (t.targetMask >> k) & 1u ? t.dTargets[k].as<float>() : nullptr;
+    for (int k = 0; k < kNumTargets; ++k) {
+        S.cum[k] = (t.targetMask >> k) & 1u ? t.dTargets[k].as<float>() : nullptr;
+        S.var[k] = (t.varianceMask >> k) & 1u ? t.dVariance[k].as<float>() : nullptr;
+    }
     S.targetMask = t.targetMask;
+    S.varianceMask = t.varianceMask;
     S.nanCounter = t.dNan.as<uint32_t>();
     return S;
 }
@@ -286,6 +293,9 @@ void reset(Context& c, PathTracer& t) {
         if (!((t.targetMask >> k) & 1u)) continue;
         t.dTargets[k].reserve(4 * (size_t)kTargetChannels[k] * num);
         MFH_HIP(hipMemsetAsync(t.dTargets[k].ptr, 0, 4 * (size_t)kTargetChannels[k] * num, c.stream));
+        if (!((t.varianceMask >> k) & 1u)) continue;
+        t.dVariance[k].reserve(4 * (size_t)kTargetChannels[k] * num);
+        MFH_HIP(hipMemsetAsync(t.dVariance[k].ptr, 0, 4 * (size_t)kTargetChannels[k] * num, c.stream));
     }
     MFH_HIP(hipMemsetAsync(t.dNan.ptr, 0, 4, c.stream));
     t.iteration = -1;
@@ -496,6 +506,7 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
                               &t->dSat, &t->dRngs, &t->dNan})
                 b->release();
             for (DevBuf& b : t->dTargets) b.release();
+        for (DevBuf& b : t->dVariance) b.release();
             delete t;
             throw;
         }
@@ -960,19 +971,67 @@ int mfh_pt_get_target(mfh_context* ctx, int32_t ptId, int32_t target, float* out
     });
 }
 
+namespace {
+
+// OutputHandler::enable_render_target / disable_render_target
+// (output_handler.hpp:391-430): enabling with variance records both,
+// disabling always drops the variance and, unless only the variance is
+// disabled, the target. A change resets the render
+// (IRenderer::on_render_target_changed, interface.cpp:3627-3628).
+void set_targets(mfh::Context& c, mfh::PathTracer& t, uint32_t mask, uint32_t varMask) {
+    varMask &= mask;
+    if (mask == t.targetMask && varMask == t.varianceMask) return;
+    MFH_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < kNumTargets; ++k) {
+        if (!((mask >> k) & 1u)) t.dTargets[k].release();
+        if (!((varMask >> k) & 1u)) t.dVariance[k].release();
+    }
+    t.targetMask = mask;
+    t.varianceMask = varMask;
+    drop_graph(t);
+    reset(c, t);
+}
+
+}  // namespace
+
 int mfh_pt_enable_target(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable) {
     return guarded([&] {
         Context& c = ctx_of(ctx);
         PathTracer& t = tracer_of(c, ptId);
         MFH_REQUIRE(target >= 0 && target < kNumTargets, "unknown render target " + std::to_string(target));
         const uint32_t bit = 1u << target;
-        const uint32_t mask = enable ? (t.targetMask | bit) : (t.targetMask & ~bit);
-        if (mask == t.targetMask) return;
+        if (enable)
+            set_targets(c, t, t.targetMask | bit, t.varianceMask);
+        else
+            set_targets(c, t, t.targetMask & ~bit, t.varianceMask & ~bit);
+    });
+}
+
+int mfh_pt_enable_variance(mfh_context* ctx, int32_t ptId, int32_t target, int32_t enable) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(target >= 0 && target < kNumTargets, "unknown render target " + std::to_string(target));
+        const uint32_t bit = 1u << target;
+        if (enable)
+            set_targets(c, t, t.targetMask | bit, t.varianceMask | bit);
+        else
+            set_targets(c, t, t.targetMask, t.varianceMask & ~bit);
+    });
+}
+
+int mfh_pt_get_variance(mfh_context* ctx, int32_t ptId, int32_t target, float* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output pointer");
+        MFH_REQUIRE(target >= 0 && target < kNumTargets, "unknown render target " + std::to_string(target));
+        MFH_REQUIRE((t.varianceMask >> target) & 1u,
+                    "render target " + std::to_string(target) + " does not record variance");
+        const size_t num = (size_t)t.camera.width * t.camera.height;
         MFH_HIP(hipStreamSynchronize(c.stream));
-        t.targetMask = mask;
-        if (!enable) t.dTargets[target].release();
-        drop_graph(t);
-        reset(c, t);
+        MFH_HIP(hipMemcpy(out, t.dVariance[target].ptr, 4 * (size_t)kTargetChannels[target] * num,
+                          hipMemcpyDeviceToHost));
     });
 }
 

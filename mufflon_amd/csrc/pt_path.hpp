@@ -291,14 +291,30 @@ __device__ __forceinline__ void camera_ray(const PtScene& S, int px, int py, u64
 // Running means of the enabled targets (output_handler_variance.hpp:7-31).
 __device__ __forceinline__ void accumulate(const PtScene& S, size_t pixel, int iteration, const float* v, u32 mask) {
     const float denom = mx(1.0f, (float)iteration + 1);
+    const float varDenom = mx(1.0f, (float)iteration);
+    const u32 varMask = S.varianceMask & mask;
     int slot = 0;
 #pragma unroll
     for (int t = 0; t < kNumTargets; ++t) {
         const int nc = kTargetChannels[t];
         if ((mask >> t) & 1u) {
             float* c = S.cum[t] + (size_t)nc * pixel;
+            if ((varMask >> t) & 1u) {
+                // UpdateIter (output_handler_variance.hpp:7-31): mean and
+                // variance with the iteration index as the divisor base
+                float* var = S.var[t] + (size_t)nc * pixel;
 #pragma unroll
-            for (int k = 0; k < nc; ++k) c[k] += (v[slot + k] - c[k]) / denom;
+                for (int k = 0; k < nc; ++k) {
+                    const float iter = v[slot + k];
+                    const float diff = iter - c[k];
+                    const float cum = c[k] + diff / denom;
+                    c[k] = cum;
+                    var[k] += (diff * (iter - cum) - var[k]) / varDenom;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < nc; ++k) c[k] += (v[slot + k] - c[k]) / denom;
+            }
         }
         slot += nc;
     }

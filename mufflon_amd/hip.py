@@ -78,6 +78,8 @@ _SIGS = {
     "mfh_pt_iterate": [vp, i32, i32],
     "mfh_pt_get_target": [vp, i32, i32, f32p],
     "mfh_pt_enable_target": [vp, i32, i32, i32],
+    "mfh_pt_enable_variance": [vp, i32, i32, i32],
+    "mfh_pt_get_variance": [vp, i32, i32, f32p],
     "mfh_pt_set_mode": [vp, i32, i32, i32],
     "mfh_pt_adjoint": [vp, i32, i32, f32p],
     "mfh_batch_create": [vp, i32p, i32, i32, i32p],
@@ -492,6 +494,16 @@ class HipContext:
 
     def pt_enable_target(self, pt, target, enable=True):
         self._call("mfh_pt_enable_target", self.handle, pt, target, 1 if enable else 0)
+
+    def pt_enable_variance(self, pt, target, enable=True):
+        self._call("mfh_pt_enable_variance", self.handle, pt, target, 1 if enable else 0)
+
+    def pt_variance(self, pt, target=TARGET_RADIANCE):
+        w, h = self._pt_size[pt]
+        ch = TARGET_CHANNELS[target]
+        out = np.zeros(w * h * ch, np.float32)
+        self._call("mfh_pt_get_variance", self.handle, pt, target, _ptr(out, f32p))
+        return out.reshape(h, w, ch)
 
     def pt_target(self, pt, target=TARGET_RADIANCE):
         w, h = self._pt_size[pt]

@@ -438,27 +438,40 @@ def pt_render(pt_scene, seed, min_path_length, max_path_length, nee_count, itera
 TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)  # Radiance, Hit ID, Position, Depth, Normal, UV, Albedo, Lightness
 
 
-def pt_render_targets(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, targets):
+def pt_render_targets(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, targets,
+                      variance=()):
     """Oracle path tracer: {target: running mean (h, w, channels)} for the
-    requested target ids, and the NaN count."""
+    requested target ids, and the NaN count. With `variance` (target ids, a
+    subset of `targets`) it returns ({target: mean}, {target: variance}, nans)."""
     lib = load()
     if not hasattr(lib, "_ptt_ready"):
-        lib.orc_pt_render_targets.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                                              C.c_int32, C.c_uint32, C.POINTER(f32p), u32p]
-        lib.orc_pt_render_targets.restype = C.c_int
+        lib.orc_pt_render_targets_var.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32,
+                                                  C.c_int32, C.c_int32, C.c_uint32, C.POINTER(f32p), C.c_uint32,
+                                                  C.POINTER(f32p), u32p]
+        lib.orc_pt_render_targets_var.restype = C.c_int
         lib._ptt_ready = True
     w, h = pt_scene.s.width, pt_scene.s.height
-    mask = 0
-    bufs = {}
+    mask = vmask = 0
+    bufs, vbufs = {}, {}
     ptrs = (f32p * 8)()
+    vptrs = (f32p * 8)()
     for t in targets:
         mask |= 1 << t
         bufs[t] = np.zeros(w * h * TARGET_CHANNELS[t], np.float32)
         ptrs[t] = _p(bufs[t], f32p)
+    for t in variance:
+        if t not in bufs:
+            raise ValueError("variance of target %d requested without the target" % t)
+        vmask |= 1 << t
+        vbufs[t] = np.zeros(w * h * TARGET_CHANNELS[t], np.float32)
+        vptrs[t] = _p(vbufs[t], f32p)
     nans = C.c_uint32()
-    lib.orc_pt_render_targets(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
-                              mask, ptrs, C.byref(nans))
-    return {t: b.reshape(h, w, TARGET_CHANNELS[t]) for t, b in bufs.items()}, nans.value
+    lib.orc_pt_render_targets_var(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count,
+                                  iterations, mask, ptrs, vmask, vptrs, C.byref(nans))
+    means = {t: b.reshape(h, w, TARGET_CHANNELS[t]) for t, b in bufs.items()}
+    if variance:
+        return means, {t: b.reshape(h, w, TARGET_CHANNELS[t]) for t, b in vbufs.items()}, nans.value
+    return means, nans.value
 
 
 def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, delta, tex_shapes):

@@ -1564,19 +1564,26 @@ int orc_pt_adjoint(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int
 
 // Runs `iterations` iterations from a reset (fresh per-pixel RNGs) and returns
 // the running means of the enabled targets (targetMask bit t -> outTargets[t],
-// kTargetChannels[t] floats per pixel, row-major) plus the number of NaN
-// contributions dropped.
-int orc_pt_render_targets(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
-                          int32_t neeCount, int32_t iterations, uint32_t targetMask, float** outTargets,
-                          uint32_t* outNans) {
+// kTargetChannels[t] floats per pixel, row-major), the running variances of
+// the targets in varianceMask (-> outVariance[t]; UpdateIter,
+// output_handler_variance.hpp:7-31, called with the iteration index of
+// output_handler.hpp:138-139,223-225) plus the number of NaN contributions
+// dropped.
+int orc_pt_render_targets_var(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
+                              int32_t neeCount, int32_t iterations, uint32_t targetMask, float** outTargets,
+                              uint32_t varianceMask, float** outVariance, uint32_t* outNans) {
+    varianceMask &= targetMask;
     const int W = S->width, H = S->height;
     const int num = W * H;
     std::vector<Pcg64> rngs;
     rngs.reserve(num);
     const int s = seed * (num + 1);  // cpu_pt.cpp:40-45
     for (int i = 0; i < num; ++i) rngs.emplace_back((u32)(i + s));
-    for (int t = 0; t < kNumTargets; ++t)
+    for (int t = 0; t < kNumTargets; ++t) {
         if (targetMask & (1u << t)) std::memset(outTargets[t], 0, sizeof(float) * kTargetChannels[t] * (size_t)num);
+        if (varianceMask & (1u << t))
+            std::memset(outVariance[t], 0, sizeof(float) * kTargetChannels[t] * (size_t)num);
+    }
     const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
     u32 nans = 0;
     for (int it = 0; it < iterations; ++it) {
@@ -1587,17 +1594,33 @@ int orc_pt_render_targets(const OrcPtScene* S, int32_t seed, int32_t minPathLeng
             pt_sample(*S, params, pixel % W, pixel / W, rngs[pixel], acc);
             nans += acc.nans;
             const float denom = mx(1.0f, (float)it + 1);
+            const float varDenom = mx(1.0f, (float)it);
             for (int t = 0; t < kNumTargets; ++t) {
                 if (!(targetMask & (1u << t))) continue;
                 for (int c = 0; c < kTargetChannels[t]; ++c) {
-                    float& cum = outTargets[t][(size_t)kTargetChannels[t] * pixel + c];
-                    cum += (acc.v[kTargetOffset[t] + c] - cum) / denom;
+                    const size_t idx = (size_t)kTargetChannels[t] * pixel + c;
+                    const float iter = acc.v[kTargetOffset[t] + c];
+                    float& cum = outTargets[t][idx];
+                    const float diff = iter - cum;
+                    cum += diff / denom;
+                    if (varianceMask & (1u << t)) {
+                        float& var = outVariance[t][idx];
+                        const float varIter = diff * (iter - cum);
+                        var += (varIter - var) / varDenom;
+                    }
                 }
             }
         }
     }
     if (outNans) *outNans = nans;
     return 0;
+}
+
+int orc_pt_render_targets(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
+                          int32_t neeCount, int32_t iterations, uint32_t targetMask, float** outTargets,
+                          uint32_t* outNans) {
+    return orc_pt_render_targets_var(S, seed, minPathLength, maxPathLength, neeCount, iterations, targetMask,
+                                     outTargets, 0u, nullptr, outNans);
 }
 
 // Radiance and Hit ID only.

@@ -138,3 +138,33 @@ def test_pt_all_targets(hip_ctx, oracle):
     hip_ctx.pt_enable_target(pt, hip.TARGET_DEPTH, False)
     with pytest.raises(hip.HipError):
         hip_ctx.pt_target(pt, hip.TARGET_DEPTH)
+
+
+@pytest.mark.parametrize("mode", [hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT])
+def test_pt_target_variance(hip_ctx, oracle, mode):
+    """Running variance of Radiance and Depth (render_enable_render_target with
+    variance) against the oracle, megakernel and wavefront forms; disabling
+    the variance keeps the mean."""
+    c = S.cornell_materials(12, 24, 40, 40)
+    print("pt parity variance mode=%d" % mode, flush=True)
+    pt, sc, bg = P.gpu_pt_materials(hip_ctx, c, "envmap")
+    hip_ctx.pt_set_mode(pt, mode, mode == hip.PT_WAVEFRONT)
+    hip_ctx.pt_enable_variance(pt, hip.TARGET_RADIANCE, True)
+    hip_ctx.pt_enable_variance(pt, hip.TARGET_DEPTH, True)
+    hip_ctx.pt_iterate(pt, 6)
+    box, _, _, _ = hip_ctx.scene_get(sc, 1)
+    ps = P.oracle_pt_scene_materials(oracle, c, bg, light_tree=hip_ctx.pt_light_tree(pt), box=box)
+    mean, var, _ = oracle.pt_render_targets(ps, 1, 0, 8, 1, 6, [0, 1, 3], variance=[0, 3])
+    for t in (0, 3):
+        for g, o, what in ((hip_ctx.pt_target(pt, t), mean[t], "mean"), (hip_ctx.pt_variance(pt, t), var[t], "var")):
+            bad = np.abs(g - o) > RTOL * np.abs(o) + ATOL
+            print("target %d %s: bit-exact=%s, out of tolerance=%d" %
+                  (t, what, np.array_equal(g.view(np.uint32), o.view(np.uint32)), int(bad.sum())), flush=True)
+            assert not bad.any(), (t, what)
+    with pytest.raises(hip.HipError):
+        hip_ctx.pt_variance(pt, hip.TARGET_HIT_ID)
+    hip_ctx.pt_enable_variance(pt, hip.TARGET_DEPTH, False)
+    hip_ctx.pt_iterate(pt, 1)
+    hip_ctx.pt_target(pt, hip.TARGET_DEPTH)
+    with pytest.raises(hip.HipError):
+        hip_ctx.pt_variance(pt, hip.TARGET_DEPTH)
