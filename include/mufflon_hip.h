@@ -189,6 +189,10 @@ typedef struct mfh_pt_params {
     int32_t minPathLength; /* "Min. path length", default 0 */
     int32_t maxPathLength; /* "Max. path length", default 16 */
     int32_t neeCount;      /* "NEE count", default 1 */
+    int32_t neePositionGuide; /* "NEE position guide", default 0 (false): light tree
+                                 traversal by flux over squared distance to the
+                                 shading point (guide_flux_pos,
+                                 light_tree_sampling.hpp:64-71) */
 } mfh_pt_params;
 
 /* Render targets of the pt renderer (PtTargets, pt_params.hpp:16-26;
@@ -249,6 +253,13 @@ int mfh_lod_set_shading(mfh_context* ctx, int32_t lodId, const float* normals, c
 int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materials, int32_t numMaterials,
                   const mfh_point_light* pointLights, int32_t numPointLights, const mfh_pinhole* camera,
                   const mfh_pt_params* params, int32_t* outPtId);
+/* Replace the parameters (renderer_set_parameter_*, interface.cpp; a change
+ * resets the render as IRenderer::on_renderer_parameter_changed does,
+ * renderer.hpp:91-143). */
+int mfh_pt_set_params(mfh_context* ctx, int32_t ptId, const mfh_pt_params* params);
+/* Replace the camera, resolution included (scenario_set_camera /
+ * scenario_set_resolution / world_set_camera_*; resets the render). */
+int mfh_pt_set_camera(mfh_context* ctx, int32_t ptId, const mfh_pinhole* camera);
 /* Re-seed the per-pixel RNGs and clear the targets (render_reset,
  * GpuPathTracer::post_reset, gpu_pt.cpp:33-38). */
 int mfh_pt_reset(mfh_context* ctx, int32_t ptId);

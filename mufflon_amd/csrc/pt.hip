@@ -72,9 +72,10 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
         throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
         footA += 1.0f / fsqrt(pdfF);
         guideWeight = 1.f / (1.f + footA * footA / 1.0f);
+        const V3 prevPos = pathLen == 0 ? rayOrigin : vx.position;
         make_vertex(S, rayOrigin, dir, pdfF, hit.t, hit.inst, hit.prim, hit.w0, hit.w1, vx);
         ++pathLen;
-        vertex_targets(S, vx, pathLen, throughput, guideWeight, acc);
+        vertex_targets(S, vx, pathLen, throughput, guideWeight, acc, prevPos);
         if (vx.isVoid) break;
     } while (pathLen < S.maxPathLength);
 

@@ -146,6 +146,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
             throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
             footA += 1.0f / fsqrt(pdfF);
             guideWeight = 1.f / (1.f + footA * footA / 1.0f);
+            const V3 prevPos = pathLen == 0 ? rayOrigin : vx.position;
             make_vertex(S, rayOrigin, dir, pdfF, hit.t, hit.inst, hit.prim, hit.w0, hit.w1, vx);
             ++pathLen;
             AdjVertex& r = rec[pathLen];
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
             }
             if (pathLen >= S.minPathLength) {
                 V3 em;
-                const V3 cc = emission_contribution(S, vx, pathLen, throughput, em);
+                const V3 cc = emission_contribution(S, vx, pathLen, throughput, em, prevPos);
                 r.emW = delta * cc;
                 if (matIdx >= 0 && vx.mat.type == kMatEmissive) add_emission_grad(S, A, matIdx, r.emW);
             }

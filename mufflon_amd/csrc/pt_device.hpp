@@ -98,6 +98,7 @@ struct PtScene {
     int32_t width, height;
     // PtParameters (pt_params.hpp:8-14)
     int32_t seed, minPathLength, maxPathLength, neeCount;
+    int32_t posGuide;  // "NEE position guide": distance-weighted light tree traversal
     // per-pixel state and targets
     uint64_t* rngs;
     float* cum[kNumTargets];  // running means of the enabled targets (null when disabled)

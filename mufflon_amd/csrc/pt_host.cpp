@@ -32,7 +32,7 @@ struct PathTracer {
     int32_t mode = MFH_PT_MEGAKERNEL;
     bool useGraph = false;
     WfTuning tune;
-    DevBuf wfCounters, wfTp, wfAcc, wfRay[2][4], wfSh[3], wfShPath, wfSortKeys[2], wfSortIds[2];
+    DevBuf wfCounters, wfTp, wfAcc, wfRay[2][4], wfRayP[2], wfSh[3], wfShPath, wfSortKeys[2], wfSortIds[2];
     int wfPaths = 0, wfMaxLen = -1, wfNee = -1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -58,6 +58,7 @@ void release_wavefront(PathTracer& t) {
         b->release();
     for (auto& r : t.wfRay)
         for (DevBuf& b : r) b.release();
+    for (DevBuf& b : t.wfRayP) b.release();
     t.wfPaths = 0;
     t.wfMaxLen = t.wfNee = -1;
 }
@@ -70,7 +71,7 @@ struct PtBatch {
     WfTuning tune;
     int32_t iteration = -1;
     DevBuf dScenes, dOffsets;
-    DevBuf counters, tp, acc, ray[2][4], sh[3], shPath, sortKeys[2], sortIds[2];
+    DevBuf counters, tp, acc, ray[2][4], rayP[2], sh[3], shPath, sortKeys[2], sortIds[2];
     int paths = 0, maxLen = 0, nee = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -88,7 +89,8 @@ void drop_batch_graph(PtBatch& b) {
 void destroy_batch(PtBatch* b) {
     drop_batch_graph(*b);
     for (DevBuf* d : {&b->dScenes, &b->dOffsets, &b->counters, &b->tp, &b->acc, &b->sh[0], &b->sh[1], &b->sh[2],
-                      &b->shPath, &b->sortKeys[0], &b->sortKeys[1], &b->sortIds[0], &b->sortIds[1]})
+                      &b->shPath, &b->sortKeys[0], &b->sortKeys[1], &b->sortIds[0], &b->sortIds[1], &b->rayP[0],
+                      &b->rayP[1]})
         d->release();
     for (auto& r : b->ray)
         for (DevBuf& d : r) d.release();
@@ -275,6 +277,7 @@ PtScene device_scene(Context& c, PathTracer& t) {
     S.minPathLength = t.params.minPathLength;
     S.maxPathLength = t.params.maxPathLength;
     S.neeCount = t.params.neeCount;
+    S.posGuide = t.params.neePositionGuide ? 1 : 0;
     S.rngs = t.dRngs.as<uint64_t>();
     for (int k = 0; k < kNumTargets; ++k) {
         S.cum[k] = (t.targetMask >> k) & 1u ? t.dTargets[k].as<float>() : nullptr;
@@ -358,6 +361,10 @@ WfState wavefront_state(Context& c, PathTracer& t) {
         W.rayD[b] = t.wfRay[b][1].as<float4>();
         W.rayM[b] = t.wfRay[b][2].as<float4>();
         W.hit[b] = t.wfRay[b][3].as<float4>();
+        if (t.params.neePositionGuide) {
+            t.wfRayP[b].reserve(16 * (size_t)N);
+            W.rayP[b] = t.wfRayP[b].as<float4>();
+        }
     }
     W.shO = t.wfSh[0].as<float4>();
     W.shD = t.wfSh[1].as<float4>();
@@ -506,12 +513,41 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
                               &t->dSat, &t->dRngs, &t->dNan})
                 b->release();
             for (DevBuf& b : t->dTargets) b.release();
-        for (DevBuf& b : t->dVariance) b.release();
+            for (DevBuf& b : t->dVariance) b.release();
             delete t;
             throw;
         }
         c.tracers.push_back(t);
         *outPtId = (int32_t)c.tracers.size() - 1;
+    });
+}
+
+int mfh_pt_set_params(mfh_context* ctx, int32_t ptId, const mfh_pt_params* params) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(params != nullptr, "null pointer");
+        MFH_REQUIRE(params->maxPathLength >= 0 && params->neeCount >= 0, "invalid path tracer parameters");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        t.params = *params;
+        drop_graph(t);
+        reset(c, t);
+    });
+}
+
+int mfh_pt_set_camera(mfh_context* ctx, int32_t ptId, const mfh_pinhole* camera) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(camera != nullptr, "null pointer");
+        MFH_REQUIRE(camera->width > 0 && camera->height > 0 && camera->width < 65536 && camera->height < 65536,
+                    "camera resolution out of range");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        t.camera = *camera;
+        t.dRngs.reserve(8 * (size_t)camera->width * camera->height);
+        t.adjValid = false;
+        drop_graph(t);
+        reset(c, t);
     });
 }
 
@@ -694,11 +730,17 @@ WfState batch_state(Context& c, PtBatch& b, std::vector<PtScene>& scenes) {
     W.iteration = b.counters.as<int>() + numCounters;
     W.tp = b.tp.as<float4>();
     W.acc = b.acc.as<float>();
+    bool posGuide = false;
+    for (const PtScene& S : scenes) posGuide = posGuide || S.posGuide;
     for (int k = 0; k < 2; ++k) {
         W.rayO[k] = b.ray[k][0].as<float4>();
         W.rayD[k] = b.ray[k][1].as<float4>();
         W.rayM[k] = b.ray[k][2].as<float4>();
         W.hit[k] = b.ray[k][3].as<float4>();
+        if (posGuide) {
+            b.rayP[k].reserve(16 * (size_t)N);
+            W.rayP[k] = b.rayP[k].as<float4>();
+        }
     }
     W.shO = b.sh[0].as<float4>();
     W.shD = b.sh[1].as<float4>();

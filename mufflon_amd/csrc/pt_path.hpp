@@ -142,7 +142,7 @@ __device__ __forceinline__ void make_vertex(const PtScene& S, V3 rayOrigin, V3 d
 // background) with the MIS weight of hitting the emitter, times the
 // throughput; em receives the weighted emission.
 __device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
-                                                    V3& em) {
+                                                    V3& em, V3 prevPos) {
     float connectPdf = 0.0f;
     if (vx.isVoid) {  // get_emission, VOID (path_util.hpp:415-424)
         float pdfBack;
@@ -150,7 +150,7 @@ __device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vert
         if (nonzero(em)) connectPdf = background_pdf(S, pdfBack);
     } else {
         em = emission(vx.mat, vx.ts.geoN, neg(vx.incident()));
-        if (nonzero(em)) connectPdf = light_pdf(S, vx.inst, vx.prim);
+        if (nonzero(em)) connectPdf = light_pdf(S, vx.inst, vx.prim, prevPos);
     }
     if (nonzero(em) && pathLen > 1) {
         const float mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / vx.incidentPdf));
@@ -163,14 +163,14 @@ __device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vert
 // Hit ID on the first segment, then emission with MIS and the guide-weighted
 // auxiliary targets.
 __device__ __forceinline__ void vertex_targets(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
-                                               float guideWeight, Acc& acc) {
+                                               float guideWeight, Acc& acc, V3 prevPos) {
     if (pathLen == 1) {
         acc.add(1, 3, (float)vx.inst);
         acc.add(1, 4, (float)vx.prim);
     }
     if (pathLen < S.minPathLength) return;
     V3 em;
-    const V3 cc = emission_contribution(S, vx, pathLen, throughput, em);
+    const V3 cc = emission_contribution(S, vx, pathLen, throughput, em, prevPos);
     acc.add(0, 0, cc.x);
     acc.add(0, 1, cc.y);
     acc.add(0, 2, cc.z);

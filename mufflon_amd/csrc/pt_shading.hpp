@@ -642,6 +642,37 @@ __device__ Nee connect_light(const PtScene& S, uint16_t type, u32 offset, V3 pos
     return n;
 }
 
+// get_center (light_tree.hpp:284-296, lights.hpp:210-228) of a child of a
+// light tree node: an internal node's stored center, a point light's
+// position, a triangle's p0 + (e1 + e2) / 3.
+__device__ __forceinline__ V3 light_center(const PtScene& S, u32 offset, uint16_t type) {
+    const uint8_t* p = S.lightMem + offset;
+    if (type == kLightInternalNode) {
+        const float4 b = ld4(p + 16);
+        return v3(b.y, b.z, b.w);
+    }
+    const float4 q0 = ld4(p);
+    if (type == kLightPoint) return v3(q0.x, q0.y, q0.z);
+    const float4 q1 = ld4(p + 16), q2 = ld4(p + 32);
+    const V3 p0 = v3(q0.x, q0.y, q0.z), e1 = v3(q0.w, q1.x, q1.y), e2 = v3(q1.z, q1.w, q2.x);
+    return p0 + (e1 + e2) / 3.0f;
+}
+
+// Probability of the left child of a node {a, b} (the node's two float4s):
+// guide_flux, or guide_flux_pos with the "NEE position guide" parameter
+// (light_tree_sampling.hpp:58-71, :218-226, :356-364).
+__device__ __forceinline__ float guide_left(const PtScene& S, float4 a, float4 b, V3 ref) {
+    float lFlux = a.x, rFlux = b.x;
+    if (S.posGuide) {
+        const u32 types = __float_as_uint(a.z);
+        const V3 dl = light_center(S, __float_as_uint(a.y), (uint16_t)(types & 0xFFFFu)) - ref;
+        const V3 dr = light_center(S, __float_as_uint(a.w), (uint16_t)(types >> 16)) - ref;
+        lFlux /= dot(dl, dl);
+        rFlux /= dot(dr, dr);
+    }
+    return lFlux / (lFlux + rFlux);
+}
+
 // lights::connect (light_tree_sampling.hpp:195-329): background first, then
 // the (empty) directional subtree, then the positional subtree.
 __device__ Nee connect(const PtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, float u0, float u1) {
@@ -675,9 +706,8 @@ __device__ Nee connect(const PtScene& S, u64 index, u64 numIndices, u64 seed, V3
         u32 offset = 0;
         for (int guard = 0; type == kLightInternalNode && guard < 64; ++guard) {
             const float4 a = ld4(S.lightMem + offset), b = ld4(S.lightMem + offset + 16);
-            const float lFlux = a.x, rFlux = b.x;
             const u32 types = __float_as_uint(a.z);
-            const float probLeft = lFlux / (lFlux + rFlux);
+            const float probLeft = guide_left(S, a, b, pos);
             const u64 boundary = left + percentage_of(right - left, probLeft);
             if (rndChoice < boundary) {
                 type = (uint16_t)(types & 0xFFFFu);
@@ -698,8 +728,9 @@ __device__ Nee connect(const PtScene& S, u64 index, u64 numIndices, u64 seed, V3
     return n;
 }
 
-// light_pdf (light_tree_sampling.hpp:340-413): connect pdf of a hit emitter.
-__device__ float light_pdf(const PtScene& S, int inst, int prim) {
+// light_pdf (light_tree_sampling.hpp:340-413): connect pdf of a hit emitter
+// seen from refPos (the previous vertex of the path).
+__device__ float light_pdf(const PtScene& S, int inst, int prim, V3 refPos) {
     const float treeFlux = S.dirRootFlux + S.posRootFlux + S.bgFlux;
     float p = S.posRootFlux / treeFlux;
     const u64 key = ((u64)(u32)inst << 32) | (u64)(u32)prim;
@@ -722,7 +753,7 @@ __device__ float light_pdf(const PtScene& S, int inst, int prim) {
     uint16_t type = (uint16_t)S.posRootType;
     for (int guard = 0; type == kLightInternalNode && guard < 64; ++guard) {
         const float4 a = ld4(S.lightMem + offset), b = ld4(S.lightMem + offset + 16);
-        const float pLeft = a.x / (a.x + b.x);
+        const float pLeft = guide_left(S, a, b, refPos);
         const u32 types = __float_as_uint(a.z);
         if (code & 0x80000000u) {
             p *= (1.0f - pLeft);

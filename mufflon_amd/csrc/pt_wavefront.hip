@@ -190,6 +190,7 @@ __global__ __launch_bounds__(256) void k_wf_primary(PtScene S0, WfState W) {
     W.rayO[0][p] = float4{o.x, o.y, o.z, pdfF};
     W.rayD[0][p] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
     W.rayM[0][p] = float4{0.0f, 0.0f, 0.0f, h.w1};
+    if (W.rayP[0]) W.rayP[0][p] = float4{o.x, o.y, o.z, 0.0f};
     W.hit[0][p] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
     W.tp[p] = float4{1.0f, 1.0f, 1.0f, 0.0f};
 #pragma unroll
@@ -305,7 +306,12 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S0, WfState W, int k, 
                 guideWeight = 1.f / (1.f + footA * footA / 1.0f);
                 make_vertex(S, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), o.w, h.x, inst, __float_as_int(h.z), h.w, m.w, vx);
                 load_acc(W, S.targetMask, p, acc);
-                vertex_targets(S, vx, pathLen, throughput, guideWeight, acc);
+                V3 prevPos = v3(0.0f, 0.0f, 0.0f);
+                if (W.rayP[buf]) {
+                    const float4 pp = W.rayP[buf][q];
+                    prevPos = v3(pp.x, pp.y, pp.z);
+                }
+                vertex_targets(S, vx, pathLen, throughput, guideWeight, acc, prevPos);
                 const bool more = !vx.isVoid && pathLen < S.maxPathLength;
                 doNee = more && pathLen + 1 >= S.minPathLength && pathLen + 1 <= S.maxPathLength;
                 cont = more;
@@ -340,6 +346,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S0, WfState W, int k, 
                 W.rayO[nbuf][nq] = float4{rayOrigin.x, rayOrigin.y, rayOrigin.z, pdfF};
                 W.rayD[nbuf][nq] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
                 W.rayM[nbuf][nq] = float4{absorption.x, absorption.y, absorption.z, 0.0f};
+                if (W.rayP[nbuf]) W.rayP[nbuf][nq] = float4{vx.position.x, vx.position.y, vx.position.z, 0.0f};
                 W.tp[p] = float4{throughput.x, throughput.y, throughput.z, footA};
             }
             if (active) {

@@ -34,7 +34,7 @@ SAMPLING_LINEAR = 1
 PINHOLE_DTYPE = np.dtype([("position", "<f4", 3), ("tanVFov", "<f4"), ("viewDir", "<f4", 3), ("up", "<f4", 3),
                           ("width", "<i4"), ("height", "<i4")])
 PT_PARAMS_DTYPE = np.dtype([("seed", "<i4"), ("minPathLength", "<i4"), ("maxPathLength", "<i4"),
-                            ("neeCount", "<i4")])
+                            ("neeCount", "<i4"), ("neePositionGuide", "<i4")])
 POINT_LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("intensity", "<f4", 3)])
 AREA_LIGHT_DTYPE = np.dtype([("points", "<f4", 9), ("uv", "<f4", 6), ("material", "<i4"), ("instance", "<i4"),
                              ("primitive", "<i4")])
@@ -75,6 +75,8 @@ _SIGS = {
     "mfh_lod_set_shading": [vp, i32, f32p, f32p, vp],
     "mfh_pt_create": [vp, i32, vp, i32, vp, i32, vp, vp, i32p],
     "mfh_pt_reset": [vp, i32],
+    "mfh_pt_set_params": [vp, i32, vp],
+    "mfh_pt_set_camera": [vp, i32, vp],
     "mfh_pt_iterate": [vp, i32, i32],
     "mfh_pt_get_target": [vp, i32, i32, f32p],
     "mfh_pt_enable_target": [vp, i32, i32, i32],
@@ -139,10 +141,11 @@ def camera_record(cam):
     return r
 
 
-def params_record(seed=0, min_path_length=0, max_path_length=16, nee_count=1):
+def params_record(seed=0, min_path_length=0, max_path_length=16, nee_count=1, nee_position_guide=False):
     r = np.zeros(1, PT_PARAMS_DTYPE)
     r["seed"], r["minPathLength"], r["maxPathLength"], r["neeCount"] = seed, min_path_length, max_path_length, \
         nee_count
+    r["neePositionGuide"] = 1 if nee_position_guide else 0
     return r
 
 
@@ -423,6 +426,15 @@ class HipContext:
 
     def pt_reset(self, pt):
         self._call("mfh_pt_reset", self.handle, pt)
+
+    def pt_set_params(self, pt, params):
+        par = params if isinstance(params, np.ndarray) else params_record(**params)
+        self._call("mfh_pt_set_params", self.handle, pt, par.ctypes.data)
+
+    def pt_set_camera(self, pt, camera):
+        cam = camera if isinstance(camera, np.ndarray) else camera_record(camera)
+        self._call("mfh_pt_set_camera", self.handle, pt, cam.ctypes.data)
+        self._pt_size[pt] = (int(cam["width"][0]), int(cam["height"][0]))
 
     def pt_iterate(self, pt, iterations=1):
         self._call("mfh_pt_iterate", self.handle, pt, int(iterations))

@@ -252,6 +252,7 @@ class OrcPtScene(C.Structure):
         ("sceneBox", C.c_float * 6), ("camPos", C.c_float * 3), ("tanVFov", C.c_float),
         ("camDir", C.c_float * 3), ("camUp", C.c_float * 3), ("width", C.c_int32), ("height", C.c_int32),
         ("textures", C.POINTER(OrcTexture)), ("numTextures", C.c_int32), ("background", OrcBackground),
+        ("posGuide", C.c_int32),
     ]
 
 
@@ -373,10 +374,11 @@ def background_evaluate(bscene, direction):
 class PtScene:
     """Oracle path-tracer scene over an oracle traversal Scene (single LoD list
     order), with shading arrays concatenated like the geometry, the host-built
-    light tree (info, mem, keys, codes) and a pinhole camera dict."""
+    light tree (info, mem, keys, codes) and a pinhole camera dict.
+    pos_guide: the "NEE position guide" parameter."""
 
     def __init__(self, scene, normals, uvs, mat_idx, materials, light_tree, scene_box, camera, textures=None,
-                 background=None, background_sat=None, background_flux=(0.0, 0.0, 0.0)):
+                 background=None, background_sat=None, background_flux=(0.0, 0.0, 0.0), pos_guide=False):
         self.scene = scene
         self.textures = textures or Textures()
         self.sat = None if background_sat is None else np.ascontiguousarray(background_sat, np.float32)
@@ -415,7 +417,36 @@ class PtScene:
         s.width = int(camera["width"])
         s.height = int(camera["height"])
         _fill_background(s, background, self.textures, self.sat, background_flux)
+        s.posGuide = 1 if pos_guide else 0
         self.s = s
+
+
+def _light_lib():
+    lib = load()
+    if not hasattr(lib, "_lc_ready"):
+        lib.orc_light_connect.argtypes = [C.POINTER(OrcPtScene), C.c_uint64, C.c_uint64, C.c_uint64, f32p, C.c_float,
+                                          C.c_float, f32p]
+        lib.orc_light_pdf.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, f32p]
+        lib.orc_light_pdf.restype = C.c_float
+        lib._lc_ready = True
+    return lib
+
+
+def light_connect(pt_scene, index, num_indices, seed, pos, u0=0.25, u1=0.5):
+    """One NEE connection through the light tree: (light position (3,),
+    creation pdf, diffIrradiance (3,), byte offset of the chosen positional
+    light or -1 for the background)."""
+    p = np.ascontiguousarray(pos, np.float32)
+    out = np.zeros(8, np.float32)
+    _light_lib().orc_light_connect(C.byref(pt_scene.s), index, num_indices, seed, _p(p, f32p), u0, u1,
+                                   _p(out, f32p))
+    return out[:3], float(out[3]), out[4:7], int(out[7])
+
+
+def light_pdf(pt_scene, inst, prim, ref_pos):
+    """Connect pdf (per area) of hitting emitter triangle (inst, prim) from ref_pos."""
+    p = np.ascontiguousarray(ref_pos, np.float32)
+    return float(_light_lib().orc_light_pdf(C.byref(pt_scene.s), inst, prim, _p(p, f32p)))
 
 
 def pt_render(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations):

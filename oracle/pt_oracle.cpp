@@ -181,6 +181,7 @@ struct OrcPtScene {
     const OrcTexture* textures;
     int32_t numTextures;
     OrcBackground background;
+    int32_t posGuide;  // "NEE position guide" (PNeePositionGuide, parameter.hpp:359-364)
 };
 }
 
@@ -1060,8 +1061,46 @@ inline Nee adjust_pdf(Nee n, float p) {  // light_tree_sampling.hpp:16-19
     return n;
 }
 
+// get_center (light_tree.hpp:284-296; lights.hpp:210-228) of a light tree
+// child: an internal node's center, a point light's position, a triangle's
+// posV[0] + (posV[1] + posV[2]) / 3 (its record holds the edges).
+V3 light_center(const OrcPtScene& S, uint32_t offset, uint16_t type) {
+    const uint8_t* p = S.lightMem + offset;
+    if (type == kInternalNode) {
+        NodeRec n;
+        std::memcpy(&n, p, sizeof(n));
+        return {n.center[0], n.center[1], n.center[2]};
+    }
+    if (type == 0) {
+        PointLightRec L;
+        std::memcpy(&L, p, sizeof(L));
+        return {L.position[0], L.position[1], L.position[2]};
+    }
+    TriLight t;
+    std::memcpy(&t, p, sizeof(t));
+    const V3 p0{t.posV[0][0], t.posV[0][1], t.posV[0][2]};
+    const V3 e1{t.posV[1][0], t.posV[1][1], t.posV[1][2]};
+    const V3 e2{t.posV[2][0], t.posV[2][1], t.posV[2][2]};
+    return p0 + (e1 + e2) / 3.0f;
+}
+
+// guide_flux / guide_flux_pos (light_tree_sampling.hpp:58-71): probability of
+// the left child, by flux or by flux over the squared distance to ref.
+float guide_left(const OrcPtScene& S, const NodeRec& node, V3 ref) {
+    float leftFlux = node.leftFlux, rightFlux = node.rightFlux;
+    if (S.posGuide) {
+        const V3 dl = light_center(S, node.leftOffset, node.leftType) - ref;
+        const V3 dr = light_center(S, node.rightOffset, node.rightType) - ref;
+        leftFlux /= dot(dl, dl);
+        rightFlux /= dot(dr, dr);
+    }
+    return leftFlux / (leftFlux + rightFlux);
+}
+
 // lights::connect (light_tree_sampling.hpp:195-329); the directional subtree is empty
-Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, float u0, float u1) {
+Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, float u0, float u1,
+            int32_t* outLeaf = nullptr) {
+    if (outLeaf) *outLeaf = -1;
     u64 rndChoice = numIndices > 0 ? seed + index * (~0ull / numIndices) : seed;
     if (rndChoice == ~0ull) --rndChoice;
     const float fluxSum = S.bgFlux + S.dirRootFlux + S.posRootFlux;
@@ -1081,7 +1120,7 @@ Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, fl
     while (type == kInternalNode) {
         NodeRec node;
         std::memcpy(&node, S.lightMem + offset, sizeof(node));
-        const float probLeft = node.leftFlux / (node.leftFlux + node.rightFlux);
+        const float probLeft = guide_left(S, node, pos);
         const u64 boundary = left + percentage_of(right - left, probLeft);
         if (rndChoice < boundary) {
             type = node.leftType;
@@ -1095,11 +1134,13 @@ Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, fl
             lightProb *= (1.0f - probLeft);
         }
     }
+    if (outLeaf) *outLeaf = (int32_t)offset;
     return adjust_pdf(connect_light(S, type, S.lightMem + offset, pos, u0, u1), lightProb);
 }
 
 // light_pdf (light_tree_sampling.hpp:340-413) -> connect pdf of a hit emitter
-float light_pdf(const OrcPtScene& S, i32 inst, i32 prim) {
+// seen from refPos, the previous vertex of the path (path_util.hpp:443-444)
+float light_pdf(const OrcPtScene& S, i32 inst, i32 prim, V3 refPos) {
     const float treeFlux = S.dirRootFlux + S.posRootFlux + S.bgFlux;
     float p = S.posRootFlux / treeFlux;
     const u64 key = ((u64)(u32)inst << 32) | (u64)(u32)prim;
@@ -1122,7 +1163,7 @@ float light_pdf(const OrcPtScene& S, i32 inst, i32 prim) {
     while (type == kInternalNode) {
         NodeRec node;
         std::memcpy(&node, S.lightMem + offset, sizeof(node));
-        const float pLeft = node.leftFlux / (node.leftFlux + node.rightFlux);
+        const float pLeft = guide_left(S, node, refPos);
         if (code & 0x80000000u) {
             p *= (1.0f - pLeft);
             type = node.rightType;
@@ -1283,6 +1324,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
             next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
         }
+        const V3 prevPos = vtx.position;
         vtx = next;
         if (pathLen == 0) {
             const float hid[2] = {(float)vtx.inst, (float)vtx.prim};
@@ -1298,7 +1340,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                 if (nonzero(emission)) connectPdf = background_pdf(S, b.pdfBack);
             } else {
                 emission = mat_emission(vtx.mat, vtx.ts.geoN, -vtx.incident_dir());
-                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim);
+                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos);
             }
             if (nonzero(emission) && pathLen > 1) {
                 const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
@@ -1478,6 +1520,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
             next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
         }
+        const V3 prevPos = vtx.position;
         vtx = next;
         ++pathLen;
         AdjRec& r = rec[pathLen];
@@ -1496,7 +1539,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
                 if (nonzero(emission)) connectPdf = background_pdf(S, b.pdfBack);
             } else {
                 emission = mat_emission(vtx.mat, vtx.ts.geoN, -vtx.incident_dir());
-                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim);
+                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos);
             }
             if (nonzero(emission) && pathLen > 1) {
                 const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
@@ -1694,6 +1737,27 @@ int orc_background_sample(const OrcPtScene* S, float u0, float u1, float* out7) 
     out7[5] = d.radiance.y;
     out7[6] = d.radiance.z;
     return 0;
+}
+// Light tree sampling hooks: one NEE connection from pos (light position
+// xyz, creation pdf, diffIrradiance xyz, byte offset of the chosen
+// positional light or -1 for the background) and the connect pdf of a hit
+// emitter triangle seen from refPos.
+int orc_light_connect(const OrcPtScene* S, uint64_t index, uint64_t numIndices, uint64_t seed, const float* pos,
+                      float u0, float u1, float* out8) {
+    int32_t leaf = -1;
+    const Nee n = connect(*S, index, numIndices, seed, {pos[0], pos[1], pos[2]}, u0, u1, &leaf);
+    out8[0] = n.position.x;
+    out8[1] = n.position.y;
+    out8[2] = n.position.z;
+    out8[3] = n.creationPdf;
+    out8[4] = n.diffIrradiance.x;
+    out8[5] = n.diffIrradiance.y;
+    out8[6] = n.diffIrradiance.z;
+    out8[7] = (float)leaf;
+    return 0;
+}
+float orc_light_pdf(const OrcPtScene* S, int32_t inst, int32_t prim, const float* refPos) {
+    return light_pdf(*S, inst, prim, {refPos[0], refPos[1], refPos[2]});
 }
 int orc_background_evaluate(const OrcPtScene* S, const float* dir, float* out4) {
     const BgEval b = evaluate_background(*S, {dir[0], dir[1], dir[2]});

@@ -120,7 +120,7 @@ def background_of(c, kind):
     return hip.background_record(hip.BACKGROUND_COLORED)
 
 
-def gpu_pt_materials(ctx, c, background="envmap", seed=1, max_path_length=8, nee_count=1):
+def gpu_pt_materials(ctx, c, background="envmap", seed=1, max_path_length=8, nee_count=1, pos_guide=False):
     ids = np.array([ctx.texture_create(t, s) for t, s in c["textures"]], np.int32)
     mats = c["materials"].copy()
     for f in ("albedoTexture", "roughnessTexture"):
@@ -130,7 +130,8 @@ def gpu_pt_materials(ctx, c, background="envmap", seed=1, max_path_length=8, nee
     ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
     ctx.lbvh_build(lod, scene_box(c))
     sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
-    pt = ctx.pt_create(sc, mats, c["camera"], dict(seed=seed, max_path_length=max_path_length, nee_count=nee_count),
+    pt = ctx.pt_create(sc, mats, c["camera"], dict(seed=seed, max_path_length=max_path_length, nee_count=nee_count,
+                                                   nee_position_guide=pos_guide),
                        point_lights=c["point_lights"])
     bg = background_of(c, background)
     dev_bg = bg.copy()
@@ -140,7 +141,7 @@ def gpu_pt_materials(ctx, c, background="envmap", seed=1, max_path_length=8, nee
     return pt, sc, bg
 
 
-def oracle_pt_scene_materials(O, c, background, light_tree=None, box=None):
+def oracle_pt_scene_materials(O, c, background, light_tree=None, box=None, pos_guide=False):
     """Oracle scene of cornell_materials; without light_tree the host builders
     make the light tree (area + point lights) and the background."""
     v, t = c["verts"], c["tris"]
@@ -157,7 +158,7 @@ def oracle_pt_scene_materials(O, c, background, light_tree=None, box=None):
     lt = (info,) + tuple(light_tree[1:])
     tx = O.Textures(c["textures"])
     return O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], lt, box, c["camera"],
-                     textures=tx, background=bg[0], background_sat=sat, background_flux=flux)
+                     textures=tx, background=bg[0], background_sat=sat, background_flux=flux, pos_guide=pos_guide)
 
 
 def gpu_pt_textured(ctx, c, seed=1, max_path_length=8, nee_count=1):

@@ -168,3 +168,34 @@ def test_pt_target_variance(hip_ctx, oracle, mode):
     hip_ctx.pt_target(pt, hip.TARGET_DEPTH)
     with pytest.raises(hip.HipError):
         hip_ctx.pt_variance(pt, hip.TARGET_DEPTH)
+
+
+@pytest.mark.parametrize("mode", [hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT])
+def test_pt_nee_position_guide(hip_ctx, oracle, mode):
+    """"NEE position guide" (distance-weighted light tree traversal in NEE and
+    in the hit MIS) against the oracle: area and point lights plus an env
+    map, megakernel and wavefront; changing the parameter resets and changes
+    the image."""
+    c = S.cornell_materials(12, 24, 40, 40)
+    print("pt parity position guide mode=%d" % mode, flush=True)
+    pt, sc, bg = P.gpu_pt_materials(hip_ctx, c, "envmap", pos_guide=True, nee_count=2)
+    hip_ctx.pt_set_mode(pt, mode, mode == hip.PT_WAVEFRONT)
+    hip_ctx.pt_iterate(pt, 4)
+    g = hip_ctx.pt_target(pt)
+    box, _, _, _ = hip_ctx.scene_get(sc, 1)
+    lt = hip_ctx.pt_light_tree(pt)
+    ps = P.oracle_pt_scene_materials(oracle, c, bg, light_tree=lt, box=box, pos_guide=True)
+    o, _ = oracle.pt_render_targets(ps, 1, 0, 8, 2, 4, [0, 1])
+    o = o[0]
+    bad = np.abs(g - o) > RTOL * np.abs(o) + ATOL
+    print("position guide: bit-exact=%s, out of tolerance=%d" %
+          (np.array_equal(g.view(np.uint32), o.view(np.uint32)), int(bad.any(axis=2).sum())), flush=True)
+    assert not bad.any()
+    hip_ctx.pt_set_params(pt, dict(seed=1, max_path_length=8, nee_count=2, nee_position_guide=False))
+    assert hip_ctx.pt_iteration(pt) == -1
+    hip_ctx.pt_iterate(pt, 4)
+    g2 = hip_ctx.pt_target(pt)
+    ps2 = P.oracle_pt_scene_materials(oracle, c, bg, light_tree=lt, box=box, pos_guide=False)
+    o2, _ = oracle.pt_render_targets(ps2, 1, 0, 8, 2, 4, [0, 1])
+    assert not (np.abs(g2 - o2[0]) > RTOL * np.abs(o2[0]) + ATOL).any()
+    assert not np.array_equal(g, g2)

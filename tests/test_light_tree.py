@@ -181,3 +181,105 @@ def test_oracle_targets_closed_forms(oracle):
     expect = np.where(mats["type"][c["mat_idx"][prim]][..., None] == S.MAT_LAMBERT,
                       mats["albedo"][c["mat_idx"][prim]], 0.0)
     np.testing.assert_allclose(alb[hitmask], expect[hitmask], rtol=1e-5, atol=1e-6)
+
+
+def _light_scene(oracle, al, pl, mats, pos_guide):
+    """Oracle PtScene over a dummy one-triangle scene carrying the tree of
+    the given lights (only the light fields are read by the hooks)."""
+    import scene_util as U
+    box = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    tree = hip.light_tree_build(al, pl, box, mats)
+    v = np.array([[5, 5, 5], [6, 5, 5], [5, 6, 5]], np.float32)
+    t = np.array([[0, 1, 2]], np.uint32)
+    scene, _ = U.oracle_scene(oracle, [(v, t)], np.array([0], np.int32), S.IDENTITY34.reshape(1, 12))
+    cam = dict(position=(0.5, 0.5, -2.0), view_dir=(0, 0, 1), up=(0, 1, 0), tan_vfov=0.5, width=4, height=4)
+    ps = oracle.PtScene(scene, np.zeros((3, 3), np.float32), np.zeros((3, 2), np.float32), np.zeros(1, np.uint16),
+                        mats, tree, box, cam, pos_guide=pos_guide)
+    return ps, tree
+
+
+def _leaf_probabilities(tree, ref, pos_guide):
+    """Choice probability of every positional leaf (by byte offset), from the
+    node records: guide_flux = lf / (lf + rf), or with the position guide
+    lf / |cl - ref|^2 over the same sum (light_tree_sampling.hpp:58-71),
+    with cl the child's center (node center, point position, triangle
+    p0 + (e1 + e2) / 3). Also returns a relative error bound of the float
+    evaluation: each factor q (pLeft or 1 - pLeft, rounded in float) adds
+    about 2^-23 / q."""
+    info, mem, _, _ = tree
+
+    def center(off, typ):
+        if typ == INTERNAL:
+            return np.frombuffer(mem[off:off + 32].tobytes(), NODE)[0]["center"].astype(np.float64)
+        if typ == 0:
+            return np.frombuffer(mem[off:off + 32].tobytes(), POINT)[0]["position"].astype(np.float64)
+        pv = np.frombuffer(mem[off:off + 64].tobytes(), TRI)[0]["posV"].astype(np.float64)
+        return pv[:3] + (pv[3:6] + pv[6:9]) / 3
+
+    probs = {}
+    stack = [(0, int(info["posRootType"]), 1.0, 0.0)]
+    ref = np.asarray(ref, np.float64)
+    while stack:
+        off, typ, p, err = stack.pop()
+        if typ != INTERNAL:
+            probs[off] = (typ, p, err)
+            continue
+        nd = np.frombuffer(mem[off:off + 32].tobytes(), NODE)[0]
+        lf, rf = float(nd["lflux"]), float(nd["rflux"])
+        if pos_guide:
+            lf /= np.sum((center(int(nd["loff"]), int(nd["ltype"])) - ref) ** 2)
+            rf /= np.sum((center(int(nd["roff"]), int(nd["rtype"])) - ref) ** 2)
+        pl = lf / (lf + rf)
+        eps = 2.0 ** -23
+        stack.append((int(nd["loff"]), int(nd["ltype"]), p * pl, err + 4 * eps + eps / pl))
+        stack.append((int(nd["roff"]), int(nd["rtype"]), p * (1 - pl), err + 4 * eps + eps / (1 - pl)))
+    return probs
+
+
+@pytest.mark.parametrize("pos_guide", [False, True])
+def test_oracle_light_choice_probabilities(oracle, pos_guide):
+    """NEE light choice through the tree, with and without the "NEE position
+    guide": (i) the probability connect reports for the chosen light equals
+    the product of the guide probabilities along its path (closed form from
+    the node records); (ii) a uniform sweep of the choice variable lands on
+    every light with that probability; (iii) light_pdf of a hit emitter
+    triangle times its area equals the same probability (the hit-MIS walk
+    agrees with the sampling walk)."""
+    al, pl, mats = _random_lights(6, 5, 99)
+    ps, tree = _light_scene(oracle, al, pl, mats, pos_guide)
+    info, mem, keys, _ = tree
+    n_sweep = 1 << 13
+    for ref in ([0.5, 0.5, 0.5], [0.05, 0.9, 0.2], [1.5, -0.3, 0.7]):
+        probs = _leaf_probabilities(tree, ref, pos_guide)
+        hits = {}
+        for i in range(n_sweep):
+            _, pdf, _, leaf = oracle.light_connect(ps, i, n_sweep, 12345, ref)
+            assert leaf in probs
+            typ, p, err = probs[leaf]
+            if typ == 0:
+                got = pdf / 4294967296.0
+            else:
+                pv = np.frombuffer(mem[leaf:leaf + 64].tobytes(), TRI)[0]["posV"].astype(np.float64)
+                got = pdf * np.linalg.norm(np.cross(pv[3:6], pv[6:9])) / 2
+            assert abs(got - p) <= p * (2 * err + 1e-6), (ref, leaf, got, p, err)
+            hits[leaf] = hits.get(leaf, 0) + 1
+        for leaf, (_, p, _) in probs.items():
+            assert abs(hits.get(leaf, 0) / n_sweep - p) <= 2.0 / n_sweep + 1e-6, (ref, leaf)
+        for a in al:
+            inst, prim = int(a["instance"]), int(a["primitive"])
+            lp = oracle.light_pdf(ps, inst, prim, ref)
+            key = (inst << 32) | prim
+            j = int(np.nonzero(keys == key)[0][0])
+            # the leaf of this triangle: the record whose first point matches
+            leaf = [o for o, (t, _, _) in probs.items()
+                    if t == 2 and np.array_equal(np.frombuffer(mem[o:o + 12].tobytes(), "<f4"), a["points"][:3])]
+            assert len(leaf) == 1, j
+            pts = a["points"].reshape(3, 3).astype(np.float32)
+            area = np.linalg.norm(np.cross((pts[1] - pts[0]).astype(np.float64), (pts[2] - pts[0]).astype(np.float64))) / 2
+            _, p, err = probs[leaf[0]]
+            assert abs(lp * area - p) <= p * (2 * err + 1e-6), (ref, j)
+    if pos_guide:
+        # the guide changes the choice with the reference position
+        a = _leaf_probabilities(tree, [0.05, 0.9, 0.2], True)
+        b = _leaf_probabilities(tree, [0.9, 0.1, 0.8], True)
+        assert max(abs(a[k][1] - b[k][1]) for k in a) > 0.05
