@@ -163,7 +163,7 @@ int mfh_texture_update(mfh_context* ctx, int32_t textureId, const float* texels)
 typedef struct mfh_background {
     int32_t type;
     float color[3];
-    float scale;
+    float scale[3];        /* per channel (Background::set_scale, background.hpp:119-123) */
     int32_t envmapTexture; /* ENVMAP: texture id (RGB, polar) */
 } mfh_background;
 

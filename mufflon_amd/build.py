@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "pt_adjoint.hip", "capi.cpp",
-           "light_tree.cpp", "pt_host.cpp"]
+           "light_tree.cpp", "pt_host.cpp", "core_api.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

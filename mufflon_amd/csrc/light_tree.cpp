@@ -354,12 +354,14 @@ void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vect
     }
 }
 
-void build_background(int32_t type, const float color[3], float scale, const float* envRgba, int32_t width,
+void build_background(int32_t type, const float color[3], const float scale[3], const float* envRgba, int32_t width,
                       int32_t height, const float sceneBox[6], BackgroundHost& out) {
     out = BackgroundHost{};
     out.type = type;
-    for (int k = 0; k < 3; ++k) out.color[k] = color[k];
-    out.scale = scale;
+    for (int k = 0; k < 3; ++k) {
+        out.color[k] = color[k];
+        out.scale[k] = scale[k];
+    }
     const V3 diag{sceneBox[3] - sceneBox[0], sceneBox[4] - sceneBox[1], sceneBox[5] - sceneBox[2]};
     if (type == 0) {
         // compute_constant_flux (background.cpp:125-135); the COLORED descriptor
@@ -402,10 +404,10 @@ void build_background(int32_t type, const float color[3], float scale, const flo
                 const float surface = diag.y * diag.z * std::fabs(dir.x) + diag.x * diag.z * std::fabs(dir.y) +
                                       diag.x * diag.y * std::fabs(dir.z);
                 const float w = avgPixelSolidAngle / (2.0f * kPi * kPi * sinTheta);
-                for (int k = 0; k < 3; ++k) flux[k] += (t[k] * scale) * w * surface;
+                for (int k = 0; k < 3; ++k) flux[k] += (t[k] * scale[k]) * w * surface;
             }
         }
-        for (int k = 0; k < 3; ++k) out.flux[k] = flux[k] * scale;
+        for (int k = 0; k < 3; ++k) out.flux[k] = flux[k] * scale[k];
     }
     out.fluxSum = out.flux[0] + out.flux[1] + out.flux[2];
 }

@@ -58,7 +58,7 @@ struct LightTreeHost {
 struct BackgroundHost {
     int32_t type = 0;
     float color[3] = {0.0f, 0.0f, 0.0f};
-    float scale = 1.0f;
+    float scale[3] = {1.0f, 1.0f, 1.0f};  // Spectrum m_scale (background.hpp:119-123)
     int32_t envmapTexture = -1;
     int32_t width = 0, height = 0;
     std::vector<float> sat;  // width * height, R32F
@@ -69,7 +69,7 @@ struct BackgroundHost {
 // Background::acquire_const (background.cpp:30-66): flux for the scene box and,
 // for ENVMAP, create_summed_area_table (texture_sampling.hpp:49-110).
 // envRgba: 4 floats per texel (NULL for COLORED).
-void build_background(int32_t type, const float color[3], float scale, const float* envRgba, int32_t width,
+void build_background(int32_t type, const float color[3], const float scale[3], const float* envRgba, int32_t width,
                       int32_t height, const float sceneBox[6], BackgroundHost& out);
 
 void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vector<HostPointLight>& points,

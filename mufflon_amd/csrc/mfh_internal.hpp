@@ -183,5 +183,9 @@ void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* o
 // Re-upload the LoD table of a scene (and rebuild its instance level) when a
 // LoD was rebuilt after the scene was created.
 void refresh_scene(Context& ctx, Scene& sc);
+// Inverse of an affine row-major 3x4 transform as the reference computes it
+// on the host (descriptors.hpp:108-129): adjugate of the 3x3 part over its
+// determinant, translation -(inverse * t).
+void instance_to_world(const float* w, float* o);
 
 }  // namespace mfh

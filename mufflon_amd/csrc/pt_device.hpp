@@ -60,7 +60,7 @@ struct DevTexture {
 struct DevBackground {
     int32_t type;
     float color[3];
-    float scale;
+    float scale[3];  // Spectrum scale per channel
     int32_t envmapTexture;
     const float* sat;  // width * height summed-area table (ENVMAP)
 };

@@ -116,8 +116,6 @@ void destroy_tracers(Context& c) {
     c.tracers.clear();
 }
 
-namespace {
-
 // descriptors.hpp:108-129 (host copy of the device inversion)
 void instance_to_world(const float* w, float* o) {
     const float r00 = w[0], r01 = w[1], r02 = w[2];
@@ -140,6 +138,8 @@ void instance_to_world(const float* w, float* o) {
     const float t[3] = {w[3], w[7], w[11]};
     for (int r = 0; r < 3; ++r) o[4 * r + 3] = -(o[4 * r] * t[0] + o[4 * r + 1] * t[1] + o[4 * r + 2] * t[2]);
 }
+
+namespace {
 
 struct MaterialEmission : EmissionLookup {
     const std::vector<PtMaterialRecord>& mats;
@@ -255,7 +255,7 @@ PtScene device_scene(Context& c, PathTracer& t) {
     S.textures = t.dTextures.as<DevTexture>();
     S.background.type = t.background.type;
     for (int k = 0; k < 3; ++k) S.background.color[k] = t.background.color[k];
-    S.background.scale = t.background.scale;
+    for (int k = 0; k < 3; ++k) S.background.scale[k] = t.background.scale[k];
     S.background.envmapTexture = t.background.envmapTexture;
     S.background.sat = t.dSat.as<float>();
     std::memcpy(S.sceneBox, sc.box, sizeof(S.sceneBox));
@@ -490,8 +490,8 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
             const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t->materials);
             t->pointLights = pl;
             build_light_tree(area, pl, sc.box, MaterialEmission(t->materials), t->tree);
-            const float black[3] = {0.0f, 0.0f, 0.0f};
-            build_background(MFH_BACKGROUND_COLORED, black, 1.0f, nullptr, 0, 0, sc.box, t->background);
+            const float black[3] = {0.0f, 0.0f, 0.0f}, one[3] = {1.0f, 1.0f, 1.0f};
+            build_background(MFH_BACKGROUND_COLORED, black, one, nullptr, 0, 0, sc.box, t->background);
             t->tree.backgroundFlux = t->background.fluxSum;
             t->dSat.reserve(16);
             upload(c, t->dMaterials, t->materials.data(), sizeof(PtMaterialRecord) * t->materials.size());

@@ -514,7 +514,7 @@ __device__ void background_sample_dir(const PtScene& S, float u0, float u1, V3& 
         const float phi = u1 * 2.0f * kPi;
         dir = v3(sinTheta * fsin(phi), sinTheta * fcos(phi), cosTheta);
         pdf = 1.f / (4.f * kPi);
-        radiance = v3(bg.color[0], bg.color[1], bg.color[2]) * bg.scale;
+        radiance = v3(bg.color[0], bg.color[1], bg.color[2]) * v3(bg.scale[0], bg.scale[1], bg.scale[2]);
         return;
     }
     const DevTexture& env = S.textures[bg.envmapTexture];
@@ -539,7 +539,7 @@ __device__ void background_sample_dir(const PtScene& S, float u0, float u1, V3& 
     dir = v3(sinTheta * fcos(phi), cosTheta, sinTheta * fsin(phi));
     pdf = texPdf / (2.f * kPi * kPi * sinTheta);
     const float4 t = texel(env, column, row);
-    radiance = v3(t.x, t.y, t.z) * bg.scale;
+    radiance = v3(t.x, t.y, t.z) * v3(bg.scale[0], bg.scale[1], bg.scale[2]);
 }
 
 // evaluate_background (light_sampling.hpp:467-515): radiance and pdf.back.
@@ -547,7 +547,7 @@ __device__ V3 evaluate_background(const PtScene& S, V3 direction, float& pdfBack
     const DevBackground& bg = S.background;
     if (bg.type == 0) {
         pdfBack = 1.0f / (4.0f * kPi);
-        return v3(bg.color[0], bg.color[1], bg.color[2]) * bg.scale;
+        return v3(bg.color[0], bg.color[1], bg.color[2]) * v3(bg.scale[0], bg.scale[1], bg.scale[2]);
     }
     const DevTexture& env = S.textures[bg.envmapTexture];
     const float v0 = facos(direction.y) / kPi;
@@ -564,7 +564,7 @@ __device__ V3 evaluate_background(const PtScene& S, V3 direction, float& pdfBack
     const float sinJac = fsqrt(1.0f - direction.y * direction.y);
     pdfScale *= sinPixel / (2.0f * kPi * kPi * sinJac);
     pdfBack = pdfScale / cdf;
-    return radiance * bg.scale;
+    return radiance * v3(bg.scale[0], bg.scale[1], bg.scale[2]);
 }
 
 // ------------------------------------------------------------------ lights

@@ -26,7 +26,7 @@ MATERIAL_DTYPE = np.dtype([("type", "<i4"), ("albedo", "<f4", 3), ("radiance", "
                            ("outerAbsorption", "<f4", 3), ("albedoTexture", "<i4"), ("roughnessTexture", "<i4"),
                            ("emissionTexture", "<i4"), ("reserved", "<i4", 7)])
 assert MATERIAL_DTYPE.itemsize == 128
-BACKGROUND_DTYPE = np.dtype([("type", "<i4"), ("color", "<f4", 3), ("scale", "<f4"), ("envmapTexture", "<i4")])
+BACKGROUND_DTYPE = np.dtype([("type", "<i4"), ("color", "<f4", 3), ("scale", "<f4", 3), ("envmapTexture", "<i4")])
 BACKGROUND_COLORED = 0
 BACKGROUND_ENVMAP = 1
 SAMPLING_NEAREST = 0

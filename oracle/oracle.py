@@ -237,7 +237,7 @@ class OrcTexture(C.Structure):
 
 
 class OrcBackground(C.Structure):
-    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("scale", C.c_float), ("envmapTexture", C.c_int32),
+    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("scale", C.c_float * 3), ("envmapTexture", C.c_int32),
                 ("sat", f32p), ("flux", C.c_float * 3)]
 
 
@@ -350,13 +350,16 @@ def _fill_background(s, background, textures, sat, flux):
         for k in range(3):
             s.background.color[k] = float(background["color"][k])
             s.background.flux[k] = float(flux[k])
-        s.background.scale = float(background["scale"])
+        sc = np.broadcast_to(np.asarray(background["scale"], np.float32), (3,))
+        for k in range(3):
+            s.background.scale[k] = float(sc[k])
         s.background.envmapTexture = int(background["envmapTexture"])
         if sat is not None:
             s.background.sat = _p(sat, f32p)
     else:
         s.background.type = 0
-        s.background.scale = 1.0
+        for k in range(3):
+            s.background.scale[k] = 1.0
 
 
 def background_sample(bscene, u0, u1):

@@ -150,7 +150,7 @@ struct OrcTexture {
 struct OrcBackground {
     int32_t type;  // 0 colored, 1 envmap
     float color[3];
-    float scale;
+    float scale[3];
     int32_t envmapTexture;
     const float* sat;  // width * height (envmap)
     float flux[3];
@@ -896,7 +896,7 @@ EnvDir background_sample_dir(const OrcPtScene& S, float u0, float u1) {
         const float phi = u1 * 2.0f * kPi;
         r.dir = {sinTheta * fsin(phi), sinTheta * fcos(phi), cosTheta};
         r.pdf = 1.f / (4.f * kPi);
-        r.radiance = V3{bg.color[0], bg.color[1], bg.color[2]} * bg.scale;
+        r.radiance = V3{bg.color[0], bg.color[1], bg.color[2]} * V3{bg.scale[0], bg.scale[1], bg.scale[2]};
         return r;
     }
     const OrcTexture& env = S.textures[bg.envmapTexture];
@@ -910,7 +910,7 @@ EnvDir background_sample_dir(const OrcPtScene& S, float u0, float u1) {
     r.dir = {sinTheta * cosPhi, cosTheta, sinTheta * sinPhi};
     r.pdf = s.pdf / (2.f * kPi * kPi * sinTheta);
     const F4 t = texel(env, s.tx, s.ty);
-    r.radiance = V3{t.x, t.y, t.z} * bg.scale;
+    r.radiance = V3{t.x, t.y, t.z} * V3{bg.scale[0], bg.scale[1], bg.scale[2]};
     return r;
 }
 
@@ -921,7 +921,9 @@ struct BgEval {
 // evaluate_background (light_sampling.hpp:467-515): COLORED and polar ENVMAP
 BgEval evaluate_background(const OrcPtScene& S, V3 direction) {
     const OrcBackground& bg = S.background;
-    if (bg.type == 0) return {V3{bg.color[0], bg.color[1], bg.color[2]} * bg.scale, 1.0f / (4.0f * kPi)};
+    if (bg.type == 0)
+        return {V3{bg.color[0], bg.color[1], bg.color[2]} * V3{bg.scale[0], bg.scale[1], bg.scale[2]},
+                1.0f / (4.0f * kPi)};
     const OrcTexture& env = S.textures[bg.envmapTexture];
     // sample_polar_map / direction_to_uv (textures/interface.hpp:180-200)
     const float v0 = facos(direction.y) / kPi;
@@ -938,7 +940,7 @@ BgEval evaluate_background(const OrcPtScene& S, V3 direction) {
     const float sinPixel = fsin(kPi * (float)((float)pixelY + 0.5f) / (float)H);
     const float sinJac = std::sqrt(1.0f - direction.y * direction.y);
     pdfScale *= sinPixel / (2.0f * kPi * kPi * sinJac);
-    radiance = radiance * bg.scale;
+    radiance = radiance * V3{bg.scale[0], bg.scale[1], bg.scale[2]};
     return {radiance, pdfScale / cdf};
 }
 

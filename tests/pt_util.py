@@ -114,7 +114,7 @@ def oracle_pt_scene_instanced(O, c, light_tree, scene_box):
 def background_of(c, kind):
     """'none' (black), 'colored' or 'envmap' background record for scene c."""
     if kind == "envmap":
-        return hip.background_record(hip.BACKGROUND_ENVMAP, scale=0.5, envmap_texture=c["envmap"])
+        return hip.background_record(hip.BACKGROUND_ENVMAP, scale=(0.5, 0.45, 0.6), envmap_texture=c["envmap"])
     if kind == "colored":
         return hip.background_record(hip.BACKGROUND_COLORED, color=(0.3, 0.4, 0.6), scale=0.8)
     return hip.background_record(hip.BACKGROUND_COLORED)

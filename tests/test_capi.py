@@ -17,7 +17,8 @@ def _declared():
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mfh_\w+|mufflon_\w+|render_\w+|renderer_\w+|"
-                             r"world_\w+|core_\w+|polygon_\w+|object_\w+|instance_\w+|scenario_\w+|scene_\w+)"
+                             r"world_\w+|core_\w+|polygon_\w+|object_\w+|instance_\w+|scenario_\w+|scene_\w+|"
+                             r"profiling_\w+)"
                              r"\s*\(", txt, flags=re.M):
             names.add(m.group(1))
     return sorted(names)

@@ -1,0 +1,180 @@
+"""Mufflon's core C API subset (include/mufflon_core.h) without a device:
+record layouts, world building through the loader calls, the renderer /
+target / parameter registry, and the error convention (false or a null
+handle plus core_get_dll_error, interface.cpp:54-83)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from mufflon_amd import core as K
+from mufflon_amd import scenes as S
+
+
+@pytest.fixture()
+def mi():
+    c = K.CoreInterface()
+    yield c
+    c.close()
+
+
+def test_record_layouts():
+    assert C.sizeof(K.MaterialParams) == 88
+    assert C.sizeof(K.LightHdl) == 4
+    assert C.sizeof(K.Mat3x4) == 48
+    assert K.MaterialParams.inner.offset == 56
+
+
+def test_world_building(mi, tmp_path):
+    c = S.cornell_materials(6, 12, 16, 16)
+    scen = K.build_world(mi, c, str(tmp_path), background=("envmap", (0.5, 0.45, 0.6)))
+    lib = mi.lib
+    obj = lib.world_get_object(mi.muffInst, b"scene")
+    assert obj
+    assert lib.world_get_material_count(mi.muffInst) == len(c["materials"])
+    assert lib.scenario_get_material_slot_count(scen) == len(c["materials"])
+    assert lib.world_get_instance_count(mi.muffInst, K.NO_ANIMATION_FRAME) == 1
+    assert mi.world_find_scenario("scene") == scen
+    assert lib.scenario_get_name(scen) == b"scene"
+    ok, msg = mi.world_finalize(c["verts"].min(0), c["verts"].max(0))
+    assert ok and msg == ""
+    # the camera is stored as Camera::set_view_dir leaves it
+    cam = mi.world_get_camera("camera")
+    d, u = K.Vec3(), K.Vec3()
+    assert lib.world_get_camera_direction(cam, C.byref(d), 0) and lib.world_get_camera_up(cam, C.byref(u), 0)
+    assert (d.x, d.y, d.z) == (0.0, 0.0, -1.0) and (u.x, u.y, u.z) == (0.0, 1.0, 0.0)
+    # a PFM texture keeps its size; value textures are shared by value
+    size = (C.c_int32 * 2)()
+    t = lib.world_add_texture(mi.muffInst, os.path.join(str(tmp_path), "scene_tex0.pfm").encode(), 1, 0, None, None)
+    assert t and lib.world_get_texture_size(t, C.byref(size)) and tuple(size) == (64, 64)
+    a = mi.world_add_texture_value([0.25, 0.5, 0.75])
+    assert a and a == mi.world_add_texture_value([0.25, 0.5, 0.75]) != mi.world_add_texture_value([0.25])
+
+
+def test_lod_calls(mi):
+    obj = mi.world_create_object("tri")
+    lod = mi.object_add_lod(obj)
+    lib = mi.lib
+    for p in ((0, 0, 0), (1, 0, 0), (0, 2, 0)):
+        assert lib.polygon_add_vertex(lod, K.Vec3(*p), K.Vec3(0, 0, 1), K.Vec2(0, 0)) >= 0
+    assert lib.polygon_add_triangle_material(lod, K.UVec3(0, 1, 2), 3) == 0
+    assert lib.polygon_add_triangle(lod, K.UVec3(0, 1, 7)) == -1  # vertex out of range
+    assert "out of range" in mi.core_get_dll_error()
+    assert lib.polygon_add_quad(lod, K.UVec4(0, 1, 2, 0)) == -1
+    assert "quads" in mi.core_get_dll_error()
+    assert lib.polygon_get_vertex_count(lod) == 3 and lib.polygon_get_triangle_count(lod) == 1
+    lo, hi = K.Vec3(), K.Vec3()
+    assert lib.polygon_get_bounding_box(lod, C.byref(lo), C.byref(hi))
+    assert (lo.x, lo.y, lo.z, hi.x, hi.y, hi.z) == (0, 0, 0, 1, 2, 0)
+    # bulk vertices from an array, then material indices in bulk
+    pts = np.arange(12, dtype=np.float32).reshape(4, 3)
+    uvs = np.zeros((4, 2), np.float32)
+    read = C.c_size_t()
+    first = lib.polygon_add_vertex_bulk(lod, 4, C.byref(K.bulk_array(pts)), None, C.byref(K.bulk_array(uvs)), None,
+                                        C.byref(read), None, None)
+    assert first == 3 and read.value == 4 and lib.polygon_get_vertex_count(lod) == 7
+    assert lib.polygon_add_triangle(lod, K.UVec3(3, 4, 5)) == 1
+    idx = np.array([5, 6], np.uint16)
+    assert lib.polygon_set_material_idx_bulk(lod, 0, 2, C.byref(K.bulk_array(idx))) == 2
+
+
+def test_instance_transforms(mi):
+    obj = mi.world_create_object("o")
+    inst = mi.world_create_instance(obj)
+    # instance -> world is stored inverted (world_container.cpp:240-245) and read back inverted again
+    th = np.radians(30.0)
+    M = np.array([[2 * np.cos(th), 0, 2 * np.sin(th), 1.0], [0, 2, 0, -2.0], [-2 * np.sin(th), 0, 2 * np.cos(th), 0.5]],
+                 np.float32)
+    assert mi.instance_set_transformation_matrix(inst, M, False)
+    np.testing.assert_allclose(mi.instance_get_transformation_matrix(inst).reshape(3, 4), M, rtol=1e-5, atol=1e-6)
+    assert mi.instance_set_transformation_matrix(inst, np.eye(4)[:3], True)
+    assert np.array_equal(mi.instance_get_transformation_matrix(inst).reshape(3, 4), np.eye(4)[:3])
+
+
+def test_finalize_sanity(mi):
+    ok, msg = mi.world_finalize((0, 0, 0), (1, 1, 1))
+    assert not ok and msg == "No instances"
+    obj = mi.world_create_object("o")
+    mi.world_create_instance(obj)
+    ok, msg = mi.world_finalize((0, 0, 0), (1, 1, 1))
+    assert not ok and msg == "No lights or emitters"
+    h = mi.world_add_light("p", K.LIGHT_POINT)
+    assert h.type == K.LIGHT_POINT and h.index == 0
+    ok, msg = mi.world_finalize((0, 0, 0), (1, 1, 1))
+    assert not ok and msg == "No camera"
+    mi.world_add_pinhole_camera("c", (0, 0, 0), (0, 0, 1), (0, 1, 0), 1e-4, 1e10, 0.7)
+    assert mi.world_finalize((0, 0, 0), (1, 1, 1)) == (True, "")
+    scen = mi.world_create_scenario("s")
+    assert mi.world_finalize_scenario(scen) == (False, "No camera")
+
+
+def test_unsupported_features_fail_cleanly(mi):
+    p = K.MaterialParams()
+    p.innerType = K.MATERIAL_WALTER
+    p.inner.walter.roughness = mi.world_add_texture_value([0.1])
+    assert not mi.world_add_material("w", p)
+    assert "not supported" in mi.core_get_dll_error()
+    p = K.MaterialParams()
+    p.innerType = 42
+    assert not mi.world_add_material("x", p)
+    h = mi.world_add_light("spot", K.LIGHT_SPOT)
+    assert h.type == 7
+    h = mi.world_add_background_light("sky", K.BACKGROUND_SKY_HOSEK)
+    assert h.type == 7 and "Hosek" in mi.core_get_dll_error()
+    assert not mi.world_add_texture("/nonexistent/file.pfm", 0)
+    assert not mi.world_add_texture("texture.png", 0)
+    assert "PFM" in mi.core_get_dll_error()
+    p = K.MaterialParams()
+    p.innerType = K.MATERIAL_TORRANCE
+    p.inner.torrance.albedo = mi.world_add_texture_value([0.5, 0.5, 0.5])
+    p.inner.torrance.roughness = mi.world_add_texture_value([0.2])
+    p.inner.torrance.ndf = K.NDF_COSINE
+    assert not mi.world_add_material("t", p)
+
+
+def test_renderer_registry_and_parameters(mi):
+    assert mi.render_get_renderer_count() == 1
+    assert mi.render_get_renderer_name(0) == "Pathtracer" and mi.render_get_renderer_short_name(0) == "PT"
+    assert mi.render_get_renderer_variations(0) == 2
+    assert [mi.render_get_renderer_devices(0, v) for v in range(2)] == [K.DEVICE_CUDA, K.DEVICE_CUDA]
+    # nothing is enabled yet
+    assert mi.render_get_render_target_count() == 0
+    assert not mi.renderer_set_parameter_int("Seed", 3)
+    ok, _ = mi.render_iterate()
+    assert not ok and "renderer" in mi.core_get_dll_error()
+    assert mi.render_enable_renderer(0, 0)
+    assert mi.renderer_parameters() == [("Seed", K.PARAM_INT), ("Min. path length", K.PARAM_INT),
+                                        ("Max. path length", K.PARAM_INT), ("NEE count", K.PARAM_INT),
+                                        ("NEE position guide", K.PARAM_BOOL)]
+    assert mi.renderer_get_parameter_int("Max. path length") == 16 and mi.renderer_get_parameter_int("NEE count") == 1
+    assert mi.renderer_set_parameter_int("Max. path length", 5) and mi.renderer_get_parameter_int("Max. path length") == 5
+    assert mi.renderer_set_parameter_bool("NEE position guide", True) and mi.renderer_get_parameter_bool(
+        "NEE position guide")
+    # unknown names are ignored (logged), wrong types leave the value alone
+    assert mi.renderer_set_parameter_int("No such parameter", 1)
+    assert mi.renderer_set_parameter_bool("Seed", True) and mi.renderer_get_parameter_int("Seed") == 0
+    # parameters belong to the variation
+    assert mi.render_enable_renderer(0, 1) and mi.renderer_get_parameter_int("Max. path length") == 16
+    assert mi.render_enable_renderer(0, 0) and mi.renderer_get_parameter_int("Max. path length") == 5
+
+
+def test_render_targets(mi):
+    assert mi.render_enable_renderer(0, 0)
+    names = [mi.render_get_render_target_name(i) for i in range(mi.render_get_render_target_count())]
+    assert names == ["Radiance", "Position", "Depth", "Normal", "UV", "Albedo", "Lightness", "Hit ID"]
+    assert not mi.render_is_render_target_enabled("Radiance")
+    assert mi.render_enable_render_target("Radiance", True)
+    assert mi.render_is_render_target_enabled("Radiance", True) and not mi.render_is_render_target_enabled("Depth")
+    assert mi.render_disable_render_target("Radiance", True)
+    assert not mi.render_is_render_target_enabled("Radiance", True)
+    assert not mi.render_is_render_target_enabled("Nope")
+    assert mi.mufflon_get_target_image("Radiance", False, 1, 1) is None  # no scene yet
+
+
+@pytest.mark.skipif(K.load().mufflon_is_cuda_available(), reason="checks the no-device error path")
+def test_load_without_device_fails_cleanly(mi, tmp_path):
+    c = S.cornell_materials(6, 12, 8, 8)
+    scen = K.build_world(mi, c, str(tmp_path))
+    assert not mi.world_load_scenario(scen)
+    assert "device" in mi.core_get_dll_error()

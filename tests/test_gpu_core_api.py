@@ -1,0 +1,101 @@
+"""The core C API (include/mufflon_core.h) end to end on the device: a scene
+built through the loader calls, rendered with render_iterate and read with
+mufflon_get_target_image, against the oracle on the same inputs; both
+renderer variations; variance targets; PFM screenshots."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import pt_util as P
+from mufflon_amd import core as K
+from mufflon_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-4
+ATOL = 1e-6
+
+
+def _tanf(x):
+    libm = C.CDLL("libm.so.6")
+    libm.tanf.restype = C.c_float
+    libm.tanf.argtypes = [C.c_float]
+    return np.float32(libm.tanf(np.float32(x)))
+
+
+def _oracle_scene(oracle, mi, c, bg):
+    """Oracle scene with the camera as the core stores it (set_view_dir, and
+    tanVFov = tanf(vFov / 2) of the float vFov it was given)."""
+    cam = mi.world_get_camera("camera")
+    d, u = K.Vec3(), K.Vec3()
+    mi.lib.world_get_camera_direction(cam, C.byref(d), 0)
+    mi.lib.world_get_camera_up(cam, C.byref(u), 0)
+    vfov = np.float32(2.0 * np.arctan(float(c["camera"]["tan_vfov"])))
+    c = dict(c)
+    c["camera"] = dict(c["camera"], view_dir=np.array([d.x, d.y, d.z], np.float32),
+                       up=np.array([u.x, u.y, u.z], np.float32), tan_vfov=_tanf(vfov / np.float32(2)))
+    return P.oracle_pt_scene_materials(oracle, c, bg)
+
+
+def _compare(g, o, what):
+    bad = np.abs(g - o) > RTOL * np.abs(o) + ATOL
+    print("%s: bit-exact=%s, out of tolerance=%d" % (what, np.array_equal(g.view(np.uint32), o.view(np.uint32)),
+                                                      int(bad.sum())), flush=True)
+    assert not bad.any(), what
+
+
+def test_core_api_render_matches_oracle(oracle, tmp_path):
+    print("core api: material zoo through the loader calls", flush=True)
+    c = S.cornell_materials(12, 24, 40, 32)
+    mi = K.CoreInterface()
+    try:
+        scen = K.build_world(mi, c, str(tmp_path), background=("envmap", (0.5, 0.45, 0.6)))
+        assert mi.render_enable_renderer(0, 0)
+        for name, v in (("Seed", 1), ("Max. path length", 8), ("NEE count", 1)):
+            assert mi.renderer_set_parameter_int(name, v)
+        assert mi.world_load_scenario(scen), mi.core_get_dll_error()
+        for t in ("Radiance", "Hit ID", "Depth"):
+            mi.render_enable_render_target(t, t == "Radiance")
+        for _ in range(4):
+            ok, t = mi.render_iterate()
+            assert ok, mi.core_get_dll_error()
+        assert mi.render_get_current_iteration() == 4
+        W, H = 40, 32
+        rad = mi.mufflon_get_target_image("Radiance", False, W, H)
+        var = mi.mufflon_get_target_image("Radiance", True, W, H)
+        depth = mi.mufflon_get_target_image("Depth", False, W, H)
+        with pytest.raises(RuntimeError):
+            mi.mufflon_get_target_image("Depth", True, W, H)
+        ps = _oracle_scene(oracle, mi, c, P.background_of(c, "envmap"))
+        mean, ovar, _ = oracle.pt_render_targets(ps, 1, 0, 8, 1, 4, [0, 1, 3], variance=[0])
+        _compare(rad, mean[0], "core radiance")
+        _compare(var, ovar[0], "core radiance variance")
+        _compare(depth, mean[3], "core depth")
+        # the wavefront variation renders the same image (a renderer switch
+        # starts a fresh output: targets are enabled again)
+        assert mi.render_enable_renderer(0, 1)
+        for name, v in (("Seed", 1), ("Max. path length", 8), ("NEE count", 1)):
+            assert mi.renderer_set_parameter_int(name, v)
+        mi.render_enable_render_target("Radiance", False)
+        for _ in range(4):
+            assert mi.render_iterate()[0], mi.core_get_dll_error()
+        wf = mi.mufflon_get_target_image("Radiance", False, W, H)
+        assert np.array_equal(wf, rad)
+        # a screenshot with tag replacement (interface.cpp:168-195)
+        assert mi.render_save_screenshot(str(tmp_path / "#scenario-#shortrenderer-#iteration-#target"), "Radiance",
+                                         False), mi.core_get_dll_error()
+        path = tmp_path / "scene-PT-4-Radiance.pfm"
+        raw = path.read_bytes()
+        head = b"PF\n40 32\n-1.000000\n"
+        assert raw.startswith(head)
+        img = np.frombuffer(raw[len(head):], "<f4").reshape(H, W, 3)
+        assert np.array_equal(img, wf)
+        # a parameter change restarts the render (on_renderer_parameter_changed)
+        assert mi.renderer_set_parameter_int("Seed", 2)
+        assert mi.render_get_current_iteration() == 0
+        assert mi.render_iterate()[0]
+        assert mi.render_get_current_iteration() == 1
+    finally:
+        mi.close()

@@ -196,7 +196,7 @@ def test_envmap_pdf_consistency(oracle):
 
 def test_colored_background_closed_form(oracle):
     box = np.array([-1, 0, -2, 3, 2, 1], np.float32)
-    bg = hip.background_record(hip.BACKGROUND_COLORED, color=(0.5, 1.0, 2.0), scale=3.0)
+    bg = hip.background_record(hip.BACKGROUND_COLORED, color=(0.5, 1.0, 2.0), scale=(3.0, 2.0, 0.5))
     flux, sat = hip.background_build(bg, None, box)
     dx, dy, dz = 4.0, 2.0, 3.0
     expect = 4 * np.pi * 2 * np.pi * (dx * dy + dx * dz + dy * dz) * np.array([0.5, 1.0, 2.0])
@@ -204,7 +204,34 @@ def test_colored_background_closed_form(oracle):
     tx = oracle.Textures()
     bs = oracle.background_scene(bg[0], tx, box, flux=flux)
     d, pdf, rad = oracle.background_sample(bs, 0.3, 0.8)
-    assert pdf == pytest.approx(1 / (4 * np.pi)) and np.allclose(rad, [1.5, 3.0, 6.0])
+    assert pdf == pytest.approx(1 / (4 * np.pi)) and np.allclose(rad, [1.5, 2.0, 1.0])
     assert abs(np.linalg.norm(d) - 1) < 1e-6
     rad_e, pdf_e = oracle.background_evaluate(bs, d)
     assert np.allclose(rad_e, rad) and pdf_e == pytest.approx(1 / (4 * np.pi))
+
+
+def test_envmap_scale_per_channel(oracle):
+    """The background scale is a spectrum (Background::set_scale,
+    background.hpp:119-123): radiance scales per channel, and the env-map flux
+    is scaled twice (once per texel in compute_envmap_flux, once more in
+    acquire_const, background.cpp:46-54,78-122), so flux(s) = flux(1) * s^2."""
+    env = _env()
+    box = np.array([-1, 0, -1, 1, 2, 1], np.float32)
+    s = np.array([1.5, 0.5, 2.0], np.float32)
+    bg1 = hip.background_record(hip.BACKGROUND_ENVMAP, scale=1.0, envmap_texture=0)
+    bgs = hip.background_record(hip.BACKGROUND_ENVMAP, scale=s, envmap_texture=0)
+    f1, sat1 = hip.background_build(bg1, env, box)
+    fs, sats = hip.background_build(bgs, env, box)
+    np.testing.assert_allclose(fs, np.asarray(f1, np.float64) * s.astype(np.float64) ** 2, rtol=2e-6)
+    assert np.array_equal(sat1, sats)  # the sampling table ignores the scale
+    tx = oracle.Textures([(env, hip.SAMPLING_NEAREST)])
+    b1 = oracle.background_scene(bg1[0], tx, box, sat=sat1, flux=f1)
+    bs = oracle.background_scene(bgs[0], tx, box, sat=sats, flux=fs)
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        r1, p1 = oracle.background_evaluate(b1, d)
+        rs, ps = oracle.background_evaluate(bs, d)
+        np.testing.assert_allclose(rs, r1 * s, rtol=1e-6)
+        assert ps == p1
