@@ -424,6 +424,22 @@ class PtScene:
         self.s = s
 
 
+def scramble_seed(x):
+    lib = load()
+    lib.orc_scramble_seed.argtypes = [C.c_uint32]
+    lib.orc_scramble_seed.restype = C.c_uint64
+    return int(lib.orc_scramble_seed(x))
+
+
+def pcg_sequence(seed, n):
+    """First n outputs of the oracle's PCG64(seed)."""
+    lib = load()
+    lib.orc_pcg_sequence.argtypes = [C.c_uint32, C.c_int32, u64p]
+    out = np.zeros(n, np.uint64)
+    lib.orc_pcg_sequence(seed, n, _p(out, u64p))
+    return [int(x) for x in out]
+
+
 def _light_lib():
     lib = load()
     if not hasattr(lib, "_lc_ready"):

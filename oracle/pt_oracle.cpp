@@ -1740,6 +1740,14 @@ int orc_background_sample(const OrcPtScene* S, float u0, float u1, float* out7) 
     out7[6] = d.radiance.z;
     return 0;
 }
+// RNG hooks: scramble_seed (SplitMix64 finalizer) and the first n outputs of
+// PCG64(seed) (rng.hpp:12-17, 70-93).
+uint64_t orc_scramble_seed(uint32_t x) { return Pcg64::scramble(x); }
+int orc_pcg_sequence(uint32_t seed, int32_t n, uint64_t* out) {
+    Pcg64 r(seed);
+    for (int32_t i = 0; i < n; ++i) out[i] = r.next();
+    return 0;
+}
 // Light tree sampling hooks: one NEE connection from pos (light position
 // xyz, creation pdf, diffIrradiance xyz, byte offset of the chosen
 // positional light or -1 for the background) and the connect pdf of a hit
