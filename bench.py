@@ -46,6 +46,10 @@ def parse():
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
     ap.add_argument("--pt-iterations", type=int, default=64,
                     help="Cornell-box path-tracer iterations measured next to the main workload (0: skip)")
+    ap.add_argument("--terrain-iterations", type=int, default=16,
+                    help="iterations of the terrain megakernel / wavefront comparison (0 skips it)")
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint measurement")
+    ap.add_argument("--batch-scenes", type=int, default=48, help="scenes of the uneven batch (0 skips it)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (torch), whose FETCH_SIZE / "
                          "WRITE_SIZE counters scale the traversal kernel's to HBM bytes")
@@ -98,10 +102,21 @@ def pt_cornell(args, cpu_baseline_too):
     ctx.timer_begin()
     ctx.pt_iterate(pt, n)
     ms = ctx.timer_end()
+    mega = ctx.pt_target(pt)
     pixels = 256 * 256
     out = {"workload": "Cornell box 256x256, 65,060 triangles, depth 8, NEE 1, seed 1, megakernel",
            "iterations": n, "ms_per_iteration": round(ms / n, 4),
            "mpaths_s": round(pixels * n / (ms / 1e3) / 1e6, 3)}
+    from mufflon_amd import hip
+    ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
+    ctx.pt_reset(pt)
+    ctx.pt_iterate(pt, 2)
+    ctx.pt_reset(pt)
+    ctx.timer_begin()
+    ctx.pt_iterate(pt, n)
+    wms = ctx.timer_end()
+    out["wavefront"] = {"ms_per_iteration": round(wms / n, 4), "mpaths_s": round(pixels * n / (wms / 1e3) / 1e6, 3),
+                        "identical_to_megakernel": bool(np.array_equal(ctx.pt_target(pt), mega))}
     if cpu_baseline_too:
         from oracle import oracle as O
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -120,6 +135,145 @@ def pt_cornell(args, cpu_baseline_too):
                                          % (k, os.cpu_count() or 1, dt)}
     ctx.close()
     return out
+
+
+def pt_terrain(args):
+    """Plain megakernel vs wavefront (HIP graph, autotuned) on the procedural
+    terrain: 9,990,450-triangle fBm heightfield + 4096 instances of a
+    2000-triangle rock, 60/30/10 Lambert / GGX / dielectric, 1024 point
+    lights, 2048x1024 environment map; 1920x1080, depth 16, 16 iterations.
+    Also the LBVH build time of the 10M-triangle heightfield."""
+    from mufflon_amd import hip, scenes as S, workloads as W
+    from mufflon_amd.hip import HipContext
+    t0 = time.perf_counter()
+    sc = S.terrain_scene()
+    gen_s = time.perf_counter() - t0
+    ctx = HipContext(0)
+    L = sc["lods"][0]
+    lod = ctx.lod_create(L["verts"], L["tris"])
+    ctx.lbvh_build(lod)
+    builds = []
+    for _ in range(5):
+        ctx.timer_begin()
+        ctx.lbvh_build(lod)
+        builds.append(ctx.timer_end())
+    pt, _, _ = W.path_tracer(ctx, sc, max_path_length=16)
+    its = args.terrain_iterations
+    pixels = sc["camera"]["width"] * sc["camera"]["height"]
+
+    def run(mode, graph):
+        ctx.pt_set_mode(pt, mode, graph)
+        ctx.pt_reset(pt)
+        ctx.pt_iterate(pt, 1)
+        ctx.pt_reset(pt)
+        ctx.timer_begin()
+        ctx.pt_iterate(pt, its)
+        ms = ctx.timer_end() / its
+        return ms, ctx.pt_target(pt)
+
+    mk_ms, mk_img = run(hip.PT_MEGAKERNEL, False)
+    ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
+    tuned, _ = ctx.pt_autotune(pt, 1)
+    wf_ms, wf_img = run(hip.PT_WAVEFRONT, True)
+    ctx.close()
+    return {"workload": "terrain 2236^2 fBm heightfield (9,990,450 tris) + 4096 rock instances (2000 tris), "
+                        "1024 point lights, 2048x1024 env map, 1920x1080, depth 16, NEE 1, seed 1",
+            "scene_generation_s": round(gen_s, 2),
+            "lbvh_10m_build_ms": round(float(np.mean(builds)), 3), "lbvh_10m_build_ms_min": round(min(builds), 3),
+            "iterations": its,
+            "megakernel": {"ms_per_iteration": round(mk_ms, 3), "mpaths_s": round(pixels / mk_ms / 1e3, 3)},
+            "wavefront": {"ms_per_iteration": round(wf_ms, 3), "mpaths_s": round(pixels / wf_ms / 1e3, 3),
+                          "tuning": tuned, "graph": True},
+            "wavefront_over_megakernel": round(mk_ms / wf_ms, 3),
+            "identical_images": bool(np.array_equal(mk_img, wf_img))}
+
+
+def pt_adjoint(args):
+    """Differentiable PT: the Cornell box with a 512x512 RGB albedo texture on
+    the back wall and a 256x256 roughness texture on the GGX sphere, 256x256
+    pixels, L2 loss to a target rendered from textures of seeds 11/12; primal
+    4 spp + adjoint path replay of the same 4 paths per pixel (albedo,
+    roughness and emission gradients)."""
+    from mufflon_amd import scenes as S
+    from mufflon_amd.hip import HipContext
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    c = S.cornell_textured(128, 256, 256, 512, 256)
+    ctx = HipContext(0)
+    ids = np.array([ctx.texture_create(t, s) for t, s in c["textures"]], np.int32)
+    mats = c["materials"].copy()
+    used = mats["albedoTexture"] >= 0
+    mats["albedoTexture"][used] = ids[mats["albedoTexture"][used]]
+    lod = ctx.lod_create(c["verts"], c["tris"])
+    ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
+    ctx.lbvh_build(lod)
+    sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+    pt = ctx.pt_create(sc, mats, c["camera"], dict(seed=1, max_path_length=8, nee_count=1))
+    target_tex = [S.procedural_texture(512, 512, 3, 0.15, 0.85, seed=11),
+                  S.procedural_texture(256, 256, 1, 0.05, 0.8, seed=12)]
+    for i, t in zip(ids, target_tex):
+        ctx.texture_update(int(i), t)
+    ctx.pt_iterate(pt, 4)
+    target = ctx.pt_target(pt).astype(np.float64)
+    for i, (t, _) in zip(ids, c["textures"]):
+        ctx.texture_update(int(i), t)
+    times = []
+    for rep in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.pt_reset(pt)
+        ctx.pt_iterate(pt, 4)
+        img = ctx.pt_target(pt).astype(np.float64)
+        ctx.pt_adjoint(pt, 4, (2.0 * (img - target)).astype(np.float32))
+        g = ctx.pt_texture_gradient(pt, int(ids[0]), (512, 512, 3))
+        gr = ctx.pt_texture_gradient(pt, int(ids[1]), (256, 256, 1))
+        times.append(time.perf_counter() - t0)
+    ctx.timer_begin()
+    ctx.pt_adjoint(pt, 4, (2.0 * (img - target)).astype(np.float32))
+    adj_ms = ctx.timer_end()
+    ctx.close()
+    return {"workload": "Cornell 256x256, albedo texture 512^2 (back wall) + roughness texture 256^2 (GGX sphere), "
+                        "4 spp, L2 loss",
+            "s_per_iteration": round(float(np.median(times)), 5),
+            "adjoint_kernel_ms": round(adj_ms, 3),
+            "loss": float(np.sum((img - target) ** 2)), "albedo_grad_l1": float(np.abs(g).sum()),
+            "roughness_grad_l1": float(np.abs(gr).sum())}
+
+
+def pt_batch(args):
+    """48 procedural scenes (seeds 100-147; 10k-2M triangles, 128^2-1024^2
+    pixels, depth 4-16) in one wavefront launch per stage, HIP-graph replayed
+    and autotuned, against the megakernel running them one after another."""
+    from mufflon_amd import workloads as W
+    from mufflon_amd.hip import HipContext
+    t0 = time.perf_counter()
+    ctx = HipContext(0)
+    b, pts, texs, scenes = W.uneven_batch(ctx, count=args.batch_scenes)
+    setup_s = time.perf_counter() - t0
+    paths = sum(sc["camera"]["width"] * sc["camera"]["height"] for sc in scenes)
+    tuned, _ = ctx.batch_autotune(b, 1)
+    ctx.batch_iterate(b, 1)
+    ctx.batch_reset(b)
+    ctx.timer_begin()
+    ctx.batch_iterate(b, 4)
+    bms = ctx.timer_end() / 4
+    ctx.timer_begin()
+    for pt in pts:
+        ctx.pt_reset(pt)
+        ctx.pt_iterate(pt, 4)
+    mms = ctx.timer_end() / 4
+    t0 = time.perf_counter()
+    for pt, tex in list(zip(pts, texs))[:8]:
+        ctx.pt_adjoint(pt, 4, (2.0 * ctx.pt_target(pt)).astype(np.float32))
+        ctx.pt_texture_gradient(pt, int(tex[1]), (64, 64, 3))
+    adj_s = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "%d scenes, %d triangles, %d paths per iteration" %
+                        (len(scenes), sum(len(sc["lods"][0]["tris"]) for sc in scenes), paths),
+            "setup_s": round(setup_s, 1), "tuning": tuned,
+            "batch_ms_per_iteration": round(bms, 3), "batch_mpaths_s": round(paths / bms / 1e3, 3),
+            "megakernel_sequential_ms_per_iteration": round(mms, 3),
+            "megakernel_sequential_mpaths_s": round(paths / mms / 1e3, 3),
+            "albedo_gradients_8_scenes_s": round(adj_s, 3)}
 
 
 def main():
@@ -233,6 +387,15 @@ def main():
     if args.pt_iterations > 0 and not args.profile_only:
         print("path tracer ...", file=sys.stderr, flush=True)
         out["pt_cornell"] = pt_cornell(args, not args.no_cpu_baseline)
+    if args.terrain_iterations > 0 and not args.profile_only:
+        print("terrain: megakernel vs wavefront ...", file=sys.stderr, flush=True)
+        out["pt_terrain"] = pt_terrain(args)
+    if args.adjoint and not args.profile_only:
+        print("adjoint ...", file=sys.stderr, flush=True)
+        out["pt_adjoint"] = pt_adjoint(args)
+    if args.batch_scenes > 0 and not args.profile_only:
+        print("uneven batch ...", file=sys.stderr, flush=True)
+        out["pt_batch"] = pt_batch(args)
     print(json.dumps(out))
 
 

@@ -348,13 +348,16 @@ int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* 
  * reference (it has no differentiable renderer). For the image of
  * `iterations` iterations from a reset, and radianceAdjoint = dLoss/dRadiance
  * (host, 3 floats per pixel, row-major), replays the same paths and
- * accumulates dLoss/d(texel) of every albedo texture (LAMBERT, TORRANCE) and
- * dLoss/d(emission scale) of every material: the derivative of the
- * estimator with its random numbers fixed (Russian roulette included;
- * light-selection probabilities held fixed, exact for one emissive material).
- * Maximum path length 32. Asynchronous; read with the getters below. */
+ * accumulates dLoss/d(texel) of every albedo texture (LAMBERT, TORRANCE),
+ * every roughness texture (TORRANCE, MICROFACET; one channel drives both
+ * roughnesses, two channels x and y) and dLoss/d(emission scale) of every
+ * material: the derivative of the estimator with its random numbers fixed
+ * (Russian roulette included; light-selection probabilities held fixed,
+ * exact for one emissive material). Roughness is differentiated with the
+ * sampled BSDF directions held fixed (detached sampling). Maximum path
+ * length 32. Asynchronous; read with the getters below. */
 int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const float* radianceAdjoint);
-/* Gradient of an albedo texture: width * height * channels floats. */
+/* Gradient of an albedo or roughness texture: width * height * channels floats. */
 int mfh_pt_get_texture_gradient(mfh_context* ctx, int32_t ptId, int32_t textureId, float* out);
 /* Gradient with respect to a material's emission scale: 3 floats. */
 int mfh_pt_get_emission_gradient(mfh_context* ctx, int32_t ptId, int32_t materialIndex, float* out);

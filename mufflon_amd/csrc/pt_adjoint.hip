@@ -1,6 +1,7 @@
 // Adjoint (path replay) of the path tracer for gfx950: gradients of a scalar
 // loss of the Radiance target with respect to albedo textures (LAMBERT and
-// TORRANCE albedo) and the emission scale of every emissive material.
+// TORRANCE albedo), roughness textures (TORRANCE, MICROFACET) and the
+// emission scale of every emissive material.
 //
 // The primal image of `iterations` iterations from a reset is
 // I = mean_s L_s over one path per pixel and iteration. Given
@@ -18,6 +19,14 @@
 //     throughput; with D_m = dln(cont_m)/dalbedo_j[c'] (zero when cont_m = 1)
 //       D_m = T'_m[a*] / cont_m * ([a* = c'] / albedo_j[c'] - sum_{j<=l<m} D_l)
 //     every contribution after walk m picks up -D_m.
+// Roughness enters with the BSDF sample's direction held fixed (detached
+// sampling): the walk weight value * cos / pdf, the pdf the next vertex's
+// emission MIS uses, and each next-event estimate's BSDF value and MIS
+// weight are differentiated at the recorded directions (dln_mat_evaluate),
+// and the same continuation terms D_m follow with d ln(walk weight) in place
+// of 1 / albedo. This is the derivative of the estimator whose random
+// decisions and sampled directions are fixed; its expectation is the
+// derivative of the expected image.
 // Texel gradients follow the texture's own sampling weights (bilinear or
 // nearest, 1x1 constant) and are added with float atomics. Emission-scale
 // gradients hold the light-tree selection probabilities fixed, which is the
@@ -36,14 +45,19 @@ constexpr int kAdjThreads = 128;
 
 struct AdjVertex {
     int tex;        // albedo texture of the vertex's material, -1 when it has no albedo parameter
+    int rtex;       // roughness texture, -1 when the roughness is not textured
     float hu, hv;   // uv of the hit
     V3 rho;         // fetched albedo
     V3 neeW;        // delta-weighted next-event contributions at this vertex
     V3 emW;         // delta-weighted emission contribution of this vertex
+    float misC;     // 1 - MIS weight of that emission (0 without MIS)
     float cont;     // continuation probability of the walk from this vertex
     float tp;       // throughput before the roulette in its largest channel
     int astar;      // that channel
     V3 G;           // weighted contributions after the walk from this vertex
+    float rw[2];    // d ln(walk weight) / d(rx, ry) at the sampled direction
+    float rp[2];    // d ln(pdf of the sampled direction) / d(rx, ry)
+    float rn[2];    // sum over next-event estimates: weighted contribution * d ln / d(rx, ry)
 };
 
 __device__ __forceinline__ void add_emission_grad(const PtScene& S, const AdjointState& A, int mat, V3 weighted) {
@@ -104,8 +118,9 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
         pcg_next(rng);
         int pathLen = 0;
         rec[0].neeW = rec[0].emW = v3(0, 0, 0);
-        rec[0].tex = -1;
+        rec[0].tex = rec[0].rtex = -1;
         rec[0].cont = 1.0f;
+        rec[0].misC = 0.0f;
         do {
             if (pathLen > 0 && pathLen + 1 >= S.minPathLength && pathLen + 1 <= S.maxPathLength) {
                 const u64 neeSeed = pcg_next(rng);
@@ -116,9 +131,16 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                         const TraceHit sh = trace_ray<true, false, kSingle, kAdjThreads>(S.trace, c.a, c.dir, c.tmax,
                                                                                          lstack);
                         if (sh.inst == -1) {
+                            AdjVertex& r = rec[pathLen];
                             const V3 wgt = delta * c.radiance;
-                            rec[pathLen].neeW = rec[pathLen].neeW + wgt;
+                            r.neeW = r.neeW + wgt;
                             if (c.lightMaterial >= 0) add_emission_grad(S, A, c.lightMaterial, wgt);
+                            float dv[2], dp[2];
+                            if (r.rtex >= 0 && dln_mat_evaluate(vx.ts, vx.mat, vx.incident(), c.lightDir, dv, dp)) {
+                                const float w = wgt.x + wgt.y + wgt.z;
+                                r.rn[0] += w * (dv[0] - c.misFactor * dp[0]);
+                                r.rn[1] += w * (dv[1] - c.misFactor * dp[1]);
+                            }
                         }
                     }
                 }
@@ -137,6 +159,14 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                     walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, dir, pdfF, absorption, tPrime, cont);
                 AdjVertex& r = rec[pathLen];
                 r.cont = cont;
+                if (ok && r.rtex >= 0) {
+                    float dv[2], dp[2];
+                    dln_mat_evaluate(vx.ts, vx.mat, vx.incident(), dir, dv, dp);
+                    r.rw[0] = dv[0] - dp[0];
+                    r.rw[1] = dv[1] - dp[1];
+                    r.rp[0] = pdfF < 1e9f ? dp[0] : 0.0f;
+                    r.rp[1] = pdfF < 1e9f ? dp[1] : 0.0f;
+                }
                 r.astar = (tPrime.x < tPrime.y) ? ((tPrime.y < tPrime.z) ? 2 : 1) : ((tPrime.x < tPrime.z) ? 2 : 0);
                 r.tp = r.astar == 0 ? tPrime.x : (r.astar == 1 ? tPrime.y : tPrime.z);
                 if (!ok) break;
@@ -151,8 +181,10 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
             ++pathLen;
             AdjVertex& r = rec[pathLen];
             r.neeW = r.emW = v3(0, 0, 0);
-            r.tex = -1;
+            r.tex = r.rtex = -1;
             r.cont = 1.0f;
+            r.misC = 0.0f;
+            r.rw[0] = r.rw[1] = r.rp[0] = r.rp[1] = r.rn[0] = r.rn[1] = 0.0f;
             r.hu = vx.hu;
             r.hv = vx.hv;
             r.rho = vx.mat.albedo;
@@ -160,11 +192,15 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
             if (!vx.isVoid) {
                 matIdx = material_index(S, vx.inst, vx.prim);
                 if (vx.mat.type == kMatLambert || vx.mat.type == kMatTorrance) r.tex = S.materials[matIdx].albedoTexture;
+                if (vx.mat.type == kMatTorrance || vx.mat.type == kMatMicrofacet)
+                    r.rtex = S.materials[matIdx].roughnessTexture;
             }
             if (pathLen >= S.minPathLength) {
                 V3 em;
-                const V3 cc = emission_contribution(S, vx, pathLen, throughput, em, prevPos);
+                float mis;
+                const V3 cc = emission_contribution(S, vx, pathLen, throughput, em, prevPos, mis);
                 r.emW = delta * cc;
+                r.misC = 1.0f - mis;
                 if (matIdx >= 0 && vx.mat.type == kMatEmissive) add_emission_grad(S, A, matIdx, r.emW);
             }
             if (vx.isVoid) break;
@@ -174,6 +210,33 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
         rec[last].G = v3(0, 0, 0);
         for (int v = last - 1; v >= 0; --v) rec[v].G = rec[v + 1].G + rec[v + 1].emW + rec[v + 1].neeW;
         for (int j = 1; j <= last; ++j) {
+            if (rec[j].rtex >= 0 && A.texGrad[rec[j].rtex]) {
+                // dL / d(rx, ry) of vertex j (see the header)
+                const float Gj = rec[j].G.x + rec[j].G.y + rec[j].G.z;
+                const float em1 = j < last ? (rec[j + 1].emW.x + rec[j + 1].emW.y + rec[j + 1].emW.z) * rec[j + 1].misC
+                                           : 0.0f;
+                float g[2];
+                for (int q = 0; q < 2; ++q) {
+                    g[q] = rec[j].rw[q] * Gj + rec[j].rn[q] + em1 * rec[j].rp[q];
+                    float sumD = 0.0f;
+                    for (int m = j; m < last; ++m) {
+                        if (!(rec[m].cont < 1.0f)) continue;
+                        const float D = rec[m].tp / rec[m].cont * (rec[j].rw[q] - sumD);
+                        g[q] -= D * (rec[m].G.x + rec[m].G.y + rec[m].G.z);
+                        sumD += D;
+                    }
+                }
+                const DevTexture& tx = S.textures[rec[j].rtex];
+                float* grad = A.texGrad[rec[j].rtex];
+                int idx[4];
+                float w[4];
+                const int nt = texel_weights(tx, rec[j].hu, rec[j].hv, idx, w);
+                // one channel drives both roughnesses; otherwise x and y
+                const float gc[2] = {tx.channels > 1 ? g[0] : g[0] + g[1], g[1]};
+                for (int ch = 0; ch < (tx.channels > 1 ? 2 : 1); ++ch)
+                    if (gc[ch] != 0.0f && gc[ch] == gc[ch])
+                        for (int q = 0; q < nt; ++q) atomicAdd(grad + 4 * (size_t)idx[q] + ch, w[q] * gc[ch]);
+            }
             if (rec[j].tex < 0) continue;
             float* grad = A.texGrad[rec[j].tex];
             if (!grad) continue;

@@ -592,16 +592,19 @@ int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const flo
         const size_t nm = t.materials.size();
         t.adjEmGrad.reserve(12 * nm);
         MFH_HIP(hipMemsetAsync(t.adjEmGrad.ptr, 0, 12 * nm, c.stream));
-        // gradient buffers for every texture used as an albedo
+        // gradient buffers for every texture used as an albedo or a roughness
         t.adjTexGrad.resize(c.textures.size());
         std::vector<float*> ptrs(std::max<size_t>(c.textures.size(), 1), nullptr);
+        auto want = [&](int32_t id) {
+            if (id < 0 || ptrs[id]) return;
+            const Texture& x = *c.textures[id];
+            DevBuf& g = t.adjTexGrad[id];
+            g.reserve(16 * (size_t)x.width * x.height);
+            ptrs[id] = g.as<float>();
+        };
         for (const PtMaterialRecord& m : t.materials) {
-            if ((m.type == kMatLambert || m.type == kMatTorrance) && m.albedoTexture >= 0) {
-                const Texture& x = *c.textures[m.albedoTexture];
-                DevBuf& g = t.adjTexGrad[m.albedoTexture];
-                g.reserve(16 * (size_t)x.width * x.height);
-                ptrs[m.albedoTexture] = g.as<float>();
-            }
+            if (m.type == kMatLambert || m.type == kMatTorrance) want(m.albedoTexture);
+            if (m.type == kMatTorrance || m.type == kMatMicrofacet) want(m.roughnessTexture);
         }
         for (size_t k = 0; k < c.textures.size(); ++k)
             if (ptrs[k]) MFH_HIP(hipMemsetAsync(ptrs[k], 0, 16 * (size_t)c.textures[k]->width * c.textures[k]->height,
@@ -625,7 +628,7 @@ int mfh_pt_get_texture_gradient(mfh_context* ctx, int32_t ptId, int32_t textureI
         MFH_REQUIRE(out != nullptr, "null output");
         MFH_REQUIRE(t.adjValid, "no adjoint has been run");
         MFH_REQUIRE(textureId >= 0 && textureId < (int32_t)t.adjTexGrad.size() && t.adjTexGrad[textureId].ptr,
-                    "the texture has no gradient (not an albedo texture of this path tracer)");
+                    "the texture has no gradient (not an albedo or roughness texture of this path tracer)");
         const Texture& x = *c.textures[textureId];
         const size_t n = (size_t)x.width * x.height;
         std::vector<float> rgba(4 * n);

@@ -142,8 +142,9 @@ __device__ __forceinline__ void make_vertex(const PtScene& S, V3 rayOrigin, V3 d
 // background) with the MIS weight of hitting the emitter, times the
 // throughput; em receives the weighted emission.
 __device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
-                                                    V3& em, V3 prevPos) {
+                                                    V3& em, V3 prevPos, float& mis) {
     float connectPdf = 0.0f;
+    mis = 1.0f;
     if (vx.isVoid) {  // get_emission, VOID (path_util.hpp:415-424)
         float pdfBack;
         em = evaluate_background(S, vx.incEnc, pdfBack);
@@ -153,10 +154,15 @@ __device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vert
         if (nonzero(em)) connectPdf = light_pdf(S, vx.inst, vx.prim, prevPos);
     }
     if (nonzero(em) && pathLen > 1) {
-        const float mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / vx.incidentPdf));
+        mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / vx.incidentPdf));
         em = em * mis;
     }
     return throughput * em;
+}
+__device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vertex& vx, int pathLen, V3 throughput,
+                                                    V3& em, V3 prevPos) {
+    float mis;
+    return emission_contribution(S, vx, pathLen, throughput, em, prevPos, mis);
 }
 
 // Contributions of a new vertex at path length pathLen (pt_common.hpp:128-151):
@@ -210,6 +216,8 @@ struct NeeCandidate {
     V3 radiance;
     float lightness;
     int lightMaterial;
+    V3 lightDir;       // direction to the light sample (the BSDF's excident)
+    float misFactor;   // -d ln(mis) / d ln(BSDF pdf) = mis * hitPdf / creationPdf
 };
 __device__ __forceinline__ bool nee_candidate(const PtScene& S, const Vertex& vx, V3 throughput, float guideWeight,
                                               int i, u64 neeSeed, u64 x, NeeCandidate& c) {
@@ -232,6 +240,8 @@ __device__ __forceinline__ bool nee_candidate(const PtScene& S, const Vertex& vx
     c.radiance = throughput * vCos * radiance * mis;
     c.lightness = guideWeight * vCos;
     c.lightMaterial = nee.lightMaterial;
+    c.lightDir = nee.dir;
+    c.misFactor = mis * (hitPdf / nee.creationPdf);
     return true;
 }
 

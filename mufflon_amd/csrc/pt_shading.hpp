@@ -480,6 +480,69 @@ __device__ __forceinline__ V3 mat_evaluate(const TS& ts, const MatParams& m, V3 
     return value;
 }
 
+// ------------------------------------------------------------------ roughness derivatives
+// Logarithmic derivatives with respect to the roughness (rx, ry) for the
+// adjoint. The BSDF value of TORRANCE / MICROFACET is G * D * (terms free of
+// roughness) and its pdf is G1(in) * D * (terms free of roughness)
+// (evaluate_ts), so d ln value = d ln G + d ln D and d ln pdf = d ln G1(in)
+// + d ln D. The V-cavity G does not depend on the roughness; Smith's is a
+// product of G1 terms.
+
+// d ln D / d(rx, ry) of eval_ndf at h.
+__device__ __forceinline__ void dln_ndf(const MatParams& p, V3 h, float& dx, float& dy) {
+    const float x = h.x / p.rx, y = h.y / p.ry;
+    const float cz2 = h.z * h.z;
+    if (p.ndf == kNdfGgx) {
+        // D = 1 / (pi rx ry t^2), t = x^2 + y^2 + cz2
+        const float t = x * x + y * y + cz2;
+        dx = -1.0f / p.rx + 4.0f * x * x / (p.rx * t);
+        dy = -1.0f / p.ry + 4.0f * y * y / (p.ry * t);
+    } else {
+        // D = exp(-(x^2 + y^2) / cz2) / (pi rx ry cz2^2)
+        dx = 2.0f * x * x / (p.rx * cz2) - 1.0f / p.rx;
+        dy = 2.0f * y * y / (p.ry * cz2) - 1.0f / p.ry;
+    }
+}
+
+// d ln G1 / d(rx, ry) of g_smith for direction w (zero for the V-cavity).
+__device__ __forceinline__ void dln_g1(const MatParams& p, V3 w, float& dx, float& dy) {
+    dx = dy = 0.0f;
+    if (p.shadowing != kShadowingSmith) return;
+    // G1 = 2 / (1 + s), s = sqrt(1 + A), A = ((wx rx)^2 + (wy ry)^2) / wz^2
+    const float A = (sq(w.x * p.rx) + sq(w.y * p.ry)) / (w.z * w.z);
+    const float s = fsqrt(1.0f + A);
+    const float k = -1.0f / (2.0f * s * (1.0f + s));
+    dx = k * 2.0f * w.x * w.x * p.rx / (w.z * w.z);
+    dy = k * 2.0f * w.y * w.y * p.ry / (w.z * w.z);
+}
+
+// d ln value and d ln pdf of mat_evaluate(ts, m, incident, excident) with
+// respect to (rx, ry); false where the value is zero, the material has no
+// roughness, or the pdf is clamped (then the pdf derivative is zero).
+__device__ __forceinline__ bool dln_mat_evaluate(const TS& ts, const MatParams& m, V3 incident, V3 excident,
+                                                 float dv[2], float dp[2]) {
+    dv[0] = dv[1] = dp[0] = dp[1] = 0.0f;
+    if (m.type != kMatTorrance && m.type != kMatMicrofacet) return false;
+    if (!(m.rx > 0.0f && m.ry > 0.0f)) return false;
+    float cosOut, pdfF;
+    const V3 value = mat_evaluate(ts, m, incident, excident, cosOut, pdfF);
+    if (!nonzero(value)) return false;
+    const V3 in = v3(-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), -dot(incident, ts.shadingN));
+    const V3 ex = v3(dot(excident, ts.shadingTX), dot(excident, ts.shadingTY), dot(excident, ts.shadingN));
+    const V3 h = boundary_half(m.medium(in.z), m.medium(-in.z), in, ex);
+    float ddx, ddy, gix, giy, gex, gey;
+    dln_ndf(m, h, ddx, ddy);
+    dln_g1(m, in, gix, giy);
+    dln_g1(m, ex, gex, gey);
+    dv[0] = ddx + gix + gex;
+    dv[1] = ddy + giy + gey;
+    if (pdfF < 1e9f) {
+        dp[0] = ddx + gix;
+        dp[1] = ddy + giy;
+    }
+    return true;
+}
+
 // ------------------------------------------------------------------ background
 __device__ __forceinline__ float box_diag_len(const float* box) {
     return len(v3(box[3] - box[0], box[4] - box[1], box[5] - box[2]));

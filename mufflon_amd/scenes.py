@@ -504,15 +504,23 @@ def _rot(a, tilt):
 
 
 def cornell_textured(sphere_rings=128, sphere_segments=256, size=256, wall_res=512, sphere_res=256, seed=3,
-                     wall_sampling=None, sphere_sampling=None):
-    """The Cornell box with a procedural RGB albedo texture on the back wall
-    (texture 0, LAMBERT) and on the sphere (texture 1, a GGX TORRANCE sphere
-    of roughness 0.3): the parameters of the adjoint checks."""
+                     wall_sampling=None, sphere_sampling=None, sphere="roughness", sphere_shadowing=SHADOWING_VCAVITY):
+    """The Cornell box with textures for the adjoint: texture 0 is an RGB
+    albedo texture on the back wall (LAMBERT); texture 1 is on the GGX
+    TORRANCE sphere — with sphere="roughness" a one-channel roughness texture
+    in [0.05, 0.8] (albedo (0.9, 0.8, 0.6)), the differentiable benchmark's
+    configuration; with sphere="albedo" an RGB albedo texture (roughness
+    0.3)."""
     wall_sampling = SAMPLING_LINEAR if wall_sampling is None else wall_sampling
     sphere_sampling = SAMPLING_LINEAR if sphere_sampling is None else sphere_sampling
     c = cornell_box(sphere_rings, sphere_segments, size, size)
-    mats = np.concatenate([c["materials"], material(MAT_LAMBERT, albedo_texture=0),
-                           material(MAT_TORRANCE, albedo_texture=1, roughness=0.3)])
+    if sphere == "roughness":
+        sph = material(MAT_TORRANCE, albedo=(0.9, 0.8, 0.6), roughness_texture=1, shadowing=sphere_shadowing)
+        sph_tex = procedural_texture(sphere_res, sphere_res, 1, 0.05, 0.8, seed=seed + 1)
+    else:
+        sph = material(MAT_TORRANCE, albedo_texture=1, roughness=0.3, shadowing=sphere_shadowing)
+        sph_tex = procedural_texture(sphere_res, sphere_res, 3, 0.3, 0.9, seed=seed + 1)
+    mats = np.concatenate([c["materials"], material(MAT_LAMBERT, albedo_texture=0), sph])
     BACK, SPHERE = 4, 5
     mi = c["mat_idx"].copy()
     # back wall: the third quad (triangles 4, 5); sphere: the triangles after the two boxes
@@ -521,7 +529,7 @@ def cornell_textured(sphere_rings=128, sphere_segments=256, size=256, wall_res=5
     c["mat_idx"] = mi
     c["materials"] = mats
     c["textures"] = [(procedural_texture(wall_res, wall_res, 3, 0.15, 0.85, seed=seed), wall_sampling),
-                     (procedural_texture(sphere_res, sphere_res, 3, 0.3, 0.9, seed=seed + 1), sphere_sampling)]
+                     (sph_tex, sphere_sampling)]
     c["envmap"] = None
     c["point_lights"] = None
     return c

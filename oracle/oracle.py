@@ -524,6 +524,23 @@ def pt_render_targets(pt_scene, seed, min_path_length, max_path_length, nee_coun
     return means, nans.value
 
 
+def pt_render_replay(base_scene, eval_scene, seed, min_path_length, max_path_length, nee_count, iterations):
+    """Radiance (h, w, 3) of eval_scene rendered along the BSDF directions the
+    same paths sampled in base_scene (the estimator whose roughness
+    derivative the adjoint computes)."""
+    lib = load()
+    if not hasattr(lib, "_replay_ready"):
+        lib.orc_pt_render_replay.argtypes = [C.POINTER(OrcPtScene), C.POINTER(OrcPtScene), C.c_int32, C.c_int32,
+                                             C.c_int32, C.c_int32, C.c_int32, f32p]
+        lib.orc_pt_render_replay.restype = C.c_int
+        lib._replay_ready = True
+    w, h = base_scene.s.width, base_scene.s.height
+    out = np.zeros(w * h * 3, np.float32)
+    lib.orc_pt_render_replay(C.byref(base_scene.s), C.byref(eval_scene.s), seed, min_path_length, max_path_length,
+                             nee_count, iterations, _p(out, f32p))
+    return out.reshape(h, w, 3)
+
+
 def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, delta, tex_shapes):
     """FP64 adjoint of the oracle: delta (h, w, 3) = dLoss/dRadiance of the
     image of `iterations` iterations from a reset. tex_shapes: {texture id:

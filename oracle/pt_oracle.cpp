@@ -703,6 +703,58 @@ V3 mat_emission(const MatParams& m, V3 geoN, V3 excident) {
     return m.radiance;
 }
 
+// ---------------------------------------------------------------- roughness derivatives
+// d ln value and d ln pdf of mat_evaluate with respect to the roughness
+// (rx, ry), in double, for the adjoint. value = G * D * (roughness-free
+// terms), pdf = G1(in) * D * (roughness-free terms) (evaluate_ts); the
+// V-cavity G is roughness-free, Smith's is G1(in) * G1(ex) with
+// G1 = 2 / (1 + sqrt(1 + A)), A = ((wx rx)^2 + (wy ry)^2) / wz^2; GGX
+// D = 1 / (pi rx ry t^2), t = (hx/rx)^2 + (hy/ry)^2 + hz^2; Beckmann
+// D = exp(-((hx/rx)^2 + (hy/ry)^2) / hz^2) / (pi rx ry hz^4).
+void dln_ndf_d(const MatParams& p, V3 h, double& dx, double& dy) {
+    const double rx = p.rx, ry = p.ry;
+    const double x = h.x / rx, y = h.y / ry, cz2 = (double)h.z * h.z;
+    if (p.ndf == kGgx) {
+        const double t = x * x + y * y + cz2;
+        dx = -1.0 / rx + 4.0 * x * x / (rx * t);
+        dy = -1.0 / ry + 4.0 * y * y / (ry * t);
+    } else {
+        dx = 2.0 * x * x / (rx * cz2) - 1.0 / rx;
+        dy = 2.0 * y * y / (ry * cz2) - 1.0 / ry;
+    }
+}
+void dln_g1_d(const MatParams& p, V3 w, double& dx, double& dy) {
+    dx = dy = 0.0;
+    if (p.shadowing != kSmith) return;
+    const double wz2 = (double)w.z * w.z;
+    const double A = (((double)w.x * p.rx) * ((double)w.x * p.rx) + ((double)w.y * p.ry) * ((double)w.y * p.ry)) / wz2;
+    const double sq1 = std::sqrt(1.0 + A);
+    const double k = -1.0 / (2.0 * sq1 * (1.0 + sq1));
+    dx = k * 2.0 * w.x * w.x * p.rx / wz2;
+    dy = k * 2.0 * w.y * w.y * p.ry / wz2;
+}
+bool dln_mat_evaluate_d(const TS& ts, const MatParams& m, V3 incident, V3 excident, double dv[2], double dp[2]) {
+    dv[0] = dv[1] = dp[0] = dp[1] = 0.0;
+    if (m.type != kTorrance && m.type != kMicrofacet) return false;
+    if (!(m.rx > 0.0f && m.ry > 0.0f)) return false;
+    const EvalValue e = mat_evaluate(ts, m, incident, excident);
+    if (!nonzero(e.value)) return false;
+    const V3 in{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), -dot(incident, ts.shadingN)};
+    const V3 ex{dot(excident, ts.shadingTX), dot(excident, ts.shadingTY), dot(excident, ts.shadingN)};
+    const V3 h = boundary_half(m.medium(in.z), m.medium(-in.z), in, ex);
+    double ddx, ddy, gix, giy, gex, gey;
+    dln_ndf_d(m, h, ddx, ddy);
+    dln_g1_d(m, in, gix, giy);
+    dln_g1_d(m, ex, gex, gey);
+    dv[0] = ddx + gix + gex;
+    dv[1] = ddy + giy + gey;
+    if (e.pdfF < 1e9f) {
+        dp[0] = ddx + gix;
+        dp[1] = ddy + giy;
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------- scene access
 struct Vertex {
     bool isCamera = true, isVoid = false;
@@ -1230,8 +1282,21 @@ V3 mat_albedo(const MatParams& m) {
     return {0, 0, 0};
 }
 
+// Sampled BSDF directions of one path: recorded (mode 1) by a render of one
+// scene and replayed (mode 2) by a render of another, whose walks then
+// evaluate value * cos / pdf at the recorded directions instead of sampling.
+// This is the estimator the adjoint differentiates with respect to the
+// roughness (directions held fixed); finite differences of it check that
+// derivative.
+struct PathReplay {
+    int mode = 0;
+    std::vector<V3> dirs;
+    size_t pos = 0;
+};
+
 // pt_sample (pt_common.hpp:75-156)
-void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& rng, PixelAcc& acc) {
+void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& rng, PixelAcc& acc,
+               PathReplay* replay = nullptr) {
     const int minPL = params[1], maxPL = params[2], neeCount = params[3];
     V3 throughput{1.0f, 1.0f, 1.0f};
     float guideWeight = 1.0f;
@@ -1287,6 +1352,14 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             const float cosOut = dot(viewDir, excident);
             pdfF = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
             sThroughput = {1.0f, 1.0f, 1.0f};
+        } else if (replay && replay->mode == 2) {
+            if (replay->pos >= replay->dirs.size()) break;
+            excident = replay->dirs[replay->pos++];
+            const EvalValue e = mat_evaluate(vtx.ts, vtx.mat, vtx.incident_dir(), excident);
+            if (!nonzero(e.value) || !(e.pdfF > 0.0f)) break;
+            sThroughput = e.value * (e.cosOut / e.pdfF);
+            pdfF = e.pdfF;
+            medium = vtx.mat.medium(dot(excident, vtx.ts.geoN));
         } else {
             const PathSample ps = mat_sample(vtx.ts, vtx.mat, vtx.incident_dir(), u0, u1, x1);
             if (!ps.valid) break;  // CANCEL
@@ -1294,6 +1367,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             pdfF = ps.pdfF;
             sThroughput = ps.throughput;
             medium = vtx.mat.medium(dot(excident, vtx.ts.geoN));  // path_util.hpp:346-351
+            if (replay && replay->mode == 1) replay->dirs.push_back(excident);
         }
         throughput = throughput * sThroughput;
         const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
@@ -1374,14 +1448,19 @@ MatParams params_of(const OrcMaterial& m, const OrcTexture* textures, float u, f
 // path is the primal's, the gradient arithmetic is in double.
 struct AdjRec {
     int tex = -1;
+    int rtex = -1;  // roughness texture
     float hu = 0, hv = 0;
     V3 rho{0, 0, 0};
     double neeW[3] = {0, 0, 0};
     double emW[3] = {0, 0, 0};
+    double misC = 0.0;  // 1 - MIS weight of this vertex's emission
     double G[3] = {0, 0, 0};
     float cont = 1.0f;
     float tp = 0.0f;
     int astar = 0;
+    double rw[2] = {0, 0};  // d ln(walk weight) / d(rx, ry), sampled direction held fixed
+    double rp[2] = {0, 0};  // d ln(pdf of the sampled direction) / d(rx, ry)
+    double rn[2] = {0, 0};  // next-event estimates: weighted contribution * d ln / d(rx, ry)
 };
 
 struct AdjOut {
@@ -1452,8 +1531,15 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
                         const float mis = 1.0f / ((float)neeCount + hitPdf / nee.creationPdf);
                         const V3 c = throughput * value.cosOut * radiance * mis;
                         const double wgt[3] = {delta[0] * c.x, delta[1] * c.y, delta[2] * c.z};
-                        for (int k = 0; k < 3; ++k) rec[pathLen].neeW[k] += wgt[k];
+                        AdjRec& r = rec[pathLen];
+                        for (int k = 0; k < 3; ++k) r.neeW[k] += wgt[k];
                         if (nee.lightMaterial >= 0) add_em_grad(S, out, nee.lightMaterial, wgt);
+                        double dv[2], dp[2];
+                        if (r.rtex >= 0 && dln_mat_evaluate_d(vtx.ts, vtx.mat, vtx.incident_dir(), nee.dir, dv, dp)) {
+                            // d ln mis / d ln pdf = -mis * hitPdf / creationPdf
+                            const double misF = (double)mis * ((double)hitPdf / nee.creationPdf);
+                            for (int q = 0; q < 2; ++q) r.rn[q] += (wgt[0] + wgt[1] + wgt[2]) * (dv[q] - misF * dp[q]);
+                        }
                     }
                 }
             }
@@ -1486,6 +1572,14 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             pdfF = ps.pdfF;
             sThroughput = ps.throughput;
             medium = vtx.mat.medium(dot(excident, vtx.ts.geoN));
+            AdjRec& r = rec[pathLen];
+            double dv[2], dp[2];
+            if (r.rtex >= 0 && dln_mat_evaluate_d(vtx.ts, vtx.mat, vtx.incident_dir(), excident, dv, dp)) {
+                for (int q = 0; q < 2; ++q) {
+                    r.rw[q] = dv[q] - dp[q];
+                    r.rp[q] = pdfF < 1e9f ? dp[q] : 0.0;
+                }
+            }
         }
         throughput = throughput * sThroughput;
         const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
@@ -1531,6 +1625,8 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             r.hv = hit.v;
             r.rho = vtx.mat.albedo;
             if (vtx.mat.type == kLambert || vtx.mat.type == kTorrance) r.tex = S.materials[matIdx].albedoTexture;
+            if (vtx.mat.type == kTorrance || vtx.mat.type == kMicrofacet)
+                r.rtex = S.materials[matIdx].roughnessTexture;
         }
         if (pathLen >= minPL) {
             V3 emission{0, 0, 0};
@@ -1546,6 +1642,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             if (nonzero(emission) && pathLen > 1) {
                 const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
                 emission = emission * mis;
+                r.misC = 1.0 - (double)mis;  // d ln mis / d ln(incident pdf)
             }
             const V3 c = throughput * emission;
             r.emW[0] = delta[0] * c.x;
@@ -1560,6 +1657,34 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
         for (int k = 0; k < 3; ++k) rec[v].G[k] = rec[v + 1].G[k] + rec[v + 1].emW[k] + rec[v + 1].neeW[k];
     for (int j = 1; j <= last; ++j) {
         const AdjRec& r = rec[j];
+        if (r.rtex >= 0 && out.texGrad[r.rtex]) {
+            // roughness of vertex j: the walk weight scales everything after
+            // walk j, the next vertex's emission MIS sees the sampled pdf, the
+            // next-event estimates at j their own terms, and the roulette
+            // probabilities of walks m >= j follow the throughput
+            const double Gj = r.G[0] + r.G[1] + r.G[2];
+            const double em1 = j < last ? (rec[j + 1].emW[0] + rec[j + 1].emW[1] + rec[j + 1].emW[2]) * rec[j + 1].misC
+                                        : 0.0;
+            double g[2];
+            for (int q = 0; q < 2; ++q) {
+                g[q] = r.rw[q] * Gj + r.rn[q] + em1 * r.rp[q];
+                double sumD = 0.0;
+                for (int m = j; m < last; ++m) {
+                    if (!(rec[m].cont < 1.0f)) continue;
+                    const double D = (double)rec[m].tp / rec[m].cont * (r.rw[q] - sumD);
+                    g[q] -= D * (rec[m].G[0] + rec[m].G[1] + rec[m].G[2]);
+                    sumD += D;
+                }
+            }
+            const OrcTexture& tx = S.textures[r.rtex];
+            int idx[4];
+            double w[4];
+            int nt = 0;
+            texel_weights(tx, r.hu, r.hv, idx, w, nt);
+            const double gc[2] = {tx.channels > 1 ? g[0] : g[0] + g[1], g[1]};
+            for (int ch = 0; ch < (tx.channels > 1 ? 2 : 1); ++ch)
+                for (int q = 0; q < nt; ++q) out.texGrad[r.rtex][4 * (size_t)idx[q] + ch] += w[q] * gc[ch];
+        }
         if (r.tex < 0 || !out.texGrad[r.tex]) continue;
         int idx[4];
         double w[4];
@@ -1666,6 +1791,36 @@ int orc_pt_render_targets(const OrcPtScene* S, int32_t seed, int32_t minPathLeng
                           uint32_t* outNans) {
     return orc_pt_render_targets_var(S, seed, minPathLength, maxPathLength, neeCount, iterations, targetMask,
                                      outTargets, 0u, nullptr, outNans);
+}
+
+// Radiance of `iterations` iterations of `eval` along the BSDF directions
+// that the same paths (same seeds) sampled in `base` (PathReplay).
+int orc_pt_render_replay(const OrcPtScene* base, const OrcPtScene* eval, int32_t seed, int32_t minPathLength,
+                         int32_t maxPathLength, int32_t neeCount, int32_t iterations, float* outRadiance) {
+    const int W = base->width, H = base->height;
+    const int num = W * H;
+    const int s = seed * (num + 1);
+    const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int pixel = 0; pixel < num; ++pixel) {
+        Pcg64 rng((u32)(pixel + s));
+        float cum[3] = {0.0f, 0.0f, 0.0f};
+        for (int it = 0; it < iterations; ++it) {
+            const Pcg64 start = rng;
+            PathReplay rp;
+            rp.mode = 1;
+            PixelAcc ignored;
+            pt_sample(*base, params, pixel % W, pixel / W, rng, ignored, &rp);
+            rp.mode = 2;
+            Pcg64 r2 = start;
+            PixelAcc acc;
+            pt_sample(*eval, params, pixel % W, pixel / W, r2, acc, &rp);
+            const float denom = mx(1.0f, (float)it + 1);
+            for (int c = 0; c < 3; ++c) cum[c] += (acc.v[c] - cum[c]) / denom;
+        }
+        for (int c = 0; c < 3; ++c) outRadiance[3 * (size_t)pixel + c] = cum[c];
+    }
+    return 0;
 }
 
 // Radiance and Hit ID only.

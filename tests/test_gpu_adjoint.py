@@ -1,9 +1,12 @@
 """Adjoint (path replay) on the GPU: against the oracle's FP64 adjoint on the
-same paths, against central finite differences of GPU renders with
-identical seeds (the configuration of the differentiable benchmark: the
-Cornell box with a 512x512 albedo texture on the back wall and a 256x256 one
-on the sphere, 256x256 pixels, 4 paths per pixel), and through the
-torch.autograd binding."""
+same paths (albedo and roughness textures, V-cavity and Smith), against
+central finite differences of GPU renders with identical seeds (the
+configuration of the differentiable benchmark: the Cornell box with a
+512x512 albedo texture on the back wall and a 256x256 roughness texture on
+the GGX sphere, 256x256 pixels, 4 paths per pixel; finite differences of the
+albedo texels — the roughness derivative holds the sampled directions fixed
+and is checked against the oracle's replay differences in
+test_oracle_adjoint), and through the torch.autograd binding."""
 import numpy as np
 import pytest
 
@@ -19,9 +22,11 @@ def _setup(hip_ctx, c, depth):
     return pt, ids
 
 
-def test_adjoint_matches_oracle_fp64(hip_ctx, oracle):
-    c = S.cornell_textured(12, 24, 40, 16, 16)
-    print("adjoint vs oracle fp64, 40x40", flush=True)
+@pytest.mark.parametrize("sphere,shadowing", [("albedo", S.SHADOWING_VCAVITY), ("roughness", S.SHADOWING_VCAVITY),
+                                              ("roughness", S.SHADOWING_SMITH)])
+def test_adjoint_matches_oracle_fp64(hip_ctx, oracle, sphere, shadowing):
+    c = S.cornell_textured(12, 24, 40, 16, 16, sphere=sphere, sphere_shadowing=shadowing)
+    print("adjoint vs oracle fp64, 40x40, sphere %s, shadowing %d" % (sphere, shadowing), flush=True)
     depth, its = 8, 3
     pt, ids = _setup(hip_ctx, c, depth)
     hip_ctx.pt_iterate(pt, its)
@@ -29,15 +34,17 @@ def test_adjoint_matches_oracle_fp64(hip_ctx, oracle):
     rng = np.random.default_rng(5)
     delta = rng.normal(size=img.shape).astype(np.float32)
     hip_ctx.pt_adjoint(pt, its, delta)
-    g_gpu = [hip_ctx.pt_texture_gradient(pt, int(i), (16, 16, 3)) for i in ids]
+    shapes = [t.shape for t, _ in c["textures"]]
+    g_gpu = [hip_ctx.pt_texture_gradient(pt, int(i), sh) for i, sh in zip(ids, shapes)]
     e_gpu = hip_ctx.pt_emission_gradient(pt, 0)
     ps = P.oracle_pt_scene_textured(oracle, c, light_tree=hip_ctx.pt_light_tree(pt))
     o_img, _, _ = oracle.pt_render(ps, 1, 0, depth, 1, its)
     assert np.array_equal(img.view(np.uint32), o_img.view(np.uint32))
     g_orc, e_orc = oracle.pt_adjoint(ps, 1, 0, depth, 1, its, delta, {0: (16, 16), 1: (16, 16)})
     for t in (0, 1):
-        ref = g_orc[t][..., :3]
+        ref = g_orc[t][..., :shapes[t][2]]
         scale = np.abs(ref).max()
+        assert scale > 0
         err = np.abs(g_gpu[t] - ref)
         print("texture %d: max |grad| %.4g, max abs err %.3g" % (t, scale, float(err.max())), flush=True)
         assert (err <= 1e-3 * np.abs(ref) + 1e-5 * scale).all()
@@ -53,7 +60,7 @@ def test_adjoint_finite_differences_config3(hip_ctx):
     pt, ids = _setup(hip_ctx, c, depth)
     textures = [t.copy() for t, _ in c["textures"]]
     target_tex = [S.procedural_texture(512, 512, 3, 0.15, 0.85, seed=11),
-                  S.procedural_texture(256, 256, 3, 0.3, 0.9, seed=12)]
+                  S.procedural_texture(256, 256, 1, 0.05, 0.8, seed=12)]
 
     def render(texs):
         for i, t in zip(ids, texs):
@@ -73,10 +80,11 @@ def test_adjoint_finite_differences_config3(hip_ctx):
     rng = np.random.default_rng(7)
     checked = agreed = flips = 0
     rels = []
-    for t in (0, 1):
+    assert np.abs(grads[1]).max() > 0  # the roughness texture has a gradient
+    for t in (0,):
         g = grads[t]
         cand = np.argwhere(np.abs(g) > 1e-2 * np.abs(g).max())
-        for k in rng.choice(len(cand), size=min(32, len(cand)), replace=False):
+        for k in rng.choice(len(cand), size=min(64, len(cand)), replace=False):
             y, x, ch = cand[k]
             fds = []
             for eps in (1e-3, 5e-4):
