@@ -46,6 +46,8 @@ def parse():
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
     ap.add_argument("--pt-iterations", type=int, default=64,
                     help="Cornell-box path-tracer iterations measured next to the main workload (0: skip)")
+    ap.add_argument("--reference-batches", type=int, default=16,
+                    help="256-spp batches of the converged Cornell reference for the RMSE")
     ap.add_argument("--terrain-iterations", type=int, default=16,
                     help="iterations of the terrain megakernel / wavefront comparison (0 skips it)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint measurement")
@@ -79,6 +81,29 @@ def cpu_baseline(verts, tris, rays):
                        "rays on %d OpenMP threads (%.3f s, %.2f Mrays/s)" %
                        (len(tris), build_s, len(rays), cores, trace_s, len(rays) / trace_s / 1e6),
                 lbvh_build_ms=build_s * 1e3, trace_mrays_s=len(rays) / trace_s / 1e6)
+
+
+def converged_rmse(ctx, pt, image, params, batches, batch_iterations=256):
+    """RMSE of `image` against a converged render of the same path tracer:
+    `batches` x `batch_iterations` paths per pixel with other seeds (the
+    megakernel, whose paths are bit-identical to the oracle's), batch means
+    averaged in float64. Also the reference's own standard error (from the
+    spread of the batch means) and the RMSE relative to the mean pixel."""
+    from mufflon_amd import hip
+    ctx.pt_set_mode(pt, hip.PT_MEGAKERNEL, False)
+    means = []
+    for b in range(batches):
+        ctx.pt_set_params(pt, dict(params, seed=1000 + b))
+        ctx.pt_iterate(pt, batch_iterations)
+        means.append(ctx.pt_target(pt).astype(np.float64))
+    ref = np.mean(means, axis=0)
+    stderr = np.sqrt(np.mean(np.var(means, axis=0, ddof=1) / batches)) if batches > 1 else float("nan")
+    rmse = float(np.sqrt(np.mean((image.astype(np.float64) - ref) ** 2)))
+    ctx.pt_set_params(pt, dict(params, seed=1))
+    return {"vs": "converged render, %d spp (float64 mean of %d batches of %d), seeds 1000+" %
+                  (batches * batch_iterations, batches, batch_iterations),
+            "rmse": round(rmse, 6), "relative_rmse": round(rmse / float(np.mean(ref)), 5),
+            "reference_standard_error": round(float(stderr), 6)}
 
 
 def pt_cornell(args, cpu_baseline_too):
@@ -117,6 +142,7 @@ def pt_cornell(args, cpu_baseline_too):
     wms = ctx.timer_end()
     out["wavefront"] = {"ms_per_iteration": round(wms / n, 4), "mpaths_s": round(pixels * n / (wms / 1e3) / 1e6, 3),
                         "identical_to_megakernel": bool(np.array_equal(ctx.pt_target(pt), mega))}
+    out["rmse"] = converged_rmse(ctx, pt, mega, dict(max_path_length=8, nee_count=1), args.reference_batches)
     if cpu_baseline_too:
         from oracle import oracle as O
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -129,10 +155,11 @@ def pt_cornell(args, cpu_baseline_too):
         t0 = time.perf_counter()
         O.pt_render(ps, 1, 0, 8, 1, k)
         dt = time.perf_counter() - t0
+        threads = omp_threads()
         out["cpu_baseline"] = {"value": round(pixels * k / dt / 1e6, 4), "unit": "Mpaths/s",
-                               "cores": os.cpu_count() or 1, "kind": "port",
+                               "cores": threads, "kind": "port",
                                "sample": "%d iterations of the same Cornell workload on %d OpenMP threads (%.2f s)"
-                                         % (k, os.cpu_count() or 1, dt)}
+                                         % (k, threads, dt)}
     ctx.close()
     return out
 
