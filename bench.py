@@ -42,6 +42,11 @@ def parse():
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pt-profile", choices=["terrain", "cornell"], default=None,
+                    help="only a short run of one path-tracer workload (for rocprofv3)")
+    ap.add_argument("--trace-block", type=int, default=256, help="wavefront trace block (with --pt-profile)")
+    ap.add_argument("--stack-depth", type=int, default=16, help="wavefront LDS stack depth (with --pt-profile)")
+    ap.add_argument("--refill", type=int, default=0, help="wavefront lane refill budget (with --pt-profile)")
     ap.add_argument("--profile-only", action="store_true",
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
     ap.add_argument("--pt-iterations", type=int, default=64,
@@ -63,7 +68,7 @@ def cpu_baseline(verts, tris, rays):
     cores: serial LBVH build (thrust::seq order) + OpenMP closest hit over all
     rays. Timed once after one warm-up build on a 100k-triangle sample."""
     from oracle import oracle as O
-    cores = os.cpu_count() or 1
+    cores = omp_threads()
     O.lbvh_build_tris(verts[: 300_000], tris[:100_000])  # warm-up
     t0 = time.perf_counter()
     b = O.lbvh_build_tris(verts, tris)
@@ -106,20 +111,37 @@ def converged_rmse(ctx, pt, image, params, batches, batch_iterations=256):
             "reference_standard_error": round(float(stderr), 6)}
 
 
-def pt_cornell(args, cpu_baseline_too):
-    """Megakernel path tracer on the procedural Cornell box: 256x256, 128x256
-    sphere, depth 8, NEE 1, seed 1; Mpaths/s = pixels * iterations / time."""
+def omp_threads():
+    """Threads the OpenMP CPU oracle runs on (OMP_NUM_THREADS, else all cores)."""
+    try:
+        return int(os.environ.get("OMP_NUM_THREADS", ""))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cornell_tracer(ctx):
+    """The Cornell-box path tracer: 256x256, 128x256 sphere, depth 8, NEE 1,
+    seed 1. Returns (pt id, scene dict, scene box)."""
     from mufflon_amd import scenes as S
-    from mufflon_amd.hip import HipContext
     c = S.cornell_box(128, 256, 256, 256)
     v = c["verts"]
     box = np.concatenate([v.min(0), v.max(0)]).astype(np.float32)
-    ctx = HipContext(0)
     lod = ctx.lod_create(v, c["tris"])
     ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
     ctx.lbvh_build(lod, box)
     sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
     pt = ctx.pt_create(sc, c["materials"], c["camera"], dict(seed=1, max_path_length=8, nee_count=1))
+    return pt, c, box
+
+
+def pt_cornell(args, cpu_baseline_too):
+    """Megakernel path tracer on the procedural Cornell box: 256x256, 128x256
+    sphere, depth 8, NEE 1, seed 1; Mpaths/s = pixels * iterations / time."""
+    from mufflon_amd import scenes as S
+    from mufflon_amd.hip import HipContext
+    ctx = HipContext(0)
+    pt, c, box = cornell_tracer(ctx)
+    v = c["verts"]
     ctx.pt_iterate(pt, 4)
     ctx.pt_reset(pt)
     ctx.synchronize()
@@ -303,12 +325,44 @@ def pt_batch(args):
             "albedo_gradients_8_scenes_s": round(adj_s, 3)}
 
 
+def pt_profile(args):
+    """A short run of one path-tracer workload for rocprofv3: `terrain`
+    (megakernel then wavefront with the given tuning, 2 iterations each after
+    one warm-up) or `cornell` (both forms, 8 iterations each)."""
+    from mufflon_amd import hip, scenes as S, workloads as W
+    from mufflon_amd.hip import HipContext
+    ctx = HipContext(0)
+    if args.pt_profile == "terrain":
+        sc = S.terrain_scene()
+        its = 2
+        pt, _, _ = W.path_tracer(ctx, sc, max_path_length=16)
+    else:
+        its = 8
+        pt, _, _ = cornell_tracer(ctx)
+    for mode in (hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT):
+        ctx.pt_set_mode(pt, mode, False)
+        if mode == hip.PT_WAVEFRONT:
+            ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, 0, args.refill)
+        ctx.pt_reset(pt)
+        ctx.pt_iterate(pt, 1)
+        ctx.pt_reset(pt)
+        ctx.timer_begin()
+        ctx.pt_iterate(pt, its)
+        ms = ctx.timer_end() / its
+        print("%s: %.3f ms/iteration" % ("megakernel" if mode == hip.PT_MEGAKERNEL else "wavefront", ms),
+              file=sys.stderr, flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.pt_profile:
+        pt_profile(args)
+        return
     from mufflon_amd import scenes as S
     from mufflon_amd.hip import HipContext
 

@@ -63,6 +63,12 @@ void release_wavefront(PathTracer& t) {
     t.wfMaxLen = t.wfNee = -1;
 }
 
+// Trace-stage forms the autotuners try: chunked queue walks with two chunk
+// sizes, and lane refill every 16 or 32 traversal steps.
+constexpr int kTraceModes = 4;
+constexpr int kTraceModeChunk[kTraceModes] = {256, 1024, 256, 256};
+constexpr int kTraceModeRefill[kTraceModes] = {0, 0, 16, 32};
+
 // A batch of path tracers iterated together by the wavefront stages.
 struct PtBatch {
     std::vector<int32_t> tracers;
@@ -858,8 +864,10 @@ int mfh_batch_set_tuning(mfh_context* ctx, int32_t batchId, const mfh_wavefront_
         MFH_REQUIRE(tuning->queueChunk >= 64 && tuning->queueChunk % 64 == 0 && tuning->queueChunk <= 65536,
                     "queue chunk must be a multiple of 64 in [64, 65536]");
         MFH_REQUIRE(tuning->sortBounces == 0, "batches do not sort rays");
+        MFH_REQUIRE(tuning->refillBudget >= 0 && tuning->refillBudget <= 4096, "refillBudget out of range");
         b.tune.traceVariant = v;
         b.tune.chunk = tuning->queueChunk;
+        b.tune.refill = tuning->refillBudget;
         b.tune.traceGrid = 0;
         drop_batch_graph(b);
     });
@@ -874,9 +882,10 @@ int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mf
         float best = 3.4e38f;
         WfTuning bestTune = b.tune;
         for (int v = 0; v < kWfNumTraceVariants; ++v) {
-            for (int ch : {256, 1024}) {
+            for (int m = 0; m < kTraceModes; ++m) {
                 b.tune.traceVariant = v;
-                b.tune.chunk = ch;
+                b.tune.chunk = kTraceModeChunk[m];
+                b.tune.refill = kTraceModeRefill[m];
                 b.tune.traceGrid = 0;
                 drop_batch_graph(b);
                 run_batch(c, b, 1);
@@ -902,6 +911,7 @@ int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mf
             outBest->stackDepth = kWfTraceShapes[b.tune.traceVariant].lds;
             outBest->queueChunk = b.tune.chunk;
             outBest->sortBounces = 0;
+            outBest->refillBudget = b.tune.refill;
         }
         if (outMsPerIteration) *outMsPerIteration = best;
     });
@@ -934,9 +944,11 @@ int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning
                     "queue chunk must be a multiple of 64 in [64, 65536]");
         MFH_HIP(hipStreamSynchronize(c.stream));
         MFH_REQUIRE(tuning->sortBounces >= 0 && tuning->sortBounces <= 64, "sortBounces out of range");
+        MFH_REQUIRE(tuning->refillBudget >= 0 && tuning->refillBudget <= 4096, "refillBudget out of range");
         t.tune.traceVariant = v;
         t.tune.chunk = tuning->queueChunk;
         t.tune.sortBounces = tuning->sortBounces;
+        t.tune.refill = tuning->refillBudget;
         t.tune.traceGrid = 0;
         drop_graph(t);
     });
@@ -951,6 +963,7 @@ int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* out)
         out->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
         out->queueChunk = t.tune.chunk;
         out->sortBounces = t.tune.sortBounces;
+        out->refillBudget = t.tune.refill;
     });
 }
 
@@ -963,16 +976,16 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
         MFH_REQUIRE(t.mode == MFH_PT_WAVEFRONT, "autotune applies to the wavefront form");
         MFH_REQUIRE(t.tree.posLightCount > 0 || t.background.fluxSum > 0.0f, "scene without lights");
         PtScene S = device_scene(c, t);
-        const int chunks[2] = {256, 1024};
         const int sorts[3] = {0, 1, 2};
         float best = 3.4e38f;
         WfTuning bestTune = t.tune;
         for (int v = 0; v < kWfNumTraceVariants; ++v) {
-            for (int cs = 0; cs < 6; ++cs) {
-                const int ch = chunks[cs % 2];
+            for (int cs = 0; cs < 3 * kTraceModes; ++cs) {
+                const int m = cs % kTraceModes;
                 t.tune.traceVariant = v;
-                t.tune.chunk = ch;
-                t.tune.sortBounces = sorts[cs / 2];
+                t.tune.chunk = kTraceModeChunk[m];
+                t.tune.refill = kTraceModeRefill[m];
+                t.tune.sortBounces = sorts[cs / kTraceModes];
                 t.tune.traceGrid = 0;
                 drop_graph(t);
                 run_wavefront(c, t, S, 1);  // warm-up (and graph capture)
@@ -997,6 +1010,7 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
             outBest->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
             outBest->queueChunk = t.tune.chunk;
             outBest->sortBounces = t.tune.sortBounces;
+            outBest->refillBudget = t.tune.refill;
         }
         if (outMsPerIteration) *outMsPerIteration = best;
     });

@@ -268,6 +268,141 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S0, WfState W, int
     }
 }
 
+// ---- trace stages with lane refill
+// The persistent trace kernels above give each lane whole rays from the
+// wave's chunk; a lane whose ray ends early idles until the wave's longest
+// traversal in the chunk is done. The refill forms below traverse in rounds
+// of `budget` steps (Traversal::run) and, between rounds, hand new rays to
+// idle lanes as soon as a quarter of the wave is idle: one atomic on the
+// queue's work counter per refill, lanes ranked with mbcnt. The queue is
+// consumed in order of the grabs, and a ray's result does not depend on the
+// rounds it took.
+constexpr int kRefillMinIdle = 16;
+
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Refill of the idle lanes (mask `idle`, wave-uniform): first queue index of
+// this lane or -1; drained is set once the queue has been handed out.
+__device__ __forceinline__ int refill_slot(int* work, int n, unsigned long long idle, bool& drained) {
+    const int want = __popcll(idle);
+    const int leader = __ffsll((long long)idle) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(work, want);
+    base = __shfl(base, leader);
+    if (base + want >= n) drained = true;
+    if (!((idle >> lane_id()) & 1ull)) return -1;
+    const int slot = base + lane_rank(idle);
+    return slot < n ? slot : -1;
+}
+
+template <int kMode>
+constexpr int traversal_mode() {
+    return kMode == 1 ? 1 : (kMode == 0 ? 0 : 2);
+}
+
+template <int kMode, int kBlock, int kLds>
+__global__ __launch_bounds__(kBlock) void k_wf_extend_refill(PtScene S0, WfState W, int buf, int count, int work,
+                                                             const int* perm, int budget) {
+    __shared__ int sStack[kLds * kBlock];
+    constexpr bool kBatch = kMode == 2;
+    using T = Traversal<false, false, traversal_mode<kMode>(), kBlock, kLds>;
+    const int n = W.counters[count];
+    int* const lstack = sStack + threadIdx.x;
+    int spill[T::kSpill];
+    T tr;
+    int q = -1, si = 0;
+    bool drained = false;
+    while (true) {
+        const unsigned long long idle = __ballot(q < 0);
+        if (!drained && __popcll(idle) >= kRefillMinIdle) {
+            const int slot = refill_slot(W.counters + work, n, idle, drained);
+            if (slot >= 0) {
+                q = perm ? perm[slot] : slot;
+                const float4 o = W.rayO[buf][q];
+                const float4 d = W.rayD[buf][q];
+                int local;
+                si = locate<kBatch>(W, __float_as_int(d.w), local);
+                tr.init(scene_ref<kBatch>(S0, W, si).trace, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), kMaxSceneSize);
+            }
+        }
+        if (__ballot(q >= 0) == 0) break;  // nothing in flight: the queue is drained
+        if (q >= 0 && tr.template run<true>(scene_ref<kBatch>(S0, W, si).trace, lstack, spill, budget)) {
+            const TraceHit h = tr.result();
+            W.hit[buf][q] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
+            W.rayM[buf][q].w = h.w1;
+            q = -1;
+        }
+    }
+}
+
+template <int kMode, int kBlock, int kLds>
+__global__ __launch_bounds__(kBlock) void k_wf_shadow_refill(PtScene S0, WfState W, int count, int work,
+                                                             const int* perm, int budget) {
+    __shared__ int sStack[kLds * kBlock];
+    constexpr bool kBatch = kMode == 2;
+    using T = Traversal<true, false, traversal_mode<kMode>(), kBlock, kLds>;
+    const int n = W.counters[count];
+    int* const lstack = sStack + threadIdx.x;
+    int spill[T::kSpill];
+    T tr;
+    int sl = -1, si = 0, path = 0, cur = 0;
+    u32 remaining = 0, visible = 0;
+    bool drained = false;
+    // next candidate of the slot, or its result once every candidate is traced
+    auto advance = [&]() {
+        while (sl >= 0) {
+            if (remaining) {
+                cur = __ffs(remaining) - 1;
+                remaining &= remaining - 1;
+                const size_t e = (size_t)sl * W.neeStride + cur;
+                const float4 a = W.shO[e];
+                const float4 d = W.shD[e];
+                tr.init(scene_ref<kBatch>(S0, W, si).trace, v3(a.x, a.y, a.z), v3(d.x, d.y, d.z), a.w);
+                return;
+            }
+            if (visible) {  // add the unoccluded estimates in candidate order
+                const PtScene& S = scene_ref<kBatch>(S0, W, si);
+                Acc acc;
+                load_acc(W, S.targetMask, path, acc);
+                for (int i = 0; i < S.neeCount; ++i) {
+                    if (!((visible >> i) & 1u)) continue;
+                    const float4 c = W.shC[(size_t)sl * W.neeStride + i];
+                    NeeCandidate nc;
+                    nc.radiance = v3(c.x, c.y, c.z);
+                    nc.lightness = c.w;
+                    add_nee(nc, acc);
+                }
+                store_acc(W, S.targetMask, path, acc);
+                if (acc.nans) atomicAdd(S.nanCounter, acc.nans);
+            }
+            sl = -1;
+        }
+    };
+    while (true) {
+        const unsigned long long idle = __ballot(sl < 0);
+        if (!drained && __popcll(idle) >= kRefillMinIdle) {
+            const int slot = refill_slot(W.counters + work, n, idle, drained);
+            if (slot >= 0) {
+                sl = perm ? perm[slot] : slot;
+                const int2 pm = W.shPath[sl];
+                path = pm.x;
+                remaining = (u32)pm.y;
+                visible = 0;
+                int local;
+                si = locate<kBatch>(W, path, local);
+                advance();
+            }
+        }
+        if (__ballot(sl >= 0) == 0) break;
+        if (sl >= 0 && tr.template run<true>(scene_ref<kBatch>(S0, W, si).trace, lstack, spill, budget)) {
+            if (tr.result().inst == -1) visible |= 1u << cur;
+            advance();
+        }
+    }
+}
+
 // Shade the hits of queue `buf` at path length k + 1.
 template <bool kBatch>
 __global__ __launch_bounds__(256) void k_wf_shade(PtScene S0, WfState W, int k, int buf, int count, int nextCount,
@@ -426,26 +561,41 @@ __global__ void k_wf_finish(PtScene S0, WfState W) {
 template <int kMode, int kBlock, int kLds>
 struct TraceKernels {
     static void extend(const PtScene& S, const WfState& W, int grid, int buf, int count, int work, const int* perm,
-                       hipStream_t st) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
-                           W, buf, count, work, perm);
+                       int refill, hipStream_t st) {
+        if (refill > 0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend_refill<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0,
+                               st, S, W, buf, count, work, perm, refill);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
+                               W, buf, count, work, perm);
     }
-    static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, const int* perm,
+    static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, const int* perm, int refill,
                        hipStream_t st) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
-                           W, count, work, perm);
+        if (refill > 0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow_refill<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0,
+                               st, S, W, count, work, perm, refill);
+        else
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
+                               W, count, work, perm);
     }
-    static int occupancy() {
+    static int occupancy(bool refill) {
         int a = 0, b = 0;
-        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kMode, kBlock, kLds>, kBlock, 0));
-        MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kMode, kBlock, kLds>, kBlock, 0));
+        if (refill) {
+            MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend_refill<kMode, kBlock, kLds>, kBlock,
+                                                                 0));
+            MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow_refill<kMode, kBlock, kLds>, kBlock,
+                                                                 0));
+        } else {
+            MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kMode, kBlock, kLds>, kBlock, 0));
+            MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kMode, kBlock, kLds>, kBlock, 0));
+        }
         return std::max(1, std::min(a, b));
     }
 };
 
-using ExtendFn = void (*)(const PtScene&, const WfState&, int, int, int, int, const int*, hipStream_t);
-using ShadowFn = void (*)(const PtScene&, const WfState&, int, int, int, const int*, hipStream_t);
-using OccFn = int (*)();
+using ExtendFn = void (*)(const PtScene&, const WfState&, int, int, int, int, const int*, int, hipStream_t);
+using ShadowFn = void (*)(const PtScene&, const WfState&, int, int, int, const int*, int, hipStream_t);
+using OccFn = int (*)(bool);
 
 struct TraceVariant {
     ExtendFn extend;
@@ -475,10 +625,12 @@ const WfTraceShape kWfTraceShapes[kWfNumTraceVariants] = {{64, 8}, {64, 16}, {12
 
 int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs) {
     const TraceVariant& v = kVariants[S.trace.numInstances == 1 ? 1 : 0][tune.traceVariant];
-    return v.occupancy() * numCUs;
+    return v.occupancy(tune.refill > 0) * numCUs;
 }
 
-int wf_grid_batch(const WfTuning& tune, int numCUs) { return kVariants[2][tune.traceVariant].occupancy() * numCUs; }
+int wf_grid_batch(const WfTuning& tune, int numCUs) {
+    return kVariants[2][tune.traceVariant].occupancy(tune.refill > 0) * numCUs;
+}
 
 int wf_shade_grid(int numCUs) {
     int a = 0, b = 0;
@@ -514,10 +666,11 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
                            buf, W.qBase + k, W.qBase + k + 1, W.shBase + k, work++);
         const bool sortNow = k < tune.sortBounces;
         if (sortNow) sort_queue(S, W, 0, W.shBase + k, true, scratch, stream);
-        v.shadow(S, W, tune.traceGrid, W.shBase + k, work++, sortNow ? W.sortIds[0] : nullptr, stream);
+        v.shadow(S, W, tune.traceGrid, W.shBase + k, work++, sortNow ? W.sortIds[0] : nullptr, tune.refill, stream);
         if (k + 1 < L) {
             if (sortNow) sort_queue(S, W, buf ^ 1, W.qBase + k + 1, false, scratch, stream);
-            v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, sortNow ? W.sortIds[0] : nullptr, stream);
+            v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, sortNow ? W.sortIds[0] : nullptr,
+                     tune.refill, stream);
         }
     }
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_finish<false>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S,
@@ -538,8 +691,8 @@ void launch_pt_wavefront_batch(const WfState& W, const WfTuning& tune, int maxLe
         const int buf = k & 1;
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shade<true>), dim3(tune.shadeGrid), dim3(256), 0, stream, none, W, k,
                            buf, W.qBase + k, W.qBase + k + 1, W.shBase + k, work++);
-        v.shadow(none, W, tune.traceGrid, W.shBase + k, work++, nullptr, stream);
-        if (k + 1 < L) v.extend(none, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, nullptr, stream);
+        v.shadow(none, W, tune.traceGrid, W.shBase + k, work++, nullptr, tune.refill, stream);
+        if (k + 1 < L) v.extend(none, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, nullptr, tune.refill, stream);
     }
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_finish<true>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, none,
                        W);

@@ -48,6 +48,7 @@ struct WfTuning {
     int traceVariant = 5;  // 256 threads, 16 LDS stack entries
     int chunk = 256;
     int sortBounces = 0;   // extension / shadow queues of the first bounces traced in sorted order
+    int refill = 0;        // > 0: trace stages refill idle lanes every `refill` traversal steps
     int traceGrid = 0;
     int shadeGrid = 0;
 };

@@ -80,6 +80,235 @@ struct TraceHit {
     unsigned nodeVisits, primTests;
 };
 
+// Resumable traversal of one ray: trace_ray's loops (below) with their state
+// kept in a struct, so that run() can stop after a budget of steps (node
+// visits and primitive tests) and continue later. The persistent wavefront
+// kernels use it to refill lanes whose ray has finished while the rest of
+// the wave is still traversing. It takes the same steps as trace_ray, so the
+// result of a ray does not depend on how its traversal is split; trace_ray
+// stays a separate straight-line form because the resumable one needs more
+// registers.
+// kSingleMode: 1 = one instance (no instance level), 0 = instance level,
+// 2 = chosen per ray at init (batches mix both kinds of scenes).
+// The stack's deeper entries live in a private array of the caller
+// (kSpill ints), passed to run() so that the struct itself holds only
+// scalars and stays in registers.
+template <bool kAny, bool kStats, int kSingleMode, int kStride, int kLds = kLdsDepth>
+struct Traversal {
+    static constexpr int kSpill = kStackCap - kLds;
+    FastRay cur;
+    V3 wo, wd;
+    float hitT, hw0, hw1, scale;
+    int hInst, hPrim;
+    const BvhChild* bvh;
+    const float4* tris;
+    int numInternal, objPrims, curInst;
+    int nodeAddr, primCount, primOffset, sp;
+    bool inObj, overflow, done, single;
+    unsigned nodeVisits, primTests;
+
+    __device__ __forceinline__ bool is_single() const {
+        if constexpr (kSingleMode == 2)
+            return single;
+        else
+            return kSingleMode == 1;
+    }
+    __device__ __forceinline__ void push(int* lstack, int* spill, int v) {
+        if (sp < kLds)
+            lstack[sp * kStride] = v;
+        else if (sp < kStackCap)
+            spill[sp - kLds] = v;
+        if (sp < kStackCap)
+            ++sp;
+        else
+            overflow = true;
+    }
+    __device__ __forceinline__ int pop(const int* lstack, const int* spill) {
+        --sp;
+        return sp < kLds ? lstack[sp * kStride] : spill[sp - kLds];
+    }
+
+    __device__ __forceinline__ void init(const TraceParams& p, V3 o, V3 d, float tmax) {
+        wo = o;
+        wd = d;
+        sp = 0;
+        overflow = done = false;
+        cur = make_fast_ray(wo, wd);
+        hitT = tmax;
+        hInst = hPrim = -1;
+        hw0 = hw1 = 0.0f;
+        scale = 1.0f;
+        nodeVisits = primTests = 0;
+        bvh = (const BvhChild*)p.tlasNodes;
+        tris = nullptr;
+        numInternal = p.tlasNumInternal;
+        objPrims = 0;
+        inObj = false;
+        curInst = -1;
+        nodeAddr = 0;
+        primCount = 2;
+        primOffset = 0;
+        if constexpr (kSingleMode == 2) single = p.numInstances == 1;
+        if (is_single()) {
+            DevLod lod;
+            const ObjectEntry e = enter_object(p, 0, wo, wd, hitT, lod);
+            if (kStats) ++nodeVisits;
+            if (!e.hit) {
+                primCount = 0;
+            } else {
+                cur = e.ray;
+                scale = e.scale;
+                hitT = e.tmax;
+                bvh = (const BvhChild*)lod.nodes;
+                tris = lod.triVerts;
+                numInternal = lod.numInternal;
+                inObj = true;
+                curInst = 0;
+                if (lod.numPrims == 1) primCount = 1;
+            }
+        }
+    }
+
+    // Advance by at most `budget` steps (kBudget) or to the end; true when
+    // the ray is finished.
+    template <bool kBudget>
+    __device__ __forceinline__ bool run(const TraceParams& p, int* lstack, int* spill, int budget) {
+        while (true) {
+            // descend through internal nodes, nearer child first, farther pushed
+            while (nodeAddr < numInternal && primCount > 0) {
+                if (kBudget && budget-- <= 0) return false;
+                const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
+                const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+                if (kStats) ++nodeVisits;
+                const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
+                const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
+                const int lIdx = __float_as_int(q1.z), lCnt = __float_as_int(q1.w);
+                const int rIdx = __float_as_int(q3.z), rCnt = __float_as_int(q3.w);
+                float c0, c1;
+                const bool t0 = slab(lLo, lHi, cur, hitT, c0);
+                const bool t1 = slab(rLo, rHi, cur, hitT, c1);
+                if (!t0) c0 = 1e38f;
+                if (!t1) c1 = 1e38f;
+                if (t0 && t1) {
+                    const bool farLeft = c1 < c0;
+                    const int pushed = farLeft ? lIdx : rIdx;
+                    if (pushed >= numInternal) push(lstack, spill, farLeft ? lCnt : rCnt);
+                    push(lstack, spill, pushed);
+                }
+                const bool nearLeft = c0 <= c1;
+                nodeAddr = nearLeft ? lIdx : rIdx;
+                primCount = (nodeAddr >= numInternal) ? (nearLeft ? lCnt : rCnt) : 2;
+                primOffset = 0;
+                if (!t0 && !t1) primCount = 0;
+            }
+            if (overflow) return true;
+            // the primitives of a leaf, one after the other
+            bool descend = false;
+            while (nodeAddr >= numInternal && primCount > 0) {
+                if (kBudget && budget-- <= 0) return false;
+                const int leafId = nodeAddr - numInternal + primOffset;
+                ++primOffset;
+                --primCount;
+                if (!is_single() && !inObj) {
+                    const int inst = p.tlasPrimIds[leafId];
+                    DevLod lod;
+                    const ObjectEntry e = enter_object(p, inst, wo, wd, hitT, lod);
+                    if (kStats) ++nodeVisits;
+                    if (!e.hit) continue;
+                    if (lod.numPrims == 1) {
+                        // single-primitive object: test it and return to the instance level
+                        const float4 a = lod.triVerts[0], b = lod.triVerts[1], c = lod.triVerts[2];
+                        float t, w0, w1;
+                        if (kStats) ++primTests;
+                        const float objT = e.tmax;
+                        float ht = objT;
+                        if (ray_triangle(e.ray, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
+                            t > 0.0f && better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
+                            ht = t;
+                            hInst = inst;
+                            hPrim = __float_as_int(a.w);
+                            hw0 = w0;
+                            hw1 = w1;
+                            if (kAny) done = true;
+                        }
+                        hitT = ht / e.scale;
+                        if (done) return true;
+                        continue;
+                    }
+                    if (primCount > 0) {
+                        push(lstack, spill, primCount);
+                        push(lstack, spill, nodeAddr + primOffset);
+                    }
+                    push(lstack, spill, kSentinel);
+                    cur = e.ray;
+                    scale = e.scale;
+                    hitT = e.tmax;
+                    bvh = (const BvhChild*)lod.nodes;
+                    tris = lod.triVerts;
+                    numInternal = lod.numInternal;
+                    objPrims = lod.numPrims;
+                    inObj = true;
+                    curInst = inst;
+                    nodeAddr = 0;
+                    primCount = objPrims == 1 ? 1 : 2;
+                    primOffset = 0;
+                    descend = true;
+                    break;
+                }
+                const float4 a = tris[3 * (size_t)leafId], b = tris[3 * (size_t)leafId + 1],
+                             c = tris[3 * (size_t)leafId + 2];
+                float t, w0, w1;
+                const int prim = __float_as_int(a.w);
+                if (kStats) ++primTests;
+                if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
+                    t > 0.0f && better(t, curInst, prim, hitT, hInst, hPrim)) {
+                    hitT = t;
+                    hInst = curInst;
+                    hPrim = prim;
+                    hw0 = w0;
+                    hw1 = w1;
+                    if (kAny) {
+                        done = true;
+                        return true;
+                    }
+                }
+            }
+            if (overflow) return true;
+            if (descend) continue;
+            // pop the next subtree (or return from an object to the instance level)
+            if (sp == 0) return true;
+            nodeAddr = pop(lstack, spill);
+            if (!is_single() && nodeAddr == kSentinel) {
+                if (sp == 0) return true;
+                cur = make_fast_ray(wo, wd);
+                hitT /= scale;
+                scale = 1.0f;
+                bvh = (const BvhChild*)p.tlasNodes;
+                numInternal = p.tlasNumInternal;
+                inObj = false;
+                nodeAddr = pop(lstack, spill);
+            }
+            primCount = (nodeAddr >= numInternal) ? pop(lstack, spill) : 2;
+            primOffset = 0;
+        }
+    }
+
+    // On return t is the world-space distance (or the round-tripped tmax on
+    // a miss) and inst/prim are -1 on a miss.
+    __device__ __forceinline__ TraceHit result() const {
+        TraceHit h;
+        h.t = (!kAny && inObj) ? hitT / scale : hitT;  // object_to_world_space
+        h.inst = hInst;
+        h.prim = hPrim;
+        h.w0 = hw0;
+        h.w1 = hw1;
+        h.overflow = overflow;
+        h.nodeVisits = nodeVisits;
+        h.primTests = primTests;
+        return h;
+    }
+};
+
 // Traverse from (wo, wd) up to tmax. lstack: this lane's LDS stack column
 // (stride kStride ints). On return t is the world-space distance (or the
 // round-tripped tmax on a miss) and inst/prim are -1 on a miss.

@@ -61,14 +61,15 @@ def test_wavefront_instanced_and_every_tuning(hip_ctx):
     ref = hip_ctx.pt_target(pt)
     hip_ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
     for block, depth in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8), (256, 16), (256, 24)]:
-        for chunk, sort in ((64, 0), (1024, 3)):
-            hip_ctx.pt_set_tuning(pt, block, depth, chunk, sort)
+        for chunk, sort, refill in ((64, 0, 0), (1024, 3, 0), (256, 0, 16), (256, 2, 3)):
+            hip_ctx.pt_set_tuning(pt, block, depth, chunk, sort, refill)
             hip_ctx.pt_reset(pt)
             hip_ctx.pt_iterate(pt, 1)
             hip_ctx.pt_iterate(pt, 1)
             got = hip_ctx.pt_target(pt)
-            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, chunk)
-    assert hip_ctx.pt_get_tuning(pt) == dict(traceBlock=256, stackDepth=24, queueChunk=1024, sortBounces=3)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, chunk, sort, refill)
+    assert hip_ctx.pt_get_tuning(pt) == dict(traceBlock=256, stackDepth=24, queueChunk=256, sortBounces=2,
+                                             refillBudget=3)
 
 
 def test_autotune_picks_a_shape_and_resets(hip_ctx):
@@ -78,6 +79,7 @@ def test_autotune_picks_a_shape_and_resets(hip_ctx):
     best, ms = hip_ctx.pt_autotune(pt, 1)
     print("autotune best %s %.3f ms" % (best, ms), flush=True)
     assert ms > 0 and best["queueChunk"] in (256, 1024) and best["sortBounces"] in (0, 1, 2)
+    assert best["refillBudget"] in (0, 16, 32)
     assert hip_ctx.pt_iteration(pt) == -1
     hip_ctx.pt_iterate(pt, 2)
     a = hip_ctx.pt_target(pt)
