@@ -82,6 +82,15 @@ typedef struct {
     } descriptor;
 } BulkLoader;
 
+/* Sizes of one LoD as stored in a scene file (core_interface.h:77-83). */
+typedef struct {
+    uint32_t vertices;
+    uint32_t triangles;
+    uint32_t quads;
+    uint32_t edges;
+    uint32_t spheres;
+} LodMetadata;
+
 /* A light: 3-bit type (LightType) and 29-bit index (core_interface.h:205-208);
  * type 7 marks an invalid handle. */
 typedef struct {
@@ -156,6 +165,13 @@ void mufflon_destroy(MufflonInstanceHdl instHdl);
  * addressing, interface.cpp:4470-4484). */
 Boolean mufflon_is_cuda_available(void);
 int32_t mufflon_get_cuda_device_index(void);
+/* The scene-file loader's callbacks (core_interface.h:355-357): `func` loads
+ * LoD `level` of an object on demand (world_load_scenario calls it for every
+ * LoD an active instance needs that is not loaded yet), `objFunc` reads an
+ * object's unique material indices, `metaFunc` the LoD sizes. */
+Boolean mufflon_set_lod_loader(MufflonInstanceHdl instHdl, Boolean (*func)(void*, ObjectHdl, uint32_t, Boolean),
+                               Boolean (*objFunc)(void*, uint32_t, uint16_t*, uint32_t*),
+                               Boolean (*metaFunc)(void*, LodMetadata*, size_t*), void* userParams);
 /* Running mean (or variance) of a target of the current render; the buffer
  * (width * height * channels floats, row-major) is owned by the instance and
  * valid until the next call (interface.cpp:241-262). */
@@ -199,6 +215,7 @@ LodHdl object_add_lod(ObjectHdl obj, LodLevel level, Boolean asReduced);
 Boolean object_get_id(ObjectHdl obj, uint32_t* id);
 InstanceHdl world_create_instance(MufflonInstanceHdl instHdl, ObjectHdl obj, const uint32_t animationFrame);
 uint32_t world_get_instance_count(MufflonInstanceHdl instHdl, const uint32_t frame);
+InstanceHdl world_get_instance_by_index(MufflonInstanceHdl instHdl, uint32_t index, const uint32_t animationFrame);
 Boolean instance_set_transformation_matrix(MufflonInstanceHdl instHdl, InstanceHdl inst, const Mat3x4* mat,
                                            const Boolean isWorldToInst);
 Boolean instance_get_transformation_matrix(MufflonInstanceHdl instHdl, InstanceHdl inst, Mat3x4* mat);
@@ -277,6 +294,8 @@ Boolean scenario_mask_instance(ScenarioHdl scenario, InstanceHdl inst);
 LodLevel scenario_get_object_lod(ScenarioHdl scenario, ObjectHdl obj);
 Boolean scenario_set_object_lod(ScenarioHdl scenario, ObjectHdl obj, LodLevel level);
 Boolean scenario_set_instance_lod(ScenarioHdl scenario, InstanceHdl inst, LodLevel level);
+void scenario_reserve_custom_object_properties(ScenarioHdl scenario, size_t objects);
+void scenario_reserve_custom_instance_properties(ScenarioHdl scenario, size_t instances);
 IndexType scenario_get_point_light_count(ScenarioHdl scenario);
 Boolean scenario_has_envmap_light(MufflonInstanceHdl instHdl, ScenarioHdl scenario);
 Boolean scenario_add_light(MufflonInstanceHdl instHdl, ScenarioHdl scenario, LightHdl hdl);

@@ -3,6 +3,8 @@
 Every source is compiled with -ffp-contract=off (no implicit fused
 multiply-add: float results are fixed by the source), with packed FP32
 instructions disabled, and linked into mufflon_amd/lib/libmufflon_hip.so.
+The scene-file loader (host code only, zlib) is linked against it into
+mufflon_amd/lib/libmffloader.so.
 """
 import os
 import subprocess
@@ -15,6 +17,8 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_obj")
 LIB = os.path.join(PKG, "lib")
 LIBNAME = os.path.join(LIB, "libmufflon_hip.so")
+LOADERNAME = os.path.join(LIB, "libmffloader.so")
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
@@ -56,7 +60,22 @@ def build_library(verbose=False):
     os.replace(tmp, LIBNAME)
     if verbose:
         print("built", LIBNAME)
+    build_loader(verbose)
     return LIBNAME
+
+
+def build_loader(verbose=False):
+    tmp = LOADERNAME + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+           os.path.join(CSRC, "mff_loader.cpp"), "-o", tmp, "-L" + LIB, "-lmufflon_hip", "-lz",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("loader build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, LOADERNAME)
+    if verbose:
+        print("built", LOADERNAME)
+    return LOADERNAME
 
 
 if __name__ == "__main__":

@@ -48,6 +48,11 @@ class ProcessTime(C.Structure):
     _fields_ = [("cycles", C.c_uint64), ("microseconds", C.c_uint64)]
 
 
+class LodMetadata(C.Structure):
+    _fields_ = [("vertices", C.c_uint32), ("triangles", C.c_uint32), ("quads", C.c_uint32), ("edges", C.c_uint32),
+                ("spheres", C.c_uint32)]
+
+
 class LightHdl(C.Structure):
     _fields_ = [("type", C.c_uint32, 3), ("index", C.c_uint32, 29)]
 
@@ -159,6 +164,31 @@ _SIGS = {
     "object_add_lod": (vp, [vp, u32, Boolean]),
     "world_create_instance": (vp, [vp, vp, u32]),
     "world_get_instance_count": (u32, [vp, u32]),
+    "world_get_instance_by_index": (vp, [vp, u32, u32]),
+    "world_get_camera_count": (C.c_size_t, [vp]),
+    "world_get_camera_name": (cp, [vp]),
+    "world_get_camera_position": (Boolean, [vp, C.POINTER(Vec3), u32]),
+    "world_get_pinhole_camera_fov": (Boolean, [vp, C.POINTER(C.c_float)]),
+    "world_find_light": (Boolean, [vp, cp, C.POINTER(LightHdl)]),
+    "world_get_point_light_count": (C.c_size_t, [vp]),
+    "world_get_env_light_count": (C.c_size_t, [vp]),
+    "world_get_point_light_position": (Boolean, [vp, LightHdl, C.POINTER(Vec3), u32]),
+    "world_get_point_light_intensity": (Boolean, [vp, LightHdl, C.POINTER(Vec3), u32]),
+    "world_get_env_light_color": (Boolean, [vp, LightHdl, C.POINTER(Vec3)]),
+    "world_get_env_light_scale": (Boolean, [vp, LightHdl, C.POINTER(Vec3)]),
+    "world_get_scenario_count": (u32, [vp]),
+    "world_get_scenario_by_index": (vp, [vp, u32]),
+    "scenario_get_global_lod_level": (u32, [vp]),
+    "scenario_get_resolution": (Boolean, [vp, C.POINTER(u32), C.POINTER(u32)]),
+    "scenario_get_camera": (vp, [vp]),
+    "scenario_is_object_masked": (Boolean, [vp, vp]),
+    "scenario_get_object_lod": (u32, [vp, vp]),
+    "scenario_get_point_light_count": (i32, [vp]),
+    "scenario_has_envmap_light": (Boolean, [vp, vp]),
+    "object_has_lod": (Boolean, [vp, u32]),
+    "object_get_id": (Boolean, [vp, C.POINTER(u32)]),
+    "world_get_object_name": (cp, [vp]),
+    "polygon_get_face_count": (C.c_size_t, [vp]),
     "instance_set_transformation_matrix": (Boolean, [vp, vp, C.POINTER(Mat3x4), Boolean]),
     "instance_get_transformation_matrix": (Boolean, [vp, vp, C.POINTER(Mat3x4)]),
     "world_add_material": (vp, [vp, cp, C.POINTER(MaterialParams)]),
@@ -219,6 +249,24 @@ _SIGS = {
 }
 
 _core = None
+_loader = None
+
+LOADER_SUCCESS, LOADER_ERROR, LOADER_ABORT = range(3)
+_LOADER_SIGS = {
+    "loader_get_dll_error": (cp, []),
+    "loader_set_log_level": (Boolean, [i32]),
+    "loader_initialize": (vp, [vp]),
+    "loader_destroy": (None, [vp]),
+    "loader_load_json": (i32, [vp, cp]),
+    "loader_save_scene": (i32, [vp, cp]),
+    "loader_load_lod": (Boolean, [vp, vp, u32, Boolean]),
+    "loader_load_object_material_indices": (Boolean, [vp, u32, C.POINTER(C.c_uint16), C.POINTER(u32)]),
+    "loader_load_lod_metadatas": (Boolean, [vp, C.POINTER(LodMetadata), C.POINTER(C.c_size_t)]),
+    "loader_abort": (Boolean, [vp]),
+    "loader_get_loading_status": (cp, [vp]),
+    "loader_profiling_enable": (None, []),
+    "loader_profiling_disable": (None, []),
+}
 
 
 def load():
@@ -232,6 +280,24 @@ def load():
             fn.argtypes = args
         _core = lib
     return _core
+
+
+def load_loader():
+    """The scene-file loader library (libmffloader.so) with its signatures
+    bound; it links the HIP library, whose core API it calls."""
+    global _loader
+    if _loader is None:
+        load()
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmffloader.so")
+        if not os.path.exists(path):
+            raise RuntimeError("%s is not built: run __graft_entry__.build()" % path)
+        lib = C.CDLL(path, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in _LOADER_SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _loader = lib
+    return _loader
 
 
 def _b(s):
@@ -258,6 +324,9 @@ class CoreInterface:
             raise RuntimeError(self.core_get_dll_error())
 
     def close(self):
+        if getattr(self, "_loader_inst", None):
+            self.mffLoader.loader_destroy(self._loader_inst)
+            self._loader_inst = None
         if self.muffInst:
             self.lib.mufflon_destroy(self.muffInst)
             self.muffInst = None
@@ -271,6 +340,40 @@ class CoreInterface:
     def core_get_dll_error(self):
         return self.lib.core_get_dll_error().decode()
 
+    # the scene-file loader bound to this instance (bindings.py:312-347)
+    def loader(self):
+        if getattr(self, "_loader_inst", None) is None:
+            self.mffLoader = load_loader()
+            self._loader_inst = self.mffLoader.loader_initialize(self.muffInst)
+            if not self._loader_inst:
+                raise RuntimeError(self.loader_get_dll_error())
+        return self._loader_inst
+
+    def loader_get_dll_error(self):
+        return load_loader().loader_get_dll_error().decode()
+
+    def loader_load_json(self, path):
+        inst = self.loader()
+        return self.mffLoader.loader_load_json(inst, _b(path))
+
+    def loader_load_lod_metadatas(self, capacity=4096):
+        buf = (LodMetadata * capacity)()
+        n = C.c_size_t()
+        if not self.mffLoader.loader_load_lod_metadatas(self.loader(), buf, C.byref(n)):
+            raise RuntimeError(self.loader_get_dll_error())
+        return [(m.vertices, m.triangles, m.quads, m.edges, m.spheres) for m in buf[: n.value]]
+
+    def loader_load_object_material_indices(self, obj_id, capacity=65536):
+        buf = (C.c_uint16 * capacity)()
+        n = u32()
+        if not self.mffLoader.loader_load_object_material_indices(self.loader(), obj_id, buf, C.byref(n)):
+            raise RuntimeError(self.loader_get_dll_error())
+        return list(buf[: n.value])
+
+    def loader_get_loading_status(self):
+        s = self.mffLoader.loader_get_loading_status(self.loader())
+        return None if s is None else s.decode()
+
     # world
     def world_clear_all(self):
         self.lib.world_clear_all(self.muffInst)
@@ -282,6 +385,9 @@ class CoreInterface:
 
     def world_create_object(self, name, flags=0):
         return self.lib.world_create_object(self.muffInst, _b(name), flags)
+
+    def world_get_object(self, name):
+        return self.lib.world_get_object(self.muffInst, _b(name))
 
     def object_add_lod(self, obj, level=0):
         return self.lib.object_add_lod(obj, level, 0)
@@ -525,13 +631,15 @@ def _material_params(core, m, tex_handles):
     return p
 
 
-def build_world(core, scene, texture_dir, background=None, name="scene"):
+def build_world(core, scene, texture_dir, background=None, name="scene", vfov=None):
     """Create `scene` (a mufflon_amd.scenes dict with one LoD: verts, tris,
     normals, uvs, mat_idx, materials, camera, optional textures /
     point_lights / envmap) through the core API, the way a scene loader does,
     with textures written as PFM files into texture_dir. background: None,
     ("colored", color, scale) or ("envmap", scale) with the scene's envmap
-    texture. Returns the scenario handle (finalized, not loaded)."""
+    texture. vfov: the camera's vertical field of view in radians (default
+    from the camera's tan_vfov). Returns the scenario handle (finalized, not
+    loaded)."""
     textures = scene.get("textures") or []
     handles = {}
     for i, (texels, sampling) in enumerate(textures):
@@ -564,7 +672,8 @@ def build_world(core, scene, texture_dir, background=None, name="scene"):
             raise RuntimeError(core.core_get_dll_error())
     core.world_create_instance(obj)
     cam = scene["camera"]
-    vfov = 2.0 * np.arctan(float(cam["tan_vfov"]))
+    if vfov is None:
+        vfov = 2.0 * np.arctan(float(cam["tan_vfov"]))
     c = core.world_add_pinhole_camera("camera", cam["position"], cam["view_dir"], cam["up"], 1e-4, 1e10,
                                       float(np.float32(vfov)))
     core.scenario_set_camera(scen, c)

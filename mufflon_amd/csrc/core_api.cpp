@@ -226,6 +226,11 @@ struct Core {
     int32_t iteration = -1;  // OutputHandler::m_iteration
     std::vector<float> screen;
     int screenChannels = 0;
+    // the scene-file loader's on-demand LoD callback (mufflon_set_lod_loader)
+    Boolean (*lodLoader)(void*, ObjectHdl, uint32_t, Boolean) = nullptr;
+    Boolean (*matIndexLoader)(void*, uint32_t, uint16_t*, uint32_t*) = nullptr;
+    Boolean (*lodMetaLoader)(void*, LodMetadata*, size_t*) = nullptr;
+    void* loaderParams = nullptr;
 
     ~Core() { release_device(); }
     void release_device() {
@@ -456,12 +461,16 @@ void resolve_textures(Material& m, const std::map<const Texture*, int32_t>& ids)
 }
 
 // ------------------------------------------------------------------ loading
+// Scenario::get_effective_lod (scenario.cpp:131-154): instance LoD, else
+// object LoD, else the global one, clamped to the object's last LoD slot.
 LodLevel lod_level(const Scenario& s, const ObjInstance& inst) {
+    LodLevel level = s.globalLod;
     auto ii = s.instanceLod.find(&inst);
-    if (ii != s.instanceLod.end()) return ii->second;
     auto oi = s.objectLod.find(inst.object);
-    if (oi != s.objectLod.end()) return oi->second;
-    return s.globalLod;
+    if (ii != s.instanceLod.end()) level = ii->second;
+    else if (oi != s.objectLod.end()) level = oi->second;
+    const size_t slots = inst.object->lods.size();
+    return slots > 0 ? std::min<LodLevel>(level, (LodLevel)(slots - 1)) : level;
 }
 
 bool instance_active(const Core& c, const Scenario& s, const ObjInstance& inst) {
@@ -521,6 +530,20 @@ void apply_mode(Core& c) {
 // whole scenario onto the device.
 void load_scene(Core& c, Scenario& s) {
     CORE_REQUIRE(s.camera != nullptr, "scenario '" + s.name + "' has no camera");
+    // LoDs that active instances need and that are not loaded yet come from the
+    // scene-file loader (WorldContainer::load_lod, world_container.cpp:684-693),
+    // before anything touches the device
+    if (c.lodLoader) {
+        for (const auto& ip : c.instances) {
+            if (!instance_active(c, s, *ip)) continue;
+            Object& obj = *ip->object;
+            const LodLevel level = lod_level(s, *ip);
+            if (level < obj.lods.size() && obj.lods[level]) continue;
+            CORE_REQUIRE(c.lodLoader(c.loaderParams, &obj, level, false),
+                         "the scene loader failed to load LoD " + std::to_string(level) + " of object '" + obj.name +
+                             "'");
+        }
+    }
     c.release_device();
     check_mfh(mfh_create(0, &c.ctx), "device context");
     // textures used by the materials and the background
@@ -833,6 +856,20 @@ Boolean mufflon_is_cuda_available() {
 }
 
 int32_t mufflon_get_cuda_device_index() { return mufflon_is_cuda_available() ? 0 : -1; }
+
+Boolean mufflon_set_lod_loader(MufflonInstanceHdl instHdl, Boolean (*func)(void*, ObjectHdl, uint32_t, Boolean),
+                               Boolean (*objFunc)(void*, uint32_t, uint16_t*, uint32_t*),
+                               Boolean (*metaFunc)(void*, LodMetadata*, size_t*), void* userParams) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        CORE_REQUIRE(func != nullptr, "Invalid LoD loader function (nullptr)");
+        c.lodLoader = func;
+        c.matIndexLoader = objFunc;
+        c.lodMetaLoader = metaFunc;
+        c.loaderParams = userParams;
+        return true;
+    });
+}
 
 Boolean mufflon_get_target_image(MufflonInstanceHdl instHdl, const char* name, Boolean variance, const float** ptr) {
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
@@ -1195,6 +1232,16 @@ uint32_t world_get_instance_count(MufflonInstanceHdl instHdl, const uint32_t fra
         uint32_t n = 0;
         for (const auto& i : c.instances) n += i->frame == frame ? 1 : 0;
         return n;
+    });
+}
+
+InstanceHdl world_get_instance_by_index(MufflonInstanceHdl instHdl, uint32_t index, const uint32_t animationFrame) {
+    return guard(__func__, (InstanceHdl) nullptr, [&]() -> InstanceHdl {
+        Core& c = core_of(instHdl);
+        uint32_t n = 0;
+        for (const auto& i : c.instances)
+            if (i->frame == animationFrame && n++ == index) return i.get();
+        throw ApiError("Instance index '" + std::to_string(index) + "' out of bounds (" + std::to_string(n) + ")");
     });
 }
 
@@ -1779,6 +1826,23 @@ Boolean scenario_set_instance_lod(ScenarioHdl scenario, InstanceHdl inst, LodLev
         scenario_of(scenario).instanceLod[static_cast<ObjInstance*>(inst)] = level;
         scenario_changed(static_cast<Scenario*>(scenario));
         return true;
+    });
+}
+
+// Capacity hints of the reference's hash maps; nothing to reserve here.
+void scenario_reserve_custom_object_properties(ScenarioHdl scenario, size_t objects) {
+    (void)objects;
+    guard(__func__, 0, [&] {
+        (void)scenario_of(scenario);
+        return 0;
+    });
+}
+
+void scenario_reserve_custom_instance_properties(ScenarioHdl scenario, size_t instances) {
+    (void)instances;
+    guard(__func__, 0, [&] {
+        (void)scenario_of(scenario);
+        return 0;
     });
 }
 

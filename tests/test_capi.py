@@ -11,14 +11,14 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
+def _declared(headers=None):
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in headers or glob.glob(os.path.join(ROOT, "include", "*.h")):
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mfh_\w+|mufflon_\w+|render_\w+|renderer_\w+|"
                              r"world_\w+|core_\w+|polygon_\w+|object_\w+|instance_\w+|scenario_\w+|scene_\w+|"
-                             r"profiling_\w+)"
+                             r"profiling_\w+|loader_\w+)"
                              r"\s*\(", txt, flags=re.M):
             names.add(m.group(1))
     return sorted(names)
@@ -39,7 +39,18 @@ def test_headers_declare_entry_points():
 
 
 def test_every_declared_symbol_is_exported(lib):
-    missing = [n for n in _declared() if not hasattr(lib, n)]
+    loader = os.path.join(ROOT, "include", "mff_loader.h")
+    names = _declared([h for h in glob.glob(os.path.join(ROOT, "include", "*.h")) if h != loader])
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_scene_loader_exports_its_header():
+    from mufflon_amd import core
+    ldr = core.load_loader()
+    names = _declared([os.path.join(ROOT, "include", "mff_loader.h")])
+    assert "loader_load_json" in names and "loader_load_lod" in names
+    missing = [n for n in names if not hasattr(ldr, n)]
     assert not missing, missing
 
 
