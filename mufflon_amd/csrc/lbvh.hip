@@ -273,26 +273,32 @@ __device__ __forceinline__ void merge_node(int node, const i32* __restrict__ chi
     flags[node] = (uint8_t)((C::capped(c) ? 1 : 0) | (C::cost(c) < cost ? 2 : 0));
 }
 
-// Refit, phase 1. Block b owns the leaves [b*kRefitTile, (b+1)*kRefitTile).
-// Every internal node whose leaf range lies inside that span has both
-// subtrees finished by threads of the same workgroup, so the arrival
-// protocol needs only workgroup-scope ordering (no L2 write-back or
-// invalidate across XCDs). A finished subtree whose parent straddles the
-// span is appended to `pending` for phase 2.
-constexpr int kRefitThreads = 256;
-constexpr int kRefitIpt = 8;
+// Refit, level 0. Block b owns the leaves [b*kRefitTile, (b+1)*kRefitTile).
+// A Karras internal node is one end of its leaf range, so every internal
+// node whose range lies inside the tile has its index inside the tile too:
+// the arrival counters of those nodes live in LDS, and both subtrees of such
+// a node are finished by threads of this workgroup, so the arrival protocol
+// needs only workgroup-scope ordering (no L2 write-back or invalidate across
+// XCDs). A finished subtree whose parent straddles the tile is appended to
+// `pending` for the next level.
+constexpr int kRefitThreads = 1024;
+constexpr int kRefitIpt = 2;
 constexpr int kRefitTile = kRefitThreads * kRefitIpt;
+constexpr int kSpanGrowth = 64;  // each later level covers 64 spans of the one before
 
 template <class P, class C>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, const i32* __restrict__ ids,
                                                               const i32* __restrict__ parents,
                                                               const i32* __restrict__ children,
                                                               const int2* __restrict__ ranges, float4* boxes,
-                                                              u32* counters, uint8_t* flags, float4* triVerts,
-                                                              i32* pending, u32* pendingCount) {
+                                                              uint8_t* flags, float4* triVerts, i32* pending,
+                                                              u32* pendingCount) {
+    __shared__ u32 sArrived[kRefitTile];
     const int nInt = n - 1;
     const int first = blockIdx.x * kRefitTile;
     const int last = min(first + kRefitTile, n) - 1;
+    for (int k = threadIdx.x; k < kRefitTile; k += kRefitThreads) sArrived[k] = 0u;
+    __syncthreads();
     for (int k = 0; k < kRefitIpt; ++k) {
         const int idx = first + k * kRefitThreads + (int)threadIdx.x;
         if (idx > last) break;
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, con
                 break;
             }
             __threadfence_block();
-            if (atomicAdd(&counters[node], 1u) == 0) break;  // sibling subtree not finished yet
+            if (atomicAdd(&sArrived[node - first], 1u) == 0) break;  // sibling subtree not finished yet
             __threadfence_block();
             merge_node<C>(node, children, boxes, flags);
             if (node == 0) break;
@@ -334,24 +340,43 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, con
     }
 }
 
-// Refit, phase 2: one workgroup climbs from every pending subtree root
-// through the nodes that straddle tile borders (their number is about the
-// tile count times the tree depth), with workgroup-scope ordering again.
-constexpr int kTopThreads = 1024;
+// Refit, later levels. Block b owns the leaves [b*span, (b+1)*span) and
+// climbs from the pending subtree roots of the previous level whose leaves
+// lie in its span, through the nodes inside the span (both children of such
+// a node arrive in this level: the previous one stopped below it), with
+// workgroup-scope ordering again; roots whose parent straddles the span go
+// to `pendOut`. The level whose span covers all leaves is one block and
+// ends at the root.
+constexpr int kSpanThreads = 1024;
 template <class C>
-__global__ __launch_bounds__(kTopThreads) void k_refit_top(const i32* __restrict__ parents,
-                                                          const i32* __restrict__ children, float4* boxes,
-                                                          u32* counters, uint8_t* flags, const i32* pending,
-                                                          const u32* pendingCount) {
-    const int count = (int)*pendingCount;
-    for (int k = threadIdx.x; k < count; k += kTopThreads) {
-        int node = parent_of(parents, pending[k]);
+__global__ __launch_bounds__(kSpanThreads) void k_refit_span(int n, long long span, const i32* __restrict__ parents,
+                                                             const i32* __restrict__ children,
+                                                             const int2* __restrict__ ranges, float4* boxes,
+                                                             u32* counters, uint8_t* flags,
+                                                             const i32* __restrict__ pendIn,
+                                                             const u32* __restrict__ countIn, i32* pendOut,
+                                                             u32* countOut) {
+    const int nInt = n - 1;
+    const long long lo = (long long)blockIdx.x * span;
+    const long long hi = min(lo + span, (long long)n) - 1;
+    const int count = (int)*countIn;
+    for (int k = threadIdx.x; k < count; k += kSpanThreads) {
+        int child = pendIn[k];
+        const int start = child >= nInt ? child - nInt : ranges[child].x;
+        if (start < lo || start > hi) continue;
+        int node = parent_of(parents, child);
         for (int depth = 0; depth < kMaxDepth; ++depth) {
+            const int2 r = ranges[node];
+            if (r.x < lo || r.y > hi) {
+                pendOut[atomicAdd(countOut, 1u)] = child;
+                break;
+            }
             __threadfence_block();
             if (atomicAdd(&counters[node], 1u) == 0) break;
             __threadfence_block();
             merge_node<C>(node, children, boxes, flags);
             if (node == 0) break;
+            child = node;
             node = parent_of(parents, node);
         }
     }
@@ -452,6 +477,7 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
     s.children.reserve(8 * (size_t)std::max(n - 1, 1));
     s.ranges.reserve(8 * (size_t)std::max(n - 1, 1));
     s.pending.reserve(4 * (size_t)n + 4);
+    s.pending2.reserve(4 * (size_t)n + 4);
     s.boxes.reserve(32 * (size_t)(2 * n - 1));
     s.counters.reserve(4 * (size_t)std::max(n - 1, 1));
     s.flags.reserve((size_t)std::max(n - 1, 1));
@@ -482,15 +508,23 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
     }
     i32* pending = s.pending.as<i32>();
     u32* pendingCount = reinterpret_cast<u32*>(pending + n);
+    i32* pending2 = s.pending2.as<i32>();
+    u32* pendingCount2 = reinterpret_cast<u32*>(pending2 + n);
     MFH_HIP(hipMemsetAsync(pendingCount, 0, 4, st));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_tiles<P, C>), dim3((n + kRefitTile - 1) / kRefitTile),
                        dim3(kRefitThreads), 0, st, set, n, ids, parents, children, s.ranges.as<int2>(), boxes,
-                       s.counters.as<u32>(), s.flags.as<uint8_t>(), triVerts, pending, pendingCount);
+                       s.flags.as<uint8_t>(), triVerts, pending, pendingCount);
     MFH_HIP(hipGetLastError());
-    if (n > 1) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_top<C>), dim3(1), dim3(kTopThreads), 0, st, parents, children,
-                           boxes, s.counters.as<u32>(), s.flags.as<uint8_t>(), pending, pendingCount);
+    for (long long span = kRefitTile; n > 1 && span < n;) {
+        span *= kSpanGrowth;
+        const int blocks = (int)((n + span - 1) / span);
+        MFH_HIP(hipMemsetAsync(pendingCount2, 0, 4, st));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_span<C>), dim3(blocks), dim3(kSpanThreads), 0, st, n, span,
+                           parents, children, s.ranges.as<int2>(), boxes, s.counters.as<u32>(),
+                           s.flags.as<uint8_t>(), pending, pendingCount, pending2, pendingCount2);
         MFH_HIP(hipGetLastError());
+        std::swap(pending, pending2);
+        std::swap(pendingCount, pendingCount2);
     }
     if (n == 1) {
         MFH_HIP(hipStreamSynchronize(st));

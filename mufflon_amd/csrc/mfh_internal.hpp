@@ -66,7 +66,7 @@ struct BuildScratch {
     DevBuf parents;                   // 2n-1 (reference encoding, left child as ~parent)
     DevBuf children;                  // 2(n-1): left, right node of each internal node
     DevBuf ranges;                    // n-1 int2: first and last leaf under each internal node
-    DevBuf pending;                   // n+1: roots finished by refit phase 1, then their count
+    DevBuf pending, pending2;         // n+1 each: subtree roots handed to the next refit level, then their count
     DevBuf boxes;                     // (2n-1) * 32 bytes: {lo, countBits}, {hi, cost}
     DevBuf counters;                  // n-1 arrival counters
     DevBuf flags;                     // n-1 bytes: bit0 count capped, bit1 leaf cheaper than split
