@@ -346,6 +346,25 @@ int mfh_background_build(const mfh_background* bg, const float* envTexels, int32
  * resets it. The default background is COLORED black. */
 int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* bg);
 
+/* Wireframe renderer over a path tracer's scene, camera and background: the
+ * GPU device of Mufflon's "Wireframe"/"WF" renderer (GpuWireframe,
+ * gpu_wireframe.cpp:18-50, gpu_wireframe.cu:19-51, wireframe_common.hpp:94-181).
+ * Each iteration draws one seed per pixel from the renderer's own PCG64
+ * (default-constructed when the path tracer is created; resets do not
+ * re-seed it), jitters the camera ray as the path tracer does, and walks the
+ * ray through the scene (at most maxTraceDepth hits): a hit whose closest
+ * triangle edge lies within lineWidth pixel footprints of the camera ray
+ * scores the border colour 1, a walk that leaves the scene scores the
+ * background radiance if positive. The per-iteration values go into the
+ * running mean of the Border target (3 floats per pixel, row-major) and,
+ * when recorded, its variance (the output handler's recursion).
+ * _reset clears the Border target (call it after changing the camera);
+ * _get copies the mean (variance = 0) or the variance (1) to host memory. */
+int mfh_pt_wireframe_reset(mfh_context* ctx, int32_t ptId, int32_t recordVariance);
+int mfh_pt_wireframe_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations, float lineWidth,
+                             int32_t maxTraceDepth);
+int mfh_pt_wireframe_get(mfh_context* ctx, int32_t ptId, int32_t variance, float* hostOut);
+
 /* Adjoint (path replay) of the Radiance target. No counterpart in the
  * reference (it has no differentiable renderer). For the image of
  * `iterations` iterations from a reset, and radianceAdjoint = dLoss/dRadiance

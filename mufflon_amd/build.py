@@ -22,7 +22,8 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "pt_adjoint.hip", "capi.cpp",
+SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "pt_adjoint.hip", "wireframe.hip",
+           "capi.cpp",
            "light_tree.cpp", "pt_host.cpp", "core_api.cpp"]
 
 COMMON = [

@@ -98,34 +98,6 @@ struct TriPrims {
     __device__ __forceinline__ bool masked(i32) const { return false; }
 };
 
-// Inverse of a worldToInstance transform (descriptors.hpp:108-129: adjugate
-// of the 3x3 part over its determinant, translation -(inv * t)).
-__device__ __forceinline__ M34 instance_to_world(const float* w) {
-    const float r00 = w[0], r01 = w[1], r02 = w[2];
-    const float r10 = w[4], r11 = w[5], r12 = w[6];
-    const float r20 = w[8], r21 = w[9], r22 = w[10];
-    const float m00 = r11 * r22 - r21 * r12;
-    const float m01 = r21 * r02 - r01 * r22;
-    const float m02 = r01 * r12 - r11 * r02;
-    const float m10 = r20 * r12 - r10 * r22;
-    const float m11 = r00 * r22 - r20 * r02;
-    const float m12 = r10 * r02 - r00 * r12;
-    const float m20 = r10 * r21 - r20 * r11;
-    const float m21 = r20 * r01 - r00 * r21;
-    const float m22 = r00 * r11 - r10 * r01;
-    const float det = r00 * m00 + r01 * m10 + r02 * m20;
-    const float s = 1.0f / det;
-    M34 o;
-    o.m[0] = s * m00; o.m[1] = s * m01; o.m[2] = s * m02;
-    o.m[4] = s * m10; o.m[5] = s * m11; o.m[6] = s * m12;
-    o.m[8] = s * m20; o.m[9] = s * m21; o.m[10] = s * m22;
-    const float tx = w[3], ty = w[7], tz = w[11];
-    o.m[3] = -(o.m[0] * tx + o.m[1] * ty + o.m[2] * tz);
-    o.m[7] = -(o.m[4] * tx + o.m[5] * ty + o.m[6] * tz);
-    o.m[11] = -(o.m[8] * tx + o.m[9] * ty + o.m[10] * tz);
-    return o;
-}
-
 struct InstPrims {
     const float* worldToInst;  // 12 per instance
     const i32* instLod;
@@ -138,14 +110,14 @@ struct InstPrims {
     __device__ __forceinline__ V3 centroid(i32 p) const {
         V3 lo, hi;
         obox(p, lo, hi);
-        const M34 M = instance_to_world(worldToInst + 12 * (size_t)p);
+        const M34 M = inst_to_world(worldToInst + 12 * (size_t)p);
         return xform_point(M, (lo + hi) * 0.5f);
     }
     // Transform of the 8 corners, folded in corner order x-fastest.
     __device__ __forceinline__ void box(i32 p, V3& rlo, V3& rhi) const {
         V3 lo, hi;
         obox(p, lo, hi);
-        const M34 M = instance_to_world(worldToInst + 12 * (size_t)p);
+        const M34 M = inst_to_world(worldToInst + 12 * (size_t)p);
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const V3 q = xform_point(M, v3((c & 1) ? hi.x : lo.x, (c & 2) ? hi.y : lo.y, (c & 4) ? hi.z : lo.z));

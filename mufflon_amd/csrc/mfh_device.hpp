@@ -58,6 +58,34 @@ __device__ __forceinline__ V3 xform_point(const M34& M, V3 p) {
     return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
             m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
 }
+// Inverse of a worldToInstance transform (descriptors.hpp:108-129: adjugate
+// of the 3x3 part over its determinant, translation -(inv * t)).
+__device__ __forceinline__ M34 inst_to_world(const float* w) {
+    const float r00 = w[0], r01 = w[1], r02 = w[2];
+    const float r10 = w[4], r11 = w[5], r12 = w[6];
+    const float r20 = w[8], r21 = w[9], r22 = w[10];
+    const float m00 = r11 * r22 - r21 * r12;
+    const float m01 = r21 * r02 - r01 * r22;
+    const float m02 = r01 * r12 - r11 * r02;
+    const float m10 = r20 * r12 - r10 * r22;
+    const float m11 = r00 * r22 - r20 * r02;
+    const float m12 = r10 * r02 - r00 * r12;
+    const float m20 = r10 * r21 - r20 * r11;
+    const float m21 = r20 * r01 - r00 * r21;
+    const float m22 = r00 * r11 - r10 * r01;
+    const float det = r00 * m00 + r01 * m10 + r02 * m20;
+    const float s = 1.0f / det;
+    M34 o;
+    o.m[0] = s * m00; o.m[1] = s * m01; o.m[2] = s * m02;
+    o.m[4] = s * m10; o.m[5] = s * m11; o.m[6] = s * m12;
+    o.m[8] = s * m20; o.m[9] = s * m21; o.m[10] = s * m22;
+    const float tx = w[3], ty = w[7], tz = w[11];
+    o.m[3] = -(o.m[0] * tx + o.m[1] * ty + o.m[2] * tz);
+    o.m[7] = -(o.m[4] * tx + o.m[5] * ty + o.m[6] * tz);
+    o.m[11] = -(o.m[8] * tx + o.m[9] * ty + o.m[10] * tz);
+    return o;
+}
+
 __device__ __forceinline__ V3 xform_dir(const M34& M, V3 d) {
     const float* m = M.m;
     return {m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,

@@ -24,6 +24,7 @@ constexpr int kShadowingSmith = 1;
 // Render targets: Radiance, Hit ID, Position, Depth, Normal, UV, Albedo, Lightness
 constexpr int kNumTargets = 8;
 constexpr int kTargetChannels[kNumTargets] = {3, 2, 3, 1, 3, 2, 3, 1};
+constexpr int kTargetRadiance = 0;  // MFH_TARGET_RADIANCE
 
 constexpr uint16_t kLightPoint = 0;
 constexpr uint16_t kLightAreaTriangle = 2;
@@ -111,5 +112,9 @@ struct PtScene {
 
 void launch_pt_init_rng(uint64_t* rngs, int num, int seed, hipStream_t stream);
 void launch_pt_iterate(const PtScene& S, hipStream_t stream);
+// One iteration of the wireframe renderer into S.cum[MFH_TARGET_RADIANCE]
+// (and S.var[...] when recorded), S.iteration the iteration index.
+void launch_wireframe(const PtScene& S, const uint32_t* seeds, float lineWidth, int maxTraceDepth,
+                      hipStream_t stream);
 
 }  // namespace mfh

@@ -41,6 +41,15 @@ struct PathTracer {
     DevBuf adjRngs, adjDelta, adjEmGrad, adjTexGradPtrs;
     std::vector<DevBuf> adjTexGrad;  // per texture id
     bool adjValid = false;
+    // wireframe renderer over the same scene and camera (GpuWireframe,
+    // gpu_wireframe.cpp:18-50): Border running mean (and variance), the
+    // per-pixel seeds of an iteration and the renderer's generator, a
+    // default-constructed PCG64 that no reset re-seeds
+    DevBuf wireBorder, wireVar, wireSeeds;
+    std::vector<uint32_t> wireSeedHost;
+    uint64_t wireRng = 0x2f0ae9bc6431af73ull;
+    int32_t wireIteration = -1;
+    bool wireVariance = false;
 };
 
 void drop_graph(PathTracer& t) {
@@ -293,6 +302,16 @@ PtScene device_scene(Context& c, PathTracer& t) {
     S.varianceMask = t.varianceMask;
     S.nanCounter = t.dNan.as<uint32_t>();
     return S;
+}
+
+// PCG64::next (rng.hpp:83-90) on the host, for the wireframe seeds.
+uint64_t pcg64_next(uint64_t& state) {
+    uint64_t x = state;
+    const uint64_t count = state >> 59;
+    state = state * 6364136223846793005ull + 1442695040888963407ull;
+    x ^= x >> (5 + count);
+    x *= 12605985483714917081ull;
+    return x ^ (x >> 43);
 }
 
 void reset(Context& c, PathTracer& t) {
@@ -579,6 +598,76 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations) {
             S.iteration = ++t.iteration;
             launch_pt_iterate(S, c.stream);
         }
+    });
+}
+
+int mfh_pt_wireframe_reset(mfh_context* ctx, int32_t ptId, int32_t recordVariance) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        t.wireIteration = -1;
+        t.wireVariance = recordVariance != 0;
+        const size_t num = (size_t)t.camera.width * t.camera.height;
+        t.wireBorder.reserve(12 * num);
+        MFH_HIP(hipMemsetAsync(t.wireBorder.ptr, 0, 12 * num, c.stream));
+        if (t.wireVariance) {
+            t.wireVar.reserve(12 * num);
+            MFH_HIP(hipMemsetAsync(t.wireVar.ptr, 0, 12 * num, c.stream));
+        } else {
+            t.wireVar.release();
+        }
+    });
+}
+
+int mfh_pt_wireframe_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations, float lineWidth,
+                             int32_t maxTraceDepth) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(iterations >= 0, "negative iteration count");
+        MFH_REQUIRE(maxTraceDepth >= 0, "negative maximum trace depth");
+        const size_t num = (size_t)t.camera.width * t.camera.height;
+        if (t.wireBorder.bytes < 12 * num || (t.wireVariance && t.wireVar.bytes < 12 * num)) {
+            // first use, or the resolution grew: a fresh Border target
+            t.wireIteration = -1;
+            t.wireBorder.reserve(12 * num);
+            MFH_HIP(hipMemsetAsync(t.wireBorder.ptr, 0, 12 * num, c.stream));
+            if (t.wireVariance) {
+                t.wireVar.reserve(12 * num);
+                MFH_HIP(hipMemsetAsync(t.wireVar.ptr, 0, 12 * num, c.stream));
+            }
+        }
+        PtScene S = device_scene(c, t);
+        for (int k = 0; k < kNumTargets; ++k) S.cum[k] = S.var[k] = nullptr;
+        S.cum[MFH_TARGET_RADIANCE] = t.wireBorder.as<float>();
+        S.var[MFH_TARGET_RADIANCE] = t.wireVariance ? t.wireVar.as<float>() : nullptr;
+        S.targetMask = 1u << MFH_TARGET_RADIANCE;
+        S.varianceMask = t.wireVariance ? S.targetMask : 0u;
+        t.wireSeedHost.resize(num);
+        t.wireSeeds.reserve(4 * num);
+        for (int32_t it = 0; it < iterations; ++it) {
+            // GpuWireframe::iterate (gpu_wireframe.cpp:33-48): one seed per pixel
+            for (size_t i = 0; i < num; ++i) t.wireSeedHost[i] = (uint32_t)pcg64_next(t.wireRng);
+            // the copy is ordered on the stream before the kernel that reads it;
+            // the host vector is reused only after the stream has drained
+            MFH_HIP(hipMemcpyAsync(t.wireSeeds.ptr, t.wireSeedHost.data(), 4 * num, hipMemcpyHostToDevice, c.stream));
+            S.iteration = ++t.wireIteration;
+            launch_wireframe(S, t.wireSeeds.as<uint32_t>(), lineWidth, maxTraceDepth, c.stream);
+            MFH_HIP(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int mfh_pt_wireframe_get(mfh_context* ctx, int32_t ptId, int32_t variance, float* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output pointer");
+        MFH_REQUIRE(!variance || t.wireVariance, "the Border variance is not recorded");
+        const size_t num = (size_t)t.camera.width * t.camera.height;
+        MFH_REQUIRE(t.wireBorder.bytes >= 12 * num, "the wireframe renderer has not run");
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipMemcpy(out, (variance ? t.wireVar : t.wireBorder).ptr, 12 * num, hipMemcpyDeviceToHost));
     });
 }
 

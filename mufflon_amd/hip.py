@@ -79,6 +79,9 @@ _SIGS = {
     "mfh_pt_set_params": [vp, i32, vp],
     "mfh_pt_set_camera": [vp, i32, vp],
     "mfh_pt_iterate": [vp, i32, i32],
+    "mfh_pt_wireframe_reset": [vp, i32, i32],
+    "mfh_pt_wireframe_iterate": [vp, i32, i32, C.c_float, i32],
+    "mfh_pt_wireframe_get": [vp, i32, i32, vp],
     "mfh_pt_get_target": [vp, i32, i32, f32p],
     "mfh_pt_enable_target": [vp, i32, i32, i32],
     "mfh_pt_enable_variance": [vp, i32, i32, i32],
@@ -439,6 +442,20 @@ class HipContext:
 
     def pt_iterate(self, pt, iterations=1):
         self._call("mfh_pt_iterate", self.handle, pt, int(iterations))
+
+    def wireframe_reset(self, pt, variance=False):
+        self._call("mfh_pt_wireframe_reset", self.handle, pt, 1 if variance else 0)
+
+    def wireframe_iterate(self, pt, iterations=1, line_width=1.0, max_trace_depth=1000):
+        self._call("mfh_pt_wireframe_iterate", self.handle, pt, int(iterations), float(line_width),
+                   int(max_trace_depth))
+
+    def wireframe_border(self, pt, variance=False):
+        """The Border target (height, width, 3) of the wireframe renderer."""
+        w, h = self._pt_size[pt]
+        out = np.empty(w * h * 3, np.float32)
+        self._call("mfh_pt_wireframe_get", self.handle, pt, 1 if variance else 0, out.ctypes.data)
+        return out.reshape(h, w, 3)
 
     def batch_create(self, pts, use_graph=False):
         ids = np.ascontiguousarray(pts, np.int32)

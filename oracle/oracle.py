@@ -524,6 +524,28 @@ def pt_render_targets(pt_scene, seed, min_path_length, max_path_length, nee_coun
     return means, nans.value
 
 
+WIREFRAME_RNG_START = 0x2f0ae9bc6431af73  # PCG64() (rng.hpp:72-74)
+
+
+def wireframe_render(pt_scene, iterations, line_width=1.0, max_trace_depth=1000, rng_state=WIREFRAME_RNG_START,
+                     variance=False):
+    """Oracle wireframe renderer: (Border mean (h, w, 3), Border variance or
+    None, the renderer generator's state afterwards)."""
+    lib = load()
+    if not hasattr(lib, "_wire_ready"):
+        lib.orc_wireframe_render.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_float, C.c_int32, u64p, f32p,
+                                             f32p]
+        lib.orc_wireframe_render.restype = C.c_int
+        lib._wire_ready = True
+    w, h = pt_scene.s.width, pt_scene.s.height
+    border = np.zeros(w * h * 3, np.float32)
+    var = np.zeros(w * h * 3, np.float32) if variance else None
+    state = np.array([rng_state], np.uint64)
+    lib.orc_wireframe_render(C.byref(pt_scene.s), iterations, line_width, max_trace_depth, _p(state, u64p),
+                             _p(border, f32p), _p(var, f32p) if variance else None)
+    return border.reshape(h, w, 3), (var.reshape(h, w, 3) if variance else None), int(state[0])
+
+
 def pt_render_replay(base_scene, eval_scene, seed, min_path_length, max_path_length, nee_count, iterations):
     """Radiance (h, w, 3) of eval_scene rendered along the BSDF directions the
     same paths sampled in base_scene (the estimator whose roughness

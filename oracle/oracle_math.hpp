@@ -83,6 +83,33 @@ inline V3 xform_normal(const M34& M, V3 v) {
             m[1] * v.x + m[5] * v.y + m[9] * v.z,
             m[2] * v.x + m[6] * v.y + m[10] * v.z};
 }
+// Inverse of a worldToInstance transform: descriptors.hpp:108-129 (adjugate
+// of the 3x3 part over its determinant; translation -(inv * t)).
+inline M34 instance_to_world(const float* w) {
+    const float r00 = w[0], r01 = w[1], r02 = w[2];
+    const float r10 = w[4], r11 = w[5], r12 = w[6];
+    const float r20 = w[8], r21 = w[9], r22 = w[10];
+    const float m00 = r11 * r22 - r21 * r12;
+    const float m01 = r21 * r02 - r01 * r22;
+    const float m02 = r01 * r12 - r11 * r02;
+    const float m10 = r20 * r12 - r10 * r22;
+    const float m11 = r00 * r22 - r20 * r02;
+    const float m12 = r10 * r02 - r00 * r12;
+    const float m20 = r10 * r21 - r20 * r11;
+    const float m21 = r20 * r01 - r00 * r21;
+    const float m22 = r00 * r11 - r10 * r01;
+    const float det = r00 * m00 + r01 * m10 + r02 * m20;
+    const float inv = 1.0f / det;
+    M34 o;
+    const float a[9] = {m00, m01, m02, m10, m11, m12, m20, m21, m22};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) o.m[4 * r + c] = inv * a[3 * r + c];
+    const float t[3] = {w[3], w[7], w[11]};
+    for (int r = 0; r < 3; ++r)
+        o.m[4 * r + 3] = -(o.m[4 * r] * t[0] + o.m[4 * r + 1] * t[1] + o.m[4 * r + 2] * t[2]);
+    return o;
+}
+
 // Box transform over its 8 corners (ei::transform(Box, Mat3x4)).
 inline Box xform_box(const M34& M, const Box& b) {
     Box r;

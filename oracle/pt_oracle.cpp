@@ -1295,6 +1295,21 @@ struct PathReplay {
 };
 
 // pt_sample (pt_common.hpp:75-156)
+// Direction of pinholecam_sample_ray (pinhole.hpp:56-84) for pixel (px, py)
+// jittered by (u0, u1).
+V3 pinhole_dir(const OrcPtScene& S, int px, int py, float u0, float u1) {
+    const V3 viewDir{S.camDir[0], S.camDir[1], S.camDir[2]};
+    const V3 up{S.camUp[0], S.camUp[1], S.camUp[2]};
+    const float w = (float)S.width, h = (float)S.height;
+    float cx = ((float)px + u0) / w * 2.0f - 1.0f;
+    float cy = ((float)py + u1) / h * 2.0f - 1.0f;
+    cx *= S.tanVFov;
+    cy *= S.tanVFov;
+    const float aspect = w / h;
+    const V3 xAxis = cross(viewDir, up);
+    return normalize(xAxis * (cx * aspect) + up * cy + viewDir * 1.0f);
+}
+
 void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& rng, PixelAcc& acc,
                PathReplay* replay = nullptr) {
     const int minPL = params[1], maxPL = params[2], neeCount = params[3];
@@ -1341,13 +1356,8 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         Medium medium = kVacuum;
         if (vtx.isCamera) {  // pinholecam_sample_ray (pinhole.hpp:56-84)
             const float w = (float)S.width, h = (float)S.height;
-            float cx = ((float)px + u0) / w * 2.0f - 1.0f;
-            float cy = ((float)py + u1) / h * 2.0f - 1.0f;
-            cx *= S.tanVFov;
-            cy *= S.tanVFov;
             const float aspect = w / h;
-            const V3 xAxis = cross(viewDir, up);
-            excident = normalize(xAxis * (cx * aspect) + up * cy + viewDir * 1.0f);
+            excident = pinhole_dir(S, px, py, u0, u1);
             const float pixelArea = (2 * S.tanVFov) * (2 * S.tanVFov) * aspect;
             const float cosOut = dot(viewDir, excident);
             pdfF = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
@@ -1897,6 +1907,95 @@ int orc_background_sample(const OrcPtScene* S, float u0, float u1, float* out7) 
 }
 // RNG hooks: scramble_seed (SplitMix64 finalizer) and the first n outputs of
 // PCG64(seed) (rng.hpp:12-17, 70-93).
+// The wireframe renderer (GpuWireframe::iterate, gpu_wireframe.cpp:33-48;
+// sample_wireframe, wireframe_common.hpp:94-181) for triangle meshes:
+// `iterations` iterations, each drawing one u32 seed per pixel (pixel order)
+// from the renderer's PCG64 state *rngState (updated), into the running mean
+// (and, if outVariance, variance) of the 3-channel Border target.
+int orc_wireframe_render(const OrcPtScene* S, int32_t iterations, float lineWidth, int32_t maxTraceDepth,
+                         uint64_t* rngState, float* outBorder, float* outVariance) {
+    const int W = S->width, H = S->height;
+    const int num = W * H;
+    std::memset(outBorder, 0, sizeof(float) * 3 * (size_t)num);
+    if (outVariance) std::memset(outVariance, 0, sizeof(float) * 3 * (size_t)num);
+    Pcg64 gen(0u);
+    gen.s = *rngState;
+    std::vector<u32> seeds(num);
+    const V3 camPos{S->camPos[0], S->camPos[1], S->camPos[2]};
+    for (int it = 0; it < iterations; ++it) {
+        for (int i = 0; i < num; ++i) seeds[i] = (u32)gen.next();
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int pixel = 0; pixel < num; ++pixel) {
+            const int px = pixel % W, py = pixel / W;
+            Pcg64 rng(seeds[pixel]);
+            rng.next();  // create_camera's RndSet2 (unused by the pinhole)
+            const u64 x0 = rng.next();
+            rng.next();
+            const V3 dir = pinhole_dir(*S, px, py, sample_uniform((u32)x0), sample_uniform((u32)(x0 >> 32)));
+            V3 origin = camPos;
+            float value[3] = {0.0f, 0.0f, 0.0f};
+            float totalDistance = 0.0f;
+            for (int i = 0; i < maxTraceDepth; ++i) {
+                V3 rayOrigin = origin;
+                const HitInfo hit = first_intersection(*S, rayOrigin, dir, V3{0, 0, 0}, kMaxSceneSize);
+                if (!hit.hit) {
+                    const BgEval bg = evaluate_background(*S, dir);
+                    if (bg.value.x > 0.0f || bg.value.y > 0.0f || bg.value.z > 0.0f) {
+                        value[0] = bg.value.x;
+                        value[1] = bg.value.y;
+                        value[2] = bg.value.z;
+                    }
+                    break;
+                }
+                const float hitDist = hit.t < 0.001f ? 0.001f : hit.t;
+                const V3 p = rayOrigin + dir * hitDist;
+                totalDistance += hitDist;
+                const float pixelSize = totalDistance * S->tanVFov / (float)H;
+                const OrcScene& g = S->geo;
+                const M34 M = instance_to_world(g.worldToInst + 12 * (size_t)hit.inst);
+                const i32 lod = g.instLod[hit.inst];
+                const u32* ix = g.tris + 3 * ((size_t)g.lodTriOffset[lod] + hit.prim);
+                V3 v[3];
+                for (int k = 0; k < 3; ++k) {
+                    const float* q = g.verts + 3 * ((size_t)g.lodVertOffset[lod] + ix[k]);
+                    v[k] = xform_point(M, V3{q[0], q[1], q[2]});
+                }
+                // computeClosestLinePoint (wireframe_common.hpp:16-40)
+                const V3 AB = v[1] - v[0], AC = v[2] - v[0], BC = v[2] - v[1];
+                const V3 AP = p - v[0], BP = p - v[1];
+                const V3 onAB = v[0] + AB * (dot(AP, AB) / dot(AB, AB));
+                const V3 onAC = v[0] + AC * (dot(AP, AC) / dot(AC, AC));
+                const V3 onBC = v[1] + BC * (dot(BP, BC) / dot(BC, BC));
+                const float dAB = dot(onAB - p, onAB - p), dAC = dot(onAC - p, onAC - p),
+                            dBC = dot(onBC - p, onBC - p);
+                const V3 closest = (dAB <= dAC && dAB <= dBC) ? onAB : ((dAC <= dAB && dAC <= dBC) ? onAC : onBC);
+                const V3 lineSegment = closest - camPos;
+                const V3 projected = lineSegment - dir * dot(lineSegment, dir);
+                if (len(projected) > lineWidth * pixelSize) {
+                    origin = rayOrigin + dir * hitDist;
+                } else {
+                    value[0] = value[1] = value[2] = 1.0f;
+                    break;
+                }
+            }
+            const float denom = mx(1.0f, (float)it + 1);
+            const float varDenom = mx(1.0f, (float)it);
+            for (int c = 0; c < 3; ++c) {
+                const float x = std::isnan(value[c]) ? 0.0f : value[c];
+                float& cum = outBorder[3 * (size_t)pixel + c];
+                const float diff = x - cum;
+                cum += diff / denom;
+                if (outVariance) {
+                    float& var = outVariance[3 * (size_t)pixel + c];
+                    var += (diff * (x - cum) - var) / varDenom;
+                }
+            }
+        }
+    }
+    *rngState = gen.s;
+    return 0;
+}
+
 uint64_t orc_scramble_seed(uint32_t x) { return Pcg64::scramble(x); }
 int orc_pcg_sequence(uint32_t seed, int32_t n, uint64_t* out) {
     Pcg64 r(seed);
