@@ -246,6 +246,7 @@ _SIGS = {
     "renderer_set_parameter_bool": (Boolean, [vp, cp, Boolean]),
     "renderer_get_parameter_bool": (Boolean, [vp, cp, C.POINTER(Boolean)]),
     "renderer_set_parameter_float": (Boolean, [vp, cp, C.c_float]),
+    "renderer_get_parameter_float": (Boolean, [vp, cp, C.POINTER(C.c_float)]),
 }
 
 _core = None
@@ -535,6 +536,15 @@ class CoreInterface:
     def renderer_get_parameter_int(self, name):
         v = i32()
         if not self.lib.renderer_get_parameter_int(self.muffInst, _b(name), C.byref(v)):
+            raise RuntimeError(self.core_get_dll_error())
+        return v.value
+
+    def renderer_set_parameter_float(self, name, value):
+        return bool(self.lib.renderer_set_parameter_float(self.muffInst, _b(name), float(value)))
+
+    def renderer_get_parameter_float(self, name):
+        v = C.c_float()
+        if not self.lib.renderer_get_parameter_float(self.muffInst, _b(name), C.byref(v)):
             raise RuntimeError(self.core_get_dll_error())
         return v.value
 

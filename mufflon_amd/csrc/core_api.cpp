@@ -181,6 +181,16 @@ constexpr ParamDescription kParams[] = {{"Seed", PARAM_INT},
                                         {"NEE position guide", PARAM_BOOL}};
 constexpr int kNumParams = 5;
 
+// The "Wireframe" renderer's parameters (WireframeParameters,
+// wireframe_params.hpp:10-36) and its single target (BorderTarget, :38-43).
+struct WireParams {
+    float lineWidth = 1.0f;
+    int32_t maxTraceDepth = 1000;
+};
+constexpr ParamDescription kWireParams[] = {{"Line width", PARAM_FLOAT}, {"Maximum trace depth", PARAM_INT}};
+constexpr int kNumWireParams = 2;
+constexpr int kRendererPt = 0, kRendererWire = 1;
+
 // PtTargets order (pt_params.hpp:22-26) and the device target ids.
 struct TargetName {
     const char* name;
@@ -192,11 +202,21 @@ constexpr TargetName kTargets[] = {{"Radiance", MFH_TARGET_RADIANCE, 3}, {"Posit
                                    {"UV", MFH_TARGET_UV, 2},             {"Albedo", MFH_TARGET_ALBEDO, 3},
                                    {"Lightness", MFH_TARGET_LIGHTNESS, 1}, {"Hit ID", MFH_TARGET_HIT_ID, 2}};
 constexpr int kNumTargetNames = 8;
+constexpr TargetName kWireTargets[] = {{"Border", -1, 3}};
 
-int target_index(const char* name) {
+struct TargetTable {
+    const TargetName* targets;
+    int count;
+};
+TargetTable target_table(int renderer) {
+    return renderer == kRendererWire ? TargetTable{kWireTargets, 1} : TargetTable{kTargets, kNumTargetNames};
+}
+
+int target_index(int renderer, const char* name) {
     if (!name) return -1;
-    for (int i = 0; i < kNumTargetNames; ++i)
-        if (std::strcmp(kTargets[i].name, name) == 0) return i;
+    const TargetTable tt = target_table(renderer);
+    for (int i = 0; i < tt.count; ++i)
+        if (std::strcmp(tt.targets[i].name, name) == 0) return i;
     return -1;
 }
 
@@ -220,7 +240,9 @@ struct Core {
     float sceneBox[6] = {0, 0, 0, 0, 0, 0};
     // renderer state
     int variation = -1;  // -1: no renderer enabled
+    int renderer = kRendererPt;  // kRendererPt (variations 0, 1) or kRendererWire (variation 0)
     Variation vars[2];
+    WireParams wire;
     uint32_t targetMask = 0, varianceMask = 0;  // bits: index into kTargets
     std::set<std::pair<std::string, bool>> toEnable, toDisable;
     int32_t iteration = -1;  // OutputHandler::m_iteration
@@ -510,8 +532,15 @@ mfh_pinhole device_camera(const Core& c, const Scenario& s) {
     return p;
 }
 
-void apply_targets(Core& c) {
+// The wireframe's Border target: a fresh running mean (with variance when
+// requested); the renderer's seed generator keeps running.
+void wire_reset(Core& c) {
     if (c.pt < 0) return;
+    check_mfh(mfh_pt_wireframe_reset(c.ctx, c.pt, (c.varianceMask & 1u) ? 1 : 0), "reset");
+}
+
+void apply_targets(Core& c) {
+    if (c.pt < 0 || c.renderer != kRendererPt) return;
     for (int i = 0; i < kNumTargetNames; ++i) {
         const bool on = (c.targetMask >> i) & 1u, var = (c.varianceMask >> i) & 1u;
         check_mfh(mfh_pt_enable_target(c.ctx, c.pt, kTargets[i].device, on ? 1 : 0), "enable target");
@@ -520,7 +549,7 @@ void apply_targets(Core& c) {
 }
 
 void apply_mode(Core& c) {
-    if (c.pt < 0 || c.variation < 0) return;
+    if (c.pt < 0 || c.variation < 0 || c.renderer != kRendererPt) return;
     check_mfh(mfh_pt_set_mode(c.ctx, c.pt, c.variation == 1 ? MFH_PT_WAVEFRONT : MFH_PT_MEGAKERNEL,
                               c.variation == 1 ? 1 : 0),
               "set mode");
@@ -650,6 +679,7 @@ void load_scene(Core& c, Scenario& s) {
     apply_targets(c);
     apply_mode(c);
     check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
+    if (c.renderer == kRendererWire) wire_reset(c);
     c.iteration = -1;
     c.screen.clear();
     c.sceneValid = true;
@@ -695,6 +725,11 @@ Variation& current_variation(Core& c) {
 
 void push_params(Core& c) {
     if (c.pt < 0 || !c.sceneValid) return;
+    if (c.renderer == kRendererWire) {  // a parameter change resets the output
+        wire_reset(c);
+        c.iteration = -1;
+        return;
+    }
     const mfh_pt_params p = device_params(c.vars[c.variation]);
     check_mfh(mfh_pt_set_params(c.ctx, c.pt, &p), "parameters");
     c.iteration = -1;
@@ -702,10 +737,13 @@ void push_params(Core& c) {
 
 // Looks a parameter up; unknown names are logged and ignored
 // (ParameterHandler::get, parameter.hpp:242-254).
-int find_param(const char* fn, const char* name, ParameterType expected) {
-    for (int i = 0; i < kNumParams; ++i)
-        if (name && std::strcmp(kParams[i].name, name) == 0) {
-            if (kParams[i].type != expected) {
+int find_param(const Core& c, const char* fn, const char* name, ParameterType expected) {
+    const bool wire = c.renderer == kRendererWire;
+    const ParamDescription* params = wire ? kWireParams : kParams;
+    const int count = wire ? kNumWireParams : kNumParams;
+    for (int i = 0; i < count; ++i)
+        if (name && std::strcmp(params[i].name, name) == 0) {
+            if (params[i].type != expected) {
                 log_error(fn, std::string("Parameter '") + name + "' has another type");
                 return -2;
             }
@@ -880,17 +918,21 @@ Boolean mufflon_get_target_image(MufflonInstanceHdl instHdl, const char* name, B
             if (ptr) *ptr = nullptr;
             return true;
         }
-        const int t = target_index(name);
+        const int t = target_index(c.renderer, name);
         CORE_REQUIRE(t >= 0, std::string("Cannot find render target '") + (name ? name : "") + "'");
         const uint32_t bit = 1u << t;
         CORE_REQUIRE(variance ? (c.varianceMask & bit) : (c.targetMask & bit),
                      std::string("Render target '") + name + (variance ? "' does not record variance" : "' is not enabled"));
         const Scenario& s = *c.current;
-        c.screenChannels = kTargets[t].channels;
+        const TargetName& tn = target_table(c.renderer).targets[t];
+        c.screenChannels = tn.channels;
         c.screen.resize((size_t)s.width * s.height * c.screenChannels);
-        check_mfh(variance ? mfh_pt_get_variance(c.ctx, c.pt, kTargets[t].device, c.screen.data())
-                           : mfh_pt_get_target(c.ctx, c.pt, kTargets[t].device, c.screen.data()),
-                  "target read-back");
+        if (c.renderer == kRendererWire)
+            check_mfh(mfh_pt_wireframe_get(c.ctx, c.pt, variance ? 1 : 0, c.screen.data()), "target read-back");
+        else
+            check_mfh(variance ? mfh_pt_get_variance(c.ctx, c.pt, tn.device, c.screen.data())
+                               : mfh_pt_get_target(c.ctx, c.pt, tn.device, c.screen.data()),
+                      "target read-back");
         if (ptr) *ptr = c.screen.data();
         return true;
     });
@@ -1964,41 +2006,46 @@ Boolean scenario_assign_material(ScenarioHdl scenario, MatIdx index, MaterialHdl
 }
 
 // ================================================================= rendering
+// Renderers: 0 "Pathtracer"/"PT" (variations 0 megakernel, 1 wavefront),
+// 1 "Wireframe"/"WF" (one variation), all on DEVICE_CUDA.
+constexpr const char* kRendererNames[2][2] = {{"Pathtracer", "PT"}, {"Wireframe", "WF"}};  // gpu_pt.hpp, gpu_wireframe.hpp
+constexpr uint32_t kRendererVariations[2] = {2u, 1u};
+
 uint32_t render_get_renderer_count(MufflonInstanceHdl instHdl) {
     return guard(__func__, (uint32_t)0, [&] {
         core_of(instHdl);
-        return 1u;
+        return 2u;
     });
 }
 
 uint32_t render_get_renderer_variations(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (uint32_t)0, [&]() -> uint32_t {
         core_of(instHdl);
-        CORE_REQUIRE(index == 0, "renderer index out of bounds");
-        return 2u;
+        CORE_REQUIRE(index < 2, "renderer index out of bounds");
+        return kRendererVariations[index];
     });
 }
 
 const char* render_get_renderer_name(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (const char*)nullptr, [&]() -> const char* {
         core_of(instHdl);
-        CORE_REQUIRE(index == 0, "renderer index out of bounds");
-        return "Pathtracer";  // GpuPathTracer::get_name (gpu_pt.hpp:24-27)
+        CORE_REQUIRE(index < 2, "renderer index out of bounds");
+        return kRendererNames[index][0];
     });
 }
 
 const char* render_get_renderer_short_name(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (const char*)nullptr, [&]() -> const char* {
         core_of(instHdl);
-        CORE_REQUIRE(index == 0, "renderer index out of bounds");
-        return "PT";
+        CORE_REQUIRE(index < 2, "renderer index out of bounds");
+        return kRendererNames[index][1];
     });
 }
 
 RenderDevice render_get_renderer_devices(MufflonInstanceHdl instHdl, uint32_t index, uint32_t variation) {
     return guard(__func__, DEVICE_NONE, [&]() -> RenderDevice {
         core_of(instHdl);
-        CORE_REQUIRE(index == 0 && variation < 2, "renderer index out of bounds");
+        CORE_REQUIRE(index < 2 && variation < kRendererVariations[index], "renderer index out of bounds");
         return DEVICE_CUDA;
     });
 }
@@ -2006,8 +2053,9 @@ RenderDevice render_get_renderer_devices(MufflonInstanceHdl instHdl, uint32_t in
 Boolean render_enable_renderer(MufflonInstanceHdl instHdl, uint32_t index, uint32_t variation) {
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
         Core& c = core_of(instHdl);
-        CORE_REQUIRE(index == 0 && variation < 2, "renderer index out of bounds");
+        CORE_REQUIRE(index < 2 && variation < kRendererVariations[index], "renderer index out of bounds");
         std::lock_guard<std::mutex> lock(c.iterationMutex);
+        c.renderer = (int)index;
         c.variation = (int)variation;
         // a fresh output handler: no targets, iteration -1 (interface.cpp:3587-3609)
         c.targetMask = c.varianceMask = 0;
@@ -2018,6 +2066,8 @@ Boolean render_enable_renderer(MufflonInstanceHdl instHdl, uint32_t index, uint3
         if (c.current) {
             if (!c.sceneValid || c.pt < 0) {
                 load_scene(c, *c.current);
+            } else if (c.renderer == kRendererWire) {
+                wire_reset(c);
             } else {
                 apply_targets(c);
                 apply_mode(c);
@@ -2039,13 +2089,13 @@ Boolean render_iterate(MufflonInstanceHdl instHdl, ProcessTime* iterateTime, Pro
         // requested target changes (interface.cpp:3625-3636)
         const uint32_t oldMask = c.targetMask, oldVar = c.varianceMask;
         for (const auto& t : c.toDisable) {
-            const int i = target_index(t.first.c_str());
+            const int i = target_index(c.renderer, t.first.c_str());
             CORE_REQUIRE(i >= 0, "Cannot find render target '" + t.first + "'");
             c.varianceMask &= ~(1u << i);
             if (!t.second) c.targetMask &= ~(1u << i);
         }
         for (const auto& t : c.toEnable) {
-            const int i = target_index(t.first.c_str());
+            const int i = target_index(c.renderer, t.first.c_str());
             CORE_REQUIRE(i >= 0, "Cannot find render target '" + t.first + "'");
             c.targetMask |= 1u << i;
             if (t.second) c.varianceMask |= 1u << i;
@@ -2056,7 +2106,10 @@ Boolean render_iterate(MufflonInstanceHdl instHdl, ProcessTime* iterateTime, Pro
         if (!c.sceneValid || c.pt < 0) {
             load_scene(c, *c.current);
         } else if (oldMask != c.targetMask || oldVar != c.varianceMask) {
-            apply_targets(c);  // resets
+            if (c.renderer == kRendererWire)
+                wire_reset(c);
+            else
+                apply_targets(c);  // resets
             c.iteration = -1;
         }
         if (preTime) {
@@ -2065,7 +2118,10 @@ Boolean render_iterate(MufflonInstanceHdl instHdl, ProcessTime* iterateTime, Pro
         }
         t0 = micros();
         k0 = cycles();
-        check_mfh(mfh_pt_iterate(c.ctx, c.pt, 1), "iterate");
+        if (c.renderer == kRendererWire)
+            check_mfh(mfh_pt_wireframe_iterate(c.ctx, c.pt, 1, c.wire.lineWidth, c.wire.maxTraceDepth), "iterate");
+        else
+            check_mfh(mfh_pt_iterate(c.ctx, c.pt, 1), "iterate");
         check_mfh(mfh_synchronize(c.ctx), "iterate");
         ++c.iteration;
         if (iterateTime) {
@@ -2088,7 +2144,12 @@ Boolean render_reset(MufflonInstanceHdl instHdl) {
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
         Core& c = core_of(instHdl);
         std::lock_guard<std::mutex> lock(c.iterationMutex);
-        if (c.pt >= 0 && c.sceneValid) check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
+        if (c.pt >= 0 && c.sceneValid) {
+            if (c.renderer == kRendererWire)
+                wire_reset(c);
+            else
+                check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
+        }
         c.iteration = -1;
         return true;
     });
@@ -2111,8 +2172,8 @@ Boolean render_save_screenshot(MufflonInstanceHdl instHdl, const char* filename,
         if (!file.is_absolute()) file = fs::absolute(file);
         std::string name = file.stem().u8string();
         name = replace_tag(name, "#scenario", c.current ? c.current->name : "");
-        name = replace_tag(name, "#renderer", "Pathtracer");
-        name = replace_tag(name, "#shortrenderer", "PT");
+        name = replace_tag(name, "#renderer", kRendererNames[c.renderer][0]);
+        name = replace_tag(name, "#shortrenderer", kRendererNames[c.renderer][1]);
         name = replace_tag(name, "#iteration", std::to_string(c.iteration + 1));
         name = replace_tag(name, "#frame", std::to_string(c.frameCurrent));
         name = replace_tag(name, "#target", variance ? std::string(targetName) + "(Variance)" : std::string(targetName));
@@ -2128,15 +2189,18 @@ Boolean render_save_screenshot(MufflonInstanceHdl instHdl, const char* filename,
 
 uint32_t render_get_render_target_count(MufflonInstanceHdl instHdl) {
     return guard(__func__, (uint32_t)0, [&]() -> uint32_t {
-        return core_of(instHdl).variation < 0 ? 0u : (uint32_t)kNumTargetNames;
+        const Core& c = core_of(instHdl);
+        return c.variation < 0 ? 0u : (uint32_t)target_table(c.renderer).count;
     });
 }
 
 const char* render_get_render_target_name(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (const char*)nullptr, [&]() -> const char* {
-        if (core_of(instHdl).variation < 0) return "";
-        CORE_REQUIRE(index < (uint32_t)kNumTargetNames, "render target index out of range");
-        return kTargets[index].name;
+        const Core& c = core_of(instHdl);
+        if (c.variation < 0) return "";
+        const TargetTable tt = target_table(c.renderer);
+        CORE_REQUIRE(index < (uint32_t)tt.count, "render target index out of range");
+        return tt.targets[index].name;
     });
 }
 
@@ -2168,7 +2232,7 @@ Boolean render_is_render_target_enabled(MufflonInstanceHdl instHdl, const char* 
         const std::pair<std::string, bool> key{name, variance != 0};
         if (c.toEnable.count(key)) return true;
         if (c.toDisable.count(key)) return false;
-        const int i = target_index(name);
+        const int i = target_index(c.renderer, name);
         CORE_REQUIRE(i >= 0, std::string("Cannot find render target '") + name + "'");
         return ((variance ? c.varianceMask : c.targetMask) >> i) & 1u;
     });
@@ -2179,25 +2243,34 @@ Boolean render_is_render_target_required(MufflonInstanceHdl instHdl, const char*
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
         Core& c = core_of(instHdl);
         CORE_REQUIRE(c.variation >= 0, "No renderer is enabled yet");
-        CORE_REQUIRE(target_index(name) >= 0, std::string("Cannot find render target '") + (name ? name : "") + "'");
-        return false;  // PtTargets declares no REQUIRED target (render_targets.hpp:13-74)
+        CORE_REQUIRE(target_index(c.renderer, name) >= 0,
+                     std::string("Cannot find render target '") + (name ? name : "") + "'");
+        return false;  // neither PtTargets nor WireframeTargets declares a REQUIRED target
     });
 }
 
 // ================================================================ parameters
+// Parameters of the enabled renderer: the path tracer's per variation, the
+// wireframe's "Line width" and "Maximum trace depth". Unknown names are
+// logged and ignored (ParameterHandler, parameter.hpp:242-254); a change
+// resets the render (on_renderer_parameter_changed).
 uint32_t renderer_get_num_parameters(MufflonInstanceHdl instHdl) {
     return guard(__func__, (uint32_t)0, [&]() -> uint32_t {
-        current_variation(core_of(instHdl));
-        return (uint32_t)kNumParams;
+        Core& c = core_of(instHdl);
+        current_variation(c);
+        return (uint32_t)(c.renderer == kRendererWire ? kNumWireParams : kNumParams);
     });
 }
 
 const char* renderer_get_parameter_desc(MufflonInstanceHdl instHdl, uint32_t idx, ParameterType* type) {
     return guard(__func__, (const char*)nullptr, [&]() -> const char* {
-        current_variation(core_of(instHdl));
-        CORE_REQUIRE(idx < (uint32_t)kNumParams, "parameter index out of range");
-        if (type) *type = kParams[idx].type;
-        return kParams[idx].name;
+        Core& c = core_of(instHdl);
+        current_variation(c);
+        const bool wire = c.renderer == kRendererWire;
+        CORE_REQUIRE(idx < (uint32_t)(wire ? kNumWireParams : kNumParams), "parameter index out of range");
+        const ParamDescription& d = wire ? kWireParams[idx] : kParams[idx];
+        if (type) *type = d.type;
+        return d.name;
     });
 }
 
@@ -2206,9 +2279,13 @@ Boolean renderer_set_parameter_int(MufflonInstanceHdl instHdl, const char* name,
         Core& c = core_of(instHdl);
         std::lock_guard<std::mutex> lock(c.iterationMutex);
         Variation& v = current_variation(c);
-        const int i = find_param(__func__, name, PARAM_INT);
-        if (i >= 0) *int_param(v, i) = value;
-        push_params(c);  // on_renderer_parameter_changed
+        const int i = find_param(c, __func__, name, PARAM_INT);
+        if (c.renderer == kRendererWire) {
+            if (i == 1) c.wire.maxTraceDepth = value;
+        } else if (i >= 0) {
+            *int_param(v, i) = value;
+        }
+        push_params(c);
         return true;
     });
 }
@@ -2218,19 +2295,22 @@ Boolean renderer_get_parameter_int(MufflonInstanceHdl instHdl, const char* name,
         Core& c = core_of(instHdl);
         Variation& v = current_variation(c);
         CORE_REQUIRE(value != nullptr, "Invalid value pointer (nullptr)");
-        const int i = find_param(__func__, name, PARAM_INT);
-        *value = i >= 0 ? *int_param(v, i) : 0;
+        const int i = find_param(c, __func__, name, PARAM_INT);
+        if (c.renderer == kRendererWire)
+            *value = i == 1 ? c.wire.maxTraceDepth : 0;
+        else
+            *value = i >= 0 ? *int_param(v, i) : 0;
         return true;
     });
 }
 
 Boolean renderer_set_parameter_float(MufflonInstanceHdl instHdl, const char* name, float value) {
-    (void)value;
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
         Core& c = core_of(instHdl);
         std::lock_guard<std::mutex> lock(c.iterationMutex);
         current_variation(c);
-        find_param(__func__, name, PARAM_FLOAT);  // the path tracer has no float parameter
+        const int i = find_param(c, __func__, name, PARAM_FLOAT);  // the path tracer has none
+        if (c.renderer == kRendererWire && i == 0) c.wire.lineWidth = value;
         push_params(c);
         return true;
     });
@@ -2241,8 +2321,8 @@ Boolean renderer_get_parameter_float(MufflonInstanceHdl instHdl, const char* nam
         Core& c = core_of(instHdl);
         current_variation(c);
         CORE_REQUIRE(value != nullptr, "Invalid value pointer (nullptr)");
-        find_param(__func__, name, PARAM_FLOAT);
-        *value = 0.0f;
+        const int i = find_param(c, __func__, name, PARAM_FLOAT);
+        *value = (c.renderer == kRendererWire && i == 0) ? c.wire.lineWidth : 0.0f;
         return true;
     });
 }
@@ -2252,8 +2332,8 @@ Boolean renderer_set_parameter_bool(MufflonInstanceHdl instHdl, const char* name
         Core& c = core_of(instHdl);
         std::lock_guard<std::mutex> lock(c.iterationMutex);
         Variation& v = current_variation(c);
-        const int i = find_param(__func__, name, PARAM_BOOL);
-        if (i == 4) v.neePositionGuide = value != 0;
+        const int i = find_param(c, __func__, name, PARAM_BOOL);
+        if (c.renderer == kRendererPt && i == 4) v.neePositionGuide = value != 0;
         push_params(c);
         return true;
     });
@@ -2264,8 +2344,8 @@ Boolean renderer_get_parameter_bool(MufflonInstanceHdl instHdl, const char* name
         Core& c = core_of(instHdl);
         Variation& v = current_variation(c);
         CORE_REQUIRE(value != nullptr, "Invalid value pointer (nullptr)");
-        const int i = find_param(__func__, name, PARAM_BOOL);
-        *value = i == 4 && v.neePositionGuide;
+        const int i = find_param(c, __func__, name, PARAM_BOOL);
+        *value = c.renderer == kRendererPt && i == 4 && v.neePositionGuide;
         return true;
     });
 }
@@ -2276,7 +2356,7 @@ Boolean renderer_set_parameter_enum(MufflonInstanceHdl instHdl, const char* name
         Core& c = core_of(instHdl);
         std::lock_guard<std::mutex> lock(c.iterationMutex);
         current_variation(c);
-        find_param(__func__, name, PARAM_ENUM);  // the path tracer has no enum parameter
+        find_param(c, __func__, name, PARAM_ENUM);  // neither renderer has an enum parameter
         push_params(c);
         return true;
     });
@@ -2287,7 +2367,7 @@ Boolean renderer_get_parameter_enum(MufflonInstanceHdl instHdl, const char* name
         Core& c = core_of(instHdl);
         current_variation(c);
         CORE_REQUIRE(value != nullptr, "Invalid value pointer (nullptr)");
-        find_param(__func__, name, PARAM_ENUM);
+        find_param(c, __func__, name, PARAM_ENUM);
         *value = 0;
         return true;
     });

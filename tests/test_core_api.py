@@ -134,10 +134,13 @@ def test_unsupported_features_fail_cleanly(mi):
 
 
 def test_renderer_registry_and_parameters(mi):
-    assert mi.render_get_renderer_count() == 1
+    assert mi.render_get_renderer_count() == 2
     assert mi.render_get_renderer_name(0) == "Pathtracer" and mi.render_get_renderer_short_name(0) == "PT"
-    assert mi.render_get_renderer_variations(0) == 2
+    assert mi.render_get_renderer_name(1) == "Wireframe" and mi.render_get_renderer_short_name(1) == "WF"
+    assert mi.render_get_renderer_variations(0) == 2 and mi.render_get_renderer_variations(1) == 1
     assert [mi.render_get_renderer_devices(0, v) for v in range(2)] == [K.DEVICE_CUDA, K.DEVICE_CUDA]
+    assert mi.render_get_renderer_devices(1, 0) == K.DEVICE_CUDA
+    assert not mi.render_enable_renderer(1, 1) and not mi.render_enable_renderer(2, 0)
     # nothing is enabled yet
     assert mi.render_get_render_target_count() == 0
     assert not mi.renderer_set_parameter_int("Seed", 3)
@@ -157,6 +160,16 @@ def test_renderer_registry_and_parameters(mi):
     # parameters belong to the variation
     assert mi.render_enable_renderer(0, 1) and mi.renderer_get_parameter_int("Max. path length") == 16
     assert mi.render_enable_renderer(0, 0) and mi.renderer_get_parameter_int("Max. path length") == 5
+    # the wireframe renderer's own parameters (wireframe_params.hpp:10-36)
+    assert mi.render_enable_renderer(1, 0)
+    assert mi.renderer_parameters() == [("Line width", K.PARAM_FLOAT), ("Maximum trace depth", K.PARAM_INT)]
+    assert mi.renderer_get_parameter_float("Line width") == 1.0
+    assert mi.renderer_get_parameter_int("Maximum trace depth") == 1000
+    assert mi.renderer_set_parameter_float("Line width", 2.5) and mi.renderer_get_parameter_float("Line width") == 2.5
+    assert mi.renderer_set_parameter_int("Maximum trace depth", 7)
+    assert mi.renderer_get_parameter_int("Maximum trace depth") == 7
+    assert mi.renderer_set_parameter_int("Seed", 4) and mi.renderer_get_parameter_int("Seed") == 0  # not its parameter
+    assert mi.render_enable_renderer(0, 0) and mi.renderer_get_parameter_int("Max. path length") == 5
 
 
 def test_render_targets(mi):
@@ -170,6 +183,10 @@ def test_render_targets(mi):
     assert not mi.render_is_render_target_enabled("Radiance", True)
     assert not mi.render_is_render_target_enabled("Nope")
     assert mi.mufflon_get_target_image("Radiance", False, 1, 1) is None  # no scene yet
+    assert mi.render_enable_renderer(1, 0)
+    assert [mi.render_get_render_target_name(i) for i in range(mi.render_get_render_target_count())] == ["Border"]
+    assert mi.render_enable_render_target("Border", False) and mi.render_is_render_target_enabled("Border")
+    assert not mi.render_is_render_target_enabled("Radiance")
 
 
 @pytest.mark.skipif(K.load().mufflon_is_cuda_available(), reason="checks the no-device error path")

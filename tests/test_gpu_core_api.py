@@ -99,3 +99,34 @@ def test_core_api_render_matches_oracle(oracle, tmp_path):
         assert mi.render_get_current_iteration() == 1
     finally:
         mi.close()
+
+
+def test_core_api_wireframe_renderer(oracle, tmp_path):
+    """The "Wireframe"/"WF" renderer through the core API: parameters, the
+    Border target with variance, against the oracle's wireframe on the
+    camera the core stores."""
+    print("core api: wireframe renderer", flush=True)
+    c = S.cornell_materials(12, 24, 40, 32)
+    mi = K.CoreInterface()
+    try:
+        scen = K.build_world(mi, c, str(tmp_path), background=("colored", (0.3, 0.4, 0.6), (0.8, 0.8, 0.8)))
+        assert mi.render_enable_renderer(1, 0)
+        assert mi.renderer_set_parameter_float("Line width", 1.5)
+        assert mi.renderer_set_parameter_int("Maximum trace depth", 50)
+        assert mi.world_load_scenario(scen), mi.core_get_dll_error()
+        mi.render_enable_render_target("Border", True)
+        for _ in range(3):
+            ok, _ = mi.render_iterate()
+            assert ok, mi.core_get_dll_error()
+        W, H = 40, 32
+        border = mi.mufflon_get_target_image("Border", False, W, H)
+        var = mi.mufflon_get_target_image("Border", True, W, H)
+        ps = _oracle_scene(oracle, mi, c, P.background_of(c, "colored"))
+        o, ov, _ = oracle.wireframe_render(ps, 3, line_width=1.5, max_trace_depth=50, variance=True)
+        _compare(border, o, "core border")
+        _compare(var, ov, "core border variance")
+        assert np.array_equal(border.view(np.uint32), o.view(np.uint32))
+        assert mi.render_save_screenshot(str(tmp_path / "#shortrenderer-#target"), "Border", False)
+        assert (tmp_path / "WF-Border.pfm").exists()
+    finally:
+        mi.close()
