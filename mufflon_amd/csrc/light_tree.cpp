@@ -394,16 +394,23 @@ void build_background(int32_t type, const float color[3], const float scale[3], 
         // descriptor scales the flux once more (:46-54)
         const float avgPixelSolidAngle = 4 * kPi / (float)(width * height);
         float flux[3] = {0.0f, 0.0f, 0.0f};
+        // cos / sin of a column's phi do not depend on the row: evaluated once
+        // per column (the same floats the per-texel evaluation gives)
+        std::vector<float> cosPhi(width), sinPhi(width);
+        for (int x = 0; x < width; ++x) {
+            const float phi = 2 * kPi * (x + 0.5f) / float(width);
+            cosPhi[x] = (float)std::cos((double)phi);
+            sinPhi[x] = (float)std::sin((double)phi);
+        }
         for (int y = 0; y < height; ++y) {
             const float sinTheta = std::sin(kPi * (y + 0.5f) / float(height));
+            const float dirY = std::sqrt(1.0f - sinTheta * sinTheta);
+            const float w = avgPixelSolidAngle / (2.0f * kPi * kPi * sinTheta);  // per row as well
             for (int x = 0; x < width; ++x) {
                 const float* t = read(x, y);
-                const float phi = 2 * kPi * (x + 0.5f) / float(width);
-                const V3 dir{sinTheta * (float)std::cos((double)phi), std::sqrt(1.0f - sinTheta * sinTheta),
-                             sinTheta * (float)std::sin((double)phi)};
+                const V3 dir{sinTheta * cosPhi[x], dirY, sinTheta * sinPhi[x]};
                 const float surface = diag.y * diag.z * std::fabs(dir.x) + diag.x * diag.z * std::fabs(dir.y) +
                                       diag.x * diag.y * std::fabs(dir.z);
-                const float w = avgPixelSolidAngle / (2.0f * kPi * kPi * sinTheta);
                 for (int k = 0; k < 3; ++k) flux[k] += (t[k] * scale[k]) * w * surface;
             }
         }
