@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--trace-block", type=int, default=256, help="wavefront trace block (with --pt-profile)")
     ap.add_argument("--stack-depth", type=int, default=16, help="wavefront LDS stack depth (with --pt-profile)")
     ap.add_argument("--refill", type=int, default=0, help="wavefront lane refill budget (with --pt-profile)")
+    ap.add_argument("--leaf-batch", type=int, default=1, choices=[1, 8],
+                    help="triangles of a BVH leaf loaded ahead of their tests (with --pt-profile)")
     ap.add_argument("--profile-only", action="store_true",
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
     ap.add_argument("--pt-iterations", type=int, default=64,
@@ -221,6 +223,9 @@ def pt_terrain(args):
         return ms, ctx.pt_target(pt)
 
     mk_ms, mk_img = run(hip.PT_MEGAKERNEL, False)
+    # the megakernel again with 8 triangles of a BVH leaf loaded ahead of their tests
+    ctx.pt_set_tuning(pt, leaf_batch=8)
+    mk8_ms, mk8_img = run(hip.PT_MEGAKERNEL, False)
     ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
     tuned, _ = ctx.pt_autotune(pt, 1)
     wf_ms, wf_img = run(hip.PT_WAVEFRONT, True)
@@ -231,10 +236,13 @@ def pt_terrain(args):
             "lbvh_10m_build_ms": round(float(np.mean(builds)), 3), "lbvh_10m_build_ms_min": round(min(builds), 3),
             "iterations": its,
             "megakernel": {"ms_per_iteration": round(mk_ms, 3), "mpaths_s": round(pixels / mk_ms / 1e3, 3)},
+            "megakernel_leaf_batch_8": {"ms_per_iteration": round(mk8_ms, 3),
+                                        "mpaths_s": round(pixels / mk8_ms / 1e3, 3)},
             "wavefront": {"ms_per_iteration": round(wf_ms, 3), "mpaths_s": round(pixels / wf_ms / 1e3, 3),
                           "tuning": tuned, "graph": True},
             "wavefront_over_megakernel": round(mk_ms / wf_ms, 3),
-            "identical_images": bool(np.array_equal(mk_img, wf_img))}
+            "wavefront_over_best_megakernel": round(min(mk_ms, mk8_ms) / wf_ms, 3),
+            "identical_images": bool(np.array_equal(mk_img, wf_img) and np.array_equal(mk_img, mk8_img))}
 
 
 def pt_adjoint(args):
@@ -339,10 +347,9 @@ def pt_profile(args):
     else:
         its = 8
         pt, _, _ = cornell_tracer(ctx)
+    ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, 0, args.refill, args.leaf_batch)
     for mode in (hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT):
         ctx.pt_set_mode(pt, mode, False)
-        if mode == hip.PT_WAVEFRONT:
-            ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, 0, args.refill)
         ctx.pt_reset(pt)
         ctx.pt_iterate(pt, 1)
         ctx.pt_reset(pt)

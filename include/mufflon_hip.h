@@ -301,12 +301,14 @@ typedef struct mfh_wavefront_tuning {
     int32_t sortBounces;
     int32_t refillBudget; /* 0: chunked queue walk; n > 0: lanes whose ray ended take the next ray
                              every n traversal steps once a quarter of the wave is idle */
+    int32_t leafBatch;    /* 1 or 8: triangles of a BVH leaf loaded ahead of their tests (long
+                             leaves); also used by the megakernel form; the refill kernels use 1 */
 } mfh_wavefront_tuning;
 int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning* tuning);
 int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* outTuning);
 /* Times `iterations` wavefront iterations for every tuning candidate (block x
- * stack depth x {chunk 256, chunk 1024, refill 16, refill 32} x sorted bounces
- * 0/1/2), keeps the fastest and resets the path tracer. */
+ * stack depth x leaf batch 1/8 x {chunk 256, chunk 1024, refill 16, refill 32}
+ * x sorted bounces 0/1/2), keeps the fastest and resets the path tracer. */
 int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wavefront_tuning* outBest,
                     float* outMsPerIteration);
 /* Batches: several path tracers (scenes of any size, resolution, path

@@ -27,7 +27,8 @@ using namespace path;
 constexpr int kPtTile = 16;
 constexpr int kPtThreads = kPtTile * kPtTile;
 
-template <bool kSingle>
+// kLeaf: triangles of a BVH leaf loaded ahead of their tests (1 or 8).
+template <bool kSingle, int kLeaf>
 __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
     __shared__ int sStack[kLdsDepth * kPtThreads];
     const int px = blockIdx.x * kPtTile + (threadIdx.x % kPtTile);
@@ -51,7 +52,8 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
                 const u64 x = pcg_next(rng);
                 NeeCandidate c;
                 if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
-                    const TraceHit sh = trace_ray<true, false, kSingle, kPtThreads>(S.trace, c.a, c.dir, c.tmax, lstack);
+                    const TraceHit sh = trace_ray<true, false, kSingle, kPtThreads, kLdsDepth, kLeaf>(
+                        S.trace, c.a, c.dir, c.tmax, lstack);
                     if (sh.inst == -1) add_nee(c, acc);
                 }
             }
@@ -68,7 +70,8 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
         } else if (!walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, dir, pdfF, absorption)) {
             break;
         }
-        const TraceHit hit = trace_ray<false, false, kSingle, kPtThreads>(S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
+        const TraceHit hit = trace_ray<false, false, kSingle, kPtThreads, kLdsDepth, kLeaf>(
+            S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
         throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
         footA += 1.0f / fsqrt(pdfF);
         guideWeight = 1.f / (1.f + footA * footA / 1.0f);
@@ -103,12 +106,17 @@ void launch_pt_init_rng(u64* rngs, int num, int seed, hipStream_t stream) {
     MFH_HIP(hipGetLastError());
 }
 
-void launch_pt_iterate(const PtScene& S, hipStream_t stream) {
+void launch_pt_iterate(const PtScene& S, int leafBatch, hipStream_t stream) {
     const dim3 grid((S.width + kPtTile - 1) / kPtTile, (S.height + kPtTile - 1) / kPtTile);
-    if (S.trace.numInstances == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<true>), grid, dim3(kPtThreads), 0, stream, S);
+    const bool single = S.trace.numInstances == 1;
+    if (single && leafBatch == 8)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<true, 8>), grid, dim3(kPtThreads), 0, stream, S);
+    else if (single)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<true, 1>), grid, dim3(kPtThreads), 0, stream, S);
+    else if (leafBatch == 8)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<false, 8>), grid, dim3(kPtThreads), 0, stream, S);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<false>), grid, dim3(kPtThreads), 0, stream, S);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<false, 1>), grid, dim3(kPtThreads), 0, stream, S);
     MFH_HIP(hipGetLastError());
 }
 

@@ -128,8 +128,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                     const u64 x = pcg_next(rng);
                     NeeCandidate c;
                     if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
-                        const TraceHit sh = trace_ray<true, false, kSingle, kAdjThreads>(S.trace, c.a, c.dir, c.tmax,
-                                                                                         lstack);
+                        const TraceHit sh = trace_ray<true, false, kSingle, kAdjThreads, kLdsDepth, kPtLeafBatch>(
+                            S.trace, c.a, c.dir, c.tmax, lstack);
                         if (sh.inst == -1) {
                             AdjVertex& r = rec[pathLen];
                             const V3 wgt = delta * c.radiance;
@@ -171,8 +171,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                 r.tp = r.astar == 0 ? tPrime.x : (r.astar == 1 ? tPrime.y : tPrime.z);
                 if (!ok) break;
             }
-            const TraceHit hit = trace_ray<false, false, kSingle, kAdjThreads>(S.trace, rayOrigin, dir, kMaxSceneSize,
-                                                                             lstack);
+            const TraceHit hit = trace_ray<false, false, kSingle, kAdjThreads, kLdsDepth, kPtLeafBatch>(
+                S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
             throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
             footA += 1.0f / fsqrt(pdfF);
             guideWeight = 1.f / (1.f + footA * footA / 1.0f);

@@ -111,7 +111,7 @@ struct PtScene {
 };
 
 void launch_pt_init_rng(uint64_t* rngs, int num, int seed, hipStream_t stream);
-void launch_pt_iterate(const PtScene& S, hipStream_t stream);
+void launch_pt_iterate(const PtScene& S, int leafBatch, hipStream_t stream);
 // One iteration of the wireframe renderer into S.cum[MFH_TARGET_RADIANCE]
 // (and S.var[...] when recorded), S.iteration the iteration index.
 void launch_wireframe(const PtScene& S, const uint32_t* seeds, float lineWidth, int maxTraceDepth,

@@ -596,7 +596,7 @@ int mfh_pt_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations) {
         }
         for (int32_t k = 0; k < iterations; ++k) {
             S.iteration = ++t.iteration;
-            launch_pt_iterate(S, c.stream);
+            launch_pt_iterate(S, kWfTraceShapes[t.tune.traceVariant].leafBatch, c.stream);
         }
     });
 }
@@ -941,15 +941,37 @@ int mfh_batch_iterate(mfh_context* ctx, int32_t batchId, int32_t iterations) {
     });
 }
 
+namespace mfh {
+namespace {
+
+// Trace variant of a tuning record (block, LDS stack depth, leaf batch); -1 if none matches.
+int trace_variant_of(const mfh_wavefront_tuning& tuning) {
+    for (int k = 0; k < kWfNumTraceVariants; ++k)
+        if (kWfTraceShapes[k].block == tuning.traceBlock && kWfTraceShapes[k].lds == tuning.stackDepth &&
+            kWfTraceShapes[k].leafBatch == tuning.leafBatch)
+            return k;
+    return -1;
+}
+
+void tuning_record(const WfTuning& tune, mfh_wavefront_tuning* out) {
+    out->traceBlock = kWfTraceShapes[tune.traceVariant].block;
+    out->stackDepth = kWfTraceShapes[tune.traceVariant].lds;
+    out->queueChunk = tune.chunk;
+    out->sortBounces = tune.sortBounces;
+    out->refillBudget = tune.refill;
+    out->leafBatch = kWfTraceShapes[tune.traceVariant].leafBatch;
+}
+
+}  // namespace
+}  // namespace mfh
+
 int mfh_batch_set_tuning(mfh_context* ctx, int32_t batchId, const mfh_wavefront_tuning* tuning) {
     return guarded([&] {
         Context& c = ctx_of(ctx);
         PtBatch& b = batch_of(c, batchId);
         MFH_REQUIRE(tuning != nullptr, "null tuning");
-        int v = -1;
-        for (int k = 0; k < kWfNumTraceVariants; ++k)
-            if (kWfTraceShapes[k].block == tuning->traceBlock && kWfTraceShapes[k].lds == tuning->stackDepth) v = k;
-        MFH_REQUIRE(v >= 0, "unsupported trace block size / LDS stack depth combination");
+        const int v = trace_variant_of(*tuning);
+        MFH_REQUIRE(v >= 0, "unsupported trace block size / LDS stack depth / leaf batch combination");
         MFH_REQUIRE(tuning->queueChunk >= 64 && tuning->queueChunk % 64 == 0 && tuning->queueChunk <= 65536,
                     "queue chunk must be a multiple of 64 in [64, 65536]");
         MFH_REQUIRE(tuning->sortBounces == 0, "batches do not sort rays");
@@ -972,6 +994,7 @@ int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mf
         WfTuning bestTune = b.tune;
         for (int v = 0; v < kWfNumTraceVariants; ++v) {
             for (int m = 0; m < kTraceModes; ++m) {
+                if (kTraceModeRefill[m] > 0 && kWfTraceShapes[v].leafBatch != 1) continue;  // same kernels as leaf 1
                 b.tune.traceVariant = v;
                 b.tune.chunk = kTraceModeChunk[m];
                 b.tune.refill = kTraceModeRefill[m];
@@ -995,13 +1018,7 @@ int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mf
         drop_batch_graph(b);
         for (int32_t id : b.tracers) reset(c, tracer_of(c, id));
         b.iteration = -1;
-        if (outBest) {
-            outBest->traceBlock = kWfTraceShapes[b.tune.traceVariant].block;
-            outBest->stackDepth = kWfTraceShapes[b.tune.traceVariant].lds;
-            outBest->queueChunk = b.tune.chunk;
-            outBest->sortBounces = 0;
-            outBest->refillBudget = b.tune.refill;
-        }
+        if (outBest) tuning_record(b.tune, outBest);
         if (outMsPerIteration) *outMsPerIteration = best;
     });
 }
@@ -1025,10 +1042,8 @@ int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning
         Context& c = ctx_of(ctx);
         PathTracer& t = tracer_of(c, ptId);
         MFH_REQUIRE(tuning != nullptr, "null tuning");
-        int v = -1;
-        for (int k = 0; k < kWfNumTraceVariants; ++k)
-            if (kWfTraceShapes[k].block == tuning->traceBlock && kWfTraceShapes[k].lds == tuning->stackDepth) v = k;
-        MFH_REQUIRE(v >= 0, "unsupported trace block size / LDS stack depth combination");
+        const int v = trace_variant_of(*tuning);
+        MFH_REQUIRE(v >= 0, "unsupported trace block size / LDS stack depth / leaf batch combination");
         MFH_REQUIRE(tuning->queueChunk >= 64 && tuning->queueChunk % 64 == 0 && tuning->queueChunk <= 65536,
                     "queue chunk must be a multiple of 64 in [64, 65536]");
         MFH_HIP(hipStreamSynchronize(c.stream));
@@ -1048,11 +1063,7 @@ int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* out)
         Context& c = ctx_of(ctx);
         PathTracer& t = tracer_of(c, ptId);
         MFH_REQUIRE(out != nullptr, "null output");
-        out->traceBlock = kWfTraceShapes[t.tune.traceVariant].block;
-        out->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
-        out->queueChunk = t.tune.chunk;
-        out->sortBounces = t.tune.sortBounces;
-        out->refillBudget = t.tune.refill;
+        tuning_record(t.tune, out);
     });
 }
 
@@ -1071,6 +1082,7 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
         for (int v = 0; v < kWfNumTraceVariants; ++v) {
             for (int cs = 0; cs < 3 * kTraceModes; ++cs) {
                 const int m = cs % kTraceModes;
+                if (kTraceModeRefill[m] > 0 && kWfTraceShapes[v].leafBatch != 1) continue;  // same kernels as leaf 1
                 t.tune.traceVariant = v;
                 t.tune.chunk = kTraceModeChunk[m];
                 t.tune.refill = kTraceModeRefill[m];
@@ -1094,13 +1106,7 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
         t.tune = bestTune;
         drop_graph(t);
         reset(c, t);
-        if (outBest) {
-            outBest->traceBlock = kWfTraceShapes[t.tune.traceVariant].block;
-            outBest->stackDepth = kWfTraceShapes[t.tune.traceVariant].lds;
-            outBest->queueChunk = t.tune.chunk;
-            outBest->sortBounces = t.tune.sortBounces;
-            outBest->refillBudget = t.tune.refill;
-        }
+        if (outBest) tuning_record(t.tune, outBest);
         if (outMsPerIteration) *outMsPerIteration = best;
     });
 }

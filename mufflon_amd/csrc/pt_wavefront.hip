@@ -138,12 +138,12 @@ __device__ __forceinline__ int locate(const WfState& W, int p, int& local) {
 
 // Traversal whose instance-level specialization is fixed (kSingle) or chosen
 // per lane from the ray's scene (batches mix single- and multi-instance scenes).
-template <int kMode, bool kAny, int kStride, int kLds>
+template <int kMode, bool kAny, int kStride, int kLds, int kLeaf>
 __device__ __forceinline__ TraceHit trace_mode(const TraceParams& tp, V3 o, V3 d, float tmax, int* lstack) {
-    if constexpr (kMode == 1) return trace_ray<kAny, false, true, kStride, kLds>(tp, o, d, tmax, lstack);
-    if constexpr (kMode == 0) return trace_ray<kAny, false, false, kStride, kLds>(tp, o, d, tmax, lstack);
-    if (tp.numInstances == 1) return trace_ray<kAny, false, true, kStride, kLds>(tp, o, d, tmax, lstack);
-    return trace_ray<kAny, false, false, kStride, kLds>(tp, o, d, tmax, lstack);
+    if constexpr (kMode == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    if constexpr (kMode == 0) return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    if (tp.numInstances == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
 }
 
 __global__ void k_wf_begin(WfState W) {
@@ -157,7 +157,8 @@ __global__ void k_wf_begin(WfState W) {
 
 // Camera rays and their closest hits, one thread per pixel: 16x16 tiles for
 // one scene, one path per thread in path order for a batch (kMode 2).
-template <int kMode>
+// kLeaf: triangles of a leaf loaded ahead of their tests (trace_ray's kBatch).
+template <int kMode, int kLeaf>
 __global__ __launch_bounds__(256) void k_wf_primary(PtScene S0, WfState W) {
     __shared__ int sStack[kLdsDepth * 256];
     constexpr bool kBatch = kMode == 2;
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(256) void k_wf_primary(PtScene S0, WfState W) {
     float pdfF;
     camera_ray(S, px, py, x0, dir, pdfF);
     const V3 o = v3(S.camPos[0], S.camPos[1], S.camPos[2]);
-    const TraceHit h = trace_mode<kMode, false, 256, kLdsDepth>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
+    const TraceHit h = trace_mode<kMode, false, 256, kLdsDepth, kLeaf>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
     S.rngs[local] = rng;
     W.rayO[0][p] = float4{o.x, o.y, o.z, pdfF};
     W.rayD[0][p] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_wf_primary(PtScene S0, WfState W) {
 }
 
 // Closest hit of queue `buf` (length at counters[count]).
-template <int kMode, int kBlock, int kLds>
+template <int kMode, int kBlock, int kLds, int kLeaf>
 __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S0, WfState W, int buf, int count, int work,
                                                       const int* perm) {
     __shared__ int sStack[kLds * kBlock];
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S0, WfState W, int
             int local;
             const int s = locate<kBatch>(W, __float_as_int(d.w), local);
             const PtScene& S = scene_ref<kBatch>(S0, W, s);
-            const TraceHit h = trace_mode<kMode, false, kBlock, kLds>(S.trace, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
+            const TraceHit h = trace_mode<kMode, false, kBlock, kLds, kLeaf>(S.trace, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
                                                                       kMaxSceneSize, lstack);
             W.hit[buf][q] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
             W.rayM[buf][q].w = h.w1;
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S0, WfState W, int
 
 // Any hit of the shadow candidates of one bounce; visible ones are added to
 // their path's accumulator in candidate order.
-template <int kMode, int kBlock, int kLds>
+template <int kMode, int kBlock, int kLds, int kLeaf>
 __global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S0, WfState W, int count, int work, const int* perm) {
     __shared__ int sStack[kLds * kBlock];
     constexpr bool kBatch = kMode == 2;
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S0, WfState W, int
                 const size_t e = (size_t)sl * W.neeStride + i;
                 const float4 a = W.shO[e];
                 const float4 d = W.shD[e];
-                const TraceHit h = trace_mode<kMode, true, kBlock, kLds>(S.trace, v3(a.x, a.y, a.z), v3(d.x, d.y, d.z),
+                const TraceHit h = trace_mode<kMode, true, kBlock, kLds, kLeaf>(S.trace, v3(a.x, a.y, a.z), v3(d.x, d.y, d.z),
                                                                          a.w, lstack);
                 if (h.inst == -1) visible |= 1u << i;
             }
@@ -558,7 +559,7 @@ __global__ void k_wf_finish(PtScene S0, WfState W) {
     accumulate(S, (size_t)local, *W.iteration, v, S.targetMask);
 }
 
-template <int kMode, int kBlock, int kLds>
+template <int kMode, int kBlock, int kLds, int kLeaf>
 struct TraceKernels {
     static void extend(const PtScene& S, const WfState& W, int grid, int buf, int count, int work, const int* perm,
                        int refill, hipStream_t st) {
@@ -566,8 +567,8 @@ struct TraceKernels {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend_refill<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0,
                                st, S, W, buf, count, work, perm, refill);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
-                               W, buf, count, work, perm);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_extend<kMode, kBlock, kLds, kLeaf>), dim3(grid), dim3(kBlock), 0,
+                               st, S, W, buf, count, work, perm);
     }
     static void shadow(const PtScene& S, const WfState& W, int grid, int count, int work, const int* perm, int refill,
                        hipStream_t st) {
@@ -575,8 +576,8 @@ struct TraceKernels {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow_refill<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0,
                                st, S, W, count, work, perm, refill);
         else
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kMode, kBlock, kLds>), dim3(grid), dim3(kBlock), 0, st, S,
-                               W, count, work, perm);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shadow<kMode, kBlock, kLds, kLeaf>), dim3(grid), dim3(kBlock), 0,
+                               st, S, W, count, work, perm);
     }
     static int occupancy(bool refill) {
         int a = 0, b = 0;
@@ -586,8 +587,10 @@ struct TraceKernels {
             MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow_refill<kMode, kBlock, kLds>, kBlock,
                                                                  0));
         } else {
-            MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kMode, kBlock, kLds>, kBlock, 0));
-            MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kMode, kBlock, kLds>, kBlock, 0));
+            MFH_HIP(
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_extend<kMode, kBlock, kLds, kLeaf>, kBlock, 0));
+            MFH_HIP(
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_shadow<kMode, kBlock, kLds, kLeaf>, kBlock, 0));
         }
         return std::max(1, std::min(a, b));
     }
@@ -603,25 +606,39 @@ struct TraceVariant {
     OccFn occupancy;
 };
 
-template <int kMode, int kBlock, int kLds>
+template <int kMode, int kBlock, int kLds, int kLeaf>
 constexpr TraceVariant variant() {
-    return {&TraceKernels<kMode, kBlock, kLds>::extend, &TraceKernels<kMode, kBlock, kLds>::shadow,
-            &TraceKernels<kMode, kBlock, kLds>::occupancy};
+    return {&TraceKernels<kMode, kBlock, kLds, kLeaf>::extend, &TraceKernels<kMode, kBlock, kLds, kLeaf>::shadow,
+            &TraceKernels<kMode, kBlock, kLds, kLeaf>::occupancy};
 }
 
-// Tuning space of the trace stages: block size x LDS stack depth, for
-// multi-instance scenes (0), single-instance scenes (1) and batches (2).
+// Tuning space of the trace stages: block size x LDS stack depth x leaf
+// batch (1 or 8 triangles loaded ahead; the lane-refill kernels test one at a
+// time), for multi-instance scenes (0), single-instance scenes (1) and
+// batches (2).
+template <int kMode, int kLeaf>
+constexpr std::array<TraceVariant, 7> shapes() {
+    return {variant<kMode, 64, 8, kLeaf>(),   variant<kMode, 64, 16, kLeaf>(),  variant<kMode, 128, 8, kLeaf>(),
+            variant<kMode, 128, 16, kLeaf>(), variant<kMode, 256, 8, kLeaf>(),  variant<kMode, 256, 16, kLeaf>(),
+            variant<kMode, 256, 24, kLeaf>()};
+}
 template <int kMode>
-constexpr std::array<TraceVariant, kWfNumTraceVariants> variants() {
-    return {variant<kMode, 64, 8>(), variant<kMode, 64, 16>(), variant<kMode, 128, 8>(), variant<kMode, 128, 16>(),
-            variant<kMode, 256, 8>(), variant<kMode, 256, 16>(), variant<kMode, 256, 24>()};
+std::array<TraceVariant, kWfNumTraceVariants> variants() {
+    const std::array<TraceVariant, 7> a = shapes<kMode, 1>(), b = shapes<kMode, 8>();
+    std::array<TraceVariant, kWfNumTraceVariants> v{};
+    for (int i = 0; i < 7; ++i) {
+        v[i] = a[i];
+        v[7 + i] = b[i];
+    }
+    return v;
 }
 const std::array<TraceVariant, kWfNumTraceVariants> kVariants[3] = {variants<0>(), variants<1>(), variants<2>()};
 
 }  // namespace
 
-const WfTraceShape kWfTraceShapes[kWfNumTraceVariants] = {{64, 8}, {64, 16}, {128, 8}, {128, 16},
-                                                           {256, 8}, {256, 16}, {256, 24}};
+const WfTraceShape kWfTraceShapes[kWfNumTraceVariants] = {
+    {64, 8, 1}, {64, 16, 1}, {128, 8, 1}, {128, 16, 1}, {256, 8, 1}, {256, 16, 1}, {256, 24, 1},
+    {64, 8, 8}, {64, 16, 8}, {128, 8, 8}, {128, 16, 8}, {256, 8, 8}, {256, 16, 8}, {256, 24, 8}};
 
 int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs) {
     const TraceVariant& v = kVariants[S.trace.numInstances == 1 ? 1 : 0][tune.traceVariant];
@@ -653,10 +670,15 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
     const TraceVariant& v = kVariants[single ? 1 : 0][tune.traceVariant];
     hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
     const dim3 pgrid((S.width + 15) / 16, (S.height + 15) / 16);
-    if (single)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<1>), pgrid, dim3(256), 0, stream, S, W);
+    const bool leaf8 = kWfTraceShapes[tune.traceVariant].leafBatch == 8;
+    if (single && leaf8)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<1, 8>), pgrid, dim3(256), 0, stream, S, W);
+    else if (single)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<1, 1>), pgrid, dim3(256), 0, stream, S, W);
+    else if (leaf8)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<0, 8>), pgrid, dim3(256), 0, stream, S, W);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<0>), pgrid, dim3(256), 0, stream, S, W);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<0, 1>), pgrid, dim3(256), 0, stream, S, W);
     // pt_sample's loop is a do-while: the first hit is shaded even at maximum path length 0
     const int L = S.maxPathLength > 1 ? S.maxPathLength : 1;
     int work = W.workBase;
@@ -683,8 +705,12 @@ void launch_pt_wavefront_batch(const WfState& W, const WfTuning& tune, int maxLe
     std::memset(&none, 0, sizeof(none));
     const TraceVariant& v = kVariants[2][tune.traceVariant];
     hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<2>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, none,
-                       W);
+    if (kWfTraceShapes[tune.traceVariant].leafBatch == 8)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<2, 8>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream,
+                           none, W);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<2, 1>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream,
+                           none, W);
     const int L = maxLen > 1 ? maxLen : 1;
     int work = W.workBase;
     for (int k = 0; k < L; ++k) {

@@ -39,13 +39,13 @@ struct WfState {
 };
 
 struct WfTraceShape {
-    int block, lds;
+    int block, lds, leafBatch;
 };
-constexpr int kWfNumTraceVariants = 7;
+constexpr int kWfNumTraceVariants = 14;
 extern const WfTraceShape kWfTraceShapes[kWfNumTraceVariants];
 
 struct WfTuning {
-    int traceVariant = 5;  // 256 threads, 16 LDS stack entries
+    int traceVariant = 5;  // 256 threads, 16 LDS stack entries, leaf batch 1
     int chunk = 256;
     int sortBounces = 0;   // extension / shadow queues of the first bounces traced in sorted order
     int refill = 0;        // > 0: trace stages refill idle lanes every `refill` traversal steps

@@ -24,6 +24,11 @@ constexpr int kSentinel = (int)0xFFFFFFFF;
 constexpr int kStackCap = 128;
 constexpr int kTraceThreads = 256;
 constexpr int kLdsDepth = 16;
+// Leaf batch (trace_ray's kBatch) of the path-tracing kernels: on scenes with
+// long leaves (a 10M-triangle heightfield leaves up to ~1400 triangle tests on
+// one ray) the slowest rays of a launch are bound by one memory latency per
+// triangle; the plain trace kernels keep 1, which is faster on small leaves.
+constexpr int kPtLeafBatch = 8;
 
 // Slab test, reference CPU semantics (intersection.cu:57-75).
 __device__ __forceinline__ bool slab(const float* lo, const float* hi, const FastRay& r, float tmax, float& cmin) {
@@ -312,7 +317,7 @@ struct Traversal {
 // Traverse from (wo, wd) up to tmax. lstack: this lane's LDS stack column
 // (stride kStride ints). On return t is the world-space distance (or the
 // round-tripped tmax on a miss) and inst/prim are -1 on a miss.
-template <bool kAny, bool kStats, bool kSingle, int kStride, int kLds = kLdsDepth>
+template <bool kAny, bool kStats, bool kSingle, int kStride, int kLds = kLdsDepth, int kBatch = 1>
 __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd, float tmax, int* const lstack) {
     int spill[kStackCap - kLds];
     int sp = 0;
@@ -394,10 +399,51 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
         // the primitives of a leaf, one after the other
         bool descend = false;
         while (nodeAddr >= numInternal && primCount > 0) {
+            if (kBatch > 1 && (kSingle || inObj)) {
+                // Triangles in batches of kBatch: every load of the batch is
+                // issued before the first test, so a long leaf costs one memory
+                // latency per batch instead of one per triangle. The tests and
+                // hit updates still run in leaf order (the result and the counts
+                // are those of testing one at a time).
+                const int nb = primCount < kBatch ? primCount : kBatch;
+                const int first = nodeAddr - numInternal + primOffset;
+                float4 ta[kBatch], tb[kBatch], tc[kBatch];
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    if (j < nb) {
+                        const size_t id = (size_t)(first + j);
+                        ta[j] = tris[3 * id];
+                        tb[j] = tris[3 * id + 1];
+                        tc[j] = tris[3 * id + 2];
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kBatch; ++j) {
+                    if (j >= nb || done) continue;
+                    float t, w0, w1;
+                    const int prim = __float_as_int(ta[j].w);
+                    if (kStats) ++primTests;
+                    if (ray_triangle(cur, v3(ta[j].x, ta[j].y, ta[j].z), v3(tb[j].x, tb[j].y, tb[j].z),
+                                     v3(tc[j].x, tc[j].y, tc[j].z), t, w0, w1) &&
+                        t > 0.0f && better(t, curInst, prim, hitT, hInst, hPrim)) {
+                        hitT = t;
+                        hInst = curInst;
+                        hPrim = prim;
+                        hw0 = w0;
+                        hw1 = w1;
+                        if (kAny) done = true;
+                    }
+                }
+                primOffset += nb;
+                primCount -= nb;
+                if (done) break;
+                continue;
+            }
             const int leafId = nodeAddr - numInternal + primOffset;
             ++primOffset;
             --primCount;
             if (!kSingle && !inObj) {
+                // an instance of the top level: enter its object
                 const int inst = p.tlasPrimIds[leafId];
                 DevLod lod;
                 const ObjectEntry e = enter_object(p, inst, wo, wd, hitT, lod);
@@ -443,6 +489,7 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
                 descend = true;
                 break;
             }
+            // one triangle at a time (kBatch == 1)
             const float4 a = tris[3 * (size_t)leafId], b = tris[3 * (size_t)leafId + 1],
                          c = tris[3 * (size_t)leafId + 2];
             float t, w0, w1;

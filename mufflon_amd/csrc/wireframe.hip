@@ -67,7 +67,8 @@ __global__ __launch_bounds__(kWfThreads) void k_wireframe(PtScene S, const u32* 
     acc.init(1u << kTargetRadiance);  // the Border target rides in the first slot
     float totalDistance = 0.0f;
     for (int i = 0; i < maxTraceDepth; ++i) {
-        const TraceHit h = trace_ray<false, false, kSingle, kWfThreads>(S.trace, origin, dir, kMaxSceneSize, lstack);
+        const TraceHit h = trace_ray<false, false, kSingle, kWfThreads, kLdsDepth, kPtLeafBatch>(S.trace, origin, dir,
+                                                                                                 kMaxSceneSize, lstack);
         if (h.inst == -1) {
             float pdfBack;
             const V3 bg = evaluate_background(S, dir, pdfBack);

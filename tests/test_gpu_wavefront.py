@@ -60,16 +60,24 @@ def test_wavefront_instanced_and_every_tuning(hip_ctx):
     hip_ctx.pt_iterate(pt, 2)
     ref = hip_ctx.pt_target(pt)
     hip_ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
-    for block, depth in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8), (256, 16), (256, 24)]:
+    shapes = [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8), (256, 16), (256, 24)]
+    for (block, depth), leaf in [(sh, 1) for sh in shapes] + [(sh, 8) for sh in shapes]:
         for chunk, sort, refill in ((64, 0, 0), (1024, 3, 0), (256, 0, 16), (256, 2, 3)):
-            hip_ctx.pt_set_tuning(pt, block, depth, chunk, sort, refill)
+            hip_ctx.pt_set_tuning(pt, block, depth, chunk, sort, refill, leaf)
             hip_ctx.pt_reset(pt)
             hip_ctx.pt_iterate(pt, 1)
             hip_ctx.pt_iterate(pt, 1)
             got = hip_ctx.pt_target(pt)
-            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, chunk, sort, refill)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, leaf, chunk, sort, refill)
     assert hip_ctx.pt_get_tuning(pt) == dict(traceBlock=256, stackDepth=24, queueChunk=256, sortBounces=2,
-                                             refillBudget=3)
+                                             refillBudget=3, leafBatch=8)
+    # the megakernel with 8 triangles of a leaf loaded ahead
+    hip_ctx.pt_set_mode(pt, hip.PT_MEGAKERNEL)
+    hip_ctx.pt_reset(pt)
+    hip_ctx.pt_iterate(pt, 2)
+    assert np.array_equal(hip_ctx.pt_target(pt).view(np.uint32), ref.view(np.uint32))
+    with pytest.raises(hip.HipError):
+        hip_ctx.pt_set_tuning(pt, 256, 16, 256, 0, 0, 4)
 
 
 def test_autotune_picks_a_shape_and_resets(hip_ctx):
