@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--trace-block", type=int, default=256, help="wavefront trace block (with --pt-profile)")
     ap.add_argument("--stack-depth", type=int, default=16, help="wavefront LDS stack depth (with --pt-profile)")
     ap.add_argument("--refill", type=int, default=0, help="wavefront lane refill budget (with --pt-profile)")
+    ap.add_argument("--terrain-ray-costs", action="store_true",
+                    help="only the per-ray traversal cost of the terrain's camera rays and the trace time of its "
+                         "heaviest rays")
     ap.add_argument("--leaf-batch", type=int, default=1, choices=[1, 8],
                     help="triangles of a BVH leaf loaded ahead of their tests (with --pt-profile)")
     ap.add_argument("--profile-only", action="store_true",
@@ -333,6 +336,56 @@ def pt_batch(args):
             "albedo_gradients_8_scenes_s": round(adj_s, 3)}
 
 
+def terrain_ray_costs():
+    """Closest hit of the terrain heightfield (one instance) for its 1920x1080
+    pinhole camera rays: per-ray internal nodes visited and triangles tested
+    (the stats variant of the trace kernel), and the kernel time of all rays,
+    of the heaviest 0.01 / 0.1 / 1 % of them (by nodes + triangles) and of the
+    rest, to show how much of a launch the slowest rays take."""
+    from mufflon_amd import scenes as S
+    from mufflon_amd.hip import HipContext
+    c = S.terrain_scene()
+    cam = c["camera"]
+    eye = cam["position"].astype(np.float64)
+    vfov = 2 * np.degrees(np.arctan(float(cam["tan_vfov"])))
+    rays = S.pinhole_rays(cam["width"], cam["height"], eye=eye, target=eye + cam["view_dir"] * 100.0,
+                          vfov_deg=vfov)
+    with HipContext(0) as ctx:
+        L = c["lods"][0]
+        lod = ctx.lod_create(L["verts"], L["tris"])
+        ctx.lbvh_build(lod)
+        sc = ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+        nodes, tris = ctx.trace_stats(sc, rays)
+        cost = nodes.astype(np.int64) + tris
+
+        def timed(rs):
+            dr = ctx.to_device(np.ascontiguousarray(rs))
+            dh = ctx.alloc(16 * len(rs))
+            for _ in range(2):
+                ctx.trace_closest_dev(sc, dr, len(rs), dh)
+            ts = []
+            for _ in range(5):
+                ctx.timer_begin()
+                ctx.trace_closest_dev(sc, dr, len(rs), dh)
+                ts.append(ctx.timer_end())
+            dr.free()
+            dh.free()
+            return min(ts)
+
+        print("%d rays: nodes mean %.1f p99 %d max %d; triangles mean %.1f p99 %d max %d" % (
+            len(rays), nodes.mean(), np.percentile(nodes, 99), nodes.max(), tris.mean(), np.percentile(tris, 99),
+            tris.max()), flush=True)
+        print("all rays: %.3f ms" % timed(rays), flush=True)
+        order = np.argsort(-cost, kind="stable")
+        for frac in (0.0001, 0.001, 0.01):
+            k = max(64, int(len(rays) * frac))
+            print("heaviest %d rays (nodes + triangles >= %d): %.3f ms; the other %d rays: %.3f ms" % (
+                k, cost[order[k - 1]], timed(rays[np.sort(order[:k])]), len(rays) - k,
+                timed(rays[np.sort(order[k:])])), flush=True)
+        print("heaviest ray alone (%d nodes, %d triangles): %.4f ms" % (
+            nodes[order[0]], tris[order[0]], timed(rays[order[:1]])), flush=True)
+
+
 def pt_profile(args):
     """A short run of one path-tracer workload for rocprofv3: `terrain`
     (megakernel then wavefront with the given tuning, 2 iterations each after
@@ -369,6 +422,9 @@ def main():
         sys.exit(2)
     if args.pt_profile:
         pt_profile(args)
+        return
+    if args.terrain_ray_costs:
+        terrain_ray_costs()
         return
     from mufflon_amd import scenes as S
     from mufflon_amd.hip import HipContext
