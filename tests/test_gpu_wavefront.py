@@ -95,3 +95,57 @@ def test_autotune_picks_a_shape_and_resets(hip_ctx):
     hip_ctx.pt_reset(pt)
     hip_ctx.pt_iterate(pt, 2)
     assert np.array_equal(a.view(np.uint32), hip_ctx.pt_target(pt).view(np.uint32))
+
+
+def _with_stack(c, levels=160):
+    """The Cornell box plus a stack of `levels` pairs of identical horizontal
+    triangles 1e-4 extents apart: their boxes overlap so much that the SAH
+    collapse keeps them in long leaves (the closest hit inside one is a
+    tie between the two copies, won by the lower primitive index)."""
+    box = P.scene_box(c)
+    lo, hi = box[:3].astype(np.float64), box[3:].astype(np.float64)
+    ext = float((hi - lo).max())
+    ctr = 0.5 * (lo + hi)
+    y = lo[1] + 0.3 * ext + np.arange(levels) * 1e-4 * ext
+    tri = np.array([[-0.2, 0.0, -0.15], [0.2, 0.0, -0.15], [0.0, 0.0, 0.2]]) * ext
+    v = np.repeat(tri[None], levels, 0) + np.stack([np.full(levels, ctr[0]), y, np.full(levels, ctr[2])], -1)[:, None]
+    v = np.repeat(v, 2, 0).reshape(-1, 3).astype(np.float32)
+    base = len(c["verts"])
+    tris = (base + np.arange(len(v), dtype=np.uint32)).reshape(-1, 3)
+    n = np.tile(np.array([0, 1, 0], np.float32), (len(v), 1))
+    out = dict(c)
+    out["verts"] = np.concatenate([c["verts"], v])
+    out["normals"] = np.concatenate([c["normals"], n])
+    out["uvs"] = np.concatenate([c["uvs"], np.zeros((len(v), 2), np.float32)])
+    out["tris"] = np.concatenate([c["tris"], tris]).astype(c["tris"].dtype)
+    out["mat_idx"] = np.concatenate([c["mat_idx"], np.full(len(tris), c["mat_idx"][0], c["mat_idx"].dtype)])
+    return out
+
+
+def test_leaf_batch_on_long_leaves(hip_ctx, oracle):
+    """Leaves of up to hundreds of triangles (partial batches of 8, any-hit
+    shadow rays that stop inside a batch, ties between identical triangles):
+    megakernel and wavefront with leaf batch 1 and 8 give the oracle's bits."""
+    c = _with_stack(P.cornell(8, 16, 48))
+    print("long leaves: %d triangles" % len(c["tris"]), flush=True)
+    lod = hip_ctx.lod_create(c["verts"], c["tris"])
+    hip_ctx.lod_set_shading(lod, c["normals"], c["uvs"], c["mat_idx"])
+    hip_ctx.lbvh_build(lod, P.scene_box(c))
+    sc = hip_ctx.scene_create(np.array([lod], np.int32), S.IDENTITY34.reshape(1, 12))
+    pt = hip_ctx.pt_create(sc, c["materials"], c["camera"],
+                           dict(seed=1, min_path_length=0, max_path_length=6, nee_count=2))
+    # the BVH really holds long leaves: 64-byte nodes, child (index, count) in dwords 6-7 and 14-15
+    n_int, nodes, _ = hip_ctx.lbvh_get(lod, len(c["tris"]))
+    q = nodes.view(np.int32).reshape(-1, 16)
+    counts = np.concatenate([q[:, 7][q[:, 6] >= n_int], q[:, 15][q[:, 14] >= n_int]])
+    print("largest leaf: %d triangles" % counts.max(), flush=True)
+    assert counts.max() >= 64
+    ps = P.oracle_pt_scene(oracle, c, hip_ctx.pt_light_tree(pt))
+    o_rad, o_hit, _ = oracle.pt_render(ps, 1, 0, 6, 2, 3)
+    for mode, leaf in ((hip.PT_MEGAKERNEL, 1), (hip.PT_MEGAKERNEL, 8), (hip.PT_WAVEFRONT, 8), (hip.PT_WAVEFRONT, 1)):
+        hip_ctx.pt_set_mode(pt, mode, False)
+        hip_ctx.pt_set_tuning(pt, 128, 8, 256, 0, 0, leaf)
+        hip_ctx.pt_reset(pt)
+        hip_ctx.pt_iterate(pt, 3)
+        assert np.array_equal(hip_ctx.pt_target(pt, hip.TARGET_HIT_ID), o_hit), (mode, leaf)
+        assert np.array_equal(hip_ctx.pt_target(pt).view(np.uint32), o_rad.view(np.uint32)), (mode, leaf)
