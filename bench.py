@@ -141,7 +141,8 @@ def cornell_tracer(ctx):
 
 def pt_cornell(args, cpu_baseline_too):
     """Megakernel path tracer on the procedural Cornell box: 256x256, 128x256
-    sphere, depth 8, NEE 1, seed 1; Mpaths/s = pixels * iterations / time."""
+    sphere, depth 8, NEE 1, seed 1; Mpaths/s = pixels * iterations / time.
+    Also the megakernel with leaf batch 8 and the autotuned wavefront form."""
     from mufflon_amd import scenes as S
     from mufflon_amd.hip import HipContext
     ctx = HipContext(0)
@@ -160,7 +161,17 @@ def pt_cornell(args, cpu_baseline_too):
            "iterations": n, "ms_per_iteration": round(ms / n, 4),
            "mpaths_s": round(pixels * n / (ms / 1e3) / 1e6, 3)}
     from mufflon_amd import hip
+    # the megakernel with 8 triangles of a BVH leaf loaded ahead of their tests
+    ctx.pt_set_tuning(pt, leaf_batch=8)
+    ctx.pt_reset(pt)
+    ctx.timer_begin()
+    ctx.pt_iterate(pt, n)
+    m8 = ctx.timer_end()
+    same = np.array_equal(ctx.pt_target(pt), mega)
+    out["megakernel_leaf_batch_8"] = {"ms_per_iteration": round(m8 / n, 4),
+                                      "mpaths_s": round(pixels * n / (m8 / 1e3) / 1e6, 3)}
     ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
+    tuned, _ = ctx.pt_autotune(pt, 2)
     ctx.pt_reset(pt)
     ctx.pt_iterate(pt, 2)
     ctx.pt_reset(pt)
@@ -168,7 +179,8 @@ def pt_cornell(args, cpu_baseline_too):
     ctx.pt_iterate(pt, n)
     wms = ctx.timer_end()
     out["wavefront"] = {"ms_per_iteration": round(wms / n, 4), "mpaths_s": round(pixels * n / (wms / 1e3) / 1e6, 3),
-                        "identical_to_megakernel": bool(np.array_equal(ctx.pt_target(pt), mega))}
+                        "tuning": tuned, "graph": True,
+                        "identical_to_megakernel": bool(same and np.array_equal(ctx.pt_target(pt), mega))}
     out["rmse"] = converged_rmse(ctx, pt, mega, dict(max_path_length=8, nee_count=1), args.reference_batches)
     if cpu_baseline_too:
         from oracle import oracle as O
