@@ -541,4 +541,116 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
     return h;
 }
 
+// Closest hit of a single-instance scene with postponed leaves (Aila and
+// Laine 2009, "speculative while-while"): a lane that reaches a leaf keeps it
+// and goes on descending through inner nodes until it reaches a second leaf
+// or every active lane of its wave holds one; then the held leaf is tested.
+// Node steps and leaf tests of a wave thus run with more lanes active. The
+// visiting order differs from trace_ray's, which does not change the result:
+// inside one object the closest hit is the lexicographic minimum of
+// (t, primitive) over every triangle whose box path passes the slab tests,
+// and a box is culled only when its entry distance exceeds the current best
+// (the collapse enlarges boxes, so a triangle never lies closer than its
+// box's entry). No counts (the stats kernels use trace_ray).
+template <int kStride, int kLds = kLdsDepth>
+__device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 wo, V3 wd, float tmax,
+                                                       int* const lstack) {
+    int spill[kStackCap - kLds];
+    int sp = 0;
+    bool overflow = false;
+    auto push = [&](int v) {
+        if (sp < kLds)
+            lstack[sp * kStride] = v;
+        else if (sp < kStackCap)
+            spill[sp - kLds] = v;
+        if (sp < kStackCap)
+            ++sp;
+        else
+            overflow = true;
+    };
+    auto pop = [&]() {
+        --sp;
+        return sp < kLds ? lstack[sp * kStride] : spill[sp - kLds];
+    };
+    TraceHit h;
+    h.inst = h.prim = -1;
+    h.w0 = h.w1 = 0.0f;
+    h.nodeVisits = h.primTests = 0;
+    DevLod lod;
+    const ObjectEntry e = enter_object(p, 0, wo, wd, tmax, lod);
+    if (!e.hit) {
+        h.t = tmax;
+        h.overflow = false;
+        return h;
+    }
+    const FastRay cur = e.ray;
+    float hitT = e.tmax;
+    const BvhChild* const bvh = (const BvhChild*)lod.nodes;
+    const float4* const tris = lod.triVerts;
+    const int numInternal = lod.numInternal;
+    // current node (count 2 = inner node, n = leaf of n triangles, 0 = none)
+    int nodeAddr = 0, primCount = lod.numPrims == 1 ? 1 : 2;
+    int heldAddr = 0, heldCount = 0;  // the postponed leaf
+    while (true) {
+        while (true) {
+            if (primCount > 0 && nodeAddr < numInternal) {
+                const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
+                const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+                const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
+                const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
+                const int lIdx = __float_as_int(q1.z), lCnt = __float_as_int(q1.w);
+                const int rIdx = __float_as_int(q3.z), rCnt = __float_as_int(q3.w);
+                float c0, c1;
+                const bool t0 = slab(lLo, lHi, cur, hitT, c0);
+                const bool t1 = slab(rLo, rHi, cur, hitT, c1);
+                if (!t0) c0 = 1e38f;
+                if (!t1) c1 = 1e38f;
+                if (t0 && t1) {
+                    const bool farLeft = c1 < c0;
+                    const int far = farLeft ? lIdx : rIdx;
+                    if (far >= numInternal) push(farLeft ? lCnt : rCnt);
+                    push(far);
+                }
+                const bool nearLeft = c0 <= c1;
+                nodeAddr = nearLeft ? lIdx : rIdx;
+                primCount = (!t0 && !t1) ? 0 : (nodeAddr >= numInternal ? (nearLeft ? lCnt : rCnt) : 2);
+                if (overflow) break;
+            }
+            if (primCount > 0 && nodeAddr >= numInternal) {
+                if (heldCount > 0) break;  // a second leaf: test the held one first
+                heldAddr = nodeAddr;
+                heldCount = primCount;
+                primCount = 0;
+            }
+            if (primCount == 0) {
+                if (sp == 0) break;
+                nodeAddr = pop();
+                primCount = nodeAddr >= numInternal ? pop() : 2;
+            }
+            // every active lane holds a leaf: test them together
+            if (__builtin_amdgcn_read_exec() == __ballot(heldCount > 0)) break;
+        }
+        if (overflow) break;
+        for (int k = 0; k < heldCount; ++k) {
+            const size_t id = (size_t)(heldAddr - numInternal + k);
+            const float4 a = tris[3 * id], b = tris[3 * id + 1], c = tris[3 * id + 2];
+            float t, w0, w1;
+            const int prim = __float_as_int(a.w);
+            if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
+                better(t, 0, prim, hitT, h.inst, h.prim)) {
+                hitT = t;
+                h.inst = 0;
+                h.prim = prim;
+                h.w0 = w0;
+                h.w1 = w1;
+            }
+        }
+        heldCount = 0;
+        if (primCount == 0 && sp == 0) break;
+    }
+    h.t = hitT / e.scale;  // object_to_world_space
+    h.overflow = overflow;
+    return h;
+}
+
 }  // namespace mfh
