@@ -70,8 +70,12 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
         } else if (!walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, dir, pdfF, absorption)) {
             break;
         }
-        const TraceHit hit = trace_ray<false, false, kSingle, kPtThreads, kLdsDepth, kLeaf>(
-            S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
+        TraceHit hit;
+        if constexpr (kSingle)
+            hit = trace_closest_held<kPtThreads>(S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
+        else
+            hit = trace_ray<false, false, kSingle, kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir,
+                                                                                 kMaxSceneSize, lstack);
         throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
         footA += 1.0f / fsqrt(pdfF);
         guideWeight = 1.f / (1.f + footA * footA / 1.0f);

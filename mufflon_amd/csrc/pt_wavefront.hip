@@ -140,6 +140,7 @@ __device__ __forceinline__ int locate(const WfState& W, int p, int& local) {
 // per lane from the ray's scene (batches mix single- and multi-instance scenes).
 template <int kMode, bool kAny, int kStride, int kLds, int kLeaf>
 __device__ __forceinline__ TraceHit trace_mode(const TraceParams& tp, V3 o, V3 d, float tmax, int* lstack) {
+    if constexpr (kMode == 1 && !kAny) return trace_closest_held<kStride, kLds>(tp, o, d, tmax, lstack);
     if constexpr (kMode == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
     if constexpr (kMode == 0) return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
     if (tp.numInstances == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);

@@ -544,7 +544,9 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
 // Closest hit of a single-instance scene with postponed leaves (Aila and
 // Laine 2009, "speculative while-while"): a lane that reaches a leaf keeps it
 // and goes on descending through inner nodes until it reaches a second leaf
-// or every active lane of its wave holds one; then the held leaf is tested.
+// or half of the active lanes of its wave hold one; then the held leaves are
+// tested. (1M-triangle benchmark: every lane 976, 7/8 994, 3/4 1041, 5/8 1086,
+// 1/2 1122, 3/8 1106, 1/4 1073 Mrays/s.)
 // Node steps and leaf tests of a wave thus run with more lanes active. The
 // visiting order differs from trace_ray's, which does not change the result:
 // inside one object the closest hit is the lexicographic minimum of
@@ -627,8 +629,8 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
                 nodeAddr = pop();
                 primCount = nodeAddr >= numInternal ? pop() : 2;
             }
-            // every active lane holds a leaf: test them together
-            if (__builtin_amdgcn_read_exec() == __ballot(heldCount > 0)) break;
+            // half of the active lanes hold a leaf: test them together
+            if (2 * __popcll(__ballot(heldCount > 0)) >= __popcll(__builtin_amdgcn_read_exec())) break;
         }
         if (overflow) break;
         for (int k = 0; k < heldCount; ++k) {
