@@ -72,10 +72,11 @@ __global__ __launch_bounds__(kPtThreads) void k_pt_iterate(PtScene S) {
         }
         TraceHit hit;
         if constexpr (kSingle)
-            hit = trace_closest_held<kPtThreads>(S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
+            hit = trace_closest_held<kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir, kMaxSceneSize,
+                                                                         lstack);
         else
-            hit = trace_ray<false, false, kSingle, kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir,
-                                                                                 kMaxSceneSize, lstack);
+            hit = trace_ray<false, false, false, kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir, kMaxSceneSize,
+                                                                              lstack);
         throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
         footA += 1.0f / fsqrt(pdfF);
         guideWeight = 1.f / (1.f + footA * footA / 1.0f);

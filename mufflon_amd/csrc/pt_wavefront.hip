@@ -140,10 +140,15 @@ __device__ __forceinline__ int locate(const WfState& W, int p, int& local) {
 // per lane from the ray's scene (batches mix single- and multi-instance scenes).
 template <int kMode, bool kAny, int kStride, int kLds, int kLeaf>
 __device__ __forceinline__ TraceHit trace_mode(const TraceParams& tp, V3 o, V3 d, float tmax, int* lstack) {
-    if constexpr (kMode == 1 && !kAny) return trace_closest_held<kStride, kLds>(tp, o, d, tmax, lstack);
+    // closest hits of single-instance scenes hold leaves (trace_closest_held)
+    if constexpr (kMode == 1 && !kAny)
+        return trace_closest_held<kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
     if constexpr (kMode == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
     if constexpr (kMode == 0) return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-    if (tp.numInstances == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    if (tp.numInstances == 1) {
+        if constexpr (!kAny) return trace_closest_held<kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+        return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    }
     return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
 }
 
@@ -187,7 +192,8 @@ __global__ __launch_bounds__(256) void k_wf_primary(PtScene S0, WfState W) {
     float pdfF;
     camera_ray(S, px, py, x0, dir, pdfF);
     const V3 o = v3(S.camPos[0], S.camPos[1], S.camPos[2]);
-    const TraceHit h = trace_mode<kMode, false, 256, kLdsDepth, kLeaf>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
+    const TraceHit h =
+        trace_mode<kMode, false, 256, kLdsDepth, kLeaf>(S.trace, o, dir, kMaxSceneSize, sStack + threadIdx.x);
     S.rngs[local] = rng;
     W.rayO[0][p] = float4{o.x, o.y, o.z, pdfF};
     W.rayD[0][p] = float4{dir.x, dir.y, dir.z, __int_as_float(p)};
@@ -217,8 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_extend(PtScene S0, WfState W, int
             int local;
             const int s = locate<kBatch>(W, __float_as_int(d.w), local);
             const PtScene& S = scene_ref<kBatch>(S0, W, s);
-            const TraceHit h = trace_mode<kMode, false, kBlock, kLds, kLeaf>(S.trace, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z),
-                                                                      kMaxSceneSize, lstack);
+            const TraceHit h = trace_mode<kMode, false, kBlock, kLds, kLeaf>(S.trace, v3(o.x, o.y, o.z),
+                                                                             v3(d.x, d.y, d.z), kMaxSceneSize, lstack);
             W.hit[buf][q] = float4{h.t, __int_as_float(h.inst), __int_as_float(h.prim), h.w0};
             W.rayM[buf][q].w = h.w1;
         }
@@ -248,8 +254,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow(PtScene S0, WfState W, int
                 const size_t e = (size_t)sl * W.neeStride + i;
                 const float4 a = W.shO[e];
                 const float4 d = W.shD[e];
-                const TraceHit h = trace_mode<kMode, true, kBlock, kLds, kLeaf>(S.trace, v3(a.x, a.y, a.z), v3(d.x, d.y, d.z),
-                                                                         a.w, lstack);
+                const TraceHit h = trace_mode<kMode, true, kBlock, kLds, kLeaf>(S.trace, v3(a.x, a.y, a.z),
+                                                                                v3(d.x, d.y, d.z), a.w, lstack);
                 if (h.inst == -1) visible |= 1u << i;
             }
             if (visible) {  // add the unoccluded estimates in candidate order

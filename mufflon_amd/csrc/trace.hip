@@ -21,7 +21,8 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
     const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
     TraceHit h;
     if constexpr (kSingle && !kAny && !kStats)
-        h = trace_closest_held<kThreads>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w, sStack + threadIdx.x);
+        h = trace_closest_held<kThreads>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w,
+                                                  sStack + threadIdx.x);
     else
         h = trace_ray<kAny, kStats, kSingle, kThreads>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w,
                                                        sStack + threadIdx.x);

@@ -545,16 +545,18 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
 // Laine 2009, "speculative while-while"): a lane that reaches a leaf keeps it
 // and goes on descending through inner nodes until it reaches a second leaf
 // or half of the active lanes of its wave hold one; then the held leaves are
-// tested. (1M-triangle benchmark: every lane 976, 7/8 994, 3/4 1041, 5/8 1086,
-// 1/2 1122, 3/8 1106, 1/4 1073 Mrays/s.)
-// Node steps and leaf tests of a wave thus run with more lanes active. The
-// visiting order differs from trace_ray's, which does not change the result:
-// inside one object the closest hit is the lexicographic minimum of
+// tested (kBatch triangles loaded at a time, as in trace_ray). Node steps and
+// leaf tests of a wave thus run with more lanes active. (1M-triangle
+// benchmark, test when: every lane holds 976, 7/8 994, 3/4 1041, 5/8 1086,
+// 1/2 1122, 3/8 1106, 1/4 1073 Mrays/s. On the two-level terrain, holding
+// inside objects was slower, so two-level scenes keep trace_ray.)
+// The visiting order differs from trace_ray's, which does not change the
+// result: inside one object the closest hit is the lexicographic minimum of
 // (t, primitive) over every triangle whose box path passes the slab tests,
 // and a box is culled only when its entry distance exceeds the current best
 // (the collapse enlarges boxes, so a triangle never lies closer than its
 // box's entry). No counts (the stats kernels use trace_ray).
-template <int kStride, int kLds = kLdsDepth>
+template <int kStride, int kLds = kLdsDepth, int kBatch = 1>
 __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 wo, V3 wd, float tmax,
                                                        int* const lstack) {
     int spill[kStackCap - kLds];
@@ -633,18 +635,33 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
             if (2 * __popcll(__ballot(heldCount > 0)) >= __popcll(__builtin_amdgcn_read_exec())) break;
         }
         if (overflow) break;
-        for (int k = 0; k < heldCount; ++k) {
-            const size_t id = (size_t)(heldAddr - numInternal + k);
-            const float4 a = tris[3 * id], b = tris[3 * id + 1], c = tris[3 * id + 2];
-            float t, w0, w1;
-            const int prim = __float_as_int(a.w);
-            if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
-                better(t, 0, prim, hitT, h.inst, h.prim)) {
-                hitT = t;
-                h.inst = 0;
-                h.prim = prim;
-                h.w0 = w0;
-                h.w1 = w1;
+        for (int k = 0; k < heldCount; k += kBatch) {
+            const int nb = heldCount - k < kBatch ? heldCount - k : kBatch;
+            const int first = heldAddr - numInternal + k;
+            float4 ta[kBatch], tb[kBatch], tc[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                if (j < nb) {
+                    const size_t id = (size_t)(first + j);
+                    ta[j] = tris[3 * id];
+                    tb[j] = tris[3 * id + 1];
+                    tc[j] = tris[3 * id + 2];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                if (j >= nb) continue;
+                float t, w0, w1;
+                const int prim = __float_as_int(ta[j].w);
+                if (ray_triangle(cur, v3(ta[j].x, ta[j].y, ta[j].z), v3(tb[j].x, tb[j].y, tb[j].z),
+                                 v3(tc[j].x, tc[j].y, tc[j].z), t, w0, w1) &&
+                    t > 0.0f && better(t, 0, prim, hitT, h.inst, h.prim)) {
+                    hitT = t;
+                    h.inst = 0;
+                    h.prim = prim;
+                    h.w0 = w0;
+                    h.w1 = w1;
+                }
             }
         }
         heldCount = 0;
