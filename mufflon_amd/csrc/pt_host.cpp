@@ -33,10 +33,14 @@ struct PathTracer {
     bool useGraph = false;
     WfTuning tune;
     DevBuf wfCounters, wfTp, wfAcc, wfRay[2][4], wfRayP[2], wfSh[3], wfShPath, wfSortKeys[2], wfSortIds[2];
+    // digit counts and scan partials of this tracer's queue sort: its own,
+    // never the context's build scratch, which an LBVH build may reallocate
+    // under a captured graph
+    BuildScratch wfSortScratch;
     int wfPaths = 0, wfMaxLen = -1, wfNee = -1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
-    std::vector<uint8_t> graphKey;  // bytes of (S, W, tune) the graph was captured with
+    std::vector<uint8_t> graphKey;  // bytes of (S, W, tune, sort scratch) the graph was captured with
     // adjoint
     DevBuf adjRngs, adjDelta, adjEmGrad, adjTexGradPtrs;
     std::vector<DevBuf> adjTexGrad;  // per texture id
@@ -63,7 +67,8 @@ void drop_graph(PathTracer& t) {
 void release_wavefront(PathTracer& t) {
     drop_graph(t);
     for (DevBuf* b : {&t.wfCounters, &t.wfTp, &t.wfAcc, &t.wfSh[0], &t.wfSh[1], &t.wfSh[2], &t.wfShPath,
-                      &t.wfSortKeys[0], &t.wfSortKeys[1], &t.wfSortIds[0], &t.wfSortIds[1]})
+                      &t.wfSortKeys[0], &t.wfSortKeys[1], &t.wfSortIds[0], &t.wfSortIds[1],
+                      &t.wfSortScratch.hist, &t.wfSortScratch.partials})
         b->release();
     for (auto& r : t.wfRay)
         for (DevBuf& b : r) b.release();
@@ -403,10 +408,10 @@ WfState wavefront_state(Context& c, PathTracer& t) {
         W.sortKeys[b] = t.wfSortKeys[b].as<uint64_t>();
         W.sortIds[b] = t.wfSortIds[b].as<int32_t>();
     }
-    // radix-sort scratch of the context sized up front (no allocation inside a captured graph)
+    // radix-sort scratch of the tracer sized up front (no allocation inside a captured graph)
     const size_t tiles = ((size_t)N + 4095) / 4096;
-    c.scratch.hist.reserve(4 * 256 * tiles);
-    c.scratch.partials.reserve(4 * (256 * tiles / 2048 + 64));
+    t.wfSortScratch.hist.reserve(4 * 256 * tiles);
+    t.wfSortScratch.partials.reserve(4 * (256 * tiles / 2048 + 64));
     if (t.tune.traceGrid <= 0 || t.tune.shadeGrid <= 0) {
         const int cus = num_cus(c);
         t.tune.traceGrid = 0;
@@ -425,23 +430,28 @@ void run_wavefront(Context& c, PathTracer& t, PtScene& S, int32_t iterations) {
     if (!t.useGraph) {
         for (int32_t k = 0; k < iterations; ++k) {
             S.iteration = ++t.iteration;
-            launch_pt_wavefront(S, W, t.tune, c.scratch, c.stream);
+            launch_pt_wavefront(S, W, t.tune, t.wfSortScratch, c.stream);
         }
         MFH_HIP(hipStreamSynchronize(c.stream));  // the host copy of the iteration counter is read above
         return;
     }
     PtScene Sk = S;
     Sk.iteration = 0;  // the graph reads the device counter
-    std::vector<uint8_t> key(sizeof(Sk) + sizeof(W) + sizeof(WfTuning));
-    std::memcpy(key.data(), &Sk, sizeof(Sk));
-    std::memcpy(key.data() + sizeof(Sk), &W, sizeof(W));
-    std::memcpy(key.data() + sizeof(Sk) + sizeof(W), &t.tune, sizeof(WfTuning));
+    // every device address the captured kernels read is in the key, the
+    // sort scratch included, so a reallocated buffer forces a new capture
+    const void* sortScratch[2] = {t.wfSortScratch.hist.ptr, t.wfSortScratch.partials.ptr};
+    std::vector<uint8_t> key(sizeof(Sk) + sizeof(W) + sizeof(WfTuning) + sizeof(sortScratch));
+    uint8_t* kp = key.data();
+    std::memcpy(kp, &Sk, sizeof(Sk));
+    std::memcpy(kp += sizeof(Sk), &W, sizeof(W));
+    std::memcpy(kp += sizeof(W), &t.tune, sizeof(WfTuning));
+    std::memcpy(kp += sizeof(WfTuning), sortScratch, sizeof(sortScratch));
     if (!t.graphExec || key != t.graphKey) {
         drop_graph(t);
         MFH_HIP(hipStreamSynchronize(c.stream));
         MFH_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         try {
-            launch_pt_wavefront(Sk, W, t.tune, c.scratch, c.stream);
+            launch_pt_wavefront(Sk, W, t.tune, t.wfSortScratch, c.stream);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(c.stream, &g);
