@@ -587,3 +587,80 @@ def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iter
     lib.orc_pt_adjoint(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
                        _p(d, f32p), ptrs, em.ctypes.data_as(dp))
     return grads, em
+
+
+def set_mutation(m):
+    """Switches on one planted estimator mistake of the float oracle (0 = none)
+    for the closed-form estimator tests; returns the previous setting."""
+    lib = load()
+    lib.orc_set_mutation.argtypes = [C.c_int32]
+    lib.orc_set_mutation.restype = C.c_int
+    return lib.orc_set_mutation(int(m))
+
+
+class Fp64Scene(C.Structure):
+    _fields_ = [
+        ("verts", f32p), ("numVerts", C.c_int32), ("tris", u32p), ("numTris", C.c_int32), ("normals", f32p),
+        ("matIdx", C.c_void_p), ("matType", i32p), ("albedo", f32p), ("emission", f32p),
+        ("numMaterials", C.c_int32), ("pointLights", f32p), ("numPointLights", C.c_int32),
+        ("background", C.c_float * 3), ("camPos", C.c_float * 3), ("camDir", C.c_float * 3),
+        ("camUp", C.c_float * 3), ("tanVFov", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+    ]
+
+
+def fp64_render(verts, normals, tris, mat_idx, materials, camera, max_path_length, spp, seed=1,
+                point_lights=None, background=(0.0, 0.0, 0.0)):
+    """Independent FP64 estimator (pt_fp64.cpp) of the path tracer's pixel
+    integral for Lambert / emissive scenes with point lights (records with
+    'position' and 'intensity') and a uniform background radiance. Returns the
+    per-pixel mean (h, w, 3) and the variance of one sample (h, w, 3), float64."""
+    lib = load()
+    dp = C.POINTER(C.c_double)
+    lib.orc64_render.argtypes = [C.POINTER(Fp64Scene), C.c_int32, C.c_int32, C.c_uint64, dp, dp]
+    lib.orc64_render.restype = C.c_int
+    mats = np.asarray(materials)
+    if (mats["albedoTexture"] >= 0).any():
+        raise ValueError("the FP64 estimator takes constant albedos only")
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a
+
+    mt = arr(mats["type"], np.int32)
+    if not np.isin(mt, (0, 1)).all():
+        raise ValueError("the FP64 estimator takes Lambert and emissive materials only")
+    alb = arr(mats["albedo"], np.float32)
+    em = arr(mats["radiance"].astype(np.float64) * mats["scale"].astype(np.float64), np.float32)
+    v = arr(verts, np.float32)
+    n = arr(normals, np.float32)
+    t = arr(tris, np.uint32)
+    mi = arr(mat_idx, np.uint16)
+    pl = np.zeros((0, 6), np.float32)
+    if point_lights is not None and len(point_lights):
+        pl = np.concatenate([np.asarray(point_lights["position"], np.float32),
+                             np.asarray(point_lights["intensity"], np.float32)], 1)
+    pl = arr(pl.reshape(-1) if len(pl) else np.zeros(6, np.float32), np.float32)
+    s = Fp64Scene()
+    s.verts, s.numVerts = _p(v, f32p), len(v)
+    s.tris, s.numTris = _p(t, u32p), len(t)
+    s.normals = _p(n, f32p)
+    s.matIdx = mi.ctypes.data
+    s.matType, s.albedo, s.emission, s.numMaterials = _p(mt, i32p), _p(alb, f32p), _p(em, f32p), len(mats)
+    s.pointLights = _p(pl, f32p)
+    s.numPointLights = 0 if point_lights is None else len(point_lights)
+    for k in range(3):
+        s.background[k] = float(background[k])
+        s.camPos[k] = float(camera["position"][k])
+        s.camDir[k] = float(camera["view_dir"][k])
+        s.camUp[k] = float(camera["up"][k])
+    s.tanVFov = float(camera["tan_vfov"])
+    s.width, s.height = int(camera["width"]), int(camera["height"])
+    w, h = s.width, s.height
+    mean = np.zeros(w * h * 3, np.float64)
+    var = np.zeros(w * h * 3, np.float64)
+    rc = lib.orc64_render(C.byref(s), max_path_length, spp, seed, mean.ctypes.data_as(dp), var.ctypes.data_as(dp))
+    if rc != 0:
+        raise RuntimeError("orc64_render failed (%d)" % rc)
+    return mean.reshape(h, w, 3), var.reshape(h, w, 3)

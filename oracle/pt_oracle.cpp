@@ -56,6 +56,23 @@ constexpr float kDistSqEps = 1e-10f;        // light_sampling.hpp:78
 constexpr uint16_t kInternalNode = 0xFFFF;
 constexpr u64 kU64Max = ~0ull;
 
+// Planted estimator mistakes, for the closed-form estimator tests only
+// (orc_set_mutation): each test switches one on and checks that some
+// closed-form comparison then fails. 0 = the restatement as written.
+enum Mutation : int {
+    kMutNone = 0,
+    kMutDropLightCos = 1,      // NEE to an area light without the cosine at the light
+    kMutNeeMisOne = 2,         // NEE MIS denominator with 1 instead of the NEE count
+    kMutHitMisNoCount = 3,     // hit MIS without the NEE count
+    kMutLightPdfNoTree = 4,    // light_pdf without the light-tree choice probability
+    kMutNoRouletteDiv = 5,     // Russian roulette survivors not divided by the continuation probability
+    kMutPointNoFalloff = 6,    // point light without the 1/d^2 falloff
+    kMutBackgroundNoPdf = 7,   // background NEE without the division by the direction pdf
+    kMutNeeMisNoLightCos = 8,  // NEE MIS hit pdf without the cosine at the light
+    kMutIncidentSolidAngle = 9 // hit MIS with the incident pdf left in solid angle
+};
+int g_mutation = kMutNone;
+
 inline float sgnf(float x) { return x < 0.0f ? -1.0f : 1.0f; }
 inline float clampf(float v, float lo, float hi) { return mn(mx(v, lo), hi); }
 inline float sq(float x) { return x * x; }
@@ -1045,6 +1062,7 @@ Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 p
         n.dir = d;
         n.cosOut = 0.0f;
         n.diffIrradiance = {L.intensity[0] / distSq, L.intensity[1] / distSq, L.intensity[2] / distSq};
+        if (g_mutation == kMutPointNoFalloff) n.diffIrradiance = {L.intensity[0], L.intensity[1], L.intensity[2]};
         n.dist = dist;
         n.geoNormal = {0, 0, 0};
         n.distSq = distSq;
@@ -1103,6 +1121,7 @@ Nee connect_background(const OrcPtScene& S, V3 pos, float u0, float u1) {
     n.dir = s.dir;
     n.cosOut = 1.0f;
     n.diffIrradiance = s.radiance / s.pdf;
+    if (g_mutation == kMutBackgroundNoPdf) n.diffIrradiance = s.radiance;
     n.dist = kMaxSceneSize;
     n.geoNormal = {0, 0, 0};
     n.distSq = kMaxSceneSize * kMaxSceneSize;
@@ -1234,6 +1253,7 @@ float light_pdf(const OrcPtScene& S, i32 inst, i32 prim, V3 refPos) {
     const V3 e1{L.posV[1][0], L.posV[1][1], L.posV[1][2]};
     const V3 e2{L.posV[2][0], L.posV[2][1], L.posV[2][2]};
     const float area = len(cross(e1, e2)) / 2.0f;
+    if (g_mutation == kMutLightPdfNoTree) return 1.0f / area;
     return p / area;
 }
 
@@ -1331,14 +1351,16 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                 const float nu0 = sample_uniform((u32)x), nu1 = sample_uniform((u32)(x >> 32));
                 const Nee nee = connect(S, (u64)i, (u64)neeCount, neeSeed, vtx.position, nu0, nu1);
                 EvalValue value = mat_evaluate(vtx.ts, vtx.mat, vtx.incident_dir(), nee.dir);
-                if (nee.cosOut != 0) value.cosOut *= nee.cosOut;
+                if (nee.cosOut != 0 && g_mutation != kMutDropLightCos) value.cosOut *= nee.cosOut;
                 const V3 radiance = value.value * nee.diffIrradiance;
                 if ((radiance.x > 0.0f || radiance.y > 0.0f || radiance.z > 0.0f) && value.cosOut > 0.0f) {
                     const bool occluded =
                         any_intersection(S, vtx.position, nee.position, vtx.ts.geoN, nee.geoNormal, nee.dir);
                     if (!occluded) {
-                        const float hitPdf = value.pdfF * std::fabs(nee.cosOut) / nee.distSq;
-                        const float mis = 1.0f / ((float)neeCount + hitPdf / nee.creationPdf);
+                        float hitPdf = value.pdfF * std::fabs(nee.cosOut) / nee.distSq;
+                        if (g_mutation == kMutNeeMisNoLightCos) hitPdf = value.pdfF / nee.distSq;
+                        float mis = 1.0f / ((float)neeCount + hitPdf / nee.creationPdf);
+                        if (g_mutation == kMutNeeMisOne) mis = 1.0f / (1.0f + hitPdf / nee.creationPdf);
                         acc.add3(0, throughput * value.cosOut * radiance * mis);
                         acc.add1(7, guideWeight * value.cosOut);
                     }
@@ -1382,7 +1404,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
         throughput = throughput * sThroughput;
         const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
         if (rndRoulette >= cont) break;  // CANCEL
-        throughput = throughput / cont;
+        if (g_mutation != kMutNoRouletteDiv) throughput = throughput / cont;
         V3 rayOrigin = origin;
         const HitInfo hit = first_intersection(S, rayOrigin, excident, vtx.geo_normal(), kMaxSceneSize);
         const float hitDist = hit.hit ? hit.t : kMaxSceneSize;
@@ -1409,6 +1431,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             hv = hit.v;
             const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
             next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
+            if (g_mutation == kMutIncidentSolidAngle) next.incidentPdf = pdfF;
         }
         const V3 prevPos = vtx.position;
         vtx = next;
@@ -1429,7 +1452,8 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                 if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos);
             }
             if (nonzero(emission) && pathLen > 1) {
-                const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
+                float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
+                if (g_mutation == kMutHitMisNoCount) mis = 1.0f / (1.0f + connectPdf / vtx.incidentPdf);
                 emission = emission * mis;
             }
             acc.add3(0, throughput * emission);
@@ -1767,7 +1791,7 @@ int orc_pt_render_targets_var(const OrcPtScene* S, int32_t seed, int32_t minPath
     const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
     u32 nans = 0;
     for (int it = 0; it < iterations; ++it) {
-#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nans)
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : nans)
         for (int pixel = 0; pixel < num; ++pixel) {
             PixelAcc acc;
             acc.mask = targetMask;
@@ -1997,6 +2021,14 @@ int orc_wireframe_render(const OrcPtScene* S, int32_t iterations, float lineWidt
 }
 
 uint64_t orc_scramble_seed(uint32_t x) { return Pcg64::scramble(x); }
+
+// Switches on one planted estimator mistake (Mutation; 0 = none) for the
+// closed-form estimator tests; returns the previous setting.
+int orc_set_mutation(int32_t m) {
+    const int old = g_mutation;
+    g_mutation = m;
+    return old;
+}
 int orc_pcg_sequence(uint32_t seed, int32_t n, uint64_t* out) {
     Pcg64 r(seed);
     for (int32_t i = 0; i < n; ++i) out[i] = r.next();
