@@ -56,8 +56,6 @@ def parse():
                     help="steps only (no stats pass, no CPU baseline): for rocprofv3 runs")
     ap.add_argument("--pt-iterations", type=int, default=64,
                     help="Cornell-box path-tracer iterations measured next to the main workload (0: skip)")
-    ap.add_argument("--reference-batches", type=int, default=16,
-                    help="256-spp batches of the converged Cornell reference for the RMSE")
     ap.add_argument("--terrain-iterations", type=int, default=16,
                     help="iterations of the terrain megakernel / wavefront comparison (0 skips it)")
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint measurement")
@@ -93,27 +91,23 @@ def cpu_baseline(verts, tris, rays):
                 lbvh_build_ms=build_s * 1e3, trace_mrays_s=len(rays) / trace_s / 1e6)
 
 
-def converged_rmse(ctx, pt, image, params, batches, batch_iterations=256):
-    """RMSE of `image` against a converged render of the same path tracer:
-    `batches` x `batch_iterations` paths per pixel with other seeds (the
-    megakernel, whose paths are bit-identical to the oracle's), batch means
-    averaged in float64. Also the reference's own standard error (from the
-    spread of the batch means) and the RMSE relative to the mean pixel."""
-    from mufflon_amd import hip
-    ctx.pt_set_mode(pt, hip.PT_MEGAKERNEL, False)
-    means = []
-    for b in range(batches):
-        ctx.pt_set_params(pt, dict(params, seed=1000 + b))
-        ctx.pt_iterate(pt, batch_iterations)
-        means.append(ctx.pt_target(pt).astype(np.float64))
-    ref = np.mean(means, axis=0)
-    stderr = np.sqrt(np.mean(np.var(means, axis=0, ddof=1) / batches)) if batches > 1 else float("nan")
+def converged_rmse(image):
+    """RMSE of the Cornell `image` (256x256, depth 8) against the FP64
+    converged render of the same scene (tests/golden/cornell_fp64_4096spp.npz:
+    4096 samples per pixel of the oracle's independent FP64 estimator, which
+    shares no code with the path tracer), with that reference's own noise
+    floor (root mean squared standard error) and the RMSE with that floor
+    removed in quadrature, plus the RMSE relative to the mean pixel."""
+    path = os.path.join(ROOT, "tests", "golden", "cornell_fp64_4096spp.npz")
+    with np.load(path) as z:
+        ref = z["mean"].astype(np.float64)
+        spp = int(z["spp"])
+        floor = float(np.sqrt(z["mean_sq_std_error"]))
     rmse = float(np.sqrt(np.mean((image.astype(np.float64) - ref) ** 2)))
-    ctx.pt_set_params(pt, dict(params, seed=1))
-    return {"vs": "converged render, %d spp (float64 mean of %d batches of %d), seeds 1000+" %
-                  (batches * batch_iterations, batches, batch_iterations),
+    return {"vs": "FP64 converged render, %d spp (independent FP64 estimator, oracle/pt_fp64.cpp)" % spp,
             "rmse": round(rmse, 6), "relative_rmse": round(rmse / float(np.mean(ref)), 5),
-            "reference_standard_error": round(float(stderr), 6)}
+            "reference_noise_rms": round(floor, 6),
+            "rmse_without_reference_noise": round(float(np.sqrt(max(rmse * rmse - floor * floor, 0.0))), 6)}
 
 
 def omp_threads():
@@ -181,7 +175,7 @@ def pt_cornell(args, cpu_baseline_too):
     out["wavefront"] = {"ms_per_iteration": round(wms / n, 4), "mpaths_s": round(pixels * n / (wms / 1e3) / 1e6, 3),
                         "tuning": tuned, "graph": True,
                         "identical_to_megakernel": bool(same and np.array_equal(ctx.pt_target(pt), mega))}
-    out["rmse"] = converged_rmse(ctx, pt, mega, dict(max_path_length=8, nee_count=1), args.reference_batches)
+    out["rmse"] = converged_rmse(mega)
     if cpu_baseline_too:
         from oracle import oracle as O
         sys.path.insert(0, os.path.join(ROOT, "tests"))
