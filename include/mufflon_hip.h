@@ -374,12 +374,22 @@ int mfh_pt_wireframe_get(mfh_context* ctx, int32_t ptId, int32_t variance, float
  * accumulates dLoss/d(texel) of every albedo texture (LAMBERT, TORRANCE),
  * every roughness texture (TORRANCE, MICROFACET; one channel drives both
  * roughnesses, two channels x and y) and dLoss/d(emission scale) of every
- * material: the derivative of the estimator with its random numbers fixed
- * (Russian roulette included; light-selection probabilities held fixed,
- * exact for one emissive material). Roughness is differentiated with the
- * sampled BSDF directions held fixed (detached sampling). Maximum path
- * length 32. Asynchronous; read with the getters below. */
+ * material. By default (MFH_ADJOINT_EXPECTED) the result is an unbiased
+ * estimate of the derivative of the expected image: each contribution times
+ * the d ln of the BSDF values and emission it contains and of its MIS
+ * weight (sampling pdfs and Russian-roulette probabilities need no term).
+ * Maximum path length 32. Asynchronous; read with the getters below. */
 int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const float* radianceAdjoint);
+/* Which derivative mfh_pt_adjoint forms (no counterpart in the reference):
+ * MFH_ADJOINT_EXPECTED (0, default) the derivative of the expected image;
+ * MFH_ADJOINT_FIXED_DECISIONS (1) the derivative of this realization of the
+ * estimator with every random decision and sampled direction held fixed,
+ * which central differences with common random numbers reproduce where no
+ * decision flips (for roughness: of renders replaying the sampled
+ * directions); biased as an estimate of the derivative of the expectation. */
+#define MFH_ADJOINT_EXPECTED 0
+#define MFH_ADJOINT_FIXED_DECISIONS 1
+int mfh_pt_set_adjoint_mode(mfh_context* ctx, int32_t ptId, int32_t mode);
 /* Gradient of an albedo or roughness texture: width * height * channels floats. */
 int mfh_pt_get_texture_gradient(mfh_context* ctx, int32_t ptId, int32_t textureId, float* out);
 /* Gradient with respect to a material's emission scale: 3 floats. */

@@ -10,27 +10,43 @@
 // code as the megakernel (pt_path.hpp), recording per vertex the albedo
 // parameter, the delta-weighted next-event and emission contributions and
 // the Russian-roulette data. A backward pass over the recorded vertices then
-// forms the derivative of this very estimator (the random numbers, light and
-// lobe choices held fixed):
-//   * every contribution after the walk from vertex j and every next-event
-//     estimate at j carries one factor albedo_j[c] in channel c;
-//   * the continuation probability of each later walk m,
-//     cont_m = min(max_a T'_m[a] + 0.1, 1), depends on albedo_j through the
-//     throughput; with D_m = dln(cont_m)/dalbedo_j[c'] (zero when cont_m = 1)
-//       D_m = T'_m[a*] / cont_m * ([a* = c'] / albedo_j[c'] - sum_{j<=l<m} D_l)
-//     every contribution after walk m picks up -D_m.
-// Roughness enters with the BSDF sample's direction held fixed (detached
-// sampling): the walk weight value * cos / pdf, the pdf the next vertex's
-// emission MIS uses, and each next-event estimate's BSDF value and MIS
-// weight are differentiated at the recorded directions (dln_mat_evaluate),
-// and the same continuation terms D_m follow with d ln(walk weight) in place
-// of 1 / albedo. This is the derivative of the estimator whose random
-// decisions and sampled directions are fixed; its expectation is the
-// derivative of the expected image.
+// forms one of two derivatives (AdjointState::fixedDecisions).
+//
+// Derivative of the expected image (fixedDecisions = 0, the default). Each
+// contribution C of the estimator is w f / p: a path's measurement f, the
+// pdf p of the random decisions that produced it (BSDF and light sampling,
+// the light-tree choice, the Russian-roulette survivals) and its MIS weight
+// w. With the decisions' distribution depending on the parameter theta,
+//   d/dtheta E[C] = E[C (d ln w + d ln f)]
+// (the score d ln p of the sampling and the derivative -d ln p of 1/p
+// cancel), so every contribution is multiplied by the d ln of the BSDF
+// values it contains, plus its MIS weight's d ln:
+//   * albedo: one factor albedo_j[c] per vertex j that scatters toward C
+//     (the walk from j or the next-event estimate at j): d ln f = 1/albedo;
+//   * roughness: d ln value of the walk's BSDF at its sampled direction,
+//     d ln value of a connection's BSDF, the hit MIS weight
+//     (1 - mis) d ln pdf and the next-event MIS weight
+//     -mis pdf_hit / pdf_light d ln pdf (dln_mat_evaluate);
+//   * emission scale: d ln f = 1/scale for a contribution of that emitter
+//     (the light-tree choice probabilities need no term).
+// Its expectation is the derivative of the expected image.
+//
+// Derivative of this realization (fixedDecisions = 1): the estimator with
+// every random number, light and lobe choice, roulette decision and sampled
+// direction held fixed, which is what central differences with common random
+// numbers see (where no decision flips) of renders that replay the same
+// directions. On top of the factors above, the walk weight value * cos / pdf
+// of a roughness-textured vertex contributes d ln value - d ln pdf, and the
+// continuation probability of each later walk m, cont_m =
+// min(max_a T'_m[a] + 0.1, 1), follows the throughput: with
+//   D_m = T'_m[a*] / cont_m * (d ln walk_j[a*] - sum_{j<=l<m} D_l)
+// (zero when cont_m = 1) every contribution after walk m picks up -D_m.
+// These terms have a non-zero mean: the roulette's boundary (a decision that
+// flips) and the moving sampling pdf carry the rest of the derivative of the
+// expectation, which the default form accounts for in expectation.
+//
 // Texel gradients follow the texture's own sampling weights (bilinear or
-// nearest, 1x1 constant) and are added with float atomics. Emission-scale
-// gradients hold the light-tree selection probabilities fixed, which is the
-// exact derivative when the emitters share one material (or scale together).
+// nearest, 1x1 constant) and are added with float atomics.
 #include "pt_adjoint.hpp"
 #include "pt_path.hpp"
 
@@ -162,8 +178,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                 if (ok && r.rtex >= 0) {
                     float dv[2], dp[2];
                     dln_mat_evaluate(vx.ts, vx.mat, vx.incident(), dir, dv, dp);
-                    r.rw[0] = dv[0] - dp[0];
-                    r.rw[1] = dv[1] - dp[1];
+                    r.rw[0] = A.fixedDecisions ? dv[0] - dp[0] : dv[0];
+                    r.rw[1] = A.fixedDecisions ? dv[1] - dp[1] : dv[1];
                     r.rp[0] = pdfF < 1e9f ? dp[0] : 0.0f;
                     r.rp[1] = pdfF < 1e9f ? dp[1] : 0.0f;
                 }
@@ -219,7 +235,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                 for (int q = 0; q < 2; ++q) {
                     g[q] = rec[j].rw[q] * Gj + rec[j].rn[q] + em1 * rec[j].rp[q];
                     float sumD = 0.0f;
-                    for (int m = j; m < last; ++m) {
+                    for (int m = j; A.fixedDecisions && m < last; ++m) {
                         if (!(rec[m].cont < 1.0f)) continue;
                         const float D = rec[m].tp / rec[m].cont * (rec[j].rw[q] - sumD);
                         g[q] -= D * (rec[m].G.x + rec[m].G.y + rec[m].G.z);
@@ -249,7 +265,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                 if (rho[c] == 0.0f) continue;
                 float g = gdir[c] / rho[c];
                 float sumD = 0.0f;
-                for (int m = j; m < last; ++m) {  // walks from m that continued to vertex m + 1
+                for (int m = j; A.fixedDecisions && m < last; ++m) {  // walks from m that continued to vertex m + 1
                     if (!(rec[m].cont < 1.0f)) continue;
                     const float D = rec[m].tp / rec[m].cont * ((rec[m].astar == c ? 1.0f / rho[c] : 0.0f) - sumD);
                     g -= D * (rec[m].G.x + rec[m].G.y + rec[m].G.z);

@@ -12,6 +12,7 @@ struct AdjointState {
     uint64_t* rngs;        // replay RNG states (seeded as a reset seeds them)
     float* const* texGrad; // per texture id: gradient buffer (4 floats per texel) or null
     float* emissionGrad;   // 3 floats per material: d loss / d emission scale
+    int fixedDecisions;    // 0: derivative of the expected image; 1: of this realization (MFH_ADJOINT_*)
 };
 
 void launch_pt_adjoint(const PtScene& S, const AdjointState& A, int seed, hipStream_t stream);

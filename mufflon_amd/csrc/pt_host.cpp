@@ -43,6 +43,7 @@ struct PathTracer {
     std::vector<uint8_t> graphKey;  // bytes of (S, W, tune, sort scratch) the graph was captured with
     // adjoint
     DevBuf adjRngs, adjDelta, adjEmGrad, adjTexGradPtrs;
+    int32_t adjMode = MFH_ADJOINT_EXPECTED;
     std::vector<DevBuf> adjTexGrad;  // per texture id
     bool adjValid = false;
     // wireframe renderer over the same scene and camera (GpuWireframe,
@@ -721,8 +722,18 @@ int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const flo
         A.rngs = t.adjRngs.as<uint64_t>();
         A.texGrad = t.adjTexGradPtrs.as<float* const>();
         A.emissionGrad = t.adjEmGrad.as<float>();
+        A.fixedDecisions = t.adjMode == MFH_ADJOINT_FIXED_DECISIONS ? 1 : 0;
         launch_pt_adjoint(S, A, t.params.seed * (num + 1), c.stream);
         t.adjValid = true;
+    });
+}
+
+int mfh_pt_set_adjoint_mode(mfh_context* ctx, int32_t ptId, int32_t mode) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(mode == MFH_ADJOINT_EXPECTED || mode == MFH_ADJOINT_FIXED_DECISIONS, "unknown adjoint mode");
+        t.adjMode = mode;
     });
 }
 

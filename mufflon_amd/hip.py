@@ -41,6 +41,8 @@ AREA_LIGHT_DTYPE = np.dtype([("points", "<f4", 9), ("uv", "<f4", 6), ("material"
 LIGHT_TREE_INFO_DTYPE = np.dtype([("posRootFlux", "<f4"), ("posRootType", "<u4"), ("posInternalCount", "<u4"),
                                   ("posLightCount", "<u4"), ("memBytes", "<u4"), ("mapCount", "<i4"),
                                   ("dirRootFlux", "<f4"), ("backgroundFlux", "<f4")])
+ADJOINT_EXPECTED = 0  # mfh_pt_set_adjoint_mode
+ADJOINT_FIXED_DECISIONS = 1
 TARGET_RADIANCE = 0
 TARGET_HIT_ID = 1
 TARGET_POSITION = 2
@@ -88,6 +90,7 @@ _SIGS = {
     "mfh_pt_get_variance": [vp, i32, i32, f32p],
     "mfh_pt_set_mode": [vp, i32, i32, i32],
     "mfh_pt_adjoint": [vp, i32, i32, f32p],
+    "mfh_pt_set_adjoint_mode": [vp, i32, i32],
     "mfh_batch_create": [vp, i32p, i32, i32, i32p],
     "mfh_batch_reset": [vp, i32],
     "mfh_batch_iterate": [vp, i32, i32],
@@ -494,6 +497,11 @@ class HipContext:
     def pt_adjoint(self, pt, iterations, radiance_adjoint):
         d = np.ascontiguousarray(radiance_adjoint, np.float32)
         self._call("mfh_pt_adjoint", self.handle, pt, iterations, _ptr(d, f32p))
+
+    def pt_set_adjoint_mode(self, pt, mode):
+        """ADJOINT_EXPECTED (derivative of the expected image, the default) or
+        ADJOINT_FIXED_DECISIONS (of this realization, decisions held fixed)."""
+        self._call("mfh_pt_set_adjoint_mode", self.handle, pt, mode)
 
     def pt_texture_gradient(self, pt, tex, shape):
         out = np.zeros(int(np.prod(shape)), np.float32)

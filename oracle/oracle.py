@@ -291,6 +291,8 @@ def _bsdf_lib():
         lib.orc_bsdf_sample.argtypes = [C.c_void_p, C.POINTER(OrcTexture), C.c_float, C.c_float, f32p, C.c_float,
                                         C.c_float, C.c_uint64, f32p]
         lib.orc_bsdf_evaluate.argtypes = [C.c_void_p, C.POINTER(OrcTexture), C.c_float, C.c_float, f32p, f32p, f32p]
+        lib.orc_bsdf_evaluate_batch.argtypes = [C.c_void_p, C.POINTER(OrcTexture), C.c_float, C.c_float, C.c_int32,
+                                                f32p, f32p, f32p]
         lib.orc_eval_ndf.argtypes = [C.c_int32, C.c_float, C.c_float, f32p]
         lib.orc_eval_ndf.restype = C.c_float
         lib.orc_fresnel_dielectric.argtypes = [C.c_float, C.c_float, C.c_float, f32p]
@@ -321,6 +323,18 @@ def bsdf_evaluate(material, in_ts, ex_ts, textures=None, uv=(0.0, 0.0)):
     lib.orc_bsdf_evaluate(m.ctypes.data, tx.arr, uv[0], uv[1], _p(np.ascontiguousarray(in_ts, np.float32), f32p),
                           _p(np.ascontiguousarray(ex_ts, np.float32), f32p), _p(out, f32p))
     return out[:3].copy(), float(out[3]), float(out[4])
+
+
+def bsdf_evaluate_batch(material, in_ts, ex_ts, textures=None, uv=(0.0, 0.0)):
+    """bsdf_evaluate over arrays of directions (n, 3): values (n, 3), pdf_forw (n,), pdf_back (n,)."""
+    lib = _bsdf_lib()
+    m = np.ascontiguousarray(material)
+    tx = textures or Textures()
+    a = np.ascontiguousarray(in_ts, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(np.broadcast_to(ex_ts, a.shape), np.float32)
+    out = np.zeros((len(a), 5), np.float32)
+    lib.orc_bsdf_evaluate_batch(m.ctypes.data, tx.arr, uv[0], uv[1], len(a), _p(a, f32p), _p(b, f32p), _p(out, f32p))
+    return out[:, :3], out[:, 3], out[:, 4]
 
 
 def eval_ndf(ndf, rx, ry, h):
@@ -563,16 +577,19 @@ def pt_render_replay(base_scene, eval_scene, seed, min_path_length, max_path_len
     return out.reshape(h, w, 3)
 
 
-def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, delta, tex_shapes):
+def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, delta, tex_shapes,
+               fixed_decisions=False):
     """FP64 adjoint of the oracle: delta (h, w, 3) = dLoss/dRadiance of the
     image of `iterations` iterations from a reset. tex_shapes: {texture id:
-    (h, w)} of the albedo textures to differentiate. Returns ({id: (h, w, 4)
-    float64}, emission gradient (materials, 3) float64)."""
+    (h, w)} of the albedo / roughness textures to differentiate. The
+    derivative of the expected image, or with fixed_decisions the derivative
+    of this realization (every random decision and direction held). Returns
+    ({id: (h, w, 4) float64}, emission gradient (materials, 3) float64)."""
     lib = load()
     if not hasattr(lib, "_adj_ready"):
         dp = C.POINTER(C.c_double)
         lib.orc_pt_adjoint.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                                       f32p, C.POINTER(dp), dp]
+                                       f32p, C.POINTER(dp), dp, C.c_int32]
         lib.orc_pt_adjoint.restype = C.c_int
         lib._adj_ready = True
     dp = C.POINTER(C.c_double)
@@ -585,7 +602,7 @@ def pt_adjoint(pt_scene, seed, min_path_length, max_path_length, nee_count, iter
     em = np.zeros((len(pt_scene.materials), 3), np.float64)
     d = np.ascontiguousarray(delta, np.float32)
     lib.orc_pt_adjoint(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
-                       _p(d, f32p), ptrs, em.ctypes.data_as(dp))
+                       _p(d, f32p), ptrs, em.ctypes.data_as(dp), 1 if fixed_decisions else 0)
     return grads, em
 
 

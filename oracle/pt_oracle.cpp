@@ -1500,6 +1500,7 @@ struct AdjRec {
 struct AdjOut {
     double* const* texGrad;  // per texture id: 4 doubles per texel or null
     double* emGrad;          // 3 per material
+    int fixedDecisions;      // 0: derivative of the expected image; 1: of this realization
 };
 
 void texel_weights(const OrcTexture& t, float u, float v, int* idx, double* w, int& n) {
@@ -1610,7 +1611,8 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             double dv[2], dp[2];
             if (r.rtex >= 0 && dln_mat_evaluate_d(vtx.ts, vtx.mat, vtx.incident_dir(), excident, dv, dp)) {
                 for (int q = 0; q < 2; ++q) {
-                    r.rw[q] = dv[q] - dp[q];
+                    // d ln f (expected image), or d ln (f / pdf) with the direction held (this realization)
+                    r.rw[q] = out.fixedDecisions ? dv[q] - dp[q] : dv[q];
                     r.rp[q] = pdfF < 1e9f ? dp[q] : 0.0;
                 }
             }
@@ -1692,10 +1694,11 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
     for (int j = 1; j <= last; ++j) {
         const AdjRec& r = rec[j];
         if (r.rtex >= 0 && out.texGrad[r.rtex]) {
-            // roughness of vertex j: the walk weight scales everything after
-            // walk j, the next vertex's emission MIS sees the sampled pdf, the
-            // next-event estimates at j their own terms, and the roulette
-            // probabilities of walks m >= j follow the throughput
+            // roughness of vertex j: the walk's BSDF value scales everything
+            // after walk j, the next vertex's emission MIS sees the sampled
+            // pdf, the next-event estimates at j carry their own terms; with
+            // the decisions held fixed, the roulette probabilities of walks
+            // m >= j follow the throughput
             const double Gj = r.G[0] + r.G[1] + r.G[2];
             const double em1 = j < last ? (rec[j + 1].emW[0] + rec[j + 1].emW[1] + rec[j + 1].emW[2]) * rec[j + 1].misC
                                         : 0.0;
@@ -1703,7 +1706,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             for (int q = 0; q < 2; ++q) {
                 g[q] = r.rw[q] * Gj + r.rn[q] + em1 * r.rp[q];
                 double sumD = 0.0;
-                for (int m = j; m < last; ++m) {
+                for (int m = j; out.fixedDecisions && m < last; ++m) {
                     if (!(rec[m].cont < 1.0f)) continue;
                     const double D = (double)rec[m].tp / rec[m].cont * (r.rw[q] - sumD);
                     g[q] -= D * (rec[m].G[0] + rec[m].G[1] + rec[m].G[2]);
@@ -1729,7 +1732,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             if (rho[c] == 0.0) continue;
             double g = (r.G[c] + r.neeW[c]) / rho[c];
             double sumD = 0.0;
-            for (int m = j; m < last; ++m) {
+            for (int m = j; out.fixedDecisions && m < last; ++m) {
                 if (!(rec[m].cont < 1.0f)) continue;
                 const double D = (double)rec[m].tp / rec[m].cont * ((rec[m].astar == c ? 1.0 / rho[c] : 0.0) - sumD);
                 g -= D * (rec[m].G[0] + rec[m].G[1] + rec[m].G[2]);
@@ -1747,14 +1750,16 @@ extern "C" {
 // Adjoint of the Radiance image of `iterations` iterations from a reset:
 // delta = dLoss/dRadiance (3 floats per pixel). texGrad[t] (4 doubles per
 // texel, or null) and emGrad (3 per material) receive the gradients; the sum
-// runs serially in pixel order.
+// runs serially in pixel order. fixedDecisions selects the derivative of this
+// realization (1) instead of the derivative of the expected image (0).
 int orc_pt_adjoint(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength, int32_t neeCount,
-                   int32_t iterations, const float* delta, double* const* texGrad, double* emGrad) {
+                   int32_t iterations, const float* delta, double* const* texGrad, double* emGrad,
+                   int32_t fixedDecisions) {
     const int W = S->width, H = S->height;
     const int num = W * H;
     const int s = seed * (num + 1);
     const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
-    const AdjOut out{texGrad, emGrad};
+    const AdjOut out{texGrad, emGrad, fixedDecisions ? 1 : 0};
     for (int pixel = 0; pixel < num; ++pixel) {
         const double d[3] = {(double)(delta[3 * (size_t)pixel] * (1.0f / (float)iterations)),
                              (double)(delta[3 * (size_t)pixel + 1] * (1.0f / (float)iterations)),
@@ -1894,6 +1899,26 @@ int orc_bsdf_evaluate(const OrcMaterial* m, const OrcTexture* textures, float u,
     out5[4] = e.pdfB;
     return 0;
 }
+// orc_bsdf_evaluate over n (in, ex) pairs (3 floats each) -> 5 floats each,
+// for quadratures over the hemisphere.
+int orc_bsdf_evaluate_batch(const OrcMaterial* m, const OrcTexture* textures, float u, float v, int32_t n,
+                            const float* inTS, const float* exTS, float* out5) {
+    const MatParams p = params_of(*m, textures, u, v);
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < n; ++i) {
+        const float* a = inTS + 3 * (size_t)i;
+        const float* b = exTS + 3 * (size_t)i;
+        const TsEval e = evaluate_ts(p, {a[0], a[1], a[2]}, {b[0], b[1], b[2]});
+        float* o = out5 + 5 * (size_t)i;
+        o[0] = e.value.x;
+        o[1] = e.value.y;
+        o[2] = e.value.z;
+        o[3] = e.pdfF;
+        o[4] = e.pdfB;
+    }
+    return 0;
+}
+
 // eval_ndf and fresnel_dielectric for the closed-form checks.
 float orc_eval_ndf(int32_t ndf, float rx, float ry, const float* h) {
     MatParams p;

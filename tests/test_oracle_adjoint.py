@@ -1,11 +1,13 @@
-"""The oracle's adjoint (path replay, FP64 accumulation) against central finite
-differences of the oracle's own renders with identical seeds: the
-derivative of the L2 image loss with respect to albedo texels (LAMBERT back
-wall, TORRANCE sphere), the emission scale, and roughness texels (TORRANCE
-sphere, V-cavity and Smith). Roughness is differentiated with the sampled
-BSDF directions held fixed, so its finite differences use the replay render
-(the same paths, the same directions, value * cos / pdf evaluated at the
-perturbed roughness)."""
+"""The oracle's adjoint in its fixed-decision form (the derivative of one
+realization of the estimator, every random decision and sampled direction
+held) against central finite differences of the oracle's own renders with
+identical seeds, texel by texel: the derivative of the L2 image loss with
+respect to albedo texels (LAMBERT back wall, TORRANCE sphere), the emission
+scale, and roughness texels (TORRANCE sphere, V-cavity and Smith). Roughness
+differences use the replay render (the same paths, the same directions,
+value * cos / pdf evaluated at the perturbed roughness). Differences in which
+a decision flips are skipped. The default form, the derivative of the
+expected image, is pinned by test_oracle_adjoint_expected."""
 import numpy as np
 import pytest
 
@@ -29,7 +31,7 @@ def test_oracle_adjoint_matches_finite_differences(oracle):
     img = _loss(oracle, c, c["textures"])
     delta = (2.0 * (img - target)).astype(np.float32)
     ps = P.oracle_pt_scene_textured(oracle, c)
-    grads, em = oracle.pt_adjoint(ps, SEED, 0, DEPTH, NEE, ITS, delta, {0: (8, 8), 1: (8, 8)})
+    grads, em = oracle.pt_adjoint(ps, SEED, 0, DEPTH, NEE, ITS, delta, {0: (8, 8), 1: (8, 8)}, fixed_decisions=True)
 
     def loss_of(textures, materials=None):
         return float(np.sum((_loss(oracle, c, textures, materials) - target) ** 2))
@@ -75,7 +77,7 @@ def test_oracle_roughness_adjoint_matches_replay_fd(oracle, shadowing):
     np.testing.assert_allclose(again, img, rtol=1e-5, atol=1e-7)
     target = img * 0.8 + 0.05
     delta = (2.0 * (img - target)).astype(np.float32)
-    grads, _ = oracle.pt_adjoint(base, SEED, 0, DEPTH, NEE, ITS, delta, {1: (8, 8)})
+    grads, _ = oracle.pt_adjoint(base, SEED, 0, DEPTH, NEE, ITS, delta, {1: (8, 8)}, fixed_decisions=True)
     g = grads[1][..., 0]
     assert not grads[1][..., 1:].any()  # one channel drives both roughnesses
 
