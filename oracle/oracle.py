@@ -499,6 +499,37 @@ def pt_render(pt_scene, seed, min_path_length, max_path_length, nee_count, itera
     return rad.reshape(h, w, 3), hid.reshape(h, w, 2), nans.value
 
 
+def pt_render_window(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations, x0, y0, w, h):
+    """Oracle Radiance (h, w, 3) of the pixel window at (x0, y0): equal to the
+    same pixels of a full render."""
+    lib = load()
+    lib.orc_pt_render_window.argtypes = [C.POINTER(OrcPtScene)] + [C.c_int32] * 9 + [f32p]
+    lib.orc_pt_render_window.restype = C.c_int
+    out = np.zeros(w * h * 3, np.float32)
+    rc = lib.orc_pt_render_window(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, iterations,
+                                  x0, y0, w, h, _p(out, f32p))
+    if rc != 0:
+        raise ValueError("window outside the image")
+    return out.reshape(h, w, 3)
+
+
+PATH_STAT_FIELDS = ("ext_rays", "ext_nodes", "ext_tris", "shadow_rays", "shadow_nodes", "shadow_tris",
+                    "surface_vertices", "texel_taps", "tree_nodes", "light_records", "paths")
+
+
+def pt_path_stats(pt_scene, seed, min_path_length, max_path_length, nee_count, x0, y0, w, h):
+    """Work counts of one iteration of the window's paths ({field: sum})."""
+    lib = load()
+    lib.orc_pt_path_stats.argtypes = [C.POINTER(OrcPtScene)] + [C.c_int32] * 8 + [u64p]
+    lib.orc_pt_path_stats.restype = C.c_int
+    out = np.zeros(11, np.uint64)
+    rc = lib.orc_pt_path_stats(C.byref(pt_scene.s), seed, min_path_length, max_path_length, nee_count, x0, y0, w, h,
+                               _p(out, u64p))
+    if rc != 0:
+        raise ValueError("window outside the image")
+    return {k: int(v) for k, v in zip(PATH_STAT_FIELDS, out)}
+
+
 TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)  # Radiance, Hit ID, Position, Depth, Normal, UV, Albedo, Lightness
 
 

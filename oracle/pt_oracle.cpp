@@ -73,6 +73,21 @@ enum Mutation : int {
 };
 int g_mutation = kMutNone;
 
+// Work counts of the paths a render traces, for the path tracer's
+// algorithmic-bytes figure (orc_pt_path_stats): set per thread while a
+// counted path runs, null otherwise.
+struct PathStats {
+    u64 extRays = 0, extNodes = 0, extTris = 0;           // closest-hit rays and their traversal work
+    u64 shadowRays = 0, shadowNodes = 0, shadowTris = 0;  // any-hit rays
+    u64 surfaceVertices = 0;                              // hit attributes + material fetches
+    u64 texelTaps = 0;                                    // texture lookups (1x1 values included)
+    u64 treeNodes = 0;                                    // light-tree node records read
+    u64 lightRecords = 0;                                 // point / triangle light records read
+    u64 paths = 0;
+};
+static_assert(sizeof(PathStats) == 11 * sizeof(u64), "PathStats is read as 11 counters");
+thread_local PathStats* tl_stats = nullptr;
+
 inline float sgnf(float x) { return x < 0.0f ? -1.0f : 1.0f; }
 inline float clampf(float v, float lo, float hi) { return mn(mx(v, lo), hi); }
 inline float sq(float x) { return x * x; }
@@ -249,6 +264,7 @@ inline F4 texel(const OrcTexture& t, int x, int y) {
 }
 // CpuTexture::sample_nearest / sample_linear / sample111_* (cputexture.cpp:372-411)
 F4 tex_sample(const OrcTexture& t, float u, float v) {
+    if (tl_stats) ++tl_stats->texelTaps;
     if (t.width == 1 && t.height == 1) return texel(t, 0, 0);
     if (t.sampling == 0) return texel(t, ifloor(u * (float)t.width), ifloor(v * (float)t.height));
     float fx = u * (float)t.width, fy = v * (float)t.height;
@@ -818,7 +834,12 @@ HitInfo first_intersection(const OrcPtScene& S, V3& origin, V3 dir, V3 geoNormal
     const FastRay fr = make_fast_ray(origin, dir);
     float hitT = tmax;
     Hit h;
-    traverse(S.geo, fr, hitT, h, false, nullptr, nullptr);
+    if (tl_stats) {
+        ++tl_stats->extRays;
+        traverse(S.geo, fr, hitT, h, false, &tl_stats->extNodes, &tl_stats->extTris);
+    } else {
+        traverse(S.geo, fr, hitT, h, false, nullptr, nullptr);
+    }
     HitInfo r;
     r.t = hitT;
     r.inst = h.inst;
@@ -868,6 +889,10 @@ bool any_intersection(const OrcPtScene& S, V3 a, V3 b, V3 geoNA, V3 geoNB, V3 di
     float tmax = len(c);
     const FastRay fr = make_fast_ray(a, c / tmax);
     Hit h;
+    if (tl_stats) {
+        ++tl_stats->shadowRays;
+        return traverse(S.geo, fr, tmax, h, true, &tl_stats->shadowNodes, &tl_stats->shadowTris);
+    }
     return traverse(S.geo, fr, tmax, h, true, nullptr, nullptr);
 }
 
@@ -1049,6 +1074,7 @@ struct NodeRec {
 
 Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 pos, float u0, float u1) {
     Nee n;
+    if (tl_stats) ++tl_stats->lightRecords;
     if (type == 0) {  // point (light_sampling.hpp:308-327); point lights sit in vacuum
         PointLightRec L;
         std::memcpy(&L, light, sizeof(L));
@@ -1193,6 +1219,7 @@ Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, fl
     while (type == kInternalNode) {
         NodeRec node;
         std::memcpy(&node, S.lightMem + offset, sizeof(node));
+        if (tl_stats) ++tl_stats->treeNodes;
         const float probLeft = guide_left(S, node, pos);
         const u64 boundary = left + percentage_of(right - left, probLeft);
         if (rndChoice < boundary) {
@@ -1236,6 +1263,7 @@ float light_pdf(const OrcPtScene& S, i32 inst, i32 prim, V3 refPos) {
     while (type == kInternalNode) {
         NodeRec node;
         std::memcpy(&node, S.lightMem + offset, sizeof(node));
+        if (tl_stats) ++tl_stats->treeNodes;
         const float pLeft = guide_left(S, node, refPos);
         if (code & 0x80000000u) {
             p *= (1.0f - pLeft);
@@ -1426,6 +1454,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             next.incEnc = {excident.x, excident.y, hit.t * sgnf(excident.z)};
             next.inst = hit.inst;
             next.prim = hit.prim;
+            if (tl_stats) ++tl_stats->surfaceVertices;
             next.mat = fetch(S, material_of(S, hit.inst, hit.prim), hit.u, hit.v);
             hu = hit.u;
             hv = hit.v;
@@ -1830,6 +1859,70 @@ int orc_pt_render_targets(const OrcPtScene* S, int32_t seed, int32_t minPathLeng
                           uint32_t* outNans) {
     return orc_pt_render_targets_var(S, seed, minPathLength, maxPathLength, neeCount, iterations, targetMask,
                                      outTargets, 0u, nullptr, outNans);
+}
+
+// Work counts of one iteration of the paths of a pixel window (PathStats, in
+// its field order: extension rays, their node visits and triangle tests,
+// shadow rays, their node visits and triangle tests, surface vertices, texel
+// taps, light-tree nodes, light records, paths), summed over the window.
+int orc_pt_path_stats(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
+                      int32_t neeCount, int32_t x0, int32_t y0, int32_t w, int32_t h, uint64_t* out11) {
+    const int W = S->width, H = S->height;
+    if (x0 < 0 || y0 < 0 || w < 0 || h < 0 || x0 + w > W || y0 + h > H) return 1;
+    const int s = seed * (W * H + 1);
+    const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
+    PathStats total;
+#pragma omp parallel
+    {
+        PathStats mine;
+        tl_stats = &mine;
+#pragma omp for schedule(dynamic, 4)
+        for (int k = 0; k < w * h; ++k) {
+            const int px = x0 + k % w, py = y0 + k / w;
+            Pcg64 rng((u32)(px + py * W + s));
+            PixelAcc acc;
+            pt_sample(*S, params, px, py, rng, acc);
+            ++mine.paths;
+        }
+        tl_stats = nullptr;
+#pragma omp critical
+        {
+            const u64* a = &mine.extRays;
+            u64* b = &total.extRays;
+            for (int i = 0; i < 11; ++i) b[i] += a[i];
+        }
+    }
+    std::memcpy(out11, &total, sizeof(total));
+    return 0;
+}
+
+// Radiance (3 floats per pixel, row-major over the window) of `iterations`
+// iterations from a reset of the pixels x0 <= x < x0 + w, y0 <= y < y0 + h
+// only. Every pixel runs from its own RNG, seeded as the reset seeds it, so
+// the window equals the same pixels of a full render.
+int orc_pt_render_window(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
+                         int32_t neeCount, int32_t iterations, int32_t x0, int32_t y0, int32_t w, int32_t h,
+                         float* outRadiance) {
+    const int W = S->width, H = S->height;
+    if (x0 < 0 || y0 < 0 || w < 0 || h < 0 || x0 + w > W || y0 + h > H) return 1;
+    const int s = seed * (W * H + 1);  // cpu_pt.cpp:40-45
+    const int params[4] = {seed, minPathLength, maxPathLength, neeCount};
+    std::memset(outRadiance, 0, sizeof(float) * 3 * (size_t)w * h);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int k = 0; k < w * h; ++k) {
+        const int px = x0 + k % w, py = y0 + k / w;
+        const int pixel = px + py * W;
+        Pcg64 rng((u32)(pixel + s));
+        float* cum = outRadiance + 3 * (size_t)k;
+        for (int it = 0; it < iterations; ++it) {
+            PixelAcc acc;
+            acc.mask = 1u;
+            pt_sample(*S, params, px, py, rng, acc);
+            const float denom = mx(1.0f, (float)it + 1);
+            for (int c = 0; c < 3; ++c) cum[c] += (acc.v[c] - cum[c]) / denom;
+        }
+    }
+    return 0;
 }
 
 // Radiance of `iterations` iterations of `eval` along the BSDF directions

@@ -187,3 +187,33 @@ def oracle_pt_scene_textured(O, c, textures=None, light_tree=None):
     tx = O.Textures(textures if textures is not None else c["textures"])
     return O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], light_tree, box, c["camera"],
                      textures=tx)
+
+
+def oracle_pt_scene_general(O, sc, light_tree=None, box=None, background_scale=1.0):
+    """Oracle scene of a workload scene dict (several LoDs, instances, point
+    lights, textures, an optional polar env map): the oracle's own LBVHs and
+    instance level; without light_tree / box the host builders make them."""
+    geoms = [(L["verts"], L["tris"]) for L in sc["lods"]]
+    scene, lods = U.oracle_scene(O, geoms, sc["inst_lod"], sc["world_to_inst"])
+    if box is None:
+        boxes = np.stack([L["box"] for L in lods])
+        _, box = O.instance_boxes(sc["world_to_inst"], boxes, sc["inst_lod"])
+    box = np.asarray(box, np.float32)
+    if light_tree is None:
+        light_tree = hip.light_tree_build(None, sc.get("point_lights"), box, sc["materials"])
+    tex = sc.get("textures", [])
+    if sc.get("envmap") is not None:
+        bg = hip.background_record(hip.BACKGROUND_ENVMAP, scale=background_scale, envmap_texture=sc["envmap"])
+        env = tex[sc["envmap"]][0]
+    else:
+        bg = hip.background_record(hip.BACKGROUND_COLORED)
+        env = None
+    flux, sat = hip.background_build(bg, env, box)
+    info = light_tree[0].copy()
+    info["backgroundFlux"] = np.float32(np.float32(flux[0] + flux[1]) + flux[2])
+    lt = (info,) + tuple(light_tree[1:])
+    normals = np.concatenate([L["normals"] for L in sc["lods"]])
+    uvs = np.concatenate([L["uvs"] for L in sc["lods"]])
+    mat_idx = np.concatenate([L["mat_idx"] for L in sc["lods"]])
+    return O.PtScene(scene, normals, uvs, mat_idx, sc["materials"], lt, box, sc["camera"], textures=O.Textures(tex),
+                     background=bg[0], background_sat=sat, background_flux=flux)
