@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--terrain-ray-costs", action="store_true",
                     help="only the per-ray traversal cost of the terrain's camera rays and the trace time of its "
                          "heaviest rays")
+    ap.add_argument("--leaf-limit", type=int, default=None,
+                    help="longest traversal leaf of the LoD builds (0: traverse the reference BVH; default: the "
+                         "library's)")
     ap.add_argument("--leaf-batch", type=int, default=1, choices=[1, 8],
                     help="triangles of a BVH leaf loaded ahead of their tests (with --pt-profile)")
     ap.add_argument("--profile-only", action="store_true",
@@ -399,6 +402,8 @@ def pt_profile(args):
     from mufflon_amd import hip, scenes as S, workloads as W
     from mufflon_amd.hip import HipContext
     ctx = HipContext(0)
+    if args.leaf_limit is not None:
+        ctx.set_traversal_leaf_limit(args.leaf_limit)
     if args.pt_profile == "terrain":
         sc = S.terrain_scene()
         its = 2

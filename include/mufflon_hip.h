@@ -57,6 +57,17 @@ int mfh_lod_create(mfh_context* ctx, const float* vertices, int32_t numVertices,
  * (build_lbvh.cu:782-789, lod.cpp:33). */
 int mfh_lbvh_build(mfh_context* ctx, int32_t lodId, const float* bbox);
 
+/* Traversal hierarchy of later LoD builds (no counterpart in the
+ * reference): every collapsed subtree of the reference's BVH with more than
+ * `limit` triangles stays split in the hierarchy the trace and path-tracing
+ * kernels walk, so that no traversal leaf holds more than `limit` triangles
+ * (default 16; 0 walks the reference BVH itself). Hits do not change (the
+ * closest hit is the lowest (t, instance, primitive) whatever the order of
+ * the tests); mfh_lbvh_get and mfh_trace_closest_stats keep the reference
+ * BVH and its visiting order. */
+int mfh_set_traversal_leaf_limit(mfh_context* ctx, int32_t limit);
+
+
 /* Result of the last build of a LoD: numInternalNodes of the collapsed BVH
  * (LBVH::numInternalNodes, lbvh.hpp:23) and, when the pointers are non-NULL,
  * the nodes (numInternalNodes * 64 bytes, the BvhNode layout of lbvh.hpp:13-18)

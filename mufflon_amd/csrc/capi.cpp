@@ -43,22 +43,31 @@ void refresh_scene(Context& c, Scene& s) {
     for (size_t i = 0; !stale && i < c.lods.size(); ++i)
         stale = c.lods[i] && c.lods[i]->generation != s.lodGeneration[i];
     if (!stale) return;
-    std::vector<DevLod> lods(c.lods.size());
+    std::vector<DevLod> lods(c.lods.size()), ref(c.lods.size());
     s.lodGeneration.assign(c.lods.size(), 0);
     for (size_t i = 0; i < c.lods.size(); ++i) {
         std::memset(&lods[i], 0, sizeof(DevLod));
         const Lod* l = c.lods[i];
-        if (!l || !l->built) continue;
+        if (!l || !l->built) {
+            ref[i] = lods[i];
+            continue;
+        }
         s.lodGeneration[i] = l->generation;
         lods[i].triVerts = l->triVerts.as<float4>();
         lods[i].nodes = l->nodes.ptr;
         lods[i].numInternal = l->numInternal;
         lods[i].numPrims = l->numTris;
         std::memcpy(lods[i].box, l->box, sizeof(l->box));
+        ref[i] = lods[i];
+        if (l->travNumInternal >= 0) {
+            lods[i].nodes = l->travNodes.ptr;
+            lods[i].numInternal = l->travNumInternal;
+        }
     }
     for (int32_t li : s.instLod)
         MFH_REQUIRE(c.lods[li]->built, "instance refers to a LoD whose BVH is not built");
     upload(c, s.dLods, lods.data(), sizeof(DevLod) * lods.size());
+    upload(c, s.dLodsRef, ref.data(), sizeof(DevLod) * ref.size());
     build_scene_bvh(c, s, nullptr);
     MFH_HIP(hipStreamSynchronize(c.stream));
 }
@@ -100,12 +109,13 @@ int mfh_destroy(mfh_context* ctx) {
         for (Lod* l : c.lods) {
             if (!l) continue;
             l->verts.release(); l->tris.release(); l->nodes.release(); l->primIds.release(); l->triVerts.release();
+            l->travNodes.release();
             l->normals.release(); l->uvs.release(); l->matIdx.release();
             delete l;
         }
         for (Scene* s : c.scenes) {
             if (!s) continue;
-            s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release();
+            s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release(); s->dLodsRef.release();
             s->tlasNodes.release(); s->tlasPrimIds.release();
             delete s;
         }
@@ -208,6 +218,14 @@ int mfh_lbvh_build(mfh_context* ctx, int32_t lodId, const float* bbox) {
     });
 }
 
+int mfh_set_traversal_leaf_limit(mfh_context* ctx, int32_t limit) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(limit >= 0 && limit < 1023, "traversal leaf limit out of range (0 .. 1022)");
+        c.traversalLeafLimit = limit;
+    });
+}
+
 int mfh_lbvh_build_stages(mfh_context* ctx, int32_t lodId, const float* bbox, uint64_t* unsortedKeys,
                           uint64_t* sortedKeys, int32_t* sortedIds, int32_t* parents, float* boxes,
                           int32_t* offsets) {
@@ -255,7 +273,7 @@ int mfh_scene_create(mfh_context* ctx, const int32_t* instLod, const float* worl
             refresh_scene(c, *s);
             MFH_HIP(hipStreamSynchronize(c.stream));
         } catch (...) {
-            s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release();
+            s->dInstLod.release(); s->dWorldToInst.release(); s->dLods.release(); s->dLodsRef.release();
             s->tlasNodes.release(); s->tlasPrimIds.release();
             delete s;
             throw;
@@ -303,7 +321,7 @@ int mfh_trace_closest_stats(mfh_context* ctx, int32_t sceneId, const float* dRay
         if (numRays == 0) return;
         MFH_REQUIRE(dRays && dHits && dStats, "null pointer");
         refresh_scene(c, s);
-        launch_trace_closest(trace_params(s), dRays, numRays, static_cast<float4*>(dHits),
+        launch_trace_closest(trace_params(s, true), dRays, numRays, static_cast<float4*>(dHits),
                              reinterpret_cast<float2*>(dBary), reinterpret_cast<uint2*>(dStats), c.stream);
     });
 }

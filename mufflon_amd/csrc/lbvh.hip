@@ -354,8 +354,16 @@ __global__ __launch_bounds__(kSpanThreads) void k_refit_span(int n, long long sp
     }
 }
 
+// Keep marks of the reference's collapse (marks) and, with marks2, of the
+// traversal hierarchy: a node is also kept there when it holds more than
+// refineAbove primitives, so no traversal leaf is longer than that. Both mark
+// sets are closed under descendants (a removed node's subtree is removed).
+// *differ is set when the two sets differ.
+template <class C>
 __global__ __launch_bounds__(kThreads) void k_mark(int nInt, const i32* __restrict__ parents,
-                                                  const uint8_t* __restrict__ flags, i32* __restrict__ marks) {
+                                                  const uint8_t* __restrict__ flags, i32* __restrict__ marks,
+                                                  const float4* __restrict__ boxes, int refineAbove,
+                                                  i32* __restrict__ marks2, u32* __restrict__ differ) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nInt) return;
     int keep = 1;
@@ -370,6 +378,11 @@ __global__ __launch_bounds__(kThreads) void k_mark(int nInt, const i32* __restri
         y = parent_of(parents, y);
     }
     marks[x] = keep;
+    if (marks2) {
+        const int keep2 = (keep || C::sum(__float_as_int(boxes[2 * (size_t)x].w)) > refineAbove) ? 1 : 0;
+        marks2[x] = keep2;
+        if (keep2 != keep) *differ = 1u;
+    }
 }
 
 __device__ __forceinline__ bool is_collapsed(const i32* offsets, int nInt, int node) {
@@ -414,6 +427,16 @@ __global__ __launch_bounds__(kThreads) void k_collapse(int n, const i32* __restr
     out[outIdx + off] = r;
 }
 
+// Zero fill and count hand-off of the build (kernels, not memsets, so that a
+// build can sit in a captured stream).
+__global__ void k_fill_u32(u32* p, int n, u32 v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_num_internal(const i32* offsets, int n, i32* out) {
+    if (threadIdx.x == 0) *out = n > 1 ? offsets[n - 2] : 0;
+}
+
 // World box of each instance (feeds the scene box of the instance level).
 __global__ __launch_bounds__(kThreads) void k_inst_boxes(InstPrims set, int n, float* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -434,12 +457,31 @@ void d2h(hipStream_t s, T* dst, const void* src, size_t count) {
     MFH_HIP(hipStreamSynchronize(s));
 }
 
+void fill_u32(void* p, int n, u32 v, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(kThreads), 0, st, static_cast<u32*>(p), n, v);
+    MFH_HIP(hipGetLastError());
+}
+
+// The traversal hierarchy of a build: the reference's collapsed BVH with
+// every collapsed subtree of more than refineAbove primitives kept split
+// (leaves of at most refineAbove primitives), over the same sorted primitive
+// records. Closest and any hit do not depend on the hierarchy they are found
+// through (the closest hit is the lowest (t, instance, primitive)), so the
+// traversal kernels use it while the exported BVH stays the reference's.
+struct Refinement {
+    int refineAbove = 0;  // 0: none
+    DevBuf* nodes = nullptr;
+    int numInternal = -1;  // -1: equal to the reference (no subtree was split)
+};
+
 // Shared orchestration (build_lbvh.cu:586-780). Returns the number of
 // internal nodes after the collapse; primIds and nodes receive the result.
+// With dNumInternal the count goes to that device int instead and nothing
+// waits for the stream (the return value is then -1).
 template <class P, class C>
-int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& nodes, DevBuf& primIds,
-                  float4* triVerts, StageSink* sink) {
-    BuildScratch& s = ctx.scratch;
+int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const float* bbox, DevBuf& nodes,
+                  DevBuf& primIds, float4* triVerts, StageSink* sink, i32* dNumInternal = nullptr,
+                  Refinement* refine = nullptr) {
     hipStream_t st = ctx.stream;
     s.keysA.reserve(8 * (size_t)n);
     s.keysB.reserve(8 * (size_t)n);
@@ -476,13 +518,13 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, keys, n, parents, children,
                            s.ranges.as<int2>());
-        MFH_HIP(hipMemsetAsync(s.counters.ptr, 0, 4 * (size_t)(n - 1), st));
+        fill_u32(s.counters.ptr, n - 1, 0u, st);
     }
     i32* pending = s.pending.as<i32>();
     u32* pendingCount = reinterpret_cast<u32*>(pending + n);
     i32* pending2 = s.pending2.as<i32>();
     u32* pendingCount2 = reinterpret_cast<u32*>(pending2 + n);
-    MFH_HIP(hipMemsetAsync(pendingCount, 0, 4, st));
+    fill_u32(pendingCount, 1, 0u, st);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_tiles<P, C>), dim3((n + kRefitTile - 1) / kRefitTile),
                        dim3(kRefitThreads), 0, st, set, n, ids, parents, children, s.ranges.as<int2>(), boxes,
                        s.flags.as<uint8_t>(), triVerts, pending, pendingCount);
@@ -490,7 +532,7 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
     for (long long span = kRefitTile; n > 1 && span < n;) {
         span *= kSpanGrowth;
         const int blocks = (int)((n + span - 1) / span);
-        MFH_HIP(hipMemsetAsync(pendingCount2, 0, 4, st));
+        fill_u32(pendingCount2, 1, 0u, st);
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_span<C>), dim3(blocks), dim3(kSpanThreads), 0, st, n, span,
                            parents, children, s.ranges.as<int2>(), boxes, s.counters.as<u32>(),
                            s.flags.as<uint8_t>(), pending, pendingCount, pending2, pendingCount2);
@@ -499,20 +541,55 @@ int build_generic(Context& ctx, const P& set, int n, const float* bbox, DevBuf& 
         std::swap(pendingCount, pendingCount2);
     }
     if (n == 1) {
+        if (dNumInternal) {
+            hipLaunchKernelGGL(k_num_internal, dim3(1), dim3(64), 0, st, nullptr, 1, dNumInternal);
+            MFH_HIP(hipGetLastError());
+            return -1;
+        }
         MFH_HIP(hipStreamSynchronize(st));
         return 0;
     }
     i32* offsets = s.offsets.as<i32>();
-    hipLaunchKernelGGL(k_mark, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, n - 1, parents,
-                       s.flags.as<uint8_t>(), offsets);
+    const bool refining = refine && refine->refineAbove > 0 && !dNumInternal;
+    i32* offsets2 = nullptr;
+    u32* differ = nullptr;
+    if (refining) {
+        s.offsets2.reserve(4 * (size_t)(n - 1) + 4);
+        offsets2 = s.offsets2.as<i32>();
+        differ = reinterpret_cast<u32*>(offsets2 + (n - 1));
+        fill_u32(differ, 1, 0u, st);
+    }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mark<C>), dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, n - 1, parents,
+                       s.flags.as<uint8_t>(), offsets, boxes, refining ? refine->refineAbove : 0, offsets2, differ);
     scan_inclusive_i32(offsets, offsets, n - 1, s, st);
     nodes.reserve(64 * (size_t)(n - 1));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st, n,
                        parents, boxes, offsets, nodes.as<BvhChild>());
     MFH_HIP(hipGetLastError());
+    if (dNumInternal) {
+        hipLaunchKernelGGL(k_num_internal, dim3(1), dim3(64), 0, st, offsets, n, dNumInternal);
+        MFH_HIP(hipGetLastError());
+        return -1;
+    }
     i32 nAfter = 0;
+    u32 differs = 0;
     MFH_HIP(hipMemcpyAsync(&nAfter, offsets + (n - 2), 4, hipMemcpyDeviceToHost, st));
+    if (refining) MFH_HIP(hipMemcpyAsync(&differs, differ, 4, hipMemcpyDeviceToHost, st));
     MFH_HIP(hipStreamSynchronize(st));
+    if (refining) {
+        refine->numInternal = -1;
+        if (differs) {
+            scan_inclusive_i32(offsets2, offsets2, n - 1, s, st);
+            refine->nodes->reserve(64 * (size_t)(n - 1));
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st,
+                               n, parents, boxes, offsets2, refine->nodes->as<BvhChild>());
+            MFH_HIP(hipGetLastError());
+            i32 n2 = 0;
+            MFH_HIP(hipMemcpyAsync(&n2, offsets2 + (n - 2), 4, hipMemcpyDeviceToHost, st));
+            MFH_HIP(hipStreamSynchronize(st));
+            refine->numInternal = n2;
+        }
+    }
     if (sink) {
         d2h(st, sink->parents, parents, 2 * (size_t)n - 1);
         d2h(st, sink->boxes, boxes, 8 * (size_t)(2 * n - 1));
@@ -529,8 +606,12 @@ void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink) {
     if (bbox) std::memcpy(lod.box, bbox, sizeof(lod.box));
     TriPrims set{lod.verts.as<float>(), lod.tris.as<u32>()};
     lod.triVerts.reserve(48 * (size_t)n);
-    lod.numInternal = build_generic<TriPrims, LodCost>(ctx, set, n, lod.box, lod.nodes, lod.primIds,
-                                                       lod.triVerts.as<float4>(), sink);
+    Refinement refine;
+    refine.refineAbove = ctx.traversalLeafLimit;
+    refine.nodes = &lod.travNodes;
+    lod.numInternal = build_generic<TriPrims, LodCost>(ctx, ctx.scratch, set, n, lod.box, lod.nodes, lod.primIds,
+                                                       lod.triVerts.as<float4>(), sink, nullptr, &refine);
+    lod.travNumInternal = refine.numInternal;
     lod.built = true;
 }
 
@@ -559,8 +640,9 @@ void build_scene_bvh(Context& ctx, Scene& sc, StageSink* sink) {
         sc.tlasNumInternal = 0;
         return;
     }
-    sc.tlasNumInternal =
-        build_generic<InstPrims, InstCost>(ctx, set, n, sc.box, sc.tlasNodes, sc.tlasPrimIds, nullptr, sink);
+    sc.tlasNumInternal = build_generic<InstPrims, InstCost>(ctx, ctx.scratch, set, n, sc.box, sc.tlasNodes,
+                                                            sc.tlasPrimIds, nullptr, sink);
 }
+
 
 }  // namespace mfh

@@ -34,12 +34,26 @@ enum : int {
         if (!(cond)) throw ::mfh::Error(::mfh::kInvalidArgument, (msg));       \
     } while (0)
 
-// Device buffer owned by a context; grows on demand, never shrinks.
+// Device buffer owned by a context; grows on demand, never shrinks. A fixed
+// buffer is a view of memory owned elsewhere (a caller's buffer, a static
+// device array): it never allocates or frees, and asking for more than it
+// holds is an error.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
+    static DevBuf view(void* p, size_t n) {
+        DevBuf b;
+        b.ptr = p;
+        b.bytes = n;
+        b.fixed = true;
+        return b;
+    }
     void reserve(size_t n) {
         if (n <= bytes) return;
+        if (fixed)
+            throw Error(kInvalidArgument, "fixed buffer of " + std::to_string(bytes) + " bytes, " +
+                                              std::to_string(n) + " needed");
         if (ptr) MFH_HIP(hipFree(ptr));
         ptr = nullptr;
         bytes = 0;
@@ -50,7 +64,7 @@ struct DevBuf {
         bytes = n;
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !fixed) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -71,6 +85,7 @@ struct BuildScratch {
     DevBuf counters;                  // n-1 arrival counters
     DevBuf flags;                     // n-1 bytes: bit0 count capped, bit1 leaf cheaper than split
     DevBuf offsets;                   // n-1 inclusive scan of the keep marks
+    DevBuf offsets2;                  // n-1 (+1 flag) the same for the traversal hierarchy
     DevBuf hist;                      // radix digit counts
     DevBuf partials;                  // scan partial sums
     DevBuf misc;                      // small values (scene box, counts)
@@ -104,6 +119,10 @@ struct Lod {
     DevBuf nodes;             // numInternal * 64 bytes
     DevBuf primIds;           // numTris, sorted order
     DevBuf triVerts;          // numTris * 3 float4 in sorted order: {v0, id}, {v1, 0}, {v2, 0}
+    // traversal hierarchy: the collapsed BVH with its long leaves kept split
+    // (travNumInternal = -1: none was, traversal uses nodes)
+    DevBuf travNodes;
+    int32_t travNumInternal = -1;
 };
 
 struct Scene {
@@ -112,6 +131,7 @@ struct Scene {
     std::vector<int32_t> instLod;
     std::vector<float> worldToInst;
     DevBuf dInstLod, dWorldToInst, dLods;
+    DevBuf dLodsRef;  // the LoD table over the reference BVHs (visit counts of the reference order)
     float box[6] = {0, 0, 0, 0, 0, 0};
     int32_t tlasNumInternal = 0;
     DevBuf tlasNodes, tlasPrimIds;
@@ -139,6 +159,9 @@ struct Context {
     std::vector<Texture*> textures;
     BuildScratch scratch;
     int lastBuilt = -1;
+    // LoD builds keep collapsed subtrees of more than this many triangles
+    // split in their traversal hierarchy (0: traverse the reference BVH)
+    int traversalLeafLimit = 16;
 };
 
 // Device view of one LoD, as the traversal kernels read it.
@@ -176,7 +199,9 @@ struct TraceParams {
     const int32_t* tlasPrimIds;
     int32_t tlasNumInternal;
 };
-TraceParams trace_params(const Scene& sc);
+// Traversal over the LoDs' traversal hierarchies, or (reference = true) over
+// the reference BVHs, whose visit counts follow the reference's order.
+TraceParams trace_params(const Scene& sc, bool reference = false);
 void launch_trace_closest(const TraceParams& p, const float* rays, int n, float4* hits, float2* bary, uint2* stats,
                           hipStream_t stream);
 void launch_trace_any(const TraceParams& p, const float* rays, int n, int32_t* occluded, hipStream_t stream);

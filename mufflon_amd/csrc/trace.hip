@@ -51,12 +51,12 @@ void launch_trace(const TraceParams& p, const float* rays, int n, float4* hits, 
 
 }  // namespace
 
-TraceParams trace_params(const Scene& sc) {
+TraceParams trace_params(const Scene& sc, bool reference) {
     TraceParams p;
     p.numInstances = sc.numInstances;
     p.worldToInst = sc.dWorldToInst.as<float>();
     p.instLod = sc.dInstLod.as<int32_t>();
-    p.lods = sc.dLods.as<DevLod>();
+    p.lods = (reference ? sc.dLodsRef : sc.dLods).as<DevLod>();
     p.tlasNodes = sc.tlasNodes.ptr;
     p.tlasPrimIds = sc.tlasPrimIds.as<int32_t>();
     p.tlasNumInternal = sc.tlasNumInternal;

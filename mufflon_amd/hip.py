@@ -90,6 +90,7 @@ _SIGS = {
     "mfh_pt_get_variance": [vp, i32, i32, f32p],
     "mfh_pt_set_mode": [vp, i32, i32, i32],
     "mfh_pt_adjoint": [vp, i32, i32, f32p],
+    "mfh_set_traversal_leaf_limit": [vp, i32],
     "mfh_pt_set_adjoint_mode": [vp, i32, i32],
     "mfh_batch_create": [vp, i32p, i32, i32, i32p],
     "mfh_batch_reset": [vp, i32],
@@ -299,6 +300,10 @@ class HipContext:
     def lbvh_build(self, lod, bbox=None):
         b = None if bbox is None else np.ascontiguousarray(bbox, dtype=np.float32)
         self._call("mfh_lbvh_build", self.handle, lod, _ptr(b, f32p))
+
+    def set_traversal_leaf_limit(self, limit):
+        """Longest traversal leaf of later LoD builds (0: the reference BVH)."""
+        self._call("mfh_set_traversal_leaf_limit", self.handle, int(limit))
 
     def lbvh_get(self, lod, num_prims):
         n = i32()
