@@ -26,11 +26,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one closest-hit ray: ray in, hit +
-# barycentrics out, 64 B per internal node visit, 48 B per triangle test
+# algorithmic bytes of one closest-hit ray: ray in, hit + barycentrics out,
+# 64 B per internal node visit, 48 B per triangle test
 RAY_IO_BYTES = 32 + 16 + 8
 NODE_BYTES = 64
 TRI_BYTES = 48
+# algorithmic bytes of the path tracer, per unit of the oracle's work counts
+PT_PATH_BYTES = 16 + 2 * 12       # RNG state read + write; Radiance running mean read + write
+PT_RAY_BYTES = 32 + 16            # ray in, hit out (extension and shadow rays)
+PT_VERTEX_BYTES = 60 + 32         # 3 normals + 3 uvs gathered, material record
+PT_TEXEL_BYTES = 16               # one RGBA32F texel per tap
+PT_TREE_NODE_BYTES = 32           # light-tree node
+CPU_RUNS = 5                      # CPU baselines: median of this many runs after one warm-up
 
 
 def parse():
@@ -47,6 +54,7 @@ def parse():
     ap.add_argument("--trace-block", type=int, default=256, help="wavefront trace block (with --pt-profile)")
     ap.add_argument("--stack-depth", type=int, default=16, help="wavefront LDS stack depth (with --pt-profile)")
     ap.add_argument("--refill", type=int, default=0, help="wavefront lane refill budget (with --pt-profile)")
+    ap.add_argument("--sort-bounces", type=int, default=0, help="wavefront sorted bounces (with --pt-profile)")
     ap.add_argument("--terrain-ray-costs", action="store_true",
                     help="only the per-ray traversal cost of the terrain's camera rays and the trace time of its "
                          "heaviest rays")
@@ -69,29 +77,85 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    """CPU model name of this host (the first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, runs=CPU_RUNS):
+    """Median wall time (s) of `runs` calls of fn after one warm-up call."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def cpu_baseline(verts, tris, rays):
     """The C++ restatement of the reference CPU path (oracle/) on the host
-    cores: serial LBVH build (thrust::seq order) + OpenMP closest hit over all
-    rays. Timed once after one warm-up build on a 100k-triangle sample."""
+    cores, on the full workload: LBVH build with the OpenMP stages (the
+    reference's path for LoDs of 1000 or more primitives, lod.cpp:26) and
+    serially (thrust::seq order), each bit-identical to the other, + OpenMP
+    closest hit of all rays. Medians of CPU_RUNS runs after one warm-up."""
     from oracle import oracle as O
+    from mufflon_amd.scenes import IDENTITY34
     cores = omp_threads()
-    O.lbvh_build_tris(verts[: 300_000], tris[:100_000])  # warm-up
-    t0 = time.perf_counter()
-    b = O.lbvh_build_tris(verts, tris)
-    t1 = time.perf_counter()
+    built = {}
+
+    def build_omp():
+        built["b"] = O.lbvh_build_tris_omp(verts, tris)
+
+    omp_s = median_time(build_omp)
+    serial_s = median_time(lambda: O.lbvh_build_tris(verts, tris))
+    b = built["b"]
     lod = dict(verts=verts, tris=tris, nodes=b["nodes"], num_internal=b["num_internal"],
                prim_ids=b["sorted_ids"], box=b["bbox"])
-    from mufflon_amd.scenes import IDENTITY34
     scene = O.Scene([lod], np.array([0], np.int32), IDENTITY34.reshape(1, 12))
-    t2 = time.perf_counter()
-    O.trace_closest(scene, rays)
-    t3 = time.perf_counter()
-    build_s, trace_s = t1 - t0, t3 - t2
-    return dict(value=len(rays) / (build_s + trace_s) / 1e6, unit="Mrays/s", cores=cores, kind="port",
-                sample="full workload: serial LBVH build of all %d triangles (%.3f s) + closest hit of all %d "
-                       "rays on %d OpenMP threads (%.3f s, %.2f Mrays/s)" %
-                       (len(tris), build_s, len(rays), cores, trace_s, len(rays) / trace_s / 1e6),
-                lbvh_build_ms=build_s * 1e3, trace_mrays_s=len(rays) / trace_s / 1e6)
+    trace_s = median_time(lambda: O.trace_closest(scene, rays))
+    return dict(value=len(rays) / (omp_s + trace_s) / 1e6, unit="Mrays/s", cores=cores, kind="port",
+                cpu_model=cpu_model(),
+                sample="full workload, median of %d runs: OpenMP LBVH build of all %d triangles (%.3f s; serial "
+                       "%.3f s) + closest hit of all %d rays (%.3f s, %.2f Mrays/s), %d OpenMP threads" %
+                       (CPU_RUNS, len(tris), omp_s, serial_s, len(rays), trace_s, len(rays) / trace_s / 1e6, cores),
+                lbvh_build_ms=round(omp_s * 1e3, 2), lbvh_build_serial_ms=round(serial_s * 1e3, 2),
+                trace_mrays_s=round(len(rays) / trace_s / 1e6, 3))
+
+
+def path_bytes(stats, light_record_bytes):
+    """Algorithmic bytes of one path-tracer iteration from the oracle's work
+    counts of its paths (oracle.pt_path_stats): per path the RNG state and the
+    Radiance mean; per extension or shadow ray the ray and its hit, 64 B per
+    internal node visited, 48 B per triangle tested (the reference order's
+    counts); per surface vertex its normal / uv gathers and material record;
+    16 B per texel tap, 32 B per light-tree node, one light record per NEE
+    light. Environment-map table searches are not counted."""
+    st = stats
+    return (PT_PATH_BYTES * st["paths"] + PT_RAY_BYTES * (st["ext_rays"] + st["shadow_rays"])
+            + NODE_BYTES * (st["ext_nodes"] + st["shadow_nodes"]) + TRI_BYTES * (st["ext_tris"] + st["shadow_tris"])
+            + PT_VERTEX_BYTES * st["surface_vertices"] + PT_TEXEL_BYTES * st["texel_taps"]
+            + PT_TREE_NODE_BYTES * st["tree_nodes"] + light_record_bytes * st["light_records"])
+
+
+def path_roofline(stats, light_record_bytes, ms_per_iteration, kernel):
+    """Roofline record of a path-tracer leg: algorithmic bytes of one
+    iteration over its event-timed duration, against the HBM peak."""
+    b = path_bytes(stats, light_record_bytes)
+    achieved = b / (ms_per_iteration / 1e3) / 1e9
+    p = stats["paths"]
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "achieved_is": "algorithmic bytes (oracle work counts of the same paths) / iteration time",
+            "algorithmic_bytes_per_iteration": int(b), "bytes_per_path": round(b / p, 1),
+            "per_path": {k: round(v / p, 3) for k, v in stats.items() if k != "paths"}}
 
 
 def converged_rmse(image):
@@ -179,23 +243,23 @@ def pt_cornell(args, cpu_baseline_too):
                         "tuning": tuned, "graph": True,
                         "identical_to_megakernel": bool(same and np.array_equal(ctx.pt_target(pt), mega))}
     out["rmse"] = converged_rmse(mega)
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    lt = ctx.pt_light_tree(pt)
+    import scene_util as U
+    scene, _ = U.oracle_scene(O, [(v, c["tris"])], np.array([0], np.int32), S.IDENTITY34.reshape(1, 12))
+    ps = O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], lt, box, c["camera"])
+    stats = O.pt_path_stats(ps, 1, 0, 8, 1, 0, 0, 256, 256)
+    out["roofline"] = path_roofline(stats, 64, ms / n, "k_pt_iterate (megakernel)")
+    out["roofline"]["wavefront_achieved"] = round(path_bytes(stats, 64) / (wms / n / 1e3) / 1e9, 1)
     if cpu_baseline_too:
-        from oracle import oracle as O
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        lt = ctx.pt_light_tree(pt)
-        import scene_util as U
-        scene, _ = U.oracle_scene(O, [(v, c["tris"])], np.array([0], np.int32), S.IDENTITY34.reshape(1, 12))
-        ps = O.PtScene(scene, c["normals"], c["uvs"], c["mat_idx"], c["materials"], lt, box, c["camera"])
-        O.pt_render(ps, 1, 0, 8, 1, 1)
         k = 8
-        t0 = time.perf_counter()
-        O.pt_render(ps, 1, 0, 8, 1, k)
-        dt = time.perf_counter() - t0
+        dt = median_time(lambda: O.pt_render(ps, 1, 0, 8, 1, k))
         threads = omp_threads()
         out["cpu_baseline"] = {"value": round(pixels * k / dt / 1e6, 4), "unit": "Mpaths/s",
-                               "cores": threads, "kind": "port",
-                               "sample": "%d iterations of the same Cornell workload on %d OpenMP threads (%.2f s)"
-                                         % (k, threads, dt)}
+                               "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "%d iterations of the same Cornell workload (all 65,536 pixels) on %d "
+                                         "OpenMP threads, median of %d runs: %.3f s" % (k, threads, CPU_RUNS, dt)}
     ctx.close()
     return out
 
@@ -220,7 +284,7 @@ def pt_terrain(args):
         ctx.timer_begin()
         ctx.lbvh_build(lod)
         builds.append(ctx.timer_end())
-    pt, _, _ = W.path_tracer(ctx, sc, max_path_length=16)
+    pt, scid, _ = W.path_tracer(ctx, sc, max_path_length=16)
     its = args.terrain_iterations
     pixels = sc["camera"]["width"] * sc["camera"]["height"]
 
@@ -241,20 +305,46 @@ def pt_terrain(args):
     ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
     tuned, _ = ctx.pt_autotune(pt, 1)
     wf_ms, wf_img = run(hip.PT_WAVEFRONT, True)
+    box, _, _, _ = ctx.scene_get(scid, len(sc["inst_lod"]))
+    lt = ctx.pt_light_tree(pt)
     ctx.close()
-    return {"workload": "terrain 2236^2 fBm heightfield (9,990,450 tris) + 4096 rock instances (2000 tris), "
-                        "1024 point lights, 2048x1024 env map, 1920x1080, depth 16, NEE 1, seed 1",
-            "scene_generation_s": round(gen_s, 2),
-            "lbvh_10m_build_ms": round(float(np.mean(builds)), 3), "lbvh_10m_build_ms_min": round(min(builds), 3),
-            "iterations": its,
-            "megakernel": {"ms_per_iteration": round(mk_ms, 3), "mpaths_s": round(pixels / mk_ms / 1e3, 3)},
-            "megakernel_leaf_batch_8": {"ms_per_iteration": round(mk8_ms, 3),
-                                        "mpaths_s": round(pixels / mk8_ms / 1e3, 3)},
-            "wavefront": {"ms_per_iteration": round(wf_ms, 3), "mpaths_s": round(pixels / wf_ms / 1e3, 3),
-                          "tuning": tuned, "graph": True},
-            "wavefront_over_megakernel": round(mk_ms / wf_ms, 3),
-            "wavefront_over_best_megakernel": round(min(mk_ms, mk8_ms) / wf_ms, 3),
-            "identical_images": bool(np.array_equal(mk_img, wf_img) and np.array_equal(mk_img, mk8_img))}
+    best = min(mk_ms, mk8_ms)
+    out = {"workload": "terrain 2236^2 fBm heightfield (9,990,450 tris) + 4096 rock instances (2000 tris), "
+                       "1024 point lights, 2048x1024 env map, 1920x1080, depth 16, NEE 1, seed 1",
+           "scene_generation_s": round(gen_s, 2),
+           "lbvh_10m_build_ms": round(float(np.mean(builds)), 3), "lbvh_10m_build_ms_min": round(min(builds), 3),
+           "iterations": its,
+           "megakernel": {"ms_per_iteration": round(mk_ms, 3), "mpaths_s": round(pixels / mk_ms / 1e3, 3)},
+           "megakernel_leaf_batch_8": {"ms_per_iteration": round(mk8_ms, 3),
+                                       "mpaths_s": round(pixels / mk8_ms / 1e3, 3)},
+           "wavefront": {"ms_per_iteration": round(wf_ms, 3), "mpaths_s": round(pixels / wf_ms / 1e3, 3),
+                         "tuning": tuned, "graph": True},
+           "wavefront_over_megakernel": round(mk_ms / wf_ms, 3),
+           "wavefront_over_best_megakernel": round(best / wf_ms, 3),
+           "identical_images": bool(np.array_equal(mk_img, wf_img) and np.array_equal(mk_img, mk8_img)),
+           "equal_rmse": "the three forms give identical images, so their RMSE against any reference is equal"}
+    # algorithmic bytes from the oracle's work counts of every path of one
+    # iteration, and the oracle itself as the CPU baseline
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pt_util as P
+    t0 = time.perf_counter()
+    ps = P.oracle_pt_scene_general(O, sc, lt, box, parallel=True)
+    setup_s = time.perf_counter() - t0
+    stats = O.pt_path_stats(ps, 1, 0, 16, 1, 0, 0, sc["camera"]["width"], sc["camera"]["height"])
+    fast = "megakernel" if mk_ms <= mk8_ms else "megakernel_leaf_batch_8"
+    out["roofline"] = path_roofline(stats, 32, best, "k_pt_iterate (%s)" % fast)
+    out["roofline"]["wavefront_achieved"] = round(path_bytes(stats, 32) / (wf_ms / 1e3) / 1e9, 1)
+    if not args.no_cpu_baseline:
+        w, h = sc["camera"]["width"], sc["camera"]["height"]
+        dt = median_time(lambda: O.pt_render_window(ps, 1, 0, 16, 1, 1, 0, 0, w, h))
+        threads = omp_threads()
+        out["cpu_baseline"] = {"value": round(pixels / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads,
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "one iteration of every pixel of the terrain workload on %d OpenMP threads, "
+                                         "median of %d runs: %.3f s (oracle scene set-up %.1f s not included)"
+                                         % (threads, CPU_RUNS, dt, setup_s)}
+    return out
 
 
 def pt_adjoint(args):
@@ -299,13 +389,33 @@ def pt_adjoint(args):
     ctx.timer_begin()
     ctx.pt_adjoint(pt, 4, (2.0 * (img - target)).astype(np.float32))
     adj_ms = ctx.timer_end()
+    lt = ctx.pt_light_tree(pt)
     ctx.close()
-    return {"workload": "Cornell 256x256, albedo texture 512^2 (back wall) + roughness texture 256^2 (GGX sphere), "
-                        "4 spp, L2 loss",
-            "s_per_iteration": round(float(np.median(times)), 5),
-            "adjoint_kernel_ms": round(adj_ms, 3),
-            "loss": float(np.sum((img - target) ** 2)), "albedo_grad_l1": float(np.abs(g).sum()),
-            "roughness_grad_l1": float(np.abs(gr).sum())}
+    out = {"workload": "Cornell 256x256, albedo texture 512^2 (back wall) + roughness texture 256^2 (GGX sphere), "
+                       "4 spp, L2 loss",
+           "s_per_iteration": round(float(np.median(times)), 5),
+           "adjoint_kernel_ms": round(adj_ms, 3),
+           "loss": float(np.sum((img - target) ** 2)), "albedo_grad_l1": float(np.abs(g).sum()),
+           "roughness_grad_l1": float(np.abs(gr).sum())}
+    if not args.no_cpu_baseline:
+        # the FP64 CPU gradient of the oracle: primal 4 spp + adjoint of the same paths
+        from oracle import oracle as O
+        import pt_util as P
+        ps = P.oracle_pt_scene_textured(O, c, light_tree=lt)
+        delta = (2.0 * (img - target)).astype(np.float32)
+
+        def step():
+            O.pt_render(ps, 1, 0, 8, 1, 4)
+            O.pt_adjoint(ps, 1, 0, 8, 1, 4, delta, {0: (512, 512), 1: (256, 256)})
+
+        dt = median_time(step)
+        threads = omp_threads()
+        out["cpu_baseline"] = {"value": round(dt, 4), "unit": "s/iteration", "cores": threads, "kind": "port",
+                               "cpu_model": cpu_model(),
+                               "sample": "the same step on the oracle (float primal 4 spp + FP64 adjoint of the "
+                                         "same paths, albedo and roughness textures) on %d OpenMP threads, median "
+                                         "of %d runs" % (threads, CPU_RUNS)}
+    return out
 
 
 def pt_batch(args):
@@ -316,7 +426,7 @@ def pt_batch(args):
     from mufflon_amd.hip import HipContext
     t0 = time.perf_counter()
     ctx = HipContext(0)
-    b, pts, texs, scenes = W.uneven_batch(ctx, count=args.batch_scenes)
+    b, pts, texs, scenes, sids = W.uneven_batch(ctx, count=args.batch_scenes, with_scene_ids=True)
     setup_s = time.perf_counter() - t0
     paths = sum(sc["camera"]["width"] * sc["camera"]["height"] for sc in scenes)
     tuned, _ = ctx.batch_autotune(b, 1)
@@ -335,14 +445,41 @@ def pt_batch(args):
         ctx.pt_adjoint(pt, 4, (2.0 * ctx.pt_target(pt)).astype(np.float32))
         ctx.pt_texture_gradient(pt, int(tex[1]), (64, 64, 3))
     adj_s = time.perf_counter() - t0
+    gpu_scenes = [(ctx.pt_light_tree(pt), ctx.scene_get(sid, len(sc["inst_lod"]))[0])
+                  for pt, sid, sc in zip(pts, sids, scenes)]
     ctx.close()
-    return {"workload": "%d scenes, %d triangles, %d paths per iteration" %
-                        (len(scenes), sum(len(sc["lods"][0]["tris"]) for sc in scenes), paths),
-            "setup_s": round(setup_s, 1), "tuning": tuned,
-            "batch_ms_per_iteration": round(bms, 3), "batch_mpaths_s": round(paths / bms / 1e3, 3),
-            "megakernel_sequential_ms_per_iteration": round(mms, 3),
-            "megakernel_sequential_mpaths_s": round(paths / mms / 1e3, 3),
-            "albedo_gradients_8_scenes_s": round(adj_s, 3)}
+    out = {"workload": "%d scenes, %d triangles, %d paths per iteration" %
+                       (len(scenes), sum(len(sc["lods"][0]["tris"]) for sc in scenes), paths),
+           "setup_s": round(setup_s, 1), "tuning": tuned,
+           "batch_ms_per_iteration": round(bms, 3), "batch_mpaths_s": round(paths / bms / 1e3, 3),
+           "megakernel_sequential_ms_per_iteration": round(mms, 3),
+           "megakernel_sequential_mpaths_s": round(paths / mms / 1e3, 3),
+           "albedo_gradients_8_scenes_s": round(adj_s, 3)}
+    # algorithmic bytes from the oracle's work counts of every path of every
+    # scene (one iteration), and the oracle on every 12th scene as CPU baseline
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import pt_util as P
+    total = {}
+    cpu_paths, cpu_s, sample = 0, 0.0, []
+    for i, (sc, (lt, box)) in enumerate(zip(scenes, gpu_scenes)):
+        ps = P.oracle_pt_scene_general(O, sc, lt, box, parallel=True)
+        w, h, d = sc["camera"]["width"], sc["camera"]["height"], sc["max_path_length"]
+        for k, v in O.pt_path_stats(ps, 1, 0, d, 1, 0, 0, w, h).items():
+            total[k] = total.get(k, 0) + v
+        if not args.no_cpu_baseline and i % 12 == 0:
+            cpu_s += median_time(lambda: O.pt_render_window(ps, 1, 0, d, 1, 1, 0, 0, w, h))
+            cpu_paths += w * h
+            sample.append("%d (%dx%d, %d tris, depth %d)" % (i, w, h, len(sc["lods"][0]["tris"]), d))
+        del ps
+    out["roofline"] = path_roofline(total, 32, bms, "k_wf_* (wavefront batch, all stages of one iteration)")
+    if not args.no_cpu_baseline:
+        threads = omp_threads()
+        out["cpu_baseline"] = {"value": round(cpu_paths / cpu_s / 1e6, 4), "unit": "Mpaths/s", "cores": threads,
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "one iteration of every pixel of scenes %s on %d OpenMP threads, median "
+                                         "of %d runs each: %.3f s" % (", ".join(sample), threads, CPU_RUNS, cpu_s)}
+    return out
 
 
 def terrain_ray_costs():
@@ -411,7 +548,7 @@ def pt_profile(args):
     else:
         its = 8
         pt, _, _ = cornell_tracer(ctx)
-    ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, 0, args.refill, args.leaf_batch)
+    ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, args.sort_bounces, args.refill, args.leaf_batch)
     for mode in (hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT):
         ctx.pt_set_mode(pt, mode, False)
         ctx.pt_reset(pt)

@@ -350,6 +350,33 @@ Boolean renderer_get_parameter_bool(MufflonInstanceHdl instHdl, const char* name
 Boolean renderer_set_parameter_enum(MufflonInstanceHdl instHdl, const char* name, int value);
 Boolean renderer_get_parameter_enum(MufflonInstanceHdl instHdl, const char* name, int* value);
 
+/* ---- Differentiable rendering. No counterpart in the reference (it has no
+ * differentiable renderer): the adjoint (path replay) of the "Pathtracer"
+ * renderer made reachable for callers of this API, such as the ctypes
+ * binding of bindings.py:373-386, which binds core_interface.h. */
+/* Replace the texels of a texture (width * height * channels floats of its
+ * stored format, channels as world_get_texture_size's texture was created:
+ * 1 for R32F, 4 for RGBA32F). Resets the renderer (the scene is reloaded at
+ * the next iteration if the texture is a 1x1 value or the environment map). */
+Boolean render_set_texture_data(MufflonInstanceHdl instHdl, TextureHdl tex, const float* texels);
+/* Set the radiance scale of the EMISSIVE material assigned to a material slot
+ * of the current scenario (MaterialParams' emissive.scale); resets. */
+Boolean render_set_emission_scale(MufflonInstanceHdl instHdl, MatIdx slot, Vec3 scale);
+/* Derivative form of render_adjoint: 0 (default) an unbiased estimate of the
+ * derivative of the expected image; 1 the derivative of this realization
+ * with every random decision held fixed. */
+Boolean render_set_adjoint_mode(MufflonInstanceHdl instHdl, int32_t mode);
+/* Given radianceAdjoint = dLoss/dRadiance (height x width x 3 floats, the
+ * layout of mufflon_get_target_image) of the current Radiance target, the
+ * mean of render_get_current_iteration() iterations since the last reset,
+ * replay those paths and accumulate the gradients read below. Synchronous. */
+Boolean render_adjoint(MufflonInstanceHdl instHdl, const float* radianceAdjoint);
+/* dLoss/d(texel) of an albedo or roughness texture larger than 1x1 used by
+ * the loaded scene: width * height * channels floats, the texel layout. */
+Boolean render_get_texture_gradient(MufflonInstanceHdl instHdl, TextureHdl tex, float* gradient);
+/* dLoss/d(emission scale) of the material in a slot of the current scenario. */
+Boolean render_get_emission_gradient(MufflonInstanceHdl instHdl, MatIdx slot, Vec3* gradient);
+
 /* ---- Profiling (interface.cpp:4180-4260): accepted and ignored; no core
  * profiler is kept. */
 void profiling_enable(void);

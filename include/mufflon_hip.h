@@ -123,12 +123,13 @@ int mfh_trace_any(mfh_context* ctx, int32_t sceneId, const float* dRays, int32_t
  * replaces is core_interface.h's MaterialParams (inner/outer medium, layer
  * parameters). Texture slots hold a texture id of mfh_texture_create or -1
  * for the constant in the record (Mufflon's 1x1 value textures,
- * world_add_texture_value). The inner medium of MICROFACET is
+ * world_add_texture_value). The inner medium of WALTER and MICROFACET is
  * {refractionIndex, absorption}; every other type has inner = outer medium
  * (Material::compute_medium, material.cpp:54-60). */
 #define MFH_MATERIAL_EMISSIVE 0
 #define MFH_MATERIAL_LAMBERT 1
 #define MFH_MATERIAL_TORRANCE 3
+#define MFH_MATERIAL_WALTER 4 /* rough refraction (microfacet_refractive.hpp); inner medium as MICROFACET */
 #define MFH_MATERIAL_MICROFACET 10
 #define MFH_NDF_GGX 0
 #define MFH_NDF_BECKMANN 1
@@ -139,11 +140,11 @@ typedef struct mfh_material {
     float albedo[3];             /* LAMBERT, TORRANCE: albedo (ALBEDO texture) */
     float radiance[3];           /* EMISSIVE: radiance (EMISSION texture) */
     float scale[3];              /* EMISSIVE: radiance scale (MatEmissive::NonTexParams) */
-    float roughness[2];          /* TORRANCE, MICROFACET: alpha (ROUGHNESS texture) */
-    int32_t ndf;                 /* TORRANCE, MICROFACET: MFH_NDF_* */
-    int32_t shadowing;           /* TORRANCE, MICROFACET: MFH_SHADOWING_* */
-    float refractionIndex;       /* MICROFACET: inner medium */
-    float absorption[3];         /* MICROFACET: inner medium absorption per unit length */
+    float roughness[2];          /* TORRANCE, WALTER, MICROFACET: alpha (ROUGHNESS texture) */
+    int32_t ndf;                 /* TORRANCE, WALTER, MICROFACET: MFH_NDF_* */
+    int32_t shadowing;           /* TORRANCE, WALTER, MICROFACET: MFH_SHADOWING_* */
+    float refractionIndex;       /* WALTER, MICROFACET: inner medium */
+    float absorption[3];         /* WALTER, MICROFACET: inner medium absorption per unit length */
     float outerRefractionIndex;  /* medium on the normal side, 1 = vacuum */
     float outerAbsorption[3];
     int32_t albedoTexture;       /* -1 or a texture id */
@@ -383,7 +384,7 @@ int mfh_pt_wireframe_get(mfh_context* ctx, int32_t ptId, int32_t variance, float
  * `iterations` iterations from a reset, and radianceAdjoint = dLoss/dRadiance
  * (host, 3 floats per pixel, row-major), replays the same paths and
  * accumulates dLoss/d(texel) of every albedo texture (LAMBERT, TORRANCE),
- * every roughness texture (TORRANCE, MICROFACET; one channel drives both
+ * every roughness texture (TORRANCE, WALTER, MICROFACET; one channel drives both
  * roughnesses, two channels x and y) and dLoss/d(emission scale) of every
  * material. By default (MFH_ADJOINT_EXPECTED) the result is an unbiased
  * estimate of the derivative of the expected image: each contribution times

@@ -247,6 +247,13 @@ _SIGS = {
     "renderer_get_parameter_bool": (Boolean, [vp, cp, C.POINTER(Boolean)]),
     "renderer_set_parameter_float": (Boolean, [vp, cp, C.c_float]),
     "renderer_get_parameter_float": (Boolean, [vp, cp, C.POINTER(C.c_float)]),
+    "world_get_texture": (vp, [vp, cp]),
+    "render_set_texture_data": (Boolean, [vp, vp, C.POINTER(C.c_float)]),
+    "render_set_emission_scale": (Boolean, [vp, C.c_uint16, Vec3]),
+    "render_set_adjoint_mode": (Boolean, [vp, i32]),
+    "render_adjoint": (Boolean, [vp, C.POINTER(C.c_float)]),
+    "render_get_texture_gradient": (Boolean, [vp, vp, C.POINTER(C.c_float)]),
+    "render_get_emission_gradient": (Boolean, [vp, C.c_uint16, C.POINTER(Vec3)]),
 }
 
 _core = None
@@ -565,6 +572,43 @@ class CoreInterface:
             out.append((name, t.value))
         return out
 
+    # differentiable rendering (additions of this library, include/mufflon_core.h)
+    def world_get_texture(self, path):
+        return self.lib.world_get_texture(self.muffInst, _b(path))
+
+    def render_set_texture_data(self, tex, texels):
+        """Replace a texture's texels: (h, w, channels) in its stored format."""
+        t = np.ascontiguousarray(texels, np.float32)
+        if not self.lib.render_set_texture_data(self.muffInst, tex, t.ctypes.data_as(C.POINTER(C.c_float))):
+            raise RuntimeError(self.core_get_dll_error())
+
+    def render_set_emission_scale(self, slot, scale):
+        if not self.lib.render_set_emission_scale(self.muffInst, int(slot), _v3(scale)):
+            raise RuntimeError(self.core_get_dll_error())
+
+    def render_set_adjoint_mode(self, mode):
+        if not self.lib.render_set_adjoint_mode(self.muffInst, int(mode)):
+            raise RuntimeError(self.core_get_dll_error())
+
+    def render_adjoint(self, radiance_adjoint):
+        """Replay the current Radiance image's paths with dLoss/dRadiance (h, w, 3)."""
+        d = np.ascontiguousarray(radiance_adjoint, np.float32)
+        if not self.lib.render_adjoint(self.muffInst, d.ctypes.data_as(C.POINTER(C.c_float))):
+            raise RuntimeError(self.core_get_dll_error())
+
+    def render_get_texture_gradient(self, tex, shape):
+        """dLoss/d(texel) of a texture, shape (h, w, channels) of its stored format."""
+        out = np.zeros(shape, np.float32)
+        if not self.lib.render_get_texture_gradient(self.muffInst, tex, out.ctypes.data_as(C.POINTER(C.c_float))):
+            raise RuntimeError(self.core_get_dll_error())
+        return out
+
+    def render_get_emission_gradient(self, slot):
+        g = Vec3()
+        if not self.lib.render_get_emission_gradient(self.muffInst, int(slot), C.byref(g)):
+            raise RuntimeError(self.core_get_dll_error())
+        return np.array([g.x, g.y, g.z], np.float32)
+
     def mufflon_get_target_image(self, name, variance, width, height):
         """The target as a (height, width, channels) copy, or None before a scene is loaded."""
         ptr = C.POINTER(C.c_float)()
@@ -625,8 +669,8 @@ def _material_params(core, m, tex_handles):
         p.inner.torrance.roughness = roughness()
         p.inner.torrance.ndf = _NDF[int(m["ndf"])]
         p.inner.torrance.shadowingModel = _SHADOWING[int(m["shadowing"])]
-    elif kind == S.MAT_MICROFACET:
-        p.innerType = MATERIAL_MICROFACET
+    elif kind in (S.MAT_WALTER, S.MAT_MICROFACET):
+        p.innerType = MATERIAL_WALTER if kind == S.MAT_WALTER else MATERIAL_MICROFACET
         p.inner.walter.roughness = roughness()
         p.inner.walter.ndf = _NDF[int(m["ndf"])]
         p.inner.walter.shadowingModel = _SHADOWING[int(m["shadowing"])]

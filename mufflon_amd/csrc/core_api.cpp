@@ -238,6 +238,9 @@ struct Core {
     mfh_context* ctx = nullptr;
     int32_t pt = -1, scene = -1;
     float sceneBox[6] = {0, 0, 0, 0, 0, 0};
+    std::map<const Texture*, int32_t> texIds;  // device texture of every texture larger than 1x1 in use
+    const Texture* envMap = nullptr;           // the loaded scene's environment map
+    int32_t adjointMode = 0;                   // MFH_ADJOINT_*
     // renderer state
     int variation = -1;  // -1: no renderer enabled
     int renderer = kRendererPt;  // kRendererPt (variations 0, 1) or kRendererWire (variation 0)
@@ -259,6 +262,8 @@ struct Core {
         if (ctx) (void)mfh_destroy(ctx);
         ctx = nullptr;
         pt = scene = -1;
+        texIds.clear();
+        envMap = nullptr;
     }
     void invalidate() { sceneValid = false; }
 };
@@ -429,8 +434,9 @@ Material make_material(const char* name, const MaterialParams& p) {
             r.ndf = ndf(p.inner.torrance.ndf);
             r.shadowing = shadowing(p.inner.torrance.shadowingModel);
             break;
-        case MATERIAL_MICROFACET:
-            r.type = MFH_MATERIAL_MICROFACET;
+        case MATERIAL_WALTER:      // interface.cpp:1275-1279
+        case MATERIAL_MICROFACET:  // the same parameters (interface.cpp:1280-1284)
+            r.type = p.innerType == MATERIAL_WALTER ? MFH_MATERIAL_WALTER : MFH_MATERIAL_MICROFACET;
             m.textures[1] = tex(p.inner.walter.roughness);
             r.ndf = ndf(p.inner.walter.ndf);
             r.shadowing = shadowing(p.inner.walter.shadowingModel);
@@ -449,11 +455,10 @@ Material make_material(const char* name, const MaterialParams& p) {
             r.scale[2] = p.inner.emissive.scale.z;
             break;
         case MATERIAL_ORENNAYAR:
-        case MATERIAL_WALTER:
         case MATERIAL_BLEND:
         case MATERIAL_FRESNEL:
             throw ApiError("material type " + std::to_string((int)p.innerType) +
-                           " is not supported by the HIP path tracer (LAMBERT, TORRANCE, MICROFACET and "
+                           " is not supported by the HIP path tracer (LAMBERT, TORRANCE, WALTER, MICROFACET and "
                            "EMISSIVE are)");
         default:
             throw ApiError("Invalid material params: type unknown.");
@@ -676,6 +681,9 @@ void load_scene(Core& c, Scenario& s) {
         }
         check_mfh(mfh_pt_set_background(c.ctx, c.pt, &b), "background");
     }
+    c.texIds = ids;
+    c.envMap = (bg && bg->type == BACKGROUND_ENVMAP) ? bg->map : nullptr;
+    check_mfh(mfh_pt_set_adjoint_mode(c.ctx, c.pt, c.adjointMode), "adjoint mode");
     apply_targets(c);
     apply_mode(c);
     check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
@@ -2151,6 +2159,121 @@ Boolean render_reset(MufflonInstanceHdl instHdl) {
                 check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
         }
         c.iteration = -1;
+        return true;
+    });
+}
+
+// ---- differentiable rendering (no counterpart in the reference)
+
+Boolean render_set_texture_data(MufflonInstanceHdl instHdl, TextureHdl tex, const float* texels) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        std::lock_guard<std::mutex> lock(c.iterationMutex);
+        CORE_REQUIRE(tex != nullptr && texels != nullptr, "Invalid texture handle or texel pointer (nullptr)");
+        Texture* t = static_cast<Texture*>(tex);
+        bool owned = false;
+        for (const auto& p : c.textures) owned = owned || p.get() == t;
+        CORE_REQUIRE(owned, "texture handle of another instance");
+        std::copy(texels, texels + t->texels.size(), t->texels.begin());
+        if (!c.sceneValid || c.pt < 0) return true;  // uploaded with the next scene load
+        auto it = c.texIds.find(t);
+        const bool inRecord = t->width == 1 && t->height == 1;  // constants inside material records
+        if (inRecord || t == c.envMap) {
+            // material constants and the environment map's flux / sampling
+            // tables are built at load: reload the scene
+            c.invalidate();
+        } else if (it != c.texIds.end()) {
+            check_mfh(mfh_texture_update(c.ctx, it->second, t->texels.data()), "texture update");
+            if (c.renderer == kRendererWire)
+                wire_reset(c);
+            else
+                check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
+        }
+        c.iteration = -1;
+        return true;
+    });
+}
+
+Boolean render_set_emission_scale(MufflonInstanceHdl instHdl, MatIdx slot, Vec3 scale) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        std::lock_guard<std::mutex> lock(c.iterationMutex);
+        CORE_REQUIRE(c.current != nullptr, "No scenario is loaded");
+        CORE_REQUIRE(slot < c.current->assigned.size() && c.current->assigned[slot] != nullptr,
+                     "material slot " + std::to_string(slot) + " has no material assigned");
+        Material& m = *c.current->assigned[slot];
+        CORE_REQUIRE(m.params.innerType == MATERIAL_EMISSIVE, "material '" + m.name + "' is not EMISSIVE");
+        m.params.inner.emissive.scale = scale;
+        m.record.scale[0] = scale.x;
+        m.record.scale[1] = scale.y;
+        m.record.scale[2] = scale.z;
+        if (c.sceneValid && c.pt >= 0) {
+            // every slot of the loaded scenario that holds this material
+            for (size_t i = 0; i < c.current->assigned.size(); ++i)
+                if (c.current->assigned[i] == &m)
+                    check_mfh(mfh_pt_set_material(c.ctx, c.pt, (int32_t)i, &m.record), "material update");
+            if (c.renderer == kRendererWire) wire_reset(c);
+        }
+        c.iteration = -1;
+        return true;
+    });
+}
+
+Boolean render_set_adjoint_mode(MufflonInstanceHdl instHdl, int32_t mode) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        std::lock_guard<std::mutex> lock(c.iterationMutex);
+        CORE_REQUIRE(mode == MFH_ADJOINT_EXPECTED || mode == MFH_ADJOINT_FIXED_DECISIONS, "unknown adjoint mode");
+        c.adjointMode = mode;
+        if (c.sceneValid && c.pt >= 0) check_mfh(mfh_pt_set_adjoint_mode(c.ctx, c.pt, mode), "adjoint mode");
+        return true;
+    });
+}
+
+namespace {
+void require_adjoint_state(const Core& c) {
+    CORE_REQUIRE(c.variation >= 0 && c.renderer == kRendererPt, "the Pathtracer renderer is not enabled");
+    CORE_REQUIRE(c.sceneValid && c.pt >= 0, "no scene is loaded on the device");
+}
+}  // namespace
+
+Boolean render_adjoint(MufflonInstanceHdl instHdl, const float* radianceAdjoint) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        std::lock_guard<std::mutex> lock(c.iterationMutex);
+        require_adjoint_state(c);
+        CORE_REQUIRE(radianceAdjoint != nullptr, "Invalid adjoint image (nullptr)");
+        CORE_REQUIRE(c.iteration >= 0, "no iteration was rendered since the last reset");
+        check_mfh(mfh_pt_adjoint(c.ctx, c.pt, c.iteration + 1, radianceAdjoint), "adjoint");
+        check_mfh(mfh_synchronize(c.ctx), "adjoint");
+        return true;
+    });
+}
+
+Boolean render_get_texture_gradient(MufflonInstanceHdl instHdl, TextureHdl tex, float* gradient) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        std::lock_guard<std::mutex> lock(c.iterationMutex);
+        require_adjoint_state(c);
+        CORE_REQUIRE(tex != nullptr && gradient != nullptr, "Invalid texture handle or output (nullptr)");
+        auto it = c.texIds.find(static_cast<const Texture*>(tex));
+        CORE_REQUIRE(it != c.texIds.end(), "the texture is not a device texture of the loaded scene");
+        check_mfh(mfh_pt_get_texture_gradient(c.ctx, c.pt, it->second, gradient), "texture gradient");
+        return true;
+    });
+}
+
+Boolean render_get_emission_gradient(MufflonInstanceHdl instHdl, MatIdx slot, Vec3* gradient) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Core& c = core_of(instHdl);
+        std::lock_guard<std::mutex> lock(c.iterationMutex);
+        require_adjoint_state(c);
+        CORE_REQUIRE(gradient != nullptr, "Invalid output (nullptr)");
+        float g[3] = {0, 0, 0};
+        check_mfh(mfh_pt_get_emission_gradient(c.ctx, c.pt, (int32_t)slot, g), "emission gradient");
+        gradient->x = g[0];
+        gradient->y = g[1];
+        gradient->z = g[2];
         return true;
     });
 }

@@ -758,8 +758,9 @@ bool JsonScene::material(const Value& m, MaterialParams& p, std::string& unsuppo
         p.inner.torrance.ndf = ndf_of(m);
         p.inner.torrance.roughness = roughness_of(m);
         p.inner.torrance.albedo = albedo_of(m, SAMPLING_LINEAR);
-    } else if (type == "microfacet") {
-        p.innerType = MATERIAL_MICROFACET;
+    } else if (type == "microfacet" || type == "walter") {
+        // the same parameters, loaded as a Walter pack (json_loader.cpp:287-292)
+        p.innerType = type == "walter" ? MATERIAL_WALTER : MATERIAL_MICROFACET;
         p.inner.walter.shadowingModel = shadowing_of(m);
         p.inner.walter.ndf = ndf_of(m);
         p.inner.walter.absorption = as_vec3(get(m, "absorption"), "absorption");
@@ -784,7 +785,7 @@ bool JsonScene::material(const Value& m, MaterialParams& p, std::string& unsuppo
             unsupported = "black-body emissive material";
             return false;
         }
-    } else if (type == "walter" || type == "orennayar" || type == "blend" || type == "fresnel" || type == "glass" ||
+    } else if (type == "orennayar" || type == "blend" || type == "fresnel" || type == "glass" ||
                type == "opaque") {
         unsupported = type + " material";
         return false;

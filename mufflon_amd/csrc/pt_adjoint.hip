@@ -1,6 +1,6 @@
 // Adjoint (path replay) of the path tracer for gfx950: gradients of a scalar
 // loss of the Radiance target with respect to albedo textures (LAMBERT and
-// TORRANCE albedo), roughness textures (TORRANCE, MICROFACET) and the
+// TORRANCE albedo), roughness textures (TORRANCE, WALTER, MICROFACET) and the
 // emission scale of every emissive material.
 //
 // The primal image of `iterations` iterations from a reset is
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
             if (!vx.isVoid) {
                 matIdx = material_index(S, vx.inst, vx.prim);
                 if (vx.mat.type == kMatLambert || vx.mat.type == kMatTorrance) r.tex = S.materials[matIdx].albedoTexture;
-                if (vx.mat.type == kMatTorrance || vx.mat.type == kMatMicrofacet)
+                if (mat_has_roughness(vx.mat.type))
                     r.rtex = S.materials[matIdx].roughnessTexture;
             }
             if (pathLen >= S.minPathLength) {

@@ -15,7 +15,14 @@ constexpr float kDistSqEps = 1e-10f;        // light_sampling.hpp:78
 constexpr int kMatEmissive = 0;  // Materials enum (material_definitions.hpp:25-39)
 constexpr int kMatLambert = 1;
 constexpr int kMatTorrance = 3;
+constexpr int kMatWalter = 4;
 constexpr int kMatMicrofacet = 10;
+// half-vector-based materials with a roughness (texture); materials with an
+// inner medium of their own (Material::compute_medium)
+__host__ __device__ constexpr bool mat_has_roughness(int t) {
+    return t == kMatTorrance || t == kMatWalter || t == kMatMicrofacet;
+}
+__host__ __device__ constexpr bool mat_has_inner_medium(int t) { return t == kMatWalter || t == kMatMicrofacet; }
 constexpr int kNdfGgx = 0;
 constexpr int kNdfBeckmann = 1;
 constexpr int kShadowingVCavity = 0;

@@ -174,21 +174,21 @@ struct MaterialEmission : EmissionLookup {
 void validate_material(const Context& c, const mfh_material& m) {
     const int32_t t = m.type;
     MFH_REQUIRE(t == MFH_MATERIAL_EMISSIVE || t == MFH_MATERIAL_LAMBERT || t == MFH_MATERIAL_TORRANCE ||
-                    t == MFH_MATERIAL_MICROFACET,
+                    t == MFH_MATERIAL_WALTER || t == MFH_MATERIAL_MICROFACET,
                 "unsupported material type " + std::to_string(t));
     auto tex_ok = [&](int32_t id) {
         return id == -1 || (id >= 0 && id < (int32_t)c.textures.size() && c.textures[id] != nullptr);
     };
     MFH_REQUIRE(tex_ok(m.albedoTexture) && tex_ok(m.roughnessTexture), "unknown texture id in a material");
     MFH_REQUIRE(m.emissionTexture == -1, "emission textures are not supported");
-    if (t == MFH_MATERIAL_TORRANCE || t == MFH_MATERIAL_MICROFACET) {
+    if (mat_has_roughness(t)) {
         MFH_REQUIRE(m.ndf == MFH_NDF_GGX || m.ndf == MFH_NDF_BECKMANN, "unknown NDF");
         MFH_REQUIRE(m.shadowing == MFH_SHADOWING_VCAVITY || m.shadowing == MFH_SHADOWING_SMITH,
                     "unknown shadowing model");
         MFH_REQUIRE(!(m.ndf == MFH_NDF_BECKMANN && m.shadowing == MFH_SHADOWING_SMITH),
                     "Smith shadowing with the Beckmann NDF is not supported");
     }
-    if (t == MFH_MATERIAL_MICROFACET) MFH_REQUIRE(m.refractionIndex > 0.0f, "refraction index must be positive");
+    if (mat_has_inner_medium(t)) MFH_REQUIRE(m.refractionIndex > 0.0f, "refraction index must be positive");
     MFH_REQUIRE(m.outerRefractionIndex > 0.0f, "outer refraction index must be positive");
 }
 
@@ -710,7 +710,7 @@ int mfh_pt_adjoint(mfh_context* ctx, int32_t ptId, int32_t iterations, const flo
         };
         for (const PtMaterialRecord& m : t.materials) {
             if (m.type == kMatLambert || m.type == kMatTorrance) want(m.albedoTexture);
-            if (m.type == kMatTorrance || m.type == kMatMicrofacet) want(m.roughnessTexture);
+            if (mat_has_roughness(m.type)) want(m.roughnessTexture);
         }
         for (size_t k = 0; k < c.textures.size(); ++k)
             if (ptrs[k]) MFH_HIP(hipMemsetAsync(ptrs[k], 0, 16 * (size_t)c.textures[k]->width * c.textures[k]->height,

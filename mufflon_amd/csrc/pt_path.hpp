@@ -196,7 +196,7 @@ __device__ __forceinline__ void vertex_targets(const PtScene& S, const Vertex& v
         if (!vx.isVoid) {
             if (vx.mat.type == kMatLambert || vx.mat.type == kMatTorrance)
                 alb = vx.mat.albedo;
-            else if (vx.mat.type == kMatMicrofacet)
+            else if (mat_has_inner_medium(vx.mat.type))
                 alb = v3(1.0f / (1.0f + vx.mat.inner.absorption.x), 1.0f / (1.0f + vx.mat.inner.absorption.y),
                          1.0f / (1.0f + vx.mat.inner.absorption.z));
         }

@@ -1,5 +1,5 @@
 // Device shading core of the path tracer: RNG, texture sampling, material
-// fetch / sample / evaluate (EMISSIVE, LAMBERT, TORRANCE, MICROFACET), media,
+// fetch / sample / evaluate (EMISSIVE, LAMBERT, TORRANCE, WALTER, MICROFACET), media,
 // background and light-tree next-event estimation. Used by the megakernel and
 // the wavefront stages alike, so both produce the same bits.
 //
@@ -12,7 +12,7 @@
 // are evaluated in double and rounded once; float -> u64 conversion maps NaN
 // and negatives to 0 and saturates at 2^64 - 1; mod is the non-negative
 // remainder. Media are carried by value: each material holds its outer medium
-// and (MICROFACET) its inner medium; the camera and point lights sit in vacuum.
+// and (WALTER, MICROFACET) its inner medium; the camera and point lights sit in vacuum.
 #pragma once
 #include "pt_device.hpp"
 
@@ -137,9 +137,9 @@ struct MatParams {
     Medium inner, outer;
     __device__ __forceinline__ const Medium& medium(float xDotN) const { return xDotN < 0.0f ? inner : outer; }
     __device__ __forceinline__ bool reflective() const {
-        return type == kMatLambert || type == kMatTorrance || type == kMatMicrofacet;
+        return type == kMatLambert || type == kMatTorrance || type == kMatWalter || type == kMatMicrofacet;
     }
-    __device__ __forceinline__ bool refractive() const { return type == kMatMicrofacet; }
+    __device__ __forceinline__ bool refractive() const { return mat_has_inner_medium(type); }
 };
 
 __device__ __forceinline__ MatParams fetch(const PtScene& S, int idx, float u, float v) {
@@ -148,7 +148,7 @@ __device__ __forceinline__ MatParams fetch(const PtScene& S, int idx, float u, f
     p.type = m.type;
     p.outer.ior = m.outerIor;
     p.outer.absorption = v3(m.outerAbsorption[0], m.outerAbsorption[1], m.outerAbsorption[2]);
-    if (p.type == kMatMicrofacet) {
+    if (mat_has_inner_medium(p.type)) {
         p.inner.ior = m.ior;
         p.inner.absorption = v3(m.absorption[0], m.absorption[1], m.absorption[2]);
     } else {
@@ -167,7 +167,7 @@ __device__ __forceinline__ MatParams fetch(const PtScene& S, int idx, float u, f
             p.albedo = v3(m.albedo[0], m.albedo[1], m.albedo[2]);
         }
     }
-    if (p.type == kMatTorrance || p.type == kMatMicrofacet) {
+    if (mat_has_roughness(p.type)) {
         if (m.roughnessTexture >= 0) {
             const DevTexture& tx = S.textures[m.roughnessTexture];
             const float4 t = tex_sample(tx, u, v);
@@ -238,7 +238,7 @@ __device__ __forceinline__ float eval_ndf(const MatParams& p, V3 h) {
     return sdiv(e, norm * cz2 * cz2);
 }
 
-// Half vector and its pdf for TORRANCE / MICROFACET sampling:
+// Half vector and its pdf for TORRANCE / WALTER / MICROFACET sampling:
 // Smith visible normals (GGX, microfacet_base.hpp:213-278) or V-cavity
 // (sample_ndf :312-366 + sample_visible_normal_vcavity :137-156).
 __device__ V3 sample_half(const MatParams& p, V3 in, float u0, float u1, u64& rnd, float& cavityPdf, float& iDotH) {
@@ -341,7 +341,7 @@ __device__ bool sample_ts(const MatParams& p, V3 in, float u0, float u1, u64 rnd
         pdfF = cosT / kPi;
         return true;
     }
-    if (p.type != kMatTorrance && p.type != kMatMicrofacet) return false;
+    if (!mat_has_roughness(p.type)) return false;
     float cavityPdf, iDotH;
     const V3 h = sample_half(p, in, u0, u1, rnd, cavityPdf, iDotH);
     if (p.type == kMatTorrance) {
@@ -360,6 +360,33 @@ __device__ bool sample_ts(const MatParams& p, V3 in, float u0, float u1, u64 rnd
     const float n_t = other.ior;
     const float eta = n_i / n_t;
     const float iDotHabs = fabsf(iDotH);
+    if (p.type == kMatWalter) {  // microfacet_refractive.hpp:26-121: refraction, reflection only when total
+        const float t = 1.0f - eta * eta * (1.0f - iDotHabs * iDotHabs);
+        float eDotH, eDotHabs;
+        if (t <= 0.0f) {
+            ex = smul(2.0f * iDotH, h) - in;
+            eDotH = iDotH;
+            eDotHabs = iDotHabs;
+        } else {
+            eDotHabs = fsqrt(t);
+            eDotH = eDotHabs * -sgnf(iDotH);
+            ex = smul(sgnf(iDotH) * (eta * iDotHabs - eDotHabs), h) - smul(eta, in);
+        }
+        const float ge = g_term(p, eDotH, ex, h.z);
+        const float gi = g_term(p, iDotH, in, h.z);
+        if (ge == 0.0f || gi == 0.0f) return false;
+        float tp;
+        if (t <= 0.0f) {
+            tp = sdiv(g_reflection(p.shadowing, gi, ge), gi);
+            pdfF = cavityPdf * sdiv(gi, fabsf(4.0f * in.z * h.z));
+        } else {
+            tp = sdiv(g_transmission(p.shadowing, gi, ge), gi);
+            const float common = cavityPdf * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_t * eDotH) * h.z);
+            pdfF = common * sdiv(gi * n_t * n_t, fabsf(in.z));
+        }
+        throughput = v3(tp, tp, tp);
+        return true;
+    }
     float cosTAbs;
     const float f = fresnel_dielectric(n_i, n_t, iDotHabs, cosTAbs);
     const bool reflect = ((float)rnd / (float)(~0ull - 1)) < f;
@@ -397,7 +424,7 @@ __device__ V3 evaluate_ts(const MatParams& p, V3 in, V3 ex, float& pdfF) {
         pdfF = fabsf(ex.z) / kPi;
         return p.albedo / kPi;
     }
-    if (p.type != kMatTorrance && p.type != kMatMicrofacet) return v3(0, 0, 0);
+    if (!mat_has_roughness(p.type)) return v3(0, 0, 0);
     const bool isReflection = in.z * ex.z > 0.0f;
     if (p.type == kMatTorrance && in.z * ex.z < 0.0f) return v3(0, 0, 0);
     const Medium& incident = p.medium(in.z);
@@ -416,10 +443,24 @@ __device__ V3 evaluate_ts(const MatParams& p, V3 in, V3 ex, float& pdfF) {
     const float eDotH = dot(ex, h);
     const float n_i = incident.ior;
     const float n_t = other.ior;
+    // Walter (microfacet_refractive.hpp:124-187): a reflection only under total internal reflection
+    if (p.type == kMatWalter && isReflection && sq(n_i / n_t) * (1.0f - iDotH * iDotH) < 1.0f) return v3(0, 0, 0);
     const float ge = g_term(p, eDotH, ex, h.z);
     const float gi = g_term(p, iDotH, in, h.z);
     if (ge == 0.0f || gi == 0.0f) return v3(0, 0, 0);
     const float d = eval_ndf(p, h);
+    if (p.type == kMatWalter) {
+        float val;
+        if (isReflection) {
+            val = sdiv(g_reflection(p.shadowing, gi, ge) * d, 4.0f * in.z * ex.z);
+            pdfF = sdiv(gi * d, 4.0f * fabsf(in.z));
+        } else {
+            const float common = sdiv(fabsf(d * iDotH * eDotH), sq(n_i * iDotH + n_t * eDotH));
+            val = g_transmission(p.shadowing, gi, ge) * common * sdiv(n_t * n_t, fabsf(in.z * ex.z));
+            pdfF = gi * common * sdiv(n_t * n_t, fabsf(in.z));
+        }
+        return v3(val, val, val);
+    }
     float cosTAbs;
     const float f = fresnel_dielectric(n_i, n_t, fabsf(iDotH), cosTAbs);
     float val;
@@ -482,7 +523,7 @@ __device__ __forceinline__ V3 mat_evaluate(const TS& ts, const MatParams& m, V3 
 
 // ------------------------------------------------------------------ roughness derivatives
 // Logarithmic derivatives with respect to the roughness (rx, ry) for the
-// adjoint. The BSDF value of TORRANCE / MICROFACET is G * D * (terms free of
+// adjoint. The BSDF value of TORRANCE / WALTER / MICROFACET is G * D * (terms free of
 // roughness) and its pdf is G1(in) * D * (terms free of roughness)
 // (evaluate_ts), so d ln value = d ln G + d ln D and d ln pdf = d ln G1(in)
 // + d ln D. The V-cavity G does not depend on the roughness; Smith's is a
@@ -522,7 +563,7 @@ __device__ __forceinline__ void dln_g1(const MatParams& p, V3 w, float& dx, floa
 __device__ __forceinline__ bool dln_mat_evaluate(const TS& ts, const MatParams& m, V3 incident, V3 excident,
                                                  float dv[2], float dp[2]) {
     dv[0] = dv[1] = dp[0] = dp[1] = 0.0f;
-    if (m.type != kMatTorrance && m.type != kMatMicrofacet) return false;
+    if (!mat_has_roughness(m.type)) return false;
     if (!(m.rx > 0.0f && m.ry > 0.0f)) return false;
     float cosOut, pdfF;
     const V3 value = mat_evaluate(ts, m, incident, excident, cosOut, pdfF);

@@ -183,8 +183,8 @@ def _material_json(m, texture_file):
     elif kind == S.MAT_TORRANCE:
         out.update(type="torrance", albedo=albedo(), roughness=roughness(), ndf=_NDF_NAMES[int(m["ndf"])],
                    shadowingModel=_SHADOWING_NAMES[int(m["shadowing"])])
-    elif kind == S.MAT_MICROFACET:
-        out.update(type="microfacet", roughness=roughness(), ndf=_NDF_NAMES[int(m["ndf"])],
+    elif kind in (S.MAT_WALTER, S.MAT_MICROFACET):
+        out.update(type="walter" if kind == S.MAT_WALTER else "microfacet", roughness=roughness(), ndf=_NDF_NAMES[int(m["ndf"])],
                    shadowingModel=_SHADOWING_NAMES[int(m["shadowing"])], ior=_f(m["refractionIndex"]),
                    absorption=[_f(x) for x in m["absorption"]])
     elif kind == S.MAT_EMISSIVE:

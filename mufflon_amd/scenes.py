@@ -122,6 +122,7 @@ IDENTITY34 = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], np.float32)
 MAT_EMISSIVE = 0  # Mufflon's Materials enum (material_definitions.hpp:25-39)
 MAT_LAMBERT = 1
 MAT_TORRANCE = 3
+MAT_WALTER = 4
 MAT_MICROFACET = 10
 NDF_GGX = 0
 NDF_BECKMANN = 1
@@ -281,10 +282,11 @@ def cornell_materials(sphere_rings=32, sphere_segments=64, width=64, height=64):
     """Cornell box with every supported material kind: textured Lambert back
     wall (texture 0), a GGX V-cavity Torrance sphere with a roughness texture
     (texture 1), an anisotropic GGX Smith Torrance box, an absorbing
-    dielectric MICROFACET box (ior 1.5), a Beckmann Torrance sphere, the
+    dielectric MICROFACET box (ior 1.5), a Beckmann Torrance sphere, a
+    rough-refracting WALTER sphere (GGX Smith, ior 1.33, absorbing), the
     ceiling light, and two point lights. Textures are returned as
     (texels, sampling) in id order; the environment map is texture 2."""
-    WHITE, RED, GREEN, LIGHT, BACK, ROUGH_SPHERE, ANISO, GLASS, BECK = 1, 2, 3, 0, 4, 5, 6, 7, 8
+    WHITE, RED, GREEN, LIGHT, BACK, ROUGH_SPHERE, ANISO, GLASS, BECK, WALTER = 1, 2, 3, 0, 4, 5, 6, 7, 8, 9
     mats = np.concatenate([
         material(MAT_EMISSIVE, radiance=(17.0, 12.0, 4.0), scale=(1.0, 1.0, 1.0)),
         material(MAT_LAMBERT, albedo=(0.73, 0.73, 0.73)),
@@ -295,6 +297,7 @@ def cornell_materials(sphere_rings=32, sphere_segments=64, width=64, height=64):
         material(MAT_TORRANCE, albedo=(0.7, 0.75, 0.8), roughness=(0.08, 0.4), shadowing=SHADOWING_SMITH),
         material(MAT_MICROFACET, roughness=0.15, ior=1.5, absorption=(0.4, 0.1, 0.05)),
         material(MAT_TORRANCE, albedo=(0.95, 0.6, 0.3), roughness=0.3, ndf=NDF_BECKMANN),
+        material(MAT_WALTER, roughness=0.12, ior=1.33, absorption=(0.3, 0.1, 0.05), shadowing=SHADOWING_SMITH),
     ])
     mb = MeshBuilder()
     mb.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0), WHITE)
@@ -306,6 +309,7 @@ def cornell_materials(sphere_rings=32, sphere_segments=64, width=64, height=64):
     mb.add_box((0.45, 0.0, -0.40), (0.50, 0.5, 0.50), -17.0, GLASS)
     mb.add_uv_sphere((0.05, 0.32, 0.45), 0.32, sphere_rings, sphere_segments, ROUGH_SPHERE)
     mb.add_uv_sphere((-0.6, 1.45, 0.2), 0.18, sphere_rings // 2, sphere_segments // 2, BECK)
+    mb.add_uv_sphere((0.55, 1.05, 0.55), 0.2, sphere_rings // 2, sphere_segments // 2, WALTER)
     L = 0.25
     mb.add_quad((-L, 1.99, -L), (L, 1.99, -L), (L, 1.99, L), (-L, 1.99, L), (0, -1, 0), LIGHT)
     v, n, uv, t, m = mb.arrays()

@@ -37,17 +37,21 @@ def path_tracer(ctx, scene, seed=1, max_path_length=16, nee_count=1, background_
     return pt, sc, tex
 
 
-def uneven_batch(ctx, count=48, seed0=100, use_graph=True, scenes=None):
+def uneven_batch(ctx, count=48, seed0=100, use_graph=True, scenes=None, with_scene_ids=False):
     """The uneven batch: `count` procedural scenes (scenes.batch_scene) as path
     tracers and one batch over them. Returns (batch id, path tracer ids,
-    texture ids per scene, scene dicts)."""
+    texture ids per scene, scene dicts), plus the scene ids with
+    `with_scene_ids`."""
     from mufflon_amd import scenes as S
     if scenes is None:
         scenes = [S.batch_scene(i, seed0) for i in range(count)]
-    pts, texs = [], []
+    pts, texs, sids = [], [], []
     for sc in scenes:
-        pt, _, tex = path_tracer(ctx, sc, max_path_length=sc["max_path_length"])
+        pt, sid, tex = path_tracer(ctx, sc, max_path_length=sc["max_path_length"])
         pts.append(pt)
         texs.append(tex)
+        sids.append(sid)
     batch = ctx.batch_create(pts, use_graph)
+    if with_scene_ids:
+        return batch, pts, texs, scenes, sids
     return batch, pts, texs, scenes

@@ -17,6 +17,8 @@
 // reference's GCC branch.
 #include "oracle_math.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -372,6 +374,201 @@ void build_lbvh(const Set& set, const Box& bb, BuildOut& o, u64* unsortedKeys) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// OpenMP-parallel variant of the same stages: the reference builds LoDs of
+// 1000 or more primitives with its parallel path (lod.cpp:26, the OpenMP /
+// thrust::par branches of build_lbvh.cu:658-760). Used as the multi-core CPU
+// baseline; its output equals build_lbvh's bit for bit:
+// * Morton codes, Karras nodes, collapse marks and the collapsed copy are per
+//   element and independent (parallel for);
+// * the sort is a stable LSD radix sort of (key, id) in 8 passes of 8 bits
+//   (per-thread digit counts over contiguous chunks, scattered in thread
+//   order), which gives std::stable_sort's order;
+// * the refit climbs with one atomic arrival flag per node; the second
+//   arriver merges the node in the serial path's operand order (right child
+//   as the climbing box, left child as the other) and with the serial path's
+//   counts: the right child's carried (unsaturated) count plus the left
+//   child's stored (saturated) count, kept per node in a side array.
+template <class Set, class C>
+void build_lbvh_omp(const Set& set, const Box& bb, BuildOut& o) {
+    const i32 n = set.n;
+    if (n == 1) {
+        build_lbvh<Set, C>(set, bb, o, nullptr);
+        return;
+    }
+    const i32 nInt = n - 1;
+    const i32 nNodes = nInt + n;
+    std::vector<u64> keys(n), keys2(n);
+    std::vector<i32> ids(n), ids2(n);
+#pragma omp parallel for schedule(static)
+    for (i32 i = 0; i < n; ++i) {
+        keys[i] = set.masked(i) ? ~0ull : morton_of_point(set.centroid(i), bb);
+        ids[i] = i;
+    }
+    // stable LSD radix sort
+    const int T = std::max(1, omp_get_max_threads());
+    std::vector<size_t> cnt((size_t)T * 256);
+    for (int shift = 0; shift < 64; shift += 8) {
+#pragma omp parallel num_threads(T)
+        {
+            const int t = omp_get_thread_num();
+            const int nt = omp_get_num_threads();
+            const int64_t lo = (int64_t)n * t / nt, hi = (int64_t)n * (t + 1) / nt;
+            size_t* c = cnt.data() + (size_t)t * 256;
+            std::fill(c, c + 256, (size_t)0);
+            for (int64_t i = lo; i < hi; ++i) ++c[(keys[i] >> shift) & 255u];
+#pragma omp barrier
+#pragma omp single
+            {
+                size_t run = 0;
+                for (int d = 0; d < 256; ++d)
+                    for (int u = 0; u < nt; ++u) {
+                        const size_t v = cnt[(size_t)u * 256 + d];
+                        cnt[(size_t)u * 256 + d] = run;
+                        run += v;
+                    }
+            }
+            for (int64_t i = lo; i < hi; ++i) {
+                const size_t dst = c[(keys[i] >> shift) & 255u]++;
+                keys2[dst] = keys[i];
+                ids2[dst] = ids[i];
+            }
+        }
+        keys.swap(keys2);
+        ids.swap(ids2);
+    }
+    o.keys = keys;
+    o.ids = ids;
+    o.parents.assign(nNodes, 0);
+#pragma omp parallel for schedule(static)
+    for (i32 i = 0; i < nInt; ++i) karras_node(n, o.keys.data(), o.parents.data(), i);
+    // explicit children of every internal node
+    std::vector<i32> left(nInt), right(nInt);
+#pragma omp parallel for schedule(static)
+    for (i32 c = 1; c < nNodes; ++c) {
+        const i32 p = o.parents[c];
+        if (p < 0)
+            left[~p] = c;
+        else
+            right[p] = c;
+    }
+    // refit
+    o.boxes.assign((size_t)nNodes * 8, 0.0f);
+    std::vector<typename C::Count> carried(nNodes);
+    std::vector<i32> arrived(nInt, 0);
+    float* B = o.boxes.data();
+#pragma omp parallel for schedule(static)
+    for (i32 idx = 0; idx < n; ++idx) {
+        const i32 prim = o.ids[idx];
+        const Box lb0 = set.box(prim);
+        const typename C::Count c0 = C::leaf(prim);
+        const i32 leaf = idx + nInt;
+        float* lb = B + (size_t)leaf * 8;
+        lb[0] = lb0.lo.x; lb[1] = lb0.lo.y; lb[2] = lb0.lo.z; lb[3] = i2f(C::encode(c0));
+        lb[4] = lb0.hi.x; lb[5] = lb0.hi.y; lb[6] = lb0.hi.z; lb[7] = C::cost(c0);
+        carried[leaf] = c0;
+        i32 node = o.parents[leaf];
+        if (node < 0) node = ~node;
+        for (;;) {
+            if (__atomic_exchange_n(&arrived[node], 1, __ATOMIC_ACQ_REL) == 0) break;  // first arriver
+            const float* rb = B + (size_t)right[node] * 8;
+            const float* ob = B + (size_t)left[node] * 8;
+            const Box rBox{{rb[0], rb[1], rb[2]}, {rb[4], rb[5], rb[6]}};
+            const Box oBox{{ob[0], ob[1], ob[2]}, {ob[4], ob[5], ob[6]}};
+            Box cur = rBox;
+            cur.lo = vmin(cur.lo, oBox.lo);
+            cur.hi = vmax(cur.hi, oBox.hi);
+            const float curSA = surface(rBox);
+            const float newSA = surface(cur);
+            const float otherSA = surface(oBox);
+            const typename C::Count rc = carried[right[node]];
+            const typename C::Count oc = C::extract(f2i(ob[3]));
+            const float cost = C::Ct + curSA / newSA * C::cost(rc) + otherSA / newSA * C::cost(oc);
+            const typename C::Count sum = C::add(rc, oc);
+            carried[node] = sum;
+            float* nb = B + (size_t)node * 8;
+            nb[0] = cur.lo.x; nb[1] = cur.lo.y; nb[2] = cur.lo.z; nb[3] = i2f(C::encode(sum));
+            nb[4] = cur.hi.x; nb[5] = cur.hi.y; nb[6] = cur.hi.z; nb[7] = cost;
+            if (node == 0) break;
+            node = o.parents[node];
+            if (node < 0) node = ~node;
+        }
+    }
+    // collapse marks, one leaf walk per thread (every write stores 0)
+    std::vector<i32> marks(nInt, 1);
+#pragma omp parallel for schedule(static)
+    for (i32 idx = 0; idx < n; ++idx) {
+        const i32 leaf = idx + nInt;
+        i32 collapse = -1;
+        i32 node = o.parents[leaf];
+        if (node < 0) node = ~node;
+        while (node != 0) {
+            const typename C::Count c = C::extract(f2i(B[(size_t)node * 8 + 3]));
+            if (C::capped(c)) break;
+            if (C::cost(c) < B[(size_t)node * 8 + 7]) collapse = node;
+            node = o.parents[node];
+            if (node < 0) node = ~node;
+        }
+        if (collapse != -1) {
+            i32 m = o.parents[leaf];
+            if (m < 0) m = ~m;
+            for (;;) {
+                __atomic_store_n(&marks[m], 0, __ATOMIC_RELAXED);
+                if (m == collapse) break;
+                m = o.parents[m];
+                if (m < 0) m = ~m;
+            }
+        }
+    }
+    o.offsets.resize(nInt);
+    i32 acc = 0;
+    for (i32 i = 0; i < nInt; ++i) {
+        acc += marks[i];
+        o.offsets[i] = acc;
+    }
+    const i32 nAfter = o.offsets[nInt - 1];
+    o.numInternalAfter = nAfter;
+    o.nodes.assign((size_t)nAfter * 64, 0);
+    auto collapsed = [&](i32 node) {
+        return node > 0 && node < nInt && (o.offsets[node] - o.offsets[node - 1]) == 0;
+    };
+#pragma omp parallel for schedule(static)
+    for (i32 node = 1; node < nNodes; ++node) {
+        if (collapsed(node)) continue;
+        i32 parent = o.parents[node];
+        i32 off = parent < 0 ? 0 : 1;
+        if (parent < 0) parent = ~parent;
+        i32 fin = node;
+        bool skip = false;
+        if (fin >= nInt) {
+            while (collapsed(parent)) {
+                if (off == 1) {
+                    skip = true;
+                    break;
+                }
+                fin = parent;
+                parent = o.parents[parent];
+                off = parent < 0 ? 0 : 1;
+                if (parent < 0) parent = ~parent;
+            }
+        }
+        if (skip) continue;
+        const float* fb = B + (size_t)fin * 8;
+        const i32 primCount = C::sum(C::extract(f2i(fb[3])));
+        const i32 outIdx = (o.offsets[parent] - 1) * 2;
+        const i32 outNode = (node >= nInt) ? node - nInt + nAfter : o.offsets[node] - 1;
+        const V3 lo{fb[0], fb[1], fb[2]}, hi{fb[4], fb[5], fb[6]};
+        const V3 c = (lo + hi) * 0.5f;
+        const V3 bl = (lo - c) * 1.0001f + c;
+        const V3 bh = (hi - c) * 1.0001f + c;
+        uint8_t* dst = o.nodes.data() + (size_t)(outIdx + off) * 32;
+        const float f[6] = {bl.x, bl.y, bl.z, bh.x, bh.y, bh.z};
+        std::memcpy(dst, f, 24);
+        std::memcpy(dst + 24, &outNode, 4);
+        std::memcpy(dst + 28, &primCount, 4);
+    }
+}
+
 void copy_out(const BuildOut& o, i32 n, u64* sortedKeys, i32* ids, i32* parents, float* boxes,
               i32* offsets, void* nodes, i32* numInternalAfter) {
     if (sortedKeys) std::memcpy(sortedKeys, o.keys.data(), 8 * (size_t)n);
@@ -413,6 +610,20 @@ int orc_lbvh_build_tris(const float* verts, const uint32_t* idx, int32_t numTris
     const Box bb{{bbox[0], bbox[1], bbox[2]}, {bbox[3], bbox[4], bbox[5]}};
     BuildOut o;
     build_lbvh<TriSet, LodCost>(set, bb, o, unsortedKeys);
+    copy_out(o, numTris, sortedKeys, ids, parents, boxes, offsets, nodes, numInternalAfter);
+    return 0;
+}
+
+// The same build with the OpenMP-parallel stages (build_lbvh_omp); equal
+// outputs. Runs on omp_get_max_threads() threads.
+int orc_lbvh_build_tris_omp(const float* verts, const uint32_t* idx, int32_t numTris, const float bbox[6],
+                            uint64_t* sortedKeys, int32_t* ids, int32_t* parents, float* boxes, int32_t* offsets,
+                            void* nodes, int32_t* numInternalAfter) {
+    if (numTris <= 0) return 1;
+    TriSet set{verts, idx, numTris};
+    const Box bb{{bbox[0], bbox[1], bbox[2]}, {bbox[3], bbox[4], bbox[5]}};
+    BuildOut o;
+    build_lbvh_omp<TriSet, LodCost>(set, bb, o);
     copy_out(o, numTris, sortedKeys, ids, parents, boxes, offsets, nodes, numInternalAfter);
     return 0;
 }

@@ -114,6 +114,32 @@ def lbvh_build_tris(verts, tris, bbox=None):
                 nodes=nodes[: nint.value * 64].copy(), num_internal=nint.value, bbox=bb)
 
 
+def lbvh_build_tris_omp(verts, tris, bbox=None):
+    """The OpenMP-parallel LBVH build (the reference's parallel CPU path for
+    LoDs of 1000+ primitives); returns the same dict as lbvh_build_tris
+    without the unsorted keys."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    n = len(t)
+    bb = vertex_box(v) if bbox is None else np.ascontiguousarray(bbox, np.float32)
+    sk = np.zeros(n, np.uint64)
+    ids = np.zeros(n, np.int32)
+    par = np.zeros(max(2 * n - 1, 1), np.int32)
+    boxes = np.zeros(max(2 * n - 1, 1) * 8, np.float32)
+    offs = np.zeros(max(n - 1, 1), np.int32)
+    nodes = np.zeros(max(n - 1, 1) * 64, np.uint8)
+    nint = C.c_int32()
+    lib.orc_lbvh_build_tris_omp.argtypes = [f32p, u32p, C.c_int32, f32p, u64p, i32p, i32p, f32p, i32p, C.c_void_p,
+                                            C.POINTER(C.c_int32)]
+    rc = lib.orc_lbvh_build_tris_omp(_p(v, f32p), _p(t, u32p), n, _p(bb, f32p), _p(sk, u64p), _p(ids, i32p),
+                                     _p(par, i32p), _p(boxes, f32p), _p(offs, i32p), nodes.ctypes.data,
+                                     C.byref(nint))
+    assert rc == 0
+    return dict(sorted_keys=sk, sorted_ids=ids, parents=par, boxes=boxes, offsets=offs,
+                nodes=nodes[: nint.value * 64].copy(), num_internal=nint.value, bbox=bb)
+
+
 def instance_boxes(world_to_inst, obj_boxes, inst_lod):
     lib = load()
     w = np.ascontiguousarray(world_to_inst, np.float32).reshape(-1, 12)

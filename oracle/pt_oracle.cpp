@@ -219,7 +219,7 @@ struct OrcPtScene {
 
 namespace {
 
-enum Mat : int32_t { kEmissive = 0, kLambert = 1, kTorrance = 3, kMicrofacet = 10 };
+enum Mat : int32_t { kEmissive = 0, kLambert = 1, kTorrance = 3, kWalter = 4, kMicrofacet = 10 };
 enum Ndf : int32_t { kGgx = 0, kBeckmann = 1 };
 enum Shadowing : int32_t { kVCavity = 0, kSmith = 1 };
 
@@ -248,9 +248,9 @@ struct MatParams {
     Medium inner = kVacuum, outer = kVacuum;
     // MaterialDescriptorBase::get_medium (material_definitions.hpp:53-56)
     const Medium& medium(float xDotN) const { return xDotN < 0.0f ? inner : outer; }
-    bool reflective() const { return type == kLambert || type == kTorrance || type == kMicrofacet; }
-    bool refractive() const { return type == kMicrofacet; }
-    bool halfvector() const { return type == kTorrance || type == kMicrofacet; }
+    bool reflective() const { return type == kLambert || type == kTorrance || type == kWalter || type == kMicrofacet; }
+    bool refractive() const { return type == kWalter || type == kMicrofacet; }
+    bool halfvector() const { return type == kTorrance || type == kWalter || type == kMicrofacet; }
 };
 
 // ---------------------------------------------------------------- textures
@@ -298,7 +298,11 @@ MatParams fetch(const OrcPtScene& S, const OrcMaterial& m, float u, float v) {
     MatParams p;
     p.type = m.type;
     p.outer = {m.outerIor, {m.outerAbsorption[0], m.outerAbsorption[1], m.outerAbsorption[2]}};
-    p.inner = m.type == kMicrofacet ? Medium{m.ior, {m.absorption[0], m.absorption[1], m.absorption[2]}} : p.outer;
+    // Material::compute_medium: WALTER and MICROFACET carry their inner medium
+    // (material_definitions.hpp:226-228, 269-271)
+    p.inner = (m.type == kWalter || m.type == kMicrofacet)
+                  ? Medium{m.ior, {m.absorption[0], m.absorption[1], m.absorption[2]}}
+                  : p.outer;
     if (m.type == kLambert || m.type == kTorrance) {
         if (m.albedoTexture >= 0) {
             const F4 t = tex_sample(S.textures[m.albedoTexture], u, v);
@@ -307,7 +311,7 @@ MatParams fetch(const OrcPtScene& S, const OrcMaterial& m, float u, float v) {
             p.albedo = {m.albedo[0], m.albedo[1], m.albedo[2]};
         }
     }
-    if (m.type == kTorrance || m.type == kMicrofacet) {
+    if (m.type == kTorrance || m.type == kWalter || m.type == kMicrofacet) {
         if (m.roughnessTexture >= 0) {
             const OrcTexture& tx = S.textures[m.roughnessTexture];
             const F4 t = tex_sample(tx, u, v);
@@ -584,6 +588,55 @@ TsSample microfacet_sample(const MatParams& p, V3 in, const Medium& incident, co
     return r;
 }
 
+// WALTER sample (microfacet_refractive.hpp:26-121): the refracted direction
+// about the sampled half vector, or its mirror under total internal
+// reflection; no Fresnel choice (the view-path throughput has no eta^2).
+TsSample walter_sample(const MatParams& p, V3 in, const Medium& incident, const Medium& other, float u0, float u1,
+                       u64 i0) {
+    TsSample r;
+    u64 rnd = i0;
+    float cavityPdf, iDotH;
+    const V3 h = sample_half(p, in, u0, u1, rnd, cavityPdf, iDotH);
+    const float n_i = incident.ior;
+    const float n_e = other.ior;
+    const float eta = n_i / n_e;
+    const float etaSq = eta * eta;
+    const float iDotHabs = std::fabs(iDotH);
+    const float t = 1.0f - etaSq * (1.0f - iDotHabs * iDotHabs);  // Snell: cos^2 of the refracted angle
+    const bool total = t <= 0.0f;
+    V3 ex;
+    float eDotH, eDotHabs;
+    if (total) {
+        ex = (2.0f * iDotH) * h - in;
+        eDotH = iDotH;
+        eDotHabs = iDotHabs;
+    } else {
+        eDotHabs = std::sqrt(t);
+        eDotH = eDotHabs * -sgnf(iDotH);
+        ex = (sgnf(iDotH) * (eta * iDotHabs - eDotHabs)) * h - eta * in;
+    }
+    const float ge = g_term(p, eDotH, ex, h.z);
+    const float gi = g_term(p, iDotH, in, h.z);
+    if (ge == 0.0f || gi == 0.0f) return r;
+    float throughput, pdfF, pdfB;
+    if (total) {
+        throughput = sdiv(g_reflection(p.shadowing, gi, ge), gi);
+        pdfF = cavityPdf * sdiv(gi, std::fabs(4.0f * in.z * h.z));
+        pdfB = cavityPdf * sdiv(ge, std::fabs(4.0f * ex.z * h.z));
+    } else {
+        throughput = sdiv(g_transmission(p.shadowing, gi, ge), gi);
+        const float common = cavityPdf * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_e * eDotH) * h.z);
+        pdfF = common * sdiv(gi * n_e * n_e, std::fabs(in.z));
+        pdfB = common * sdiv(ge * n_i * n_i, std::fabs(ex.z));
+    }
+    r.throughput = {throughput, throughput, throughput};
+    r.excident = ex;
+    r.pdfF = pdfF;
+    r.pdfB = pdfB;
+    r.valid = true;
+    return r;
+}
+
 // Lambert sample (lambert.hpp:18-35), sample_dir_cosine (sampling.hpp:95-101)
 TsSample lambert_sample(const MatParams& p, V3 in, float u0, float u1) {
     TsSample r;
@@ -605,6 +658,7 @@ TsSample sample_ts(const MatParams& p, V3 inTS, float u0, float u1, u64 i0) {
     switch (p.type) {
         case kLambert: return lambert_sample(p, inTS, u0, u1);
         case kTorrance: return torrance_sample(p, inTS, u0, u1, i0);
+        case kWalter: return walter_sample(p, inTS, incident, other, u0, u1, i0);
         case kMicrofacet: return microfacet_sample(p, inTS, incident, other, u0, u1, i0);
         default: return TsSample{};  // EMISSIVE: not sampleable
     }
@@ -638,6 +692,33 @@ TsEval evaluate_ts(const MatParams& p, V3 in, V3 ex) {
         r.value = p.albedo * sdiv(g * d, std::fabs(4.0f * in.z * ex.z));
         r.pdfF = sdiv(gi * d, std::fabs(4.0f * in.z));
         r.pdfB = sdiv(ge * d, std::fabs(4.0f * ex.z));
+        return r;
+    }
+    if (p.type == kWalter) {  // microfacet_refractive.hpp:124-187
+        const bool isReflection = in.z * ex.z > 0.0f;
+        const V3 h = boundary_half(incident, other, in, ex);
+        const float iDotH = dot(in, h);
+        const float eDotH = dot(ex, h);
+        const float n_i = incident.ior;
+        const float n_e = other.ior;
+        // a reflection exists only under total internal reflection
+        if (isReflection && sq(n_i / n_e) * (1.0f - iDotH * iDotH) < 1.0f) return r;
+        const float ge = g_term(p, eDotH, ex, h.z);
+        const float gi = g_term(p, iDotH, in, h.z);
+        if (ge == 0.0f || gi == 0.0f) return r;
+        const float d = eval_ndf(p, h);
+        if (isReflection) {
+            const float v = sdiv(g_reflection(p.shadowing, gi, ge) * d, 4.0f * in.z * ex.z);
+            r.value = {v, v, v};
+            r.pdfF = sdiv(gi * d, 4.0f * std::fabs(in.z));
+            r.pdfB = sdiv(ge * d, 4.0f * std::fabs(ex.z));
+            return r;
+        }
+        const float common = sdiv(std::fabs(d * iDotH * eDotH), sq(n_i * iDotH + n_e * eDotH));
+        const float bsdf = g_transmission(p.shadowing, gi, ge) * common * sdiv(n_e * n_e, std::fabs(in.z * ex.z));
+        r.value = {bsdf, bsdf, bsdf};
+        r.pdfF = gi * common * sdiv(n_e * n_e, std::fabs(in.z));
+        r.pdfB = ge * common * sdiv(n_i * n_i, std::fabs(ex.z));
         return r;
     }
     if (p.type == kMicrofacet) {  // microfacet_full.hpp:129-187
@@ -768,7 +849,7 @@ void dln_g1_d(const MatParams& p, V3 w, double& dx, double& dy) {
 }
 bool dln_mat_evaluate_d(const TS& ts, const MatParams& m, V3 incident, V3 excident, double dv[2], double dp[2]) {
     dv[0] = dv[1] = dp[0] = dp[1] = 0.0;
-    if (m.type != kTorrance && m.type != kMicrofacet) return false;
+    if (!m.halfvector()) return false;
     if (!(m.rx > 0.0f && m.ry > 0.0f)) return false;
     const EvalValue e = mat_evaluate(ts, m, incident, excident);
     if (!nonzero(e.value)) return false;
@@ -1324,7 +1405,7 @@ struct PixelAcc {
 // materials::albedo (lambert, microfacet_specular, microfacet_full, emissive)
 V3 mat_albedo(const MatParams& m) {
     if (m.type == kLambert || m.type == kTorrance) return m.albedo;
-    if (m.type == kMicrofacet)
+    if (m.type == kWalter || m.type == kMicrofacet)  // microfacet_refractive.hpp:190-194
         return {1.0f / (1.0f + m.inner.absorption.x), 1.0f / (1.0f + m.inner.absorption.y),
                 1.0f / (1.0f + m.inner.absorption.z)};
     return {0, 0, 0};
@@ -1690,8 +1771,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             r.hv = hit.v;
             r.rho = vtx.mat.albedo;
             if (vtx.mat.type == kLambert || vtx.mat.type == kTorrance) r.tex = S.materials[matIdx].albedoTexture;
-            if (vtx.mat.type == kTorrance || vtx.mat.type == kMicrofacet)
-                r.rtex = S.materials[matIdx].roughnessTexture;
+            if (vtx.mat.halfvector()) r.rtex = S.materials[matIdx].roughnessTexture;
         }
         if (pathLen >= minPL) {
             V3 emission{0, 0, 0};

@@ -16,15 +16,16 @@ def random_transforms(n, seed, scale=(0.5, 2.0), spread=4.0):
     return w
 
 
-def oracle_lod(O, verts, tris):
-    b = O.lbvh_build_tris(verts, tris)
+def oracle_lod(O, verts, tris, parallel=False):
+    """Oracle LoD record; `parallel` builds with the OpenMP stages (same output)."""
+    b = (O.lbvh_build_tris_omp if parallel else O.lbvh_build_tris)(verts, tris)
     return dict(verts=verts, tris=tris, nodes=b["nodes"], num_internal=b["num_internal"],
                 prim_ids=b["sorted_ids"], box=b["bbox"]), b
 
 
-def oracle_scene(O, lod_geoms, inst_lod, w):
+def oracle_scene(O, lod_geoms, inst_lod, w, parallel=False):
     """Build LoD BVHs and (for >1 instance) the instance BVH with the oracle."""
-    lods = [oracle_lod(O, v, t)[0] for v, t in lod_geoms]
+    lods = [oracle_lod(O, v, t, parallel)[0] for v, t in lod_geoms]
     tlas = None
     if len(inst_lod) > 1:
         boxes = np.stack([L["box"] for L in lods])

@@ -111,10 +111,15 @@ def test_finalize_sanity(mi):
 
 def test_unsupported_features_fail_cleanly(mi):
     p = K.MaterialParams()
+    p.innerType = K.MATERIAL_ORENNAYAR
+    assert not mi.world_add_material("o", p)
+    assert "not supported" in mi.core_get_dll_error()
+    # WALTER is supported (rough refraction, the MICROFACET parameters)
+    p = K.MaterialParams()
     p.innerType = K.MATERIAL_WALTER
     p.inner.walter.roughness = mi.world_add_texture_value([0.1])
-    assert not mi.world_add_material("w", p)
-    assert "not supported" in mi.core_get_dll_error()
+    p.inner.walter.refractionIndex = 1.5
+    assert mi.world_add_material("w", p)
     p = K.MaterialParams()
     p.innerType = 42
     assert not mi.world_add_material("x", p)
@@ -195,3 +200,27 @@ def test_load_without_device_fails_cleanly(mi, tmp_path):
     scen = K.build_world(mi, c, str(tmp_path))
     assert not mi.world_load_scenario(scen)
     assert "device" in mi.core_get_dll_error()
+
+
+def test_adjoint_calls_without_a_loaded_scene(mi, tmp_path):
+    """The differentiable-rendering additions refuse, with a message, what
+    they cannot do without a loaded scene (no device is touched)."""
+    delta = np.zeros((4, 4, 3), np.float32)
+    with pytest.raises(RuntimeError, match="not enabled"):
+        mi.render_adjoint(delta)
+    with pytest.raises(RuntimeError, match="scenario"):
+        mi.render_set_emission_scale(0, (1.0, 1.0, 1.0))
+    with pytest.raises(RuntimeError, match="adjoint mode"):
+        mi.render_set_adjoint_mode(7)
+    mi.render_set_adjoint_mode(1)
+    # a texture's texels can be replaced before any scene is loaded
+    c = S.cornell_textured(6, 12, 16, 8, 8)
+    K.build_world(mi, c, str(tmp_path), name="adj")
+    tex = mi.world_get_texture(os.path.join(str(tmp_path), "adj_tex1.pfm"))
+    assert tex
+    mi.render_set_texture_data(tex, np.full((8, 8, 1), 0.5, np.float32))
+    assert mi.render_enable_renderer(0, 0)
+    with pytest.raises(RuntimeError, match="device"):
+        mi.render_get_texture_gradient(tex, (8, 8, 1))
+    with pytest.raises(RuntimeError, match="device"):
+        mi.render_get_emission_gradient(0)

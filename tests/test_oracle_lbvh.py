@@ -201,3 +201,27 @@ def test_instance_level_build(oracle):
     leaves = LC.walk_collapsed(b["nodes"], b["num_internal"], n)
     assert sum(c for _, c, _, _ in leaves) == n
     assert (sb[:3] <= ib[:, :3].min(axis=0)).all() and (sb[3:] >= ib[:, 3:].max(axis=0)).all()
+
+
+@pytest.mark.parametrize("n,seed,mu", [(2, 1, -5.0), (1500, 2, -5.0), (60000, 3, -3.0), (200000, 4, -6.0)])
+def test_parallel_build_equals_serial(oracle, n, seed, mu):
+    """The OpenMP-parallel build (the CPU baseline of the reference's parallel
+    path) returns the serial build's stages bit for bit, SAH costs included,
+    also where counts saturate (LoDs of 200k triangles have nodes over 1023)."""
+    v, t = S.random_triangle_soup(n, seed=seed, mu=mu)
+    a = oracle.lbvh_build_tris(v, t)
+    b = oracle.lbvh_build_tris_omp(v, t)
+    assert a["num_internal"] == b["num_internal"]
+    for k in ("sorted_keys", "sorted_ids", "parents", "offsets", "nodes"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["boxes"].view(np.uint32), b["boxes"].view(np.uint32))
+
+
+def test_parallel_build_duplicate_keys(oracle):
+    v = np.tile(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), (4000, 1))
+    t = np.arange(12000, dtype=np.uint32).reshape(-1, 3)
+    a = oracle.lbvh_build_tris(v, t)
+    b = oracle.lbvh_build_tris_omp(v, t)
+    for k in ("sorted_keys", "sorted_ids", "parents", "offsets", "nodes"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["boxes"].view(np.uint32), b["boxes"].view(np.uint32))

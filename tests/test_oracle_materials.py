@@ -68,6 +68,9 @@ MATS = {
                                      shadowing=S.SHADOWING_SMITH),
     "microfacet_ggx": S.material(S.MAT_MICROFACET, roughness=0.2, ior=1.5),
     "microfacet_ggx_smith": S.material(S.MAT_MICROFACET, roughness=0.3, ior=1.5, shadowing=S.SHADOWING_SMITH),
+    "walter_ggx": S.material(S.MAT_WALTER, roughness=0.2, ior=1.5),
+    "walter_ggx_smith": S.material(S.MAT_WALTER, roughness=0.3, ior=1.33, shadowing=S.SHADOWING_SMITH),
+    "walter_beckmann": S.material(S.MAT_WALTER, roughness=0.25, ior=1.5, ndf=S.NDF_BECKMANN),
 }
 
 
@@ -102,7 +105,7 @@ def test_sample_matches_evaluate(oracle, name):
 
 
 @pytest.mark.parametrize("name", ["lambert", "torrance_ggx_vcavity", "torrance_ggx_smith", "torrance_beckmann_vcavity",
-                                  "microfacet_ggx"])
+                                  "microfacet_ggx", "walter_ggx", "walter_ggx_smith"])
 def test_sampled_distribution_integrates_the_bsdf(oracle, name):
     """E[throughput] over the sampler = integral of f |cos| over the sphere."""
     m = MATS[name]
@@ -121,6 +124,91 @@ def test_sampled_distribution_integrates_the_bsdf(oracle, name):
     f = np.array([oracle.bsdf_evaluate(m, inc, e)[0][0] for e in d.astype(np.float32)])
     quad = float(np.sum(f * np.abs(d[:, 2]) * w))
     assert mc[0] == pytest.approx(quad, rel=0.03, abs=2e-3)
+
+
+def _snell(inc_ts, eta):
+    """Refracted direction of an incidentTS (the direction towards where the
+    path came from, z > 0) about the normal (0, 0, 1), n_i / n_t = eta;
+    None beyond the critical angle."""
+    c = inc_ts[2]
+    k = 1.0 - eta * eta * (1.0 - c * c)
+    if k < 0:
+        return None
+    return -eta * inc_ts + (eta * c - np.sqrt(k)) * np.array([0, 0, 1.0])
+
+
+@pytest.mark.parametrize("ndf", [S.NDF_GGX, S.NDF_BECKMANN])
+def test_walter_smooth_limit_is_snell_refraction(oracle, ndf):
+    """At vanishing roughness a WALTER sample is Snell's refraction of the
+    incident direction about the normal (the half vector is the normal),
+    with throughput 1: no Fresnel term, no reflection below the critical
+    angle."""
+    m = S.material(S.MAT_WALTER, roughness=1e-4, ior=1.5, ndf=ndf)
+    rng = np.random.default_rng(8)
+    for k in range(200):
+        th = rng.uniform(0.0, 1.45)
+        inc = _incident(th, rng.uniform(0, 2 * np.pi))
+        u0, u1 = (float(x) for x in rng.random(2).astype(np.float32))
+        valid, tp, ex, pf, _ = oracle.bsdf_sample(m, inc, u0, u1, int(rng.integers(0, 2 ** 63)))
+        assert valid
+        expect = _snell(inc.astype(np.float64), 1.0 / 1.5)
+        np.testing.assert_allclose(ex, expect, atol=2e-3)
+        np.testing.assert_allclose(tp, 1.0, atol=2e-3)
+
+
+def test_walter_total_internal_reflection(oracle):
+    """From inside (below, n_i = 1.5 to n_t = 1) beyond the critical angle
+    asin(1 / 1.5) every smooth sample is the mirror direction; evaluating a
+    reflection below the critical angle gives 0, and the transmitted
+    direction of a sample from inside evaluates to the sample's pdf."""
+    m = S.material(S.MAT_WALTER, roughness=1e-4, ior=1.5)
+    crit = np.arcsin(1 / 1.5)
+    rng = np.random.default_rng(12)
+    for k in range(100):
+        th = rng.uniform(crit + 0.05, 1.45)
+        phi = rng.uniform(0, 2 * np.pi)
+        inc = _incident(th, phi, below=True)
+        valid, tp, ex, pf, _ = oracle.bsdf_sample(m, inc, *(float(x) for x in rng.random(2).astype(np.float32)),
+                                                  int(rng.integers(0, 2 ** 63)))
+        assert valid
+        mirror = np.array([-inc[0], -inc[1], inc[2]], np.float64)  # the mirror stays on the incident's side
+        np.testing.assert_allclose(ex, mirror, atol=2e-3)
+        np.testing.assert_allclose(tp, 1.0, atol=2e-3)
+    # no reflection outside total internal reflection (rough, from above)
+    m = S.material(S.MAT_WALTER, roughness=0.3, ior=1.5)
+    inc = _incident(0.4, 0.2)
+    refl = np.array([-inc[0], -inc[1], inc[2]], np.float32)
+    val, pf, pb = oracle.bsdf_evaluate(m, inc, refl)
+    assert not np.any(val) and pf == 0.0
+
+
+def test_walter_transmits_what_microfacet_does_not_reflect(oracle):
+    """For the same roughness and ior, the MICROFACET material's transmitted
+    lobe is WALTER's lobe weighted by 1 - Fresnel of the half vector: on any
+    refracted pair their values differ by exactly that factor."""
+    w = S.material(S.MAT_WALTER, roughness=0.3, ior=1.5)
+    mf = S.material(S.MAT_MICROFACET, roughness=0.3, ior=1.5)
+    rng = np.random.default_rng(14)
+    checked = 0
+    for k in range(300):
+        inc = _incident(rng.uniform(0.05, 1.3), rng.uniform(0, 2 * np.pi))
+        valid, tp, ex, pf, _ = oracle.bsdf_sample(w, inc, *(float(x) for x in rng.random(2).astype(np.float32)),
+                                                  int(rng.integers(0, 2 ** 63)))
+        if not valid or ex[2] > 0:
+            continue
+        vw, pw, _ = oracle.bsdf_evaluate(w, inc, ex)
+        vm, pm, _ = oracle.bsdf_evaluate(mf, inc, ex)
+        if vw[0] <= 0:
+            continue
+        # half vector of the refracted pair (n_i = 1, n_t = 1.5), facing the incident side
+        i = inc.astype(np.float64)
+        h = 1.0 * i + 1.5 * ex.astype(np.float64)
+        h /= np.linalg.norm(h) * np.sign(h[2])
+        f, _ = oracle.fresnel_dielectric(1.0, 1.5, abs(float(i @ h)))
+        assert vm[0] == pytest.approx((1 - f) * vw[0], rel=2e-4)
+        assert pm == pytest.approx((1 - f) * pw, rel=2e-4)
+        checked += 1
+    assert checked > 100
 
 
 def test_torrance_reciprocity(oracle):
