@@ -75,6 +75,28 @@ int mfh_set_traversal_leaf_limit(mfh_context* ctx, int32_t limit);
  * Replaces LBVHBuilder::acquire_const (lbvh.hpp:70-86). */
 int mfh_lbvh_get(mfh_context* ctx, int32_t lodId, int32_t* outNumInternal, void* outNodes, int32_t* outPrimIds);
 
+/* Largest LoD of mfh_lbvh_build_noalloc, in triangles. */
+#define MFH_NOALLOC_MAX_TRIANGLES 262144
+
+/* LBVH build that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The same
+ * build as mfh_lbvh_build + mfh_lbvh_get over a triangle LoD held in the
+ * caller's device memory: dVertices (numVertices float xyz), dTriIndices
+ * (numTriangles x 3 uint32), bbox = {min xyz, max xyz} on the host
+ * (required). Writes the collapsed BVH into dNodes (room for
+ * 64 * (numTriangles - 1) bytes; may be NULL for one triangle), the primitive
+ * ids in BVH order into dPrimIds (numTriangles int32) and the number of
+ * internal nodes into *outNumInternal; returns after the build has finished.
+ * Its scratch is one static device array per GPU (44.3 MiB, held until the
+ * process ends), so at most MFH_NOALLOC_MAX_TRIANGLES triangles; larger LoDs
+ * and out-of-range vertex indices are rejected with
+ * MFH_ERROR_INVALID_ARGUMENT. Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other host threads, other contexts) race. */
+int mfh_lbvh_build_noalloc(mfh_context* ctx, const float* dVertices, int32_t numVertices,
+                           const uint32_t* dTriIndices, int32_t numTriangles, const float* bbox,
+                           int32_t* outNumInternal, void* dNodes, int32_t* dPrimIds);
+
 /* Build a LoD and copy every intermediate stage to the host (NULL skips one):
  * Morton codes in primitive order and sorted, sorted ids (N each), parents
  * (2N-1, Mufflon's encoding: left child stores ~parent), node boxes

@@ -92,6 +92,7 @@ int mfh_create(int32_t deviceIndex, mfh_context** outCtx) {
             MFH_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
             MFH_HIP(hipEventCreate(&ctx->c.evBegin));
             MFH_HIP(hipEventCreate(&ctx->c.evEnd));
+            noalloc_scratch_init(deviceIndex);
         } catch (...) {
             delete ctx;
             throw;
@@ -250,6 +251,17 @@ int mfh_lbvh_get(mfh_context* ctx, int32_t lodId, int32_t* outNumInternal, void*
         if (outNodes && l.numInternal > 0)
             MFH_HIP(hipMemcpy(outNodes, l.nodes.ptr, 64 * (size_t)l.numInternal, hipMemcpyDeviceToHost));
         if (outPrimIds) MFH_HIP(hipMemcpy(outPrimIds, l.primIds.ptr, 4 * (size_t)l.numTris, hipMemcpyDeviceToHost));
+    });
+}
+
+int mfh_lbvh_build_noalloc(mfh_context* ctx, const float* dVertices, int32_t numVertices,
+                           const uint32_t* dTriIndices, int32_t numTriangles, const float* bbox,
+                           int32_t* outNumInternal, void* dNodes, int32_t* dPrimIds) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(outNumInternal != nullptr, "null output pointer");
+        *outNumInternal = build_lod_bvh_noalloc(c, dVertices, numVertices, dTriIndices, numTriangles, bbox, dNodes,
+                                                dPrimIds);
     });
 }
 

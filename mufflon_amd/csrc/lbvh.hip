@@ -27,10 +27,14 @@
 // reference mark, because the counts only grow towards the root.
 #include "mfh_device.hpp"
 #include "mfh_internal.hpp"
+#include "mufflon_hip.h"
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <type_traits>
+#include <vector>
 
 namespace mfh {
 
@@ -642,6 +646,126 @@ void build_scene_bvh(Context& ctx, Scene& sc, StageSink* sink) {
     }
     sc.tlasNumInternal = build_generic<InstPrims, InstCost>(ctx, ctx.scratch, set, n, sc.box, sc.tlasNodes,
                                                             sc.tlasPrimIds, nullptr, sink);
+}
+
+// ------------------------------------------------- build without allocation
+// The LoD build over caller-owned device buffers with its scratch in one
+// static device array per GPU (no hipMalloc / hipFree anywhere on the path).
+// Every build on a GPU shares that GPU's array and leaves its contents behind;
+// what must start at zero (arrival counters, queue counts, the index check
+// flag) is cleared by fill kernels inside the build. Builds that overlap on
+// one GPU (two host threads, or two contexts whose streams run at once) race.
+
+namespace {
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of every scratch buffer of a build over n primitives, in carve order.
+constexpr int kNoAllocParts = 18;
+struct NoAllocSizes {
+    size_t b[kNoAllocParts];
+};
+constexpr NoAllocSizes noalloc_sizes(long long n) {
+    const long long n1 = n > 1 ? n - 1 : 1;
+    const long long tiles = (n + 4095) / 4096;                          // radix sort tiles of 4096 keys
+    const long long scanned = n > 256 * tiles ? n : 256 * tiles;        // longest scan: marks or digit counts
+    const long long scanBlocks = (scanned + 2047) / 2048;               // scan chunks of 2048 values
+    return NoAllocSizes{{(size_t)(8 * n), (size_t)(8 * n), (size_t)(4 * n), (size_t)(4 * n),  // keys, ids A/B
+                         (size_t)(4 * (2 * n - 1)),                                            // parents
+                         (size_t)(8 * n1), (size_t)(8 * n1),                                   // children, ranges
+                         (size_t)(4 * n + 4), (size_t)(4 * n + 4),                             // pending lists
+                         (size_t)(32 * (2 * n - 1)),                                           // node boxes
+                         (size_t)(4 * n1), (size_t)n1, (size_t)(4 * n1),                       // counters, flags, offsets
+                         (size_t)(4 * 256 * tiles), (size_t)(4 * scanBlocks),                  // digit counts, partials
+                         (size_t)(48 * n),                                                     // BVH-order triangles
+                         (size_t)16, (size_t)256}};                                            // index check, spare
+}
+constexpr size_t noalloc_total(long long n) {
+    size_t t = 0;
+    const NoAllocSizes s = noalloc_sizes(n);
+    for (int i = 0; i < kNoAllocParts; ++i) t += align256(s.b[i]);
+    return t;
+}
+
+constexpr int kNoAllocMaxTris = MFH_NOALLOC_MAX_TRIANGLES;
+constexpr size_t kNoAllocBytes = noalloc_total(kNoAllocMaxTris);
+static_assert(kNoAllocBytes < ((size_t)64 << 20), "scratch of the allocation-free build above 64 MiB");
+
+// The scratch of the allocation-free build (one copy per GPU, held until the
+// process ends). Host code never passes this name to a kernel: it uses the
+// per-device address that hipGetSymbolAddress returned for it.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex gScratchMutex;
+std::vector<void*> gScratchPtr;  // per device index; nullptr until looked up
+
+// 1 in *flag if any of the 3n vertex indices is numVerts or more.
+__global__ __launch_bounds__(kThreads) void k_check_indices(const u32* __restrict__ tris, long long count,
+                                                           u32 numVerts, u32* flag) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count && tris[i] >= numVerts) *flag = 1u;
+}
+
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (device >= (int)gScratchPtr.size()) gScratchPtr.resize(device + 1, nullptr);
+    if (gScratchPtr[device]) return;
+    void* p = nullptr;
+    MFH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MFH_REQUIRE(p != nullptr, "no device address for the build scratch");
+    gScratchPtr[device] = p;
+}
+
+size_t noalloc_scratch_bytes() { return kNoAllocBytes; }
+
+int build_lod_bvh_noalloc(Context& ctx, const float* dVerts, int numVerts, const uint32_t* dTris, int n,
+                          const float* bbox, void* dNodes, int32_t* dPrimIds) {
+    MFH_REQUIRE(n >= 1, "LoD has no triangles");
+    MFH_REQUIRE(n <= kNoAllocMaxTris, "allocation-free build holds at most " + std::to_string(kNoAllocMaxTris) +
+                                          " triangles, got " + std::to_string(n));
+    MFH_REQUIRE(numVerts >= 1, "LoD has no vertices");
+    MFH_REQUIRE(dVerts && dTris && bbox && dPrimIds && (dNodes || n == 1), "null pointer");
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(gScratchMutex);
+        if (ctx.device < (int)gScratchPtr.size()) base = gScratchPtr[ctx.device];
+    }
+    MFH_REQUIRE(base != nullptr, "build scratch of this device not set up (mfh_create does it)");
+    // Carve the per-device array in the order and sizes of noalloc_sizes.
+    const NoAllocSizes sz = noalloc_sizes(n);
+    DevBuf part[kNoAllocParts];
+    size_t off = 0;
+    for (int i = 0; i < kNoAllocParts; ++i) {
+        part[i] = DevBuf::view(static_cast<unsigned char*>(base) + off, sz.b[i]);
+        off += align256(sz.b[i]);
+    }
+    MFH_REQUIRE(off <= kNoAllocBytes, "build scratch layout exceeds the array");
+    BuildScratch s;
+    s.keysA = part[0]; s.keysB = part[1]; s.idsA = part[2]; s.idsB = part[3];
+    s.parents = part[4]; s.children = part[5]; s.ranges = part[6];
+    s.pending = part[7]; s.pending2 = part[8]; s.boxes = part[9];
+    s.counters = part[10]; s.flags = part[11]; s.offsets = part[12];
+    s.hist = part[13]; s.partials = part[14];
+    float4* triVerts = part[15].as<float4>();
+    u32* badIndex = part[16].as<u32>();
+
+    hipStream_t st = ctx.stream;
+    fill_u32(badIndex, 1, 0u, st);
+    const long long count = 3ll * n;
+    hipLaunchKernelGGL(k_check_indices, dim3((unsigned)((count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                       dTris, count, (u32)numVerts, badIndex);
+    MFH_HIP(hipGetLastError());
+    u32 bad = 0;
+    MFH_HIP(hipMemcpyAsync(&bad, badIndex, 4, hipMemcpyDeviceToHost, st));
+    MFH_HIP(hipStreamSynchronize(st));
+    MFH_REQUIRE(bad == 0, "triangle index out of range");
+
+    TriPrims set{dVerts, dTris};
+    DevBuf nodes = DevBuf::view(dNodes, 64 * (size_t)std::max(n - 1, 1));
+    DevBuf primIds = DevBuf::view(dPrimIds, 4 * (size_t)n);
+    return build_generic<TriPrims, LodCost>(ctx, s, set, n, bbox, nodes, primIds, triVerts, nullptr);
 }
 
 

@@ -181,6 +181,16 @@ void upload(Context& c, DevBuf& b, const void* src, size_t bytes);
 void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink);
 // LBVH over the instances of a scene (instance-level hierarchy).
 void build_scene_bvh(Context& ctx, Scene& sc, StageSink* sink);
+// Device address of the allocation-free build's static scratch array on the
+// current device (looked up once per device; the device must be current).
+void noalloc_scratch_init(int device);
+size_t noalloc_scratch_bytes();
+// LBVH of a triangle LoD in caller-owned device memory (dVerts: float xyz,
+// dTris: u32 x3), scratch in the static array, result into dNodes (64 bytes
+// per internal node, room for n - 1) and dPrimIds (n). Returns the number of
+// internal nodes of the collapsed BVH. Allocates nothing.
+int build_lod_bvh_noalloc(Context& ctx, const float* dVerts, int numVerts, const uint32_t* dTris, int n,
+                          const float* bbox, void* dNodes, int32_t* dPrimIds);
 
 // Stable LSD radix sort of (u64 key, i32 value) pairs; result ends in keysA/idsA.
 void radix_sort_pairs(uint64_t* keysA, uint64_t* keysB, int32_t* idsA, int32_t* idsB, int n, int beginBit,

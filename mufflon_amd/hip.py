@@ -70,6 +70,7 @@ _SIGS = {
     "mfh_lbvh_build": [vp, i32, f32p],
     "mfh_lbvh_get": [vp, i32, i32p, vp, i32p],
     "mfh_lbvh_build_stages": [vp, i32, f32p, u64p, u64p, i32p, i32p, f32p, i32p],
+    "mfh_lbvh_build_noalloc": [vp, vp, i32, vp, i32, f32p, i32p, vp, vp],
     "mfh_scene_create": [vp, i32p, f32p, i32, i32p],
     "mfh_scene_get": [vp, i32, f32p, i32p, vp, i32p],
     "mfh_trace_closest": [vp, i32, vp, i32, vp, vp],
@@ -313,6 +314,15 @@ class HipContext:
         self._call("mfh_lbvh_get", self.handle, lod, C.byref(n), nodes.ctypes.data if len(nodes) else None,
                    _ptr(ids, i32p))
         return n.value, nodes, ids
+
+    def lbvh_build_noalloc(self, d_verts, num_verts, d_tris, num_tris, bbox, d_nodes, d_prim_ids):
+        """LBVH of a LoD in device buffers without any device allocation
+        (mfh_lbvh_build_noalloc); returns the number of internal nodes."""
+        b = np.ascontiguousarray(bbox, dtype=np.float32)
+        n = i32()
+        self._call("mfh_lbvh_build_noalloc", self.handle, d_verts, int(num_verts), d_tris, int(num_tris),
+                   _ptr(b, f32p), C.byref(n), d_nodes, d_prim_ids)
+        return n.value
 
     def lbvh_build_stages(self, lod, num_prims, bbox=None):
         n = num_prims
