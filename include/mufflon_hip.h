@@ -337,12 +337,17 @@ typedef struct mfh_wavefront_tuning {
                              every n traversal steps once a quarter of the wave is idle */
     int32_t leafBatch;    /* 1 or 8: triangles of a BVH leaf loaded ahead of their tests (long
                              leaves); also used by the megakernel form; the refill kernels use 1 */
+    int32_t tailAfter;    /* 0: one launch per stage and bounce; k > 0: after the extension rays of
+                             bounce k one launch finishes every remaining path (megakernel-style,
+                             same bits), so the last bounces of a few long paths do not cost three
+                             launches each */
 } mfh_wavefront_tuning;
 int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning* tuning);
 int mfh_pt_get_tuning(mfh_context* ctx, int32_t ptId, mfh_wavefront_tuning* outTuning);
 /* Times `iterations` wavefront iterations for every tuning candidate (block x
  * stack depth x leaf batch 1/8 x {chunk 256, chunk 1024, refill 16, refill 32}
- * x sorted bounces 0/1/2), keeps the fastest and resets the path tracer. */
+ * x sorted bounces 0/1/2), then the tail bounce (never, 1, 2, 3, 4, 6) on the
+ * fastest of those, keeps the fastest and resets the path tracer. */
 int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wavefront_tuning* outBest,
                     float* outMsPerIteration);
 /* Batches: several path tracers (scenes of any size, resolution, path

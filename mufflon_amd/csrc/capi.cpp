@@ -2,6 +2,8 @@
 // Every entry point converts exceptions into a status code plus a
 // thread-local message, the convention of Mufflon's core API
 // (interface.cpp:54-83, core_get_dll_error).
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -58,6 +60,7 @@ void refresh_scene(Context& c, Scene& s) {
         lods[i].numInternal = l->numInternal;
         lods[i].numPrims = l->numTris;
         std::memcpy(lods[i].box, l->box, sizeof(l->box));
+        lods[i].coordMax = l->coordMax;
         ref[i] = lods[i];
         if (l->travNumInternal >= 0) {
             lods[i].nodes = l->travNodes.ptr;
@@ -198,6 +201,7 @@ int mfh_lod_create(mfh_context* ctx, const float* vertices, int32_t numVertices,
             upload(c, l->verts, vertices, 12 * (size_t)numVertices);
             upload(c, l->tris, triIndices, 12 * (size_t)numTriangles);
             vertex_box(vertices, numVertices, l->box);
+            for (int k = 0; k < 6; ++k) l->coordMax = std::max(l->coordMax, std::fabs(l->box[k]));
             MFH_HIP(hipStreamSynchronize(c.stream));
         } catch (...) {
             l->verts.release();

@@ -113,6 +113,7 @@ struct Lod {
     bool hasShading = false;
     DevBuf normals, uvs, matIdx;
     float box[6] = {0, 0, 0, 0, 0, 0};
+    float coordMax = 0.0f;    // largest |coordinate| of the vertices (bounds every node box of its BVHs)
     bool built = false;
     uint64_t generation = 0;  // incremented by every build
     int32_t numInternal = 0;  // internal nodes of the collapsed BVH
@@ -171,7 +172,8 @@ struct DevLod {
     int32_t numInternal;
     int32_t numPrims;
     float box[6];
-    int32_t pad[2];
+    float coordMax;  // largest |coordinate| of the vertices
+    int32_t pad;
 };
 
 void destroy_tracers(Context& ctx);
@@ -208,6 +210,7 @@ struct TraceParams {
     const void* tlasNodes;
     const int32_t* tlasPrimIds;
     int32_t tlasNumInternal;
+    float tlasCoordMax;  // largest |coordinate| of the scene box (bounds the instance-level node boxes)
 };
 // Traversal over the LoDs' traversal hierarchies, or (reference = true) over
 // the reference BVHs, whose visit counts follow the reference's order.

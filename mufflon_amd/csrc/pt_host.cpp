@@ -418,6 +418,7 @@ WfState wavefront_state(Context& c, PathTracer& t) {
         t.tune.traceGrid = 0;
         t.tune.shadeGrid = wf_shade_grid(cus);
     }
+    if (t.tune.tailGrid <= 0) t.tune.tailGrid = wf_tail_grid(num_cus(c));
     return W;
 }
 
@@ -871,6 +872,7 @@ WfState batch_state(Context& c, PtBatch& b, std::vector<PtScene>& scenes) {
     W.numScenes = (int)b.tracers.size();
     if (b.tune.traceGrid <= 0) b.tune.traceGrid = wf_grid_batch(b.tune, num_cus(c));
     if (b.tune.shadeGrid <= 0) b.tune.shadeGrid = wf_shade_grid(num_cus(c));
+    if (b.tune.tailGrid <= 0) b.tune.tailGrid = wf_tail_grid(num_cus(c));
     return W;
 }
 
@@ -981,7 +983,11 @@ void tuning_record(const WfTuning& tune, mfh_wavefront_tuning* out) {
     out->sortBounces = tune.sortBounces;
     out->refillBudget = tune.refill;
     out->leafBatch = kWfTraceShapes[tune.traceVariant].leafBatch;
+    out->tailAfter = tune.tailAfter;
 }
+
+// Bounces after which the tail launch finishes the paths (0: never), tried by the autotuners.
+constexpr int kTailChoices[] = {0, 1, 2, 3, 4, 6};
 
 }  // namespace
 }  // namespace mfh
@@ -997,9 +1003,11 @@ int mfh_batch_set_tuning(mfh_context* ctx, int32_t batchId, const mfh_wavefront_
                     "queue chunk must be a multiple of 64 in [64, 65536]");
         MFH_REQUIRE(tuning->sortBounces == 0, "batches do not sort rays");
         MFH_REQUIRE(tuning->refillBudget >= 0 && tuning->refillBudget <= 4096, "refillBudget out of range");
+        MFH_REQUIRE(tuning->tailAfter >= 0 && tuning->tailAfter <= 64, "tailAfter out of range");
         b.tune.traceVariant = v;
         b.tune.chunk = tuning->queueChunk;
         b.tune.refill = tuning->refillBudget;
+        b.tune.tailAfter = tuning->tailAfter;
         b.tune.traceGrid = 0;
         drop_batch_graph(b);
     });
@@ -1033,6 +1041,24 @@ int mfh_batch_autotune(mfh_context* ctx, int32_t batchId, int32_t iterations, mf
                     best = ms;
                     bestTune = b.tune;
                 }
+            }
+        }
+        // then the bounce of the tail launch, on the best trace shape
+        for (int tail : kTailChoices) {
+            b.tune = bestTune;
+            b.tune.tailAfter = tail;
+            drop_batch_graph(b);
+            run_batch(c, b, 1);
+            MFH_HIP(hipEventRecord(c.evBegin, c.stream));
+            run_batch(c, b, iterations);
+            MFH_HIP(hipEventRecord(c.evEnd, c.stream));
+            MFH_HIP(hipEventSynchronize(c.evEnd));
+            float ms = 0.0f;
+            MFH_HIP(hipEventElapsedTime(&ms, c.evBegin, c.evEnd));
+            ms /= (float)iterations;
+            if (ms < best) {
+                best = ms;
+                bestTune = b.tune;
             }
         }
         b.tune = bestTune;
@@ -1070,10 +1096,12 @@ int mfh_pt_set_tuning(mfh_context* ctx, int32_t ptId, const mfh_wavefront_tuning
         MFH_HIP(hipStreamSynchronize(c.stream));
         MFH_REQUIRE(tuning->sortBounces >= 0 && tuning->sortBounces <= 64, "sortBounces out of range");
         MFH_REQUIRE(tuning->refillBudget >= 0 && tuning->refillBudget <= 4096, "refillBudget out of range");
+        MFH_REQUIRE(tuning->tailAfter >= 0 && tuning->tailAfter <= 64, "tailAfter out of range");
         t.tune.traceVariant = v;
         t.tune.chunk = tuning->queueChunk;
         t.tune.sortBounces = tuning->sortBounces;
         t.tune.refill = tuning->refillBudget;
+        t.tune.tailAfter = tuning->tailAfter;
         t.tune.traceGrid = 0;
         drop_graph(t);
     });
@@ -1122,6 +1150,24 @@ int mfh_pt_autotune(mfh_context* ctx, int32_t ptId, int32_t iterations, mfh_wave
                     best = ms;
                     bestTune = t.tune;
                 }
+            }
+        }
+        // then the bounce of the tail launch, on the best trace shape
+        for (int tail : kTailChoices) {
+            t.tune = bestTune;
+            t.tune.tailAfter = tail;
+            drop_graph(t);
+            run_wavefront(c, t, S, 1);
+            MFH_HIP(hipEventRecord(c.evBegin, c.stream));
+            run_wavefront(c, t, S, iterations);
+            MFH_HIP(hipEventRecord(c.evEnd, c.stream));
+            MFH_HIP(hipEventSynchronize(c.evEnd));
+            float ms = 0.0f;
+            MFH_HIP(hipEventElapsedTime(&ms, c.evBegin, c.evEnd));
+            ms /= (float)iterations;
+            if (ms < best) {
+                best = ms;
+                bestTune = t.tune;
             }
         }
         t.tune = bestTune;

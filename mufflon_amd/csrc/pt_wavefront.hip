@@ -502,6 +502,87 @@ __global__ __launch_bounds__(256) void k_wf_shade(PtScene S0, WfState W, int k, 
     }
 }
 
+// The rest of every path of queue `buf` (rays of bounce k traced to their
+// hits) in one launch: a lane takes a path and runs the megakernel's loop
+// from the vertex of that hit to the end of the path, tracing its next-event
+// rays and extensions itself. Few but long paths (glass, deep bounces) then
+// cost one launch instead of three per bounce, each waiting for its slowest
+// ray. Random numbers are drawn, and contributions reach the accumulator, in
+// the order of the staged kernels and of the megakernel (emission of a vertex,
+// its next-event estimates, the next vertex), so the bits are the same.
+template <int kMode, int kLeaf>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_wf_tail(PtScene S0, WfState W, int k,
+                                                                                       int buf, int count, int work) {
+    __shared__ int sStack[kLdsDepth * 256];
+    constexpr bool kBatch = kMode == 2;
+    const int n = W.counters[count];
+    int* const lstack = sStack + threadIdx.x;
+    for (QueueWalk wk(n, W.chunk); wk.base < n; wk.next(W.counters + work)) {
+        const int end = min(wk.base + wk.chunk, n);
+        for (int q = wk.base + lane_id(); q < end; q += 64) {
+            const float4 o = W.rayO[buf][q];
+            const float4 d = W.rayD[buf][q];
+            const float4 m = W.rayM[buf][q];
+            const float4 h = W.hit[buf][q];
+            const int p = __float_as_int(d.w);
+            int local;
+            const int si = locate<kBatch>(W, p, local);
+            const PtScene& S = scene_ref<kBatch>(S0, W, si);
+            u64 rng = S.rngs[local];
+            const float4 t4 = W.tp[p];
+            V3 throughput = v3(t4.x, t4.y, t4.z);
+            float footA = t4.w;
+            Acc acc;
+            load_acc(W, S.targetMask, p, acc);
+            V3 prevPos = v3(0.0f, 0.0f, 0.0f);
+            if (W.rayP[buf]) {
+                const float4 pp = W.rayP[buf][q];
+                prevPos = v3(pp.x, pp.y, pp.z);
+            }
+            V3 rayOrigin = v3(o.x, o.y, o.z), dir = v3(d.x, d.y, d.z), absorption = v3(m.x, m.y, m.z);
+            float pdfF = o.w;
+            TraceHit hit;
+            hit.t = h.x;
+            hit.inst = __float_as_int(h.y);
+            hit.prim = __float_as_int(h.z);
+            hit.w0 = h.w;
+            hit.w1 = m.w;
+            int pathLen = k;
+            Vertex vx;
+            while (true) {
+                throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
+                footA += 1.0f / fsqrt(pdfF);
+                const float guideWeight = 1.f / (1.f + footA * footA / 1.0f);
+                make_vertex(S, rayOrigin, dir, pdfF, hit.t, hit.inst, hit.prim, hit.w0, hit.w1, vx);
+                ++pathLen;
+                vertex_targets(S, vx, pathLen, throughput, guideWeight, acc, prevPos);
+                if (vx.isVoid || pathLen >= S.maxPathLength) break;
+                if (pathLen + 1 >= S.minPathLength && pathLen + 1 <= S.maxPathLength) {
+                    const u64 neeSeed = pcg_next(rng);
+                    for (int i = 0; i < S.neeCount; ++i) {
+                        const u64 x = pcg_next(rng);
+                        NeeCandidate c;
+                        if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
+                            const TraceHit sh =
+                                trace_mode<kMode, true, 256, kLdsDepth, kLeaf>(S.trace, c.a, c.dir, c.tmax, lstack);
+                            if (sh.inst == -1) add_nee(c, acc);
+                        }
+                    }
+                }
+                const u64 x0 = pcg_next(rng);
+                const u64 x1 = pcg_next(rng);
+                const float rndRoulette = sample_uniform((u32)pcg_next(rng));
+                if (!walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, dir, pdfF, absorption)) break;
+                prevPos = vx.position;
+                hit = trace_mode<kMode, false, 256, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
+            }
+            S.rngs[local] = rng;
+            store_acc(W, S.targetMask, p, acc);
+            if (acc.nans) atomicAdd(S.nanCounter, acc.nans);
+        }
+    }
+}
+
 // 9-bit-per-axis Morton code of the origin in the scene box, then the
 // direction octant: rays from nearby origins heading the same way become
 // neighbours. Entries past the queue length sort last.
@@ -663,6 +744,31 @@ int wf_shade_grid(int numCUs) {
     return std::max(1, std::min(a, b)) * numCUs;
 }
 
+int wf_tail_grid(int numCUs) {
+    int a = 0, b = 0, c = 0;
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_tail<0, 1>, 256, 0));
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_tail<1, 8>, 256, 0));
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_wf_tail<2, 1>, 256, 0));
+    return std::max(1, std::min(a, std::min(b, c))) * numCUs;
+}
+
+namespace {
+void launch_tail(int mode, bool leaf8, const PtScene& S, const WfState& W, int grid, int k, int buf, int count,
+                 int work, hipStream_t st) {
+#define MFH_TAIL(M, L) \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_tail<M, L>), dim3(grid), dim3(256), 0, st, S, W, k, buf, count, work)
+    if (mode == 0) {
+        if (leaf8) MFH_TAIL(0, 8); else MFH_TAIL(0, 1);
+    } else if (mode == 1) {
+        if (leaf8) MFH_TAIL(1, 8); else MFH_TAIL(1, 1);
+    } else {
+        if (leaf8) MFH_TAIL(2, 8); else MFH_TAIL(2, 1);
+    }
+#undef MFH_TAIL
+    MFH_HIP(hipGetLastError());
+}
+}  // namespace
+
 // Sorted order of a queue: keys, then a 4-pass LSD radix sort over every path
 // slot (the queue length stays on the device); the permutation ends in sortIds[0].
 void sort_queue(const PtScene& S, const WfState& W, int buf, int count, bool shadow, BuildScratch& scratch,
@@ -700,6 +806,11 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
             if (sortNow) sort_queue(S, W, buf ^ 1, W.qBase + k + 1, false, scratch, stream);
             v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, sortNow ? W.sortIds[0] : nullptr,
                      tune.refill, stream);
+            if (tune.tailAfter > 0 && k + 1 == tune.tailAfter) {  // the rest of every path in one launch
+                launch_tail(single ? 1 : 0, leaf8, S, W, tune.tailGrid, k + 1, buf ^ 1, W.qBase + k + 1, work++,
+                            stream);
+                break;
+            }
         }
     }
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_finish<false>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, S,
@@ -725,7 +836,14 @@ void launch_pt_wavefront_batch(const WfState& W, const WfTuning& tune, int maxLe
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_shade<true>), dim3(tune.shadeGrid), dim3(256), 0, stream, none, W, k,
                            buf, W.qBase + k, W.qBase + k + 1, W.shBase + k, work++);
         v.shadow(none, W, tune.traceGrid, W.shBase + k, work++, nullptr, tune.refill, stream);
-        if (k + 1 < L) v.extend(none, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, nullptr, tune.refill, stream);
+        if (k + 1 < L) {
+            v.extend(none, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, nullptr, tune.refill, stream);
+            if (tune.tailAfter > 0 && k + 1 == tune.tailAfter) {
+                launch_tail(2, kWfTraceShapes[tune.traceVariant].leafBatch == 8, none, W, tune.tailGrid, k + 1,
+                            buf ^ 1, W.qBase + k + 1, work++, stream);
+                break;
+            }
+        }
     }
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_finish<true>), dim3((W.numPaths + 255) / 256), dim3(256), 0, stream, none,
                        W);

@@ -49,13 +49,16 @@ struct WfTuning {
     int chunk = 256;
     int sortBounces = 0;   // extension / shadow queues of the first bounces traced in sorted order
     int refill = 0;        // > 0: trace stages refill idle lanes every `refill` traversal steps
+    int tailAfter = 0;     // > 0: after the extension rays of this bounce, one launch finishes every path
     int traceGrid = 0;
     int shadeGrid = 0;
+    int tailGrid = 0;
 };
 
 // Grid of the persistent trace kernels for a variant (occupancy x CUs) and of the shade kernel.
 int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs);
 int wf_shade_grid(int numCUs);
+int wf_tail_grid(int numCUs);
 // One iteration: begin, primary, maxPathLength x (shade, shadow, extend), finish.
 void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, BuildScratch& scratch,
                          hipStream_t stream);

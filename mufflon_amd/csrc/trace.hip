@@ -60,6 +60,8 @@ TraceParams trace_params(const Scene& sc, bool reference) {
     p.tlasNodes = sc.tlasNodes.ptr;
     p.tlasPrimIds = sc.tlasPrimIds.as<int32_t>();
     p.tlasNumInternal = sc.tlasNumInternal;
+    p.tlasCoordMax = 0.0f;
+    for (int k = 0; k < 6; ++k) p.tlasCoordMax = std::max(p.tlasCoordMax, std::fabs(sc.box[k]));
     return p;
 }
 

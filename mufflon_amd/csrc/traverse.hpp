@@ -30,6 +30,26 @@ constexpr int kLdsDepth = 16;
 // triangle; the plain trace kernels keep 1, which is faster on small leaves.
 constexpr int kPtLeafBatch = 8;
 
+// A load of BVH-node or triangle data as a global-memory load: the pointers
+// arrive in structs as generic pointers, whose loads would compile to flat
+// loads that also wait on the LDS counter.
+typedef float GlobalF4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gload(const float4* p) {
+    const GlobalF4 v = *(const __attribute__((address_space(1))) GlobalF4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// An entry of a lane's LDS stack column, addressed as LDS so that a pop,
+// which reads either the LDS column or the private spill array, keeps a
+// ds_read and a scratch load instead of one flat load through a merged
+// pointer.
+__device__ __forceinline__ __attribute__((address_space(3))) int& lds_at(int* column, int i) {
+    return ((__attribute__((address_space(3))) int*)column)[i];
+}
+__device__ __forceinline__ int lds_at(const int* column, int i) {
+    return ((const __attribute__((address_space(3))) int*)column)[i];
+}
+
 // Slab test, reference CPU semantics (intersection.cu:57-75).
 __device__ __forceinline__ bool slab(const float* lo, const float* hi, const FastRay& r, float tmax, float& cmin) {
     const float x0 = lo[0] * r.inv.x - r.odiv.x, x1 = hi[0] * r.inv.x - r.odiv.x;
@@ -41,6 +61,54 @@ __device__ __forceinline__ bool slab(const float* lo, const float* hi, const Fas
     const float cmx = mn(mn(mxx, mxy), mxz);
     cmin = mx(mx(mnx, mny), mnz);
     return (cmx >= 0.0f) && (cmin <= tmax) && (cmin <= cmx);
+}
+
+// The same slab test with the hardware min / max (v_min_f32, v_max_f32,
+// v_min3_f32, v_max3_f32: about half the instructions of the compare-selects
+// above, and no VCC wait states). Where no x0 .. z1 is NaN its accept / reject
+// and the order of its entry distances equal slab()'s: the two min / max
+// forms then differ at most in the sign of a zero result, and every use of
+// cmx, cmin and the entry distances is a value comparison. slab_finite()
+// tells a lane where that holds.
+__device__ __forceinline__ bool slab_fast(const float* lo, const float* hi, const FastRay& r, float tmax,
+                                          float& cmin) {
+    const float x0 = lo[0] * r.inv.x - r.odiv.x, x1 = hi[0] * r.inv.x - r.odiv.x;
+    const float y0 = lo[1] * r.inv.y - r.odiv.y, y1 = hi[1] * r.inv.y - r.odiv.y;
+    const float z0 = lo[2] * r.inv.z - r.odiv.z, z1 = hi[2] * r.inv.z - r.odiv.z;
+    const float cmx = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
+                                      __builtin_fmaxf(z0, z1));
+    cmin = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
+                           __builtin_fminf(z0, z1));
+    return (cmx >= 0.0f) && (cmin <= tmax) && (cmin <= cmx);
+}
+
+// True when the slab tests of ray r against boxes whose coordinates are at
+// most coordMax in magnitude can produce no NaN: with |1/d| <= 2^64 on every
+// axis and |origin|, coordMax <= 2^62 (node boxes are enlarged by 1.0001
+// about their centre, so they stay within 2^63) every product is below
+// 2^127 and every difference below 2^128, hence finite. A direction with a
+// zero component (1/d infinite) or NaN input fails it.
+__device__ __forceinline__ bool slab_finite(const FastRay& r, float coordMax) {
+    constexpr float kInvMax = 18446744073709551616.0f;   // 2^64
+    constexpr float kCoordMax = 4611686018427387904.0f;  // 2^62
+    const float inv = __builtin_fmaxf(__builtin_fmaxf(fabsf(r.inv.x), fabsf(r.inv.y)), fabsf(r.inv.z));
+    const float org = __builtin_fmaxf(__builtin_fmaxf(fabsf(r.o.x), fabsf(r.o.y)), fabsf(r.o.z));
+    // written so that a NaN anywhere makes it false
+    return inv <= kInvMax && org <= kCoordMax && coordMax <= kCoordMax && r.inv.x == r.inv.x &&
+           r.inv.y == r.inv.y && r.inv.z == r.inv.z && r.o.x == r.o.x && r.o.y == r.o.y && r.o.z == r.o.z;
+}
+
+// Both child boxes of a node; fast (wave-uniform) selects slab_fast.
+__device__ __forceinline__ void slab_pair(bool fast, const float* lLo, const float* lHi, const float* rLo,
+                                          const float* rHi, const FastRay& r, float tmax, bool& t0, bool& t1,
+                                          float& c0, float& c1) {
+    if (fast) {
+        t0 = slab_fast(lLo, lHi, r, tmax, c0);
+        t1 = slab_fast(rLo, rHi, r, tmax, c1);
+    } else {
+        t0 = slab(lLo, lHi, r, tmax, c0);
+        t1 = slab(rLo, rHi, r, tmax, c1);
+    }
 }
 
 // Accept a hit closer than the current one; at equal distance the lower
@@ -120,7 +188,7 @@ struct Traversal {
     }
     __device__ __forceinline__ void push(int* lstack, int* spill, int v) {
         if (sp < kLds)
-            lstack[sp * kStride] = v;
+            lds_at(lstack, sp * kStride) = v;
         else if (sp < kStackCap)
             spill[sp - kLds] = v;
         if (sp < kStackCap)
@@ -130,7 +198,7 @@ struct Traversal {
     }
     __device__ __forceinline__ int pop(const int* lstack, const int* spill) {
         --sp;
-        return sp < kLds ? lstack[sp * kStride] : spill[sp - kLds];
+        return sp < kLds ? lds_at(lstack, sp * kStride) : spill[sp - kLds];
     }
 
     __device__ __forceinline__ void init(const TraceParams& p, V3 o, V3 d, float tmax) {
@@ -183,7 +251,7 @@ struct Traversal {
             while (nodeAddr < numInternal && primCount > 0) {
                 if (kBudget && budget-- <= 0) return false;
                 const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
-                const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+                const float4 q0 = gload(nd), q1 = gload(nd + 1), q2 = gload(nd + 2), q3 = gload(nd + 3);
                 if (kStats) ++nodeVisits;
                 const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
                 const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
@@ -260,8 +328,8 @@ struct Traversal {
                     descend = true;
                     break;
                 }
-                const float4 a = tris[3 * (size_t)leafId], b = tris[3 * (size_t)leafId + 1],
-                             c = tris[3 * (size_t)leafId + 2];
+                const float4 a = gload(tris + 3 * (size_t)leafId), b = gload(tris + 3 * (size_t)leafId + 1),
+                             c = gload(tris + 3 * (size_t)leafId + 2);
                 float t, w0, w1;
                 const int prim = __float_as_int(a.w);
                 if (kStats) ++primTests;
@@ -325,12 +393,12 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
 #define MFH_PUSH(v)                                     \
     do {                                                \
         if (sp < kLds)                                  \
-            lstack[sp * kStride] = (v);                 \
+            lds_at(lstack, sp * kStride) = (v);                 \
         else if (sp < kStackCap)                        \
             spill[sp - kLds] = (v);                     \
         if (sp < kStackCap) ++sp; else overflow = true; \
     } while (0)
-#define MFH_POP() (--sp, (sp < kLds) ? lstack[sp * kStride] : spill[sp - kLds])
+#define MFH_POP() (--sp, (sp < kLds) ? lds_at(lstack, sp * kStride) : spill[sp - kLds])
 
     FastRay cur = make_fast_ray(wo, wd);
     float hitT = tmax;
@@ -348,6 +416,8 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
     int curInst = -1;
     int nodeAddr = 0, primCount = 2, primOffset = 0;
     bool done = false;
+    // no NaN can reach this lane's slab tests at the current level
+    bool finite = slab_finite(cur, p.tlasCoordMax);
 
     if (kSingle) {
         DevLod lod;
@@ -357,6 +427,7 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
             primCount = 0;
         } else {
             cur = e.ray;
+            finite = slab_finite(cur, lod.coordMax);
             scale = e.scale;
             hitT = e.tmax;
             bvh = (const BvhChild*)lod.nodes;
@@ -372,15 +443,15 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
         // descend through internal nodes, nearer child first, farther pushed
         while (nodeAddr < numInternal && primCount > 0) {
             const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
-            const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+            const float4 q0 = gload(nd), q1 = gload(nd + 1), q2 = gload(nd + 2), q3 = gload(nd + 3);
             if (kStats) ++nodeVisits;
             const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
             const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
             const int lIdx = __float_as_int(q1.z), lCnt = __float_as_int(q1.w);
             const int rIdx = __float_as_int(q3.z), rCnt = __float_as_int(q3.w);
             float c0, c1;
-            const bool t0 = slab(lLo, lHi, cur, hitT, c0);
-            const bool t1 = slab(rLo, rHi, cur, hitT, c1);
+            bool t0, t1;
+            slab_pair(__all(finite), lLo, lHi, rLo, rHi, cur, hitT, t0, t1, c0, c1);
             if (!t0) c0 = 1e38f;
             if (!t1) c1 = 1e38f;
             if (t0 && t1) {
@@ -412,9 +483,9 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
                 for (int j = 0; j < kBatch; ++j) {
                     if (j < nb) {
                         const size_t id = (size_t)(first + j);
-                        ta[j] = tris[3 * id];
-                        tb[j] = tris[3 * id + 1];
-                        tc[j] = tris[3 * id + 2];
+                        ta[j] = gload(tris + 3 * id);
+                        tb[j] = gload(tris + 3 * id + 1);
+                        tc[j] = gload(tris + 3 * id + 2);
                     }
                 }
 #pragma unroll
@@ -475,6 +546,7 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
                 }
                 MFH_PUSH(kSentinel);
                 cur = e.ray;
+                finite = slab_finite(cur, lod.coordMax);
                 scale = e.scale;
                 hitT = e.tmax;
                 bvh = (const BvhChild*)lod.nodes;
@@ -490,8 +562,8 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
                 break;
             }
             // one triangle at a time (kBatch == 1)
-            const float4 a = tris[3 * (size_t)leafId], b = tris[3 * (size_t)leafId + 1],
-                         c = tris[3 * (size_t)leafId + 2];
+            const float4 a = gload(tris + 3 * (size_t)leafId), b = gload(tris + 3 * (size_t)leafId + 1),
+                         c = gload(tris + 3 * (size_t)leafId + 2);
             float t, w0, w1;
             const int prim = __float_as_int(a.w);
             if (kStats) ++primTests;
@@ -516,6 +588,7 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
         if (!kSingle && nodeAddr == kSentinel) {
             if (sp == 0) break;
             cur = make_fast_ray(wo, wd);
+            finite = slab_finite(cur, p.tlasCoordMax);
             hitT /= scale;
             scale = 1.0f;
             bvh = (const BvhChild*)p.tlasNodes;
@@ -564,7 +637,7 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
     bool overflow = false;
     auto push = [&](int v) {
         if (sp < kLds)
-            lstack[sp * kStride] = v;
+            lds_at(lstack, sp * kStride) = v;
         else if (sp < kStackCap)
             spill[sp - kLds] = v;
         if (sp < kStackCap)
@@ -574,7 +647,7 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
     };
     auto pop = [&]() {
         --sp;
-        return sp < kLds ? lstack[sp * kStride] : spill[sp - kLds];
+        return sp < kLds ? lds_at(lstack, sp * kStride) : spill[sp - kLds];
     };
     TraceHit h;
     h.inst = h.prim = -1;
@@ -588,6 +661,8 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
         return h;
     }
     const FastRay cur = e.ray;
+    // every lane of the wave that traverses is active here: the choice holds for the whole traversal
+    const bool fast = __all(slab_finite(cur, lod.coordMax));
     float hitT = e.tmax;
     const BvhChild* const bvh = (const BvhChild*)lod.nodes;
     const float4* const tris = lod.triVerts;
@@ -599,14 +674,14 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
         while (true) {
             if (primCount > 0 && nodeAddr < numInternal) {
                 const float4* nd = reinterpret_cast<const float4*>(bvh + 2 * (size_t)nodeAddr);
-                const float4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+                const float4 q0 = gload(nd), q1 = gload(nd + 1), q2 = gload(nd + 2), q3 = gload(nd + 3);
                 const float lLo[3] = {q0.x, q0.y, q0.z}, lHi[3] = {q0.w, q1.x, q1.y};
                 const float rLo[3] = {q2.x, q2.y, q2.z}, rHi[3] = {q2.w, q3.x, q3.y};
                 const int lIdx = __float_as_int(q1.z), lCnt = __float_as_int(q1.w);
                 const int rIdx = __float_as_int(q3.z), rCnt = __float_as_int(q3.w);
                 float c0, c1;
-                const bool t0 = slab(lLo, lHi, cur, hitT, c0);
-                const bool t1 = slab(rLo, rHi, cur, hitT, c1);
+                bool t0, t1;
+                slab_pair(fast, lLo, lHi, rLo, rHi, cur, hitT, t0, t1, c0, c1);
                 if (!t0) c0 = 1e38f;
                 if (!t1) c1 = 1e38f;
                 if (t0 && t1) {
@@ -643,9 +718,9 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
             for (int j = 0; j < kBatch; ++j) {
                 if (j < nb) {
                     const size_t id = (size_t)(first + j);
-                    ta[j] = tris[3 * id];
-                    tb[j] = tris[3 * id + 1];
-                    tc[j] = tris[3 * id + 2];
+                    ta[j] = gload(tris + 3 * id);
+                    tb[j] = gload(tris + 3 * id + 1);
+                    tc[j] = gload(tris + 3 * id + 2);
                 }
             }
 #pragma unroll

@@ -55,7 +55,7 @@ TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)
 PT_MEGAKERNEL = 0
 PT_WAVEFRONT = 1
 TUNING_DTYPE = np.dtype([("traceBlock", "<i4"), ("stackDepth", "<i4"), ("queueChunk", "<i4"), ("sortBounces", "<i4"),
-                         ("refillBudget", "<i4"), ("leafBatch", "<i4")])
+                         ("refillBudget", "<i4"), ("leafBatch", "<i4"), ("tailAfter", "<i4")])
 
 # name -> argtypes; every function returns int status
 _SIGS = {
@@ -488,11 +488,12 @@ class HipContext:
         self._call("mfh_batch_iterate", self.handle, batch, iterations)
 
     def batch_set_tuning(self, batch, trace_block=256, stack_depth=16, queue_chunk=256, refill_budget=0,
-                         leaf_batch=1):
+                         leaf_batch=1, tail_after=0):
         r = np.zeros(1, TUNING_DTYPE)
         r["traceBlock"], r["stackDepth"], r["queueChunk"], r["sortBounces"] = trace_block, stack_depth, queue_chunk, 0
         r["refillBudget"] = refill_budget
         r["leafBatch"] = leaf_batch
+        r["tailAfter"] = tail_after
         self._call("mfh_batch_set_tuning", self.handle, batch, r.ctypes.data)
 
     def batch_autotune(self, batch, iterations=1):
@@ -532,13 +533,15 @@ class HipContext:
         self._call("mfh_pt_set_mode", self.handle, pt, mode, 1 if use_graph else 0)
 
     def pt_set_tuning(self, pt, trace_block=256, stack_depth=16, queue_chunk=256, sort_bounces=0, refill_budget=0,
-                      leaf_batch=1):
-        """Wavefront launch shape; leaf_batch (1 or 8) also applies to the megakernel."""
+                      leaf_batch=1, tail_after=0):
+        """Wavefront launch shape; leaf_batch (1 or 8) also applies to the megakernel;
+        tail_after > 0 finishes every path in one launch after that bounce."""
         r = np.zeros(1, TUNING_DTYPE)
         r["traceBlock"], r["stackDepth"], r["queueChunk"] = trace_block, stack_depth, queue_chunk
         r["sortBounces"] = sort_bounces
         r["refillBudget"] = refill_budget
         r["leafBatch"] = leaf_batch
+        r["tailAfter"] = tail_after
         self._call("mfh_pt_set_tuning", self.handle, pt, r.ctypes.data)
 
     def pt_get_tuning(self, pt):

@@ -41,12 +41,14 @@ def test_batch_equals_single_scene_runs(hip_ctx, use_graph):
         for t, want in zip((0, 1), r):
             got = hip_ctx.pt_target(pt, t)
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (pt, t)
-    for chunk, refill, leaf in ((1024, 0, 1), (256, 16, 1), (256, 5, 1), (256, 0, 8), (1024, 16, 8)):
-        hip_ctx.batch_set_tuning(b, 128, 16, chunk, refill, leaf)
+    for chunk, refill, leaf, tail in ((1024, 0, 1, 0), (256, 16, 1, 0), (256, 5, 1, 2), (256, 0, 8, 1),
+                                      (1024, 16, 8, 0), (256, 0, 8, 3)):
+        hip_ctx.batch_set_tuning(b, 128, 16, chunk, refill, leaf, tail)
         hip_ctx.batch_reset(b)
         hip_ctx.batch_iterate(b, 3)
         for pt, r in zip(pts, ref):
-            assert np.array_equal(hip_ctx.pt_target(pt, 0).view(np.uint32), r[0].view(np.uint32)), (chunk, refill, leaf)
+            assert np.array_equal(hip_ctx.pt_target(pt, 0).view(np.uint32), r[0].view(np.uint32)), (chunk, refill, leaf,
+                                                                                                    tail)
     with pytest.raises(hip.HipError):
         hip_ctx.pt_iterate(pts[0], 1)
         hip_ctx.batch_iterate(b, 1)

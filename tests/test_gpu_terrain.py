@@ -34,12 +34,13 @@ def terrain(hip_ctx, oracle):
     return pt, windows
 
 
-@pytest.mark.parametrize("form,leaf,sort", [("megakernel", 1, 0), ("megakernel", 8, 0), ("wavefront", 1, 0),
-                                            ("wavefront", 8, 0), ("wavefront", 8, 2)])
-def test_terrain_windows_match_oracle(hip_ctx, terrain, form, leaf, sort):
+@pytest.mark.parametrize("form,leaf,sort,tail", [("megakernel", 1, 0, 0), ("megakernel", 8, 0, 0),
+                                                 ("wavefront", 1, 0, 0), ("wavefront", 8, 0, 0),
+                                                 ("wavefront", 8, 2, 0), ("wavefront", 1, 1, 3)])
+def test_terrain_windows_match_oracle(hip_ctx, terrain, form, leaf, sort, tail):
     pt, windows = terrain
-    print("terrain parity: %s, leaf batch %d, sorted bounces %d" % (form, leaf, sort), flush=True)
-    hip_ctx.pt_set_tuning(pt, leaf_batch=leaf, sort_bounces=sort)
+    print("terrain parity: %s, leaf batch %d, sorted bounces %d, tail after %d" % (form, leaf, sort, tail), flush=True)
+    hip_ctx.pt_set_tuning(pt, leaf_batch=leaf, sort_bounces=sort, tail_after=tail)
     mode = hip.PT_MEGAKERNEL if form == "megakernel" else hip.PT_WAVEFRONT
     hip_ctx.pt_set_mode(pt, mode, form == "wavefront")
     hip_ctx.pt_reset(pt)

@@ -19,16 +19,17 @@ def _same(a, b):
     return all(np.array_equal(a[t].view(np.uint32), b[t].view(np.uint32)) for t in a)
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_wavefront_equals_megakernel_material_zoo(hip_ctx, use_graph):
+@pytest.mark.parametrize("use_graph,tail", [(False, 0), (True, 0), (True, 1), (False, 3)])
+def test_wavefront_equals_megakernel_material_zoo(hip_ctx, use_graph, tail):
     c = S.cornell_materials(12, 24, 40, 40)
-    print("wavefront vs megakernel, material zoo, graph=%s" % use_graph, flush=True)
+    print("wavefront vs megakernel, material zoo, graph=%s, tail after %d" % (use_graph, tail), flush=True)
     targets = list(range(8))
     out = []
     for mode in (hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT):
         pt, sc, bg = P.gpu_pt_materials(hip_ctx, c, "envmap")
         for t in targets:
             hip_ctx.pt_enable_target(pt, t, True)
+        hip_ctx.pt_set_tuning(pt, tail_after=tail)
         hip_ctx.pt_set_mode(pt, mode, use_graph)
         hip_ctx.pt_iterate(pt, 3)
         assert hip_ctx.pt_iteration(pt) == 2
@@ -62,15 +63,16 @@ def test_wavefront_instanced_and_every_tuning(hip_ctx):
     hip_ctx.pt_set_mode(pt, hip.PT_WAVEFRONT, True)
     shapes = [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8), (256, 16), (256, 24)]
     for (block, depth), leaf in [(sh, 1) for sh in shapes] + [(sh, 8) for sh in shapes]:
-        for chunk, sort, refill in ((64, 0, 0), (1024, 3, 0), (256, 0, 16), (256, 2, 3)):
-            hip_ctx.pt_set_tuning(pt, block, depth, chunk, sort, refill, leaf)
+        for chunk, sort, refill, tail in ((64, 0, 0, 0), (1024, 3, 0, 1), (256, 0, 16, 3), (256, 2, 3, 2)):
+            hip_ctx.pt_set_tuning(pt, block, depth, chunk, sort, refill, leaf, tail)
             hip_ctx.pt_reset(pt)
             hip_ctx.pt_iterate(pt, 1)
             hip_ctx.pt_iterate(pt, 1)
             got = hip_ctx.pt_target(pt)
-            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, leaf, chunk, sort, refill)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (block, depth, leaf, chunk, sort, refill,
+                                                                              tail)
     assert hip_ctx.pt_get_tuning(pt) == dict(traceBlock=256, stackDepth=24, queueChunk=256, sortBounces=2,
-                                             refillBudget=3, leafBatch=8)
+                                             refillBudget=3, leafBatch=8, tailAfter=2)
     # the megakernel with 8 triangles of a leaf loaded ahead
     hip_ctx.pt_set_mode(pt, hip.PT_MEGAKERNEL)
     hip_ctx.pt_reset(pt)
