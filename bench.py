@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--stack-depth", type=int, default=16, help="wavefront LDS stack depth (with --pt-profile)")
     ap.add_argument("--refill", type=int, default=0, help="wavefront lane refill budget (with --pt-profile)")
     ap.add_argument("--sort-bounces", type=int, default=0, help="wavefront sorted bounces (with --pt-profile)")
+    ap.add_argument("--tail-after", type=int, default=0, help="wavefront tail bounce (with --pt-profile)")
     ap.add_argument("--terrain-ray-costs", action="store_true",
                     help="only the per-ray traversal cost of the terrain's camera rays and the trace time of its "
                          "heaviest rays")
@@ -548,7 +549,8 @@ def pt_profile(args):
     else:
         its = 8
         pt, _, _ = cornell_tracer(ctx)
-    ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, args.sort_bounces, args.refill, args.leaf_batch)
+    ctx.pt_set_tuning(pt, args.trace_block, args.stack_depth, 256, args.sort_bounces, args.refill, args.leaf_batch,
+                      args.tail_after)
     for mode in (hip.PT_MEGAKERNEL, hip.PT_WAVEFRONT):
         ctx.pt_set_mode(pt, mode, False)
         ctx.pt_reset(pt)
