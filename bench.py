@@ -663,15 +663,19 @@ def main():
         v_node, v_prim = int(st[:, 0].sum()), int(st[:, 1].sum())
         traffic_alg = n_rays * RAY_IO_BYTES + NODE_BYTES * v_node + TRI_BYTES * v_prim
         achieved = traffic_alg / (np.mean(trace_ms) / 1e3) / 1e9
-        traffic = None
+        traffic, traffic_is = None, None
         tf = os.path.join(ROOT, "profiles", "trace_closest_traffic.json")
         if os.path.exists(tf):
             with open(tf) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                rec = json.load(f)
+            traffic, traffic_is = rec.get("hbm_bytes_per_launch"), rec.get("counts")
         out["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "k_trace<closest>", "algorithmic_bytes": traffic_alg,
+            "achieved_is": "algorithmic bytes (ray and hit I/O + 64 B per node visit + 48 B per triangle test, "
+                           "counted per ray by the visit-count kernel) / mean event-timed trace launch",
+            "traffic_is": traffic_is,
             "node_visits_per_ray": round(v_node / n_rays, 3), "tri_tests_per_ray": round(v_prim / n_rays, 3),
         }
         if not args.no_cpu_baseline:
