@@ -51,8 +51,20 @@ int mfh_download(mfh_context* ctx, void* dst, const void* dSrc, size_t bytes);
 int mfh_lod_create(mfh_context* ctx, const float* vertices, int32_t numVertices, const uint32_t* triIndices,
                    int32_t numTriangles, int32_t* outLodId);
 
+/* LoD of triangles, quads and spheres (PolygonsDescriptor + SpheresDescriptor,
+ * descriptors.hpp:33-68): quads as four uint32 vertex indices (p00, p10,
+ * p11, p01 of the bilinear patch, intersection.cu:99-179), spheres as four
+ * floats (center xyz, radius). Primitive ids run triangles, quads, spheres.
+ * Any count may be 0 but not all; vertices may be NULL without polygons. The
+ * LoD box joins the vertices' box with the spheres' boxes
+ * (Lod::get_bounding_box). Replaces the polygon and sphere uploads of
+ * Lod::get_descriptor (lod.cpp:16-20). */
+int mfh_lod_create_mixed(mfh_context* ctx, const float* vertices, int32_t numVertices, const uint32_t* triIndices,
+                         int32_t numTriangles, const uint32_t* quadIndices, int32_t numQuads, const float* spheres,
+                         int32_t numSpheres, int32_t* outLodId);
+
 /* Build the LBVH of a LoD on the device. bbox = {min xyz, max xyz} of the
- * LoD (Lod::get_bounding_box); NULL computes it from the vertices.
+ * LoD (Lod::get_bounding_box); NULL takes the box computed at creation.
  * Replaces LBVHBuilder::build(LodDescriptor&, const ei::Box&, bool)
  * (build_lbvh.cu:782-789, lod.cpp:33). */
 int mfh_lbvh_build(mfh_context* ctx, int32_t lodId, const float* bbox);
@@ -71,7 +83,7 @@ int mfh_set_traversal_leaf_limit(mfh_context* ctx, int32_t limit);
 /* Result of the last build of a LoD: numInternalNodes of the collapsed BVH
  * (LBVH::numInternalNodes, lbvh.hpp:23) and, when the pointers are non-NULL,
  * the nodes (numInternalNodes * 64 bytes, the BvhNode layout of lbvh.hpp:13-18)
- * and the primitive ids in BVH order (numTriangles int32).
+ * and the primitive ids in BVH order (one int32 per primitive).
  * Replaces LBVHBuilder::acquire_const (lbvh.hpp:70-86). */
 int mfh_lbvh_get(mfh_context* ctx, int32_t lodId, int32_t* outNumInternal, void* outNodes, int32_t* outPrimIds);
 
@@ -274,8 +286,38 @@ int mfh_light_tree_build(const mfh_area_light* areaLights, int32_t numAreaLights
                          int32_t numMaterials, mfh_light_tree_info* outInfo, void* outMem, size_t memCapacity,
                          uint64_t* outMapKeys, uint32_t* outMapCodes, int32_t mapCapacity);
 
+/* Emissive triangle, quad or sphere in world space for the light tree
+ * (AreaLightTriangleDesc / AreaLightQuadDesc / AreaLightSphereDesc,
+ * lights.hpp:77-91, plus the primitive it was made from). kind 0: points
+ * 0..2 and uv 0..2 of a triangle; 1: points 0..3 and uv 0..3 of a quad (its
+ * vertices in patch order p00, p10, p11, p01); 2: a sphere, center in
+ * points 0..2, and radius. */
+typedef struct mfh_area_light_ex {
+    int32_t kind;
+    float points[12];
+    float uv[8];
+    float radius;
+    int32_t material;
+    int32_t instance;
+    int32_t primitive;
+} mfh_area_light_ex;
+
+/* mfh_light_tree_build over triangles, quads and spheres. Records:
+ * AreaLightQuad 96 bytes, AreaLightSphere 32 bytes (lights.hpp:108-137); a
+ * quad's flux samples its area density at 128 golden-ratio points seeded with
+ * the bits of its parallelogram area (the reference seeds with the record's
+ * memory address, lights.cpp:46). Capacities: outMem at least
+ * 96 * (numAreaLights + numPointLights) * 2 bytes. */
+int mfh_light_tree_build_ex(const mfh_area_light_ex* areaLights, int32_t numAreaLights,
+                            const mfh_point_light* pointLights, int32_t numPointLights, const float* sceneBox,
+                            const mfh_material* materials, int32_t numMaterials, mfh_light_tree_info* outInfo,
+                            void* outMem, size_t memCapacity, uint64_t* outMapKeys, uint32_t* outMapCodes,
+                            int32_t mapCapacity);
+
 /* Shading attributes of a LoD: per-vertex normals (xyz) and uvs, and a
- * material index per triangle (PolygonsDescriptor normals/uvs/matIndices). */
+ * material index per primitive (PolygonsDescriptor normals/uvs/matIndices,
+ * then SpheresDescriptor matIndices); normals and uvs may be NULL for a LoD
+ * without vertices. */
 int mfh_lod_set_shading(mfh_context* ctx, int32_t lodId, const float* normals, const float* uvs,
                         const uint16_t* materialIndices);
 

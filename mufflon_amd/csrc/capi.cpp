@@ -58,7 +58,8 @@ void refresh_scene(Context& c, Scene& s) {
         lods[i].triVerts = l->triVerts.as<float4>();
         lods[i].nodes = l->nodes.ptr;
         lods[i].numInternal = l->numInternal;
-        lods[i].numPrims = l->numTris;
+        lods[i].numPrims = l->numPrims();
+        lods[i].quadV3 = l->numQuads > 0 ? l->quadV3.as<float4>() : nullptr;
         std::memcpy(lods[i].box, l->box, sizeof(l->box));
         lods[i].coordMax = l->coordMax;
         ref[i] = lods[i];
@@ -67,8 +68,11 @@ void refresh_scene(Context& c, Scene& s) {
             lods[i].numInternal = l->travNumInternal;
         }
     }
-    for (int32_t li : s.instLod)
+    s.mixed = false;
+    for (int32_t li : s.instLod) {
         MFH_REQUIRE(c.lods[li]->built, "instance refers to a LoD whose BVH is not built");
+        if (c.lods[li]->mixed()) s.mixed = true;
+    }
     upload(c, s.dLods, lods.data(), sizeof(DevLod) * lods.size());
     upload(c, s.dLodsRef, ref.data(), sizeof(DevLod) * ref.size());
     build_scene_bvh(c, s, nullptr);
@@ -113,6 +117,7 @@ int mfh_destroy(mfh_context* ctx) {
         for (Lod* l : c.lods) {
             if (!l) continue;
             l->verts.release(); l->tris.release(); l->nodes.release(); l->primIds.release(); l->triVerts.release();
+            l->quads.release(); l->spheres.release(); l->quadV3.release();
             l->travNodes.release();
             l->normals.release(); l->uvs.release(); l->matIdx.release();
             delete l;
@@ -183,34 +188,100 @@ int mfh_download(mfh_context* ctx, void* dst, const void* dSrc, size_t bytes) {
     });
 }
 
+namespace mfh {
+namespace {
+
+// A LoD of triangles, quads and spheres (any of them may be absent, not all).
+// Box: the vertices' box joined with every sphere's center -+ radius
+// (Lod::get_bounding_box: polygons and spheres).
+int32_t create_lod(Context& c, const float* vertices, int32_t numVertices, const uint32_t* triIndices,
+                   int32_t numTriangles, const uint32_t* quadIndices, int32_t numQuads, const float* spheres,
+                   int32_t numSpheres) {
+    MFH_REQUIRE(numVertices >= 0 && numTriangles >= 0 && numQuads >= 0 && numSpheres >= 0, "negative count");
+    MFH_REQUIRE((numTriangles == 0 || triIndices) && (numQuads == 0 || quadIndices) && (numSpheres == 0 || spheres) &&
+                    (numVertices == 0 || vertices),
+                "null pointer");
+    const long long prims = (long long)numTriangles + numQuads + numSpheres;
+    MFH_REQUIRE(prims > 0, "empty LoD");
+    MFH_REQUIRE(prims * 2 - 1 < (1ll << 31), "too many primitives");
+    MFH_REQUIRE(numTriangles + numQuads == 0 || numVertices > 0, "polygons without vertices");
+    for (int64_t i = 0; i < 3 * (int64_t)numTriangles; ++i)
+        MFH_REQUIRE(triIndices[i] < (uint32_t)numVertices, "triangle index out of range");
+    for (int64_t i = 0; i < 4 * (int64_t)numQuads; ++i)
+        MFH_REQUIRE(quadIndices[i] < (uint32_t)numVertices, "quad index out of range");
+    for (int32_t i = 0; i < numSpheres; ++i)
+        MFH_REQUIRE(spheres[4 * (size_t)i + 3] >= 0.0f, "negative sphere radius");
+    auto* l = new Lod();
+    try {
+        l->numVerts = numVertices;
+        l->numTris = numTriangles;
+        l->numQuads = numQuads;
+        l->numSpheres = numSpheres;
+        if (numVertices > 0) {
+            l->hostVerts.assign(vertices, vertices + 3 * (size_t)numVertices);
+            upload(c, l->verts, vertices, 12 * (size_t)numVertices);
+        }
+        if (numTriangles > 0) {
+            l->hostTris.assign(triIndices, triIndices + 3 * (size_t)numTriangles);
+            upload(c, l->tris, triIndices, 12 * (size_t)numTriangles);
+        }
+        if (numQuads > 0) {
+            l->hostQuads.assign(quadIndices, quadIndices + 4 * (size_t)numQuads);
+            upload(c, l->quads, quadIndices, 16 * (size_t)numQuads);
+        }
+        if (numSpheres > 0) {
+            l->hostSpheres.assign(spheres, spheres + 4 * (size_t)numSpheres);
+            upload(c, l->spheres, spheres, 16 * (size_t)numSpheres);
+        }
+        bool first = true;
+        if (numVertices > 0) {
+            vertex_box(vertices, numVertices, l->box);
+            first = false;
+        }
+        for (int32_t i = 0; i < numSpheres; ++i) {
+            const float* sp = spheres + 4 * (size_t)i;
+            for (int k = 0; k < 3; ++k) {
+                const float lo = sp[k] - sp[3], hi = sp[k] + sp[3];
+                l->box[k] = (first || lo < l->box[k]) ? lo : l->box[k];
+                l->box[3 + k] = (first || l->box[3 + k] < hi) ? hi : l->box[3 + k];
+            }
+            first = false;
+        }
+        for (int k = 0; k < 6; ++k) l->coordMax = std::max(l->coordMax, std::fabs(l->box[k]));
+        MFH_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+        l->verts.release();
+        l->tris.release();
+        l->quads.release();
+        l->spheres.release();
+        delete l;
+        throw;
+    }
+    c.lods.push_back(l);
+    return (int32_t)c.lods.size() - 1;
+}
+
+}  // namespace
+}  // namespace mfh
+
 int mfh_lod_create(mfh_context* ctx, const float* vertices, int32_t numVertices, const uint32_t* triIndices,
                    int32_t numTriangles, int32_t* outLodId) {
     return guarded([&] {
         Context& c = ctx_of(ctx);
         MFH_REQUIRE(vertices && triIndices && outLodId, "null pointer");
         MFH_REQUIRE(numVertices > 0 && numTriangles > 0, "empty LoD");
-        MFH_REQUIRE((long long)numTriangles * 2 - 1 < (1ll << 31), "too many triangles");
-        for (int64_t i = 0; i < 3 * (int64_t)numTriangles; ++i)
-            MFH_REQUIRE(triIndices[i] < (uint32_t)numVertices, "triangle index out of range");
-        auto* l = new Lod();
-        try {
-            l->numVerts = numVertices;
-            l->numTris = numTriangles;
-            l->hostVerts.assign(vertices, vertices + 3 * (size_t)numVertices);
-            l->hostTris.assign(triIndices, triIndices + 3 * (size_t)numTriangles);
-            upload(c, l->verts, vertices, 12 * (size_t)numVertices);
-            upload(c, l->tris, triIndices, 12 * (size_t)numTriangles);
-            vertex_box(vertices, numVertices, l->box);
-            for (int k = 0; k < 6; ++k) l->coordMax = std::max(l->coordMax, std::fabs(l->box[k]));
-            MFH_HIP(hipStreamSynchronize(c.stream));
-        } catch (...) {
-            l->verts.release();
-            l->tris.release();
-            delete l;
-            throw;
-        }
-        c.lods.push_back(l);
-        *outLodId = (int32_t)c.lods.size() - 1;
+        *outLodId = create_lod(c, vertices, numVertices, triIndices, numTriangles, nullptr, 0, nullptr, 0);
+    });
+}
+
+int mfh_lod_create_mixed(mfh_context* ctx, const float* vertices, int32_t numVertices, const uint32_t* triIndices,
+                         int32_t numTriangles, const uint32_t* quadIndices, int32_t numQuads, const float* spheres,
+                         int32_t numSpheres, int32_t* outLodId) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        MFH_REQUIRE(outLodId, "null pointer");
+        *outLodId = create_lod(c, vertices, numVertices, triIndices, numTriangles, quadIndices, numQuads, spheres,
+                               numSpheres);
     });
 }
 
@@ -254,7 +325,8 @@ int mfh_lbvh_get(mfh_context* ctx, int32_t lodId, int32_t* outNumInternal, void*
         if (outNumInternal) *outNumInternal = l.numInternal;
         if (outNodes && l.numInternal > 0)
             MFH_HIP(hipMemcpy(outNodes, l.nodes.ptr, 64 * (size_t)l.numInternal, hipMemcpyDeviceToHost));
-        if (outPrimIds) MFH_HIP(hipMemcpy(outPrimIds, l.primIds.ptr, 4 * (size_t)l.numTris, hipMemcpyDeviceToHost));
+        if (outPrimIds)
+            MFH_HIP(hipMemcpy(outPrimIds, l.primIds.ptr, 4 * (size_t)l.numPrims(), hipMemcpyDeviceToHost));
     });
 }
 

@@ -100,6 +100,87 @@ struct TriPrims {
         hi = vmax(vmax(a, b), c);
     }
     __device__ __forceinline__ bool masked(i32) const { return false; }
+    __device__ __forceinline__ int kind(i32) const { return kPrimTriangle; }
+    static constexpr bool kRecords = true;
+    // The traversal record of triangle `prim` at sorted position idx; its box.
+    __device__ __forceinline__ void record(i32 prim, int idx, float4* recs, V3& lo, V3& hi) const {
+        V3 a, b, c;
+        tri(prim, a, b, c);
+        lo = vmin(vmin(a, b), c);
+        hi = vmax(vmax(a, b), c);
+        recs[3 * (size_t)idx] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
+        recs[3 * (size_t)idx + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+        recs[3 * (size_t)idx + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+};
+
+// A LoD with quads and / or spheres besides triangles, primitive ids in the
+// order triangles, quads, spheres (descriptors.hpp:33-68). Centroids and
+// boxes of accel_structs_commons.hpp:56-100: a quad's centroid is the mean of
+// its four vertices, a sphere's its center, its box center -+ radius.
+struct MixedPrims {
+    const float* verts;
+    const u32* tris;
+    const u32* quads;
+    const float4* spheres;
+    float4* quadV3;  // out: fourth quad vertex per sorted position
+    i32 nt, nq;
+    __device__ __forceinline__ V3 vert(u32 i) const { return v3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]); }
+    __device__ __forceinline__ int kind(i32 p) const {
+        return p < nt ? kPrimTriangle : (p < nt + nq ? kPrimQuad : kPrimSphere);
+    }
+    __device__ __forceinline__ void quad(i32 q, V3& a, V3& b, V3& c, V3& d) const {
+        const u32* ix = quads + 4 * (size_t)q;
+        a = vert(ix[0]);
+        b = vert(ix[1]);
+        c = vert(ix[2]);
+        d = vert(ix[3]);
+    }
+    __device__ __forceinline__ V3 centroid(i32 p) const {
+        const int k = kind(p);
+        if (k == kPrimTriangle) {
+            const u32* ix = tris + 3 * (size_t)p;
+            return (vert(ix[0]) + vert(ix[1]) + vert(ix[2])) / 3.0f;
+        }
+        if (k == kPrimQuad) {
+            V3 a, b, c, d;
+            quad(p - nt, a, b, c, d);
+            return (a + b + c + d) / 4.0f;
+        }
+        const float4 s = spheres[p - nt - nq];
+        return v3(s.x, s.y, s.z);
+    }
+    __device__ __forceinline__ bool masked(i32) const { return false; }
+    static constexpr bool kRecords = true;
+    __device__ __forceinline__ void record(i32 prim, int idx, float4* recs, V3& lo, V3& hi) const {
+        const int k = kind(prim);
+        float4* r = recs + 3 * (size_t)idx;
+        if (k == kPrimTriangle) {
+            const u32* ix = tris + 3 * (size_t)prim;
+            const V3 a = vert(ix[0]), b = vert(ix[1]), c = vert(ix[2]);
+            lo = vmin(vmin(a, b), c);
+            hi = vmax(vmax(a, b), c);
+            r[0] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
+            r[1] = make_float4(b.x, b.y, b.z, __int_as_float(kPrimTriangle));
+            r[2] = make_float4(c.x, c.y, c.z, 0.0f);
+        } else if (k == kPrimQuad) {
+            V3 a, b, c, d;
+            quad(prim - nt, a, b, c, d);
+            lo = vmin(vmin(vmin(a, b), c), d);
+            hi = vmax(vmax(vmax(a, b), c), d);
+            r[0] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
+            r[1] = make_float4(b.x, b.y, b.z, __int_as_float(kPrimQuad));
+            r[2] = make_float4(c.x, c.y, c.z, 0.0f);
+            quadV3[idx] = make_float4(d.x, d.y, d.z, 0.0f);
+        } else {
+            const float4 s = spheres[prim - nt - nq];
+            lo = v3(s.x - s.w, s.y - s.w, s.z - s.w);
+            hi = v3(s.x + s.w, s.y + s.w, s.z + s.w);
+            r[0] = make_float4(s.x, s.y, s.z, __int_as_float(prim));
+            r[1] = make_float4(s.w, 0.0f, 0.0f, __int_as_float(kPrimSphere));
+            r[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
 };
 
 struct InstPrims {
@@ -135,6 +216,9 @@ struct InstPrims {
         }
     }
     __device__ __forceinline__ bool masked(i32) const { return false; }
+    __device__ __forceinline__ int kind(i32) const { return 0; }
+    static constexpr bool kRecords = false;
+    __device__ __forceinline__ void record(i32, int, float4*, V3&, V3&) const {}
 };
 
 // ------------------------------------------------------------- cost models
@@ -142,7 +226,8 @@ struct InstPrims {
 // counts packed 3 x 10 bits, each saturating at 1023.
 struct LodCost {
     static constexpr float Ct = 2.0f;
-    __device__ static i32 leaf() { return 1 << 20; }
+    // count code of one primitive of kind k (get_count, build_lbvh.cu:285-295)
+    __device__ static i32 leaf(int k) { return k == kPrimTriangle ? 1 << 20 : (k == kPrimQuad ? 1 << 10 : 1); }
     __device__ static float cost(i32 e) {
         const float x = (float)((e & 0x3FF00000) >> 20), y = (float)((e & 0x000FFC00) >> 10),
                     z = (float)(e & 0x3FF);
@@ -162,7 +247,7 @@ struct LodCost {
 // Instance level (build_lbvh.cu:41-48): Ct = 1, Cp = 8.5, plain count.
 struct InstCost {
     static constexpr float Ct = 1.0f;
-    __device__ static i32 leaf() { return 1; }
+    __device__ static i32 leaf(int) { return 1; }
     __device__ static float cost(i32 e) { return Ct + (float)e * 8.5f; }
     __device__ static i32 add(i32 a, i32 b) { return a + b; }
     __device__ static bool capped(i32 e) { return e >= 1023; }
@@ -280,19 +365,11 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, con
         if (idx > last) break;
         const int prim = ids[idx];
         V3 lo, hi;
-        if constexpr (std::is_same<P, TriPrims>::value) {
-            // Triangle LoD: also lay the triangle out in BVH order for traversal.
-            V3 a, b, c;
-            set.tri(prim, a, b, c);
-            lo = vmin(vmin(a, b), c);
-            hi = vmax(vmax(a, b), c);
-            triVerts[3 * (size_t)idx] = make_float4(a.x, a.y, a.z, __int_as_float(prim));
-            triVerts[3 * (size_t)idx + 1] = make_float4(b.x, b.y, b.z, 0.0f);
-            triVerts[3 * (size_t)idx + 2] = make_float4(c.x, c.y, c.z, 0.0f);
-        } else {
+        if constexpr (P::kRecords)
+            set.record(prim, idx, triVerts, lo, hi);  // LoD: also lay the primitive out in BVH order
+        else
             set.box(prim, lo, hi);
-        }
-        const i32 cnt = C::leaf();
+        const i32 cnt = C::leaf(set.kind(prim));
         const int leaf = idx + nInt;
         boxes[2 * (size_t)leaf] = make_float4(lo.x, lo.y, lo.z, __int_as_float(cnt));
         boxes[2 * (size_t)leaf + 1] = make_float4(hi.x, hi.y, hi.z, C::cost(cnt));
@@ -605,16 +682,26 @@ int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const floa
 }  // namespace
 
 void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink) {
-    const int n = lod.numTris;
-    MFH_REQUIRE(n >= 1, "LoD has no triangles");
+    const int n = lod.numPrims();
+    MFH_REQUIRE(n >= 1, "LoD has no primitives");
     if (bbox) std::memcpy(lod.box, bbox, sizeof(lod.box));
-    TriPrims set{lod.verts.as<float>(), lod.tris.as<u32>()};
     lod.triVerts.reserve(48 * (size_t)n);
     Refinement refine;
     refine.refineAbove = ctx.traversalLeafLimit;
     refine.nodes = &lod.travNodes;
-    lod.numInternal = build_generic<TriPrims, LodCost>(ctx, ctx.scratch, set, n, lod.box, lod.nodes, lod.primIds,
-                                                       lod.triVerts.as<float4>(), sink, nullptr, &refine);
+    if (lod.mixed()) {
+        if (lod.numQuads > 0) lod.quadV3.reserve(16 * (size_t)n);
+        MixedPrims set{lod.verts.as<float>(), lod.tris.as<u32>(),  lod.quads.as<u32>(), lod.spheres.as<float4>(),
+                       lod.numQuads > 0 ? lod.quadV3.as<float4>() : nullptr, lod.numTris, lod.numQuads};
+        lod.numInternal = build_generic<MixedPrims, LodCost>(ctx, ctx.scratch, set, n, lod.box, lod.nodes,
+                                                             lod.primIds, lod.triVerts.as<float4>(), sink, nullptr,
+                                                             &refine);
+    } else {
+        TriPrims set{lod.verts.as<float>(), lod.tris.as<u32>()};
+        lod.numInternal = build_generic<TriPrims, LodCost>(ctx, ctx.scratch, set, n, lod.box, lod.nodes,
+                                                           lod.primIds, lod.triVerts.as<float4>(), sink, nullptr,
+                                                           &refine);
+    }
     lod.travNumInternal = refine.numInternal;
     lod.built = true;
 }

@@ -1,5 +1,5 @@
 // Host light-tree builder: LightTreeBuilder::build (light_tree.cpp:340-447),
-// create_light_tree (:119-266), fill_map (:268-302), get_flux (lights.cpp:11-90)
+// create_light_tree (:119-266), fill_map (:268-302), get_flux (lights.cpp:11-81)
 // and the Hilbert ordering (sfcurves.hpp), producing Mufflon's byte layout.
 #include "light_tree.hpp"
 
@@ -83,19 +83,24 @@ inline float sample_uniform(u32 u) { return (float)u / 4294967810.0f; }
 inline u32 f2u(float v) { return (u32)(int64_t)v; }
 
 struct PosLight {
-    bool isTri;
-    HostAreaTriangle tri;
+    bool isArea;
+    HostAreaLight area;
     HostPointLight point;
 };
 
+inline V3 point_of(const float p[3]) { return {p[0], p[1], p[2]}; }
+
 // The sort key reads the light description as the stored record
 // (light_tree.cpp:366-379 passes the variant's storage to get_center): for a
-// triangle it is points[0] + (points[1] + points[2]) / 3.
+// triangle points[0] + (points[1] + points[2]) / 3, for a quad points[0] +
+// (points[1] + points[2]) / 2 + points[3] / 4 (lights.hpp:216-224 over the
+// description's points), a sphere's center, a point light's position.
 V3 sort_center(const PosLight& l) {
-    if (!l.isTri) return {l.point.position[0], l.point.position[1], l.point.position[2]};
-    const V3 p0{l.tri.points[0][0], l.tri.points[0][1], l.tri.points[0][2]};
-    const V3 p1{l.tri.points[1][0], l.tri.points[1][1], l.tri.points[1][2]};
-    const V3 p2{l.tri.points[2][0], l.tri.points[2][1], l.tri.points[2][2]};
+    if (!l.isArea) return point_of(l.point.position);
+    const HostAreaLight& a = l.area;
+    if (a.kind == kAreaSphere) return point_of(a.points[0]);
+    const V3 p0 = point_of(a.points[0]), p1 = point_of(a.points[1]), p2 = point_of(a.points[2]);
+    if (a.kind == kAreaQuad) return add(add(p0, mul(add(p1, p2), 0.5f)), mul(point_of(a.points[3]), 0.25f));
     return add(p0, divs(add(p1, p2), 3.0f));
 }
 
@@ -106,6 +111,22 @@ struct TriRecord {  // AreaLightTriangle (lights.hpp:91-107), 64 bytes
     float uvV[3][2];
 };
 static_assert(sizeof(TriRecord) == 64, "AreaLightTriangle record");
+struct QuadRecord {  // AreaLightQuad (lights.hpp:108-124), 96 bytes
+    float posV[4][3];  // v0, v3 - v0, v1 - v0, v2 - v3 - (v1 - v0)
+    float uvV[4][2];
+    u16 material;
+    u16 pad[7];
+};
+static_assert(sizeof(QuadRecord) == 96, "AreaLightQuad record");
+struct SphereRecord {  // AreaLightSphere (lights.hpp:125-137), 32 bytes
+    float position[3];
+    float radius;
+    u16 material;
+    u16 pad[7];
+};
+static_assert(sizeof(SphereRecord) == 32, "AreaLightSphere record");
+constexpr size_t kQuadBytes = 96;
+constexpr size_t kSphereBytes = 32;
 struct PointRecord {  // PointLight (lights.hpp:46-50), 32 bytes
     float position[3];
     u16 mediumIndex;
@@ -143,6 +164,8 @@ struct Builder {
                         f2 = r.intensity[2] * 4.f * kPi;
             return f0 + f1 + f2;
         }
+        if (type == kAreaLightQuad) return quad_flux(p);
+        if (type == kAreaLightSphere) return sphere_flux(p);
         // area triangle: 128 golden-ratio samples of the emission (lights.cpp:24-41)
         TriRecord r;
         std::memcpy(&r, p, sizeof(r));
@@ -167,6 +190,60 @@ struct Builder {
         for (int k = 0; k < 3; ++k) f[k] = sum[k] / 128.0f * area * 2 * kPi;
         return f[0] + f[1] + f[2];
     }
+    // lights.cpp:43-60: 128 golden-ratio samples of the local area (the
+    // parametric density |dP/du1 x dP/du0|) times the emission. The reference
+    // seeds the sampler with the record's memory address, which no rerun
+    // reproduces; this build seeds it with the bits of the parallelogram area
+    // |posV[1] x posV[2]| (the triangle's rule, lights.cpp:28).
+    float quad_flux(const uint8_t* p) const {
+        QuadRecord r;
+        std::memcpy(&r, p, sizeof(r));
+        V3 pv[4];
+        for (int k = 0; k < 4; ++k) pv[k] = point_of(r.posV[k]);
+        const float seedArea = len(cross(pv[1], pv[2]));
+        u32 seed;
+        std::memcpy(&seed, &seedArea, 4);
+        GoldenRatio2D gen(seed);
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 128; ++i) {
+            const u64 x = gen.next();
+            const float ux = sample_uniform((u32)(x & 0xffffffffull)), uy = sample_uniform((u32)(x >> 32));
+            const float u = r.uvV[0][0] + r.uvV[1][0] * uy + r.uvV[2][0] * ux + r.uvV[3][0] * (ux * uy);
+            const float v = r.uvV[0][1] + r.uvV[1][1] * uy + r.uvV[2][1] * ux + r.uvV[3][1] * (ux * uy);
+            const V3 tangentX = add(pv[1], mul(pv[3], ux));
+            const V3 tangentY = add(pv[2], mul(pv[3], uy));
+            const float area = len(cross(tangentY, tangentX));
+            float rad[3];
+            emission.radiance(r.material, u, v, rad);
+            for (int k = 0; k < 3; ++k) sum[k] += area * rad[k];
+        }
+        float f[3];
+        for (int k = 0; k < 3; ++k) f[k] = sum[k] / 128.0f * 2 * kPi;
+        return f[0] + f[1] + f[2];
+    }
+    // lights.cpp:63-80: the emission sampled at 128 golden-ratio lon-lat
+    // points, times the surface 4 pi r^2 (the role of ei::surface(Sphere)).
+    float sphere_flux(const uint8_t* p) const {
+        SphereRecord r;
+        std::memcpy(&r, p, sizeof(r));
+        const float area = 4 * kPi * r.radius * r.radius;
+        u32 seed;
+        std::memcpy(&seed, &area, 4);
+        GoldenRatio2D gen(seed);
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < 128; ++i) {
+            const u64 x = gen.next();
+            const float ux = sample_uniform((u32)(x & 0xffffffffull)), uy = sample_uniform((u32)(x >> 32));
+            const float theta = std::acos(ux * 2.0f - 1.0f) / kPi;
+            const float phi = uy;
+            float rad[3];
+            emission.radiance(r.material, phi, theta, rad);
+            for (int k = 0; k < 3; ++k) sum[k] += rad[k];
+        }
+        float f[3];
+        for (int k = 0; k < 3; ++k) f[k] = sum[k] / 128.0f * area * 2 * kPi;
+        return f[0] + f[1] + f[2];
+    }
     V3 center_of(u32 offset, u16 type) const {
         const uint8_t* p = mem.data() + offset;
         if (type == kInternalNode) {
@@ -174,10 +251,16 @@ struct Builder {
             std::memcpy(&n, p, sizeof(n));
             return {n.center[0], n.center[1], n.center[2]};
         }
-        if (type == kPointLight) {
+        if (type == kPointLight || type == kAreaLightSphere) {  // position first in both records
             PointRecord r;
             std::memcpy(&r, p, sizeof(r));
             return {r.position[0], r.position[1], r.position[2]};
+        }
+        if (type == kAreaLightQuad) {  // lights.hpp:221-224
+            QuadRecord r;
+            std::memcpy(&r, p, sizeof(r));
+            return add(add(point_of(r.posV[0]), mul(add(point_of(r.posV[1]), point_of(r.posV[2])), 0.5f)),
+                       mul(point_of(r.posV[3]), 0.25f));
         }
         TriRecord r;
         std::memcpy(&r, p, sizeof(r));
@@ -209,11 +292,11 @@ size_t num_internal_nodes(size_t n) {
 
 }  // namespace
 
-void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vector<HostPointLight>& points,
+void build_light_tree(const std::vector<HostAreaLight>& area, const std::vector<HostPointLight>& points,
                       const float sceneBox[6], const EmissionLookup& emission, LightTreeHost& out) {
     out = LightTreeHost();
     std::vector<PosLight> lights;
-    for (const auto& t : tris) lights.push_back(PosLight{true, t, {}});
+    for (const auto& t : area) lights.push_back(PosLight{true, t, {}});
     for (const auto& p : points) lights.push_back(PosLight{false, {}, p});
     const size_t n = lights.size();
     out.posLightCount = (u32)n;
@@ -240,14 +323,45 @@ void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vect
     size_t memSize = nInt * kNodeBytes;
     for (size_t i = 0; i < n; ++i) {
         offsets[i] = (u32)memSize;
-        types[i] = lights[i].isTri ? kAreaLightTriangle : kPointLight;
-        memSize += lights[i].isTri ? kTriBytes : kPointBytes;
+        const int kind = lights[i].isArea ? lights[i].area.kind : -1;
+        types[i] = kind == kAreaTriangle ? kAreaLightTriangle
+                                         : (kind == kAreaQuad ? kAreaLightQuad
+                                                              : (kind == kAreaSphere ? kAreaLightSphere : kPointLight));
+        memSize += kind == kAreaTriangle ? kTriBytes
+                                         : (kind == kAreaQuad ? kQuadBytes : (kind == kAreaSphere ? kSphereBytes : kPointBytes));
     }
     out.mem.assign(memSize, 0);
     for (size_t i = 0; i < n; ++i) {
         uint8_t* dst = out.mem.data() + offsets[i];
-        if (lights[i].isTri) {
-            const HostAreaTriangle& d = lights[i].tri;
+        if (lights[i].isArea && lights[i].area.kind == kAreaQuad) {
+            // AreaLightQuad::operator= (lights.hpp:112-123)
+            const HostAreaLight& d = lights[i].area;
+            QuadRecord r;
+            std::memset(&r, 0, sizeof(r));
+            for (int k = 0; k < 3; ++k) {
+                r.posV[0][k] = d.points[0][k];
+                r.posV[1][k] = d.points[3][k] - d.points[0][k];
+                r.posV[2][k] = d.points[1][k] - d.points[0][k];
+                r.posV[3][k] = d.points[2][k] - d.points[3][k] - r.posV[2][k];
+            }
+            for (int k = 0; k < 2; ++k) {
+                r.uvV[0][k] = d.uv[0][k];
+                r.uvV[1][k] = d.uv[3][k] - d.uv[0][k];
+                r.uvV[2][k] = d.uv[1][k] - d.uv[0][k];
+                r.uvV[3][k] = d.uv[2][k] - d.uv[3][k] - r.uvV[2][k];
+            }
+            r.material = d.material;
+            std::memcpy(dst, &r, sizeof(r));
+        } else if (lights[i].isArea && lights[i].area.kind == kAreaSphere) {
+            const HostAreaLight& d = lights[i].area;
+            SphereRecord r;
+            std::memset(&r, 0, sizeof(r));
+            std::memcpy(r.position, d.points[0], 12);
+            r.radius = d.radius;
+            r.material = d.material;
+            std::memcpy(dst, &r, sizeof(r));
+        } else if (lights[i].isArea) {
+            const HostAreaLight& d = lights[i].area;
             TriRecord r;
             std::memset(&r, 0, sizeof(r));
             for (int k = 0; k < 3; ++k) {
@@ -331,20 +445,20 @@ void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vect
         const u32 extraNodes = (u32)n - (1u << height);
         const u32 lowerLightCount = (1u << height) - extraNodes;
         u32 i = 0;
-        auto key = [](const HostAreaTriangle& t) { return ((u64)(u32)t.instance << 32) | (u64)(u32)t.primitive; };
+        auto key = [](const HostAreaLight& t) { return ((u64)(u32)t.instance << 32) | (u64)(u32)t.primitive; };
         if (extraNodes > 0) {
             for (; i < lowerLightCount; ++i) {
-                if (!lights[i].isTri) continue;
+                if (!lights[i].isArea) continue;
                 u32 code = ((1u << height) - 1) - (lowerLightCount - i - 1);
                 code <<= (32 - height) & 31;
-                map.push_back({key(lights[i].tri), code});
+                map.push_back({key(lights[i].area), code});
             }
             height += 1;
         }
         for (; i < (u32)n; ++i) {
-            if (!lights[i].isTri) continue;
+            if (!lights[i].isArea) continue;
             const u32 code = (u32)(i - lowerLightCount) << ((32 - height) & 31);
-            map.push_back({key(lights[i].tri), code});
+            map.push_back({key(lights[i].area), code});
         }
     }
     std::sort(map.begin(), map.end());

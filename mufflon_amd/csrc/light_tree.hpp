@@ -1,6 +1,7 @@
 // Host-built light tree of the path tracer, in the byte layout of Mufflon's
 // LightSubTree (light_tree.hpp:56-110): internal 32-byte nodes first, then the
-// light records (AreaLightTriangle 64 B, PointLight 32 B, lights.hpp:46-124).
+// light records (AreaLightTriangle 64 B, AreaLightQuad 96 B, AreaLightSphere
+// 32 B, PointLight 32 B, lights.hpp:46-137).
 // As in Mufflon, the tree is built once on the host per scene and copied to
 // the device verbatim; every device path reads the same bytes.
 #pragma once
@@ -20,12 +21,21 @@ enum LightType : uint16_t {
     kInternalNode = 0xFFFF,
 };
 
-struct HostAreaTriangle {
-    float points[3][3];  // world space
-    float uv[3][2];
-    uint16_t material;
-    int32_t instance, primitive;
+// One emissive primitive of one instance in world space
+// (AreaLightTriangleDesc / AreaLightQuadDesc / AreaLightSphereDesc,
+// lights.hpp:77-91, with the primitive it was made from).
+constexpr int32_t kAreaTriangle = 0;
+constexpr int32_t kAreaQuad = 1;
+constexpr int32_t kAreaSphere = 2;
+struct HostAreaLight {
+    int32_t kind = kAreaTriangle;
+    float points[4][3] = {};  // triangle: 3 vertices, quad: 4, sphere: the center in points[0]
+    float uv[4][2] = {};
+    float radius = 0.0f;      // sphere
+    uint16_t material = 0;
+    int32_t instance = -1, primitive = -1;
 };
+using HostAreaTriangle = HostAreaLight;
 struct HostPointLight {
     float position[3];
     float intensity[3];
@@ -72,7 +82,7 @@ struct BackgroundHost {
 void build_background(int32_t type, const float color[3], const float scale[3], const float* envRgba, int32_t width,
                       int32_t height, const float sceneBox[6], BackgroundHost& out);
 
-void build_light_tree(const std::vector<HostAreaTriangle>& tris, const std::vector<HostPointLight>& points,
+void build_light_tree(const std::vector<HostAreaLight>& area, const std::vector<HostPointLight>& points,
                       const float sceneBox[6], const EmissionLookup& emission, LightTreeHost& out);
 
 }  // namespace mfh

@@ -133,6 +133,154 @@ __device__ __forceinline__ bool ray_triangle(const FastRay& r, V3 v0, V3 v1, V3 
     return true;
 }
 
+// Component k (0, 1, 2) of v, as selects.
+__device__ __forceinline__ float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Real roots x0 <= x1 of a x^2 + b x + c = 0 (the role of ei::solveSquarePoly):
+// q = -(b + sgn(b) sqrt(b^2 - 4ac)) / 2, roots q / a and c / q, the
+// cancellation-free form of Press et al., Numerical Recipes, 3rd ed., 5.6
+// (sgn(0) = 1). a == 0: the root -c / b of the linear equation, twice.
+__device__ __forceinline__ bool solve_square_poly(float a, float b, float c, float& x0, float& x1) {
+    if (a == 0.0f) {
+        if (b == 0.0f) return false;
+        x0 = -c / b;
+        x1 = x0;
+        return true;
+    }
+    const float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f) return false;
+    const float r = __builtin_sqrtf(disc);
+    const float q = b < 0.0f ? -0.5f * (b - r) : -0.5f * (b + r);
+    if (q == 0.0f) {  // b == 0 and c == 0: the double root 0
+        x0 = 0.0f;
+        x1 = 0.0f;
+        return true;
+    }
+    const float ra = q / a, rb = c / q;
+    x0 = rb < ra ? rb : ra;
+    x1 = rb < ra ? ra : rb;
+    return true;
+}
+
+// u of the bilinear patch at a root v: of the two linear equations in u, the
+// one with the larger coefficient (intersection.cu:86-95).
+__device__ __forceinline__ float quad_u(float v, float A1, float A2, float B1, float B2, float C1, float C2, float D1,
+                                        float D2) {
+    const float a = v * A2 + B2;
+    const float b = v * (A2 - A1) + B2 - B1;
+    if (__builtin_fabsf(b) >= __builtin_fabsf(a)) return (v * (C1 - C2) + D1 - D2) / b;
+    return (-v * C2 - D2) / a;
+}
+
+// Ramsey, Potter and Hansen, "Ray Bilinear Patch Intersections" (JGT 2004), in
+// the steps of intersection.cu:99-179: the patch
+//   P(u, v) = p00 + u (p10 - p00) + v (p01 - p00) + uv (p11 - p10 - p01 + p00),
+// the two equations taken along the largest direction component, roots in v,
+// u per root, t along the largest component. t: the nearer of the up to two
+// hits with t > 0 (the first root's when the patch is planar), (u, v) its
+// patch coordinates.
+__device__ __forceinline__ bool ray_quad(const FastRay& r, V3 p00, V3 p10, V3 p11, V3 p01, float& t, float& u,
+                                         float& v) {
+    int d0 = 0, d1 = 1, d2 = 2;
+    const float ax = __builtin_fabsf(r.d.x), ay = __builtin_fabsf(r.d.y), az = __builtin_fabsf(r.d.z);
+    if (ax < ay || ax < az) {
+        if (ay >= az) {
+            d0 = 1;
+            d1 = 0;
+            d2 = 2;
+        } else {
+            d0 = 2;
+            d1 = 1;
+            d2 = 0;
+        }
+    }
+    const V3 a = p11 - p10 - p01 + p00;
+    const V3 b = p10 - p00;
+    const V3 c = p01 - p00;
+    const V3 d = p00 - r.o;
+    const float r0 = comp(r.d, d0), r1 = comp(r.d, d1), r2 = comp(r.d, d2);
+    const float A1 = comp(a, d1) * r0 - comp(a, d0) * r1;
+    const float A2 = comp(a, d2) * r0 - comp(a, d0) * r2;
+    const float B1 = comp(b, d1) * r0 - comp(b, d0) * r1;
+    const float B2 = comp(b, d2) * r0 - comp(b, d0) * r2;
+    const float C1 = comp(c, d1) * r0 - comp(c, d0) * r1;
+    const float C2 = comp(c, d2) * r0 - comp(c, d0) * r2;
+    const float D1 = comp(d, d1) * r0 - comp(d, d0) * r1;
+    const float D2 = comp(d, d2) * r0 - comp(d, d0) * r2;
+    float v0 = 0.0f, v1 = 0.0f;
+    if (!solve_square_poly(A2 * C1 - A1 * C2, A2 * D1 - A1 * D2 + B2 * C1 - B1 * C2, B2 * D1 - B1 * D2, v0, v1))
+        return false;
+    const float ad = comp(a, d0), bd = comp(b, d0), cd = comp(c, d0), dd = comp(d, d0), inv = comp(r.inv, d0);
+    float u0 = 0.0f, u1 = 0.0f, t0 = -1.0f, t1 = -1.0f;
+    if (v0 >= 0.0f && v0 <= 1.0f) {
+        u0 = quad_u(v0, A1, A2, B1, B2, C1, C2, D1, D2);
+        if (u0 >= 0.0f && u0 <= 1.0f) t0 = (u0 * v0 * ad + u0 * bd + v0 * cd + dd) * inv;
+    }
+    if (v1 == v0) {  // one root (planar patch)
+        t = t0;
+        u = u0;
+        v = v0;
+    } else {
+        if (v1 >= 0.0f && v1 <= 1.0f) {
+            u1 = quad_u(v1, A1, A2, B1, B2, C1, C2, D1, D2);
+            if (u1 >= 0.0f && u1 <= 1.0f) t1 = (u1 * v1 * ad + u1 * bd + v1 * cd + dd) * inv;
+        }
+        const bool second = !(t0 > 0.0f) || (t1 > 0.0f && t1 < t0);
+        t = second ? t1 : t0;
+        u = second ? u1 : u0;
+        v = second ? v1 : v0;
+    }
+    return t > 0.0f;
+}
+
+// Ray / sphere after Haines, Guenther and Akenine-Moeller, "Precision
+// Improvements for Ray/Sphere Intersection" (Ray Tracing Gems, 2019): the
+// discriminant from the ray's closest approach l = f + b d (f = o - center,
+// b = -f.d, unit d), roots c / q and q with q = b + sgn(b) sqrt(disc) (the
+// role of ei::intersects(Ray, Sphere)). t: the nearer root when it is
+// positive, else the farther one (origin inside); false when both are <= 0.
+__device__ __forceinline__ bool ray_sphere(const FastRay& r, V3 center, float radius, float& t) {
+    const V3 f = r.o - center;
+    const float b = -dot(f, r.d);
+    const V3 l = f + r.d * b;
+    const float rr = radius * radius;
+    const float disc = rr - dot(l, l);
+    if (disc < 0.0f) return false;
+    const float c = dot(f, f) - rr;
+    const float s = __builtin_sqrtf(disc);
+    const float q = b < 0.0f ? b - s : b + s;
+    if (q == 0.0f) return false;  // then c == 0 too: the only root is t = 0
+    const float ra = c / q;
+    const float tn = ra < q ? ra : q, tf = ra < q ? q : ra;
+    t = tn > 0.0f ? tn : tf;
+    return tf > 0.0f;
+}
+
+// A vector perpendicular to v (the role of ei::perpendicular): the
+// smallest-magnitude component set to zero, the other two swapped with one
+// negated (Hughes and Moeller, "Building an Orthonormal Basis from a Unit
+// Vector", JGT 1999).
+__device__ __forceinline__ V3 perpendicular(V3 v) {
+    const float ax = __builtin_fabsf(v.x), ay = __builtin_fabsf(v.y), az = __builtin_fabsf(v.z);
+    if (ax <= ay && ax <= az) return {0.0f, -v.z, v.y};
+    if (ay <= az) return {-v.z, 0.0f, v.x};
+    return {-v.y, v.x, 0.0f};
+}
+
+// Bilinear interpolation over the corners x00 (s = t = 0), x01 (s = 1),
+// x10 (t = 1), x11: lerp(lerp(x00, x01, s), lerp(x10, x11, s), t) with
+// lerp(a, b, s) = (1 - s) a + s b, which returns the corners exactly (the
+// role of ei::bilerp).
+__device__ __forceinline__ float bilerp(float x00, float x01, float x10, float x11, float s, float t) {
+    const float a = (1.0f - s) * x00 + s * x01;
+    const float b = (1.0f - s) * x10 + s * x11;
+    return (1.0f - t) * a + t * b;
+}
+__device__ __forceinline__ V3 bilerp(V3 x00, V3 x01, V3 x10, V3 x11, float s, float t) {
+    return {bilerp(x00.x, x01.x, x10.x, x11.x, s, t), bilerp(x00.y, x01.y, x10.y, x11.y, s, t),
+            bilerp(x00.z, x01.z, x10.z, x11.z, s, t)};
+}
+
 // 32-byte child record of a collapsed BVH node (two per 64-byte node):
 // the layout of Mufflon's BvhNode (lbvh.hpp:13-18).
 struct __attribute__((packed)) BvhChild {

@@ -101,13 +101,23 @@ struct StageSink {
     int32_t* offsets = nullptr;
 };
 
+// Primitive kinds of a LoD, in primitive-id order (descriptors.hpp:33-68):
+// triangles [0, numTris), quads [numTris, numTris + numQuads), spheres after.
+constexpr int kPrimTriangle = 0;
+constexpr int kPrimQuad = 1;
+constexpr int kPrimSphere = 2;
+
 struct Lod {
-    int32_t numVerts = 0, numTris = 0;
+    int32_t numVerts = 0, numTris = 0, numQuads = 0, numSpheres = 0;
     DevBuf verts;     // float xyz per vertex
     DevBuf tris;      // u32 x3 per triangle
-    // host copies and shading attributes (normals, uvs, material per triangle)
+    DevBuf quads;     // u32 x4 per quad
+    DevBuf spheres;   // float4 per sphere: center xyz, radius
+    // host copies and shading attributes (normals, uvs, material per primitive)
     std::vector<float> hostVerts;
     std::vector<uint32_t> hostTris;
+    std::vector<uint32_t> hostQuads;
+    std::vector<float> hostSpheres;
     std::vector<float> hostUvs;
     std::vector<uint16_t> hostMatIdx;
     bool hasShading = false;
@@ -118,12 +128,19 @@ struct Lod {
     uint64_t generation = 0;  // incremented by every build
     int32_t numInternal = 0;  // internal nodes of the collapsed BVH
     DevBuf nodes;             // numInternal * 64 bytes
-    DevBuf primIds;           // numTris, sorted order
-    DevBuf triVerts;          // numTris * 3 float4 in sorted order: {v0, id}, {v1, 0}, {v2, 0}
+    DevBuf primIds;           // numPrims(), sorted order
+    // 48-byte primitive records in sorted order (three float4): a triangle
+    // {v0, id}, {v1, 0}, {v2, 0}; a quad {v0, id}, {v1, 1}, {v2, 0} with its
+    // fourth vertex in quadV3 at the same position; a sphere {center, id},
+    // {radius, 0, 0, 2}, {0}. The kind is the int bits of the second .w.
+    DevBuf triVerts;
+    DevBuf quadV3;            // float4 per sorted position (LoDs with quads only)
     // traversal hierarchy: the collapsed BVH with its long leaves kept split
     // (travNumInternal = -1: none was, traversal uses nodes)
     DevBuf travNodes;
     int32_t travNumInternal = -1;
+    int32_t numPrims() const { return numTris + numQuads + numSpheres; }
+    bool mixed() const { return numQuads > 0 || numSpheres > 0; }
 };
 
 struct Scene {
@@ -136,6 +153,7 @@ struct Scene {
     float box[6] = {0, 0, 0, 0, 0, 0};
     int32_t tlasNumInternal = 0;
     DevBuf tlasNodes, tlasPrimIds;
+    bool mixed = false;  // an instance's LoD holds quads or spheres
 };
 
 struct PathTracer;
@@ -169,10 +187,11 @@ struct Context {
 struct DevLod {
     const float4* triVerts;
     const void* nodes;
+    const float4* quadV3;  // fourth quad vertices per sorted position (null without quads)
     int32_t numInternal;
     int32_t numPrims;
     float box[6];
-    float coordMax;  // largest |coordinate| of the vertices
+    float coordMax;  // largest |coordinate| of the primitives
     int32_t pad;
 };
 
@@ -211,6 +230,7 @@ struct TraceParams {
     const int32_t* tlasPrimIds;
     int32_t tlasNumInternal;
     float tlasCoordMax;  // largest |coordinate| of the scene box (bounds the instance-level node boxes)
+    int32_t mixed;       // an instance's LoD holds quads or spheres (kernels with kMixed)
 };
 // Traversal over the LoDs' traversal hierarchies, or (reference = true) over
 // the reference BVHs, whose visit counts follow the reference's order.

@@ -33,7 +33,9 @@ constexpr int kPtThreads = kPtTile * kPtTile;
 // traversal's memory latency; at three it spills some state to scratch and
 // the 10M-triangle terrain iteration drops from 27.4 to 14.7 ms (leaf batch
 // 8; 34.8 -> 18.3 ms with 1), two waves 17.6 ms, four 17.6 ms (more spills).
-template <bool kSingle, int kLeaf>
+// kMode: traversal specialization (trace_mode_of: one instance, instance
+// level, or a scene with quads or spheres).
+template <int kMode, int kLeaf>
 __global__ __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_pt_iterate(PtScene S) {
     __shared__ int sStack[kLdsDepth * kPtThreads];
     const int px = blockIdx.x * kPtTile + (threadIdx.x % kPtTile);
@@ -57,8 +59,8 @@ __global__ __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(3)))
                 const u64 x = pcg_next(rng);
                 NeeCandidate c;
                 if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
-                    const TraceHit sh = trace_ray<true, false, kSingle, kPtThreads, kLdsDepth, kLeaf>(
-                        S.trace, c.a, c.dir, c.tmax, lstack);
+                    const TraceHit sh =
+                        trace_mode<kMode, true, kPtThreads, kLdsDepth, kLeaf>(S.trace, c.a, c.dir, c.tmax, lstack);
                     if (sh.inst == -1) add_nee(c, acc);
                 }
             }
@@ -75,13 +77,8 @@ __global__ __launch_bounds__(kPtThreads) __attribute__((amdgpu_waves_per_eu(3)))
         } else if (!walk_surface(vx, x0, x1, rndRoulette, throughput, rayOrigin, dir, pdfF, absorption)) {
             break;
         }
-        TraceHit hit;
-        if constexpr (kSingle)
-            hit = trace_closest_held<kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir, kMaxSceneSize,
-                                                                         lstack);
-        else
-            hit = trace_ray<false, false, false, kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir, kMaxSceneSize,
-                                                                              lstack);
+        const TraceHit hit =
+            trace_mode<kMode, false, kPtThreads, kLdsDepth, kLeaf>(S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
         throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
         footA += 1.0f / fsqrt(pdfF);
         guideWeight = 1.f / (1.f + footA * footA / 1.0f);
@@ -118,15 +115,16 @@ void launch_pt_init_rng(u64* rngs, int num, int seed, hipStream_t stream) {
 
 void launch_pt_iterate(const PtScene& S, int leafBatch, hipStream_t stream) {
     const dim3 grid((S.width + kPtTile - 1) / kPtTile, (S.height + kPtTile - 1) / kPtTile);
-    const bool single = S.trace.numInstances == 1;
-    if (single && leafBatch == 8)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<true, 8>), grid, dim3(kPtThreads), 0, stream, S);
-    else if (single)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<true, 1>), grid, dim3(kPtThreads), 0, stream, S);
-    else if (leafBatch == 8)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<false, 8>), grid, dim3(kPtThreads), 0, stream, S);
-    else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<false, 1>), grid, dim3(kPtThreads), 0, stream, S);
+    const int mode = trace_mode_of(S.trace);
+#define MFH_ITERATE(M, L) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_iterate<M, L>), grid, dim3(kPtThreads), 0, stream, S)
+    if (mode == kTraceSingle) {
+        if (leafBatch == 8) MFH_ITERATE(kTraceSingle, 8); else MFH_ITERATE(kTraceSingle, 1);
+    } else if (mode == kTraceMulti) {
+        if (leafBatch == 8) MFH_ITERATE(kTraceMulti, 8); else MFH_ITERATE(kTraceMulti, 1);
+    } else {
+        if (leafBatch == 8) MFH_ITERATE(kTraceMixed, 8); else MFH_ITERATE(kTraceMixed, 1);
+    }
+#undef MFH_ITERATE
     MFH_HIP(hipGetLastError());
 }
 

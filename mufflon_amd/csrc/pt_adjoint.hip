@@ -113,7 +113,7 @@ __device__ __forceinline__ int texel_weights(const DevTexture& t, float u, float
     return 4;
 }
 
-template <bool kSingle>
+template <int kMode>
 __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointState A) {
     __shared__ int sStack[kLdsDepth * kAdjThreads];
     const int pixel = blockIdx.x * kAdjThreads + threadIdx.x;
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                     const u64 x = pcg_next(rng);
                     NeeCandidate c;
                     if (nee_candidate(S, vx, throughput, guideWeight, i, neeSeed, x, c)) {
-                        const TraceHit sh = trace_ray<true, false, kSingle, kAdjThreads, kLdsDepth, kPtLeafBatch>(
+                        const TraceHit sh = trace_mode<kMode, true, kAdjThreads, kLdsDepth, kPtLeafBatch, false>(
                             S.trace, c.a, c.dir, c.tmax, lstack);
                         if (sh.inst == -1) {
                             AdjVertex& r = rec[pathLen];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_pt_adjoint(PtScene S, AdjointSt
                 r.tp = r.astar == 0 ? tPrime.x : (r.astar == 1 ? tPrime.y : tPrime.z);
                 if (!ok) break;
             }
-            const TraceHit hit = trace_ray<false, false, kSingle, kAdjThreads, kLdsDepth, kPtLeafBatch>(
+            const TraceHit hit = trace_mode<kMode, false, kAdjThreads, kLdsDepth, kPtLeafBatch, false>(
                 S.trace, rayOrigin, dir, kMaxSceneSize, lstack);
             throughput = throughput * transmission(Medium{1.0f, absorption}, hit.inst == -1 ? kMaxSceneSize : hit.t);
             footA += 1.0f / fsqrt(pdfF);
@@ -295,10 +295,13 @@ void launch_pt_adjoint(const PtScene& S, const AdjointState& A, int seed, hipStr
     const int num = S.width * S.height;
     hipLaunchKernelGGL(k_adjoint_rng, dim3((num + 255) / 256), dim3(256), 0, stream, A.rngs, num, seed);
     const dim3 grid((num + kAdjThreads - 1) / kAdjThreads);
-    if (S.trace.numInstances == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_adjoint<true>), grid, dim3(kAdjThreads), 0, stream, S, A);
+    const int mode = trace_mode_of(S.trace);
+    if (mode == kTraceSingle)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_adjoint<kTraceSingle>), grid, dim3(kAdjThreads), 0, stream, S, A);
+    else if (mode == kTraceMulti)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_adjoint<kTraceMulti>), grid, dim3(kAdjThreads), 0, stream, S, A);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_adjoint<false>), grid, dim3(kAdjThreads), 0, stream, S, A);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pt_adjoint<kTraceMixed>), grid, dim3(kAdjThreads), 0, stream, S, A);
     MFH_HIP(hipGetLastError());
 }
 

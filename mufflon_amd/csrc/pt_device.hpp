@@ -35,6 +35,8 @@ constexpr int kTargetRadiance = 0;  // MFH_TARGET_RADIANCE
 
 constexpr uint16_t kLightPoint = 0;
 constexpr uint16_t kLightAreaTriangle = 2;
+constexpr uint16_t kLightAreaQuad = 3;
+constexpr uint16_t kLightAreaSphere = 4;
 constexpr uint16_t kLightInternalNode = 0xFFFF;
 
 // mfh_material of include/mufflon_hip.h (128 bytes)
@@ -73,13 +75,17 @@ struct DevBackground {
     const float* sat;  // width * height summed-area table (ENVMAP)
 };
 
-// Object-space shading attributes of one LoD.
+// Object-space shading attributes of one LoD (primitive ids: triangles,
+// quads, spheres).
 struct DevPtLod {
     const float* verts;     // xyz per vertex
     const uint32_t* tris;   // 3 per triangle
     const float* normals;   // xyz per vertex
     const float* uvs;       // uv per vertex
-    const uint16_t* matIdx; // per triangle
+    const uint16_t* matIdx; // per primitive
+    const uint32_t* quads;  // 4 per quad
+    const float4* spheres;  // center xyz, radius
+    int32_t numTris, numQuads;
 };
 
 struct PtScene {

@@ -197,11 +197,14 @@ PathTracer& tracer_of(Context& c, int32_t id) {
     return *c.tracers[id];
 }
 
-// Area lights of every emissive triangle, per LoD and instance, in world space
-// (world_container.cpp:895-937).
-std::vector<HostAreaTriangle> gather_area_lights(const Context& c, const Scene& sc,
-                                                 const std::vector<PtMaterialRecord>& mats) {
-    std::vector<HostAreaTriangle> out;
+// Area lights of every emissive triangle, quad and sphere, per LoD and
+// instance, in world space (world_container.cpp:895-960): quads keep their
+// vertex order (v1 and v3 swap under a mirroring transform), a sphere's
+// radius scales with the length of the transform's first column
+// (Instance::extract_scale, instance.hpp:27-33).
+std::vector<HostAreaLight> gather_area_lights(const Context& c, const Scene& sc,
+                                              const std::vector<PtMaterialRecord>& mats) {
+    std::vector<HostAreaLight> out;
     for (size_t lodId = 0; lodId < c.lods.size(); ++lodId) {
         const Lod* l = c.lods[lodId];
         if (!l || !l->hasShading) continue;
@@ -236,6 +239,45 @@ std::vector<HostAreaTriangle> gather_area_lights(const Context& c, const Scene& 
                 }
                 out.push_back(t);
             }
+            for (int32_t q = 0; q < l->numQuads; ++q) {
+                const int32_t p = l->numTris + q;
+                const uint16_t mi = l->hostMatIdx[p];
+                if (mats[mi].type != kMatEmissive) continue;
+                HostAreaLight a;
+                a.kind = kAreaQuad;
+                a.material = mi;
+                a.instance = inst;
+                a.primitive = p;
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t vi = l->hostQuads[4 * (size_t)q + k];
+                    const float* v = l->hostVerts.data() + 3 * (size_t)vi;
+                    for (int r = 0; r < 3; ++r)
+                        a.points[k][r] = M[4 * r] * v[0] + M[4 * r + 1] * v[1] + M[4 * r + 2] * v[2] + M[4 * r + 3];
+                    a.uv[k][0] = l->hostUvs[2 * (size_t)vi];
+                    a.uv[k][1] = l->hostUvs[2 * (size_t)vi + 1];
+                }
+                if (mirroring) {
+                    std::swap(a.points[1], a.points[3]);
+                    std::swap(a.uv[1], a.uv[3]);
+                }
+                out.push_back(a);
+            }
+            const float scale = std::sqrt(M[0] * M[0] + M[4] * M[4] + M[8] * M[8]);
+            for (int32_t k = 0; k < l->numSpheres; ++k) {
+                const int32_t p = l->numTris + l->numQuads + k;
+                const uint16_t mi = l->hostMatIdx[p];
+                if (mats[mi].type != kMatEmissive) continue;
+                const float* sp = l->hostSpheres.data() + 4 * (size_t)k;
+                HostAreaLight a;
+                a.kind = kAreaSphere;
+                a.material = mi;
+                a.instance = inst;
+                a.primitive = p;
+                for (int r = 0; r < 3; ++r)
+                    a.points[0][r] = M[4 * r] * sp[0] + M[4 * r + 1] * sp[1] + M[4 * r + 2] * sp[2] + M[4 * r + 3];
+                a.radius = scale * sp[3];
+                out.push_back(a);
+            }
         }
     }
     return out;
@@ -255,6 +297,10 @@ PtScene device_scene(Context& c, PathTracer& t) {
         lods[i].normals = l->normals.as<float>();
         lods[i].uvs = l->uvs.as<float>();
         lods[i].matIdx = l->matIdx.as<uint16_t>();
+        lods[i].quads = l->quads.as<uint32_t>();
+        lods[i].spheres = l->spheres.as<float4>();
+        lods[i].numTris = l->numTris;
+        lods[i].numQuads = l->numQuads;
     }
     upload(c, t.dPtLods, lods.data(), sizeof(DevPtLod) * lods.size());
     std::vector<DevTexture> texs(std::max<size_t>(c.textures.size(), 1));
@@ -483,12 +529,14 @@ int mfh_lod_set_shading(mfh_context* ctx, int32_t lodId, const float* normals, c
     return guarded([&] {
         Context& c = ctx_of(ctx);
         Lod& l = lod_of(c, lodId);
-        MFH_REQUIRE(normals && uvs && materialIndices, "null pointer");
-        l.hostUvs.assign(uvs, uvs + 2 * (size_t)l.numVerts);
-        l.hostMatIdx.assign(materialIndices, materialIndices + (size_t)l.numTris);
-        upload(c, l.normals, normals, 12 * (size_t)l.numVerts);
-        upload(c, l.uvs, uvs, 8 * (size_t)l.numVerts);
-        upload(c, l.matIdx, materialIndices, 2 * (size_t)l.numTris);
+        MFH_REQUIRE(materialIndices && ((normals && uvs) || l.numVerts == 0), "null pointer");
+        if (l.numVerts > 0) {
+            l.hostUvs.assign(uvs, uvs + 2 * (size_t)l.numVerts);
+            upload(c, l.normals, normals, 12 * (size_t)l.numVerts);
+            upload(c, l.uvs, uvs, 8 * (size_t)l.numVerts);
+        }
+        l.hostMatIdx.assign(materialIndices, materialIndices + (size_t)l.numPrims());
+        upload(c, l.matIdx, materialIndices, 2 * (size_t)l.numPrims());
         MFH_HIP(hipStreamSynchronize(c.stream));
         l.hasShading = true;
     });
@@ -524,7 +572,7 @@ int mfh_pt_create(mfh_context* ctx, int32_t sceneId, const mfh_material* materia
                 std::memcpy(pl[i].position, pointLights[i].position, 12);
                 std::memcpy(pl[i].intensity, pointLights[i].intensity, 12);
             }
-            const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t->materials);
+            const std::vector<HostAreaLight> area = gather_area_lights(c, sc, t->materials);
             t->pointLights = pl;
             build_light_tree(area, pl, sc.box, MaterialEmission(t->materials), t->tree);
             const float black[3] = {0.0f, 0.0f, 0.0f}, one[3] = {1.0f, 1.0f, 1.0f};
@@ -779,7 +827,7 @@ int mfh_pt_set_material(mfh_context* ctx, int32_t ptId, int32_t materialIndex, c
         std::memcpy(&t.materials[materialIndex], material, sizeof(mfh_material));
         Scene& sc = scene_of(c, t.scene);
         std::vector<HostPointLight> pl = t.pointLights;
-        const std::vector<HostAreaTriangle> area = gather_area_lights(c, sc, t.materials);
+        const std::vector<HostAreaLight> area = gather_area_lights(c, sc, t.materials);
         build_light_tree(area, pl, sc.box, MaterialEmission(t.materials), t.tree);
         t.tree.backgroundFlux = t.background.fluxSum;
         upload(c, t.dMaterials, t.materials.data(), sizeof(PtMaterialRecord) * t.materials.size());
@@ -924,6 +972,9 @@ int mfh_batch_create(mfh_context* ctx, const int32_t* ptIds, int32_t count, int3
             for (int32_t i = 0; i < count; ++i) {
                 PathTracer& t = tracer_of(c, ptIds[i]);
                 MFH_REQUIRE(t.params.neeCount <= 32, "the wavefront form supports at most 32 NEE samples per vertex");
+                MFH_REQUIRE(!c.scenes[t.scene]->mixed,
+                            "a batch holds scenes of triangles only (scene of path tracer " + std::to_string(ptIds[i]) +
+                                " has quads or spheres)");
                 b->tracers.push_back(ptIds[i]);
                 b->offsets.push_back((int32_t)total);
                 total += (int64_t)t.camera.width * t.camera.height;
@@ -1386,18 +1437,21 @@ int mfh_pt_set_background(mfh_context* ctx, int32_t ptId, const mfh_background* 
     });
 }
 
+namespace {
+int light_tree_build(const std::vector<HostAreaLight>& area, const mfh_point_light* pointLights,
+                     int32_t numPointLights, const float* sceneBox, const mfh_material* materials,
+                     int32_t numMaterials, mfh_light_tree_info* outInfo, void* outMem, size_t memCapacity,
+                     uint64_t* outMapKeys, uint32_t* outMapCodes, int32_t mapCapacity);
+}  // namespace
+
 int mfh_light_tree_build(const mfh_area_light* areaLights, int32_t numAreaLights, const mfh_point_light* pointLights,
                          int32_t numPointLights, const float* sceneBox, const mfh_material* materials,
                          int32_t numMaterials, mfh_light_tree_info* outInfo, void* outMem, size_t memCapacity,
                          uint64_t* outMapKeys, uint32_t* outMapCodes, int32_t mapCapacity) {
     return guarded([&] {
-        MFH_REQUIRE(sceneBox && materials && outInfo, "null pointer");
-        MFH_REQUIRE(numAreaLights >= 0 && numPointLights >= 0, "negative light count");
+        MFH_REQUIRE(numAreaLights >= 0, "negative light count");
         MFH_REQUIRE(numAreaLights == 0 || areaLights, "null area lights");
-        MFH_REQUIRE(numPointLights == 0 || pointLights, "null point lights");
-        std::vector<PtMaterialRecord> mats(numMaterials);
-        std::memcpy(mats.data(), materials, sizeof(mfh_material) * (size_t)numMaterials);
-        std::vector<HostAreaTriangle> area(numAreaLights);
+        std::vector<HostAreaLight> area(numAreaLights);
         for (int32_t i = 0; i < numAreaLights; ++i) {
             const mfh_area_light& a = areaLights[i];
             MFH_REQUIRE(a.material >= 0 && a.material < numMaterials, "material index out of range");
@@ -1410,6 +1464,53 @@ int mfh_light_tree_build(const mfh_area_light* areaLights, int32_t numAreaLights
             area[i].instance = a.instance;
             area[i].primitive = a.primitive;
         }
+        light_tree_build(area, pointLights, numPointLights, sceneBox, materials, numMaterials, outInfo, outMem,
+                         memCapacity, outMapKeys, outMapCodes, mapCapacity);
+    });
+}
+
+int mfh_light_tree_build_ex(const mfh_area_light_ex* areaLights, int32_t numAreaLights,
+                            const mfh_point_light* pointLights, int32_t numPointLights, const float* sceneBox,
+                            const mfh_material* materials, int32_t numMaterials, mfh_light_tree_info* outInfo,
+                            void* outMem, size_t memCapacity, uint64_t* outMapKeys, uint32_t* outMapCodes,
+                            int32_t mapCapacity) {
+    return guarded([&] {
+        MFH_REQUIRE(numAreaLights >= 0, "negative light count");
+        MFH_REQUIRE(numAreaLights == 0 || areaLights, "null area lights");
+        std::vector<HostAreaLight> area(numAreaLights);
+        for (int32_t i = 0; i < numAreaLights; ++i) {
+            const mfh_area_light_ex& a = areaLights[i];
+            MFH_REQUIRE(a.kind >= kAreaTriangle && a.kind <= kAreaSphere, "unknown area light kind");
+            MFH_REQUIRE(a.material >= 0 && a.material < numMaterials, "material index out of range");
+            area[i].kind = a.kind;
+            for (int k = 0; k < 4; ++k) {
+                for (int r = 0; r < 3; ++r) area[i].points[k][r] = a.points[3 * k + r];
+                area[i].uv[k][0] = a.uv[2 * k];
+                area[i].uv[k][1] = a.uv[2 * k + 1];
+            }
+            area[i].radius = a.radius;
+            area[i].material = (uint16_t)a.material;
+            area[i].instance = a.instance;
+            area[i].primitive = a.primitive;
+        }
+        light_tree_build(area, pointLights, numPointLights, sceneBox, materials, numMaterials, outInfo, outMem,
+                         memCapacity, outMapKeys, outMapCodes, mapCapacity);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+int light_tree_build(const std::vector<HostAreaLight>& area, const mfh_point_light* pointLights,
+                     int32_t numPointLights, const float* sceneBox, const mfh_material* materials,
+                     int32_t numMaterials, mfh_light_tree_info* outInfo, void* outMem, size_t memCapacity,
+                     uint64_t* outMapKeys, uint32_t* outMapCodes, int32_t mapCapacity) {
+    {
+        MFH_REQUIRE(sceneBox && materials && outInfo, "null pointer");
+        MFH_REQUIRE(numPointLights >= 0, "negative light count");
+        MFH_REQUIRE(numPointLights == 0 || pointLights, "null point lights");
+        std::vector<PtMaterialRecord> mats(numMaterials);
+        std::memcpy(mats.data(), materials, sizeof(mfh_material) * (size_t)numMaterials);
         std::vector<HostPointLight> pl(numPointLights);
         for (int32_t i = 0; i < numPointLights; ++i) {
             std::memcpy(pl[i].position, pointLights[i].position, 12);
@@ -1431,7 +1532,7 @@ int mfh_light_tree_build(const mfh_area_light* areaLights, int32_t numAreaLights
         if (outMapKeys && !tree.mapKeys.empty()) std::memcpy(outMapKeys, tree.mapKeys.data(), 8 * tree.mapKeys.size());
         if (outMapCodes && !tree.mapCodes.empty())
             std::memcpy(outMapCodes, tree.mapCodes.data(), 4 * tree.mapCodes.size());
-    });
+    }
+    return 0;
 }
-
-}  // extern "C"
+}  // namespace

@@ -25,13 +25,85 @@ __device__ __forceinline__ V3 add_epsilon(V3 o, V3 dir, V3 geoN) {
     return dot(geoN, dir) >= 0.0f ? o + off : o - off;
 }
 
+__device__ __forceinline__ V3 lod_vert(const DevPtLod& L, u32 i) {
+    return v3(L.verts[3 * (size_t)i], L.verts[3 * (size_t)i + 1], L.verts[3 * (size_t)i + 2]);
+}
+__device__ __forceinline__ V3 lod_normal(const DevPtLod& L, u32 i) {
+    return v3(L.normals[3 * (size_t)i], L.normals[3 * (size_t)i + 1], L.normals[3 * (size_t)i + 2]);
+}
+
+// Hit attributes of a sphere (intersection.cu:524-541; the sphere of
+// primitive p is sphere p - (triangles + quads) of the LoD) and its frame
+// (tangent_space.hpp:14-21: the geometric one). position: the world hit.
+__device__ TS sphere_frame(const DevPtLod& L, const M34& M, int prim, V3 position, float& hu, float& hv) {
+    const float4 sp = L.spheres[prim - L.numTris - L.numQuads];
+    const V3 n = normalize(xform_point(M, position) - v3(sp.x, sp.y, sp.z));
+    V3 tangentX = v3(1.0f, 0.0f, 0.0f);
+    if (!(n.x == 0.0f && n.y == 0.0f)) {
+        const float l2 = fsqrt(n.y * n.y + n.x * n.x);
+        tangentX = v3(n.y / l2, -n.x / l2, 0.0f);
+    }
+    hu = fatan2(n.z, n.x) / (2.0f * kPi) + 0.5f;
+    hv = facos(-n.y) / kPi;
+    TS ts;
+    ts.geoN = normalize(xform_normal(M, n));
+    ts.shadingN = ts.geoN;
+    ts.shadingTX = normalize(xform_normal(M, tangentX));
+    ts.shadingTY = cross(ts.geoN, ts.shadingTX);
+    return ts;
+}
+
+// Hit attributes of a bilinear quad at patch coordinates (su, sv)
+// (intersection.cu:572-610) and its frame (tangent_space.hpp:36-47, 50-60).
+__device__ TS quad_frame(const DevPtLod& L, const M34& M, int prim, float su, float sv, float& hu, float& hv) {
+    const uint32_t* q = L.quads + 4 * (size_t)(prim - L.numTris);
+    const u32 i0 = q[0], i1 = q[1], i2 = q[2], i3 = q[3];
+    const V3 v0 = lod_vert(L, i0), v1 = lod_vert(L, i1), v2 = lod_vert(L, i2), v3v = lod_vert(L, i3);
+    const float* uv = L.uvs;
+    const float u0x = uv[2 * (size_t)i0], u0y = uv[2 * (size_t)i0 + 1];
+    const float u1x = uv[2 * (size_t)i1], u1y = uv[2 * (size_t)i1 + 1];
+    const float u2x = uv[2 * (size_t)i2], u2y = uv[2 * (size_t)i2 + 1];
+    const float u3x = uv[2 * (size_t)i3], u3y = uv[2 * (size_t)i3 + 1];
+    const V3 dxds = smul(1.f - su, v3v - v0) + smul(su, v2 - v1);
+    const V3 dxdt = smul(1.f - sv, v1 - v0) + smul(sv, v2 - v3v);
+    const float dudsx = (1.f - su) * (u3x - u0x) + su * (u2x - u1x);
+    const float dudsy = (1.f - su) * (u3y - u0y) + su * (u2y - u1y);
+    const float dudtx = (1.f - sv) * (u1x - u0x) + sv * (u2x - u3x);
+    const float dudty = (1.f - sv) * (u1y - u0y) + sv * (u2y - u3y);
+    const float det = dudsx * dudty - dudtx * dudsy;
+    V3 tangentX;
+    if (det >= 1e-9f || det <= -1e-9f) {
+        const float a = dudty / det, b = -dudsy / det;
+        tangentX = v3(dxds.x * a + dxdt.x * b, dxds.y * a + dxdt.y * b, dxds.z * a + dxdt.z * b);
+    } else {
+        tangentX = dxds;
+    }
+    const V3 geoN0 = cross(dxdt, dxds);
+    hu = bilerp(u0x, u1x, u3x, u2x, su, sv);
+    hv = bilerp(u0y, u1y, u3y, u2y, su, sv);
+    TS ts;
+    ts.geoN = normalize(xform_normal(M, geoN0));
+    tangentX = normalize(xform_normal(M, tangentX));
+    V3 sn = bilerp(lod_normal(L, i0), lod_normal(L, i1), lod_normal(L, i3), lod_normal(L, i2), su, sv);
+    sn = normalize(xform_normal(M, sn));
+    ts.shadingN = sn;
+    ts.shadingTX = normalize(tangentX - sn * dot(tangentX, sn));
+    ts.shadingTY = cross(sn, ts.shadingTX);
+    return ts;
+}
+
 // Hit attributes of first_intersection (intersection.cu:524-621): shading
 // frame of tangent_space_geom_to_shader (tangent_space.hpp:10-60) and the
-// interpolated uv.
-__device__ __forceinline__ TS hit_frame(const PtScene& S, int inst, int prim, float w0, float w1, float& hu,
-                                        float& hv) {
+// interpolated uv. position: the world-space hit point (spheres use it).
+__device__ __forceinline__ TS hit_frame(const PtScene& S, int inst, int prim, float w0, float w1, V3 position,
+                                        float& hu, float& hv) {
     const int lod = S.trace.instLod[inst];
     const DevPtLod L = S.ptLods[lod];
+    if (prim >= L.numTris) {  // only LoDs with quads or spheres have primitives past their triangles
+        const M34 M = load_m34(S.trace.worldToInst + 12 * (size_t)inst);
+        if (prim >= L.numTris + L.numQuads) return sphere_frame(L, M, prim, position, hu, hv);
+        return quad_frame(L, M, prim, w0, w1, hu, hv);
+    }
     const u32 i0 = L.tris[3 * (size_t)prim], i1 = L.tris[3 * (size_t)prim + 1], i2 = L.tris[3 * (size_t)prim + 2];
     const V3 v0 = v3(L.verts[3 * (size_t)i0], L.verts[3 * (size_t)i0 + 1], L.verts[3 * (size_t)i0 + 2]);
     const V3 v1 = v3(L.verts[3 * (size_t)i1], L.verts[3 * (size_t)i1 + 1], L.verts[3 * (size_t)i1 + 2]);
@@ -108,6 +180,7 @@ struct Vertex {
     MatParams mat;
     float incidentPdf;
     float hu, hv;
+    float sp0, sp1;  // surface parameters of the hit (a quad's (u, v))
     __device__ __forceinline__ V3 incident() const {
         const float iz = sgnf(incEnc.z) * fsqrt(clampf(1.0f - incEnc.x * incEnc.x - incEnc.y * incEnc.y, 0.0f, 1.0f));
         return v3(incEnc.x, incEnc.y, iz);
@@ -124,6 +197,8 @@ __device__ __forceinline__ void make_vertex(const PtScene& S, V3 rayOrigin, V3 d
     vx.prim = prim;
     vx.hu = 0.0f;
     vx.hv = 0.0f;
+    vx.sp0 = 0.0f;
+    vx.sp1 = 0.0f;
     if (vx.isVoid) {
         vx.incEnc = dir;
         vx.incidentPdf = pdfF * 1.0f;
@@ -131,7 +206,9 @@ __device__ __forceinline__ void make_vertex(const PtScene& S, V3 rayOrigin, V3 d
         return;
     }
     vx.position = rayOrigin + dir * t;
-    vx.ts = hit_frame(S, inst, prim, w0, w1, vx.hu, vx.hv);
+    vx.sp0 = w0;
+    vx.sp1 = w1;
+    vx.ts = hit_frame(S, inst, prim, w0, w1, vx.position, vx.hu, vx.hv);
     vx.incEnc = v3(dir.x, dir.y, t * sgnf(dir.z));
     vx.mat = fetch(S, material_index(S, inst, prim), vx.hu, vx.hv);
     const float inCosAbs = fabsf(dot(dir, vx.ts.shadingN));
@@ -151,7 +228,7 @@ __device__ __forceinline__ V3 emission_contribution(const PtScene& S, const Vert
         if (nonzero(em)) connectPdf = background_pdf(S, pdfBack);
     } else {
         em = emission(vx.mat, vx.ts.geoN, neg(vx.incident()));
-        if (nonzero(em)) connectPdf = light_pdf(S, vx.inst, vx.prim, prevPos);
+        if (nonzero(em)) connectPdf = light_pdf(S, vx.inst, vx.prim, prevPos, vx.sp0, vx.sp1);
     }
     if (nonzero(em) && pathLen > 1) {
         mis = 1.0f / (1.0f + (float)S.neeCount * (connectPdf / vx.incidentPdf));

@@ -685,8 +685,116 @@ struct Nee {
 
 __device__ __forceinline__ float4 ld4(const uint8_t* p) { return *reinterpret_cast<const float4*>(p); }
 
-// connect_light for point lights (light_sampling.hpp:308-327, vacuum) and
-// area triangles (:118-140, :350-375).
+// AreaLightQuad record (lights.hpp:108-124, 96 bytes): posV[4] = v0, v3 - v0,
+// v1 - v0, v2 - v3 - (v1 - v0) at 0, uvV[4] at 48, material at 80.
+__device__ __forceinline__ void quad_record(const uint8_t* light, V3 pv[4]) {
+    const float4 q0 = ld4(light), q1 = ld4(light + 16), q2 = ld4(light + 32);
+    pv[0] = v3(q0.x, q0.y, q0.z);
+    pv[1] = v3(q0.w, q1.x, q1.y);
+    pv[2] = v3(q1.z, q1.w, q2.x);
+    pv[3] = v3(q2.y, q2.z, q2.w);
+}
+
+// sample_light_pos + connect_light of a quad (light_sampling.hpp:142-171,
+// 376-399): (u0, u1) are the patch coordinates of the sample.
+__device__ Nee connect_quad(const PtScene& S, const uint8_t* light, V3 pos, float u0, float u1) {
+    V3 pv[4];
+    quad_record(light, pv);
+    const float4 w0 = ld4(light + 48), w1 = ld4(light + 64);
+    const int matIdx = (int)(*reinterpret_cast<const uint16_t*>(light + 80));
+    const V3 tangentX = pv[1] + smul(u0, pv[3]);
+    const V3 tangentY = pv[2] + smul(u1, pv[3]);
+    V3 normal = cross(tangentY, tangentX);
+    const float area = len(normal);
+    normal = normal / area;
+    const V3 position = pv[0] + tangentX * u1 + pv[2] * u0;
+    // uvV[0..3]: (w0.x, w0.y), (w0.z, w0.w), (w1.x, w1.y), (w1.z, w1.w)
+    const float lu = w0.x + w0.z * u1 + w1.x * u0 + w1.z * (u0 * u1);
+    const float lv = w0.y + w0.w * u1 + w1.y * u0 + w1.w * (u0 * u1);
+    const MatParams m = fetch(S, matIdx, lu, lv);
+    const V3 intensity = emission(m, v3(0, 0, 1), v3(0, 0, 1)) * area;
+    V3 d = position - pos;
+    const float distSq = dot(d, d) + kDistSqEps;
+    const float dist = fsqrt(distSq);
+    d = d / dist;
+    const float cosOut = dot(normal, neg(d));
+    V3 value = v3(0, 0, 0);
+    float vc = 0.0f;
+    if (cosOut > 0.0f) {
+        value = intensity;
+        vc = cosOut;
+    }
+    Nee n;
+    n.position = position;
+    n.creationPdf = 1.0f / area;
+    n.dir = d;
+    n.cosOut = vc;
+    n.diffIrradiance = v3(value.x / distSq, value.y / distSq, value.z / distSq) * transmission(m.outer, dist);
+    n.geoNormal = normal;
+    n.distSq = distSq;
+    n.lightMaterial = matIdx;
+    return n;
+}
+
+// connect_light of a sphere (light_sampling.hpp:400-432): a point of the cap
+// visible from pos, uniform in area (a uniform cone of directions from the
+// center, sample_cone_uniform, sampling.hpp:112-119). Record
+// (AreaLightSphere, lights.hpp:125-137): center, radius, material at 16.
+__device__ Nee connect_sphere(const PtScene& S, const uint8_t* light, V3 pos, float u0, float u1) {
+    const float4 q0 = ld4(light);
+    const int matIdx = (int)(*reinterpret_cast<const uint16_t*>(light + 16));
+    const V3 c = v3(q0.x, q0.y, q0.z);
+    const float radius = q0.w;
+    Nee n;
+    n.lightMaterial = matIdx;
+    V3 centerDir = pos - c;
+    const float cd = len(centerDir);
+    if (cd <= radius) {  // inside: no estimate
+        n.position = v3(0, 0, 0);
+        n.creationPdf = 0.0f;
+        n.dir = v3(0, 0, 0);
+        n.cosOut = 0.0f;
+        n.diffIrradiance = v3(0, 0, 0);
+        n.geoNormal = v3(0, 0, 0);
+        n.distSq = 0.0f;
+        return n;
+    }
+    centerDir = centerDir / cd;
+    const float cosSphere = radius / cd;
+    const float cosTheta = (1.f - u0) + u0 * cosSphere;
+    const float sinTheta = fsqrt(1.f - cosTheta * cosTheta);
+    const float phi = 2.f * u1 * kPi;
+    const float lx = fcos(phi) * sinTheta, ly = fsin(phi) * sinTheta;
+    const float conePdf = 1.f / (2.f * kPi * (1.f - cosSphere));
+    const V3 tangentX = normalize(perpendicular(centerDir));
+    const V3 tangentY = cross(centerDir, tangentX);
+    const V3 globalDir = smul(lx, tangentX) + smul(ly, tangentY) + smul(cosTheta, centerDir);
+    const V3 surfPos = c + smul(radius, globalDir);
+    V3 cdir = surfPos - pos;
+    const float cDistSq = dot(cdir, cdir) + kDistSqEps;
+    const float cDist = fsqrt(cDistSq);
+    cdir = cdir / cDist;
+    // direction_to_uv (textures/interface.hpp:169-174)
+    const float tv = facos(globalDir.y) / kPi;
+    const float tu = fatan2(globalDir.z, globalDir.x) / (kPi * 2.0f);
+    const MatParams m = fetch(S, matIdx, tu, 1.0f - tv);
+    V3 radiance = emission(m, v3(0, 0, 1), v3(0, 0, 1));
+    const float sampleArea = radius * radius / conePdf;
+    const float cosOut = mx(0.0f, -dot(globalDir, cdir));
+    radiance = radiance * (sampleArea / cDistSq);
+    radiance = radiance * transmission(m.outer, cDist);
+    n.position = surfPos;
+    n.creationPdf = 1.0f / sampleArea;
+    n.dir = cdir;
+    n.cosOut = cosOut;
+    n.diffIrradiance = radiance;
+    n.geoNormal = globalDir;
+    n.distSq = cDistSq;
+    return n;
+}
+
+// connect_light for point lights (light_sampling.hpp:308-327, vacuum),
+// area triangles (:118-140, :350-375), quads and spheres (above).
 __device__ Nee connect_light(const PtScene& S, uint16_t type, u32 offset, V3 pos, float u0, float u1) {
     const uint8_t* light = S.lightMem + offset;
     Nee n;
@@ -707,6 +815,8 @@ __device__ Nee connect_light(const PtScene& S, uint16_t type, u32 offset, V3 pos
         n.lightMaterial = -1;
         return n;
     }
+    if (type == kLightAreaQuad) return connect_quad(S, light, pos, u0, u1);
+    if (type == kLightAreaSphere) return connect_sphere(S, light, pos, u0, u1);
     const float4 q0 = ld4(light), q1 = ld4(light + 16), q2 = ld4(light + 32), q3 = ld4(light + 48);
     const V3 p0 = v3(q0.x, q0.y, q0.z), e1 = v3(q0.w, q1.x, q1.y), e2 = v3(q1.z, q1.w, q2.x);
     const int matIdx = (int)(__float_as_uint(q2.y) & 0xFFFFu);
@@ -756,7 +866,12 @@ __device__ __forceinline__ V3 light_center(const PtScene& S, u32 offset, uint16_
         return v3(b.y, b.z, b.w);
     }
     const float4 q0 = ld4(p);
-    if (type == kLightPoint) return v3(q0.x, q0.y, q0.z);
+    if (type == kLightPoint || type == kLightAreaSphere) return v3(q0.x, q0.y, q0.z);
+    if (type == kLightAreaQuad) {  // posV[0] + (posV[1] + posV[2]) / 2 + posV[3] / 4
+        V3 pv[4];
+        quad_record(p, pv);
+        return pv[0] + (pv[1] + pv[2]) * 0.5f + pv[3] * 0.25f;
+    }
     const float4 q1 = ld4(p + 16), q2 = ld4(p + 32);
     const V3 p0 = v3(q0.x, q0.y, q0.z), e1 = v3(q0.w, q1.x, q1.y), e2 = v3(q1.z, q1.w, q2.x);
     return p0 + (e1 + e2) / 3.0f;
@@ -833,8 +948,9 @@ __device__ Nee connect(const PtScene& S, u64 index, u64 numIndices, u64 seed, V3
 }
 
 // light_pdf (light_tree_sampling.hpp:340-413): connect pdf of a hit emitter
-// seen from refPos (the previous vertex of the path).
-__device__ float light_pdf(const PtScene& S, int inst, int prim, V3 refPos) {
+// seen from refPos (the previous vertex of the path); (sp0, sp1): the hit's
+// surface parameters (a quad's (u, v)).
+__device__ float light_pdf(const PtScene& S, int inst, int prim, V3 refPos, float sp0, float sp1) {
     const float treeFlux = S.dirRootFlux + S.posRootFlux + S.bgFlux;
     float p = S.posRootFlux / treeFlux;
     const u64 key = ((u64)(u32)inst << 32) | (u64)(u32)prim;
@@ -869,6 +985,19 @@ __device__ float light_pdf(const PtScene& S, int inst, int prim, V3 refPos) {
             offset = __float_as_uint(a.y);
         }
         code <<= 1;
+    }
+    if (type == kLightAreaQuad) {  // the local area density at (sp0, sp1)
+        V3 pv[4];
+        quad_record(S.lightMem + offset, pv);
+        const V3 tangentX = pv[1] + smul(sp0, pv[3]);
+        const V3 tangentY = pv[2] + smul(sp1, pv[3]);
+        return p / len(cross(tangentY, tangentX));
+    }
+    if (type == kLightAreaSphere) {  // uniform over the cap visible from refPos
+        const float4 q = ld4(S.lightMem + offset);
+        const float cosSphere = q.w / len(refPos - v3(q.x, q.y, q.z));
+        const float solidAngle = 2 * kPi * (1.0f - cosSphere);
+        return p / (solidAngle * (q.w * q.w));
     }
     const float4 q0 = ld4(S.lightMem + offset), q1 = ld4(S.lightMem + offset + 16);
     const V3 e1 = v3(q0.w, q1.x, q1.y), e2 = v3(q1.z, q1.w, *reinterpret_cast<const float*>(S.lightMem + offset + 32));

@@ -136,22 +136,6 @@ __device__ __forceinline__ int locate(const WfState& W, int p, int& local) {
     return 0;
 }
 
-// Traversal whose instance-level specialization is fixed (kSingle) or chosen
-// per lane from the ray's scene (batches mix single- and multi-instance scenes).
-template <int kMode, bool kAny, int kStride, int kLds, int kLeaf>
-__device__ __forceinline__ TraceHit trace_mode(const TraceParams& tp, V3 o, V3 d, float tmax, int* lstack) {
-    // closest hits of single-instance scenes hold leaves (trace_closest_held)
-    if constexpr (kMode == 1 && !kAny)
-        return trace_closest_held<kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-    if constexpr (kMode == 1) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-    if constexpr (kMode == 0) return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-    if (tp.numInstances == 1) {
-        if constexpr (!kAny) return trace_closest_held<kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-        return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-    }
-    return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
-}
-
 __global__ void k_wf_begin(WfState W) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < W.numCounters) W.counters[i] = 0;
@@ -305,17 +289,12 @@ __device__ __forceinline__ int refill_slot(int* work, int n, unsigned long long 
     return slot < n ? slot : -1;
 }
 
-template <int kMode>
-constexpr int traversal_mode() {
-    return kMode == 1 ? 1 : (kMode == 0 ? 0 : 2);
-}
-
 template <int kMode, int kBlock, int kLds>
 __global__ __launch_bounds__(kBlock) void k_wf_extend_refill(PtScene S0, WfState W, int buf, int count, int work,
                                                              const int* perm, int budget) {
     __shared__ int sStack[kLds * kBlock];
     constexpr bool kBatch = kMode == 2;
-    using T = Traversal<false, false, traversal_mode<kMode>(), kBlock, kLds>;
+    using T = Traversal<false, false, traversal_mode<kMode>(), kBlock, kLds, kMode == kTraceMixed>;
     const int n = W.counters[count];
     int* const lstack = sStack + threadIdx.x;
     int spill[T::kSpill];
@@ -350,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shadow_refill(PtScene S0, WfState
                                                              const int* perm, int budget) {
     __shared__ int sStack[kLds * kBlock];
     constexpr bool kBatch = kMode == 2;
-    using T = Traversal<true, false, traversal_mode<kMode>(), kBlock, kLds>;
+    using T = Traversal<true, false, traversal_mode<kMode>(), kBlock, kLds, kMode == kTraceMixed>;
     const int n = W.counters[count];
     int* const lstack = sStack + threadIdx.x;
     int spill[T::kSpill];
@@ -702,8 +681,8 @@ constexpr TraceVariant variant() {
 
 // Tuning space of the trace stages: block size x LDS stack depth x leaf
 // batch (1 or 8 triangles loaded ahead; the lane-refill kernels test one at a
-// time), for multi-instance scenes (0), single-instance scenes (1) and
-// batches (2).
+// time), for multi-instance scenes (0), single-instance scenes (1), batches
+// (2) and scenes with quads or spheres (3).
 template <int kMode, int kLeaf>
 constexpr std::array<TraceVariant, 7> shapes() {
     return {variant<kMode, 64, 8, kLeaf>(),   variant<kMode, 64, 16, kLeaf>(),  variant<kMode, 128, 8, kLeaf>(),
@@ -720,7 +699,8 @@ std::array<TraceVariant, kWfNumTraceVariants> variants() {
     }
     return v;
 }
-const std::array<TraceVariant, kWfNumTraceVariants> kVariants[3] = {variants<0>(), variants<1>(), variants<2>()};
+const std::array<TraceVariant, kWfNumTraceVariants> kVariants[4] = {variants<0>(), variants<1>(), variants<2>(),
+                                                                     variants<3>()};
 
 }  // namespace
 
@@ -729,7 +709,7 @@ const WfTraceShape kWfTraceShapes[kWfNumTraceVariants] = {
     {64, 8, 8}, {64, 16, 8}, {128, 8, 8}, {128, 16, 8}, {256, 8, 8}, {256, 16, 8}, {256, 24, 8}};
 
 int wf_grid(const PtScene& S, const WfTuning& tune, int numCUs) {
-    const TraceVariant& v = kVariants[S.trace.numInstances == 1 ? 1 : 0][tune.traceVariant];
+    const TraceVariant& v = kVariants[trace_mode_of(S.trace)][tune.traceVariant];
     return v.occupancy(tune.refill > 0) * numCUs;
 }
 
@@ -745,11 +725,12 @@ int wf_shade_grid(int numCUs) {
 }
 
 int wf_tail_grid(int numCUs) {
-    int a = 0, b = 0, c = 0;
+    int a = 0, b = 0, c = 0, d = 0;
     MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_tail<0, 1>, 256, 0));
     MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_tail<1, 8>, 256, 0));
     MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_wf_tail<2, 1>, 256, 0));
-    return std::max(1, std::min(a, std::min(b, c))) * numCUs;
+    MFH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_wf_tail<3, 8>, 256, 0));
+    return std::max(1, std::min(std::min(a, d), std::min(b, c))) * numCUs;
 }
 
 namespace {
@@ -761,8 +742,10 @@ void launch_tail(int mode, bool leaf8, const PtScene& S, const WfState& W, int g
         if (leaf8) MFH_TAIL(0, 8); else MFH_TAIL(0, 1);
     } else if (mode == 1) {
         if (leaf8) MFH_TAIL(1, 8); else MFH_TAIL(1, 1);
-    } else {
+    } else if (mode == 2) {
         if (leaf8) MFH_TAIL(2, 8); else MFH_TAIL(2, 1);
+    } else {
+        if (leaf8) MFH_TAIL(3, 8); else MFH_TAIL(3, 1);
     }
 #undef MFH_TAIL
     MFH_HIP(hipGetLastError());
@@ -779,19 +762,20 @@ void sort_queue(const PtScene& S, const WfState& W, int buf, int count, bool sha
 
 void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tune, BuildScratch& scratch,
                          hipStream_t stream) {
-    const bool single = S.trace.numInstances == 1;
-    const TraceVariant& v = kVariants[single ? 1 : 0][tune.traceVariant];
+    const int mode = trace_mode_of(S.trace);
+    const TraceVariant& v = kVariants[mode][tune.traceVariant];
     hipLaunchKernelGGL(k_wf_begin, dim3((W.numCounters + 255) / 256), dim3(256), 0, stream, W);
     const dim3 pgrid((S.width + 15) / 16, (S.height + 15) / 16);
     const bool leaf8 = kWfTraceShapes[tune.traceVariant].leafBatch == 8;
-    if (single && leaf8)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<1, 8>), pgrid, dim3(256), 0, stream, S, W);
-    else if (single)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<1, 1>), pgrid, dim3(256), 0, stream, S, W);
-    else if (leaf8)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<0, 8>), pgrid, dim3(256), 0, stream, S, W);
-    else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<0, 1>), pgrid, dim3(256), 0, stream, S, W);
+#define MFH_PRIMARY(M, L) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wf_primary<M, L>), pgrid, dim3(256), 0, stream, S, W)
+    if (mode == kTraceSingle) {
+        if (leaf8) MFH_PRIMARY(1, 8); else MFH_PRIMARY(1, 1);
+    } else if (mode == kTraceMulti) {
+        if (leaf8) MFH_PRIMARY(0, 8); else MFH_PRIMARY(0, 1);
+    } else {
+        if (leaf8) MFH_PRIMARY(3, 8); else MFH_PRIMARY(3, 1);
+    }
+#undef MFH_PRIMARY
     // pt_sample's loop is a do-while: the first hit is shaded even at maximum path length 0
     const int L = S.maxPathLength > 1 ? S.maxPathLength : 1;
     int work = W.workBase;
@@ -807,8 +791,7 @@ void launch_pt_wavefront(const PtScene& S, const WfState& W, const WfTuning& tun
             v.extend(S, W, tune.traceGrid, buf ^ 1, W.qBase + k + 1, work++, sortNow ? W.sortIds[0] : nullptr,
                      tune.refill, stream);
             if (tune.tailAfter > 0 && k + 1 == tune.tailAfter) {  // the rest of every path in one launch
-                launch_tail(single ? 1 : 0, leaf8, S, W, tune.tailGrid, k + 1, buf ^ 1, W.qBase + k + 1, work++,
-                            stream);
+                launch_tail(mode, leaf8, S, W, tune.tailGrid, k + 1, buf ^ 1, W.qBase + k + 1, work++, stream);
                 break;
             }
         }

@@ -11,7 +11,8 @@ constexpr int kThreads = kTraceThreads;
 // kAny: stop at the first accepted hit. kStats: count per ray the internal
 // nodes visited (an object-box test at instance entry counts as one) and the
 // triangles tested. kSingle: the scene has one instance (no instance level).
-template <bool kAny, bool kStats, bool kSingle>
+// kMixed: the scene's LoDs hold quads or spheres.
+template <bool kAny, bool kStats, bool kSingle, bool kMixed>
 __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4* __restrict__ rays, int n,
                                                    float4* __restrict__ hits, float2* __restrict__ bary,
                                                    int* __restrict__ occluded, uint2* __restrict__ stats) {
@@ -21,11 +22,12 @@ __global__ __launch_bounds__(kThreads) void k_trace(TraceParams p, const float4*
     const float4 r0 = rays[2 * (size_t)i], r1 = rays[2 * (size_t)i + 1];
     TraceHit h;
     if constexpr (kSingle && !kAny && !kStats)
-        h = trace_closest_held<kThreads>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w,
-                                                  sStack + threadIdx.x);
+        h = trace_closest_held<kThreads, kLdsDepth, 1, kMixed>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w,
+                                                               sStack + threadIdx.x);
     else
-        h = trace_ray<kAny, kStats, kSingle, kThreads>(p, v3(r0.x, r0.y, r0.z), v3(r1.x, r1.y, r1.z), r0.w,
-                                                       sStack + threadIdx.x);
+        h = trace_ray<kAny, kStats, kSingle, kThreads, kLdsDepth, 1, kMixed>(p, v3(r0.x, r0.y, r0.z),
+                                                                           v3(r1.x, r1.y, r1.z), r0.w,
+                                                                           sStack + threadIdx.x);
     if (kAny) {
         occluded[i] = h.overflow ? -1 : (h.inst != -1 ? 1 : 0);
     } else {
@@ -40,12 +42,18 @@ void launch_trace(const TraceParams& p, const float* rays, int n, float4* hits, 
                   hipStream_t stream) {
     const dim3 grid((n + kThreads - 1) / kThreads), block(kThreads);
     const float4* r = reinterpret_cast<const float4*>(rays);
-    if (p.numInstances == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, true>), grid, block, 0, stream, p, r, n, hits, bary,
-                           occ, stats);
-    else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, false>), grid, block, 0, stream, p, r, n, hits,
+    if (p.numInstances == 1 && p.mixed)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, true, true>), grid, block, 0, stream, p, r, n, hits,
                            bary, occ, stats);
+    else if (p.numInstances == 1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, true, false>), grid, block, 0, stream, p, r, n, hits,
+                           bary, occ, stats);
+    else if (p.mixed)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, false, true>), grid, block, 0, stream, p, r, n, hits,
+                           bary, occ, stats);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_trace<kAny, kStats, false, false>), grid, block, 0, stream, p, r, n,
+                           hits, bary, occ, stats);
     MFH_HIP(hipGetLastError());
 }
 
@@ -62,6 +70,7 @@ TraceParams trace_params(const Scene& sc, bool reference) {
     p.tlasNumInternal = sc.tlasNumInternal;
     p.tlasCoordMax = 0.0f;
     for (int k = 0; k < 6; ++k) p.tlasCoordMax = std::max(p.tlasCoordMax, std::fabs(sc.box[k]));
+    p.mixed = sc.mixed ? 1 : 0;
     return p;
 }
 

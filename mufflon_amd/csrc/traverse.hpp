@@ -5,8 +5,10 @@
 // One ray per lane. The stack keeps its lowest kLdsDepth entries in LDS (one
 // column per lane, so the 64 lanes of a wave touch 64 consecutive dwords) and
 // spills deeper entries to a per-lane private array. Nodes are Mufflon's
-// 64-byte collapsed LBVH nodes, read as four 16-byte loads; the triangles of a
-// leaf are contiguous 48-byte records in BVH order ({v0, primId}, {v1}, {v2}).
+// 64-byte collapsed LBVH nodes, read as four 16-byte loads; the primitives of
+// a leaf are contiguous 48-byte records in BVH order ({v0, primId}, {v1, kind},
+// {v2} for triangles and quads, {center, primId}, {radius, .., kind} for
+// spheres).
 //
 // Visiting order and box test are those of the reference CPU path: the slab
 // test of intersection.cu:57-75 (computed without its early outs, which give
@@ -111,6 +113,50 @@ __device__ __forceinline__ void slab_pair(bool fast, const float* lLo, const flo
     }
 }
 
+// Quad or sphere of a mixed LoD (intersects_primitve, intersection.cu:220-300,
+// without alpha testing): the record (a, b, c) at sorted position pos, the
+// quad's fourth vertex from quadV3.
+struct PrimHit {
+    float t, w0, w1;
+    int hit;
+};
+__device__ __forceinline__ PrimHit prim_hit_other(const FastRay& r, float4 a, float4 b, float4 c,
+                                                  const float4* quadV3, int pos) {
+    PrimHit h;
+    h.t = 0.0f;
+    h.w0 = 0.0f;
+    h.w1 = 0.0f;
+    if (__float_as_int(b.w) == kPrimQuad) {
+        const float4 d = quadV3[pos];
+        h.hit = ray_quad(r, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), v3(d.x, d.y, d.z), h.t, h.w0,
+                         h.w1)
+                    ? 1
+                    : 0;
+    } else {
+        h.hit = ray_sphere(r, v3(a.x, a.y, a.z), b.x, h.t) ? 1 : 0;
+    }
+    return h;
+}
+
+// Test of the primitive whose 48-byte record (a, b, c) sits at sorted
+// position pos of a LoD: true with t > 0 and the surface parameters (the
+// barycentric weights of v0 and v1, or a quad's (u, v)). quadV3() yields the
+// LoD's fourth quad vertices; it is only evaluated for a quad. kMixed: the
+// scene has LoDs with quads or spheres; without, every record is a triangle
+// and the kernels carry no quad or sphere code (the quad test alone would
+// raise the single-instance trace kernel from 53 to 70 VGPRs, 8 to 7 waves).
+template <bool kMixed, class QuadV3>
+__device__ __forceinline__ bool prim_hit(const FastRay& r, float4 a, float4 b, float4 c, int pos, QuadV3 quadV3,
+                                         float& t, float& w0, float& w1) {
+    if (!kMixed || __float_as_int(b.w) == kPrimTriangle)
+        return ray_triangle(r, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f;
+    const PrimHit h = prim_hit_other(r, a, b, c, __float_as_int(b.w) == kPrimQuad ? quadV3() : nullptr, pos);
+    t = h.t;
+    w0 = h.w0;
+    w1 = h.w1;
+    return h.hit != 0;
+}
+
 // Accept a hit closer than the current one; at equal distance the lower
 // (instance, primitive) pair wins. A hit exactly at the initial tmax (no hit
 // yet) is rejected, as the reference's strict t < tmax test does.
@@ -166,7 +212,7 @@ struct TraceHit {
 // The stack's deeper entries live in a private array of the caller
 // (kSpill ints), passed to run() so that the struct itself holds only
 // scalars and stays in registers.
-template <bool kAny, bool kStats, int kSingleMode, int kStride, int kLds = kLdsDepth>
+template <bool kAny, bool kStats, int kSingleMode, int kStride, int kLds = kLdsDepth, bool kMixed = false>
 struct Traversal {
     static constexpr int kSpill = kStackCap - kLds;
     FastRay cur;
@@ -295,8 +341,8 @@ struct Traversal {
                         if (kStats) ++primTests;
                         const float objT = e.tmax;
                         float ht = objT;
-                        if (ray_triangle(e.ray, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
-                            t > 0.0f && better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
+                        if (prim_hit<kMixed>(e.ray, a, b, c, 0, [&] { return lod.quadV3; }, t, w0, w1) &&
+                            better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
                             ht = t;
                             hInst = inst;
                             hPrim = __float_as_int(a.w);
@@ -333,8 +379,9 @@ struct Traversal {
                 float t, w0, w1;
                 const int prim = __float_as_int(a.w);
                 if (kStats) ++primTests;
-                if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
-                    t > 0.0f && better(t, curInst, prim, hitT, hInst, hPrim)) {
+                if (prim_hit<kMixed>(cur, a, b, c, leafId, [&] { return p.lods[p.instLod[curInst]].quadV3; }, t, w0,
+                                     w1) &&
+                    better(t, curInst, prim, hitT, hInst, hPrim)) {
                     hitT = t;
                     hInst = curInst;
                     hPrim = prim;
@@ -385,7 +432,8 @@ struct Traversal {
 // Traverse from (wo, wd) up to tmax. lstack: this lane's LDS stack column
 // (stride kStride ints). On return t is the world-space distance (or the
 // round-tripped tmax on a miss) and inst/prim are -1 on a miss.
-template <bool kAny, bool kStats, bool kSingle, int kStride, int kLds = kLdsDepth, int kBatch = 1>
+template <bool kAny, bool kStats, bool kSingle, int kStride, int kLds = kLdsDepth, int kBatch = 1,
+          bool kMixed = false>
 __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd, float tmax, int* const lstack) {
     int spill[kStackCap - kLds];
     int sp = 0;
@@ -494,9 +542,9 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
                     float t, w0, w1;
                     const int prim = __float_as_int(ta[j].w);
                     if (kStats) ++primTests;
-                    if (ray_triangle(cur, v3(ta[j].x, ta[j].y, ta[j].z), v3(tb[j].x, tb[j].y, tb[j].z),
-                                     v3(tc[j].x, tc[j].y, tc[j].z), t, w0, w1) &&
-                        t > 0.0f && better(t, curInst, prim, hitT, hInst, hPrim)) {
+                    if (prim_hit<kMixed>(cur, ta[j], tb[j], tc[j], first + j,
+                                 [&] { return p.lods[p.instLod[kSingle ? 0 : curInst]].quadV3; }, t, w0, w1) &&
+                        better(t, curInst, prim, hitT, hInst, hPrim)) {
                         hitT = t;
                         hInst = curInst;
                         hPrim = prim;
@@ -527,8 +575,8 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
                     if (kStats) ++primTests;
                     const float objT = e.tmax;
                     float ht = objT;
-                    if (ray_triangle(e.ray, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) &&
-                        t > 0.0f && better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
+                    if (prim_hit<kMixed>(e.ray, a, b, c, 0, [&] { return lod.quadV3; }, t, w0, w1) &&
+                        better(t, inst, __float_as_int(a.w), ht, hInst, hPrim)) {
                         ht = t;
                         hInst = inst;
                         hPrim = __float_as_int(a.w);
@@ -567,7 +615,8 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
             float t, w0, w1;
             const int prim = __float_as_int(a.w);
             if (kStats) ++primTests;
-            if (ray_triangle(cur, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, w0, w1) && t > 0.0f &&
+            if (prim_hit<kMixed>(cur, a, b, c, leafId, [&] { return p.lods[p.instLod[kSingle ? 0 : curInst]].quadV3; },
+                                 t, w0, w1) &&
                 better(t, curInst, prim, hitT, hInst, hPrim)) {
                 hitT = t;
                 hInst = curInst;
@@ -629,7 +678,7 @@ __device__ __forceinline__ TraceHit trace_ray(const TraceParams& p, V3 wo, V3 wd
 // and a box is culled only when its entry distance exceeds the current best
 // (the collapse enlarges boxes, so a triangle never lies closer than its
 // box's entry). No counts (the stats kernels use trace_ray).
-template <int kStride, int kLds = kLdsDepth, int kBatch = 1>
+template <int kStride, int kLds = kLdsDepth, int kBatch = 1, bool kMixed = false>
 __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 wo, V3 wd, float tmax,
                                                        int* const lstack) {
     int spill[kStackCap - kLds];
@@ -728,9 +777,8 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
                 if (j >= nb) continue;
                 float t, w0, w1;
                 const int prim = __float_as_int(ta[j].w);
-                if (ray_triangle(cur, v3(ta[j].x, ta[j].y, ta[j].z), v3(tb[j].x, tb[j].y, tb[j].z),
-                                 v3(tc[j].x, tc[j].y, tc[j].z), t, w0, w1) &&
-                    t > 0.0f && better(t, 0, prim, hitT, h.inst, h.prim)) {
+                if (prim_hit<kMixed>(cur, ta[j], tb[j], tc[j], first + j, [&] { return lod.quadV3; }, t, w0, w1) &&
+                    better(t, 0, prim, hitT, h.inst, h.prim)) {
                     hitT = t;
                     h.inst = 0;
                     h.prim = prim;
@@ -745,6 +793,41 @@ __device__ __forceinline__ TraceHit trace_closest_held(const TraceParams& p, V3 
     h.t = hitT / e.scale;  // object_to_world_space
     h.overflow = overflow;
     return h;
+}
+
+// Traversal specializations of the kernels that trace (their kMode): fixed
+// for one scene, or chosen per ray (a batch of scenes mixes single- and
+// multi-instance scenes; a scene with quads or spheres takes the general
+// form).
+constexpr int kTraceMulti = 0;   // instance level, triangle LoDs
+constexpr int kTraceSingle = 1;  // one instance, triangle LoDs
+constexpr int kTraceBatch = 2;   // per ray, triangle LoDs
+constexpr int kTraceMixed = 3;   // per ray, LoDs with quads and spheres
+// The mode of a kernel tracing one scene.
+inline int trace_mode_of(const TraceParams& p) {
+    return p.mixed ? kTraceMixed : (p.numInstances == 1 ? kTraceSingle : kTraceMulti);
+}
+// Traversal kind of the resumable Traversal for a mode.
+template <int kMode>
+constexpr int traversal_mode() {
+    return kMode == kTraceSingle ? 1 : (kMode == kTraceMulti ? 0 : 2);
+}
+
+// Closest (kAny = false) or any hit in mode kMode; closest hits of
+// single-instance scenes hold leaves (trace_closest_held) unless kHeld is
+// false.
+template <int kMode, bool kAny, int kStride, int kLds, int kLeaf, bool kHeld = true>
+__device__ __forceinline__ TraceHit trace_mode(const TraceParams& tp, V3 o, V3 d, float tmax, int* lstack) {
+    constexpr bool kMixed = kMode == kTraceMixed;
+    if constexpr (kMode == kTraceSingle && !kAny && kHeld)
+        return trace_closest_held<kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    if constexpr (kMode == kTraceSingle) return trace_ray<kAny, false, true, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    if constexpr (kMode == kTraceMulti) return trace_ray<kAny, false, false, kStride, kLds, kLeaf>(tp, o, d, tmax, lstack);
+    if (tp.numInstances == 1) {
+        if constexpr (!kAny && kHeld) return trace_closest_held<kStride, kLds, kLeaf, kMixed>(tp, o, d, tmax, lstack);
+        return trace_ray<kAny, false, true, kStride, kLds, kLeaf, kMixed>(tp, o, d, tmax, lstack);
+    }
+    return trace_ray<kAny, false, false, kStride, kLds, kLeaf, kMixed>(tp, o, d, tmax, lstack);
 }
 
 }  // namespace mfh

@@ -41,7 +41,7 @@ __device__ __forceinline__ V3 closest_line_point(V3 A, V3 B, V3 C, V3 hit) {
     return onBC;
 }
 
-template <bool kSingle>
+template <int kMode>
 __global__ __launch_bounds__(kWfThreads) void k_wireframe(PtScene S, const u32* __restrict__ seeds, float lineWidth,
                                                           int maxTraceDepth) {
     __shared__ int sStack[kLdsDepth * kWfThreads];
@@ -67,8 +67,8 @@ __global__ __launch_bounds__(kWfThreads) void k_wireframe(PtScene S, const u32* 
     acc.init(1u << kTargetRadiance);  // the Border target rides in the first slot
     float totalDistance = 0.0f;
     for (int i = 0; i < maxTraceDepth; ++i) {
-        const TraceHit h = trace_ray<false, false, kSingle, kWfThreads, kLdsDepth, kPtLeafBatch>(S.trace, origin, dir,
-                                                                                                 kMaxSceneSize, lstack);
+        const TraceHit h = trace_mode<kMode, false, kWfThreads, kLdsDepth, kPtLeafBatch, false>(
+            S.trace, origin, dir, kMaxSceneSize, lstack);
         if (h.inst == -1) {
             float pdfBack;
             const V3 bg = evaluate_background(S, dir, pdfBack);
@@ -112,11 +112,15 @@ __global__ __launch_bounds__(kWfThreads) void k_wireframe(PtScene S, const u32* 
 void launch_wireframe(const PtScene& S, const uint32_t* seeds, float lineWidth, int maxTraceDepth,
                       hipStream_t stream) {
     const dim3 grid((S.width + kWfTile - 1) / kWfTile, (S.height + kWfTile - 1) / kWfTile);
-    if (S.trace.numInstances == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wireframe<true>), grid, dim3(kWfThreads), 0, stream, S, seeds,
+    const int mode = trace_mode_of(S.trace);
+    if (mode == kTraceSingle)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wireframe<kTraceSingle>), grid, dim3(kWfThreads), 0, stream, S, seeds,
+                           lineWidth, maxTraceDepth);
+    else if (mode == kTraceMulti)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wireframe<kTraceMulti>), grid, dim3(kWfThreads), 0, stream, S, seeds,
                            lineWidth, maxTraceDepth);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wireframe<false>), grid, dim3(kWfThreads), 0, stream, S, seeds,
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wireframe<kTraceMixed>), grid, dim3(kWfThreads), 0, stream, S, seeds,
                            lineWidth, maxTraceDepth);
     MFH_HIP(hipGetLastError());
 }

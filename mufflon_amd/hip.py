@@ -38,6 +38,9 @@ PT_PARAMS_DTYPE = np.dtype([("seed", "<i4"), ("minPathLength", "<i4"), ("maxPath
 POINT_LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("intensity", "<f4", 3)])
 AREA_LIGHT_DTYPE = np.dtype([("points", "<f4", 9), ("uv", "<f4", 6), ("material", "<i4"), ("instance", "<i4"),
                              ("primitive", "<i4")])
+AREA_LIGHT_EX_DTYPE = np.dtype([("kind", "<i4"), ("points", "<f4", 12), ("uv", "<f4", 8), ("radius", "<f4"),
+                                ("material", "<i4"), ("instance", "<i4"), ("primitive", "<i4")])
+AREA_TRIANGLE, AREA_QUAD, AREA_SPHERE = 0, 1, 2
 LIGHT_TREE_INFO_DTYPE = np.dtype([("posRootFlux", "<f4"), ("posRootType", "<u4"), ("posInternalCount", "<u4"),
                                   ("posLightCount", "<u4"), ("memBytes", "<u4"), ("mapCount", "<i4"),
                                   ("dirRootFlux", "<f4"), ("backgroundFlux", "<f4")])
@@ -67,6 +70,7 @@ _SIGS = {
     "mfh_upload": [vp, vp, vp, C.c_size_t],
     "mfh_download": [vp, vp, vp, C.c_size_t],
     "mfh_lod_create": [vp, f32p, i32, u32p, i32, i32p],
+    "mfh_lod_create_mixed": [vp, f32p, i32, u32p, i32, u32p, i32, f32p, i32, i32p],
     "mfh_lbvh_build": [vp, i32, f32p],
     "mfh_lbvh_get": [vp, i32, i32p, vp, i32p],
     "mfh_lbvh_build_stages": [vp, i32, f32p, u64p, u64p, i32p, i32p, f32p, i32p],
@@ -109,6 +113,7 @@ _SIGS = {
     "mfh_pt_get_nan_count": [vp, i32, u32p],
     "mfh_pt_get_light_tree": [vp, i32, vp, vp, u64p, u32p],
     "mfh_light_tree_build": [vp, i32, vp, i32, f32p, vp, i32, vp, vp, C.c_size_t, u64p, u32p, i32],
+    "mfh_light_tree_build_ex": [vp, i32, vp, i32, f32p, vp, i32, vp, vp, C.c_size_t, u64p, u32p, i32],
     "mfh_texture_create": [vp, i32, i32, i32, i32, f32p, i32p],
     "mfh_background_build": [vp, f32p, i32, i32, i32, f32p, f32p, f32p],
     "mfh_pt_set_background": [vp, i32, vp],
@@ -191,20 +196,23 @@ def background_build(background, env_texels, scene_box):
 
 
 def light_tree_build(area_lights, point_lights, scene_box, materials):
-    """Host-only light-tree build (no device): returns (info, mem, keys, codes)."""
+    """Host-only light-tree build (no device): returns (info, mem, keys, codes).
+    area_lights: AREA_LIGHT_DTYPE (triangles) or AREA_LIGHT_EX_DTYPE records."""
     lib = load()
-    area = np.ascontiguousarray(area_lights, AREA_LIGHT_DTYPE) if area_lights is not None else \
-        np.zeros(0, AREA_LIGHT_DTYPE)
+    ex = area_lights is not None and getattr(area_lights, "dtype", None) == AREA_LIGHT_EX_DTYPE
+    dt = AREA_LIGHT_EX_DTYPE if ex else AREA_LIGHT_DTYPE
+    area = np.ascontiguousarray(area_lights, dt) if area_lights is not None else np.zeros(0, dt)
     pts = np.ascontiguousarray(point_lights, POINT_LIGHT_DTYPE) if point_lights is not None else \
         np.zeros(0, POINT_LIGHT_DTYPE)
     mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
     box = np.ascontiguousarray(scene_box, np.float32)
     n = len(area) + len(pts)
     info = np.zeros(1, LIGHT_TREE_INFO_DTYPE)
-    mem = np.zeros(max(128 * n, 16), np.uint8)
+    mem = np.zeros(max(192 * n, 16), np.uint8)
     keys = np.zeros(max(len(area), 1), np.uint64)
     codes = np.zeros(max(len(area), 1), np.uint32)
-    rc = lib.mfh_light_tree_build(area.ctypes.data if len(area) else None, len(area),
+    fn = lib.mfh_light_tree_build_ex if ex else lib.mfh_light_tree_build
+    rc = fn(area.ctypes.data if len(area) else None, len(area),
                                   pts.ctypes.data if len(pts) else None, len(pts), _ptr(box, f32p), mats.ctypes.data,
                                   len(mats), info.ctypes.data, mem.ctypes.data, mem.nbytes, _ptr(keys, u64p),
                                   _ptr(codes, u32p), len(keys))
@@ -296,6 +304,19 @@ class HipContext:
         t = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
         out = i32()
         self._call("mfh_lod_create", self.handle, _ptr(v, f32p), len(v), _ptr(t, u32p), len(t), C.byref(out))
+        return out.value
+
+    def lod_create_mixed(self, vertices, tris=None, quads=None, spheres=None):
+        """LoD of triangles (n x 3 vertex indices), quads (n x 4: p00, p10, p11,
+        p01) and spheres (n x 4: center xyz, radius); ids run in that order."""
+        v = np.ascontiguousarray(vertices if vertices is not None else np.zeros((0, 3)), np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(tris if tris is not None else np.zeros((0, 3)), np.uint32).reshape(-1, 3)
+        q = np.ascontiguousarray(quads if quads is not None else np.zeros((0, 4)), np.uint32).reshape(-1, 4)
+        sp = np.ascontiguousarray(spheres if spheres is not None else np.zeros((0, 4)), np.float32).reshape(-1, 4)
+        out = i32()
+        self._call("mfh_lod_create_mixed", self.handle, _ptr(v, f32p) if len(v) else None, len(v),
+                   _ptr(t, u32p) if len(t) else None, len(t), _ptr(q, u32p) if len(q) else None, len(q),
+                   _ptr(sp, f32p) if len(sp) else None, len(sp), C.byref(out))
         return out.value
 
     def lbvh_build(self, lod, bbox=None):
@@ -417,7 +438,8 @@ class HipContext:
         n = np.ascontiguousarray(normals, np.float32)
         uv = np.ascontiguousarray(uvs, np.float32)
         m = np.ascontiguousarray(mat_idx, np.uint16)
-        self._call("mfh_lod_set_shading", self.handle, lod, _ptr(n, f32p), _ptr(uv, f32p), m.ctypes.data)
+        self._call("mfh_lod_set_shading", self.handle, lod, _ptr(n, f32p) if n.size else None,
+                   _ptr(uv, f32p) if uv.size else None, m.ctypes.data)
 
     def texture_create(self, texels, sampling=SAMPLING_LINEAR):
         """Texture from (h, w) or (h, w, c) float32 texels; returns its id."""

@@ -44,6 +44,27 @@ def random_triangle_soup(n, seed=7, mu=-5.0, sigma=0.5):
     return verts, tris
 
 
+def random_mixed_soup(nt, nq, ns, seed=7, mu=-3.0, sigma=0.5, twist=0.3):
+    """Random triangles, bilinear quads (a square of edge e, one corner lifted
+    by twist * e out of its plane) and spheres (radius e / 2) with centroids in
+    the unit cube -> (vertices, tris (nt, 3), quads (nq, 4), spheres (ns, 4)).
+    Vertices: the triangles' 3 nt, then the quads' 4 nq."""
+    rng = np.random.default_rng(seed)
+    vt, tt = random_triangle_soup(nt, seed=seed + 1, mu=mu, sigma=sigma) if nt else (np.zeros((0, 3), np.float32),
+                                                                                   np.zeros((0, 3), np.uint32))
+    c = rng.random((nq, 3))
+    e = rng.lognormal(mu, sigma, nq)
+    R = _random_rotations(rng, nq)
+    local = np.array([[-0.5, -0.5, 0.0], [0.5, -0.5, 0.0], [0.5, 0.5, twist], [-0.5, 0.5, 0.0]])
+    pts = np.matmul(R, local.T).transpose(0, 2, 1) * e[:, None, None] + c[:, None, :]
+    vq = pts.reshape(-1, 3).astype(np.float32)
+    quads = (np.arange(4 * nq, dtype=np.uint32).reshape(nq, 4) + len(vt)).astype(np.uint32)
+    sc = rng.random((ns, 3))
+    sr = rng.lognormal(mu, sigma, ns) * 0.5
+    spheres = np.concatenate([sc, sr[:, None]], 1).astype(np.float32)
+    return np.concatenate([vt, vq]).astype(np.float32), tt, quads, spheres
+
+
 def pinhole_rays(width, height, eye=(0.5, 0.5, -1.2), target=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0),
                  vfov_deg=50.0, tmax=MAX_SCENE_SIZE):
     eye = np.asarray(eye, np.float64)
@@ -325,6 +346,202 @@ def cornell_materials(sphere_rings=32, sphere_segments=64, width=64, height=64):
     points["intensity"] = [(0.8, 0.6, 0.3), (0.2, 0.3, 0.6)]
     return dict(verts=v, normals=n, uvs=uv, tris=t, mat_idx=m, materials=mats, camera=cam, textures=textures,
                 point_lights=points, envmap=2)
+
+
+class MixedBuilder:
+    """LoD of triangles, quads and spheres (primitive ids in that order) with
+    per-vertex normals / uvs and a material per primitive."""
+
+    def __init__(self):
+        self.v, self.n, self.uv = [], [], []
+        self.t, self.tm, self.q, self.qm, self.s, self.sm = [], [], [], [], [], []
+
+    def _vert(self, p, nrm, uv):
+        self.v.append(p)
+        self.n.append(nrm)
+        self.uv.append(uv)
+        return len(self.v) - 1
+
+    def add_quad(self, p00, p10, p11, p01, mat, normals=None):
+        """Bilinear patch p00 p10 p11 p01 (corners in patch order); vertex
+        normals default to the patch normal at each corner."""
+        P = [np.asarray(x, np.float64) for x in (p00, p10, p11, p01)]
+        if normals is None:
+            normals = [np.cross(P[3] - P[0], P[1] - P[0]), np.cross(P[2] - P[1], P[0] - P[1]),
+                       np.cross(P[1] - P[2], P[3] - P[2]), np.cross(P[0] - P[3], P[2] - P[3])]
+            normals = [-x for x in normals]
+        uvs = [(0, 0), (1, 0), (1, 1), (0, 1)]
+        self.q.append([self._vert(P[k], np.asarray(normals[k], np.float64), uvs[k]) for k in range(4)])
+        self.qm.append(mat)
+
+    def add_box_tris(self, center, size, angle_deg, mat):
+        mb = MeshBuilder()
+        mb.add_box(center, size, angle_deg, mat)
+        v, n, uv, t, m = mb.arrays()
+        base = len(self.v)
+        for k in range(len(v)):
+            self._vert(v[k].astype(np.float64), n[k].astype(np.float64), tuple(uv[k]))
+        self.t += (t.astype(np.int64) + base).tolist()
+        self.tm += [mat] * len(t)
+
+    def add_sphere(self, center, radius, mat):
+        self.s.append([center[0], center[1], center[2], radius])
+        self.sm.append(mat)
+
+    def lod(self):
+        n = np.asarray(self.n, np.float64).reshape(-1, 3)
+        n = n / np.linalg.norm(n, axis=1, keepdims=True)
+        return dict(verts=np.asarray(self.v, np.float32).reshape(-1, 3), normals=n.astype(np.float32),
+                    uvs=np.asarray(self.uv, np.float32).reshape(-1, 2),
+                    tris=np.asarray(self.t, np.uint32).reshape(-1, 3), quads=np.asarray(self.q, np.uint32).reshape(-1, 4),
+                    spheres=np.asarray(self.s, np.float32).reshape(-1, 4),
+                    mat_idx=np.array(self.tm + self.qm + self.sm, np.uint16))
+
+
+def mixed_scene(width=64, height=64, instanced=True):
+    """A room of bilinear quads (walls, a slightly twisted back wall, a quad
+    light), triangle boxes and spheres (Lambert, GGX, glass, Walter, a sphere
+    light), plus, with `instanced`, a prop LoD of quads and spheres placed
+    twice (rotated, uniformly scaled): every primitive kind, quad and sphere
+    lights, one point light. A general scene dict (lods, inst_lod,
+    world_to_inst, materials, point_lights, camera)."""
+    LIGHT, WHITE, RED, GREEN, GGX, GLASS, WALTER, SLIGHT, BECK = range(9)
+    mats = np.concatenate([
+        material(MAT_EMISSIVE, radiance=(12.0, 9.0, 4.0), scale=(1.0, 1.0, 1.0)),
+        material(MAT_LAMBERT, albedo=(0.73, 0.73, 0.73)),
+        material(MAT_LAMBERT, albedo=(0.65, 0.05, 0.05)),
+        material(MAT_LAMBERT, albedo=(0.12, 0.45, 0.15)),
+        material(MAT_TORRANCE, albedo=(0.9, 0.8, 0.6), roughness=0.25),
+        material(MAT_MICROFACET, roughness=0.1, ior=1.5, absorption=(0.3, 0.1, 0.05)),
+        material(MAT_WALTER, roughness=0.15, ior=1.33, absorption=(0.1, 0.2, 0.3), shadowing=SHADOWING_SMITH),
+        material(MAT_EMISSIVE, radiance=(3.0, 5.0, 9.0), scale=(1.0, 1.0, 1.0)),
+        material(MAT_TORRANCE, albedo=(0.95, 0.6, 0.3), roughness=0.3, ndf=NDF_BECKMANN),
+    ])
+    room = MixedBuilder()
+    room.add_box_tris((-0.45, 0.0, -0.4), (0.5, 1.1, 0.5), 18.0, WHITE)
+    room.add_box_tris((0.5, 0.0, -0.45), (0.45, 0.45, 0.45), -17.0, GGX)
+    room.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), WHITE)      # floor
+    room.add_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), WHITE)      # ceiling
+    room.add_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1.25), (-1, 2, -1), BECK)  # back wall, twisted
+    room.add_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), RED)
+    room.add_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), GREEN)
+    L = 0.22
+    room.add_quad((-L, 1.99, -L), (L, 1.99, -L), (L, 1.99, L), (-L, 1.99, L), LIGHT)  # faces down
+    room.add_sphere((0.05, 0.3, 0.4), 0.3, GLASS)
+    room.add_sphere((-0.55, 1.45, 0.25), 0.17, WALTER)
+    room.add_sphere((0.6, 1.2, 0.5), 0.12, SLIGHT)
+    room.add_sphere((0.55, 0.62, -0.45), 0.16, GGX)
+    lods = [room.lod()]
+    world = [IDENTITY34.reshape(12)]
+    if instanced:
+        prop = MixedBuilder()
+        prop.add_quad((-0.3, 0.0, -0.2), (0.3, 0.0, -0.2), (0.3, 0.05, 0.2), (-0.3, 0.0, 0.2), RED)
+        prop.add_quad((-0.3, 0.0, 0.2), (0.3, 0.05, 0.2), (0.3, 0.3, 0.25), (-0.3, 0.3, 0.2), WHITE)
+        prop.add_sphere((0.0, 0.15, 0.0), 0.1, GGX)
+        prop.add_sphere((0.2, 0.12, -0.1), 0.05, SLIGHT)
+        lods.append(prop.lod())
+        for ang, s, t in ((30.0, 1.0, (-0.35, 0.0, 0.55)), (-50.0, 0.8, (0.4, 1.1, 0.1))):
+            a = np.radians(ang)
+            R = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]]) * s
+            inv = np.linalg.inv(R)
+            w = np.zeros(12, np.float32)
+            w[[0, 1, 2, 4, 5, 6, 8, 9, 10]] = inv.reshape(-1)
+            w[[3, 7, 11]] = -inv @ np.asarray(t)
+            world.append(w)
+    vfov = np.radians(40.0)
+    cam = dict(position=np.array([0.0, 1.0, 3.8], np.float32), view_dir=np.array([0, 0, -1], np.float32),
+               up=np.array([0, 1, 0], np.float32), tan_vfov=np.float32(np.tan(vfov / 2)), width=width,
+               height=height)
+    points = np.zeros(1, POINT_LIGHT_DTYPE)
+    points["position"] = [(-0.6, 1.7, 0.7)]
+    points["intensity"] = [(0.5, 0.4, 0.3)]
+    return dict(lods=lods, inst_lod=np.array([0] + [1] * (len(world) - 1), np.int32),
+                world_to_inst=np.stack(world).astype(np.float32), materials=mats, point_lights=points,
+                camera=cam, textures=[])
+
+
+def instance_to_world_f32(w):
+    """Inverse of a worldToInstance transform in float32, operation for
+    operation as the library computes it (adjugate over determinant,
+    descriptors.hpp:108-129): a 3 x 4 array."""
+    w = [np.float32(x) for x in np.asarray(w, np.float32).reshape(12)]
+    r00, r01, r02, r10, r11, r12, r20, r21, r22 = w[0], w[1], w[2], w[4], w[5], w[6], w[8], w[9], w[10]
+    m = [r11 * r22 - r21 * r12, r21 * r02 - r01 * r22, r01 * r12 - r11 * r02,
+         r20 * r12 - r10 * r22, r00 * r22 - r20 * r02, r10 * r02 - r00 * r12,
+         r10 * r21 - r20 * r11, r20 * r01 - r00 * r21, r00 * r11 - r10 * r01]
+    det = r00 * m[0] + r01 * m[3] + r02 * m[6]
+    inv = np.float32(1.0) / det
+    o = np.zeros((3, 4), np.float32)
+    for r in range(3):
+        for c in range(3):
+            o[r, c] = inv * m[3 * r + c]
+    t = (w[3], w[7], w[11])
+    for r in range(3):
+        o[r, 3] = -(o[r, 0] * t[0] + o[r, 1] * t[1] + o[r, 2] * t[2])
+    return o
+
+
+def emissive_area_lights_ex(lod, materials, instance=0, world_to_inst=None):
+    """Area lights (AREA_LIGHT_EX_DTYPE) of the emissive triangles, quads and
+    spheres of one instance of a LoD dict, in Mufflon's gathering order
+    (world_container.cpp:895-960): world-space points, v1 / v3 swapped under
+    a mirroring transform, a sphere's radius scaled by the length of the
+    instance-to-world transform's first column."""
+    from mufflon_amd.hip import AREA_LIGHT_EX_DTYPE, AREA_QUAD, AREA_SPHERE, AREA_TRIANGLE
+    M = np.eye(4, dtype=np.float32)[:3] if world_to_inst is None else instance_to_world_f32(world_to_inst)
+
+    def xf(p):
+        p = np.asarray(p, np.float32)
+        return np.array([M[r, 0] * p[0] + M[r, 1] * p[1] + M[r, 2] * p[2] + M[r, 3] for r in range(3)], np.float32)
+
+    det = M[0, 0] * (M[1, 1] * M[2, 2] - M[2, 1] * M[1, 2]) + M[0, 1] * (M[2, 0] * M[1, 2] - M[1, 0] * M[2, 2]) + \
+        M[0, 2] * (M[1, 0] * M[2, 1] - M[2, 0] * M[1, 1])
+    mirroring = det < 0
+    emissive = materials["type"] == MAT_EMISSIVE
+    tris = lod["tris"]
+    quads = lod.get("quads", np.zeros((0, 4), np.uint32))
+    spheres = lod.get("spheres", np.zeros((0, 4), np.float32))
+    m = lod["mat_idx"]
+    out = []
+    for p in range(len(tris)):
+        if not emissive[m[p]]:
+            continue
+        r = np.zeros(1, AREA_LIGHT_EX_DTYPE)
+        r["kind"] = AREA_TRIANGLE
+        pts = [xf(lod["verts"][i]) for i in tris[p]]
+        uvs = [lod["uvs"][i] for i in tris[p]]
+        if mirroring:
+            pts[1], pts[2], uvs[1], uvs[2] = pts[2], pts[1], uvs[2], uvs[1]
+        r["points"][0, :9] = np.concatenate(pts)
+        r["uv"][0, :6] = np.concatenate(uvs)
+        r["material"], r["instance"], r["primitive"] = m[p], instance, p
+        out.append(r)
+    for q in range(len(quads)):
+        p = len(tris) + q
+        if not emissive[m[p]]:
+            continue
+        r = np.zeros(1, AREA_LIGHT_EX_DTYPE)
+        r["kind"] = AREA_QUAD
+        pts = [xf(lod["verts"][i]) for i in quads[q]]
+        uvs = [lod["uvs"][i] for i in quads[q]]
+        if mirroring:
+            pts[1], pts[3], uvs[1], uvs[3] = pts[3], pts[1], uvs[3], uvs[1]
+        r["points"][0] = np.concatenate(pts)
+        r["uv"][0] = np.concatenate(uvs)
+        r["material"], r["instance"], r["primitive"] = m[p], instance, p
+        out.append(r)
+    scale = np.float32(np.sqrt(np.float32(M[0, 0] * M[0, 0] + M[1, 0] * M[1, 0]) + M[2, 0] * M[2, 0]))
+    for k in range(len(spheres)):
+        p = len(tris) + len(quads) + k
+        if not emissive[m[p]]:
+            continue
+        r = np.zeros(1, AREA_LIGHT_EX_DTYPE)
+        r["kind"] = AREA_SPHERE
+        r["points"][0, :3] = xf(spheres[k][:3])
+        r["radius"] = np.float32(scale * spheres[k][3])
+        r["material"], r["instance"], r["primitive"] = m[p], instance, p
+        out.append(r)
+    return np.concatenate(out) if out else np.zeros(0, AREA_LIGHT_EX_DTYPE)
 
 
 def emissive_area_lights(verts, uvs, tris, mat_idx, materials, instance=0):

@@ -9,7 +9,10 @@ from mufflon_amd import hip
 def upload_lods(ctx, lods, build=True):
     ids = []
     for L in lods:
-        lod = ctx.lod_create(L["verts"], L["tris"])
+        if len(L.get("quads", ())) or len(L.get("spheres", ())):
+            lod = ctx.lod_create_mixed(L["verts"], L["tris"], L.get("quads"), L.get("spheres"))
+        else:
+            lod = ctx.lod_create(L["verts"], L["tris"])
         ctx.lod_set_shading(lod, L["normals"], L["uvs"], L["mat_idx"])
         if build:
             ctx.lbvh_build(lod)

@@ -77,6 +77,43 @@ struct TriSet {
         return tri_box(vert(idx[3 * p]), vert(idx[3 * p + 1]), vert(idx[3 * p + 2]));
     }
     bool masked(i32) const { return false; }
+    int kind(i32) const { return 0; }
+};
+
+// LoD with quads and spheres besides triangles; primitive ids run
+// triangles, quads, spheres (descriptors.hpp:33-68). accel_structs_commons.hpp:
+// 56-71 (centroids: triangle and quad means, sphere center) and :85-100
+// (boxes; a sphere's is center -+ radius).
+struct MixedSet {
+    const float* verts;
+    const u32* tris;
+    const u32* quads;
+    const float* spheres;  // center xyz, radius
+    i32 nt, nq, ns;
+    i32 n;
+    V3 vert(u32 i) const { return {verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]}; }
+    int kind(i32 p) const { return p < nt ? 0 : (p < nt + nq ? 1 : 2); }
+    V3 centroid(i32 p) const {
+        const int k = kind(p);
+        if (k == 0) return (vert(tris[3 * p]) + vert(tris[3 * p + 1]) + vert(tris[3 * p + 2])) / 3.0f;
+        if (k == 1) {
+            const u32* q = quads + 4 * (size_t)(p - nt);
+            return (vert(q[0]) + vert(q[1]) + vert(q[2]) + vert(q[3])) / 4.0f;
+        }
+        const float* sp = spheres + 4 * (size_t)(p - nt - nq);
+        return {sp[0], sp[1], sp[2]};
+    }
+    Box box(i32 p) const {
+        const int k = kind(p);
+        if (k == 0) return tri_box(vert(tris[3 * p]), vert(tris[3 * p + 1]), vert(tris[3 * p + 2]));
+        if (k == 1) {
+            const u32* q = quads + 4 * (size_t)(p - nt);
+            return quad_box(vert(q[0]), vert(q[1]), vert(q[2]), vert(q[3]));
+        }
+        const float* sp = spheres + 4 * (size_t)(p - nt - nq);
+        return sphere_box({sp[0], sp[1], sp[2]}, sp[3]);
+    }
+    bool masked(i32) const { return false; }
 };
 
 // Instance set: centroid/box of each instance's object AABB in world space
@@ -99,6 +136,7 @@ struct InstSet {
     }
     Box box(i32 p) const { return xform_box(mat(p), obox(p)); }
     bool masked(i32 p) const { return present && !present[p]; }
+    int kind(i32) const { return 0; }
 };
 
 // Cost model of a triangle LoD (build_lbvh.cu:34-40): Ct = 2, Cp = (1.2, 4, 1).
@@ -107,7 +145,9 @@ struct LodCost {
     struct Count {
         i32 x = 0, y = 0, z = 0;
     };
-    static Count leaf(i32) { return {1, 0, 0}; }  // triangles only
+    // get_count (build_lbvh.cu:285-295): one primitive of kind 0 (triangle),
+    // 1 (quad) or 2 (sphere)
+    static Count leaf(int k) { return {k == 0 ? 1 : 0, k == 1 ? 1 : 0, k == 2 ? 1 : 0}; }
     static i32 encode(const Count& c) {
         return (std::min(c.x, 0x3FF) << 20) | (std::min(c.y, 0x3FF) << 10) | std::min(c.z, 0x3FF);
     }
@@ -132,7 +172,7 @@ struct InstCost {
     struct Count {
         i32 x = 0;
     };
-    static Count leaf(i32) { return {1}; }
+    static Count leaf(int) { return {1}; }
     static i32 encode(const Count& c) { return c.x; }
     static Count extract(i32 v) { return {v}; }
     static float cost(const Count& c) { return Ct + (float)c.x * 8.5f; }
@@ -250,7 +290,7 @@ void build_lbvh(const Set& set, const Box& bb, BuildOut& o, u64* unsortedKeys) {
         const i32 prim = o.ids[idx];
         Box cur = set.box(prim);
         float curSA = surface(cur);
-        typename C::Count cnt = C::leaf(prim);
+        typename C::Count cnt = C::leaf(set.kind(prim));
         float cost = C::cost(cnt);
         const i32 leaf = idx + nInt;
         float* lb = B + (size_t)leaf * 8;
@@ -461,7 +501,7 @@ void build_lbvh_omp(const Set& set, const Box& bb, BuildOut& o) {
     for (i32 idx = 0; idx < n; ++idx) {
         const i32 prim = o.ids[idx];
         const Box lb0 = set.box(prim);
-        const typename C::Count c0 = C::leaf(prim);
+        const typename C::Count c0 = C::leaf(set.kind(prim));
         const i32 leaf = idx + nInt;
         float* lb = B + (size_t)leaf * 8;
         lb[0] = lb0.lo.x; lb[1] = lb0.lo.y; lb[2] = lb0.lo.z; lb[3] = i2f(C::encode(c0));
@@ -625,6 +665,22 @@ int orc_lbvh_build_tris_omp(const float* verts, const uint32_t* idx, int32_t num
     BuildOut o;
     build_lbvh_omp<TriSet, LodCost>(set, bb, o);
     copy_out(o, numTris, sortedKeys, ids, parents, boxes, offsets, nodes, numInternalAfter);
+    return 0;
+}
+
+// The same build over a LoD of triangles, quads (4 vertex indices: p00, p10,
+// p11, p01) and spheres (center xyz, radius); bbox: the LoD box.
+int orc_lbvh_build_mixed(const float* verts, const uint32_t* tris, int32_t numTris, const uint32_t* quads,
+                         int32_t numQuads, const float* spheres, int32_t numSpheres, const float bbox[6],
+                         uint64_t* sortedKeys, int32_t* ids, int32_t* parents, float* boxes, int32_t* offsets,
+                         void* nodes, int32_t* numInternalAfter) {
+    const i32 n = numTris + numQuads + numSpheres;
+    if (n <= 0) return 1;
+    MixedSet set{verts, tris, quads, spheres, numTris, numQuads, numSpheres, n};
+    const Box bb{{bbox[0], bbox[1], bbox[2]}, {bbox[3], bbox[4], bbox[5]}};
+    BuildOut o;
+    build_lbvh<MixedSet, LodCost>(set, bb, o, nullptr);
+    copy_out(o, n, sortedKeys, ids, parents, boxes, offsets, nodes, numInternalAfter);
     return 0;
 }
 

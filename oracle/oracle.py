@@ -28,6 +28,8 @@ class OrcScene(C.Structure):
         ("verts", f32p), ("tris", u32p), ("nodes", C.c_void_p), ("primIds", i32p), ("lodBoxes", f32p),
         ("numInstances", C.c_int32), ("worldToInst", f32p), ("instLod", i32p),
         ("tlasNodes", C.c_void_p), ("tlasNumInternal", C.c_int32), ("tlasPrimIds", i32p),
+        ("lodQuadOffset", i32p), ("quads", u32p), ("lodSphereOffset", i32p), ("spheres", f32p),
+        ("lodPrimOffset", i32p),
     ]
 
 
@@ -51,6 +53,8 @@ def load():
         lib.orc_morton_points.restype = None
         lib.orc_lbvh_build_tris.argtypes = [f32p, u32p, C.c_int32, f32p, u64p, u64p, i32p, i32p, f32p, i32p,
                                             C.c_void_p, i32p]
+        lib.orc_lbvh_build_mixed.argtypes = [f32p, u32p, C.c_int32, u32p, C.c_int32, f32p, C.c_int32, f32p, u64p,
+                                             i32p, i32p, f32p, i32p, C.c_void_p, i32p]
         lib.orc_lbvh_build_instances.argtypes = [f32p, f32p, i32p, C.c_int32, f32p, u64p, i32p, i32p, f32p, i32p,
                                                  C.c_void_p, i32p]
         lib.orc_instance_boxes.argtypes = [f32p, f32p, i32p, C.c_int32, f32p, f32p]
@@ -111,6 +115,48 @@ def lbvh_build_tris(verts, tris, bbox=None):
                                  nodes.ctypes.data, C.byref(nint))
     assert rc == 0
     return dict(unsorted_keys=uk, sorted_keys=sk, sorted_ids=ids, parents=par, boxes=boxes, offsets=offs,
+                nodes=nodes[: nint.value * 64].copy(), num_internal=nint.value, bbox=bb)
+
+
+def mixed_box(verts, spheres):
+    """LoD box of Lod::get_bounding_box: the vertices' box joined with every
+    sphere's center -+ radius."""
+    parts = []
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    if len(v):
+        parts += [v.min(axis=0), v.max(axis=0)]
+    sp = np.asarray(spheres, np.float32).reshape(-1, 4)
+    if len(sp):
+        parts += [(sp[:, :3] - sp[:, 3:]).min(axis=0), (sp[:, :3] + sp[:, 3:]).max(axis=0)]
+    lo = np.minimum.reduce(parts[0::2])
+    hi = np.maximum.reduce(parts[1::2])
+    return np.concatenate([lo, hi]).astype(np.float32)
+
+
+def lbvh_build_mixed(verts, tris, quads, spheres, bbox=None):
+    """Serial LBVH build of a LoD of triangles, quads (4 vertex indices) and
+    spheres (center, radius); primitive ids run triangles, quads, spheres."""
+    lib = load()
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    q = np.ascontiguousarray(quads, np.uint32).reshape(-1, 4)
+    sp = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+    n = len(t) + len(q) + len(sp)
+    bb = mixed_box(v, sp) if bbox is None else np.ascontiguousarray(bbox, np.float32)
+    sk = np.zeros(n, np.uint64)
+    ids = np.zeros(n, np.int32)
+    par = np.zeros(max(2 * n - 1, 1), np.int32)
+    boxes = np.zeros(max(2 * n - 1, 1) * 8, np.float32)
+    offs = np.zeros(max(n - 1, 1), np.int32)
+    nodes = np.zeros(max(n - 1, 1) * 64, np.uint8)
+    nint = C.c_int32()
+    keep = [np.zeros(4, np.uint32), np.zeros(4, np.float32), np.zeros(3, np.float32)]
+    rc = lib.orc_lbvh_build_mixed(_p(v if len(v) else keep[2], f32p), _p(t if len(t) else keep[0], u32p), len(t),
+                                  _p(q if len(q) else keep[0], u32p), len(q), _p(sp if len(sp) else keep[1], f32p),
+                                  len(sp), _p(bb, f32p), _p(sk, u64p), _p(ids, i32p), _p(par, i32p),
+                                  _p(boxes, f32p), _p(offs, i32p), nodes.ctypes.data, C.byref(nint))
+    assert rc == 0
+    return dict(sorted_keys=sk, sorted_ids=ids, parents=par, boxes=boxes, offsets=offs,
                 nodes=nodes[: nint.value * 64].copy(), num_internal=nint.value, bbox=bb)
 
 
@@ -185,7 +231,8 @@ class Scene:
     """Flat multi-LoD, multi-instance scene for the oracle tracer."""
 
     def __init__(self, lods, inst_lod, world_to_inst, tlas=None):
-        # lods: list of dict(verts, tris, nodes, num_internal, prim_ids, box)
+        # lods: list of dict(verts, tris, nodes, num_internal, prim_ids, box),
+        # optionally quads (n x 4 vertex indices) and spheres (n x 4: center, radius)
         self._keep = []
         vo, to, no = [0], [0], [0]
         for L in lods:
@@ -218,6 +265,23 @@ class Scene:
         s.numInstances = len(self.il); s.worldToInst = _p(self.w, f32p); s.instLod = _p(self.il, i32p)
         s.tlasNodes = self.tn.ctypes.data; s.tlasNumInternal = int(tlas["num_internal"])
         s.tlasPrimIds = _p(self.tp, i32p)
+        if any(len(L.get("quads", ())) or len(L.get("spheres", ())) for L in lods):
+            qo, so, po = [0], [0], [0]
+            for L in lods:
+                nq, ns = len(L.get("quads", ())), len(L.get("spheres", ()))
+                qo.append(qo[-1] + nq)
+                so.append(so[-1] + ns)
+                po.append(po[-1] + len(L["tris"]) + nq + ns)
+            self.qo, self.so, self.po = (np.array(x, np.int32) for x in (qo, so, po))
+            self.quads = np.ascontiguousarray(np.concatenate(
+                [np.asarray(L.get("quads", np.zeros((0, 4))), np.uint32).reshape(-1, 4) for L in lods]
+                + [np.zeros((1, 4), np.uint32)]))
+            self.spheres = np.ascontiguousarray(np.concatenate(
+                [np.asarray(L.get("spheres", np.zeros((0, 4))), np.float32).reshape(-1, 4) for L in lods]
+                + [np.zeros((1, 4), np.float32)]))
+            s.lodQuadOffset = _p(self.qo, i32p); s.quads = _p(self.quads, u32p)
+            s.lodSphereOffset = _p(self.so, i32p); s.spheres = _p(self.spheres, f32p)
+            s.lodPrimOffset = _p(self.po, i32p)
         self.s = s
 
 
@@ -485,7 +549,7 @@ def _light_lib():
     if not hasattr(lib, "_lc_ready"):
         lib.orc_light_connect.argtypes = [C.POINTER(OrcPtScene), C.c_uint64, C.c_uint64, C.c_uint64, f32p, C.c_float,
                                           C.c_float, f32p]
-        lib.orc_light_pdf.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, f32p]
+        lib.orc_light_pdf.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, f32p, C.c_float, C.c_float]
         lib.orc_light_pdf.restype = C.c_float
         lib._lc_ready = True
     return lib
@@ -502,10 +566,12 @@ def light_connect(pt_scene, index, num_indices, seed, pos, u0=0.25, u1=0.5):
     return out[:3], float(out[3]), out[4:7], int(out[7])
 
 
-def light_pdf(pt_scene, inst, prim, ref_pos):
-    """Connect pdf (per area) of hitting emitter triangle (inst, prim) from ref_pos."""
+def light_pdf(pt_scene, inst, prim, ref_pos, surface_params=(0.0, 0.0)):
+    """Connect pdf (per area) of hitting emitter (inst, prim) from ref_pos;
+    surface_params: the hit's (u, v) on a quad."""
     p = np.ascontiguousarray(ref_pos, np.float32)
-    return float(_light_lib().orc_light_pdf(C.byref(pt_scene.s), inst, prim, _p(p, f32p)))
+    return float(_light_lib().orc_light_pdf(C.byref(pt_scene.s), inst, prim, _p(p, f32p), float(surface_params[0]),
+                                             float(surface_params[1])))
 
 
 def pt_render(pt_scene, seed, min_path_length, max_path_length, nee_count, iterations):

@@ -161,4 +161,147 @@ inline bool ray_triangle(const FastRay& r, V3 v0, V3 v1, V3 v2, float& t, float&
     return true;
 }
 
+// Component k (0, 1, 2) of v.
+inline float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Real roots x0 <= x1 of a x^2 + b x + c = 0 (the role of ei::solveSquarePoly):
+// q = -(b + sgn(b) sqrt(b^2 - 4ac)) / 2, roots q / a and c / q, the
+// cancellation-free form of Press et al., Numerical Recipes, 3rd ed., 5.6
+// (sgn(0) = 1). a == 0: the root -c / b of the linear equation, twice.
+inline bool solve_square_poly(float a, float b, float c, float& x0, float& x1) {
+    if (a == 0.0f) {
+        if (b == 0.0f) return false;
+        x0 = -c / b;
+        x1 = x0;
+        return true;
+    }
+    const float disc = b * b - 4.0f * a * c;
+    if (disc < 0.0f) return false;
+    const float r = std::sqrt(disc);
+    const float q = b < 0.0f ? -0.5f * (b - r) : -0.5f * (b + r);
+    if (q == 0.0f) {  // b == 0 and c == 0: the double root 0
+        x0 = 0.0f;
+        x1 = 0.0f;
+        return true;
+    }
+    const float ra = q / a, rb = c / q;
+    x0 = rb < ra ? rb : ra;
+    x1 = rb < ra ? ra : rb;
+    return true;
+}
+
+// intersection.cu:86-95 (computeU)
+inline float quad_u(float v, float A1, float A2, float B1, float B2, float C1, float C2, float D1, float D2) {
+    const float a = v * A2 + B2;
+    const float b = v * (A2 - A1) + B2 - B1;
+    if (std::fabs(b) >= std::fabs(a)) return (v * (C1 - C2) + D1 - D2) / b;
+    return (-v * C2 - D2) / a;
+}
+
+// Ramsey, Potter and Hansen, "Ray Bilinear Patch Intersections" (JGT 2004), in
+// the steps of intersection.cu:99-179 (intersectQuad): the patch
+//   P(u, v) = p00 + u (p10 - p00) + v (p01 - p00) + uv (p11 - p10 - p01 + p00),
+// equations along the largest direction component, roots in v, u per root, t
+// along the largest component. Returns the nearer hit with t > 0 (the first
+// root's when the patch is planar) and its (u, v).
+inline bool ray_quad(const FastRay& r, V3 p00, V3 p10, V3 p11, V3 p01, float& t, float& u, float& v) {
+    int d0 = 0, d1 = 1, d2 = 2;
+    if (std::fabs(r.d.x) < std::fabs(r.d.y) || std::fabs(r.d.x) < std::fabs(r.d.z)) {
+        if (std::fabs(r.d.y) >= std::fabs(r.d.z)) {
+            d0 = 1; d1 = 0; d2 = 2;
+        } else {
+            d0 = 2; d1 = 1; d2 = 0;
+        }
+    }
+    const V3 a = p11 - p10 - p01 + p00;
+    const V3 b = p10 - p00;
+    const V3 c = p01 - p00;
+    const V3 d = p00 - r.o;
+    const V3& dir = r.d;
+    const float A1 = a[d1] * dir[d0] - a[d0] * dir[d1];
+    const float A2 = a[d2] * dir[d0] - a[d0] * dir[d2];
+    const float B1 = b[d1] * dir[d0] - b[d0] * dir[d1];
+    const float B2 = b[d2] * dir[d0] - b[d0] * dir[d2];
+    const float C1 = c[d1] * dir[d0] - c[d0] * dir[d1];
+    const float C2 = c[d2] * dir[d0] - c[d0] * dir[d2];
+    const float D1 = d[d1] * dir[d0] - d[d0] * dir[d1];
+    const float D2 = d[d2] * dir[d0] - d[d0] * dir[d2];
+    float v0 = 0.0f, v1 = 0.0f;
+    if (!solve_square_poly(A2 * C1 - A1 * C2, A2 * D1 - A1 * D2 + B2 * C1 - B1 * C2, B2 * D1 - B1 * D2, v0, v1))
+        return false;
+    float u0 = 0.0f, u1 = 0.0f, t0 = -1.0f, t1 = -1.0f;
+    if (v0 >= 0.0f && v0 <= 1.0f) {
+        u0 = quad_u(v0, A1, A2, B1, B2, C1, C2, D1, D2);
+        if (u0 >= 0.0f && u0 <= 1.0f) t0 = (u0 * v0 * a[d0] + u0 * b[d0] + v0 * c[d0] + d[d0]) * r.inv[d0];
+    }
+    if (v1 == v0) {
+        t = t0; u = u0; v = v0;
+    } else {
+        if (v1 >= 0.0f && v1 <= 1.0f) {
+            u1 = quad_u(v1, A1, A2, B1, B2, C1, C2, D1, D2);
+            if (u1 >= 0.0f && u1 <= 1.0f) t1 = (u1 * v1 * a[d0] + u1 * b[d0] + v1 * c[d0] + d[d0]) * r.inv[d0];
+        }
+        if (t0 > 0.0f) {
+            if (t1 > 0.0f && t1 < t0) {
+                t = t1; u = u1; v = v1;
+            } else {
+                t = t0; u = u0; v = v0;
+            }
+        } else {
+            t = t1; u = u1; v = v1;
+        }
+    }
+    return t > 0.0f;
+}
+
+// Ray / sphere after Haines, Guenther and Akenine-Moeller, "Precision
+// Improvements for Ray/Sphere Intersection" (Ray Tracing Gems, 2019), the role
+// of ei::intersects(Ray, Sphere): discriminant from the closest approach
+// l = f + b d (f = o - center, b = -f.d, unit d), roots c / q and q with
+// q = b + sgn(b) sqrt(disc). t: the nearer root when positive, else the
+// farther one (origin inside); false when both are <= 0.
+inline bool ray_sphere(const FastRay& r, V3 center, float radius, float& t) {
+    const V3 f = r.o - center;
+    const float b = -dot(f, r.d);
+    const V3 l = f + r.d * b;
+    const float rr = radius * radius;
+    const float disc = rr - dot(l, l);
+    if (disc < 0.0f) return false;
+    const float c = dot(f, f) - rr;
+    const float s = std::sqrt(disc);
+    const float q = b < 0.0f ? b - s : b + s;
+    if (q == 0.0f) return false;
+    const float ra = c / q;
+    const float tn = ra < q ? ra : q, tf = ra < q ? q : ra;
+    t = tn > 0.0f ? tn : tf;
+    return tf > 0.0f;
+}
+
+// A vector perpendicular to v (the role of ei::perpendicular): the
+// smallest-magnitude component set to zero, the other two swapped with one
+// negated (Hughes and Moeller, "Building an Orthonormal Basis from a Unit
+// Vector", JGT 1999).
+inline V3 perpendicular(V3 v) {
+    const float ax = std::fabs(v.x), ay = std::fabs(v.y), az = std::fabs(v.z);
+    if (ax <= ay && ax <= az) return {0.0f, -v.z, v.y};
+    if (ay <= az) return {-v.z, 0.0f, v.x};
+    return {-v.y, v.x, 0.0f};
+}
+
+// lerp(lerp(x00, x01, s), lerp(x10, x11, s), t), lerp(a, b, s) = (1 - s) a + s b
+// (the role of ei::bilerp; exact at the corners).
+inline float bilerp(float x00, float x01, float x10, float x11, float s, float t) {
+    const float a = (1.0f - s) * x00 + s * x01;
+    const float b = (1.0f - s) * x10 + s * x11;
+    return (1.0f - t) * a + t * b;
+}
+inline V3 bilerp(V3 x00, V3 x01, V3 x10, V3 x11, float s, float t) {
+    return {bilerp(x00.x, x01.x, x10.x, x11.x, s, t), bilerp(x00.y, x01.y, x10.y, x11.y, s, t),
+            bilerp(x00.z, x01.z, x10.z, x11.z, s, t)};
+}
+
+// Box of four points, of a sphere (center -+ radius: the role of ei::Box(Sphere)).
+inline Box quad_box(V3 a, V3 b, V3 c, V3 d) { return {vmin(vmin(vmin(a, b), c), d), vmax(vmax(vmax(a, b), c), d)}; }
+inline Box sphere_box(V3 c, float r) { return {{c.x - r, c.y - r, c.z - r}, {c.x + r, c.y + r, c.z + r}}; }
+
 }  // namespace orc

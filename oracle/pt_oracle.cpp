@@ -191,7 +191,7 @@ struct OrcPtScene {
     OrcScene geo;
     const float* normals;     // 3 per vertex, concatenated like geo.verts
     const float* uvs;         // 2 per vertex
-    const uint16_t* matIdx;   // 1 per triangle, concatenated like geo.tris
+    const uint16_t* matIdx;   // 1 per primitive, concatenated like geo.primIds
     const OrcMaterial* materials;
     int32_t numMaterials;
     const uint8_t* lightMem;  // positional light subtree (Mufflon layout)
@@ -876,6 +876,7 @@ struct Vertex {
     V3 incEnc{0, 0, 0};  // (x, y, dist * sgn(z)) for surfaces, raw dir for void
     TS ts;
     i32 inst = -1, prim = -1;
+    float sp0 = 0.0f, sp1 = 0.0f;  // surface parameters of the hit (a quad's (u, v))
     MatParams mat;
     float incidentPdf = 1.0f;  // PtVertexExt
     V3 incident_dir() const {
@@ -904,12 +905,33 @@ struct HitInfo {
     bool hit;
     float t;
     i32 inst, prim;
-    float b0, b1;
+    int kind;        // 0 triangle, 1 quad, 2 sphere
+    float b0, b1;    // barycentrics of v0 and v1, or the quad's (u, v)
     V3 geoN, tangentX;
     float u, v;
 };
 
-// first_intersection (intersection.cu:366-621), triangle meshes.
+// Primitive counts and offsets of a LoD (triangles, quads, spheres in id order).
+inline i32 lod_tris(const OrcScene& g, i32 l) { return g.lodTriOffset[l + 1] - g.lodTriOffset[l]; }
+inline i32 lod_quads(const OrcScene& g, i32 l) {
+    return g.lodQuadOffset ? g.lodQuadOffset[l + 1] - g.lodQuadOffset[l] : 0;
+}
+inline i32 lod_prim_offset(const OrcScene& g, i32 l) { return g.lodPrimOffset ? g.lodPrimOffset[l] : g.lodTriOffset[l]; }
+inline int prim_kind(const OrcScene& g, i32 l, i32 prim) {
+    const i32 nt = lod_tris(g, l);
+    return prim < nt ? 0 : (prim < nt + lod_quads(g, l) ? 1 : 2);
+}
+inline const u32* quad_of(const OrcScene& g, i32 l, i32 prim) {
+    return g.quads + 4 * ((size_t)g.lodQuadOffset[l] + (prim - lod_tris(g, l)));
+}
+inline const float* sphere_of(const OrcScene& g, i32 l, i32 prim) {
+    return g.spheres + 4 * ((size_t)g.lodSphereOffset[l] + (prim - lod_tris(g, l) - lod_quads(g, l)));
+}
+
+// first_intersection (intersection.cu:366-621): the hit and its attributes
+// for triangles (:543-571), quads (:572-610) and spheres (:524-541; the
+// sphere of primitive id p is sphere p - (triangles + quads) of the LoD, as
+// in accel_structs_commons.hpp:56-60).
 HitInfo first_intersection(const OrcPtScene& S, V3& origin, V3 dir, V3 geoNormal, float tmax) {
     add_epsilon(origin, dir, geoNormal);
     const FastRay fr = make_fast_ray(origin, dir);
@@ -932,6 +954,57 @@ HitInfo first_intersection(const OrcPtScene& S, V3& origin, V3 dir, V3 geoNormal
     const OrcScene& g = S.geo;
     const i32 lod = g.instLod[h.inst];
     const i32 vb = g.lodVertOffset[lod], tb = g.lodTriOffset[lod];
+    r.kind = prim_kind(g, lod, h.prim);
+    const M34 M = mat_of(g, h.inst);
+    if (r.kind == 2) {
+        const float* sp = sphere_of(g, lod, h.prim);
+        const V3 hitPoint = origin + dir * hitT;
+        const V3 local = xform_point(M, hitPoint);
+        const V3 n = normalize(local - V3{sp[0], sp[1], sp[2]});
+        V3 tangentX{1.0f, 0.0f, 0.0f};
+        if (!(n.x == 0.0f && n.y == 0.0f)) {
+            const float l2 = std::sqrt(n.y * n.y + n.x * n.x);
+            tangentX = {n.y / l2, -n.x / l2, 0.0f};
+        }
+        r.u = fatan2(n.z, n.x) / (2.0f * kPi) + 0.5f;
+        r.v = facos(-n.y) / kPi;
+        r.geoN = normalize(xform_normal(M, n));
+        r.tangentX = normalize(xform_normal(M, tangentX));
+        return r;
+    }
+    if (r.kind == 1) {
+        const u32* q = quad_of(g, lod, h.prim);
+        V3 v[4];
+        float uv[4][2];
+        for (int k = 0; k < 4; ++k) {
+            const float* p = g.verts + 3 * ((size_t)vb + q[k]);
+            v[k] = {p[0], p[1], p[2]};
+            const float* w = S.uvs + 2 * ((size_t)vb + q[k]);
+            uv[k][0] = w[0];
+            uv[k][1] = w[1];
+        }
+        const float su = h.w0, sv = h.w1;
+        const V3 dxds = (1.f - su) * (v[3] - v[0]) + su * (v[2] - v[1]);
+        const V3 dxdt = (1.f - sv) * (v[1] - v[0]) + sv * (v[2] - v[3]);
+        const float dudsx = (1.f - su) * (uv[3][0] - uv[0][0]) + su * (uv[2][0] - uv[1][0]);
+        const float dudsy = (1.f - su) * (uv[3][1] - uv[0][1]) + su * (uv[2][1] - uv[1][1]);
+        const float dudtx = (1.f - sv) * (uv[1][0] - uv[0][0]) + sv * (uv[2][0] - uv[3][0]);
+        const float dudty = (1.f - sv) * (uv[1][1] - uv[0][1]) + sv * (uv[2][1] - uv[3][1]);
+        const float det = dudsx * dudty - dudtx * dudsy;
+        V3 tangentX;
+        if (det >= 1e-9f || det <= -1e-9f) {
+            const float a = dudty / det, b = -dudsy / det;
+            tangentX = {dxds.x * a + dxdt.x * b, dxds.y * a + dxdt.y * b, dxds.z * a + dxdt.z * b};
+        } else {
+            tangentX = dxds;
+        }
+        const V3 geoN = cross(dxdt, dxds);
+        r.u = bilerp(uv[0][0], uv[1][0], uv[3][0], uv[2][0], su, sv);
+        r.v = bilerp(uv[0][1], uv[1][1], uv[3][1], uv[2][1], su, sv);
+        r.geoN = normalize(xform_normal(M, geoN));
+        r.tangentX = normalize(xform_normal(M, tangentX));
+        return r;
+    }
     const u32* ix = g.tris + 3 * ((size_t)tb + h.prim);
     V3 v[3];
     float uvs[3][2];
@@ -956,7 +1029,6 @@ HitInfo first_intersection(const OrcPtScene& S, V3& origin, V3 dir, V3 geoNormal
     const float w2 = 1.f - h.w0 - h.w1;
     r.u = uvs[0][0] * h.w0 + uvs[1][0] * h.w1 + uvs[2][0] * w2;
     r.v = uvs[0][1] * h.w0 + uvs[1][1] * h.w1 + uvs[2][1] * w2;
-    const M34 M = mat_of(g, h.inst);
     r.geoN = normalize(xform_normal(M, geoN));
     r.tangentX = normalize(xform_normal(M, tangentX));
     return r;
@@ -977,28 +1049,43 @@ bool any_intersection(const OrcPtScene& S, V3 a, V3 b, V3 geoNA, V3 geoNB, V3 di
     return traverse(S.geo, fr, tmax, h, true, nullptr, nullptr);
 }
 
-// tangent_space_geom_to_shader (tangent_space.hpp:10-60), triangles
+// tangent_space_geom_to_shader (tangent_space.hpp:10-60): spheres keep the
+// geometric frame; triangles and quads interpolate the vertex normals
+// (barycentric, bilinear) and orthogonalize the tangent.
 TS tangent_space(const OrcPtScene& S, const HitInfo& hit) {
+    if (hit.kind == 2) return {hit.geoN, hit.geoN, hit.tangentX, cross(hit.geoN, hit.tangentX)};
     const OrcScene& g = S.geo;
     const i32 lod = g.instLod[hit.inst];
     const i32 vb = g.lodVertOffset[lod], tb = g.lodTriOffset[lod];
-    const u32* ix = g.tris + 3 * ((size_t)tb + hit.prim);
-    V3 n[3];
-    for (int k = 0; k < 3; ++k) {
-        const float* p = S.normals + 3 * ((size_t)vb + ix[k]);
-        n[k] = {p[0], p[1], p[2]};
+    V3 sn;
+    if (hit.kind == 1) {
+        const u32* q = quad_of(g, lod, hit.prim);
+        V3 n[4];
+        for (int k = 0; k < 4; ++k) {
+            const float* p = S.normals + 3 * ((size_t)vb + q[k]);
+            n[k] = {p[0], p[1], p[2]};
+        }
+        sn = bilerp(n[0], n[1], n[3], n[2], hit.b0, hit.b1);
+    } else {
+        const u32* ix = g.tris + 3 * ((size_t)tb + hit.prim);
+        V3 n[3];
+        for (int k = 0; k < 3; ++k) {
+            const float* p = S.normals + 3 * ((size_t)vb + ix[k]);
+            n[k] = {p[0], p[1], p[2]};
+        }
+        sn = hit.b0 * n[0] + hit.b1 * n[1] + (1.f - (hit.b0 + hit.b1)) * n[2];
     }
-    V3 sn = hit.b0 * n[0] + hit.b1 * n[1] + (1.f - (hit.b0 + hit.b1)) * n[2];
     sn = normalize(xform_normal(mat_of(g, hit.inst), sn));
     const V3 tx = normalize(hit.tangentX - sn * dot(hit.tangentX, sn));
     const V3 ty = cross(sn, tx);
     return {sn, hit.geoN, tx, ty};
 }
 
+int material_index(const OrcPtScene& S, i32 inst, i32 prim) {
+    return S.matIdx[(size_t)lod_prim_offset(S.geo, S.geo.instLod[inst]) + prim];
+}
 const OrcMaterial& material_of(const OrcPtScene& S, i32 inst, i32 prim) {
-    const i32 lod = S.geo.instLod[inst];
-    const i32 tb = S.geo.lodTriOffset[lod];
-    return S.materials[S.matIdx[(size_t)tb + prim]];
+    return S.materials[material_index(S, inst, prim)];
 }
 
 // ---------------------------------------------------------------- background
@@ -1144,6 +1231,19 @@ struct PointLightRec {
     float intensity[3];
     float pad1;
 };
+struct QuadLight {  // AreaLightQuad record (lights.hpp:108-124), 96 bytes
+    float posV[4][3];  // v0, v3 - v0, v1 - v0, v2 - v3 - (v1 - v0)
+    float uvV[4][2];
+    uint16_t material;
+    uint16_t pad[7];
+};
+struct SphereLight {  // AreaLightSphere record (lights.hpp:125-137), 32 bytes
+    float position[3];
+    float radius;
+    uint16_t material;
+    uint16_t pad[7];
+};
+static_assert(sizeof(QuadLight) == 96 && sizeof(SphereLight) == 32, "light records");
 struct NodeRec {
     float leftFlux;
     uint32_t leftOffset;
@@ -1173,6 +1273,97 @@ Nee connect_light(const OrcPtScene& S, uint16_t type, const uint8_t* light, V3 p
         n.dist = dist;
         n.geoNormal = {0, 0, 0};
         n.distSq = distSq;
+        return n;
+    }
+    if (type == 3) {  // area quad (light_sampling.hpp:142-171, 376-399)
+        QuadLight L;
+        std::memcpy(&L, light, sizeof(L));
+        V3 pv[4];
+        for (int k = 0; k < 4; ++k) pv[k] = {L.posV[k][0], L.posV[k][1], L.posV[k][2]};
+        const V3 tangentX = pv[1] + u0 * pv[3];
+        const V3 tangentY = pv[2] + u1 * pv[3];
+        V3 normal = cross(tangentY, tangentX);
+        const float area = len(normal);
+        normal = normal / area;
+        const V3 position = pv[0] + tangentX * u1 + pv[2] * u0;
+        const float lu = L.uvV[0][0] + L.uvV[1][0] * u1 + L.uvV[2][0] * u0 + L.uvV[3][0] * (u0 * u1);
+        const float lv = L.uvV[0][1] + L.uvV[1][1] * u1 + L.uvV[2][1] * u0 + L.uvV[3][1] * (u0 * u1);
+        const MatParams m = fetch(S, S.materials[L.material], lu, lv);
+        const V3 rad = mat_emission(m, {0, 0, 1}, {0, 0, 1});
+        const V3 intensity = rad * area;
+        V3 d = position - pos;
+        const float distSq = dot(d, d) + kDistSqEps;
+        const float dist = std::sqrt(distSq);
+        d = d / dist;
+        const float cosOut = dot(normal, -d);
+        V3 value{0, 0, 0};
+        float vc = 0.0f;
+        if (cosOut > 0.0f) {
+            value = intensity;
+            vc = cosOut;
+        }
+        n.position = position;
+        n.creationPdf = 1.0f / area;
+        n.dir = d;
+        n.cosOut = vc;
+        n.diffIrradiance = V3{value.x / distSq, value.y / distSq, value.z / distSq} * transmission(m.outer, dist);
+        n.dist = dist;
+        n.geoNormal = normal;
+        n.distSq = distSq;
+        n.lightMaterial = L.material;
+        return n;
+    }
+    if (type == 4) {  // area sphere (light_sampling.hpp:400-432): the visible cap, uniform in area
+        SphereLight L;
+        std::memcpy(&L, light, sizeof(L));
+        const V3 c{L.position[0], L.position[1], L.position[2]};
+        V3 centerDir = pos - c;
+        const float cd = len(centerDir);
+        n.lightMaterial = L.material;
+        if (cd <= L.radius) {  // inside: no estimate
+            n.position = {0, 0, 0};
+            n.creationPdf = 0.0f;
+            n.dir = {0, 0, 0};
+            n.cosOut = 0.0f;
+            n.diffIrradiance = {0, 0, 0};
+            n.dist = 0.0f;
+            n.geoNormal = {0, 0, 0};
+            n.distSq = 0.0f;
+            return n;
+        }
+        centerDir = centerDir / cd;
+        const float cosSphere = L.radius / cd;
+        // sample_cone_uniform (sampling.hpp:112-119)
+        const float cosTheta = (1.f - u0) + u0 * cosSphere;
+        const float sinTheta = std::sqrt(1.f - cosTheta * cosTheta);
+        const float phi = 2.f * u1 * kPi;
+        const V3 local{fcos(phi) * sinTheta, fsin(phi) * sinTheta, cosTheta};
+        const float conePdf = 1.f / (2.f * kPi * (1.f - cosSphere));
+        const V3 tangentX = normalize(perpendicular(centerDir));
+        const V3 tangentY = cross(centerDir, tangentX);
+        const V3 globalDir = local.x * tangentX + local.y * tangentY + local.z * centerDir;
+        const V3 surfPos = c + L.radius * globalDir;
+        V3 cdir = surfPos - pos;
+        const float cDistSq = dot(cdir, cdir) + kDistSqEps;
+        const float cDist = std::sqrt(cDistSq);
+        cdir = cdir / cDist;
+        // direction_to_uv (textures/interface.hpp:169-174)
+        const float tv = facos(globalDir.y) / kPi;
+        const float tu = fatan2(globalDir.z, globalDir.x) / (kPi * 2.0f);
+        const MatParams m = fetch(S, S.materials[L.material], tu, 1.0f - tv);
+        V3 radiance = mat_emission(m, {0, 0, 1}, {0, 0, 1});
+        const float sampleArea = L.radius * L.radius / conePdf;
+        const float cosOut = mx(0.0f, -dot(globalDir, cdir));
+        radiance = radiance * (sampleArea / cDistSq);
+        radiance = radiance * transmission(m.outer, cDist);
+        n.position = surfPos;
+        n.creationPdf = 1.0f / sampleArea;
+        n.dir = cdir;
+        n.cosOut = cosOut;
+        n.diffIrradiance = radiance;
+        n.dist = cDist;
+        n.geoNormal = globalDir;
+        n.distSq = cDistSq;
         return n;
     }
     // area triangle (light_sampling.hpp:118-140, 350-375)
@@ -1251,10 +1442,17 @@ V3 light_center(const OrcPtScene& S, uint32_t offset, uint16_t type) {
         std::memcpy(&n, p, sizeof(n));
         return {n.center[0], n.center[1], n.center[2]};
     }
-    if (type == 0) {
+    if (type == 0 || type == 4) {  // point light position, sphere center
         PointLightRec L;
         std::memcpy(&L, p, sizeof(L));
         return {L.position[0], L.position[1], L.position[2]};
+    }
+    if (type == 3) {  // quad: posV[0] + (posV[1] + posV[2]) / 2 + posV[3] / 4
+        QuadLight q;
+        std::memcpy(&q, p, sizeof(q));
+        V3 pv[4];
+        for (int k = 0; k < 4; ++k) pv[k] = {q.posV[k][0], q.posV[k][1], q.posV[k][2]};
+        return pv[0] + (pv[1] + pv[2]) * 0.5f + pv[3] * 0.25f;
     }
     TriLight t;
     std::memcpy(&t, p, sizeof(t));
@@ -1320,8 +1518,9 @@ Nee connect(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, V3 pos, fl
 }
 
 // light_pdf (light_tree_sampling.hpp:340-413) -> connect pdf of a hit emitter
-// seen from refPos, the previous vertex of the path (path_util.hpp:443-444)
-float light_pdf(const OrcPtScene& S, i32 inst, i32 prim, V3 refPos) {
+// seen from refPos, the previous vertex of the path (path_util.hpp:443-444);
+// (sp0, sp1): the hit's surface parameters (a quad's (u, v)).
+float light_pdf(const OrcPtScene& S, i32 inst, i32 prim, V3 refPos, float sp0, float sp1) {
     const float treeFlux = S.dirRootFlux + S.posRootFlux + S.bgFlux;
     float p = S.posRootFlux / treeFlux;
     const u64 key = ((u64)(u32)inst << 32) | (u64)(u32)prim;
@@ -1356,6 +1555,23 @@ float light_pdf(const OrcPtScene& S, i32 inst, i32 prim, V3 refPos) {
             offset = node.leftOffset;
         }
         code <<= 1;
+    }
+    if (type == 3) {  // the local area density at (sp0, sp1)
+        QuadLight q;
+        std::memcpy(&q, S.lightMem + offset, sizeof(q));
+        V3 pv[4];
+        for (int k = 0; k < 4; ++k) pv[k] = {q.posV[k][0], q.posV[k][1], q.posV[k][2]};
+        const V3 tangentX = pv[1] + sp0 * pv[3];
+        const V3 tangentY = pv[2] + sp1 * pv[3];
+        return p / len(cross(tangentY, tangentX));
+    }
+    if (type == 4) {  // connect pdf: uniform over the cap visible from refPos
+        SphereLight sl;
+        std::memcpy(&sl, S.lightMem + offset, sizeof(sl));
+        const float cosSphere = sl.radius / len(refPos - V3{sl.position[0], sl.position[1], sl.position[2]});
+        const float solidAngle = 2 * kPi * (1.0f - cosSphere);
+        const float rsq = sl.radius * sl.radius;
+        return p / (solidAngle * rsq);
     }
     TriLight L;
     std::memcpy(&L, S.lightMem + offset, sizeof(L));
@@ -1535,6 +1751,8 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
             next.incEnc = {excident.x, excident.y, hit.t * sgnf(excident.z)};
             next.inst = hit.inst;
             next.prim = hit.prim;
+            next.sp0 = hit.b0;
+            next.sp1 = hit.b1;
             if (tl_stats) ++tl_stats->surfaceVertices;
             next.mat = fetch(S, material_of(S, hit.inst, hit.prim), hit.u, hit.v);
             hu = hit.u;
@@ -1559,7 +1777,7 @@ void pt_sample(const OrcPtScene& S, const int params[4], int px, int py, Pcg64& 
                 if (nonzero(emission)) connectPdf = background_pdf(S, b.pdfBack);
             } else {
                 emission = mat_emission(vtx.mat, vtx.ts.geoN, -vtx.incident_dir());
-                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos);
+                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos, vtx.sp0, vtx.sp1);
             }
             if (nonzero(emission) && pathLen > 1) {
                 float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
@@ -1756,8 +1974,9 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
             next.incEnc = {excident.x, excident.y, hit.t * sgnf(excident.z)};
             next.inst = hit.inst;
             next.prim = hit.prim;
-            const i32 lod = S.geo.instLod[hit.inst];
-            matIdx = S.matIdx[(size_t)S.geo.lodTriOffset[lod] + hit.prim];
+            next.sp0 = hit.b0;
+            next.sp1 = hit.b1;
+            matIdx = material_index(S, hit.inst, hit.prim);
             next.mat = fetch(S, S.materials[matIdx], hit.u, hit.v);
             const float inCosAbs = std::fabs(dot(excident, next.ts.shadingN));
             next.incidentPdf = pdfF * std::fabs(inCosAbs) / (hit.t * hit.t);
@@ -1782,7 +2001,7 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
                 if (nonzero(emission)) connectPdf = background_pdf(S, b.pdfBack);
             } else {
                 emission = mat_emission(vtx.mat, vtx.ts.geoN, -vtx.incident_dir());
-                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos);
+                if (nonzero(emission)) connectPdf = light_pdf(S, vtx.inst, vtx.prim, prevPos, vtx.sp0, vtx.sp1);
             }
             if (nonzero(emission) && pathLen > 1) {
                 const float mis = 1.0f / (1.0f + (float)neeCount * (connectPdf / vtx.incidentPdf));
@@ -2250,8 +2469,8 @@ int orc_light_connect(const OrcPtScene* S, uint64_t index, uint64_t numIndices, 
     out8[7] = (float)leaf;
     return 0;
 }
-float orc_light_pdf(const OrcPtScene* S, int32_t inst, int32_t prim, const float* refPos) {
-    return light_pdf(*S, inst, prim, {refPos[0], refPos[1], refPos[2]});
+float orc_light_pdf(const OrcPtScene* S, int32_t inst, int32_t prim, const float* refPos, float sp0, float sp1) {
+    return light_pdf(*S, inst, prim, {refPos[0], refPos[1], refPos[2]}, sp0, sp1);
 }
 int orc_background_evaluate(const OrcPtScene* S, const float* dir, float* out4) {
     const BgEval b = evaluate_background(*S, {dir[0], dir[1], dir[2]});

@@ -23,6 +23,14 @@ struct OrcScene {
     const void* tlasNodes;
     int32_t tlasNumInternal;
     const int32_t* tlasPrimIds;
+    // LoDs with quads and spheres (all may be null: triangles only). A LoD's
+    // primitive ids run triangles, quads, spheres; its sorted primitive ids
+    // then start at lodPrimOffset[l] instead of lodTriOffset[l].
+    const int32_t* lodQuadOffset;    // numLods+1 (in quads)
+    const uint32_t* quads;           // 4 local vertex indices per quad (p00, p10, p11, p01)
+    const int32_t* lodSphereOffset;  // numLods+1 (in spheres)
+    const float* spheres;            // center xyz, radius
+    const int32_t* lodPrimOffset;    // numLods+1 (in primitives)
 };
 }
 

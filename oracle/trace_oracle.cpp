@@ -3,7 +3,7 @@
 // CPU restatement of Mufflon's BVH traversal (reference CPU path):
 //   intersection.cu:16-24    add_epsilon
 //   intersection.cu:57-75    slab test, CPU branch (early outs, no clamp)
-//   intersection.cu:181-240  triangle branch of intersects_primitve
+//   intersection.cu:181-316  intersects_primitve (triangle, quad, sphere)
 //   intersection.cu:321-361  world_to_object_space / object_to_world_space
 //   intersection.cu:366-621  first_intersection (stack traversal, two levels)
 //   intersection.cu:624-753  any_intersection
@@ -41,19 +41,27 @@ struct Lod {
     const i32* primIds;
     i32 numInternal;
     i32 numPrims;
+    i32 nt, nq, ns;
     const float* verts;
     const u32* tris;
+    const u32* quads;
+    const float* spheres;
     Box box;
 };
 
 Lod get_lod(const OrcScene& s, i32 l) {
     Lod o;
     o.bvh = (const Node*)s.nodes + 2 * (size_t)s.lodNodeOffset[l];
-    o.primIds = s.primIds + s.lodTriOffset[l];
     o.numInternal = s.lodNumInternal[l];
-    o.numPrims = s.lodTriOffset[l + 1] - s.lodTriOffset[l];
+    o.nt = s.lodTriOffset[l + 1] - s.lodTriOffset[l];
+    o.nq = s.lodQuadOffset ? s.lodQuadOffset[l + 1] - s.lodQuadOffset[l] : 0;
+    o.ns = s.lodSphereOffset ? s.lodSphereOffset[l + 1] - s.lodSphereOffset[l] : 0;
+    o.numPrims = o.nt + o.nq + o.ns;
+    o.primIds = s.primIds + (s.lodPrimOffset ? s.lodPrimOffset[l] : s.lodTriOffset[l]);
     o.verts = s.verts + 3 * (size_t)s.lodVertOffset[l];
     o.tris = s.tris + 3 * (size_t)s.lodTriOffset[l];
+    o.quads = o.nq ? s.quads + 4 * (size_t)s.lodQuadOffset[l] : nullptr;
+    o.spheres = o.ns ? s.spheres + 4 * (size_t)s.lodSphereOffset[l] : nullptr;
     const float* b = s.lodBoxes + 6 * l;
     o.box = {{b[0], b[1], b[2]}, {b[3], b[4], b[5]}};
     return o;
@@ -100,13 +108,24 @@ inline bool better(float t, i32 inst, i32 prim, float hitT, i32 hInst, i32 hPrim
     return prim < hPrim;
 }
 
-inline bool tri_test(const Lod& L, const FastRay& r, i32 prim, float& t, float& w0, float& w1) {
-    const u32* ix = L.tris + 3 * (size_t)prim;
-    const float* v = L.verts;
-    const V3 a{v[3 * ix[0]], v[3 * ix[0] + 1], v[3 * ix[0] + 2]};
-    const V3 b{v[3 * ix[1]], v[3 * ix[1] + 1], v[3 * ix[1] + 2]};
-    const V3 c{v[3 * ix[2]], v[3 * ix[2] + 1], v[3 * ix[2] + 2]};
-    return ray_triangle(r, a, b, c, t, w0, w1);
+inline V3 lod_vert(const Lod& L, u32 i) { return {L.verts[3 * i], L.verts[3 * i + 1], L.verts[3 * i + 2]}; }
+
+// intersects_primitve (intersection.cu:181-316) without alpha testing: a
+// triangle (accepted with t > 0), a bilinear quad or a sphere (their tests
+// return only hits with t > 0). (w0, w1): barycentrics of v0 and v1, or the
+// quad's (u, v).
+inline bool prim_test(const Lod& L, const FastRay& r, i32 prim, float& t, float& w0, float& w1) {
+    if (prim < L.nt) {
+        const u32* ix = L.tris + 3 * (size_t)prim;
+        return ray_triangle(r, lod_vert(L, ix[0]), lod_vert(L, ix[1]), lod_vert(L, ix[2]), t, w0, w1) && t > 0.0f;
+    }
+    if (prim < L.nt + L.nq) {
+        const u32* ix = L.quads + 4 * (size_t)(prim - L.nt);
+        return ray_quad(r, lod_vert(L, ix[0]), lod_vert(L, ix[1]), lod_vert(L, ix[2]), lod_vert(L, ix[3]), t, w0, w1);
+    }
+    const float* sp = L.spheres + 4 * (size_t)(prim - L.nt - L.nq);
+    w0 = w1 = 0.0f;
+    return ray_sphere(r, {sp[0], sp[1], sp[2]}, sp[3], t);
 }
 
 // world_to_object_space (intersection.cu:321-349)
@@ -226,8 +245,7 @@ bool traverse(const OrcScene& s, const FastRay& fray, float& hitT, Hit& h, bool 
                     if (obj.numPrims == 1) {
                         float t, w0, w1;
                         if (primCount_) ++*primCount_;
-                        if (tri_test(obj, cur, 0, t, w0, w1) && t > 0.0f &&
-                            better(t, primId, 0, hitT, h.inst, h.prim)) {
+                        if (prim_test(obj, cur, 0, t, w0, w1) && better(t, primId, 0, hitT, h.inst, h.prim)) {
                             hitT = t;
                             h.inst = primId;
                             h.prim = 0;
@@ -255,8 +273,7 @@ bool traverse(const OrcScene& s, const FastRay& fray, float& hitT, Hit& h, bool 
             } else {
                 float t, w0, w1;
                 if (primCount_) ++*primCount_;
-                if (tri_test(obj, cur, primId, t, w0, w1) && t > 0.0f &&
-                    better(t, curInst, primId, hitT, h.inst, h.prim)) {
+                if (prim_test(obj, cur, primId, t, w0, w1) && better(t, curInst, primId, hitT, h.inst, h.prim)) {
                     hitT = t;
                     h.inst = curInst;
                     h.prim = primId;
@@ -302,6 +319,30 @@ int orc_trace_closest(const OrcScene* s, const float* rays, int32_t numRays, flo
     return 0;
 }
 
+// The primitive tests alone (closed-form checks): ray = origin xyz, direction
+// xyz (unit). Quad: verts = p00, p10, p11, p01 (12 floats); out = t, u, v.
+int orc_ray_quad(const float* ray, const float* verts, float* out) {
+    const FastRay r = make_fast_ray({ray[0], ray[1], ray[2]}, {ray[3], ray[4], ray[5]});
+    V3 p[4];
+    for (int k = 0; k < 4; ++k) p[k] = {verts[3 * k], verts[3 * k + 1], verts[3 * k + 2]};
+    return ray_quad(r, p[0], p[1], p[2], p[3], out[0], out[1], out[2]) ? 1 : 0;
+}
+// Sphere = center xyz, radius; out = t.
+int orc_ray_sphere(const float* ray, const float* sphere, float* out) {
+    const FastRay r = make_fast_ray({ray[0], ray[1], ray[2]}, {ray[3], ray[4], ray[5]});
+    return ray_sphere(r, {sphere[0], sphere[1], sphere[2]}, sphere[3], out[0]) ? 1 : 0;
+}
+int orc_solve_square_poly(float a, float b, float c, float* out) {
+    return solve_square_poly(a, b, c, out[0], out[1]) ? 1 : 0;
+}
+void orc_perpendicular(const float* v, float* out) {
+    const V3 p = perpendicular({v[0], v[1], v[2]});
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+}
+float orc_bilerp(const float* x, float s, float t) { return bilerp(x[0], x[1], x[2], x[3], s, t); }
+
 // Any hit (occlusion) for the same ray records; 1 = occluded.
 int orc_trace_any(const OrcScene* s, const float* rays, int32_t numRays, int32_t* outHit) {
 #pragma omp parallel for schedule(dynamic, 1024)
@@ -335,7 +376,7 @@ int orc_trace_brute(const OrcScene* s, const float* rays, int32_t numRays, float
             float tmaxObj = bestT * sc;
             for (i32 p = 0; p < L.numPrims; ++p) {
                 float t, w0, w1;
-                if (tri_test(L, fr, p, t, w0, w1) && t > 0.0f && better(t, inst, p, tmaxObj, bi, bp)) {
+                if (prim_test(L, fr, p, t, w0, w1) && better(t, inst, p, tmaxObj, bi, bp)) {
                     tmaxObj = t;
                     bestT = t / sc;
                     bi = inst;

@@ -194,7 +194,7 @@ def oracle_pt_scene_general(O, sc, light_tree=None, box=None, background_scale=1
     lights, textures, an optional polar env map): the oracle's own LBVHs and
     instance level; without light_tree / box the host builders make them.
     `parallel` builds the LoD BVHs with the OpenMP stages (same output)."""
-    geoms = [(L["verts"], L["tris"]) for L in sc["lods"]]
+    geoms = [(L["verts"], L["tris"], L.get("quads"), L.get("spheres")) for L in sc["lods"]]
     scene, lods = U.oracle_scene(O, geoms, sc["inst_lod"], sc["world_to_inst"], parallel)
     if box is None:
         boxes = np.stack([L["box"] for L in lods])

@@ -16,16 +16,25 @@ def random_transforms(n, seed, scale=(0.5, 2.0), spread=4.0):
     return w
 
 
-def oracle_lod(O, verts, tris, parallel=False):
-    """Oracle LoD record; `parallel` builds with the OpenMP stages (same output)."""
+def oracle_lod(O, verts, tris, parallel=False, quads=None, spheres=None):
+    """Oracle LoD record; `parallel` builds with the OpenMP stages (same
+    output). With quads (n x 4) or spheres (n x 4) the mixed build."""
+    if (quads is not None and len(quads)) or (spheres is not None and len(spheres)):
+        q = np.zeros((0, 4), np.uint32) if quads is None else quads
+        sp = np.zeros((0, 4), np.float32) if spheres is None else spheres
+        b = O.lbvh_build_mixed(verts, tris, q, sp)
+        return dict(verts=verts, tris=tris, quads=q, spheres=sp, nodes=b["nodes"], num_internal=b["num_internal"],
+                    prim_ids=b["sorted_ids"], box=b["bbox"]), b
     b = (O.lbvh_build_tris_omp if parallel else O.lbvh_build_tris)(verts, tris)
     return dict(verts=verts, tris=tris, nodes=b["nodes"], num_internal=b["num_internal"],
                 prim_ids=b["sorted_ids"], box=b["bbox"]), b
 
 
 def oracle_scene(O, lod_geoms, inst_lod, w, parallel=False):
-    """Build LoD BVHs and (for >1 instance) the instance BVH with the oracle."""
-    lods = [oracle_lod(O, v, t, parallel)[0] for v, t in lod_geoms]
+    """Build LoD BVHs and (for >1 instance) the instance BVH with the oracle.
+    lod_geoms: (verts, tris) or (verts, tris, quads, spheres) per LoD."""
+    lods = [oracle_lod(O, g[0], g[1], parallel, *g[2:]) for g in lod_geoms]
+    lods = [L for L, _ in lods]
     tlas = None
     if len(inst_lod) > 1:
         boxes = np.stack([L["box"] for L in lods])
