@@ -7,8 +7,10 @@
 // iteration's seed (drawn on the host from the renderer's own generator),
 // the pinhole camera ray is jittered as the path tracer's, and the ray walks
 // through the scene hit after hit (up to maxTraceDepth traversals): at each
-// hit, the closest point on the hit triangle's edges (in world space) is
-// measured against the camera ray; within lineWidth pixel footprints the
+// hit, the closest point on the hit triangle's or quad's edges (in world
+// space) is measured against the camera ray (for a sphere, 0.05 r times the
+// cosine between the ray and the direction to the center); within lineWidth
+// pixel footprints the
 // pixel gets the border colour 1, otherwise the walk continues from the hit
 // (at least 1e-3 further). A walk that leaves the scene adds the background
 // radiance if it is positive. Values go into the running mean (and
@@ -39,6 +41,25 @@ __device__ __forceinline__ V3 closest_line_point(V3 A, V3 B, V3 C, V3 hit) {
     if (dAB <= dAC && dAB <= dBC) return onAB;
     if (dAC <= dAB && dAC <= dBC) return onAC;
     return onBC;
+}
+
+// Closest point of a quad's four edge lines (A = v0, B = v1, C = v3, D = v2:
+// edges AB, AC, BD, CD; computeClosestLinePoint, wireframe_common.hpp:42-77).
+__device__ __forceinline__ V3 closest_quad_line_point(V3 A, V3 B, V3 C, V3 D, V3 hit) {
+    const V3 AB = B - A, AC = C - A, BD = D - B, CD = D - C;
+    const V3 AP = hit - A, BP = hit - B, CP = hit - C;
+    const V3 onAB = A + AB * (dot(AP, AB) / dot(AB, AB));
+    const V3 onAC = A + AC * (dot(AP, AC) / dot(AC, AC));
+    const V3 onBD = B + BD * (dot(BP, BD) / dot(BD, BD));
+    const V3 onCD = C + CD * (dot(CP, CD) / dot(CD, CD));
+    const float dAB = dot(onAB - hit, onAB - hit);
+    const float dAC = dot(onAC - hit, onAC - hit);
+    const float dBD = dot(onBD - hit, onBD - hit);
+    const float dCD = dot(onCD - hit, onCD - hit);
+    if (dAB <= dAC && dAB <= dBD && dAB <= dCD) return onAB;
+    if (dAC <= dAB && dAC <= dBD && dAC <= dCD) return onAC;
+    if (dBD <= dAB && dBD <= dAC && dBD <= dCD) return onBD;
+    return onCD;
 }
 
 template <int kMode>
@@ -86,14 +107,29 @@ __global__ __launch_bounds__(kWfThreads) void k_wireframe(PtScene S, const u32* 
         // the hit triangle in world space (compute_instance_to_world_transformation)
         const M34 M = inst_to_world(S.trace.worldToInst + 12 * (size_t)h.inst);
         const DevPtLod L = S.ptLods[S.trace.instLod[h.inst]];
-        const u32 i0 = L.tris[3 * (size_t)h.prim], i1 = L.tris[3 * (size_t)h.prim + 1],
-                  i2 = L.tris[3 * (size_t)h.prim + 2];
-        const V3 A = xform_point(M, v3(L.verts[3 * (size_t)i0], L.verts[3 * (size_t)i0 + 1], L.verts[3 * (size_t)i0 + 2]));
-        const V3 B = xform_point(M, v3(L.verts[3 * (size_t)i1], L.verts[3 * (size_t)i1 + 1], L.verts[3 * (size_t)i1 + 2]));
-        const V3 C = xform_point(M, v3(L.verts[3 * (size_t)i2], L.verts[3 * (size_t)i2 + 1], L.verts[3 * (size_t)i2 + 2]));
-        const V3 lineSegment = closest_line_point(A, B, C, hit) - camPos;
-        const V3 projected = lineSegment - dir * dot(lineSegment, dir);
-        const float projDistToRim = len(projected);
+        float projDistToRim;
+        if (h.prim >= L.numTris + L.numQuads) {
+            // sphere (computeDistToRim, wireframe_common.hpp:79-89, scaled by 0.05 at :168)
+            const float4 sp = L.spheres[h.prim - L.numTris - L.numQuads];
+            const V3 center = xform_point(M, v3(sp.x, sp.y, sp.z));
+            projDistToRim = 0.05f * (sp.w * dot(normalize(center - hit), dir));
+        } else {
+            V3 closest;
+            if (h.prim >= L.numTris) {
+                const uint32_t* q = L.quads + 4 * (size_t)(h.prim - L.numTris);
+                const V3 A = xform_point(M, lod_vert(L, q[0])), B = xform_point(M, lod_vert(L, q[1]));
+                const V3 C = xform_point(M, lod_vert(L, q[3])), D = xform_point(M, lod_vert(L, q[2]));
+                closest = closest_quad_line_point(A, B, C, D, hit);
+            } else {
+                const u32 i0 = L.tris[3 * (size_t)h.prim], i1 = L.tris[3 * (size_t)h.prim + 1],
+                          i2 = L.tris[3 * (size_t)h.prim + 2];
+                closest = closest_line_point(xform_point(M, lod_vert(L, i0)), xform_point(M, lod_vert(L, i1)),
+                                             xform_point(M, lod_vert(L, i2)), hit);
+            }
+            const V3 lineSegment = closest - camPos;
+            const V3 projected = lineSegment - dir * dot(lineSegment, dir);
+            projDistToRim = len(projected);
+        }
         if (projDistToRim > lineWidth * pixelSize) {
             origin = origin + dir * hitDist;
         } else {

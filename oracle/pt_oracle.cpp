@@ -2395,24 +2395,59 @@ int orc_wireframe_render(const OrcPtScene* S, int32_t iterations, float lineWidt
                 const OrcScene& g = S->geo;
                 const M34 M = instance_to_world(g.worldToInst + 12 * (size_t)hit.inst);
                 const i32 lod = g.instLod[hit.inst];
-                const u32* ix = g.tris + 3 * ((size_t)g.lodTriOffset[lod] + hit.prim);
-                V3 v[3];
-                for (int k = 0; k < 3; ++k) {
-                    const float* q = g.verts + 3 * ((size_t)g.lodVertOffset[lod] + ix[k]);
-                    v[k] = xform_point(M, V3{q[0], q[1], q[2]});
+                auto world_vert = [&](u32 i) {
+                    const float* q = g.verts + 3 * ((size_t)g.lodVertOffset[lod] + i);
+                    return xform_point(M, V3{q[0], q[1], q[2]});
+                };
+                float projDistToRim;
+                if (hit.kind == 2) {
+                    // computeDistToRim (wireframe_common.hpp:79-89), scaled by 0.05 (:164-169)
+                    const float* sp = sphere_of(g, lod, hit.prim);
+                    const V3 center = xform_point(M, V3{sp[0], sp[1], sp[2]});
+                    projDistToRim = 0.05f * (sp[3] * dot(normalize(center - p), dir));
+                } else {
+                    V3 closest;
+                    if (hit.kind == 1) {
+                        // computeClosestLinePoint, quads (wireframe_common.hpp:42-77):
+                        // A = v0, B = v1, C = v3, D = v2; edges AB, AC, BD, CD
+                        const u32* q = quad_of(g, lod, hit.prim);
+                        const V3 A = world_vert(q[0]), B = world_vert(q[1]), Cc = world_vert(q[3]),
+                                 D = world_vert(q[2]);
+                        const V3 AB = B - A, AC = Cc - A, BD = D - B, CD = D - Cc;
+                        const V3 AP = p - A, BP = p - B, CP = p - Cc;
+                        const V3 onAB = A + AB * (dot(AP, AB) / dot(AB, AB));
+                        const V3 onAC = A + AC * (dot(AP, AC) / dot(AC, AC));
+                        const V3 onBD = B + BD * (dot(BP, BD) / dot(BD, BD));
+                        const V3 onCD = Cc + CD * (dot(CP, CD) / dot(CD, CD));
+                        const float dAB = dot(onAB - p, onAB - p), dAC = dot(onAC - p, onAC - p),
+                                    dBD = dot(onBD - p, onBD - p), dCD = dot(onCD - p, onCD - p);
+                        if (dAB <= dAC && dAB <= dBD && dAB <= dCD)
+                            closest = onAB;
+                        else if (dAC <= dAB && dAC <= dBD && dAC <= dCD)
+                            closest = onAC;
+                        else if (dBD <= dAB && dBD <= dAC && dBD <= dCD)
+                            closest = onBD;
+                        else
+                            closest = onCD;
+                    } else {
+                        const u32* ix = g.tris + 3 * ((size_t)g.lodTriOffset[lod] + hit.prim);
+                        V3 v[3];
+                        for (int k = 0; k < 3; ++k) v[k] = world_vert(ix[k]);
+                        // computeClosestLinePoint (wireframe_common.hpp:16-40)
+                        const V3 AB = v[1] - v[0], AC = v[2] - v[0], BC = v[2] - v[1];
+                        const V3 AP = p - v[0], BP = p - v[1];
+                        const V3 onAB = v[0] + AB * (dot(AP, AB) / dot(AB, AB));
+                        const V3 onAC = v[0] + AC * (dot(AP, AC) / dot(AC, AC));
+                        const V3 onBC = v[1] + BC * (dot(BP, BC) / dot(BC, BC));
+                        const float dAB = dot(onAB - p, onAB - p), dAC = dot(onAC - p, onAC - p),
+                                    dBC = dot(onBC - p, onBC - p);
+                        closest = (dAB <= dAC && dAB <= dBC) ? onAB : ((dAC <= dAB && dAC <= dBC) ? onAC : onBC);
+                    }
+                    const V3 lineSegment = closest - camPos;
+                    const V3 projected = lineSegment - dir * dot(lineSegment, dir);
+                    projDistToRim = len(projected);
                 }
-                // computeClosestLinePoint (wireframe_common.hpp:16-40)
-                const V3 AB = v[1] - v[0], AC = v[2] - v[0], BC = v[2] - v[1];
-                const V3 AP = p - v[0], BP = p - v[1];
-                const V3 onAB = v[0] + AB * (dot(AP, AB) / dot(AB, AB));
-                const V3 onAC = v[0] + AC * (dot(AP, AC) / dot(AC, AC));
-                const V3 onBC = v[1] + BC * (dot(BP, BC) / dot(BC, BC));
-                const float dAB = dot(onAB - p, onAB - p), dAC = dot(onAC - p, onAC - p),
-                            dBC = dot(onBC - p, onBC - p);
-                const V3 closest = (dAB <= dAC && dAB <= dBC) ? onAB : ((dAC <= dAB && dAC <= dBC) ? onAC : onBC);
-                const V3 lineSegment = closest - camPos;
-                const V3 projected = lineSegment - dir * dot(lineSegment, dir);
-                if (len(projected) > lineWidth * pixelSize) {
+                if (projDistToRim > lineWidth * pixelSize) {
                     origin = rayOrigin + dir * hitDist;
                 } else {
                     value[0] = value[1] = value[2] = 1.0f;
