@@ -296,7 +296,10 @@ void build_light_tree(const std::vector<HostAreaLight>& area, const std::vector<
                       const float sceneBox[6], const EmissionLookup& emission, LightTreeHost& out) {
     out = LightTreeHost();
     std::vector<PosLight> lights;
-    for (const auto& t : area) lights.push_back(PosLight{true, t, {}});
+    for (const auto& t : area) {
+        lights.push_back(PosLight{true, t, {}});
+        if (t.kind == kAreaSphere) out.sphereLights = true;
+    }
     for (const auto& p : points) lights.push_back(PosLight{false, {}, p});
     const size_t n = lights.size();
     out.posLightCount = (u32)n;

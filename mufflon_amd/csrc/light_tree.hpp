@@ -58,6 +58,7 @@ struct LightTreeHost {
     // directional subtree and background (absent in the supported scenes)
     float dirRootFlux = 0.0f;
     float backgroundFlux = 0.0f;  // sum of the background flux channels
+    bool sphereLights = false;  // the connect pdf of a hit then depends on the previous vertex
     // primitive (instance << 32 | primitive) -> left-aligned tree path, sorted by key
     std::vector<uint64_t> mapKeys;
     std::vector<uint32_t> mapCodes;

@@ -438,7 +438,9 @@ WfState wavefront_state(Context& c, PathTracer& t) {
         W.rayD[b] = t.wfRay[b][1].as<float4>();
         W.rayM[b] = t.wfRay[b][2].as<float4>();
         W.hit[b] = t.wfRay[b][3].as<float4>();
-        if (t.params.neePositionGuide) {
+        // the previous vertex of each path: the light-tree guide and sphere
+        // lights' connect pdfs depend on it
+        if (t.params.neePositionGuide || t.tree.sphereLights) {
             t.wfRayP[b].reserve(16 * (size_t)N);
             W.rayP[b] = t.wfRayP[b].as<float4>();
         }

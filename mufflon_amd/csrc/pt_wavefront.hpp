@@ -22,7 +22,7 @@ struct WfState {
     float4* rayD[2];    // direction xyz, path index (int bits)
     float4* rayM[2];    // medium absorption xyz, hit barycentric w1
     float4* rayP[2];    // position of the vertex the ray leaves (camera or surface); only
-                        // with the NEE position guide, null otherwise
+                        // with the NEE position guide or sphere lights, null otherwise
     float4* hit[2];     // t, instance, primitive (int bits), w0
     float4* shO;        // per shadow entry (slot * neeCount + i): origin xyz, tmax
     float4* shD;        // direction xyz
