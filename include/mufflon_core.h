@@ -16,10 +16,9 @@
  * The device is created on the first scene load, so the world-building calls
  * run on a host without a GPU.
  *
- * Not supported by the HIP path (the call fails with a message): quads and
- * spheres, alpha and displacement maps, spot and directional lights, focus
- * cameras, the Hosek sky, and the ORENNAYAR, WALTER, BLEND and FRESNEL
- * materials.
+ * Not supported by the HIP path (the call fails with a message): alpha and
+ * displacement maps, spot and directional lights, focus cameras, the Hosek
+ * sky, and the ORENNAYAR, BLEND and FRESNEL materials.
  */
 #ifndef MUFFLON_CORE_H
 #define MUFFLON_CORE_H
@@ -101,6 +100,7 @@ typedef struct {
 typedef int32_t IndexType;
 typedef IndexType VertexHdl;
 typedef IndexType FaceHdl;
+typedef IndexType SphereHdl;
 typedef uint32_t LodLevel;
 typedef uint16_t MatIdx;
 typedef void* ObjectHdl;
@@ -204,6 +204,19 @@ size_t polygon_get_face_count(LodHdl lvlDtl);
 size_t polygon_get_triangle_count(LodHdl lvlDtl);
 size_t polygon_get_quad_count(LodHdl lvlDtl);
 Boolean polygon_get_bounding_box(LodHdl lvlDtl, Vec3* min, Vec3* max);
+
+/* ---- Spheres (interface.cpp:754-940): a sphere stream holds center xyz and
+ * radius per sphere (16 bytes, ei::Sphere). Sphere ids follow the LoD's
+ * triangles and quads in its primitive order. */
+Boolean spheres_reserve(LodHdl lvlDtl, size_t count);
+SphereHdl spheres_add_sphere(LodHdl lvlDtl, Vec3 point, float radius);
+SphereHdl spheres_add_sphere_material(LodHdl lvlDtl, Vec3 point, float radius, MatIdx idx);
+SphereHdl spheres_add_sphere_bulk(LodHdl lvlDtl, size_t count, const BulkLoader* stream, const AABB* aabb,
+                                  size_t* readSpheres);
+Boolean spheres_set_material_idx(LodHdl lvlDtl, SphereHdl sphere, MatIdx idx);
+size_t spheres_set_material_idx_bulk(LodHdl lvlDtl, SphereHdl startSphere, size_t count, const BulkLoader* stream);
+size_t spheres_get_sphere_count(LodHdl lvlDtl);
+Boolean spheres_get_bounding_box(LodHdl lvlDtl, Vec3* min, Vec3* max);
 
 /* ---- Objects and instances (interface.cpp:880-1060) */
 ObjectHdl world_create_object(MufflonInstanceHdl instHdl, const char* name, ObjectFlags flags);

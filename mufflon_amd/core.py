@@ -152,6 +152,18 @@ _SIGS = {
     "polygon_add_triangle": (i32, [vp, UVec3]),
     "polygon_add_triangle_material": (i32, [vp, UVec3, C.c_uint16]),
     "polygon_add_quad": (i32, [vp, UVec4]),
+    "polygon_add_quad_material": (i32, [vp, UVec4, C.c_uint16]),
+    "polygon_get_quad_count": (C.c_size_t, [vp]),
+    "polygon_get_face_count": (C.c_size_t, [vp]),
+    "spheres_reserve": (Boolean, [vp, C.c_size_t]),
+    "spheres_add_sphere": (i32, [vp, Vec3, C.c_float]),
+    "spheres_add_sphere_material": (i32, [vp, Vec3, C.c_float, C.c_uint16]),
+    "spheres_add_sphere_bulk": (i32, [vp, C.c_size_t, C.POINTER(BulkLoader), C.POINTER(AABB),
+                                      C.POINTER(C.c_size_t)]),
+    "spheres_set_material_idx": (Boolean, [vp, i32, C.c_uint16]),
+    "spheres_set_material_idx_bulk": (C.c_size_t, [vp, i32, C.c_size_t, C.POINTER(BulkLoader)]),
+    "spheres_get_sphere_count": (C.c_size_t, [vp]),
+    "spheres_get_bounding_box": (Boolean, [vp, C.POINTER(Vec3), C.POINTER(Vec3)]),
     "polygon_add_vertex_bulk": (i32, [vp, C.c_size_t, C.POINTER(BulkLoader), C.POINTER(BulkLoader),
                                       C.POINTER(BulkLoader), C.POINTER(AABB), C.POINTER(C.c_size_t),
                                       C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
@@ -687,7 +699,8 @@ def _material_params(core, m, tex_handles):
 
 def build_world(core, scene, texture_dir, background=None, name="scene", vfov=None):
     """Create `scene` (a mufflon_amd.scenes dict with one LoD: verts, tris,
-    normals, uvs, mat_idx, materials, camera, optional textures /
+    optional quads / spheres, normals, uvs, mat_idx (per primitive),
+    materials, camera, optional textures /
     point_lights / envmap) through the core API, the way a scene loader does,
     with textures written as PFM files into texture_dir. background: None,
     ("colored", color, scale) or ("envmap", scale) with the scene's envmap
@@ -721,9 +734,22 @@ def build_world(core, scene, texture_dir, background=None, name="scene", vfov=No
         if not h:
             raise RuntimeError(core.core_get_dll_error())
         core.scenario_assign_material(scen, slots[i], h)
-    for t, mi in zip(np.asarray(scene["tris"], np.uint32), np.asarray(scene["mat_idx"])):
+    tris = np.asarray(scene["tris"], np.uint32).reshape(-1, 3)
+    quads = np.asarray(scene.get("quads", np.zeros((0, 4))), np.uint32).reshape(-1, 4)
+    spheres = np.ascontiguousarray(scene.get("spheres", np.zeros((0, 4))), np.float32).reshape(-1, 4)
+    mat_idx = np.asarray(scene["mat_idx"])
+    for t, mi in zip(tris, mat_idx):
         if lib.polygon_add_triangle_material(lod, UVec3(int(t[0]), int(t[1]), int(t[2])), int(mi)) < 0:
             raise RuntimeError(core.core_get_dll_error())
+    for q, mi in zip(quads, mat_idx[len(tris):]):
+        if lib.polygon_add_quad_material(lod, UVec4(*(int(x) for x in q)), int(mi)) < 0:
+            raise RuntimeError(core.core_get_dll_error())
+    if len(spheres):
+        read = C.c_size_t()
+        if lib.spheres_add_sphere_bulk(lod, len(spheres), C.byref(bulk_array(spheres)), None, C.byref(read)) < 0:
+            raise RuntimeError(core.core_get_dll_error())
+        sm = np.ascontiguousarray(mat_idx[len(tris) + len(quads):], np.uint16)
+        lib.spheres_set_material_idx_bulk(lod, 0, len(sm), C.byref(bulk_array(sm)))
     core.world_create_instance(obj)
     cam = scene["camera"]
     if vfov is None:

@@ -97,15 +97,39 @@ struct Lod {
     Instance* owner = nullptr;
     std::vector<float> points, normals, uvs;  // 3, 3, 2 floats per vertex
     std::vector<uint32_t> tris;               // 3 per triangle
-    std::vector<uint16_t> mat;                // per triangle (scenario material slot)
+    std::vector<uint32_t> quads;              // 4 per quad (added after every triangle)
+    std::vector<uint16_t> mat;                // per face: triangles, then quads (scenario material slot)
+    std::vector<float> spheres;               // center xyz, radius
+    std::vector<uint16_t> sphereMat;          // per sphere
+    // the spheres' box (Spheres::m_boundingBox; the polygons' is box)
+    float sphereBox[6] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
+                          std::numeric_limits<float>::max(), -std::numeric_limits<float>::max(),
+                          -std::numeric_limits<float>::max(), -std::numeric_limits<float>::max()};
     float box[6] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
                     std::numeric_limits<float>::max(), -std::numeric_limits<float>::max(),
                     -std::numeric_limits<float>::max(), -std::numeric_limits<float>::max()};
     size_t vertices() const { return points.size() / 3; }
+    size_t triangles() const { return tris.size() / 3; }
+    size_t quad_count() const { return quads.size() / 4; }
+    size_t sphere_count() const { return spheres.size() / 4; }
     void grow_box(const float* p) {
         for (int k = 0; k < 3; ++k) {
             box[k] = p[k] < box[k] ? p[k] : box[k];
             box[3 + k] = box[3 + k] < p[k] ? p[k] : box[3 + k];
+        }
+    }
+    void grow_sphere_box(const float* s) {
+        for (int k = 0; k < 3; ++k) {
+            const float lo = s[k] - s[3], hi = s[k] + s[3];
+            sphereBox[k] = lo < sphereBox[k] ? lo : sphereBox[k];
+            sphereBox[3 + k] = sphereBox[3 + k] < hi ? hi : sphereBox[3 + k];
+        }
+    }
+    // Lod::get_bounding_box: the polygons' box joined with the spheres'
+    void lod_box(float out[6]) const {
+        for (int k = 0; k < 3; ++k) {
+            out[k] = sphereBox[k] < box[k] ? sphereBox[k] : box[k];
+            out[3 + k] = box[3 + k] < sphereBox[3 + k] ? sphereBox[3 + k] : box[3 + k];
         }
     }
 };
@@ -615,17 +639,28 @@ void load_scene(Core& c, Scenario& s) {
         auto it = lodIds.find(key);
         if (it == lodIds.end()) {
             const Lod& L = *obj.lods[level];
-            CORE_REQUIRE(!L.tris.empty(), "LoD " + std::to_string(level) + " of object '" + obj.name +
-                                              "' has no triangles");
-            for (uint16_t mi : L.mat)
+            CORE_REQUIRE(!L.mat.empty() || !L.sphereMat.empty(), "LoD " + std::to_string(level) + " of object '" +
+                                                                     obj.name + "' has no primitives");
+            // material per primitive: triangles, quads, spheres (descriptors.hpp:43, 64)
+            std::vector<uint16_t> primMat(L.mat);
+            primMat.insert(primMat.end(), L.sphereMat.begin(), L.sphereMat.end());
+            for (uint16_t mi : primMat)
                 CORE_REQUIRE(mi < mats.size(), "object '" + obj.name + "' uses material slot " + std::to_string(mi) +
                                                    " which the scenario does not declare");
             int32_t id = -1;
-            check_mfh(mfh_lod_create(c.ctx, L.points.data(), (int32_t)L.vertices(), L.tris.data(),
-                                     (int32_t)(L.tris.size() / 3), &id),
-                      "LoD upload");
-            check_mfh(mfh_lod_set_shading(c.ctx, id, L.normals.data(), L.uvs.data(), L.mat.data()), "LoD shading");
-            check_mfh(mfh_lbvh_build(c.ctx, id, L.box), "LBVH build");
+            if (L.quads.empty() && L.spheres.empty())
+                check_mfh(mfh_lod_create(c.ctx, L.points.data(), (int32_t)L.vertices(), L.tris.data(),
+                                         (int32_t)L.triangles(), &id),
+                          "LoD upload");
+            else
+                check_mfh(mfh_lod_create_mixed(c.ctx, L.points.data(), (int32_t)L.vertices(), L.tris.data(),
+                                               (int32_t)L.triangles(), L.quads.data(), (int32_t)L.quad_count(),
+                                               L.spheres.data(), (int32_t)L.sphere_count(), &id),
+                          "LoD upload");
+            check_mfh(mfh_lod_set_shading(c.ctx, id, L.normals.data(), L.uvs.data(), primMat.data()), "LoD shading");
+            float box[6];
+            L.lod_box(box);
+            check_mfh(mfh_lbvh_build(c.ctx, id, box), "LBVH build");
             it = lodIds.emplace(key, id).first;
         }
         instLod.push_back(it->second);
@@ -781,10 +816,33 @@ FaceHdl add_triangle(void* h, UVec3 v, MatIdx idx) {
     Lod& L = lod_of(h);
     const uint32_t n = (uint32_t)L.vertices();
     CORE_REQUIRE(v.x < n && v.y < n && v.z < n, "triangle vertex index out of range");
+    // faces keep triangles before quads (polygon.cpp:200, "to keep the order implicitly")
+    CORE_REQUIRE(L.quads.empty(), "triangles must be added before the LoD's first quad");
     L.tris.insert(L.tris.end(), {v.x, v.y, v.z});
     L.mat.push_back(idx);
     touch_lod(h);
     return (FaceHdl)(L.mat.size() - 1);
+}
+
+FaceHdl add_quad(void* h, UVec4 v, MatIdx idx) {
+    Lod& L = lod_of(h);
+    const uint32_t n = (uint32_t)L.vertices();
+    CORE_REQUIRE(v.x < n && v.y < n && v.z < n && v.w < n, "quad vertex index out of range");
+    L.quads.insert(L.quads.end(), {v.x, v.y, v.z, v.w});
+    L.mat.push_back(idx);
+    touch_lod(h);
+    return (FaceHdl)(L.mat.size() - 1);
+}
+
+SphereHdl add_sphere(void* h, Vec3 p, float radius, MatIdx idx) {
+    Lod& L = lod_of(h);
+    CORE_REQUIRE(radius >= 0.0f, "negative sphere radius");
+    const float s[4] = {p.x, p.y, p.z, radius};
+    L.spheres.insert(L.spheres.end(), s, s + 4);
+    L.sphereMat.push_back(idx);
+    L.grow_sphere_box(s);
+    touch_lod(h);
+    return (SphereHdl)(L.sphere_count() - 1);
 }
 
 // Bytes of a bulk stream: an array, or the next `bytes` of a FILE
@@ -1033,12 +1091,12 @@ Boolean world_get_frame_current(MufflonInstanceHdl instHdl, uint32_t* animationF
 Boolean polygon_reserve(LodHdl lvlDtl, size_t vertices, size_t tris, size_t quads) {
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
         Lod& L = lod_of(lvlDtl);
-        (void)quads;
         L.points.reserve(3 * vertices);
         L.normals.reserve(3 * vertices);
         L.uvs.reserve(2 * vertices);
         L.tris.reserve(3 * tris);
-        L.mat.reserve(tris);
+        L.quads.reserve(4 * quads);
+        L.mat.reserve(tris + quads);
         return true;
     });
 }
@@ -1065,16 +1123,11 @@ FaceHdl polygon_add_triangle_material(LodHdl lvlDtl, UVec3 vertices, MatIdx idx)
 }
 
 FaceHdl polygon_add_quad(LodHdl lvlDtl, UVec4 vertices) {
-    (void)vertices;
-    return guard(__func__, (FaceHdl)kInvalidIndex, [&]() -> FaceHdl {
-        lod_of(lvlDtl);
-        throw ApiError("quads are not supported by the HIP path tracer (triangulate them)");
-    });
+    return guard(__func__, (FaceHdl)kInvalidIndex, [&] { return add_quad(lvlDtl, vertices, 0); });
 }
 
 FaceHdl polygon_add_quad_material(LodHdl lvlDtl, UVec4 vertices, MatIdx idx) {
-    (void)idx;
-    return polygon_add_quad(lvlDtl, vertices);
+    return guard(__func__, (FaceHdl)kInvalidIndex, [&] { return add_quad(lvlDtl, vertices, idx); });
 }
 
 VertexHdl polygon_add_vertex_bulk(LodHdl lvlDtl, size_t count, const BulkLoader* points, const BulkLoader* normals,
@@ -1167,13 +1220,10 @@ size_t polygon_get_face_count(LodHdl lvlDtl) {
     return guard(__func__, (size_t)0, [&] { return lod_of(lvlDtl).mat.size(); });
 }
 size_t polygon_get_triangle_count(LodHdl lvlDtl) {
-    return guard(__func__, (size_t)0, [&] { return lod_of(lvlDtl).mat.size(); });
+    return guard(__func__, (size_t)0, [&] { return lod_of(lvlDtl).triangles(); });
 }
 size_t polygon_get_quad_count(LodHdl lvlDtl) {
-    return guard(__func__, (size_t)0, [&] {
-        lod_of(lvlDtl);
-        return (size_t)0;
-    });
+    return guard(__func__, (size_t)0, [&] { return lod_of(lvlDtl).quad_count(); });
 }
 
 Boolean polygon_get_bounding_box(LodHdl lvlDtl, Vec3* min, Vec3* max) {
@@ -1181,6 +1231,88 @@ Boolean polygon_get_bounding_box(LodHdl lvlDtl, Vec3* min, Vec3* max) {
         const Lod& L = lod_of(lvlDtl);
         if (min) *min = {L.box[0], L.box[1], L.box[2]};
         if (max) *max = {L.box[3], L.box[4], L.box[5]};
+        return true;
+    });
+}
+
+// ================================================================== spheres
+Boolean spheres_reserve(LodHdl lvlDtl, size_t count) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Lod& L = lod_of(lvlDtl);
+        L.spheres.reserve(4 * count);
+        L.sphereMat.reserve(count);
+        return true;
+    });
+}
+
+SphereHdl spheres_add_sphere(LodHdl lvlDtl, Vec3 point, float radius) {
+    return guard(__func__, (SphereHdl)kInvalidIndex, [&] { return add_sphere(lvlDtl, point, radius, 0); });
+}
+
+SphereHdl spheres_add_sphere_material(LodHdl lvlDtl, Vec3 point, float radius, MatIdx idx) {
+    return guard(__func__, (SphereHdl)kInvalidIndex, [&] { return add_sphere(lvlDtl, point, radius, idx); });
+}
+
+SphereHdl spheres_add_sphere_bulk(LodHdl lvlDtl, size_t count, const BulkLoader* stream, const AABB* aabb,
+                                  size_t* readSpheres) {
+    return guard(__func__, (SphereHdl)kInvalidIndex, [&]() -> SphereHdl {
+        Lod& L = lod_of(lvlDtl);
+        CORE_REQUIRE(stream != nullptr, "Invalid sphere stream descriptor (nullptr)");
+        const size_t start = L.sphere_count();
+        std::vector<uint8_t> b;
+        const size_t n = read_bulk(stream, 16, count, b);
+        // Spheres::add_bulk (sphere.cpp:87-115): the spheres read; the box
+        // grows by the given AABB, else by every sphere read
+        L.spheres.resize(4 * (start + n));
+        L.sphereMat.resize(start + n, 0);
+        if (n) std::memcpy(L.spheres.data() + 4 * start, b.data(), 16 * n);
+        if (aabb) {
+            const float lo[4] = {aabb->min.x, aabb->min.y, aabb->min.z, 0.0f};
+            const float hi[4] = {aabb->max.x, aabb->max.y, aabb->max.z, 0.0f};
+            L.grow_sphere_box(lo);
+            L.grow_sphere_box(hi);
+        } else {
+            for (size_t i = start; i < start + n; ++i) L.grow_sphere_box(L.spheres.data() + 4 * i);
+        }
+        if (readSpheres) *readSpheres = n;
+        touch_lod(lvlDtl);
+        return (SphereHdl)start;
+    });
+}
+
+Boolean spheres_set_material_idx(LodHdl lvlDtl, SphereHdl sphere, MatIdx idx) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        Lod& L = lod_of(lvlDtl);
+        CORE_REQUIRE(sphere >= 0 && (size_t)sphere < L.sphere_count(), "sphere index out of range");
+        L.sphereMat[(size_t)sphere] = idx;
+        touch_lod(lvlDtl);
+        return true;
+    });
+}
+
+size_t spheres_set_material_idx_bulk(LodHdl lvlDtl, SphereHdl startSphere, size_t count, const BulkLoader* stream) {
+    return guard(__func__, (size_t)0, [&]() -> size_t {
+        Lod& L = lod_of(lvlDtl);
+        CORE_REQUIRE(stream != nullptr, "Invalid material index stream (nullptr)");
+        CORE_REQUIRE(startSphere >= 0 && (size_t)startSphere + count <= L.sphere_count(),
+                     "sphere range out of bounds");
+        std::vector<uint8_t> b;
+        const size_t n = read_bulk(stream, 2, count, b);
+        std::memcpy(L.sphereMat.data() + startSphere, b.data(), 2 * n);
+        touch_lod(lvlDtl);
+        return n;
+    });
+}
+
+size_t spheres_get_sphere_count(LodHdl lvlDtl) {
+    return guard(__func__, (size_t)0, [&] { return lod_of(lvlDtl).sphere_count(); });
+}
+
+Boolean spheres_get_bounding_box(LodHdl lvlDtl, Vec3* min, Vec3* max) {
+    return guard(__func__, (Boolean) false, [&]() -> Boolean {
+        const Lod& L = lod_of(lvlDtl);
+        if (min) *min = {L.sphereBox[0], L.sphereBox[1], L.sphereBox[2]};
+        if (max) *max = {L.sphereBox[3], L.sphereBox[4], L.sphereBox[5]};
         return true;
     });
 }

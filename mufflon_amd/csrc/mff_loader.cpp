@@ -226,19 +226,13 @@ uint32_t read_lod(MufflonInstanceHdl core, const FileState& fs_, uint32_t objInd
     const uint32_t numVerts = f.read<uint32_t>();
     (void)f.read<uint32_t>();  // edges
     const uint32_t vertAttribs = f.read<uint32_t>(), faceAttribs = f.read<uint32_t>();
-    (void)f.read<uint32_t>();  // sphere attributes
-    if (numQuads > 0)
-        throw LoaderError("Object '" + obj.name + "' LoD " + std::to_string(level) + " has " +
-                          std::to_string(numQuads) + " quads: the HIP path tracer renders triangles only");
-    if (numSpheres > 0)
-        throw LoaderError("Object '" + obj.name + "' LoD " + std::to_string(level) + " has " +
-                          std::to_string(numSpheres) + " spheres: the HIP path tracer renders triangles only");
+    const uint32_t sphereAttribs = f.read<uint32_t>();
 
     const bool deflate = (fs_.flags & kFlagDeflate) != 0;
     const bool packedNormals = (fs_.flags & kFlagCompressedNormals) != 0;
-    const size_t V = numVerts, F = numTris;
+    const size_t V = numVerts, T = numTris, Q = numQuads, S = numSpheres, F = T + Q;
     const size_t vertexBytes = V * (12 + (packedNormals ? 4 : 12) + 8);
-    std::vector<uint8_t> vertexData, triData, matData;
+    std::vector<uint8_t> vertexData, triData, quadData, matData, sphereData, sphereMatData;
     auto skip_attributes = [&](uint32_t count) {
         f.expect(kAttrTag, "attribute");
         if (deflate) {
@@ -265,16 +259,31 @@ uint32_t read_lod(MufflonInstanceHdl core, const FileState& fs_, uint32_t objInd
         }
     };
     // order of binary.cpp:711-733: vertices, vertex attributes, triangles,
-    // quads, material indices, face attributes, spheres
+    // quads, material indices (triangles, then quads), face attributes,
+    // spheres (center and radius, then material indices: one stream when
+    // deflated, binary.cpp:539-567), sphere attributes
     if (V > 0) block(vertexBytes, vertexData, "vertex");
     if (V > 0 && vertAttribs > 0) skip_attributes(vertAttribs);
-    if (F > 0) block(F * 12, triData, "triangle");
+    if (T > 0) block(T * 12, triData, "triangle");
+    if (Q > 0) block(Q * 16, quadData, "quad");
     if (F > 0) block(F * 2, matData, "material index");
     if (F > 0 && faceAttribs > 0) skip_attributes(faceAttribs);
+    if (S > 0) {
+        if (deflate) {
+            block(S * 18, sphereData, "sphere");
+            sphereMatData.assign(sphereData.begin() + 16 * S, sphereData.end());
+            sphereData.resize(16 * S);
+        } else {
+            block(S * 16, sphereData, "sphere");
+            block(S * 2, sphereMatData, "sphere material index");
+        }
+        if (sphereAttribs > 0) skip_attributes(sphereAttribs);
+    }
 
     const LodHdl lodHdl = object_add_lod(obj.hdl, level, asReduced);
     if (!lodHdl) throw LoaderError(std::string("Failed to add LoD: ") + core_get_dll_error());
-    if (!polygon_reserve(lodHdl, V, F, 0)) throw LoaderError("Failed to reserve LoD polygon memory");
+    if (!polygon_reserve(lodHdl, V, T, Q)) throw LoaderError("Failed to reserve LoD polygon memory");
+    if (!spheres_reserve(lodHdl, S)) throw LoaderError("Failed to reserve LoD sphere memory");
     if (V > 0) {
         const char* base = (const char*)vertexData.data();
         BulkLoader points{BULK_ARRAY, {}}, normals{BULK_ARRAY, {}}, uvs{BULK_ARRAY, {}};
@@ -295,15 +304,34 @@ uint32_t read_lod(MufflonInstanceHdl core, const FileState& fs_, uint32_t objInd
                     throw LoaderError("Failed to set normal for object '" + obj.name + "'");
         }
     }
-    if (F > 0) {
+    if (T > 0) {
         const UVec3* tris = (const UVec3*)triData.data();
-        for (size_t t = 0; t < F; ++t)
+        for (size_t t = 0; t < T; ++t)
             if (polygon_add_triangle(lodHdl, tris[t]) < 0)
                 throw LoaderError("Failed to add triangle to object '" + obj.name + "': " + core_get_dll_error());
+    }
+    if (Q > 0) {
+        const UVec4* quads = (const UVec4*)quadData.data();
+        for (size_t q = 0; q < Q; ++q)
+            if (polygon_add_quad(lodHdl, quads[q]) < 0)
+                throw LoaderError("Failed to add quad to object '" + obj.name + "': " + core_get_dll_error());
+    }
+    if (F > 0) {
         BulkLoader mats{BULK_ARRAY, {}};
         mats.descriptor.bytes = (const char*)matData.data();
         if (polygon_set_material_idx_bulk(lodHdl, 0, F, &mats) != F)
             throw LoaderError("Failed to set face material for object '" + obj.name + "'");
+    }
+    if (S > 0) {
+        BulkLoader spheres{BULK_ARRAY, {}}, mats{BULK_ARRAY, {}};
+        spheres.descriptor.bytes = (const char*)sphereData.data();
+        mats.descriptor.bytes = (const char*)sphereMatData.data();
+        AABB box{{obj.box[0], obj.box[1], obj.box[2]}, {obj.box[3], obj.box[4], obj.box[5]}};
+        size_t read = 0;
+        if (spheres_add_sphere_bulk(lodHdl, S, &spheres, &box, &read) < 0 || read != S)
+            throw LoaderError("Failed to add spheres to object '" + obj.name + "'");
+        if (spheres_set_material_idx_bulk(lodHdl, 0, S, &mats) != S)
+            throw LoaderError("Failed to set sphere materials for object '" + obj.name + "'");
     }
     return level;
 }

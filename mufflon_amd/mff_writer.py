@@ -7,8 +7,8 @@ The binary layout follows the format description section by section: the
 materials header, the objects section (flags, jump table, per object the
 name, flags, keyframes, box, LoD jump table and - from version 1.7 - the
 sorted unique material indices), LoDs (counts, vertex data, triangles,
-material indices; DEFLATE streams when compressed) and the instances with
-world-to-instance matrices.
+quads, face material indices, spheres with their material indices; DEFLATE
+streams when compressed) and the instances with world-to-instance matrices.
 """
 import json
 import os
@@ -81,27 +81,42 @@ def _block(data, deflate):
 
 
 def _lod_bytes(lod, deflate, packed_normals):
-    v = np.ascontiguousarray(lod["verts"], "<f4")
-    t = np.ascontiguousarray(lod["tris"], "<u4")
+    v = np.ascontiguousarray(lod["verts"], "<f4").reshape(-1, 3)
+    t = np.ascontiguousarray(lod.get("tris", np.zeros((0, 3))), "<u4").reshape(-1, 3)
+    q = np.ascontiguousarray(lod.get("quads", np.zeros((0, 4))), "<u4").reshape(-1, 4)
+    sp = np.ascontiguousarray(lod.get("spheres", np.zeros((0, 4))), "<f4").reshape(-1, 4)
     n = lod.get("normals")
     n = np.zeros_like(v) if n is None else np.ascontiguousarray(n, "<f4")
     uv = lod.get("uvs")
     uv = np.zeros((len(v), 2), "<f4") if uv is None else np.ascontiguousarray(uv, "<f4")
-    mats = np.ascontiguousarray(lod.get("mat_idx", np.zeros(len(t))), "<u2")
+    mats = np.ascontiguousarray(lod.get("mat_idx", np.zeros(len(t) + len(q))), "<u2")
+    sphere_mats = np.ascontiguousarray(lod.get("sphere_mat_idx", np.zeros(len(sp))), "<u2")
+    assert len(mats) == len(t) + len(q) and len(sphere_mats) == len(sp)
     out = bytearray(b"LOD_")
-    out += struct.pack("<8I", len(t), 0, 0, len(v), 0, 0, 0, 0)
+    out += struct.pack("<8I", len(t), len(q), len(sp), len(v), 0, 0, 0, 0)
     if len(v):
         normal_bytes = pack_normals(n).astype("<u4").tobytes() if packed_normals else n.tobytes()
         out += _block(v.tobytes() + normal_bytes + uv.tobytes(), deflate)
     if len(t):
         out += _block(t.tobytes(), deflate)
+    if len(q):
+        out += _block(q.tobytes(), deflate)
+    if len(t) + len(q):
         out += _block(mats.tobytes(), deflate)
+    if len(sp):
+        # centre and radius, then the material indices: one stream when deflated
+        if deflate:
+            out += _block(sp.tobytes() + sphere_mats.tobytes(), True)
+        else:
+            out += sp.tobytes() + sphere_mats.tobytes()
     return bytes(out)
 
 
 def write_mff(path, material_names, objects, instances, deflate=False, packed_normals=False, version=(1, 7)):
     """Write the binary file. objects: dicts with name, lods (list of dicts
-    with verts, tris, normals, uvs, mat_idx), optional box (6 floats; default
+    with verts, tris, normals, uvs, mat_idx - one per triangle, then one per
+    quad - and optional quads (n x 4 vertex indices), spheres (n x 4: centre,
+    radius) and sphere_mat_idx), optional box (6 floats; default
     the union of the LoDs' vertex bounds), flags, keyframe. instances: dicts
     with name, obj (index), world_to_inst (3x4), optional keyframe."""
     flags = (FLAG_DEFLATE if deflate else 0) | (FLAG_COMPRESSED_NORMALS if packed_normals else 0)
@@ -115,11 +130,17 @@ def write_mff(path, material_names, objects, instances, deflate=False, packed_no
         lods = [_lod_bytes(L, deflate, packed_normals) for L in o["lods"]]
         box = o.get("box")
         if box is None:
-            allv = np.concatenate([np.asarray(L["verts"], np.float32) for L in o["lods"]])
-            box = np.concatenate([allv.min(0), allv.max(0)])
+            lo, hi = [], []
+            for L in o["lods"]:
+                v = np.asarray(L["verts"], np.float32).reshape(-1, 3)
+                sp = np.asarray(L.get("spheres", np.zeros((0, 4))), np.float32).reshape(-1, 4)
+                lo += [v, sp[:, :3] - sp[:, 3:]]
+                hi += [v, sp[:, :3] + sp[:, 3:]]
+            box = np.concatenate([np.concatenate(lo).min(0), np.concatenate(hi).max(0)])
         rec = bytearray(b"Obj_" + _str(o["name"]) + _u32(o.get("flags", 0)) + _u32(o.get("keyframe", NO_FRAME)) +
                         _u32(o.get("prev", NO_FRAME)) + np.asarray(box, "<f4").tobytes())
-        uniq = sorted({int(m) for L in o["lods"] for m in np.asarray(L.get("mat_idx", []), np.int64)})
+        uniq = sorted({int(m) for L in o["lods"] for key in ("mat_idx", "sphere_mat_idx")
+                       for m in np.asarray(L.get(key, []), np.int64)})
         tail = (_u32(len(uniq)) + b"".join(struct.pack("<H", m) for m in uniq)) if version >= (1, 7) else b""
         obj_blobs.append((rec, lods, tail))
     header_len = 4 + 8 + 4 + 4 + 8 * len(objects)
@@ -197,7 +218,7 @@ def _material_json(m, texture_file):
 def export_scene(scene, directory, name="scene", fov_degrees=None, background=None, deflate=False,
                  packed_normals=False, near=1e-4, far=1e10):
     """Write a one-LoD scene dict of mufflon_amd.scenes (verts, tris,
-    normals, uvs, mat_idx, materials, camera, optional textures,
+    optional quads / spheres, normals, uvs, mat_idx per primitive, materials, camera, optional textures,
     point_lights, envmap) as <name>.json + <name>.mff (+ PFM textures) in
     `directory`. fov_degrees defaults to the camera's vertical field of view;
     background: None, ("colored", rgb) or ("envmap", scale) with the scene's
@@ -214,8 +235,13 @@ def export_scene(scene, directory, name="scene", fov_degrees=None, background=No
 
     mats = scene["materials"]
     bin_names = ["mat%d" % i for i in range(len(mats))]
-    lod = dict(verts=scene["verts"], tris=scene["tris"], normals=scene["normals"], uvs=scene["uvs"],
-               mat_idx=scene["mat_idx"])
+    tris = np.asarray(scene["tris"]).reshape(-1, 3)
+    quads = np.asarray(scene.get("quads", np.zeros((0, 4))), np.uint32).reshape(-1, 4)
+    spheres = np.asarray(scene.get("spheres", np.zeros((0, 4))), np.float32).reshape(-1, 4)
+    faces = len(tris) + len(quads)
+    mat_idx = np.asarray(scene["mat_idx"])
+    lod = dict(verts=scene["verts"], tris=tris, quads=quads, spheres=spheres, normals=scene["normals"],
+               uvs=scene["uvs"], mat_idx=mat_idx[:faces], sphere_mat_idx=mat_idx[faces:])
     write_mff(os.path.join(directory, name + ".mff"), bin_names, [dict(name=name, lods=[lod])],
               [dict(name=name + "_inst", obj=0, world_to_inst=S.IDENTITY34)], deflate=deflate,
               packed_normals=packed_normals)

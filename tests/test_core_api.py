@@ -61,8 +61,6 @@ def test_lod_calls(mi):
     assert lib.polygon_add_triangle_material(lod, K.UVec3(0, 1, 2), 3) == 0
     assert lib.polygon_add_triangle(lod, K.UVec3(0, 1, 7)) == -1  # vertex out of range
     assert "out of range" in mi.core_get_dll_error()
-    assert lib.polygon_add_quad(lod, K.UVec4(0, 1, 2, 0)) == -1
-    assert "quads" in mi.core_get_dll_error()
     assert lib.polygon_get_vertex_count(lod) == 3 and lib.polygon_get_triangle_count(lod) == 1
     lo, hi = K.Vec3(), K.Vec3()
     assert lib.polygon_get_bounding_box(lod, C.byref(lo), C.byref(hi))
@@ -77,6 +75,34 @@ def test_lod_calls(mi):
     assert lib.polygon_add_triangle(lod, K.UVec3(3, 4, 5)) == 1
     idx = np.array([5, 6], np.uint16)
     assert lib.polygon_set_material_idx_bulk(lod, 0, 2, C.byref(K.bulk_array(idx))) == 2
+
+
+def test_quad_and_sphere_calls(mi):
+    """Quads follow the triangles in the face order (polygon.cpp:200); spheres
+    have their own ids, box and bulk stream (center, radius)."""
+    obj = mi.world_create_object("mixed")
+    lod = mi.object_add_lod(obj)
+    lib = mi.lib
+    for p in ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)):
+        assert lib.polygon_add_vertex(lod, K.Vec3(*p), K.Vec3(0, 0, 1), K.Vec2(0, 0)) >= 0
+    assert lib.polygon_add_triangle(lod, K.UVec3(0, 1, 2)) == 0
+    assert lib.polygon_add_quad_material(lod, K.UVec4(0, 1, 2, 3), 4) == 1
+    assert lib.polygon_add_quad(lod, K.UVec4(0, 1, 2, 9)) == -1 and "out of range" in mi.core_get_dll_error()
+    assert lib.polygon_add_triangle(lod, K.UVec3(0, 2, 3)) == -1 and "before" in mi.core_get_dll_error()
+    assert lib.polygon_get_triangle_count(lod) == 1 and lib.polygon_get_quad_count(lod) == 1
+    assert lib.polygon_get_face_count(lod) == 2
+    assert lib.spheres_add_sphere_material(lod, K.Vec3(2, 0, 0), 0.5, 3) == 0
+    sp = np.array([[0, 5, 0, 1.0], [-3, 0, 1, 0.25]], np.float32)
+    read = C.c_size_t()
+    assert lib.spheres_add_sphere_bulk(lod, 2, C.byref(K.bulk_array(sp)), None, C.byref(read)) == 1
+    assert read.value == 2 and lib.spheres_get_sphere_count(lod) == 3
+    assert lib.spheres_add_sphere(lod, K.Vec3(0, 0, 0), -1.0) == -1
+    assert lib.spheres_set_material_idx(lod, 2, 7) and not lib.spheres_set_material_idx(lod, 3, 7)
+    lo, hi = K.Vec3(), K.Vec3()
+    assert lib.spheres_get_bounding_box(lod, C.byref(lo), C.byref(hi))
+    assert (lo.x, lo.y, lo.z, hi.x, hi.y, hi.z) == (-3.25, -0.5, -1.0, 2.5, 6.0, 1.25)
+    assert lib.polygon_get_bounding_box(lod, C.byref(lo), C.byref(hi))
+    assert (lo.x, lo.y, lo.z, hi.x, hi.y, hi.z) == (0, 0, 0, 1, 1, 0)
 
 
 def test_instance_transforms(mi):

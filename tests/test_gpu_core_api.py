@@ -1,7 +1,7 @@
 """The core C API (include/mufflon_core.h) end to end on the device: a scene
 built through the loader calls, rendered with render_iterate and read with
 mufflon_get_target_image, against the oracle on the same inputs; both
-renderer variations; variance targets; PFM screenshots."""
+renderer variations; variance targets; PFM screenshots; quads and spheres."""
 import ctypes as C
 import os
 
@@ -9,7 +9,9 @@ import numpy as np
 import pytest
 
 import pt_util as P
+import scene_util as U
 from mufflon_amd import core as K
+from mufflon_amd import hip
 from mufflon_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
@@ -25,18 +27,21 @@ def _tanf(x):
     return np.float32(libm.tanf(np.float32(x)))
 
 
-def _oracle_scene(oracle, mi, c, bg):
-    """Oracle scene with the camera as the core stores it (set_view_dir, and
-    tanVFov = tanf(vFov / 2) of the float vFov it was given)."""
+def _oracle_scene_camera(mi, c):
+    """The camera as the core stores it (set_view_dir, and tanVFov =
+    tanf(vFov / 2) of the float vFov it was given)."""
     cam = mi.world_get_camera("camera")
     d, u = K.Vec3(), K.Vec3()
     mi.lib.world_get_camera_direction(cam, C.byref(d), 0)
     mi.lib.world_get_camera_up(cam, C.byref(u), 0)
     vfov = np.float32(2.0 * np.arctan(float(c["camera"]["tan_vfov"])))
-    c = dict(c)
-    c["camera"] = dict(c["camera"], view_dir=np.array([d.x, d.y, d.z], np.float32),
-                       up=np.array([u.x, u.y, u.z], np.float32), tan_vfov=_tanf(vfov / np.float32(2)))
-    return P.oracle_pt_scene_materials(oracle, c, bg)
+    return dict(c["camera"], view_dir=np.array([d.x, d.y, d.z], np.float32),
+                up=np.array([u.x, u.y, u.z], np.float32), tan_vfov=_tanf(vfov / np.float32(2)))
+
+
+def _oracle_scene(oracle, mi, c, bg):
+    """Oracle scene with the camera as the core stores it."""
+    return P.oracle_pt_scene_materials(oracle, dict(c, camera=_oracle_scene_camera(mi, c)), bg)
 
 
 def _compare(g, o, what):
@@ -128,5 +133,44 @@ def test_core_api_wireframe_renderer(oracle, tmp_path):
         assert np.array_equal(border.view(np.uint32), o.view(np.uint32))
         assert mi.render_save_screenshot(str(tmp_path / "#shortrenderer-#target"), "Border", False)
         assert (tmp_path / "WF-Border.pfm").exists()
+    finally:
+        mi.close()
+
+
+def test_core_api_mixed_scene_matches_oracle(oracle, tmp_path):
+    """Quads and spheres through the loader calls (polygon_add_quad_material,
+    spheres_add_sphere_bulk, spheres_set_material_idx_bulk): the room of
+    bilinear quads, triangle boxes and spheres with a quad light and a sphere
+    light renders as the oracle renders it."""
+    print("core api: quads and spheres", flush=True)
+    sc = S.mixed_scene(40, 32, instanced=False)
+    lod = sc["lods"][0]
+    c = dict(lod, materials=sc["materials"], point_lights=sc["point_lights"], camera=sc["camera"], textures=[])
+    mi = K.CoreInterface()
+    try:
+        scen = K.build_world(mi, c, str(tmp_path))
+        assert mi.render_enable_renderer(0, 0)
+        for name, v in (("Seed", 1), ("Max. path length", 8), ("NEE count", 1)):
+            assert mi.renderer_set_parameter_int(name, v)
+        assert mi.world_load_scenario(scen), mi.core_get_dll_error()
+        for t in ("Radiance", "Depth"):
+            mi.render_enable_render_target(t, False)
+        for _ in range(3):
+            assert mi.render_iterate()[0], mi.core_get_dll_error()
+        W, H = 40, 32
+        rad = mi.mufflon_get_target_image("Radiance", False, W, H)
+        depth = mi.mufflon_get_target_image("Depth", False, W, H)
+        cam = _oracle_scene_camera(mi, c)
+        _, lods = U.oracle_scene(oracle, [(lod["verts"], lod["tris"], lod["quads"], lod["spheres"])],
+                                 sc["inst_lod"], sc["world_to_inst"])
+        _, box = oracle.instance_boxes(sc["world_to_inst"], np.stack([L["box"] for L in lods]), sc["inst_lod"])
+        box = np.asarray(box, np.float32)
+        lt = hip.light_tree_build(S.emissive_area_lights_ex(lod, sc["materials"]), sc["point_lights"], box,
+                                  sc["materials"])
+        ps = P.oracle_pt_scene_general(oracle, dict(sc, camera=cam), lt, box)
+        mean, _ = oracle.pt_render_targets(ps, 1, 0, 8, 1, 3, [0, 3])
+        _compare(rad, mean[0], "core mixed radiance")
+        _compare(depth, mean[3], "core mixed depth")
+        assert mean[0].mean() > 0.01
     finally:
         mi.close()

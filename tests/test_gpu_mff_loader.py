@@ -1,6 +1,7 @@
 """A scene file (JSON + .mff, plain, DEFLATE-compressed, with packed normals)
 loaded through the scene loader renders bit-identically to the same scene
-built through the core API calls, on the device (the built scene's render is
+built through the core API calls, on the device (triangle scenes and a room
+of quads, triangles and spheres) (the built scene's render is
 checked against the oracle in test_gpu_core_api)."""
 import os
 
@@ -26,6 +27,12 @@ def _scene():
     return c
 
 
+def _mixed_scene():
+    sc = S.mixed_scene(40, 32, instanced=False)
+    return dict(sc["lods"][0], materials=sc["materials"], point_lights=sc["point_lights"], camera=sc["camera"],
+                textures=[])
+
+
 def _render(mi, scenario, iterations=4):
     assert mi.render_enable_renderer(0, 0), mi.core_get_dll_error()
     for name, v in (("Seed", 1), ("Max. path length", 8), ("NEE count", 1)):
@@ -38,11 +45,13 @@ def _render(mi, scenario, iterations=4):
     return mi.mufflon_get_target_image("Radiance", False, 40, 32)
 
 
-@pytest.mark.parametrize("deflate,packed", [(False, False), (True, False), (True, True)])
-def test_loaded_scene_renders_like_the_built_one(tmp_path, deflate, packed):
-    print("scene file: deflate=%s packed normals=%s" % (deflate, packed), flush=True)
-    c = _scene()
-    path = W.export_scene(c, str(tmp_path / "file"), fov_degrees=FOV_DEGREES, background=BACKGROUND,
+@pytest.mark.parametrize("deflate,packed,mixed", [(False, False, False), (True, False, False), (True, True, False),
+                                                  (False, False, True), (True, True, True)])
+def test_loaded_scene_renders_like_the_built_one(tmp_path, deflate, packed, mixed):
+    print("scene file: deflate=%s packed normals=%s quads and spheres=%s" % (deflate, packed, mixed), flush=True)
+    c = _mixed_scene() if mixed else _scene()
+    background = None if mixed else BACKGROUND
+    path = W.export_scene(c, str(tmp_path / "file"), fov_degrees=FOV_DEGREES, background=background,
                           deflate=deflate, packed_normals=packed)
     a = K.CoreInterface()
     try:
@@ -58,7 +67,7 @@ def test_loaded_scene_renders_like_the_built_one(tmp_path, deflate, packed):
         c2 = dict(c, normals=W.unpack_normals(W.pack_normals(c["normals"]))) if packed else c
         os.makedirs(str(tmp_path / "built"))
         vfov = np.float32(np.float32(FOV_DEGREES) * np.float32(np.pi)) / np.float32(180.0)
-        scen = K.build_world(b, c2, str(tmp_path / "built"), background=BACKGROUND, vfov=float(vfov))
+        scen = K.build_world(b, c2, str(tmp_path / "built"), background=background, vfov=float(vfov))
         img_built = _render(b, scen)
     finally:
         b.close()

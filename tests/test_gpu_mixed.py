@@ -100,8 +100,9 @@ def test_mixed_pt_matches_oracle(hip_ctx, oracle, mixed, form, tail):
     for t in range(8):
         g, o = hip_ctx.pt_target(pt, t), ref[t]
         bad = np.abs(g - o) > RTOL * np.abs(o) + ATOL
-        _same(g, o, "target %d" % t)
+        exact = _same(g, o, "target %d" % t)
         assert not bad.any(), t
+        assert exact, t
     assert hip_ctx.pt_nan_count(pt) == nans
     assert ref[hip.TARGET_RADIANCE].mean() > 0.01
 

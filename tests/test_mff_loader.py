@@ -245,15 +245,46 @@ def test_corrupt_binary(tmp_path):
     mi.close()
 
 
-def test_quads_are_rejected(tmp_path):
-    p = _write(str(tmp_path))
-    path = os.path.join(str(tmp_path), "s.mff")
-    b = bytearray(open(path, "rb").read())
-    i = b.find(b"LOD_")
-    struct.pack_into("<I", b, i + 8, 1)  # one quad in the first LoD header
-    open(path, "wb").write(bytes(b))
-    mi, status, err = _load(p)
-    assert status == K.LOADER_ERROR and "quads" in err, err
+@pytest.mark.parametrize("deflate", [False, True])
+def test_quads_and_spheres_load(tmp_path, deflate):
+    # a LoD of triangles, quads and spheres: quads follow the triangles in the
+    # face order with their material indices after the triangles' ones
+    # (binary.cpp:711-733); spheres come with their own material stream, one
+    # deflated block with the centres and radii when compressed (binary.cpp:539-567)
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1]], np.float32)
+    t = np.array([[0, 1, 4], [1, 5, 4]], np.uint32)
+    q = np.array([[0, 1, 2, 3], [0, 3, 4, 5], [1, 2, 5, 4]], np.uint32)
+    sp = np.array([[3, 0, 0, 0.5], [-2, 1, 0, 1.0], [0, 4, 0, 0.25], [0, 0, -3, 2.0]], np.float32)
+    lod = _lod(v, t, [0, 1, 1, 0, 1])
+    lod.update(quads=q, spheres=sp, sphere_mat_idx=np.array([1, 0, 1, 1], np.uint16))
+    objects = [dict(name="mixed", lods=[lod]), dict(name="balls", lods=[dict(verts=np.zeros((0, 3), np.float32),
+                                                                            spheres=sp[:2], sphere_mat_idx=[0, 0])])]
+    W.write_mff(os.path.join(str(tmp_path), "s.mff"), ["red", "light"], objects,
+                [dict(name="m", obj=0, world_to_inst=S.IDENTITY34), dict(name="b", obj=1, world_to_inst=S.IDENTITY34)],
+                deflate=deflate)
+    doc = {"version": "1.7", "binary": "s.mff",
+           "cameras": {"cam": {"type": "pinhole", "path": [[0, 0, -6]], "viewDir": [[0, 0, 1]], "fov": 40}},
+           "lights": {"pl": {"type": "point", "position": [0, 3, 0], "flux": [[1, 1, 1]]}},
+           "materials": {"redMat": {"type": "lambert", "albedo": [0.8, 0.1, 0.1]},
+                         "lamp": {"type": "emissive", "radiance": [4, 4, 4], "scale": [1, 1, 1]}},
+           "scenarios": {"main": {"camera": "cam", "resolution": [16, 16], "lights": ["pl"],
+                                  "materialAssignments": {"red": "redMat", "light": "lamp"}}}}
+    path = os.path.join(str(tmp_path), "s.json")
+    with open(path, "w") as f:
+        json.dump(doc, f)
+    mi, status, err = _load(path)
+    _loaded_up_to_device(status, err)
+    lib = mi.lib
+    obj = mi.world_get_object("mixed")
+    h = lib.object_add_lod(obj, 0, 0)
+    assert lib.polygon_get_triangle_count(h) == 2 and lib.polygon_get_quad_count(h) == 3
+    assert lib.polygon_get_face_count(h) == 5 and lib.spheres_get_sphere_count(h) == 4
+    lo, hi = K.Vec3(), K.Vec3()
+    assert lib.spheres_get_bounding_box(h, C.byref(lo), C.byref(hi))
+    # the sphere box grows by the object's box (sphere.cpp:87-115)
+    assert (lo.x, lo.y, lo.z, hi.x, hi.y, hi.z) == (-3.0, -2.0, -5.0, 3.5, 4.25, 1.0)
+    assert mi.loader_load_object_material_indices(0) == [0, 1]
+    assert lib.spheres_get_sphere_count(lib.object_add_lod(mi.world_get_object("balls"), 0, 0)) == 2
     mi.close()
 
 
