@@ -4,7 +4,8 @@ scenes (one instance and an instance level; hit distances, ids and surface
 parameters bit for bit), the path tracer over a room of bilinear quads,
 triangle boxes and spheres with a quad light and a sphere light (every
 target, megakernel and wavefront forms, instanced and not) and the wireframe
-renderer over the same scene."""
+renderer over the same scene, and the adjoint (texture and emission
+gradients) against the oracle's FP64 adjoint."""
 import numpy as np
 import pytest
 
@@ -124,3 +125,47 @@ def test_mixed_scene_is_refused_by_a_batch(hip_ctx):
     pt, _, _ = W.path_tracer(hip_ctx, sc, max_path_length=4)
     with pytest.raises(hip.HipError):
         hip_ctx.batch_create([pt])
+
+
+@pytest.mark.parametrize("mode", [hip.ADJOINT_EXPECTED, hip.ADJOINT_FIXED_DECISIONS])
+def test_mixed_adjoint_matches_oracle_fp64(hip_ctx, oracle, mode):
+    """The adjoint over quads and spheres: an albedo texture on the white
+    quads and triangle box (bilinear-patch uvs), a roughness texture on the
+    GGX box and spheres (spherical uvs), emission gradients of the quad and
+    sphere lights, against the oracle's FP64 adjoint of the same paths."""
+    print("mixed adjoint vs oracle fp64, mode %d" % mode, flush=True)
+    sc = S.mixed_scene(40, 32, instanced=True)
+    mats = sc["materials"].copy()
+    WHITE, GGX = 1, 4
+    mats["albedoTexture"][WHITE] = 0
+    mats["roughnessTexture"][GGX] = 1
+    sc = dict(sc, materials=mats,
+              textures=[(S.procedural_texture(16, 16, 3, 0.3, 0.9, seed=5), S.SAMPLING_LINEAR),
+                        (S.procedural_texture(16, 16, 1, 0.05, 0.6, seed=6), S.SAMPLING_LINEAR)])
+    depth, its = 6, 3
+    pt, scid, tex = W.path_tracer(hip_ctx, sc, max_path_length=depth)
+    hip_ctx.pt_set_adjoint_mode(pt, mode)
+    hip_ctx.pt_iterate(pt, its)
+    img = hip_ctx.pt_target(pt)
+    delta = np.random.default_rng(7).normal(size=img.shape).astype(np.float32)
+    hip_ctx.pt_adjoint(pt, its, delta)
+    g_gpu = [hip_ctx.pt_texture_gradient(pt, int(i), t.shape) for i, (t, _) in zip(tex, sc["textures"])]
+    lights = [0, 7]  # the quad light's and the sphere light's materials
+    e_gpu = np.stack([hip_ctx.pt_emission_gradient(pt, m) for m in lights])
+    box, _, _, _ = hip_ctx.scene_get(scid, len(sc["inst_lod"]))
+    ps = P.oracle_pt_scene_general(oracle, sc, hip_ctx.pt_light_tree(pt), box)
+    o_img, _, _ = oracle.pt_render(ps, 1, 0, depth, 1, its)
+    assert _same(img, o_img, "primal")
+    g_orc, e_orc = oracle.pt_adjoint(ps, 1, 0, depth, 1, its, delta, {0: (16, 16), 1: (16, 16)},
+                                     fixed_decisions=mode == hip.ADJOINT_FIXED_DECISIONS)
+    for t, ch in ((0, 3), (1, 1)):
+        ref = g_orc[t][..., :ch]
+        scale = np.abs(ref).max()
+        err = np.abs(g_gpu[t] - ref)
+        print("texture %d: max |grad| %.4g, max abs err %.3g" % (t, scale, float(err.max())), flush=True)
+        assert scale > 0
+        assert (err <= 1e-3 * np.abs(ref) + 1e-5 * scale).all()
+    assert (np.abs(e_orc[lights]) > 0).all()
+    # float atomics against double sums: relative to each value, with a floor
+    # for the channels in which large terms cancel
+    np.testing.assert_allclose(e_gpu, e_orc[lights], rtol=1e-3, atol=1e-4 * np.abs(e_orc[lights]).max())
