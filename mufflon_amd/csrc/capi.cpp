@@ -135,9 +135,9 @@ int mfh_destroy(mfh_context* ctx) {
             delete t;
         }
         BuildScratch& b = c.scratch;
-        for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.ranges, &b.pending,
-                          &b.pending2, &b.boxes, &b.counters, &b.flags, &b.offsets, &b.offsets2, &b.hist, &b.partials,
-                          &b.misc})
+        for (DevBuf* d : {&b.keysA, &b.keysB, &b.idsA, &b.idsB, &b.parents, &b.children, &b.levels, &b.straddle,
+                          &b.boxes, &b.counters, &b.flags, &b.offsets, &b.offsets2, &b.hist, &b.partials,
+                          &b.misc, &b.stage})
             d->release();
         if (c.evBegin) (void)hipEventDestroy(c.evBegin);
         if (c.evEnd) (void)hipEventDestroy(c.evEnd);

@@ -5,9 +5,13 @@
 //   k_morton    63-bit Morton code of each primitive centroid (build_lbvh.cu:70-125)
 //   radix sort  stable LSD sort of (code, id)           (sort_scan.hip)
 //   k_karras    Karras 2012 split hierarchy; writes parents[] in Mufflon's
-//               encoding plus explicit child lists       (build_lbvh.cu:164-245)
-//   k_refit     bottom-up AABB + SAH cost climb with one arrival counter per
-//               node; the second arriving thread merges   (build_lbvh.cu:326-411)
+//               encoding plus explicit child lists and each node's refit
+//               level                                     (build_lbvh.cu:164-245)
+//   k_refit_tiles, k_refit_level
+//               bottom-up AABB + SAH cost climb with one arrival counter per
+//               node; the second arriving thread merges   (build_lbvh.cu:326-411);
+//               per tile of leaves, then per span of each higher level, with
+//               the climbing state in LDS
 //   k_mark      collapse decision per internal node      (build_lbvh.cu:417-463)
 //   scan        inclusive sum of the keep marks          (build_lbvh.cu:748-758)
 //   k_collapse  64-byte-node BVH with enlarged boxes     (build_lbvh.cu:503-566)
@@ -30,6 +34,7 @@
 #include "mufflon_hip.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -102,6 +107,29 @@ struct TriPrims {
     __device__ __forceinline__ bool masked(i32) const { return false; }
     __device__ __forceinline__ int kind(i32) const { return kPrimTriangle; }
     static constexpr bool kRecords = true;
+    // Staging: the Morton pass, which reads every triangle in order anyway,
+    // writes its record in primitive order; the refit then gathers one
+    // 48-byte record per leaf instead of an index row and three vertices.
+    static constexpr bool kStaged = true;
+    __device__ __forceinline__ V3 centroid_staged(i32 p, float4* stage) const {
+        V3 a, b, c;
+        tri(p, a, b, c);
+        stage[3 * (size_t)p] = make_float4(a.x, a.y, a.z, __int_as_float(p));
+        stage[3 * (size_t)p + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+        stage[3 * (size_t)p + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+        return (a + b + c) / 3.0f;
+    }
+    // The record of `prim` from the staged copy, moved to sorted position idx; its box.
+    __device__ __forceinline__ void record_staged(i32 prim, int idx, const float4* stage, float4* recs, V3& lo,
+                                                  V3& hi) const {
+        const float4 r0 = stage[3 * (size_t)prim], r1 = stage[3 * (size_t)prim + 1], r2 = stage[3 * (size_t)prim + 2];
+        const V3 a = v3(r0.x, r0.y, r0.z), b = v3(r1.x, r1.y, r1.z), c = v3(r2.x, r2.y, r2.z);
+        lo = vmin(vmin(a, b), c);
+        hi = vmax(vmax(a, b), c);
+        recs[3 * (size_t)idx] = r0;
+        recs[3 * (size_t)idx + 1] = r1;
+        recs[3 * (size_t)idx + 2] = r2;
+    }
     // The traversal record of triangle `prim` at sorted position idx; its box.
     __device__ __forceinline__ void record(i32 prim, int idx, float4* recs, V3& lo, V3& hi) const {
         V3 a, b, c;
@@ -152,6 +180,7 @@ struct MixedPrims {
     }
     __device__ __forceinline__ bool masked(i32) const { return false; }
     static constexpr bool kRecords = true;
+    static constexpr bool kStaged = false;
     __device__ __forceinline__ void record(i32 prim, int idx, float4* recs, V3& lo, V3& hi) const {
         const int k = kind(prim);
         float4* r = recs + 3 * (size_t)idx;
@@ -218,6 +247,7 @@ struct InstPrims {
     __device__ __forceinline__ bool masked(i32) const { return false; }
     __device__ __forceinline__ int kind(i32) const { return 0; }
     static constexpr bool kRecords = false;
+    static constexpr bool kStaged = false;
     __device__ __forceinline__ void record(i32, int, float4*, V3&, V3&) const {}
 };
 
@@ -257,10 +287,15 @@ struct InstCost {
 // ------------------------------------------------------------- kernels
 template <class P>
 __global__ __launch_bounds__(kThreads) void k_morton(P set, int n, Box6 bb, u64* __restrict__ keys,
-                                                    i32* __restrict__ ids) {
+                                                    i32* __restrict__ ids, float4* __restrict__ stage) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    keys[i] = set.masked(i) ? ~0ull : morton_of_point(set.centroid(i), bb);
+    V3 c;
+    if constexpr (P::kStaged)
+        c = stage ? set.centroid_staged(i, stage) : set.centroid(i);
+    else
+        c = set.centroid(i);
+    keys[i] = set.masked(i) ? ~0ull : morton_of_point(c, bb);
     ids[i] = i;
 }
 
@@ -271,8 +306,36 @@ __device__ __forceinline__ int lcp(const u64* keys, int n, int i1, int i2, u64 k
     return __clzll((long long)(k1 ^ k2));
 }
 
+// Refit levels. Level 0 is a tile of kRefitTile consecutive leaves; the
+// spans of level L are kRefitTile * kSpanGrowth^L leaves. An internal node
+// belongs to the lowest level L whose span grid holds its leaf range inside
+// one span. A Karras node's index is one end of its range, so a node of level
+// L lies in the span (index / span) of that level. Nodes of level 1 and above
+// go to the straddle list as (level << 29) | node.
+constexpr int kRefitThreads = 1024;
+constexpr int kRefitIpt = 2;
+constexpr int kRefitTile = kRefitThreads * kRefitIpt;
+constexpr int kSpanGrowth = 64;
+constexpr int kMaxPrims = 1 << 29;  // node indices fit the 29-bit field of a straddle entry
+
+// Ordering of LDS accesses between the threads of a workgroup (no wait for
+// global memory operations in flight).
+__device__ __forceinline__ void lds_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local"); }
+__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+
+constexpr int kTileShift = 11;  // log2 kRefitTile
+constexpr int kGrowthShift = 6;  // log2 kSpanGrowth
+static_assert((1 << kTileShift) == kRefitTile && (1 << kGrowthShift) == kSpanGrowth, "refit spans");
+
+// Level of a node with leaf range [lo, hi] (indices below 2^29: at most 3).
+__device__ __forceinline__ int refit_level(int lo, int hi) {
+    int level = 0;
+    for (int shift = kTileShift; (lo >> shift) != (hi >> shift); shift += kGrowthShift) ++level;
+    return level;
+}
+
 __global__ __launch_bounds__(kThreads) void k_karras(const u64* __restrict__ keys, int n, i32* __restrict__ parents,
-                                                    i32* __restrict__ children, int2* __restrict__ ranges) {
+                                                    i32* __restrict__ children, uint8_t* __restrict__ levels) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n - 1) return;
     const u64 k1 = keys[idx];
@@ -302,7 +365,7 @@ __global__ __launch_bounds__(kThreads) void k_karras(const u64* __restrict__ key
     parents[right] = idx;
     children[2 * idx] = left;
     children[2 * idx + 1] = right;
-    ranges[idx] = make_int2(min(idx, j), max(idx, j));
+    levels[idx] = (uint8_t)refit_level(min(idx, j), max(idx, j));
     if (idx == 0) parents[0] = kTreeHead;
 }
 
@@ -311,16 +374,11 @@ __device__ __forceinline__ int parent_of(const i32* parents, int node) {
     return p < 0 ? ~p : p;
 }
 
-// Merge of internal node `node` from its two finished children, in the fixed
-// operand order (right, left); writes the node's box, count, cost and flags.
+// Merge of an internal node from its two finished children's boxes, in the
+// fixed operand order (right, left): box, count, cost and the collapse flags.
 template <class C>
-__device__ __forceinline__ void merge_node(int node, const i32* __restrict__ children, float4* boxes,
-                                           uint8_t* flags) {
-    const int L = children[2 * node], R = children[2 * node + 1];
-    const float4 lLo = boxes[2 * (size_t)L];
-    const float4 lHi = boxes[2 * (size_t)L + 1];
-    const float4 rLo = boxes[2 * (size_t)R];
-    const float4 rHi = boxes[2 * (size_t)R + 1];
+__device__ __forceinline__ void merge_boxes(float4 lLo, float4 lHi, float4 rLo, float4 rHi, float4& oLo, float4& oHi,
+                                            uint8_t& flag) {
     const V3 Llo = v3(lLo.x, lLo.y, lLo.z), Lhi = v3(lHi.x, lHi.y, lHi.z);
     const V3 Rlo = v3(rLo.x, rLo.y, rLo.z), Rhi = v3(rHi.x, rHi.y, rHi.z);
     const V3 nlo = vmin(Rlo, Llo), nhi = vmax(Rhi, Lhi);
@@ -329,109 +387,344 @@ __device__ __forceinline__ void merge_node(int node, const i32* __restrict__ chi
     const i32 cR = __float_as_int(rLo.w), cL = __float_as_int(lLo.w);
     const float cost = C::Ct + saR / sa * C::cost(cR) + saL / sa * C::cost(cL);
     const i32 c = C::add(cR, cL);
-    boxes[2 * (size_t)node] = make_float4(nlo.x, nlo.y, nlo.z, __int_as_float(c));
-    boxes[2 * (size_t)node + 1] = make_float4(nhi.x, nhi.y, nhi.z, cost);
-    flags[node] = (uint8_t)((C::capped(c) ? 1 : 0) | (C::cost(c) < cost ? 2 : 0));
+    oLo = make_float4(nlo.x, nlo.y, nlo.z, __int_as_float(c));
+    oHi = make_float4(nhi.x, nhi.y, nhi.z, cost);
+    flag = (uint8_t)((C::capped(c) ? 1 : 0) | (C::cost(c) < cost ? 2 : 0));
+}
+
+// Merge of internal node `node` (children L, R) with the boxes in global memory.
+template <class C>
+__device__ __forceinline__ void merge_node(int node, int L, int R, float4* boxes, uint8_t* flags) {
+    float4 oLo, oHi;
+    uint8_t f;
+    merge_boxes<C>(boxes[2 * (size_t)L], boxes[2 * (size_t)L + 1], boxes[2 * (size_t)R], boxes[2 * (size_t)R + 1],
+                   oLo, oHi, f);
+    boxes[2 * (size_t)node] = oLo;
+    boxes[2 * (size_t)node + 1] = oHi;
+    flags[node] = f;
 }
 
 // Refit, level 0. Block b owns the leaves [b*kRefitTile, (b+1)*kRefitTile).
-// A Karras internal node is one end of its leaf range, so every internal
-// node whose range lies inside the tile has its index inside the tile too:
-// the arrival counters of those nodes live in LDS, and both subtrees of such
-// a node are finished by threads of this workgroup, so the arrival protocol
-// needs only workgroup-scope ordering (no L2 write-back or invalidate across
-// XCDs). A finished subtree whose parent straddles the tile is appended to
-// `pending` for the next level.
-constexpr int kRefitThreads = 1024;
-constexpr int kRefitIpt = 2;
-constexpr int kRefitTile = kRefitThreads * kRefitIpt;
-constexpr int kSpanGrowth = 64;  // each later level covers 64 spans of the one before
+// Every level-0 node of the tile has its index inside the tile and both its
+// subtrees inside it, so the whole climb runs in LDS: the tile's topology
+// (tile-local 16-bit parent and child references), the leaf and internal
+// boxes and byte-sized arrival counters. The climb goes in rounds: a node
+// whose second child finishes is queued (LDS) for the next round, and each
+// round merges its queue one node per thread between two barriers, so the
+// waves work on packed queues instead of on divergent per-leaf climbs.
+// Boxes and flags go to global memory with coalesced stores at the end.
+// Nodes of level 1 and above are merged by the level kernels; the block
+// lists the tile's such nodes in the straddle list (one global atomic).
+constexpr u32 kLeafRef = 0x8000u;
+constexpr short kNoParent = -1;
+constexpr uint8_t kHigher = 0x80;  // tile byte: node of level 1 or above (low bits: collapse flags)
 
 template <class P, class C>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, const i32* __restrict__ ids,
                                                               const i32* __restrict__ parents,
                                                               const i32* __restrict__ children,
-                                                              const int2* __restrict__ ranges, float4* boxes,
-                                                              uint8_t* flags, float4* triVerts, i32* pending,
-                                                              u32* pendingCount) {
-    __shared__ u32 sArrived[kRefitTile];
+                                                              const uint8_t* __restrict__ levels, float4* boxes,
+                                                              uint8_t* flags, float4* triVerts,
+                                                              const float4* __restrict__ stage,
+                                                              i32* __restrict__ primIds, u32* straddle,
+                                                              u32* straddleCount) {
+    __shared__ float4 sLeafBox[kRefitTile][2];
+    __shared__ float4 sIntBox[kRefitTile][2];
+    __shared__ u32 sChild[kRefitTile];  // two 16-bit refs: local internal index, or kLeafRef | local leaf
+    __shared__ short sParentInt[kRefitTile];
+    __shared__ short sParentLeaf[kRefitTile];
+    __shared__ uint8_t sNodeByte[kRefitTile];  // kHigher, or the collapse flags of a merged node
+    __shared__ u32 sArrived[kRefitTile / 4];
+    __shared__ short sQueue[2][kRefitTile / 2];
+    __shared__ u32 sQCount[2];
+    __shared__ u32 sListCount, sListBase;
     const int nInt = n - 1;
     const int first = blockIdx.x * kRefitTile;
     const int last = min(first + kRefitTile, n) - 1;
-    for (int k = threadIdx.x; k < kRefitTile; k += kRefitThreads) sArrived[k] = 0u;
+    auto local_parent = [&](i32 p) -> short {
+        const int q = p < 0 ? ~p : p;
+        return (q >= first && q <= last) ? (short)(q - first) : kNoParent;
+    };
+    auto child_ref = [&](int c) -> u32 {
+        return c >= nInt ? (kLeafRef | (u32)((c - nInt - first) & 0x7FF)) : (u32)((c - first) & 0x7FF);
+    };
+    if (threadIdx.x == 0) {
+        sListCount = 0u;
+        sQCount[0] = 0u;
+        sQCount[1] = 0u;
+    }
+    for (int k = threadIdx.x; k < kRefitTile / 4; k += kRefitThreads) sArrived[k] = 0u;
+    u32 listSlot[kRefitIpt];
+#pragma unroll
+    for (int r = 0; r < kRefitIpt; ++r) {
+        const int k = r * kRefitThreads + (int)threadIdx.x;
+        const int node = first + k;
+        listSlot[r] = ~0u;
+        if (node < nInt) {
+            sChild[k] = child_ref(children[2 * node]) | (child_ref(children[2 * node + 1]) << 16);
+            sParentInt[k] = node == 0 ? kNoParent : local_parent(parents[node]);
+            sNodeByte[k] = levels[node] != 0 ? kHigher : 0;
+        }
+        if (node <= last && n > 1) sParentLeaf[k] = local_parent(parents[node + nInt]);
+    }
     __syncthreads();
+    // this tile's nodes of higher levels into the straddle list
+#pragma unroll
+    for (int r = 0; r < kRefitIpt; ++r) {
+        const int k = r * kRefitThreads + (int)threadIdx.x;
+        if (first + k < nInt && sNodeByte[k] == kHigher) listSlot[r] = atomicAdd(&sListCount, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sListCount) sListBase = atomicAdd(straddleCount, sListCount);
+    int leafIdx[kRefitIpt];
+#pragma unroll
     for (int k = 0; k < kRefitIpt; ++k) {
         const int idx = first + k * kRefitThreads + (int)threadIdx.x;
-        if (idx > last) break;
+        leafIdx[k] = idx;
+        if (idx > last) continue;
         const int prim = ids[idx];
+        primIds[idx] = prim;
         V3 lo, hi;
-        if constexpr (P::kRecords)
+        if constexpr (P::kStaged) {
+            if (stage)
+                set.record_staged(prim, idx, stage, triVerts, lo, hi);
+            else
+                set.record(prim, idx, triVerts, lo, hi);
+        } else if constexpr (P::kRecords) {
             set.record(prim, idx, triVerts, lo, hi);  // LoD: also lay the primitive out in BVH order
-        else
+        } else {
             set.box(prim, lo, hi);
+        }
         const i32 cnt = C::leaf(set.kind(prim));
-        const int leaf = idx + nInt;
-        boxes[2 * (size_t)leaf] = make_float4(lo.x, lo.y, lo.z, __int_as_float(cnt));
-        boxes[2 * (size_t)leaf + 1] = make_float4(hi.x, hi.y, hi.z, C::cost(cnt));
-        if (n == 1) continue;
-        int child = leaf;
-        int node = parent_of(parents, leaf);
-        for (int depth = 0; depth < kMaxDepth; ++depth) {
-            const int2 r = ranges[node];
-            if (r.x < first || r.y > last) {
-                pending[atomicAdd(pendingCount, 1u)] = child;
-                break;
-            }
-            __threadfence_block();
-            if (atomicAdd(&sArrived[node - first], 1u) == 0) break;  // sibling subtree not finished yet
-            __threadfence_block();
-            merge_node<C>(node, children, boxes, flags);
-            if (node == 0) break;
-            child = node;
-            node = parent_of(parents, node);
+        sLeafBox[idx - first][0] = make_float4(lo.x, lo.y, lo.z, __int_as_float(cnt));
+        sLeafBox[idx - first][1] = make_float4(hi.x, hi.y, hi.z, C::cost(cnt));
+    }
+#pragma unroll
+    for (int r = 0; r < kRefitIpt; ++r)
+        if (listSlot[r] != ~0u) {
+            const int node = first + r * kRefitThreads + (int)threadIdx.x;
+            straddle[sListBase + listSlot[r]] = ((u32)levels[node] << 29) | (u32)node;
+        }
+    // arrivals of the leaves: the second one queues the parent
+    auto arrive = [&](short p, int q) {
+        if (p == kNoParent || sNodeByte[p] == kHigher) return;  // a higher level merges it
+        const u32 sh = 8u * (u32)(p & 3);
+        if (((atomicAdd(&sArrived[p >> 2], 1u << sh) >> sh) & 0xFFu) == 1u) sQueue[q][atomicAdd(&sQCount[q], 1u)] = p;
+    };
+    if (n > 1) {
+#pragma unroll
+        for (int k = 0; k < kRefitIpt; ++k)
+            if (leafIdx[k] <= last) arrive(sParentLeaf[leafIdx[k] - first], 0);
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int round = 0; round < kMaxDepth; ++round) {
+        const u32 items = sQCount[cur];  // at most kRefitTile / 2: each queued node took two children
+        if (items == 0) break;
+        if (threadIdx.x < items) {
+            const short p = sQueue[cur][threadIdx.x];
+            const u32 ch = sChild[p];
+            const u32 a = ch & 0xFFFFu, b = ch >> 16;
+            const float4* lb = (a & kLeafRef) ? sLeafBox[a & 0x7FF] : sIntBox[a];
+            const float4* rb = (b & kLeafRef) ? sLeafBox[b & 0x7FF] : sIntBox[b];
+            float4 oLo, oHi;
+            uint8_t f;
+            merge_boxes<C>(lb[0], lb[1], rb[0], rb[1], oLo, oHi, f);
+            sIntBox[p][0] = oLo;
+            sIntBox[p][1] = oHi;
+            sNodeByte[p] = f;
+            arrive(sParentInt[p], cur ^ 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sQCount[cur] = 0u;
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int k = threadIdx.x; k < kRefitTile; k += kRefitThreads) {
+        const int leaf = first + k;
+        if (leaf <= last) {
+            boxes[2 * (size_t)(leaf + nInt)] = sLeafBox[k][0];
+            boxes[2 * (size_t)(leaf + nInt) + 1] = sLeafBox[k][1];
+        }
+        if (leaf < nInt && sNodeByte[k] != kHigher) {
+            boxes[2 * (size_t)leaf] = sIntBox[k][0];
+            boxes[2 * (size_t)leaf + 1] = sIntBox[k][1];
+            flags[leaf] = sNodeByte[k];
         }
     }
 }
 
-// Refit, later levels. Block b owns the leaves [b*span, (b+1)*span) and
-// climbs from the pending subtree roots of the previous level whose leaves
-// lie in its span, through the nodes inside the span (both children of such
-// a node arrive in this level: the previous one stopped below it), with
-// workgroup-scope ordering again; roots whose parent straddles the span go
-// to `pendOut`. The level whose span covers all leaves is one block and
-// ends at the root.
-constexpr int kSpanThreads = 1024;
+// Refit, level L >= 1. Block b merges the level-L nodes of its span (node
+// index >> shift == b): they are gathered from the straddle list into LDS
+// slots with a node -> slot hash. A child or parent of such a node that is
+// in the hash is of the same level (ranges nest and levels do not decrease
+// towards the root, so every same-level relative is in this span's set);
+// every other child is finished (lower level or leaf) and has its box staged
+// in LDS. The climb goes in rounds over an LDS queue, as in the tile kernel
+// (arrival counters start at the number of finished children; the arrival
+// that makes a count 2 queues the node). Boxes and flags go to global memory
+// at the end. A span with more than `cap` such nodes climbs through global
+// memory instead, with its own nodes' counters in `counters`
+// (workgroup-scope ordering).
+constexpr int kLevelThreads = 512;
+constexpr int kLevelCap = 1024;
+constexpr int kLevelHash = 2048;  // power of two, >= 2 * kLevelCap
+
+__device__ __forceinline__ u32 level_hash(int node) { return ((u32)node * 2654435761u) & (kLevelHash - 1); }
+
 template <class C>
-__global__ __launch_bounds__(kSpanThreads) void k_refit_span(int n, long long span, const i32* __restrict__ parents,
-                                                             const i32* __restrict__ children,
-                                                             const int2* __restrict__ ranges, float4* boxes,
-                                                             u32* counters, uint8_t* flags,
-                                                             const i32* __restrict__ pendIn,
-                                                             const u32* __restrict__ countIn, i32* pendOut,
-                                                             u32* countOut) {
+__global__ __launch_bounds__(kLevelThreads) void k_refit_level(int n, int level, int shift, int cap,
+                                                              const i32* __restrict__ parents,
+                                                              const i32* __restrict__ children,
+                                                              const uint8_t* __restrict__ levels,
+                                                              const u32* __restrict__ straddle,
+                                                              const u32* __restrict__ straddleCount, float4* boxes,
+                                                              uint8_t* flags, u32* counters) {
+    __shared__ i32 sNode[kLevelCap];
+    __shared__ short sParentSlot[kLevelCap];
+    __shared__ int2 sChild[kLevelCap];  // >= 0: slot of a same-level child; < 0: ~index into sFinal
+    __shared__ u32 sCnt[kLevelCap];
+    __shared__ uint8_t sFlag[kLevelCap];
+    __shared__ float4 sBox[kLevelCap][2];
+    __shared__ float4 sFinal[2 * kLevelCap][2];
+    __shared__ i32 sHashKey[kLevelHash];
+    __shared__ short sHashVal[kLevelHash];
+    __shared__ short sQueue[2][kLevelCap];
+    __shared__ u32 sQCount[2];
+    __shared__ u32 sCount, sFinalCount;
     const int nInt = n - 1;
-    const long long lo = (long long)blockIdx.x * span;
-    const long long hi = min(lo + span, (long long)n) - 1;
-    const int count = (int)*countIn;
-    for (int k = threadIdx.x; k < count; k += kSpanThreads) {
-        int child = pendIn[k];
-        const int start = child >= nInt ? child - nInt : ranges[child].x;
-        if (start < lo || start > hi) continue;
-        int node = parent_of(parents, child);
-        for (int depth = 0; depth < kMaxDepth; ++depth) {
-            const int2 r = ranges[node];
-            if (r.x < lo || r.y > hi) {
-                pendOut[atomicAdd(countOut, 1u)] = child;
-                break;
-            }
-            __threadfence_block();
-            if (atomicAdd(&counters[node], 1u) == 0) break;
-            __threadfence_block();
-            merge_node<C>(node, children, boxes, flags);
-            if (node == 0) break;
-            child = node;
-            node = parent_of(parents, node);
+    const int listCount = (int)*straddleCount;
+    if (threadIdx.x == 0) {
+        sCount = 0u;
+        sFinalCount = 0u;
+        sQCount[0] = 0u;
+        sQCount[1] = 0u;
+    }
+    for (int k = threadIdx.x; k < kLevelHash; k += kLevelThreads) sHashKey[k] = -1;
+    __syncthreads();
+    auto mine = [&](u32 e) { return (int)(e >> 29) == level && (int)((e & 0x1FFFFFFFu) >> shift) == (int)blockIdx.x; };
+    // the list in batches of loads issued together (an entry of all ones is
+    // level 7: no level's)
+    constexpr int kBatch = 8;
+    for (int base = 0; base < listCount; base += kBatch * kLevelThreads) {
+        u32 es[kBatch];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            const int k = base + b * kLevelThreads + (int)threadIdx.x;
+            es[b] = k < listCount ? straddle[k] : ~0u;
         }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            const u32 e = es[b];
+            if (!mine(e)) continue;
+            const int node = (int)(e & 0x1FFFFFFFu);
+            const u32 slot = atomicAdd(&sCount, 1u);
+            if ((int)slot >= cap) continue;
+            sNode[slot] = node;
+            for (u32 h = level_hash(node);; h = (h + 1) & (kLevelHash - 1))
+                if (atomicCAS(&sHashKey[h], -1, node) == -1) {
+                    sHashVal[h] = (short)slot;
+                    break;
+                }
+        }
+    }
+    __syncthreads();
+    const int count = (int)sCount;
+    if (count == 0) return;
+    if (count > cap) {
+        // global-memory climb over this span's nodes
+        auto lower = [&](int c) { return c >= nInt || (int)levels[c] < level; };  // finished before this level
+        for (int k = threadIdx.x; k < listCount; k += kLevelThreads) {
+            const u32 e = straddle[k];
+            if (!mine(e)) continue;
+            const int node = (int)(e & 0x1FFFFFFFu);
+            counters[node] = (lower(children[2 * node]) ? 1u : 0u) + (lower(children[2 * node + 1]) ? 1u : 0u);
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int k = threadIdx.x; k < listCount; k += kLevelThreads) {
+            const u32 e = straddle[k];
+            if (!mine(e)) continue;
+            int node = (int)(e & 0x1FFFFFFFu);
+            // a start node has both children finished before this level (not
+            // a counter value, which arrivals of other climbs raise meanwhile)
+            if (!lower(children[2 * node]) || !lower(children[2 * node + 1])) continue;
+            for (int depth = 0; depth < kMaxDepth; ++depth) {
+                merge_node<C>(node, children[2 * node], children[2 * node + 1], boxes, flags);
+                if (node == 0) break;
+                const int p = parent_of(parents, node);
+                if ((int)levels[p] != level) break;
+                __threadfence_block();
+                if (atomicAdd(&counters[p], 1u) != 1u) break;
+                __threadfence_block();
+                node = p;
+            }
+        }
+        return;
+    }
+    // slot of a node of this span's set, -1 for any other node (the probe
+    // ends at the first empty entry: the table is at most half full)
+    auto slot_of = [&](int node) -> int {
+        u32 h = level_hash(node);
+        for (int probe = 0; probe < kLevelHash; ++probe, h = (h + 1) & (kLevelHash - 1)) {
+            const int k = sHashKey[h];
+            if (k == node) return sHashVal[h];
+            if (k == -1) return -1;
+        }
+        return -1;
+    };
+    for (int s = threadIdx.x; s < count; s += kLevelThreads) {
+        const int node = sNode[s];
+        const int2 ch = make_int2(children[2 * node], children[2 * node + 1]);
+        const int p = node != 0 ? parent_of(parents, node) : -1;
+        const int sl = ch.x >= nInt ? -1 : slot_of(ch.x);
+        const int sr = ch.y >= nInt ? -1 : slot_of(ch.y);
+        int refs[2] = {sl, sr};
+        const int cs[2] = {ch.x, ch.y};
+        u32 fin = 0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (refs[q] < 0) {
+                const u32 f = atomicAdd(&sFinalCount, 1u);
+                sFinal[f][0] = boxes[2 * (size_t)cs[q]];
+                sFinal[f][1] = boxes[2 * (size_t)cs[q] + 1];
+                refs[q] = ~(int)f;
+                ++fin;
+            }
+        sChild[s] = make_int2(refs[0], refs[1]);
+        sCnt[s] = fin;
+        sParentSlot[s] = (short)(p >= 0 ? slot_of(p) : -1);
+        if (fin == 2u) sQueue[0][atomicAdd(&sQCount[0], 1u)] = (short)s;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int round = 0; round < kMaxDepth; ++round) {
+        const int items = (int)sQCount[cur];
+        if (items == 0) break;
+        for (int i = threadIdx.x; i < items; i += kLevelThreads) {
+            const int s = sQueue[cur][i];
+            const int2 r = sChild[s];
+            const float4* lb = r.x >= 0 ? sBox[r.x] : sFinal[~r.x];
+            const float4* rb = r.y >= 0 ? sBox[r.y] : sFinal[~r.y];
+            float4 oLo, oHi;
+            uint8_t f;
+            merge_boxes<C>(lb[0], lb[1], rb[0], rb[1], oLo, oHi, f);
+            sBox[s][0] = oLo;
+            sBox[s][1] = oHi;
+            sFlag[s] = f;
+            const int ps = sParentSlot[s];
+            if (ps >= 0 && atomicAdd(&sCnt[ps], 1u) == 1u) sQueue[cur ^ 1][atomicAdd(&sQCount[cur ^ 1], 1u)] = (short)ps;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sQCount[cur] = 0u;
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int s = threadIdx.x; s < count; s += kLevelThreads) {
+        const int node = sNode[s];
+        boxes[2 * (size_t)node] = sBox[s][0];
+        boxes[2 * (size_t)node + 1] = sBox[s][1];
+        flags[node] = sFlag[s];
     }
 }
 
@@ -473,10 +766,12 @@ __device__ __forceinline__ bool is_collapsed(const i32* offsets, int nInt, int n
 template <class C>
 __global__ __launch_bounds__(kThreads) void k_collapse(int n, const i32* __restrict__ parents,
                                                       const float4* __restrict__ boxes,
-                                                      const i32* __restrict__ offsets, BvhChild* __restrict__ out) {
+                                                      const i32* __restrict__ offsets, BvhChild* __restrict__ out,
+                                                      const u32* __restrict__ onlyIf) {
     const int node = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int nInt = n - 1;
     if (node >= 2 * n - 1) return;
+    if (onlyIf && *onlyIf == 0u) return;
     if (is_collapsed(offsets, nInt, node)) return;
     const int nAfter = offsets[nInt - 1];
     int parent = parents[node];
@@ -517,6 +812,15 @@ __global__ void k_fill_u32(u32* p, int n, u32 v) {
 __global__ void k_num_internal(const i32* offsets, int n, i32* out) {
     if (threadIdx.x == 0) *out = n > 1 ? offsets[n - 2] : 0;
 }
+// Internal node counts of the reference BVH and of the traversal hierarchy,
+// and whether the two differ, for one copy to the host.
+__global__ void k_build_counts(const i32* offsets, const i32* offsets2, const u32* differ, int n, i32* out) {
+    if (threadIdx.x == 0) {
+        out[0] = offsets[n - 2];
+        out[1] = offsets2 ? offsets2[n - 2] : 0;
+        out[2] = differ ? (i32)*differ : 0;
+    }
+}
 
 // World box of each instance (feeds the scene box of the instance level).
 __global__ __launch_bounds__(kThreads) void k_inst_boxes(InstPrims set, int n, float* out) {
@@ -536,6 +840,15 @@ void d2h(hipStream_t s, T* dst, const void* src, size_t count) {
     if (!dst || count == 0) return;
     MFH_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, s));
     MFH_HIP(hipStreamSynchronize(s));
+}
+
+// LDS slots of a refit level block (MFH_LBVH_LEVEL_CAPACITY lowers it, to
+// exercise the global-memory climb of fuller spans).
+int level_capacity() {
+    const char* e = std::getenv("MFH_LBVH_LEVEL_CAPACITY");
+    if (!e || !*e) return kLevelCap;
+    const int v = std::atoi(e);
+    return v < 0 ? 0 : (v > kLevelCap ? kLevelCap : v);
 }
 
 void fill_u32(void* p, int n, u32 v, hipStream_t st) {
@@ -562,17 +875,17 @@ struct Refinement {
 template <class P, class C>
 int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const float* bbox, DevBuf& nodes,
                   DevBuf& primIds, float4* triVerts, StageSink* sink, i32* dNumInternal = nullptr,
-                  Refinement* refine = nullptr) {
+                  Refinement* refine = nullptr, float4* stage = nullptr) {
     hipStream_t st = ctx.stream;
+    MFH_REQUIRE(n < kMaxPrims, "an LBVH build holds fewer than 2^29 primitives");
     s.keysA.reserve(8 * (size_t)n);
     s.keysB.reserve(8 * (size_t)n);
     s.idsA.reserve(4 * (size_t)n);
     s.idsB.reserve(4 * (size_t)n);
     s.parents.reserve(4 * (size_t)(2 * n - 1));
     s.children.reserve(8 * (size_t)std::max(n - 1, 1));
-    s.ranges.reserve(8 * (size_t)std::max(n - 1, 1));
-    s.pending.reserve(4 * (size_t)n + 4);
-    s.pending2.reserve(4 * (size_t)n + 4);
+    s.levels.reserve((size_t)std::max(n - 1, 1));
+    s.straddle.reserve(4 * (size_t)n + 4);
     s.boxes.reserve(32 * (size_t)(2 * n - 1));
     s.counters.reserve(4 * (size_t)std::max(n - 1, 1));
     s.flags.reserve((size_t)std::max(n - 1, 1));
@@ -584,7 +897,7 @@ int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const floa
     u64* keys = s.keysA.as<u64>();
     i32* ids = s.idsA.as<i32>();
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_morton<P>), dim3(blocks_for(n)), dim3(kThreads), 0, st, set, n, bb, keys,
-                       ids);
+                       ids, stage);
     MFH_HIP(hipGetLastError());
     if (sink) d2h(st, sink->unsortedKeys, keys, n);
     radix_sort_pairs(keys, s.keysB.as<u64>(), ids, s.idsB.as<i32>(), n, 0, 64, s, st);
@@ -592,34 +905,30 @@ int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const floa
         d2h(st, sink->sortedKeys, keys, n);
         d2h(st, sink->sortedIds, ids, n);
     }
-    MFH_HIP(hipMemcpyAsync(primIds.ptr, ids, 4 * (size_t)n, hipMemcpyDeviceToDevice, st));
     i32* parents = s.parents.as<i32>();
     i32* children = s.children.as<i32>();
     float4* boxes = s.boxes.as<float4>();
-    if (n > 1) {
+    uint8_t* levels = s.levels.as<uint8_t>();
+    u32* straddle = s.straddle.as<u32>();
+    u32* straddleCount = straddle + n;
+    fill_u32(straddleCount, 1, 0u, st);
+    if (n > 1)
         hipLaunchKernelGGL(k_karras, dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, keys, n, parents, children,
-                           s.ranges.as<int2>());
-        fill_u32(s.counters.ptr, n - 1, 0u, st);
-    }
-    i32* pending = s.pending.as<i32>();
-    u32* pendingCount = reinterpret_cast<u32*>(pending + n);
-    i32* pending2 = s.pending2.as<i32>();
-    u32* pendingCount2 = reinterpret_cast<u32*>(pending2 + n);
-    fill_u32(pendingCount, 1, 0u, st);
+                           levels);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_tiles<P, C>), dim3((n + kRefitTile - 1) / kRefitTile),
-                       dim3(kRefitThreads), 0, st, set, n, ids, parents, children, s.ranges.as<int2>(), boxes,
-                       s.flags.as<uint8_t>(), triVerts, pending, pendingCount);
+                       dim3(kRefitThreads), 0, st, set, n, ids, parents, children, levels, boxes,
+                       s.flags.as<uint8_t>(), triVerts, stage, primIds.as<i32>(), straddle, straddleCount);
     MFH_HIP(hipGetLastError());
-    for (long long span = kRefitTile; n > 1 && span < n;) {
-        span *= kSpanGrowth;
-        const int blocks = (int)((n + span - 1) / span);
-        fill_u32(pendingCount2, 1, 0u, st);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_span<C>), dim3(blocks), dim3(kSpanThreads), 0, st, n, span,
-                           parents, children, s.ranges.as<int2>(), boxes, s.counters.as<u32>(),
-                           s.flags.as<uint8_t>(), pending, pendingCount, pending2, pendingCount2);
+    const int cap = level_capacity();
+    int level = 0;
+    for (int shift = kTileShift; n > 1 && (1ll << shift) < n;) {
+        shift += kGrowthShift;
+        ++level;
+        const int blocks = (int)((n + (1ll << shift) - 1) >> shift);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_refit_level<C>), dim3(blocks), dim3(kLevelThreads), 0, st, n, level,
+                           shift, cap, parents, children, levels, straddle, straddleCount, boxes,
+                           s.flags.as<uint8_t>(), s.counters.as<u32>());
         MFH_HIP(hipGetLastError());
-        std::swap(pending, pending2);
-        std::swap(pendingCount, pendingCount2);
     }
     if (n == 1) {
         if (dNumInternal) {
@@ -645,32 +954,33 @@ int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const floa
     scan_inclusive_i32(offsets, offsets, n - 1, s, st);
     nodes.reserve(64 * (size_t)(n - 1));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st, n,
-                       parents, boxes, offsets, nodes.as<BvhChild>());
+                       parents, boxes, offsets, nodes.as<BvhChild>(), nullptr);
     MFH_HIP(hipGetLastError());
     if (dNumInternal) {
         hipLaunchKernelGGL(k_num_internal, dim3(1), dim3(64), 0, st, offsets, n, dNumInternal);
         MFH_HIP(hipGetLastError());
         return -1;
     }
-    i32 nAfter = 0;
-    u32 differs = 0;
-    MFH_HIP(hipMemcpyAsync(&nAfter, offsets + (n - 2), 4, hipMemcpyDeviceToHost, st));
-    if (refining) MFH_HIP(hipMemcpyAsync(&differs, differ, 4, hipMemcpyDeviceToHost, st));
-    MFH_HIP(hipStreamSynchronize(st));
     if (refining) {
-        refine->numInternal = -1;
-        if (differs) {
-            scan_inclusive_i32(offsets2, offsets2, n - 1, s, st);
-            refine->nodes->reserve(64 * (size_t)(n - 1));
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st,
-                               n, parents, boxes, offsets2, refine->nodes->as<BvhChild>());
-            MFH_HIP(hipGetLastError());
-            i32 n2 = 0;
-            MFH_HIP(hipMemcpyAsync(&n2, offsets2 + (n - 2), 4, hipMemcpyDeviceToHost, st));
-            MFH_HIP(hipStreamSynchronize(st));
-            refine->numInternal = n2;
-        }
+        // the traversal hierarchy is collapsed whether or not it differs (its
+        // collapse exits at once when it does not), so that the build waits
+        // for the device once
+        scan_inclusive_i32(offsets2, offsets2, n - 1, s, st);
+        refine->nodes->reserve(64 * (size_t)(n - 1));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st, n,
+                           parents, boxes, offsets2, refine->nodes->as<BvhChild>(), differ);
+        MFH_HIP(hipGetLastError());
     }
+    s.misc.reserve(16);
+    i32* dCounts = s.misc.as<i32>();
+    hipLaunchKernelGGL(k_build_counts, dim3(1), dim3(64), 0, st, offsets, refining ? offsets2 : nullptr, differ, n,
+                       dCounts);
+    MFH_HIP(hipGetLastError());
+    i32 counts[3] = {0, 0, 0};
+    MFH_HIP(hipMemcpyAsync(counts, dCounts, sizeof(counts), hipMemcpyDeviceToHost, st));
+    MFH_HIP(hipStreamSynchronize(st));
+    const i32 nAfter = counts[0];
+    if (refining) refine->numInternal = counts[2] ? counts[1] : -1;
     if (sink) {
         d2h(st, sink->parents, parents, 2 * (size_t)n - 1);
         d2h(st, sink->boxes, boxes, 8 * (size_t)(2 * n - 1));
@@ -698,9 +1008,10 @@ void build_lod_bvh(Context& ctx, Lod& lod, const float* bbox, StageSink* sink) {
                                                              &refine);
     } else {
         TriPrims set{lod.verts.as<float>(), lod.tris.as<u32>()};
+        ctx.scratch.stage.reserve(48 * (size_t)n);
         lod.numInternal = build_generic<TriPrims, LodCost>(ctx, ctx.scratch, set, n, lod.box, lod.nodes,
                                                            lod.primIds, lod.triVerts.as<float4>(), sink, nullptr,
-                                                           &refine);
+                                                           &refine, ctx.scratch.stage.as<float4>());
     }
     lod.travNumInternal = refine.numInternal;
     lod.built = true;
@@ -748,19 +1059,19 @@ namespace {
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Bytes of every scratch buffer of a build over n primitives, in carve order.
-constexpr int kNoAllocParts = 18;
+constexpr int kNoAllocParts = 17;
 struct NoAllocSizes {
     size_t b[kNoAllocParts];
 };
 constexpr NoAllocSizes noalloc_sizes(long long n) {
     const long long n1 = n > 1 ? n - 1 : 1;
-    const long long tiles = (n + 4095) / 4096;                          // radix sort tiles of 4096 keys
+    const long long tiles = (n + kRadixTileKeys - 1) / kRadixTileKeys;  // radix sort tiles
     const long long scanned = n > 256 * tiles ? n : 256 * tiles;        // longest scan: marks or digit counts
     const long long scanBlocks = (scanned + 2047) / 2048;               // scan chunks of 2048 values
     return NoAllocSizes{{(size_t)(8 * n), (size_t)(8 * n), (size_t)(4 * n), (size_t)(4 * n),  // keys, ids A/B
                          (size_t)(4 * (2 * n - 1)),                                            // parents
-                         (size_t)(8 * n1), (size_t)(8 * n1),                                   // children, ranges
-                         (size_t)(4 * n + 4), (size_t)(4 * n + 4),                             // pending lists
+                         (size_t)(8 * n1), (size_t)n1,                                         // children, levels
+                         (size_t)(4 * n + 4),                                                  // straddle list
                          (size_t)(32 * (2 * n - 1)),                                           // node boxes
                          (size_t)(4 * n1), (size_t)n1, (size_t)(4 * n1),                       // counters, flags, offsets
                          (size_t)(4 * 256 * tiles), (size_t)(4 * scanBlocks),                  // digit counts, partials
@@ -831,12 +1142,12 @@ int build_lod_bvh_noalloc(Context& ctx, const float* dVerts, int numVerts, const
     MFH_REQUIRE(off <= kNoAllocBytes, "build scratch layout exceeds the array");
     BuildScratch s;
     s.keysA = part[0]; s.keysB = part[1]; s.idsA = part[2]; s.idsB = part[3];
-    s.parents = part[4]; s.children = part[5]; s.ranges = part[6];
-    s.pending = part[7]; s.pending2 = part[8]; s.boxes = part[9];
-    s.counters = part[10]; s.flags = part[11]; s.offsets = part[12];
-    s.hist = part[13]; s.partials = part[14];
-    float4* triVerts = part[15].as<float4>();
-    u32* badIndex = part[16].as<u32>();
+    s.parents = part[4]; s.children = part[5]; s.levels = part[6];
+    s.straddle = part[7]; s.boxes = part[8];
+    s.counters = part[9]; s.flags = part[10]; s.offsets = part[11];
+    s.hist = part[12]; s.partials = part[13];
+    float4* triVerts = part[14].as<float4>();
+    u32* badIndex = part[15].as<u32>();
 
     hipStream_t st = ctx.stream;
     fill_u32(badIndex, 1, 0u, st);

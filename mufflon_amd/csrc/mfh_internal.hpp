@@ -74,21 +74,25 @@ struct DevBuf {
     }
 };
 
+// Keys per tile of the radix sort's histogram and scatter passes.
+constexpr int kRadixTileKeys = 2048;
+
 // Scratch of one LBVH build (sizes for n primitives).
 struct BuildScratch {
     DevBuf keysA, keysB, idsA, idsB;  // sort ping-pong
     DevBuf parents;                   // 2n-1 (reference encoding, left child as ~parent)
     DevBuf children;                  // 2(n-1): left, right node of each internal node
-    DevBuf ranges;                    // n-1 int2: first and last leaf under each internal node
-    DevBuf pending, pending2;         // n+1 each: subtree roots handed to the next refit level, then their count
+    DevBuf levels;                    // n-1 bytes: refit level of each internal node
+    DevBuf straddle;                  // n u32: internal nodes of refit level >= 1 (packed), then their count
     DevBuf boxes;                     // (2n-1) * 32 bytes: {lo, countBits}, {hi, cost}
-    DevBuf counters;                  // n-1 arrival counters
+    DevBuf counters;                  // n-1 arrival counters (refit levels past their LDS capacity)
     DevBuf flags;                     // n-1 bytes: bit0 count capped, bit1 leaf cheaper than split
     DevBuf offsets;                   // n-1 inclusive scan of the keep marks
     DevBuf offsets2;                  // n-1 (+1 flag) the same for the traversal hierarchy
     DevBuf hist;                      // radix digit counts
     DevBuf partials;                  // scan partial sums
     DevBuf misc;                      // small values (scene box, counts)
+    DevBuf stage;                     // n * 48 bytes: triangle records in primitive order
 };
 
 // Host copies of the intermediate stages, filled when a build is asked for them.

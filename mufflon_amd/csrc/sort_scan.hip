@@ -127,8 +127,9 @@ void scan_i32(const int* in, int* out, int n, BuildScratch& s, hipStream_t strea
 
 // ---------------------------------------------------------------- radix sort
 constexpr int kRsThreads = 256;
-constexpr int kRsIpt = 16;
-constexpr int kRsTile = kRsThreads * kRsIpt;
+constexpr int kRsTile = kRadixTileKeys;
+constexpr int kRsIpt = kRsTile / kRsThreads;
+static_assert(kRsIpt * kRsThreads == kRsTile, "radix tile");
 constexpr int kRsWaves = kRsThreads / 64;
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const u64* __restrict__ keys, int n, int shift,

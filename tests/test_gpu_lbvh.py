@@ -41,9 +41,20 @@ def _compare_build(hip_ctx, oracle, v, t, bbox=None):
     return lod, ob
 
 
-@pytest.mark.parametrize("n,seed", [(1, 1), (2, 2), (3, 3), (257, 4), (4097, 5), (100_000, 6)])
+@pytest.mark.parametrize("n,seed", [(1, 1), (2, 2), (3, 3), (257, 4), (2048, 14), (2049, 15), (4097, 5),
+                                    (100_000, 6), (131_073, 16)])
 def test_lbvh_matches_oracle(hip_ctx, oracle, n, seed):
     v, t = S.random_triangle_soup(n, seed=seed)
+    _compare_build(hip_ctx, oracle, v, t)
+
+
+@pytest.mark.parametrize("cap", [0, 16, 200])
+def test_lbvh_level_capacity_fallback(hip_ctx, oracle, cap, monkeypatch):
+    """Refit spans holding more straddling nodes than their LDS slots climb
+    through global memory instead; every capacity builds the same BVH."""
+    monkeypatch.setenv("MFH_LBVH_LEVEL_CAPACITY", str(cap))
+    print("level capacity %d" % cap, flush=True)
+    v, t = S.random_triangle_soup(300_000, seed=13)
     _compare_build(hip_ctx, oracle, v, t)
 
 
