@@ -1074,7 +1074,8 @@ constexpr NoAllocSizes noalloc_sizes(long long n) {
                          (size_t)(4 * n + 4),                                                  // straddle list
                          (size_t)(32 * (2 * n - 1)),                                           // node boxes
                          (size_t)(4 * n1), (size_t)n1, (size_t)(4 * n1),                       // counters, flags, offsets
-                         (size_t)(4 * 256 * tiles), (size_t)(4 * scanBlocks),                  // digit counts, partials
+                         (size_t)(4 * 256 * (tiles + 8 * ((tiles + 31) / 32))),               // digit counts + group sums
+                         (size_t)(4 * scanBlocks),                                             // scan partials
                          (size_t)(48 * n),                                                     // BVH-order triangles
                          (size_t)16, (size_t)256}};                                            // index check, spare
 }

@@ -220,6 +220,11 @@ int build_lod_bvh_noalloc(Context& ctx, const float* dVerts, int numVerts, const
 // Stable LSD radix sort of (u64 key, i32 value) pairs; result ends in keysA/idsA.
 void radix_sort_pairs(uint64_t* keysA, uint64_t* keysB, int32_t* idsA, int32_t* idsB, int n, int beginBit,
                       int endBit, BuildScratch& s, hipStream_t stream);
+// Scratch bytes a radix sort of n pairs over [beginBit, endBit) reserves:
+// digit counts (+ group sums) in BuildScratch::hist, scan partials in
+// BuildScratch::partials (callers that must not allocate size them up front).
+size_t radix_sort_hist_bytes(int n, int beginBit, int endBit);
+size_t radix_sort_partials_bytes(int n);
 // Inclusive prefix sum of int32 values, in place allowed.
 void scan_inclusive_i32(const int32_t* in, int32_t* out, int n, BuildScratch& s, hipStream_t stream);
 // Exclusive prefix sum of uint32 values, in place allowed.

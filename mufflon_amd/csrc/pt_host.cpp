@@ -458,9 +458,8 @@ WfState wavefront_state(Context& c, PathTracer& t) {
         W.sortIds[b] = t.wfSortIds[b].as<int32_t>();
     }
     // radix-sort scratch of the tracer sized up front (no allocation inside a captured graph)
-    const size_t tiles = ((size_t)N + 4095) / 4096;
-    t.wfSortScratch.hist.reserve(4 * 256 * tiles);
-    t.wfSortScratch.partials.reserve(4 * (256 * tiles / 2048 + 64));
+    t.wfSortScratch.hist.reserve(radix_sort_hist_bytes(N, 0, 32));
+    t.wfSortScratch.partials.reserve(radix_sort_partials_bytes(N));
     if (t.tune.traceGrid <= 0 || t.tune.shadeGrid <= 0) {
         const int cus = num_cus(c);
         t.tune.traceGrid = 0;

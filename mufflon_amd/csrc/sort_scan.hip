@@ -11,6 +11,8 @@
 //      reordered in LDS by digit, then written out as contiguous runs.
 // Stability holds because ranks follow (round, wave, lane) order, which is
 // the element order inside a tile, and tiles are laid out in order.
+#include <algorithm>
+
 #include "mfh_device.hpp"
 #include "mfh_internal.hpp"
 
@@ -125,6 +127,11 @@ void scan_i32(const int* in, int* out, int n, BuildScratch& s, hipStream_t strea
     MFH_HIP(hipGetLastError());
 }
 
+__global__ void k_zero_u32(u32* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 // ---------------------------------------------------------------- radix sort
 constexpr int kRsThreads = 256;
 constexpr int kRsTile = kRadixTileKeys;
@@ -132,8 +139,17 @@ constexpr int kRsIpt = kRsTile / kRsThreads;
 static_assert(kRsIpt * kRsThreads == kRsTile, "radix tile");
 constexpr int kRsWaves = kRsThreads / 64;
 
+// Sorts of at most kFusedTiles tiles skip the separate scan: the histogram
+// pass writes tile-major counts and adds them into per-group digit sums
+// (kGroupTiles tiles a group), and each scatter block derives its runs'
+// destinations from the group sums and the counts of the earlier tiles of
+// its group.
+constexpr int kGroupTiles = 32;
+constexpr int kFusedTiles = 1024;
+
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const u64* __restrict__ keys, int n, int shift,
-                                                       u32* __restrict__ counts, int numTiles) {
+                                                       u32* __restrict__ counts, int numTiles,
+                                                       u32* __restrict__ groupSums) {
     __shared__ u32 sHist[256];
     sHist[threadIdx.x] = 0;
     __syncthreads();
@@ -147,14 +163,20 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const u64* __restrict__ 
         }
     }
     __syncthreads();
-    counts[threadIdx.x * numTiles + blockIdx.x] = sHist[threadIdx.x];
+    const u32 c = sHist[threadIdx.x];
+    if (groupSums) {
+        counts[blockIdx.x * 256 + threadIdx.x] = c;
+        if (c) atomicAdd(&groupSums[(blockIdx.x / kGroupTiles) * 256 + threadIdx.x], c);
+    } else {
+        counts[threadIdx.x * numTiles + blockIdx.x] = c;
+    }
 }
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const u64* __restrict__ kin,
                                                           const i32* __restrict__ vin,
                                                           u64* __restrict__ kout, i32* __restrict__ vout,
                                                           int n, int shift, const u32* __restrict__ offs,
-                                                          int numTiles) {
+                                                          int numTiles, const u32* __restrict__ groupSums) {
     __shared__ u64 sKeys[kRsTile];
     __shared__ i32 sVals[kRsTile];
     __shared__ u32 sRun[256];
@@ -169,7 +191,25 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const u64* __restrict
     const u64 ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 
     sRun[tid] = 0;
-    sGlob[tid] = offs[tid * numTiles + blockIdx.x];
+    if (groupSums) {
+        // destination of this tile's run of digit tid: all keys of smaller
+        // digits, then this digit's keys in earlier groups and earlier tiles
+        const int g = blockIdx.x / kGroupTiles, groups = (numTiles + kGroupTiles - 1) / kGroupTiles;
+        u32 total = 0, before = 0;
+#pragma unroll 8
+        for (int gg = 0; gg < groups; ++gg) {
+            const u32 v = groupSums[gg * 256 + tid];
+            total += v;
+            before += gg < g ? v : 0u;
+        }
+#pragma unroll 8
+        for (int t = g * kGroupTiles; t < (int)blockIdx.x; ++t) before += offs[t * 256 + tid];
+        int all;
+        const int digitStart = block_excl_scan((int)total, sScanWave, all);
+        sGlob[tid] = (u32)digitStart + before;
+    } else {
+        sGlob[tid] = offs[tid * numTiles + blockIdx.x];
+    }
 
     u64 key[kRsIpt];
     i32 val[kRsIpt];
@@ -253,23 +293,47 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int n, BuildScratch& 
     scan_i32<false>(reinterpret_cast<const int*>(in), reinterpret_cast<int*>(out), n, s, stream);
 }
 
+size_t radix_sort_hist_bytes(int n, int beginBit, int endBit) {
+    const size_t numTiles = ((size_t)std::max(n, 1) + kRsTile - 1) / kRsTile;
+    const size_t groups = (numTiles + kGroupTiles - 1) / kGroupTiles;
+    const size_t passes8 = (size_t)(endBit - beginBit + 7) / 8;
+    return sizeof(u32) * 256 * (numTiles + (numTiles <= (size_t)kFusedTiles ? groups * passes8 : 0));
+}
+
+size_t radix_sort_partials_bytes(int n) {
+    // the longest scan of a sort is its digit counts (unfused sorts)
+    const size_t scanned = 256 * (((size_t)std::max(n, 1) + kRsTile - 1) / kRsTile);
+    return sizeof(int) * ((scanned + kScanChunk - 1) / kScanChunk + 1);
+}
+
 void radix_sort_pairs(uint64_t* keysA, uint64_t* keysB, int32_t* idsA, int32_t* idsB, int n, int beginBit,
                       int endBit, BuildScratch& s, hipStream_t stream) {
     if (n <= 1) return;
     const int numTiles = (n + kRsTile - 1) / kRsTile;
-    s.hist.reserve(sizeof(u32) * 256 * (size_t)numTiles);
+    const bool fused = numTiles <= kFusedTiles;
+    const int groups = (numTiles + kGroupTiles - 1) / kGroupTiles;
+    const int passes8 = (endBit - beginBit + 7) / 8;
+    // digit counts, then (fused) one set of group sums per pass
+    s.hist.reserve(radix_sort_hist_bytes(n, beginBit, endBit));
     u32* counts = s.hist.as<u32>();
+    u32* groupSums = fused ? counts + 256 * (size_t)numTiles : nullptr;
+    if (fused) {
+        const int words = 256 * groups * passes8;
+        hipLaunchKernelGGL(k_zero_u32, dim3((words + kScanThreads - 1) / kScanThreads), dim3(kScanThreads), 0,
+                           stream, groupSums, words);
+    }
     u64* kin = keysA;
     u64* kout = keysB;
     i32* vin = idsA;
     i32* vout = idsB;
     int passes = 0;
     for (int shift = beginBit; shift < endBit; shift += 8) {
+        u32* gs = fused ? groupSums + 256 * (size_t)groups * passes : nullptr;
         hipLaunchKernelGGL(k_rs_hist, dim3(numTiles), dim3(kRsThreads), 0, stream, kin, n, shift, counts,
-                           numTiles);
-        scan_exclusive_u32(counts, counts, 256 * numTiles, s, stream);
+                           numTiles, gs);
+        if (!fused) scan_exclusive_u32(counts, counts, 256 * numTiles, s, stream);
         hipLaunchKernelGGL(k_rs_scatter, dim3(numTiles), dim3(kRsThreads), 0, stream, kin, vin, kout, vout, n,
-                           shift, counts, numTiles);
+                           shift, counts, numTiles, gs);
         MFH_HIP(hipGetLastError());
         std::swap(kin, kout);
         std::swap(vin, vout);
