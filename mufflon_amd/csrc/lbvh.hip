@@ -763,15 +763,13 @@ __device__ __forceinline__ bool is_collapsed(const i32* offsets, int nInt, int n
     return node > 0 && node < nInt && (offsets[node] - offsets[node - 1]) == 0;
 }
 
+// The collapsed BVH's record of node `node` (its box enlarged, its index in
+// the collapsed numbering) under the keep scan `offsets`.
 template <class C>
-__global__ __launch_bounds__(kThreads) void k_collapse(int n, const i32* __restrict__ parents,
-                                                      const float4* __restrict__ boxes,
-                                                      const i32* __restrict__ offsets, BvhChild* __restrict__ out,
-                                                      const u32* __restrict__ onlyIf) {
-    const int node = blockIdx.x * blockDim.x + threadIdx.x + 1;
+__device__ __forceinline__ void collapse_node(int node, int n, const i32* __restrict__ parents,
+                                              const float4* __restrict__ boxes, const i32* __restrict__ offsets,
+                                              BvhChild* __restrict__ out) {
     const int nInt = n - 1;
-    if (node >= 2 * n - 1) return;
-    if (onlyIf && *onlyIf == 0u) return;
     if (is_collapsed(offsets, nInt, node)) return;
     const int nAfter = offsets[nInt - 1];
     int parent = parents[node];
@@ -801,6 +799,20 @@ __global__ __launch_bounds__(kThreads) void k_collapse(int n, const i32* __restr
     r.index = outNode;
     r.primCount = primCount;
     out[outIdx + off] = r;
+}
+
+// The reference's collapsed BVH and, with offsets2, the traversal hierarchy
+// (written only when *differ is set) in one pass over the nodes.
+template <class C>
+__global__ __launch_bounds__(kThreads) void k_collapse(int n, const i32* __restrict__ parents,
+                                                      const float4* __restrict__ boxes,
+                                                      const i32* __restrict__ offsets, BvhChild* __restrict__ out,
+                                                      const i32* __restrict__ offsets2, BvhChild* __restrict__ out2,
+                                                      const u32* __restrict__ differ) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (node >= 2 * n - 1) return;
+    collapse_node<C>(node, n, parents, boxes, offsets, out);
+    if (offsets2 && *differ != 0u) collapse_node<C>(node, n, parents, boxes, offsets2, out2);
 }
 
 // Zero fill and count hand-off of the build (kernels, not memsets, so that a
@@ -951,25 +963,23 @@ int build_generic(Context& ctx, BuildScratch& s, const P& set, int n, const floa
     }
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mark<C>), dim3(blocks_for(n - 1)), dim3(kThreads), 0, st, n - 1, parents,
                        s.flags.as<uint8_t>(), offsets, boxes, refining ? refine->refineAbove : 0, offsets2, differ);
-    scan_inclusive_i32(offsets, offsets, n - 1, s, st);
+    // the reference's BVH and the traversal hierarchy (collapsed whether or
+    // not it differs, so that the build waits for the device once; its
+    // records are written only when it differs)
+    if (refining)
+        scan_inclusive_i32_pair(offsets, offsets2, n - 1, s, st);
+    else
+        scan_inclusive_i32(offsets, offsets, n - 1, s, st);
     nodes.reserve(64 * (size_t)(n - 1));
+    if (refining) refine->nodes->reserve(64 * (size_t)(n - 1));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st, n,
-                       parents, boxes, offsets, nodes.as<BvhChild>(), nullptr);
+                       parents, boxes, offsets, nodes.as<BvhChild>(), offsets2,
+                       refining ? refine->nodes->as<BvhChild>() : nullptr, differ);
     MFH_HIP(hipGetLastError());
     if (dNumInternal) {
         hipLaunchKernelGGL(k_num_internal, dim3(1), dim3(64), 0, st, offsets, n, dNumInternal);
         MFH_HIP(hipGetLastError());
         return -1;
-    }
-    if (refining) {
-        // the traversal hierarchy is collapsed whether or not it differs (its
-        // collapse exits at once when it does not), so that the build waits
-        // for the device once
-        scan_inclusive_i32(offsets2, offsets2, n - 1, s, st);
-        refine->nodes->reserve(64 * (size_t)(n - 1));
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_collapse<C>), dim3(blocks_for(2 * n - 2)), dim3(kThreads), 0, st, n,
-                           parents, boxes, offsets2, refine->nodes->as<BvhChild>(), differ);
-        MFH_HIP(hipGetLastError());
     }
     s.misc.reserve(16);
     i32* dCounts = s.misc.as<i32>();

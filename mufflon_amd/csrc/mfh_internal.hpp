@@ -227,6 +227,8 @@ size_t radix_sort_hist_bytes(int n, int beginBit, int endBit);
 size_t radix_sort_partials_bytes(int n);
 // Inclusive prefix sum of int32 values, in place allowed.
 void scan_inclusive_i32(const int32_t* in, int32_t* out, int n, BuildScratch& s, hipStream_t stream);
+// The same over two arrays of n values at once, in place.
+void scan_inclusive_i32_pair(int32_t* a, int32_t* b, int n, BuildScratch& s, hipStream_t stream);
 // Exclusive prefix sum of uint32 values, in place allowed.
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int n, BuildScratch& s, hipStream_t stream);
 

@@ -53,8 +53,17 @@ __device__ __forceinline__ int block_excl_scan(int v, int* sWave, int& total) {
     return waveOff + incl - v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int* in, int n, int* partials) {
+// The scan kernels take up to two arrays of n values; blockIdx.y picks the
+// array (its partial sums follow the first array's in `partials`).
+struct ScanArrays {
+    const int* in[2];
+    int* out[2];
+};
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(ScanArrays a, int n, int* partials) {
     __shared__ int sWave[kScanThreads / 64];
+    const int* in = a.in[blockIdx.y];
+    partials += blockIdx.y * gridDim.x;
     const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanIpt;
     int s = 0;
 #pragma unroll
@@ -67,9 +76,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(const int* in, i
     if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of the partial sums in place.
+// Single block per array: exclusive scan of the partial sums in place.
 __global__ __launch_bounds__(kScanThreads) void k_scan_top(int* partials, int n) {
     __shared__ int sWave[kScanThreads / 64];
+    partials += blockIdx.y * n;
     int carry = 0;
     for (int base = 0; base < n; base += kScanChunk) {
         int v[kScanIpt];
@@ -93,9 +103,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(int* partials, int n)
 }
 
 template <bool kInclusive>
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, int* out, int n,
-                                                             const int* partials) {
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(ScanArrays a, int n, const int* partials) {
     __shared__ int sWave[kScanThreads / 64];
+    const int* in = a.in[blockIdx.y];
+    int* out = a.out[blockIdx.y];
+    partials += blockIdx.y * gridDim.x;
     const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanIpt;
     int v[kScanIpt];
     int s = 0;
@@ -115,14 +127,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const int* in, int*
 }
 
 template <bool kInclusive>
-void scan_i32(const int* in, int* out, int n, BuildScratch& s, hipStream_t stream) {
+void scan_i32(const ScanArrays& a, int arrays, int n, BuildScratch& s, hipStream_t stream) {
     if (n <= 0) return;
     const int blocks = (n + kScanChunk - 1) / kScanChunk;
-    s.partials.reserve(sizeof(int) * (size_t)blocks);
+    s.partials.reserve(sizeof(int) * (size_t)blocks * arrays);
     int* partials = s.partials.as<int>();
-    hipLaunchKernelGGL(k_scan_partials, dim3(blocks), dim3(kScanThreads), 0, stream, in, n, partials);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, stream, partials, blocks);
-    hipLaunchKernelGGL(k_scan_apply<kInclusive>, dim3(blocks), dim3(kScanThreads), 0, stream, in, out, n,
+    hipLaunchKernelGGL(k_scan_partials, dim3(blocks, arrays), dim3(kScanThreads), 0, stream, a, n, partials);
+    hipLaunchKernelGGL(k_scan_top, dim3(1, arrays), dim3(kScanThreads), 0, stream, partials, blocks);
+    hipLaunchKernelGGL(k_scan_apply<kInclusive>, dim3(blocks, arrays), dim3(kScanThreads), 0, stream, a, n,
                        partials);
     MFH_HIP(hipGetLastError());
 }
@@ -287,10 +299,15 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const u64* __restrict
 }  // namespace
 
 void scan_inclusive_i32(const int32_t* in, int32_t* out, int n, BuildScratch& s, hipStream_t stream) {
-    scan_i32<true>(in, out, n, s, stream);
+    scan_i32<true>(ScanArrays{{in, in}, {out, out}}, 1, n, s, stream);
+}
+void scan_inclusive_i32_pair(int32_t* a, int32_t* b, int n, BuildScratch& s, hipStream_t stream) {
+    scan_i32<true>(ScanArrays{{a, b}, {a, b}}, 2, n, s, stream);
 }
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, int n, BuildScratch& s, hipStream_t stream) {
-    scan_i32<false>(reinterpret_cast<const int*>(in), reinterpret_cast<int*>(out), n, s, stream);
+    const int* i = reinterpret_cast<const int*>(in);
+    int* o = reinterpret_cast<int*>(out);
+    scan_i32<false>(ScanArrays{{i, i}, {o, o}}, 1, n, s, stream);
 }
 
 size_t radix_sort_hist_bytes(int n, int beginBit, int endBit) {
