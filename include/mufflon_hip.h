@@ -100,7 +100,7 @@ int mfh_lbvh_get(mfh_context* ctx, int32_t lodId, int32_t* outNumInternal, void*
  * 64 * (numTriangles - 1) bytes; may be NULL for one triangle), the primitive
  * ids in BVH order into dPrimIds (numTriangles int32) and the number of
  * internal nodes into *outNumInternal; returns after the build has finished.
- * Its scratch is one static device array per GPU (44.3 MiB, held until the
+ * Its scratch is one static device array per GPU (41.7 MiB, held until the
  * process ends), so at most MFH_NOALLOC_MAX_TRIANGLES triangles; larger LoDs
  * and out-of-range vertex indices are rejected with
  * MFH_ERROR_INVALID_ARGUMENT. Every call on one GPU shares that array:
