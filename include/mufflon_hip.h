@@ -448,6 +448,28 @@ int mfh_pt_wireframe_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations,
                              int32_t maxTraceDepth);
 int mfh_pt_wireframe_get(mfh_context* ctx, int32_t ptId, int32_t variance, float* hostOut);
 
+/* Light tracer over a path tracer's scene, camera and lights: the GPU device
+ * of Mufflon's "Lighttracer"/"LT" renderer (GpuLightTracer, gpu_lt.cpp:25-37,
+ * gpu_lt.cu:19-101, lt_sample of lt_common.hpp:17-78). An iteration traces
+ * width * height light paths: path i emits from the light tree with index i
+ * (emit, light_tree_sampling.hpp:81-187), walks on adjoint BSDF samples up
+ * to the path tracer's maximum path length, and every vertex between its
+ * minimum and maximum length is connected to the pinhole camera and added to
+ * the pixel it projects to (float atomics: sums agree with a sequential
+ * evaluation to rounding). Targets (LtTargets, lt_params.hpp:13-17):
+ * MFH_LT_RADIANCE (3 floats per pixel), MFH_LT_LIGHTNESS (1), MFH_LT_DENSITY
+ * (1, recorded when `density` is set: it also tests the visibility of
+ * connections that carry no radiance). Seeded from the path tracer's seed as
+ * the reference seeds it. Positional lights only: a background that emits is
+ * refused. _reset clears the targets and re-seeds; _get copies the running
+ * mean (variance = 0) or variance (1) of one target to host memory. */
+#define MFH_LT_RADIANCE 0
+#define MFH_LT_LIGHTNESS 1
+#define MFH_LT_DENSITY 2
+int mfh_pt_lighttracer_reset(mfh_context* ctx, int32_t ptId, int32_t recordVariance);
+int mfh_pt_lighttracer_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations, int32_t density);
+int mfh_pt_lighttracer_get(mfh_context* ctx, int32_t ptId, int32_t target, int32_t variance, float* hostOut);
+
 /* Adjoint (path replay) of the Radiance target. No counterpart in the
  * reference (it has no differentiable renderer). For the image of
  * `iterations` iterations from a reset, and radianceAdjoint = dLoss/dRadiance

@@ -22,7 +22,7 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "pt_adjoint.hip", "wireframe.hip",
+SOURCES = ["sort_scan.hip", "lbvh.hip", "trace.hip", "pt.hip", "pt_wavefront.hip", "pt_adjoint.hip", "wireframe.hip", "lt.hip",
            "capi.cpp",
            "light_tree.cpp", "pt_host.cpp", "core_api.cpp"]
 

@@ -129,5 +129,11 @@ void launch_pt_iterate(const PtScene& S, int leafBatch, hipStream_t stream);
 // (and S.var[...] when recorded), S.iteration the iteration index.
 void launch_wireframe(const PtScene& S, const uint32_t* seeds, float lineWidth, int maxTraceDepth,
                       hipStream_t stream);
+// One iteration of the light tracer: clears the 5-channel splat image
+// (Radiance, Lightness, Density per pixel), traces width * height light
+// paths from S.rngs and folds the splats into mean (and var when non-null)
+// as iteration S.iteration.
+void launch_lighttracer(const PtScene& S, float* splat, float* mean, float* var, int minPathLength,
+                        int maxPathLength, bool density, hipStream_t stream);
 
 }  // namespace mfh

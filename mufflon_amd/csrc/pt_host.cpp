@@ -55,6 +55,12 @@ struct PathTracer {
     uint64_t wireRng = 0x2f0ae9bc6431af73ull;
     int32_t wireIteration = -1;
     bool wireVariance = false;
+    // light tracer over the same scene (GpuLightTracer, gpu_lt.cpp:25-37): its
+    // generators, the splat image of an iteration and the running means (and
+    // variances) of Radiance, Lightness, Density, 5 floats per pixel
+    DevBuf ltRngs, ltSplat, ltMean, ltVar;
+    int32_t ltIteration = -1;
+    bool ltVariance = false;
 };
 
 void drop_graph(PathTracer& t) {
@@ -729,6 +735,78 @@ int mfh_pt_wireframe_get(mfh_context* ctx, int32_t ptId, int32_t variance, float
         MFH_REQUIRE(t.wireBorder.bytes >= 12 * num, "the wireframe renderer has not run");
         MFH_HIP(hipStreamSynchronize(c.stream));
         MFH_HIP(hipMemcpy(out, (variance ? t.wireVar : t.wireBorder).ptr, 12 * num, hipMemcpyDeviceToHost));
+    });
+}
+
+namespace {
+constexpr int kLtChannels = 5;
+void lighttracer_reset(Context& c, PathTracer& t, bool recordVariance) {
+    const size_t num = (size_t)t.camera.width * t.camera.height;
+    t.ltIteration = -1;
+    t.ltVariance = recordVariance;
+    t.ltMean.reserve(4 * kLtChannels * num);
+    MFH_HIP(hipMemsetAsync(t.ltMean.ptr, 0, 4 * kLtChannels * num, c.stream));
+    if (recordVariance) {
+        t.ltVar.reserve(4 * kLtChannels * num);
+        MFH_HIP(hipMemsetAsync(t.ltVar.ptr, 0, 4 * kLtChannels * num, c.stream));
+    } else {
+        t.ltVar.release();
+    }
+    t.ltSplat.reserve(4 * kLtChannels * num);
+    // GpuLightTracer::post_reset (gpu_lt.cpp:31-36): generator i seeded with
+    // i + seed * (pixels + 1), as the path tracer's
+    t.ltRngs.reserve(8 * num);
+    launch_pt_init_rng(t.ltRngs.as<uint64_t>(), (int)num, t.params.seed * ((int)num + 1), c.stream);
+}
+}  // namespace
+
+int mfh_pt_lighttracer_reset(mfh_context* ctx, int32_t ptId, int32_t recordVariance) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        lighttracer_reset(c, tracer_of(c, ptId), recordVariance != 0);
+    });
+}
+
+int mfh_pt_lighttracer_iterate(mfh_context* ctx, int32_t ptId, int32_t iterations, int32_t density) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(iterations >= 0, "negative iteration count");
+        MFH_REQUIRE(t.tree.posLightCount > 0, "the light tracer needs a point or area light");
+        MFH_REQUIRE(!(t.background.fluxSum > 0.0f),
+                    "the light tracer emits from point and area lights only; the background emits");
+        const size_t num = (size_t)t.camera.width * t.camera.height;
+        if (t.ltMean.bytes < 4 * kLtChannels * num || t.ltRngs.bytes < 8 * num ||
+            (t.ltVariance && t.ltVar.bytes < 4 * kLtChannels * num))
+            lighttracer_reset(c, t, t.ltVariance);  // first use, or the resolution grew
+        PtScene S = device_scene(c, t);
+        S.rngs = t.ltRngs.as<uint64_t>();
+        for (int32_t it = 0; it < iterations; ++it) {
+            S.iteration = ++t.ltIteration;
+            launch_lighttracer(S, t.ltSplat.as<float>(), t.ltMean.as<float>(),
+                               t.ltVariance ? t.ltVar.as<float>() : nullptr, t.params.minPathLength,
+                               t.params.maxPathLength, density != 0, c.stream);
+        }
+        MFH_HIP(hipStreamSynchronize(c.stream));
+    });
+}
+
+int mfh_pt_lighttracer_get(mfh_context* ctx, int32_t ptId, int32_t target, int32_t variance, float* out) {
+    return guarded([&] {
+        Context& c = ctx_of(ctx);
+        PathTracer& t = tracer_of(c, ptId);
+        MFH_REQUIRE(out != nullptr, "null output pointer");
+        MFH_REQUIRE(target >= MFH_LT_RADIANCE && target <= MFH_LT_DENSITY, "unknown light tracer target");
+        MFH_REQUIRE(!variance || t.ltVariance, "the light tracer's variance is not recorded");
+        const size_t num = (size_t)t.camera.width * t.camera.height;
+        MFH_REQUIRE(t.ltIteration >= 0 && t.ltMean.bytes >= 4 * kLtChannels * num, "the light tracer has not run");
+        std::vector<float> all((size_t)kLtChannels * num);
+        MFH_HIP(hipStreamSynchronize(c.stream));
+        MFH_HIP(hipMemcpy(all.data(), (variance ? t.ltVar : t.ltMean).ptr, 4 * all.size(), hipMemcpyDeviceToHost));
+        const int first = target == MFH_LT_RADIANCE ? 0 : (target == MFH_LT_LIGHTNESS ? 3 : 4);
+        const int nc = target == MFH_LT_RADIANCE ? 3 : 1;
+        for (size_t i = 0; i < num; ++i)
+            for (int k = 0; k < nc; ++k) out[nc * i + k] = all[kLtChannels * i + first + k];
     });
 }
 

@@ -330,7 +330,7 @@ __device__ __forceinline__ float fresnel_dielectric(float n_i, float n_t, float 
 // Tangent-space BSDF sample of lambert.hpp:18-35, microfacet_specular.hpp:26-83,
 // microfacet_full.hpp:35-126. Returns false for an invalid sample.
 __device__ bool sample_ts(const MatParams& p, V3 in, float u0, float u1, u64 rnd, V3& throughput, V3& ex,
-                          float& pdfF) {
+                          float& pdfF, bool adjoint = false) {
     if (p.type == kMatLambert) {
         const float cosT = fsqrt(u0);
         const float sinT = fsqrt(1.0f - u0);
@@ -381,6 +381,7 @@ __device__ bool sample_ts(const MatParams& p, V3 in, float u0, float u1, u64 rnd
             pdfF = cavityPdf * sdiv(gi, fabsf(4.0f * in.z * h.z));
         } else {
             tp = sdiv(g_transmission(p.shadowing, gi, ge), gi);
+            if (adjoint) tp *= eta * eta;  // light sub-paths (microfacet_refractive.hpp:109-110)
             const float common = cavityPdf * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_t * eDotH) * h.z);
             pdfF = common * sdiv(gi * n_t * n_t, fabsf(in.z));
         }
@@ -408,6 +409,7 @@ __device__ bool sample_ts(const MatParams& p, V3 in, float u0, float u1, u64 rnd
         pdfF = common * sdiv(gi, fabsf(in.z));
     } else {
         tp = sdiv(g_transmission(p.shadowing, gi, ge), gi);
+        if (adjoint) tp *= eta * eta;  // light sub-paths (microfacet_full.hpp:114-115)
         const float common = cavityPdf * (1.0f - f) * sdiv(iDotHabs * cosTAbs, sq(n_i * iDotH + n_t * eDotH) * h.z);
         pdfF = common * sdiv(gi * n_t * n_t, fabsf(in.z));
     }
@@ -480,21 +482,28 @@ struct TS {
     V3 shadingN, geoN, shadingTX, shadingTY;
 };
 
-// materials::sample (material_sampling.hpp:68-146), view paths. On success
-// writes the world-space excident, throughput and forward pdf.
+constexpr float kShadingNormalEps = 1e-3f;  // SHADING_NORMAL_EPS (material_sampling.hpp:56)
+
+// materials::sample (material_sampling.hpp:68-146): view paths, or with
+// `adjoint` light sub-paths (refraction scaled by eta^2, and the shading
+// normal correction). On success writes the world-space excident, throughput
+// and forward pdf.
 __device__ __forceinline__ bool mat_sample(const TS& ts, const MatParams& m, V3 incident, float u0, float u1, u64 i0,
-                                           V3& throughput, V3& excident, float& pdfF) {
+                                           V3& throughput, V3& excident, float& pdfF, bool adjoint = false) {
     const float iDotG = -dot(incident, ts.geoN);
     const float iDotN = -dot(incident, ts.shadingN);
     if (iDotG * iDotN <= 0.0f) return false;
     const V3 inTS = v3(-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN);
     V3 ex;
-    if (!sample_ts(m, inTS, u0, u1, i0, throughput, ex, pdfF)) return false;
+    if (!sample_ts(m, inTS, u0, u1, i0, throughput, ex, pdfF, adjoint)) return false;
     if (!nonzero(throughput)) return false;
     const V3 g = normalize(smul(ex.x, ts.shadingTX) + smul(ex.y, ts.shadingTY) + smul(ex.z, ts.shadingN));
-    if (dot(g, ts.geoN) * ex.z <= 0.0f) return false;
+    const float eDotG = dot(g, ts.geoN);
+    if (eDotG * ex.z <= 0.0f) return false;
     excident = g;
     pdfF = mn(pdfF, 1e9f);
+    if (adjoint)
+        throughput = throughput * ((kShadingNormalEps + fabsf(iDotN * eDotG)) / (kShadingNormalEps + fabsf(iDotG * ex.z)));
     return true;
 }
 
@@ -518,6 +527,29 @@ __device__ __forceinline__ V3 mat_evaluate(const TS& ts, const MatParams& m, V3 
     if (!nonzero(value)) return v3(0, 0, 0);
     cosOut = fabsf(eDotN);
     pdfF = mn(pdf, 1e9f);
+    return value;
+}
+
+// materials::evaluate with adjoint = true (material_sampling.hpp:160-237), the
+// connections of light sub-paths: the BSDF of the swapped pair times the
+// shading normal correction; cosOut = |eDotN|; 0 for rejected pairs.
+__device__ __forceinline__ V3 mat_evaluate_adjoint(const TS& ts, const MatParams& m, V3 incident, V3 excident,
+                                                   float& cosOut) {
+    cosOut = 0.0f;
+    const float iDotN = -dot(incident, ts.shadingN);
+    const float eDotN = dot(excident, ts.shadingN);
+    if (!m.reflective() && iDotN * eDotN > 0.0f) return v3(0, 0, 0);
+    if (!m.refractive() && iDotN * eDotN < 0.0f) return v3(0, 0, 0);
+    const float iDotG = -dot(incident, ts.geoN);
+    const float eDotG = dot(excident, ts.geoN);
+    if (eDotG * eDotN <= 0.0f || iDotG * iDotN <= 0.0f) return v3(0, 0, 0);
+    const V3 inTS = v3(-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN);
+    const V3 exTS = v3(dot(excident, ts.shadingTX), dot(excident, ts.shadingTY), eDotN);
+    float pdf;
+    V3 value = evaluate_ts(m, exTS, inTS, pdf);
+    if (!nonzero(value)) return v3(0, 0, 0);
+    value = value * ((kShadingNormalEps + fabsf(iDotN * eDotG)) / (kShadingNormalEps + fabsf(iDotG * eDotN)));
+    cosOut = fabsf(eDotN);
     return value;
 }
 

@@ -55,6 +55,7 @@ TARGET_UV = 5
 TARGET_ALBEDO = 6
 TARGET_LIGHTNESS = 7
 TARGET_CHANNELS = (3, 2, 3, 1, 3, 2, 3, 1)
+LT_RADIANCE, LT_LIGHTNESS, LT_DENSITY = 0, 1, 2
 PT_MEGAKERNEL = 0
 PT_WAVEFRONT = 1
 TUNING_DTYPE = np.dtype([("traceBlock", "<i4"), ("stackDepth", "<i4"), ("queueChunk", "<i4"), ("sortBounces", "<i4"),
@@ -89,6 +90,9 @@ _SIGS = {
     "mfh_pt_wireframe_reset": [vp, i32, i32],
     "mfh_pt_wireframe_iterate": [vp, i32, i32, C.c_float, i32],
     "mfh_pt_wireframe_get": [vp, i32, i32, vp],
+    "mfh_pt_lighttracer_reset": [vp, i32, i32],
+    "mfh_pt_lighttracer_iterate": [vp, i32, i32, i32],
+    "mfh_pt_lighttracer_get": [vp, i32, i32, i32, vp],
     "mfh_pt_get_target": [vp, i32, i32, f32p],
     "mfh_pt_enable_target": [vp, i32, i32, i32],
     "mfh_pt_enable_variance": [vp, i32, i32, i32],
@@ -497,6 +501,21 @@ class HipContext:
         self._call("mfh_pt_wireframe_get", self.handle, pt, 1 if variance else 0, out.ctypes.data)
         return out.reshape(h, w, 3)
 
+
+    # light tracer over a path tracer's scene, camera and lights
+    def lighttracer_reset(self, pt, variance=False):
+        self._call("mfh_pt_lighttracer_reset", self.handle, pt, 1 if variance else 0)
+
+    def lighttracer_iterate(self, pt, iterations=1, density=False):
+        self._call("mfh_pt_lighttracer_iterate", self.handle, pt, iterations, 1 if density else 0)
+
+    def lighttracer_target(self, pt, target=LT_RADIANCE, variance=False):
+        """Running mean (or variance) of a light tracer target, (h, w, channels)."""
+        w, h = self._pt_size[pt]
+        nc = 3 if target == LT_RADIANCE else 1
+        out = np.zeros(w * h * nc, np.float32)
+        self._call("mfh_pt_lighttracer_get", self.handle, pt, target, 1 if variance else 0, out.ctypes.data)
+        return out.reshape(h, w, nc)
     def batch_create(self, pts, use_graph=False):
         ids = np.ascontiguousarray(pts, np.int32)
         out = i32()

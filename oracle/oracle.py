@@ -683,6 +683,23 @@ def wireframe_render(pt_scene, iterations, line_width=1.0, max_trace_depth=1000,
     return border.reshape(h, w, 3), (var.reshape(h, w, 3) if variance else None), int(state[0])
 
 
+def lt_render(pt_scene, seed, min_path_length, max_path_length, iterations, density=False, variance=False):
+    """Oracle light tracer: (means, variances or None), each (h, w, 5) with
+    Radiance (3), Lightness and Density per pixel."""
+    lib = load()
+    if not hasattr(lib, "_lt_ready"):
+        lib.orc_lt_render.argtypes = [C.POINTER(OrcPtScene), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      f32p, f32p]
+        lib.orc_lt_render.restype = C.c_int
+        lib._lt_ready = True
+    w, h = pt_scene.s.width, pt_scene.s.height
+    mean = np.zeros(w * h * 5, np.float32)
+    var = np.zeros(w * h * 5, np.float32) if variance else None
+    lib.orc_lt_render(C.byref(pt_scene.s), seed, min_path_length, max_path_length, iterations, 1 if density else 0,
+                      _p(mean, f32p), _p(var, f32p) if variance else None)
+    return mean.reshape(h, w, 5), (var.reshape(h, w, 5) if variance else None)
+
+
 def pt_render_replay(base_scene, eval_scene, seed, min_path_length, max_path_length, nee_count, iterations):
     """Radiance (h, w, 3) of eval_scene rendered along the BSDF directions the
     same paths sampled in base_scene (the estimator whose roughness

@@ -540,7 +540,7 @@ TsSample torrance_sample(const MatParams& p, V3 in, float u0, float u1, u64 i0) 
 
 // MICROFACET sample (microfacet_full.hpp:35-126)
 TsSample microfacet_sample(const MatParams& p, V3 in, const Medium& incident, const Medium& other, float u0,
-                           float u1, u64 i0) {
+                           float u1, u64 i0, bool adjoint = false) {
     TsSample r;
     u64 rnd = i0;
     float cavityPdf, iDotH;
@@ -575,6 +575,7 @@ TsSample microfacet_sample(const MatParams& p, V3 in, const Medium& incident, co
     } else {
         const float g = g_transmission(p.shadowing, gi, ge);
         throughput = sdiv(g, gi);
+        if (adjoint) throughput *= eta * eta;  // light sub-paths (microfacet_full.hpp:114-115)
         const float common =
             cavityPdf * (1.0f - f.f) * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_t * eDotH) * h.z);
         pdfF = common * sdiv(gi * n_t * n_t, std::fabs(in.z));
@@ -592,7 +593,7 @@ TsSample microfacet_sample(const MatParams& p, V3 in, const Medium& incident, co
 // about the sampled half vector, or its mirror under total internal
 // reflection; no Fresnel choice (the view-path throughput has no eta^2).
 TsSample walter_sample(const MatParams& p, V3 in, const Medium& incident, const Medium& other, float u0, float u1,
-                       u64 i0) {
+                       u64 i0, bool adjoint = false) {
     TsSample r;
     u64 rnd = i0;
     float cavityPdf, iDotH;
@@ -625,6 +626,7 @@ TsSample walter_sample(const MatParams& p, V3 in, const Medium& incident, const 
         pdfB = cavityPdf * sdiv(ge, std::fabs(4.0f * ex.z * h.z));
     } else {
         throughput = sdiv(g_transmission(p.shadowing, gi, ge), gi);
+        if (adjoint) throughput *= eta * eta;  // light sub-paths (microfacet_refractive.hpp:109-110)
         const float common = cavityPdf * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_e * eDotH) * h.z);
         pdfF = common * sdiv(gi * n_e * n_e, std::fabs(in.z));
         pdfB = common * sdiv(ge * n_i * n_i, std::fabs(ex.z));
@@ -652,14 +654,14 @@ TsSample lambert_sample(const MatParams& p, V3 in, float u0, float u1) {
     return r;
 }
 
-TsSample sample_ts(const MatParams& p, V3 inTS, float u0, float u1, u64 i0) {
+TsSample sample_ts(const MatParams& p, V3 inTS, float u0, float u1, u64 i0, bool adjoint = false) {
     const Medium& incident = p.medium(inTS.z);
     const Medium& other = p.medium(-inTS.z);
     switch (p.type) {
         case kLambert: return lambert_sample(p, inTS, u0, u1);
         case kTorrance: return torrance_sample(p, inTS, u0, u1, i0);
-        case kWalter: return walter_sample(p, inTS, incident, other, u0, u1, i0);
-        case kMicrofacet: return microfacet_sample(p, inTS, incident, other, u0, u1, i0);
+        case kWalter: return walter_sample(p, inTS, incident, other, u0, u1, i0, adjoint);
+        case kMicrofacet: return microfacet_sample(p, inTS, incident, other, u0, u1, i0, adjoint);
         default: return TsSample{};  // EMISSIVE: not sampleable
     }
 }
@@ -761,13 +763,16 @@ struct PathSample {
 };
 
 // materials::sample (material_sampling.hpp:68-146), view paths (adjoint = false)
-PathSample mat_sample(const TS& ts, const MatParams& m, V3 incident, float u0, float u1, u64 i0) {
+constexpr float kShadingNormalEps = 1e-3f;  // SHADING_NORMAL_EPS (material_sampling.hpp:56)
+
+PathSample mat_sample(const TS& ts, const MatParams& m, V3 incident, float u0, float u1, u64 i0,
+                      bool adjoint = false) {
     PathSample r{false, {0, 0, 0}, {0, 0, 0}, 0, 0};
     const float iDotG = -dot(incident, ts.geoN);
     const float iDotN = -dot(incident, ts.shadingN);
     if (iDotG * iDotN <= 0.0f) return r;
     const V3 inTS{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN};
-    const TsSample s = sample_ts(m, inTS, u0, u1, i0);
+    const TsSample s = sample_ts(m, inTS, u0, u1, i0, adjoint);
     if (!s.valid || !nonzero(s.throughput)) return r;
     V3 g = s.excident.x * ts.shadingTX + s.excident.y * ts.shadingTY + s.excident.z * ts.shadingN;
     g = normalize(g);
@@ -778,6 +783,10 @@ PathSample mat_sample(const TS& ts, const MatParams& m, V3 incident, float u0, f
     r.excident = g;
     r.pdfF = mn(s.pdfF, 1e9f);
     r.pdfB = mn(s.pdfB, 1e9f);
+    // shading normal correction of light sub-paths (material_sampling.hpp:131-135)
+    if (adjoint)
+        r.throughput = r.throughput * ((kShadingNormalEps + std::fabs(iDotN * eDotG)) /
+                                       (kShadingNormalEps + std::fabs(iDotG * eDotN)));
     r.valid = true;
     return r;
 }
@@ -806,6 +815,30 @@ EvalValue mat_evaluate(const TS& ts, const MatParams& m, V3 incident, V3 exciden
     r.cosOut = std::fabs(eDotN);
     r.pdfF = mn(e.pdfF, 1e9f);
     r.pdfB = mn(e.pdfB, 1e9f);
+    return r;
+}
+
+// materials::evaluate with adjoint = true (material_sampling.hpp:160-237):
+// the connections of light sub-paths evaluate the swapped pair and apply the
+// shading normal correction; cosOut = |eDotN|.
+EvalValue mat_evaluate_adjoint(const TS& ts, const MatParams& m, V3 incident, V3 excident) {
+    EvalValue z{{0, 0, 0}, 0, 0, 0};
+    const float iDotN = -dot(incident, ts.shadingN);
+    const float eDotN = dot(excident, ts.shadingN);
+    if (!m.reflective() && iDotN * eDotN > 0.0f) return z;
+    if (!m.refractive() && iDotN * eDotN < 0.0f) return z;
+    const float iDotG = -dot(incident, ts.geoN);
+    const float eDotG = dot(excident, ts.geoN);
+    if (eDotG * eDotN <= 0.0f || iDotG * iDotN <= 0.0f) return z;
+    const V3 inTS{-dot(incident, ts.shadingTX), -dot(incident, ts.shadingTY), iDotN};
+    const V3 exTS{dot(excident, ts.shadingTX), dot(excident, ts.shadingTY), eDotN};
+    const TsEval e = evaluate_ts(m, exTS, inTS);
+    if (!nonzero(e.value)) return z;
+    EvalValue r;
+    r.value = e.value * ((kShadingNormalEps + std::fabs(iDotN * eDotG)) / (kShadingNormalEps + std::fabs(iDotG * eDotN)));
+    r.cosOut = std::fabs(eDotN);
+    r.pdfF = mn(e.pdfB, 1e9f);
+    r.pdfB = mn(e.pdfF, 1e9f);
     return r;
 }
 
@@ -2071,6 +2104,259 @@ void pt_adjoint_path(const OrcPtScene& S, const int params[4], int px, int py, P
     }
 }
 
+
+// ---------------------------------------------------------------- light tracer
+// lt_sample (lt_common.hpp:17-78) for trees of positional lights.
+struct LtEmitter {  // Emitter (light_sampling.hpp:21-52)
+    V3 position, intensity, normal, absorption;
+    bool area;
+};
+
+// sample_light_pos (light_sampling.hpp:80-199): point, triangle, quad, sphere.
+LtEmitter light_position(const OrcPtScene& S, uint16_t type, const uint8_t* light, float u0, float u1) {
+    LtEmitter e;
+    e.absorption = {0, 0, 0};
+    if (type == 0) {  // point
+        PointLightRec L;
+        std::memcpy(&L, light, sizeof(L));
+        e.position = {L.position[0], L.position[1], L.position[2]};
+        e.intensity = {L.intensity[0], L.intensity[1], L.intensity[2]};
+        e.normal = {0, 0, 0};
+        e.area = false;
+        return e;
+    }
+    e.area = true;
+    if (type == 3) {  // area quad
+        QuadLight L;
+        std::memcpy(&L, light, sizeof(L));
+        const V3 pv[4] = {{L.posV[0][0], L.posV[0][1], L.posV[0][2]}, {L.posV[1][0], L.posV[1][1], L.posV[1][2]},
+                          {L.posV[2][0], L.posV[2][1], L.posV[2][2]}, {L.posV[3][0], L.posV[3][1], L.posV[3][2]}};
+        const V3 tangentX = pv[1] + u0 * pv[3];
+        const V3 tangentY = pv[2] + u1 * pv[3];
+        const V3 normal = cross(tangentY, tangentX);
+        const float area = len(normal);
+        e.normal = normal / area;
+        e.position = pv[0] + tangentX * u1 + pv[2] * u0;
+        const float lu = L.uvV[0][0] + L.uvV[1][0] * u1 + L.uvV[2][0] * u0 + L.uvV[3][0] * (u0 * u1);
+        const float lv = L.uvV[0][1] + L.uvV[1][1] * u1 + L.uvV[2][1] * u0 + L.uvV[3][1] * (u0 * u1);
+        const MatParams m = fetch(S, S.materials[L.material], lu, lv);
+        e.intensity = mat_emission(m, {0, 0, 1}, {0, 0, 1}) * area;
+        e.absorption = m.outer.absorption;
+        return e;
+    }
+    if (type == 4) {  // area sphere
+        SphereLight L;
+        std::memcpy(&L, light, sizeof(L));
+        // sample_dir_sphere_uniform (sampling.hpp:101-109)
+        const float cosTheta = u0 * 2.0f - 1.0f;
+        const float sinTheta = std::sqrt((1.0f - cosTheta) * (1.0f + cosTheta));
+        const float phi = u1 * 2.0f * kPi;
+        const V3 n{sinTheta * fsin(phi), sinTheta * fcos(phi), cosTheta};
+        const MatParams m = fetch(S, S.materials[L.material], u1, facos(n.z));  // uv (u1, acos(n.z))
+        const V3 radiance = mat_emission(m, {0, 0, 1}, {0, 0, 1});
+        e.absorption = m.outer.absorption;
+        const float area = 4 * kPi * L.radius * L.radius;
+        e.position = V3{L.position[0], L.position[1], L.position[2]} + n * L.radius;
+        e.normal = n;
+        e.intensity = radiance * area;
+        return e;
+    }
+    TriLight L;
+    std::memcpy(&L, light, sizeof(L));
+    const V3 p0{L.posV[0][0], L.posV[0][1], L.posV[0][2]};
+    const V3 e1{L.posV[1][0], L.posV[1][1], L.posV[1][2]}, e2{L.posV[2][0], L.posV[2][1], L.posV[2][2]};
+    const V3 normal = cross(e1, e2);
+    const float area2 = len(normal);
+    e.normal = normal / area2;
+    const float area = area2 * 0.5f;
+    const float bs = 1.f - std::sqrt(1.f - u0);  // sample_barycentric (sampling.hpp:123-128)
+    const float bt = (1.f - bs) * u1;
+    e.position = p0 + e1 * bs + e2 * bt;
+    const float lu = L.uvV[0][0] + L.uvV[1][0] * bs + L.uvV[2][0] * bt;
+    const float lv = L.uvV[0][1] + L.uvV[1][1] * bs + L.uvV[2][1] * bt;
+    const MatParams m = fetch(S, S.materials[L.material], lu, lv);
+    e.intensity = mat_emission(m, {0, 0, 1}, {0, 0, 1}) * area;
+    e.absorption = m.outer.absorption;
+    return e;
+}
+
+// emit (light_tree_sampling.hpp:81-187) without emitting background or
+// directional lights: the tree is descended on flux alone and the intensity
+// divided by the chosen light's probability (adjustPdf).
+LtEmitter emit(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, float u0, float u1) {
+    u64 rndChoice = numIndices > 0 ? seed + index * (~0ull / numIndices) : seed;
+    if (rndChoice == ~0ull) --rndChoice;
+    const float fluxSum = S.bgFlux + S.dirRootFlux + S.posRootFlux;
+    const float envProb = S.bgFlux / fluxSum;
+    const float dirProb = S.dirRootFlux / fluxSum;
+    const float posProb = S.posRootFlux / fluxSum;
+    u64 left = percentage_of(~0ull, envProb + dirProb);
+    u64 right = ~0ull;
+    float lightProb = posProb;
+    uint16_t type = (uint16_t)S.posRootType;
+    uint32_t offset = 0;
+    while (type == kInternalNode) {
+        NodeRec node;
+        std::memcpy(&node, S.lightMem + offset, sizeof(node));
+        const float probLeft = node.leftFlux / (node.leftFlux + node.rightFlux);
+        const u64 boundary = left + percentage_of(right - left, probLeft);
+        if (rndChoice < boundary) {
+            type = node.leftType;
+            offset = node.leftOffset;
+            right = boundary;
+            lightProb *= probLeft;
+        } else {
+            type = node.rightType;
+            offset = node.rightOffset;
+            left = boundary;
+            lightProb *= (1.0f - probLeft);
+        }
+    }
+    LtEmitter e = light_position(S, type, S.lightMem + offset, u0, u1);
+    e.intensity = e.intensity / lightProb;
+    return e;
+}
+
+// pinholecam_project (pinhole.hpp:86-116)
+bool project(const OrcPtScene& S, V3 dir, int& px, int& py, float& w) {
+    const V3 viewDir{S.camDir[0], S.camDir[1], S.camDir[2]};
+    const V3 up{S.camUp[0], S.camUp[1], S.camUp[2]};
+    const float cosOut = dot(viewDir, dir);
+    if (cosOut < 0.0f) return false;
+    const V3 xAxis = cross(viewDir, up);
+    const float den = cosOut * S.tanVFov;
+    float ux = dot(xAxis, dir) / den;
+    const float uy = dot(up, dir) / den;
+    const float aspect = (float)S.width / (float)S.height;
+    ux /= aspect;
+    if (!(ux > -1 && ux <= 1 && uy > -1 && uy <= 1)) return false;
+    px = (int)std::floor((ux * 0.5f + 0.5f) * (float)S.width);
+    py = (int)std::floor((uy * 0.5f + 0.5f) * (float)S.height);
+    if (px >= S.width) px = S.width - 1;
+    if (py >= S.height) py = S.height - 1;
+    const float pixelArea = (2.0f * S.tanVFov) * (2.0f * S.tanVFov) * aspect;
+    w = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
+    return true;
+}
+
+// One light path: splats (Radiance, Lightness, Density) into img, 5 per pixel.
+void lt_path(const OrcPtScene& S, int minPL, int maxPL, bool density, int index, Pcg64& rng, float* img) {
+    const int num = S.width * S.height;
+    const V3 camPos{S.camPos[0], S.camPos[1], S.camPos[2]};
+    V3 throughput{1.0f, 1.0f, 1.0f};
+    const u64 xStart = rng.next();
+    const u64 treeSeed = rng.next();
+    const LtEmitter light = emit(S, (u64)index, (u64)num, treeSeed, sample_uniform((u32)xStart),
+                                 sample_uniform((u32)(xStart >> 32)));
+    rng.next();  // create_camera's RndSet2
+    // the current surface vertex
+    V3 pos{0, 0, 0}, incident{0, 0, 0};
+    TS ts;
+    MatParams mat;
+    int pathLen = 0;
+    do {
+        if (pathLen + 1 >= minPL && pathLen + 1 <= maxPL) {
+            const V3 vpos = pathLen == 0 ? light.position : pos;
+            const V3 conn = vpos - camPos;
+            const float distSq = dot(conn, conn);
+            const float dist = std::sqrt(distSq);
+            const V3 dir = conn / (dist + 1e-20f * sgnf(dist));  // sdiv (get_connection, path_util.hpp:384-399)
+            int px, py;
+            float w;
+            if (project(S, dir, px, py, w)) {
+                V3 lval, geoN;
+                float lcos, geomFactor;
+                if (pathLen == 0) {
+                    if (light.area) {  // evaluate_area (lights.hpp:286-295)
+                        const float c = dot(light.normal, -1.0f * dir);
+                        lval = c <= 0.0f ? V3{0, 0, 0} : light.intensity;
+                        lcos = c <= 0.0f ? 0.0f : c;
+                        geoN = light.normal;
+                        geomFactor = dot(dir, light.normal);
+                    } else {  // evaluate_point (lights.hpp:262-265)
+                        lval = light.intensity;
+                        lcos = 1.0f;
+                        geoN = {0, 0, 0};
+                        geomFactor = 0.0f;  // get_geometric_factor has no point-light case
+                    }
+                } else {
+                    const EvalValue ev = mat_evaluate_adjoint(ts, mat, incident, -1.0f * dir);
+                    lval = ev.value;
+                    lcos = ev.cosOut;
+                    geoN = ts.geoN;
+                    geomFactor = dot(dir, ts.shadingN);
+                }
+                V3 bxdfProd{w * lval.x, w * lval.y, w * lval.z};
+                const float cosProd = 1.0f * lcos;
+                if (density || ((bxdfProd.x > 0.0f || bxdfProd.y > 0.0f || bxdfProd.z > 0.0f) && cosProd > 0.0f)) {
+                    if (!any_intersection(S, camPos, vpos, {0, 0, 0}, geoN, dir)) {
+                        bxdfProd = bxdfProd / distSq;
+                        const V3 rad = throughput * cosProd * bxdfProd;
+                        float* o = img + 5 * ((size_t)px + (size_t)py * S.width);
+                        o[0] += rad.x;
+                        o[1] += rad.y;
+                        o[2] += rad.z;
+                        o[3] += (throughput.x + throughput.y + throughput.z) / 3.0f * cosProd;
+                        if (density) o[4] += w * 1.0f / distSq * std::fabs(geomFactor);
+                    }
+                }
+            }
+        }
+        // walk (random_walk.hpp:38-112), adjoint
+        const u64 x0 = rng.next();
+        const u64 x1 = rng.next();
+        const float rndRoulette = sample_uniform((u32)rng.next());
+        const float u0 = sample_uniform((u32)x0), u1 = sample_uniform((u32)(x0 >> 32));
+        V3 dirOut, sThroughput, absorption, geoOut, origin;
+        if (pathLen == 0) {
+            if (light.area) {  // sample_light_dir_area (light_sampling.hpp:201-211)
+                const float cosT = std::sqrt(u0);
+                const float sinT = std::sqrt(1.0f - u0);
+                const float phi = u1 * 2 * kPi;
+                const float lx = sinT * fsin(phi), ly = sinT * fcos(phi);
+                const V3 tx = normalize(perpendicular(light.normal));
+                const V3 ty = cross(light.normal, tx);
+                dirOut = lx * tx + ly * ty + cosT * light.normal;
+                sThroughput = light.intensity * kPi;
+                geoOut = light.normal;
+            } else {  // sample_light_dir_point (light_sampling.hpp:85-89)
+                const float cosT = u0 * 2.0f - 1.0f;
+                const float sinT = std::sqrt((1.0f - cosT) * (1.0f + cosT));
+                const float phi = u1 * 2.0f * kPi;
+                dirOut = {sinT * fsin(phi), sinT * fcos(phi), cosT};
+                sThroughput = light.intensity * 4 * kPi;
+                geoOut = {0, 0, 0};
+            }
+            origin = light.position;
+            absorption = light.absorption;
+        } else {
+            const PathSample ps = mat_sample(ts, mat, incident, u0, u1, x1, true);
+            if (!ps.valid) break;
+            dirOut = ps.excident;
+            sThroughput = ps.throughput;
+            absorption = mat.medium(dot(dirOut, ts.geoN)).absorption;
+            origin = pos;
+            geoOut = ts.geoN;
+        }
+        throughput = throughput * sThroughput;
+        const float cont = mn(mx(mx(throughput.x, throughput.y), throughput.z) + 0.1f, 1.0f);
+        if (rndRoulette >= cont) break;
+        throughput = throughput / cont;
+        V3 rayOrigin = origin;
+        const HitInfo hit = first_intersection(S, rayOrigin, dirOut, geoOut, kMaxSceneSize);
+        Medium med;
+        med.ior = 1.0f;
+        med.absorption = absorption;
+        throughput = throughput * transmission(med, hit.hit ? hit.t : kMaxSceneSize);
+        if (!hit.hit) break;
+        pos = rayOrigin + dirOut * hit.t;
+        incident = dirOut;
+        ts = tangent_space(S, hit);
+        mat = fetch(S, S.materials[material_index(S, hit.inst, hit.prim)], hit.u, hit.v);
+        ++pathLen;
+    } while (pathLen < maxPL);
+}
+
 }  // namespace
 
 extern "C" {
@@ -2513,6 +2799,37 @@ int orc_background_evaluate(const OrcPtScene* S, const float* dir, float* out4) 
     out4[1] = b.value.y;
     out4[2] = b.value.z;
     out4[3] = b.pdfBack;
+    return 0;
+}
+
+// The light tracer (GpuLightTracer, gpu_lt.cpp:25-37; lt_sample,
+// lt_common.hpp:17-78): `iterations` iterations of width * height light
+// paths from generators seeded i + seed * (pixels + 1); path i's splats are
+// added in path order. out5: running means of Radiance (3), Lightness and
+// Density per pixel; outVar5 (optional) their variances.
+int orc_lt_render(const OrcPtScene* S, int32_t seed, int32_t minPathLength, int32_t maxPathLength,
+                  int32_t iterations, int32_t density, float* out5, float* outVar5) {
+    const int num = S->width * S->height;
+    std::vector<Pcg64> rngs;
+    rngs.reserve(num);
+    const int s = seed * (num + 1);
+    for (int i = 0; i < num; ++i) rngs.emplace_back((u32)(i + s));
+    std::memset(out5, 0, sizeof(float) * 5 * (size_t)num);
+    if (outVar5) std::memset(outVar5, 0, sizeof(float) * 5 * (size_t)num);
+    std::vector<float> img(5 * (size_t)num);
+    for (int it = 0; it < iterations; ++it) {
+        std::fill(img.begin(), img.end(), 0.0f);
+        for (int i = 0; i < num; ++i) lt_path(*S, minPathLength, maxPathLength, density != 0, i, rngs[i], img.data());
+        const float denom = mx(1.0f, (float)it + 1);
+        const float varDenom = mx(1.0f, (float)it);
+        for (size_t k = 0; k < img.size(); ++k) {
+            const float iter = img[k];
+            const float diff = iter - out5[k];
+            const float cum = out5[k] + diff / denom;
+            out5[k] = cum;
+            if (outVar5) outVar5[k] += (diff * (iter - cum) - outVar5[k]) / varDenom;
+        }
+    }
     return 0;
 }
 
