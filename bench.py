@@ -204,7 +204,8 @@ def cornell_tracer(ctx):
 def pt_cornell(args, cpu_baseline_too):
     """Megakernel path tracer on the procedural Cornell box: 256x256, 128x256
     sphere, depth 8, NEE 1, seed 1; Mpaths/s = pixels * iterations / time.
-    Also the megakernel with leaf batch 8 and the autotuned wavefront form."""
+    Also the megakernel with leaf batch 8, the autotuned wavefront form and
+    the light tracer over the same scene."""
     from mufflon_amd import scenes as S
     from mufflon_amd.hip import HipContext
     ctx = HipContext(0)
@@ -253,6 +254,24 @@ def pt_cornell(args, cpu_baseline_too):
     stats = O.pt_path_stats(ps, 1, 0, 8, 1, 0, 0, 256, 256)
     out["roofline"] = path_roofline(stats, 64, ms / n, "k_pt_iterate (megakernel)")
     out["roofline"]["wavefront_achieved"] = round(path_bytes(stats, 64) / (wms / n / 1e3) / 1e9, 1)
+    # the light tracer over the same scene, camera and lights: 65,536 light
+    # paths per iteration, splats with float atomics
+    ctx.lighttracer_reset(pt)
+    ctx.lighttracer_iterate(pt, 2)
+    ctx.lighttracer_reset(pt)
+    ctx.timer_begin()
+    ctx.lighttracer_iterate(pt, n)
+    lms = ctx.timer_end()
+    out["lighttracer"] = {"ms_per_iteration": round(lms / n, 4),
+                          "mpaths_s": round(pixels * n / (lms / 1e3) / 1e6, 3),
+                          "paths": "65,536 light paths per iteration, depth 8"}
+    if cpu_baseline_too:
+        dt_lt = median_time(lambda: O.lt_render(ps, 1, 0, 8, 1))
+        out["lighttracer"]["cpu_baseline"] = {
+            "value": round(pixels / dt_lt / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": "one iteration (65,536 light paths) of the oracle's sequential light tracer, median of %d "
+                      "runs: %.3f s" % (CPU_RUNS, dt_lt)}
     if cpu_baseline_too:
         k = 8
         dt = median_time(lambda: O.pt_render(ps, 1, 0, 8, 1, k))
