@@ -213,7 +213,10 @@ struct WireParams {
 };
 constexpr ParamDescription kWireParams[] = {{"Line width", PARAM_FLOAT}, {"Maximum trace depth", PARAM_INT}};
 constexpr int kNumWireParams = 2;
-constexpr int kRendererPt = 0, kRendererWire = 1;
+constexpr int kRendererPt = 0, kRendererWire = 1, kRendererLt = 2;
+// The "Lighttracer" renderer's parameters (LtParameters, lt_params.hpp:8-12):
+// the first three of kParams.
+constexpr int kNumLtParams = 3;
 
 // PtTargets order (pt_params.hpp:22-26) and the device target ids.
 struct TargetName {
@@ -227,13 +230,19 @@ constexpr TargetName kTargets[] = {{"Radiance", MFH_TARGET_RADIANCE, 3}, {"Posit
                                    {"Lightness", MFH_TARGET_LIGHTNESS, 1}, {"Hit ID", MFH_TARGET_HIT_ID, 2}};
 constexpr int kNumTargetNames = 8;
 constexpr TargetName kWireTargets[] = {{"Border", -1, 3}};
+// LtTargets (lt_params.hpp:14-17)
+constexpr TargetName kLtTargets[] = {{"Radiance", MFH_LT_RADIANCE, 3}, {"Lightness", MFH_LT_LIGHTNESS, 1},
+                                     {"Density", MFH_LT_DENSITY, 1}};
+constexpr int kLtDensityTarget = 2;
 
 struct TargetTable {
     const TargetName* targets;
     int count;
 };
 TargetTable target_table(int renderer) {
-    return renderer == kRendererWire ? TargetTable{kWireTargets, 1} : TargetTable{kTargets, kNumTargetNames};
+    if (renderer == kRendererWire) return TargetTable{kWireTargets, 1};
+    if (renderer == kRendererLt) return TargetTable{kLtTargets, 3};
+    return TargetTable{kTargets, kNumTargetNames};
 }
 
 int target_index(int renderer, const char* name) {
@@ -269,6 +278,7 @@ struct Core {
     int variation = -1;  // -1: no renderer enabled
     int renderer = kRendererPt;  // kRendererPt (variations 0, 1) or kRendererWire (variation 0)
     Variation vars[2];
+    Variation ltVars;  // the light tracer's own parameters
     WireParams wire;
     uint32_t targetMask = 0, varianceMask = 0;  // bits: index into kTargets
     std::set<std::pair<std::string, bool>> toEnable, toDisable;
@@ -568,6 +578,13 @@ void wire_reset(Core& c) {
     check_mfh(mfh_pt_wireframe_reset(c.ctx, c.pt, (c.varianceMask & 1u) ? 1 : 0), "reset");
 }
 
+// The light tracer's targets: fresh running means (with variances when any
+// target records them) and re-seeded generators.
+void lt_reset(Core& c) {
+    if (c.pt < 0) return;
+    check_mfh(mfh_pt_lighttracer_reset(c.ctx, c.pt, c.varianceMask ? 1 : 0), "reset");
+}
+
 void apply_targets(Core& c) {
     if (c.pt < 0 || c.renderer != kRendererPt) return;
     for (int i = 0; i < kNumTargetNames; ++i) {
@@ -690,7 +707,8 @@ void load_scene(Core& c, Scenario& s) {
         }
     }
     const mfh_pinhole cam = device_camera(c, s);
-    const Variation& v = c.vars[c.variation >= 0 ? c.variation : 0];
+    const Variation& v = (c.renderer == kRendererLt && c.variation >= 0) ? c.ltVars
+                                                                        : c.vars[c.variation >= 0 ? c.variation : 0];
     const mfh_pt_params params = device_params(v);
     check_mfh(mfh_pt_create(c.ctx, c.scene, mats.data(), (int32_t)mats.size(), points.empty() ? nullptr : points.data(),
                             (int32_t)points.size(), &cam, &params, &c.pt),
@@ -723,6 +741,7 @@ void load_scene(Core& c, Scenario& s) {
     apply_mode(c);
     check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
     if (c.renderer == kRendererWire) wire_reset(c);
+    if (c.renderer == kRendererLt) lt_reset(c);
     c.iteration = -1;
     c.screen.clear();
     c.sceneValid = true;
@@ -763,13 +782,20 @@ uint64_t micros() {
 
 Variation& current_variation(Core& c) {
     CORE_REQUIRE(c.variation >= 0, "Currently, no renderer is set.");
-    return c.vars[c.variation];
+    return c.renderer == kRendererLt ? c.ltVars : c.vars[c.variation];
 }
 
 void push_params(Core& c) {
     if (c.pt < 0 || !c.sceneValid) return;
     if (c.renderer == kRendererWire) {  // a parameter change resets the output
         wire_reset(c);
+        c.iteration = -1;
+        return;
+    }
+    if (c.renderer == kRendererLt) {
+        const mfh_pt_params p = device_params(c.ltVars);
+        check_mfh(mfh_pt_set_params(c.ctx, c.pt, &p), "parameters");
+        lt_reset(c);
         c.iteration = -1;
         return;
     }
@@ -783,7 +809,7 @@ void push_params(Core& c) {
 int find_param(const Core& c, const char* fn, const char* name, ParameterType expected) {
     const bool wire = c.renderer == kRendererWire;
     const ParamDescription* params = wire ? kWireParams : kParams;
-    const int count = wire ? kNumWireParams : kNumParams;
+    const int count = wire ? kNumWireParams : (c.renderer == kRendererLt ? kNumLtParams : kNumParams);
     for (int i = 0; i < count; ++i)
         if (name && std::strcmp(params[i].name, name) == 0) {
             if (params[i].type != expected) {
@@ -995,6 +1021,9 @@ Boolean mufflon_get_target_image(MufflonInstanceHdl instHdl, const char* name, B
         c.screen.resize((size_t)s.width * s.height * c.screenChannels);
         if (c.renderer == kRendererWire)
             check_mfh(mfh_pt_wireframe_get(c.ctx, c.pt, variance ? 1 : 0, c.screen.data()), "target read-back");
+        else if (c.renderer == kRendererLt)
+            check_mfh(mfh_pt_lighttracer_get(c.ctx, c.pt, tn.device, variance ? 1 : 0, c.screen.data()),
+                      "target read-back");
         else
             check_mfh(variance ? mfh_pt_get_variance(c.ctx, c.pt, tn.device, c.screen.data())
                                : mfh_pt_get_target(c.ctx, c.pt, tn.device, c.screen.data()),
@@ -2147,21 +2176,24 @@ Boolean scenario_assign_material(ScenarioHdl scenario, MatIdx index, MaterialHdl
 
 // ================================================================= rendering
 // Renderers: 0 "Pathtracer"/"PT" (variations 0 megakernel, 1 wavefront),
-// 1 "Wireframe"/"WF" (one variation), all on DEVICE_CUDA.
-constexpr const char* kRendererNames[2][2] = {{"Pathtracer", "PT"}, {"Wireframe", "WF"}};  // gpu_pt.hpp, gpu_wireframe.hpp
-constexpr uint32_t kRendererVariations[2] = {2u, 1u};
+// 1 "Wireframe"/"WF", 2 "Lighttracer"/"LT" (one variation each), all on
+// DEVICE_CUDA.
+constexpr int kNumRenderers = 3;
+constexpr const char* kRendererNames[kNumRenderers][2] = {
+    {"Pathtracer", "PT"}, {"Wireframe", "WF"}, {"Lighttracer", "LT"}};  // gpu_pt.hpp, gpu_wireframe.hpp, gpu_lt.hpp
+constexpr uint32_t kRendererVariations[kNumRenderers] = {2u, 1u, 1u};
 
 uint32_t render_get_renderer_count(MufflonInstanceHdl instHdl) {
     return guard(__func__, (uint32_t)0, [&] {
         core_of(instHdl);
-        return 2u;
+        return (uint32_t)kNumRenderers;
     });
 }
 
 uint32_t render_get_renderer_variations(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (uint32_t)0, [&]() -> uint32_t {
         core_of(instHdl);
-        CORE_REQUIRE(index < 2, "renderer index out of bounds");
+        CORE_REQUIRE(index < (uint32_t)kNumRenderers, "renderer index out of bounds");
         return kRendererVariations[index];
     });
 }
@@ -2169,7 +2201,7 @@ uint32_t render_get_renderer_variations(MufflonInstanceHdl instHdl, uint32_t ind
 const char* render_get_renderer_name(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (const char*)nullptr, [&]() -> const char* {
         core_of(instHdl);
-        CORE_REQUIRE(index < 2, "renderer index out of bounds");
+        CORE_REQUIRE(index < (uint32_t)kNumRenderers, "renderer index out of bounds");
         return kRendererNames[index][0];
     });
 }
@@ -2177,7 +2209,7 @@ const char* render_get_renderer_name(MufflonInstanceHdl instHdl, uint32_t index)
 const char* render_get_renderer_short_name(MufflonInstanceHdl instHdl, uint32_t index) {
     return guard(__func__, (const char*)nullptr, [&]() -> const char* {
         core_of(instHdl);
-        CORE_REQUIRE(index < 2, "renderer index out of bounds");
+        CORE_REQUIRE(index < (uint32_t)kNumRenderers, "renderer index out of bounds");
         return kRendererNames[index][1];
     });
 }
@@ -2185,7 +2217,8 @@ const char* render_get_renderer_short_name(MufflonInstanceHdl instHdl, uint32_t 
 RenderDevice render_get_renderer_devices(MufflonInstanceHdl instHdl, uint32_t index, uint32_t variation) {
     return guard(__func__, DEVICE_NONE, [&]() -> RenderDevice {
         core_of(instHdl);
-        CORE_REQUIRE(index < 2 && variation < kRendererVariations[index], "renderer index out of bounds");
+        CORE_REQUIRE(index < (uint32_t)kNumRenderers && variation < kRendererVariations[index],
+                     "renderer index out of bounds");
         return DEVICE_CUDA;
     });
 }
@@ -2193,7 +2226,8 @@ RenderDevice render_get_renderer_devices(MufflonInstanceHdl instHdl, uint32_t in
 Boolean render_enable_renderer(MufflonInstanceHdl instHdl, uint32_t index, uint32_t variation) {
     return guard(__func__, (Boolean) false, [&]() -> Boolean {
         Core& c = core_of(instHdl);
-        CORE_REQUIRE(index < 2 && variation < kRendererVariations[index], "renderer index out of bounds");
+        CORE_REQUIRE(index < (uint32_t)kNumRenderers && variation < kRendererVariations[index],
+                     "renderer index out of bounds");
         std::lock_guard<std::mutex> lock(c.iterationMutex);
         c.renderer = (int)index;
         c.variation = (int)variation;
@@ -2208,6 +2242,8 @@ Boolean render_enable_renderer(MufflonInstanceHdl instHdl, uint32_t index, uint3
                 load_scene(c, *c.current);
             } else if (c.renderer == kRendererWire) {
                 wire_reset(c);
+            } else if (c.renderer == kRendererLt) {
+                push_params(c);  // the light tracer's parameters, then a fresh output
             } else {
                 apply_targets(c);
                 apply_mode(c);
@@ -2248,6 +2284,8 @@ Boolean render_iterate(MufflonInstanceHdl instHdl, ProcessTime* iterateTime, Pro
         } else if (oldMask != c.targetMask || oldVar != c.varianceMask) {
             if (c.renderer == kRendererWire)
                 wire_reset(c);
+            else if (c.renderer == kRendererLt)
+                lt_reset(c);
             else
                 apply_targets(c);  // resets
             c.iteration = -1;
@@ -2260,6 +2298,9 @@ Boolean render_iterate(MufflonInstanceHdl instHdl, ProcessTime* iterateTime, Pro
         k0 = cycles();
         if (c.renderer == kRendererWire)
             check_mfh(mfh_pt_wireframe_iterate(c.ctx, c.pt, 1, c.wire.lineWidth, c.wire.maxTraceDepth), "iterate");
+        else if (c.renderer == kRendererLt)
+            check_mfh(mfh_pt_lighttracer_iterate(c.ctx, c.pt, 1, (c.targetMask >> kLtDensityTarget) & 1u),
+                      "iterate");
         else
             check_mfh(mfh_pt_iterate(c.ctx, c.pt, 1), "iterate");
         check_mfh(mfh_synchronize(c.ctx), "iterate");
@@ -2287,6 +2328,8 @@ Boolean render_reset(MufflonInstanceHdl instHdl) {
         if (c.pt >= 0 && c.sceneValid) {
             if (c.renderer == kRendererWire)
                 wire_reset(c);
+            else if (c.renderer == kRendererLt)
+                lt_reset(c);
             else
                 check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
         }
@@ -2318,6 +2361,8 @@ Boolean render_set_texture_data(MufflonInstanceHdl instHdl, TextureHdl tex, cons
             check_mfh(mfh_texture_update(c.ctx, it->second, t->texels.data()), "texture update");
             if (c.renderer == kRendererWire)
                 wire_reset(c);
+            else if (c.renderer == kRendererLt)
+                lt_reset(c);
             else
                 check_mfh(mfh_pt_reset(c.ctx, c.pt), "reset");
         }
@@ -2345,6 +2390,7 @@ Boolean render_set_emission_scale(MufflonInstanceHdl instHdl, MatIdx slot, Vec3 
                 if (c.current->assigned[i] == &m)
                     check_mfh(mfh_pt_set_material(c.ctx, c.pt, (int32_t)i, &m.record), "material update");
             if (c.renderer == kRendererWire) wire_reset(c);
+            if (c.renderer == kRendererLt) lt_reset(c);
         }
         c.iteration = -1;
         return true;
@@ -2513,7 +2559,8 @@ uint32_t renderer_get_num_parameters(MufflonInstanceHdl instHdl) {
     return guard(__func__, (uint32_t)0, [&]() -> uint32_t {
         Core& c = core_of(instHdl);
         current_variation(c);
-        return (uint32_t)(c.renderer == kRendererWire ? kNumWireParams : kNumParams);
+        return (uint32_t)(c.renderer == kRendererWire ? kNumWireParams
+                                                      : (c.renderer == kRendererLt ? kNumLtParams : kNumParams));
     });
 }
 
@@ -2522,7 +2569,8 @@ const char* renderer_get_parameter_desc(MufflonInstanceHdl instHdl, uint32_t idx
         Core& c = core_of(instHdl);
         current_variation(c);
         const bool wire = c.renderer == kRendererWire;
-        CORE_REQUIRE(idx < (uint32_t)(wire ? kNumWireParams : kNumParams), "parameter index out of range");
+        CORE_REQUIRE(idx < (uint32_t)(wire ? kNumWireParams : (c.renderer == kRendererLt ? kNumLtParams : kNumParams)),
+                     "parameter index out of range");
         const ParamDescription& d = wire ? kWireParams[idx] : kParams[idx];
         if (type) *type = d.type;
         return d.name;

@@ -165,13 +165,15 @@ def test_unsupported_features_fail_cleanly(mi):
 
 
 def test_renderer_registry_and_parameters(mi):
-    assert mi.render_get_renderer_count() == 2
+    assert mi.render_get_renderer_count() == 3
     assert mi.render_get_renderer_name(0) == "Pathtracer" and mi.render_get_renderer_short_name(0) == "PT"
     assert mi.render_get_renderer_name(1) == "Wireframe" and mi.render_get_renderer_short_name(1) == "WF"
+    assert mi.render_get_renderer_name(2) == "Lighttracer" and mi.render_get_renderer_short_name(2) == "LT"
     assert mi.render_get_renderer_variations(0) == 2 and mi.render_get_renderer_variations(1) == 1
+    assert mi.render_get_renderer_variations(2) == 1
     assert [mi.render_get_renderer_devices(0, v) for v in range(2)] == [K.DEVICE_CUDA, K.DEVICE_CUDA]
-    assert mi.render_get_renderer_devices(1, 0) == K.DEVICE_CUDA
-    assert not mi.render_enable_renderer(1, 1) and not mi.render_enable_renderer(2, 0)
+    assert mi.render_get_renderer_devices(1, 0) == K.DEVICE_CUDA and mi.render_get_renderer_devices(2, 0) == K.DEVICE_CUDA
+    assert not mi.render_enable_renderer(1, 1) and not mi.render_enable_renderer(3, 0)
     # nothing is enabled yet
     assert mi.render_get_render_target_count() == 0
     assert not mi.renderer_set_parameter_int("Seed", 3)
@@ -200,6 +202,15 @@ def test_renderer_registry_and_parameters(mi):
     assert mi.renderer_set_parameter_int("Maximum trace depth", 7)
     assert mi.renderer_get_parameter_int("Maximum trace depth") == 7
     assert mi.renderer_set_parameter_int("Seed", 4) and mi.renderer_get_parameter_int("Seed") == 0  # not its parameter
+    assert mi.render_enable_renderer(0, 0) and mi.renderer_get_parameter_int("Max. path length") == 5
+    # the light tracer's own parameters (LtParameters, lt_params.hpp:8-12) and targets
+    assert mi.render_enable_renderer(2, 0)
+    assert mi.renderer_parameters() == [("Seed", K.PARAM_INT), ("Min. path length", K.PARAM_INT),
+                                        ("Max. path length", K.PARAM_INT)]
+    assert mi.renderer_set_parameter_int("Max. path length", 9) and mi.renderer_get_parameter_int("Max. path length") == 9
+    assert mi.renderer_set_parameter_int("NEE count", 3)  # not its parameter: ignored
+    assert [mi.render_get_render_target_name(i) for i in range(mi.render_get_render_target_count())] == [
+        "Radiance", "Lightness", "Density"]
     assert mi.render_enable_renderer(0, 0) and mi.renderer_get_parameter_int("Max. path length") == 5
 
 

@@ -174,3 +174,41 @@ def test_core_api_mixed_scene_matches_oracle(oracle, tmp_path):
         assert mean[0].mean() > 0.01
     finally:
         mi.close()
+
+
+def test_core_api_lighttracer(oracle, tmp_path):
+    """The "Lighttracer"/"LT" renderer through the core API: its parameters,
+    the Radiance (with variance) and Density targets, against the oracle's
+    light tracer on the camera the core stores (float atomics: 1e-4 relative
+    plus 1e-6 of the largest value)."""
+    print("core api: light tracer", flush=True)
+    c = P.cornell(12, 24, 40)
+    mi = K.CoreInterface()
+    try:
+        scen = K.build_world(mi, c, str(tmp_path))
+        assert mi.render_enable_renderer(2, 0)
+        for name, v in (("Seed", 2), ("Max. path length", 6)):
+            assert mi.renderer_set_parameter_int(name, v)
+        assert mi.world_load_scenario(scen), mi.core_get_dll_error()
+        mi.render_enable_render_target("Radiance", True)
+        mi.render_enable_render_target("Density", False)
+        for _ in range(3):
+            assert mi.render_iterate()[0], mi.core_get_dll_error()
+        W, H = 40, 40
+        got = {"radiance": mi.mufflon_get_target_image("Radiance", False, W, H),
+               "radiance variance": mi.mufflon_get_target_image("Radiance", True, W, H),
+               "density": mi.mufflon_get_target_image("Density", False, W, H)}
+        ps = P.oracle_pt_scene(oracle, dict(c, camera=_oracle_scene_camera(mi, c)))
+        ref, rvar = oracle.lt_render(ps, 2, 0, 6, 3, density=True, variance=True)
+        want = {"radiance": ref[..., :3], "radiance variance": rvar[..., :3], "density": ref[..., 4:5]}
+        for k in got:
+            o = want[k]
+            g = got[k].reshape(o.shape)
+            tol = 1e-4 * np.abs(o) + 1e-6 * max(float(np.abs(o).max()), 1e-30)
+            print("%s: out of tolerance %d" % (k, int((np.abs(g - o) > tol).sum())), flush=True)
+            assert (np.abs(g - o) <= tol).all(), k
+        assert ref[..., :3].mean() > 0
+        assert mi.render_save_screenshot(str(tmp_path / "#shortrenderer-#target"), "Radiance", False)
+        assert (tmp_path / "LT-Radiance.pfm").exists()
+    finally:
+        mi.close()
