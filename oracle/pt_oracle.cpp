@@ -69,7 +69,12 @@ enum Mutation : int {
     kMutPointNoFalloff = 6,    // point light without the 1/d^2 falloff
     kMutBackgroundNoPdf = 7,   // background NEE without the division by the direction pdf
     kMutNeeMisNoLightCos = 8,  // NEE MIS hit pdf without the cosine at the light
-    kMutIncidentSolidAngle = 9 // hit MIS with the incident pdf left in solid angle
+    kMutIncidentSolidAngle = 9, // hit MIS with the incident pdf left in solid angle
+    kMutLtNoChoiceProb = 10,    // light tracer: emission not divided by the light's choice probability
+    kMutLtNoShadingCorr = 11,   // light tracer: adjoint BSDF without the shading normal correction
+    kMutLtNoEtaSq = 12,         // light tracer: adjoint refraction without eta^2
+    kMutLtImportanceNoCos3 = 13,// light tracer: camera importance without 1 / cos^3
+    kMutLtAreaNoCos = 14        // light tracer: area light seen by the camera without its cosine
 };
 int g_mutation = kMutNone;
 
@@ -575,7 +580,7 @@ TsSample microfacet_sample(const MatParams& p, V3 in, const Medium& incident, co
     } else {
         const float g = g_transmission(p.shadowing, gi, ge);
         throughput = sdiv(g, gi);
-        if (adjoint) throughput *= eta * eta;  // light sub-paths (microfacet_full.hpp:114-115)
+        if (adjoint && g_mutation != kMutLtNoEtaSq) throughput *= eta * eta;  // light sub-paths (microfacet_full.hpp:114-115)
         const float common =
             cavityPdf * (1.0f - f.f) * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_t * eDotH) * h.z);
         pdfF = common * sdiv(gi * n_t * n_t, std::fabs(in.z));
@@ -626,7 +631,7 @@ TsSample walter_sample(const MatParams& p, V3 in, const Medium& incident, const 
         pdfB = cavityPdf * sdiv(ge, std::fabs(4.0f * ex.z * h.z));
     } else {
         throughput = sdiv(g_transmission(p.shadowing, gi, ge), gi);
-        if (adjoint) throughput *= eta * eta;  // light sub-paths (microfacet_refractive.hpp:109-110)
+        if (adjoint && g_mutation != kMutLtNoEtaSq) throughput *= eta * eta;  // light sub-paths (microfacet_refractive.hpp:109-110)
         const float common = cavityPdf * sdiv(iDotHabs * eDotHabs, sq(n_i * iDotH + n_e * eDotH) * h.z);
         pdfF = common * sdiv(gi * n_e * n_e, std::fabs(in.z));
         pdfB = common * sdiv(ge * n_i * n_i, std::fabs(ex.z));
@@ -784,7 +789,7 @@ PathSample mat_sample(const TS& ts, const MatParams& m, V3 incident, float u0, f
     r.pdfF = mn(s.pdfF, 1e9f);
     r.pdfB = mn(s.pdfB, 1e9f);
     // shading normal correction of light sub-paths (material_sampling.hpp:131-135)
-    if (adjoint)
+    if (adjoint && g_mutation != kMutLtNoShadingCorr)
         r.throughput = r.throughput * ((kShadingNormalEps + std::fabs(iDotN * eDotG)) /
                                        (kShadingNormalEps + std::fabs(iDotG * eDotN)));
     r.valid = true;
@@ -835,7 +840,10 @@ EvalValue mat_evaluate_adjoint(const TS& ts, const MatParams& m, V3 incident, V3
     const TsEval e = evaluate_ts(m, exTS, inTS);
     if (!nonzero(e.value)) return z;
     EvalValue r;
-    r.value = e.value * ((kShadingNormalEps + std::fabs(iDotN * eDotG)) / (kShadingNormalEps + std::fabs(iDotG * eDotN)));
+    r.value = g_mutation == kMutLtNoShadingCorr
+                  ? e.value
+                  : e.value * ((kShadingNormalEps + std::fabs(iDotN * eDotG)) /
+                               (kShadingNormalEps + std::fabs(iDotG * eDotN)));
     r.cosOut = std::fabs(eDotN);
     r.pdfF = mn(e.pdfB, 1e9f);
     r.pdfB = mn(e.pdfF, 1e9f);
@@ -2213,7 +2221,7 @@ LtEmitter emit(const OrcPtScene& S, u64 index, u64 numIndices, u64 seed, float u
         }
     }
     LtEmitter e = light_position(S, type, S.lightMem + offset, u0, u1);
-    e.intensity = e.intensity / lightProb;
+    if (g_mutation != kMutLtNoChoiceProb) e.intensity = e.intensity / lightProb;
     return e;
 }
 
@@ -2235,7 +2243,7 @@ bool project(const OrcPtScene& S, V3 dir, int& px, int& py, float& w) {
     if (px >= S.width) px = S.width - 1;
     if (py >= S.height) py = S.height - 1;
     const float pixelArea = (2.0f * S.tanVFov) * (2.0f * S.tanVFov) * aspect;
-    w = 1.0f / (pixelArea * cosOut * cosOut * cosOut);
+    w = g_mutation == kMutLtImportanceNoCos3 ? 1.0f / pixelArea : 1.0f / (pixelArea * cosOut * cosOut * cosOut);
     return true;
 }
 
@@ -2270,7 +2278,7 @@ void lt_path(const OrcPtScene& S, int minPL, int maxPL, bool density, int index,
                     if (light.area) {  // evaluate_area (lights.hpp:286-295)
                         const float c = dot(light.normal, -1.0f * dir);
                         lval = c <= 0.0f ? V3{0, 0, 0} : light.intensity;
-                        lcos = c <= 0.0f ? 0.0f : c;
+                        lcos = c <= 0.0f ? 0.0f : (g_mutation == kMutLtAreaNoCos ? 1.0f : c);
                         geoN = light.normal;
                         geomFactor = dot(dir, light.normal);
                     } else {  // evaluate_point (lights.hpp:262-265)
