@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--width", type=int, default=2048)
     ap.add_argument("--height", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--pt-profile", choices=["terrain", "cornell"], default=None,
+    ap.add_argument("--pt-profile", choices=["terrain", "cornell", "cornell-lt"], default=None,
                     help="only a short run of one path-tracer workload (for rocprofv3)")
     ap.add_argument("--trace-block", type=int, default=256, help="wavefront trace block (with --pt-profile)")
     ap.add_argument("--stack-depth", type=int, default=16, help="wavefront LDS stack depth (with --pt-profile)")
@@ -555,12 +555,22 @@ def terrain_ray_costs():
 def pt_profile(args):
     """A short run of one path-tracer workload for rocprofv3: `terrain`
     (megakernel then wavefront with the given tuning, 2 iterations each after
-    one warm-up) or `cornell` (both forms, 8 iterations each)."""
+    one warm-up), `cornell` (both forms, 8 iterations each) or `cornell-lt`
+    (the light tracer on the Cornell box, 8 iterations)."""
     from mufflon_amd import hip, scenes as S, workloads as W
     from mufflon_amd.hip import HipContext
     ctx = HipContext(0)
     if args.leaf_limit is not None:
         ctx.set_traversal_leaf_limit(args.leaf_limit)
+    if args.pt_profile == "cornell-lt":  # the light tracer, 8 iterations after one warm-up
+        pt, _, _ = cornell_tracer(ctx)
+        ctx.lighttracer_iterate(pt, 1)
+        ctx.lighttracer_reset(pt)
+        ctx.timer_begin()
+        ctx.lighttracer_iterate(pt, 8)
+        print("light tracer: %.3f ms/iteration" % (ctx.timer_end() / 8), file=sys.stderr, flush=True)
+        ctx.close()
+        return
     if args.pt_profile == "terrain":
         sc = S.terrain_scene()
         its = 2
