@@ -773,6 +773,7 @@ int mfh_pt_lighttracer_iterate(mfh_context* ctx, int32_t ptId, int32_t iteration
         PathTracer& t = tracer_of(c, ptId);
         MFH_REQUIRE(iterations >= 0, "negative iteration count");
         MFH_REQUIRE(t.tree.posLightCount > 0, "the light tracer needs a point or area light");
+        MFH_REQUIRE(t.tree.posRootFlux > 0.0f, "the light tracer's lights emit nothing");
         MFH_REQUIRE(!(t.background.fluxSum > 0.0f),
                     "the light tracer emits from point and area lights only; the background emits");
         const size_t num = (size_t)t.camera.width * t.camera.height;

@@ -69,3 +69,12 @@ def test_lighttracer_refuses_an_emitting_background(hip_ctx):
     pt, _, _ = P.gpu_pt_materials(hip_ctx, c, background="envmap")
     with pytest.raises(hip.HipError):
         hip_ctx.lighttracer_iterate(pt, 1)
+
+
+def test_lighttracer_refuses_lights_that_emit_nothing(hip_ctx):
+    c = P.cornell(4, 8, 8)
+    mats = c["materials"].copy()
+    mats["radiance"][mats["type"] == S.MAT_EMISSIVE] = 0.0
+    pt = P.gpu_pt(hip_ctx, dict(c, materials=mats), max_path_length=4)
+    with pytest.raises(hip.HipError):
+        hip_ctx.lighttracer_iterate(pt, 1)
