@@ -569,7 +569,6 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_tiles(P set, int n, con
 constexpr int kLevelThreads = 512;
 constexpr int kLevelCap = 1024;
 constexpr int kLevelHash = 2048;  // power of two, >= 2 * kLevelCap
-static_assert(kLevelCap % kLevelThreads == 0, "whole slots per thread");
 
 __device__ __forceinline__ u32 level_hash(int node) { return ((u32)node * 2654435761u) & (kLevelHash - 1); }
 
@@ -674,55 +673,28 @@ __global__ __launch_bounds__(kLevelThreads) void k_refit_level(int n, int level,
         }
         return -1;
     };
-    // each thread owns at most kLevelCap / kLevelThreads slots; their
-    // topology loads are issued together, then their finished children's boxes
-    constexpr int kSlots = kLevelCap / kLevelThreads;
-    int chs[kSlots][2], par[kSlots];
+    for (int s = threadIdx.x; s < count; s += kLevelThreads) {
+        const int node = sNode[s];
+        const int2 ch = make_int2(children[2 * node], children[2 * node + 1]);
+        const int p = node != 0 ? parent_of(parents, node) : -1;
+        const int sl = ch.x >= nInt ? -1 : slot_of(ch.x);
+        const int sr = ch.y >= nInt ? -1 : slot_of(ch.y);
+        int refs[2] = {sl, sr};
+        const int cs[2] = {ch.x, ch.y};
+        u32 fin = 0;
 #pragma unroll
-    for (int r = 0; r < kSlots; ++r) {
-        const int s = (int)threadIdx.x + r * kLevelThreads;
-        chs[r][0] = chs[r][1] = par[r] = -1;
-        if (s < count) {
-            const int node = sNode[s];
-            chs[r][0] = children[2 * node];
-            chs[r][1] = children[2 * node + 1];
-            par[r] = node != 0 ? parent_of(parents, node) : -1;
-        }
-    }
-    int refs[kSlots][2];
-    float4 fin[kSlots][2][2];
-#pragma unroll
-    for (int r = 0; r < kSlots; ++r)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int c = chs[r][q];
-            refs[r][q] = (c < 0 || c >= nInt) ? -1 : slot_of(c);
-            if (c >= 0 && refs[r][q] < 0) {
-                fin[r][q][0] = boxes[2 * (size_t)c];
-                fin[r][q][1] = boxes[2 * (size_t)c + 1];
-            }
-        }
-#pragma unroll
-    for (int r = 0; r < kSlots; ++r) {
-        const int s = (int)threadIdx.x + r * kLevelThreads;
-        if (s >= count) continue;
-        u32 nFin = 0;
-        int rr[2];
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            rr[q] = refs[r][q];
-            if (rr[q] < 0) {
+        for (int q = 0; q < 2; ++q)
+            if (refs[q] < 0) {
                 const u32 f = atomicAdd(&sFinalCount, 1u);
-                sFinal[f][0] = fin[r][q][0];
-                sFinal[f][1] = fin[r][q][1];
-                rr[q] = ~(int)f;
-                ++nFin;
+                sFinal[f][0] = boxes[2 * (size_t)cs[q]];
+                sFinal[f][1] = boxes[2 * (size_t)cs[q] + 1];
+                refs[q] = ~(int)f;
+                ++fin;
             }
-        }
-        sChild[s] = make_int2(rr[0], rr[1]);
-        sCnt[s] = nFin;
-        sParentSlot[s] = (short)(par[r] >= 0 ? slot_of(par[r]) : -1);
-        if (nFin == 2u) sQueue[0][atomicAdd(&sQCount[0], 1u)] = (short)s;
+        sChild[s] = make_int2(refs[0], refs[1]);
+        sCnt[s] = fin;
+        sParentSlot[s] = (short)(p >= 0 ? slot_of(p) : -1);
+        if (fin == 2u) sQueue[0][atomicAdd(&sQCount[0], 1u)] = (short)s;
     }
     __syncthreads();
     int cur = 0;
